@@ -1,0 +1,96 @@
+/*
+ * pdos_ext.h -- extensions of the MI355X PDOS library (no reference
+ * counterpart; the reference API is pdos.h:85-87 only).  They never change the
+ * layout of the reference structs.
+ *
+ *   pdos_ex            pdos() with explicit plugin / precision / engine and
+ *                      a PdosInfo side channel (the reference's Sol carries no
+ *                      iteration count or residuals, pdos.h:37-41)
+ *   pdos_batch         many independent cone programs of uneven size in one
+ *                      persistent launch (one workgroup per instance)
+ *   pdos_family_*      a family of instances that share A and b (only c, or
+ *                      b and c, vary): factor once, solve batches on MFMA
+ *   pdos_last_error    text of the last failure on this thread
+ *
+ * Precision: PDOS_FP64 keeps every vector, SpMV and reduction in IEEE binary64
+ * (parity mode).  PDOS_MIXED stores matrices and iterates in binary32 and
+ * keeps the stopping norms, objectives and CG scalars in binary64.
+ * pdos() reads PDOS_PRECISION=fp64|mixed from the environment (default fp64).
+ */
+#ifndef PDOS_AMD_PDOS_EXT_H
+#define PDOS_AMD_PDOS_EXT_H
+
+#include "pdos.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { PDOS_INDIRECT = 0, PDOS_DIRECT = 1 };
+enum { PDOS_FP64 = 0, PDOS_MIXED = 1 };
+enum { PDOS_ENGINE_AUTO = 0, PDOS_ENGINE_WG = 1, PDOS_ENGINE_GRID = 2 };
+
+typedef struct PdosOptions {
+  int plugin;      /* PDOS_INDIRECT / PDOS_DIRECT */
+  int precision;   /* PDOS_FP64 / PDOS_MIXED */
+  int engine;      /* PDOS_ENGINE_*: WG = one workgroup per instance,
+                      GRID = whole-GPU kernels for one large instance */
+  int device;      /* HIP device ordinal */
+  int block;       /* WG engine threads per workgroup (0 = autotuned) */
+  int reserved[3];
+} PdosOptions;
+
+typedef struct PdosInfo {
+  long iters;        /* ADMM updates performed (the reference's loop index) */
+  long cg_iters;     /* total CG iterations (indirect) */
+  long status;       /* 0 Solved, 1 Indeterminate */
+  long engine;       /* engine that ran */
+  double p_res, d_res, p_obj, d_obj, eta;  /* last stop check */
+  double lambda;
+  double setup_ms;   /* host setup + factorization + upload */
+  double solve_ms;   /* device solve (host wall, synchronous) */
+  double total_ms;
+  double lnz;        /* nnz(L) for the direct plugin */
+} PdosInfo;
+
+/* Default options: indirect, fp64, auto engine, device 0. */
+void pdos_default_options(PdosOptions *opt);
+
+/* pdos() with options and info (info may be NULL). */
+Sol *pdos_ex(const Data *d, const Cone *k, const PdosOptions *opt, PdosInfo *info);
+
+/* Info of the last pdos()/pdos_ex() on this thread; returns 0 if any. */
+int pdos_last_info(PdosInfo *info);
+
+/* Solves count independent problems; sols[i] receives a malloc'd Sol (free
+ * each with freeSol), infos (optional) per-instance info.  Returns 0 or -1
+ * (see pdos_last_error). */
+int pdos_batch(const Data *const *ds, const Cone *const *ks, int count, const PdosOptions *opt,
+               Sol **sols, PdosInfo *infos);
+
+/* The same batch split in three so that callers (and benchmarks) can time the
+ * device solve with every input already resident in device memory:
+ *   prepare  host setup (+ factorization) and one upload
+ *   run      the solve on the device from the prepared start; re-runnable,
+ *            each run starts again from the inputs' warm start (or zero);
+ *            *kernel_ms (optional) receives the HIP-event time of the launch
+ *   fetch    download and unscale into malloc'd Sols (free with freeSol)
+ * A handle belongs to the thread that prepared it. */
+typedef struct PdosBatchHandle PdosBatchHandle;
+PdosBatchHandle *pdos_batch_prepare(const Data *const *ds, const Cone *const *ks, int count,
+                                    const PdosOptions *opt);
+int pdos_batch_run(PdosBatchHandle *h, double *kernel_ms);
+int pdos_batch_fetch(PdosBatchHandle *h, Sol **sols, PdosInfo *infos);
+void pdos_batch_free(PdosBatchHandle *h);
+
+/* Text of the last error on this thread ("" if none). */
+const char *pdos_last_error(void);
+
+/* Library build string. */
+const char *pdos_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PDOS_AMD_PDOS_EXT_H */

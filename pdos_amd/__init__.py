@@ -1,0 +1,201 @@
+"""MI355X PDOS: Python front-end over the C ABI of libpdos{indir,dir}_hip.so.
+
+`solve(c, A, b, dims, opts, x0, y0, s0)` keeps the contract of the reference's
+Python binding (python/pdosmodule.c:27-344): A is a sparse m x n matrix (any
+scipy sparse format; converted to CSC), dims = {'f': int, 'l': int,
+'q': [ints]}, opts may set MAX_ITERS, CG_MAX_ITS, EPS_ABS, ALPHA, CG_TOL,
+VERBOSE, NORMALIZE (defaults pdosmodule.c:63-74, NORMALIZE 0), and the result
+is a dict with 'x', 'y', 's' and 'status'.  Extra keys: 'info'.
+
+The library is the HIP build; there is no CPU fallback.  If the shared
+library is missing, `lib()` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .abi import Cone, Data, Params, Problem, Sol, sol_to_numpy, DEFAULT_PARAMS
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBS = {"indirect": os.path.join(HERE, "libpdosindir_hip.so"),
+        "direct": os.path.join(HERE, "libpdosdir_hip.so")}
+
+PLUGIN = {"indirect": 0, "direct": 1}
+PRECISION = {"fp64": 0, "mixed": 1}
+ENGINE = {"auto": 0, "wg": 1, "grid": 2}
+
+
+class PdosOptions(C.Structure):
+    _fields_ = [("plugin", C.c_int), ("precision", C.c_int), ("engine", C.c_int),
+                ("device", C.c_int), ("block", C.c_int), ("reserved", C.c_int * 3)]
+
+
+class PdosInfo(C.Structure):
+    _fields_ = [("iters", C.c_long), ("cg_iters", C.c_long), ("status", C.c_long),
+                ("engine", C.c_long), ("p_res", C.c_double), ("d_res", C.c_double),
+                ("p_obj", C.c_double), ("d_obj", C.c_double), ("eta", C.c_double),
+                ("lam", C.c_double), ("setup_ms", C.c_double), ("solve_ms", C.c_double),
+                ("total_ms", C.c_double), ("lnz", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# Every symbol include/pdos.h and include/pdos_ext.h declare.
+EXPORTS = ["pdos", "freeData", "freeSol", "tic", "toc", "tocq", "pdos_default_options",
+           "pdos_ex", "pdos_last_info", "pdos_batch", "pdos_batch_prepare", "pdos_batch_run",
+           "pdos_batch_fetch", "pdos_batch_free", "pdos_last_error", "pdos_version"]
+
+_libs = {}
+
+
+def lib(plugin: str = "indirect"):
+    """Loads the HIP library (raises if it was not built)."""
+    if plugin not in _libs:
+        path = LIBS[plugin]
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: build it with `make -C pdos_amd/csrc`")
+        L = C.CDLL(path)
+        L.pdos.restype = C.POINTER(Sol)
+        L.pdos.argtypes = [C.POINTER(Data), C.POINTER(Cone)]
+        L.pdos_ex.restype = C.POINTER(Sol)
+        L.pdos_ex.argtypes = [C.POINTER(Data), C.POINTER(Cone), C.POINTER(PdosOptions),
+                              C.POINTER(PdosInfo)]
+        L.freeSol.argtypes = [C.POINTER(C.POINTER(Sol))]
+        L.freeData.argtypes = [C.POINTER(C.POINTER(Data)), C.POINTER(C.POINTER(Cone))]
+        L.pdos_last_error.restype = C.c_char_p
+        L.pdos_version.restype = C.c_char_p
+        L.pdos_last_info.argtypes = [C.POINTER(PdosInfo)]
+        L.pdos_default_options.argtypes = [C.POINTER(PdosOptions)]
+        L.pdos_batch_prepare.restype = C.c_void_p
+        L.pdos_batch_prepare.argtypes = [C.POINTER(C.POINTER(Data)), C.POINTER(C.POINTER(Cone)),
+                                         C.c_int, C.POINTER(PdosOptions)]
+        L.pdos_batch_run.restype = C.c_int
+        L.pdos_batch_run.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+        L.pdos_batch_fetch.restype = C.c_int
+        L.pdos_batch_fetch.argtypes = [C.c_void_p, C.POINTER(C.POINTER(Sol)), C.POINTER(PdosInfo)]
+        L.pdos_batch_free.argtypes = [C.c_void_p]
+        L.pdos_batch.restype = C.c_int
+        L.pdos_batch.argtypes = [C.POINTER(C.POINTER(Data)), C.POINTER(C.POINTER(Cone)), C.c_int,
+                                 C.POINTER(PdosOptions), C.POINTER(C.POINTER(Sol)),
+                                 C.POINTER(PdosInfo)]
+        _libs[plugin] = L
+    return _libs[plugin]
+
+
+def options(plugin="indirect", precision="fp64", engine="auto", device=0) -> PdosOptions:
+    o = PdosOptions()
+    o.plugin = PLUGIN[plugin]
+    o.precision = PRECISION[precision]
+    o.engine = ENGINE[engine]
+    o.device = device
+    return o
+
+
+def _take_sol(L, sp):
+    out = sol_to_numpy(sp)
+    L.freeSol(C.byref(sp))
+    return out
+
+
+def solve_problem(p: Problem, plugin="indirect", precision="fp64", engine="auto") -> dict:
+    """Solves one Problem with the HIP library; returns x, s, y, status, info."""
+    L = lib(plugin)
+    d, k, keep = p.to_c()
+    o = options(plugin, precision, engine)
+    info = PdosInfo()
+    sp = L.pdos_ex(C.byref(d), C.byref(k), C.byref(o), C.byref(info))
+    if not sp:
+        raise RuntimeError(f"pdos_ex failed: {L.pdos_last_error().decode()}")
+    out = _take_sol(L, sp)
+    out["info"] = info.as_dict()
+    return out
+
+
+def solve_batch(problems, plugin="indirect", precision="fp64") -> list:
+    """Solves independent problems in one launch (one workgroup each)."""
+    L = lib(plugin)
+    n = len(problems)
+    keeps = [p.to_c() for p in problems]
+    DA = (C.POINTER(Data) * n)(*[C.pointer(kk[0]) for kk in keeps])
+    KA = (C.POINTER(Cone) * n)(*[C.pointer(kk[1]) for kk in keeps])
+    sols = (C.POINTER(Sol) * n)()
+    infos = (PdosInfo * n)()
+    o = options(plugin, precision, "wg")
+    rc = L.pdos_batch(DA, KA, n, C.byref(o), sols, infos)
+    if rc != 0:
+        raise RuntimeError(f"pdos_batch failed: {L.pdos_last_error().decode()}")
+    out = []
+    for i in range(n):
+        r = _take_sol(L, sols[i])
+        r["info"] = infos[i].as_dict()
+        out.append(r)
+    return out
+
+
+class BatchHandle:
+    """A prepared batch resident on the device (pdos_batch_prepare/run/fetch)."""
+
+    def __init__(self, problems, plugin="indirect", precision="fp64"):
+        self.L = lib(plugin)
+        self.n = len(problems)
+        keeps = [p.to_c() for p in problems]
+        DA = (C.POINTER(Data) * self.n)(*[C.pointer(kk[0]) for kk in keeps])
+        KA = (C.POINTER(Cone) * self.n)(*[C.pointer(kk[1]) for kk in keeps])
+        o = options(plugin, precision, "wg")
+        self.h = self.L.pdos_batch_prepare(DA, KA, self.n, C.byref(o))
+        if not self.h:
+            raise RuntimeError(f"pdos_batch_prepare failed: {self.L.pdos_last_error().decode()}")
+
+    def run(self) -> float:
+        """Solves on the device; returns the launch time in ms (HIP events)."""
+        ms = C.c_double()
+        if self.L.pdos_batch_run(self.h, C.byref(ms)) != 0:
+            raise RuntimeError(f"pdos_batch_run failed: {self.L.pdos_last_error().decode()}")
+        return ms.value
+
+    def fetch(self) -> list:
+        sols = (C.POINTER(Sol) * self.n)()
+        infos = (PdosInfo * self.n)()
+        if self.L.pdos_batch_fetch(self.h, sols, infos) != 0:
+            raise RuntimeError(f"pdos_batch_fetch failed: {self.L.pdos_last_error().decode()}")
+        out = []
+        for i in range(self.n):
+            r = _take_sol(self.L, sols[i])
+            r["info"] = infos[i].as_dict()
+            out.append(r)
+        return out
+
+    def close(self):
+        if self.h:
+            self.L.pdos_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solve(c, A, b, dims, opts=None, x0=None, y0=None, s0=None, plugin="indirect",
+          precision="fp64"):
+    """The reference binding's solve() (python/pdosmodule.c:27-344)."""
+    import scipy.sparse as sp
+    A = sp.csc_matrix(A)
+    m, n = A.shape
+    params = dict(DEFAULT_PARAMS)
+    params["VERBOSE"] = 1
+    if opts:
+        for key, val in opts.items():
+            if key not in params:
+                raise KeyError(f"unknown option {key}")
+            params[key] = val
+    q = list(dims.get("q", []))
+    p = Problem(n, m, A.indptr, A.indices, A.data, np.asarray(b, float).ravel(),
+                np.asarray(c, float).ravel(), int(dims.get("f", 0)), int(dims.get("l", 0)), q,
+                params, x0, y0, s0)
+    return solve_problem(p, plugin, precision)

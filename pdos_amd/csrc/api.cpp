@@ -1,0 +1,613 @@
+// api.cpp -- the C ABI: pdos(), freeData(), freeSol() (pdos.h:85-87), the
+// demo timers (util.h:37-47) and the extensions of pdos_ext.h.
+//
+// Flow of one call: host setup (common.h semantics, binary64) -> optional host
+// LDL' factorization -> one packed upload into a per-thread device arena ->
+// one engine launch on a per-thread stream -> one download -> host unscale
+// (pdos.c:251-294) into a malloc'd Sol.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pdos.h"
+#include "../../include/pdos_ext.h"
+#include "device.hpp"
+#include "engine.hpp"
+#include "host.hpp"
+
+#ifndef PDOS_DEFAULT_PLUGIN
+#define PDOS_DEFAULT_PLUGIN PDOS_INDIRECT
+#endif
+
+namespace pdos_amd {
+
+thread_local std::string g_err;
+thread_local PdosInfo g_info;
+thread_local bool g_info_valid = false;
+
+Ctx& ctx() {
+  thread_local Ctx c;
+  return c;
+}
+
+bool Ctx::ensure(int device, std::string& err) {
+  if (dev == device && stream) return true;
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) { err = std::string("hipSetDevice: ") + hipGetErrorString(e); return false; }
+  if (stream) (void)hipStreamDestroy(stream);
+  e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { err = std::string("hipStreamCreate: ") + hipGetErrorString(e); return false; }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+    max_lds = (int)prop.sharedMemPerBlock;
+    num_cu = prop.multiProcessorCount;
+  }
+  if (dbuf) { (void)hipFree(dbuf); dbuf = nullptr; dcap = 0; }
+  dev = device;
+  return true;
+}
+
+bool Ctx::reserve(size_t dbytes, size_t hbytes, std::string& err) {
+  if (dbytes > dcap) {
+    if (dbuf) (void)hipFree(dbuf);
+    dbuf = nullptr;
+    size_t want = std::max(dbytes, dcap + dcap / 2);
+    hipError_t e = hipMalloc(&dbuf, want);
+    if (e != hipSuccess) {
+      dcap = 0;
+      err = std::string("hipMalloc: ") + hipGetErrorString(e);
+      return false;
+    }
+    dcap = want;
+  }
+  if (hbytes > hcap) {
+    if (hbuf) (void)hipHostFree(hbuf);
+    hbuf = nullptr;
+    size_t want = std::max(hbytes, hcap + hcap / 2);
+    hipError_t e = hipHostMalloc(&hbuf, want, hipHostMallocDefault);
+    if (e != hipSuccess) {
+      hcap = 0;
+      err = std::string("hipHostMalloc: ") + hipGetErrorString(e);
+      return false;
+    }
+    hcap = want;
+  }
+  return true;
+}
+
+size_t Packer::put(const void* src, size_t bytes) {
+  size_t off = (buf.size() + 255) & ~size_t(255);
+  buf.resize(off + std::max<size_t>(bytes, 1));
+  if (bytes) std::memcpy(buf.data() + off, src, bytes);
+  return off;
+}
+
+size_t Packer::zeros(size_t bytes) {
+  size_t off = (buf.size() + 255) & ~size_t(255);
+  buf.resize(off + std::max<size_t>(bytes, 1), 0);
+  return off;
+}
+
+int lanes_for(long nnz, long rows) {
+  const double avg = rows > 0 ? double(nnz) / double(rows) : 1.0;
+  int g = 1;
+  while (g < 64 && g < avg) g <<= 1;
+  return g;
+}
+
+double now_ms() {
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+template <typename V>
+size_t put_vec(Packer& P, const std::vector<V>& v) {
+  return P.put(v.data(), v.size() * sizeof(V));
+}
+
+static size_t put_vals(Packer& P, const std::vector<double>& v, bool fp64) {
+  if (fp64) return put_vec(P, v);
+  std::vector<float> f(v.begin(), v.end());
+  return put_vec(P, f);
+}
+
+WgBatch::~WgBatch() {
+  if (own_mem && dbuf) (void)hipFree(dbuf);
+  if (ev0) (void)hipEventDestroy(ev0);
+  if (ev1) (void)hipEventDestroy(ev1);
+}
+
+bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::string& err) {
+  device = device_; fp64 = fp64_; direct = direct_; own_mem = !borrow;
+  Ctx& C = ctx();
+  if (!C.ensure(device, err)) return false;
+  stream = C.stream;
+  const double t0 = now_ms();
+  const int count = (int)S.size();
+  const int tsz = fp64 ? 8 : 4;
+  Packer P;
+  struct Off { size_t soc, rp, ri, rx, cp, ci, cx, b, c, D, E, x0, s0, y0;
+               size_t perm, lrp, lri, lrx, lcp, lci, lcx, ld, flp, flr, blp, blr; };
+  std::vector<Off> pk(count);
+  xo.resize(count); so.resize(count); yo.resize(count);
+  lds = 0;
+  for (int i = 0; i < count; ++i) {
+    const Setup& s = S[i];
+    Off& k = pk[i];
+    k.soc = put_vec(P, s.soc_ptr);
+    k.rp = put_vec(P, s.rp);
+    k.ri = put_vec(P, s.ri);
+    k.rx = put_vals(P, s.rx, fp64);
+    k.cp = put_vec(P, s.cp);
+    k.ci = put_vec(P, s.ci);
+    k.cx = put_vals(P, s.cx, fp64);
+    k.b = put_vec(P, s.b);
+    k.c = put_vec(P, s.c);
+    k.D = put_vec(P, s.D);
+    k.E = put_vec(P, s.E);
+    k.x0 = put_vec(P, s.x);
+    k.s0 = put_vec(P, s.s);
+    k.y0 = put_vec(P, s.y);
+    xo[i] = P.zeros(sizeof(double) * s.n);
+    so[i] = P.zeros(sizeof(double) * s.m);
+    yo[i] = P.zeros(sizeof(double) * s.m);
+    if (direct) {
+      const Factor& f = F[i];
+      k.perm = put_vec(P, f.P);
+      k.lrp = put_vec(P, f.Rp);
+      k.lri = put_vec(P, f.Ri);
+      k.lrx = put_vec(P, f.Rx);
+      k.lcp = put_vec(P, f.Lp);
+      k.lci = put_vec(P, f.Li);
+      k.lcx = put_vec(P, f.Lx);
+      k.ld = put_vec(P, f.Ld);
+      k.flp = put_vec(P, f.fl_ptr);
+      k.flr = put_vec(P, f.fl_rows);
+      k.blp = put_vec(P, f.bl_ptr);
+      k.blr = put_vec(P, f.bl_rows);
+    }
+    lds = std::max(lds, wg_lds_bytes(s.n, s.m, s.n + s.m, direct, tsz));
+  }
+  if (lds > C.max_lds) {
+    err = "instance too large for the workgroup engine (needs " + std::to_string(lds) +
+          " B of LDS, device has " + std::to_string(C.max_lds) + ")";
+    return false;
+  }
+  res_off = P.zeros(sizeof(DevResult) * count);
+  inst_off = P.zeros(sizeof(WgInst) * count);
+  bytes = P.buf.size();
+  if (borrow) {
+    if (!C.reserve(bytes, bytes, err)) return false;
+    dbuf = C.dbuf;
+  } else {
+    hipError_t e = hipMalloc(&dbuf, bytes);
+    if (e != hipSuccess) { dbuf = nullptr; err = std::string("hipMalloc: ") + hipGetErrorString(e); return false; }
+  }
+  unsigned char* base = static_cast<unsigned char*>(dbuf);
+  std::vector<WgInst> insts(count);
+  for (int i = 0; i < count; ++i) {
+    const Setup& s = S[i];
+    const Off& k = pk[i];
+    WgInst& w = insts[i];
+    std::memset(&w, 0, sizeof(w));
+    w.n = s.n; w.m = s.m; w.f = s.f; w.l = s.l; w.nsoc = s.nsoc(); w.N = s.n + s.m;
+    w.ga = lanes_for(s.nnz(), s.m);
+    w.gat = lanes_for(s.nnz(), s.n);
+    w.max_iters = (int)std::min<long>(std::max<long>(par[i].MAX_ITERS, 0), 0x7fffffff);
+    w.cg_max_its = (int)std::min<long>(std::max<long>(par[i].CG_MAX_ITS, 0), 0x7fffffff);
+    w.eps = par[i].EPS_ABS;
+    w.alpha = par[i].ALPHA;
+    w.cg_tol_sq = par[i].CG_TOL * par[i].CG_TOL;
+    w.lambda = s.lambda;
+    w.pscale = s.pscale;
+    w.dscale = s.dscale;
+    w.soc_ptr = reinterpret_cast<const int*>(base + k.soc);
+    w.rp = reinterpret_cast<const int*>(base + k.rp);
+    w.ri = reinterpret_cast<const int*>(base + k.ri);
+    w.rx = base + k.rx;
+    w.cp = reinterpret_cast<const int*>(base + k.cp);
+    w.ci = reinterpret_cast<const int*>(base + k.ci);
+    w.cx = base + k.cx;
+    w.b = reinterpret_cast<const double*>(base + k.b);
+    w.c = reinterpret_cast<const double*>(base + k.c);
+    w.D = reinterpret_cast<const double*>(base + k.D);
+    w.E = reinterpret_cast<const double*>(base + k.E);
+    w.x0 = reinterpret_cast<const double*>(base + k.x0);
+    w.s0 = reinterpret_cast<const double*>(base + k.s0);
+    w.y0 = reinterpret_cast<const double*>(base + k.y0);
+    w.x = reinterpret_cast<double*>(base + xo[i]);
+    w.s = reinterpret_cast<double*>(base + so[i]);
+    w.y = reinterpret_cast<double*>(base + yo[i]);
+    if (direct) {
+      const Factor& f = F[i];
+      w.perm = reinterpret_cast<const int*>(base + k.perm);
+      w.lrp = reinterpret_cast<const int*>(base + k.lrp);
+      w.lri = reinterpret_cast<const int*>(base + k.lri);
+      w.lrx = reinterpret_cast<const double*>(base + k.lrx);
+      w.lcp = reinterpret_cast<const int*>(base + k.lcp);
+      w.lci = reinterpret_cast<const int*>(base + k.lci);
+      w.lcx = reinterpret_cast<const double*>(base + k.lcx);
+      w.ld = reinterpret_cast<const double*>(base + k.ld);
+      w.fl_ptr = reinterpret_cast<const int*>(base + k.flp);
+      w.fl_rows = reinterpret_cast<const int*>(base + k.flr);
+      w.bl_ptr = reinterpret_cast<const int*>(base + k.blp);
+      w.bl_rows = reinterpret_cast<const int*>(base + k.blr);
+      w.nfl = f.nfl();
+      w.nbl = f.nbl();
+    }
+    w.res = reinterpret_cast<DevResult*>(base + res_off) + i;
+    w.lds_bytes = wg_lds_bytes(s.n, s.m, s.n + s.m, direct, tsz);
+  }
+  std::memcpy(P.buf.data() + inst_off, insts.data(), sizeof(WgInst) * count);
+  hipError_t e;
+  if (borrow) {
+    std::memcpy(C.hbuf, P.buf.data(), bytes);
+    e = hipMemcpyAsync(dbuf, C.hbuf, bytes, hipMemcpyHostToDevice, stream);
+  } else {
+    e = hipMemcpyAsync(dbuf, P.buf.data(), bytes, hipMemcpyHostToDevice, stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  if (e == hipSuccess && !ev0) e = hipEventCreate(&ev0);
+  if (e == hipSuccess && !ev1) e = hipEventCreate(&ev1);
+  if (e != hipSuccess) { err = std::string("upload: ") + hipGetErrorString(e); return false; }
+  upload_ms = now_ms() - t0;
+  return true;
+}
+
+bool WgBatch::run(std::string& err) {
+  unsigned char* base = static_cast<unsigned char*>(dbuf);
+  const double t0 = now_ms();
+  hipError_t e = hipEventRecord(ev0, stream);
+  if (e == hipSuccess)
+    e = wg_launch(reinterpret_cast<const WgInst*>(base + inst_off), (int)S.size(), lds, fp64, direct, stream);
+  if (e == hipSuccess) e = hipEventRecord(ev1, stream);
+  if (e == hipSuccess) e = hipEventSynchronize(ev1);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev0, ev1);
+  if (e != hipSuccess) { err = std::string("wg engine: ") + hipGetErrorString(e); return false; }
+  kernel_ms = ms;
+  wall_ms = now_ms() - t0;
+  return true;
+}
+
+bool WgBatch::fetch(std::vector<DevResult>& res, std::string& err) {
+  unsigned char* base = static_cast<unsigned char*>(dbuf);
+  const int count = (int)S.size();
+  res.resize(count);
+  hipError_t e = hipMemcpyAsync(res.data(), base + res_off, sizeof(DevResult) * count,
+                                hipMemcpyDeviceToHost, stream);
+  for (int i = 0; i < count && e == hipSuccess; ++i) {
+    Setup& s = S[i];
+    e = hipMemcpyAsync(s.x.data(), base + xo[i], sizeof(double) * s.n, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(s.s.data(), base + so[i], sizeof(double) * s.m, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(s.y.data(), base + yo[i], sizeof(double) * s.m, hipMemcpyDeviceToHost, stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) { err = std::string("download: ") + hipGetErrorString(e); return false; }
+  return true;
+}
+
+bool solve_wg(std::vector<Setup>& S, std::vector<Factor>& F, const std::vector<const Params*>& par,
+              bool fp64, bool direct, int device, std::vector<DevResult>& res,
+              double& upload_ms, double& solve_ms, std::string& err) {
+  WgBatch B;
+  B.S.swap(S);
+  B.F.swap(F);
+  for (const Params* p : par) B.par.push_back(*p);
+  bool ok = B.prepare(device, fp64, direct, true, err) && B.run(err) && B.fetch(res, err);
+  B.S.swap(S);
+  B.F.swap(F);
+  upload_ms = B.upload_ms;
+  solve_ms = B.wall_ms;
+  return ok;
+}
+
+// x_out = E x, y_out = D y, s_out = s / D (pdos.c:251-294).
+Sol* make_sol(const Setup& s, int status) {
+  Sol* sol = static_cast<Sol*>(std::malloc(sizeof(Sol)));
+  sol->n = s.n;
+  sol->m = s.m;
+  sol->x = static_cast<double*>(std::malloc(sizeof(double) * s.n));
+  sol->y = static_cast<double*>(std::malloc(sizeof(double) * s.m));
+  sol->s = static_cast<double*>(std::malloc(sizeof(double) * s.m));
+  for (int j = 0; j < s.n; ++j) sol->x[j] = s.E[j] * s.x[j];
+  for (int i = 0; i < s.m; ++i) sol->y[i] = s.D[i] * s.y[i];
+  for (int i = 0; i < s.m; ++i) sol->s[i] = s.s[i] / s.D[i];
+  std::memset(sol->status, 0, sizeof(sol->status));
+  std::strcpy(sol->status, status == 0 ? "Solved" : "Indeterminate");
+  return sol;
+}
+
+void fill_info(PdosInfo* info, const DevResult& r, const Setup& s, int engine, double setup_ms,
+               double solve_ms, double lnz) {
+  if (!info) return;
+  info->iters = r.iters;
+  info->cg_iters = r.cg_iters;
+  info->status = r.status;
+  info->engine = engine;
+  info->p_res = r.p_res;
+  info->d_res = r.d_res;
+  info->p_obj = r.p_obj;
+  info->d_obj = r.d_obj;
+  info->eta = r.eta;
+  info->lambda = s.lambda;
+  info->setup_ms = setup_ms;
+  info->solve_ms = solve_ms;
+  info->total_ms = setup_ms + solve_ms;
+  info->lnz = lnz;
+}
+
+int env_precision() {
+  const char* p = std::getenv("PDOS_PRECISION");
+  if (p && (!std::strcmp(p, "mixed") || !std::strcmp(p, "fp32"))) return PDOS_MIXED;
+  return PDOS_FP64;
+}
+
+void print_verbose(const Params* p, const DevResult& r, double lam, double secs) {
+  if (!p->VERBOSE) return;
+  // Banner and summary row in the format of pdos.c:79-92,306-335.
+  std::printf("\nPDOS - A Primal-Dual Operator Splitting for Cone Programming (MI355X).\n\n");
+  std::printf("lambda: %5.3e\n\n", lam);
+  std::printf(" iter |  ||Ax+s-b||    ||A'y+c||       c'x          -b'y          eta   \n");
+  std::printf("%*i | ", 5, r.iters);
+  std::printf("%*.3e   ", 11, r.p_res);
+  std::printf("%*.3e   ", 10, r.d_res);
+  std::printf("%*.3e   ", 11, r.p_obj);
+  std::printf("%*.3e   ", 11, r.d_obj);
+  std::printf("%*.3e   \n", 9, r.eta);
+  std::printf("Total solve time is %4.8fs\n", secs);
+}
+
+}  // namespace pdos_amd
+
+using namespace pdos_amd;
+
+extern "C" {
+
+void pdos_default_options(PdosOptions* opt) {
+  std::memset(opt, 0, sizeof(*opt));
+  opt->plugin = PDOS_DEFAULT_PLUGIN;
+  opt->precision = PDOS_FP64;
+  opt->engine = PDOS_ENGINE_AUTO;
+}
+
+const char* pdos_last_error(void) { return g_err.c_str(); }
+
+const char* pdos_version(void) { return "pdos-mi355x 0.1 (gfx950)"; }
+
+int pdos_last_info(PdosInfo* info) {
+  if (!g_info_valid || !info) return -1;
+  *info = g_info;
+  return 0;
+}
+
+Sol* pdos_ex(const Data* d, const Cone* k, const PdosOptions* opt, PdosInfo* info) {
+  g_err.clear();
+  g_info_valid = false;
+  if (d == nullptr || k == nullptr) return nullptr;  // pdos.c:61-63
+  PdosOptions o;
+  if (opt) o = *opt;
+  else pdos_default_options(&o);
+  double t0 = now_ms();
+  std::vector<Setup> S(1);
+  std::vector<Factor> F;
+  std::string err;
+  if (!ctx().ensure(o.device, err)) {
+    g_err = err;
+    return nullptr;
+  }
+  if (!build_setup(d, k, S[0], err)) {
+    g_err = err;
+    return nullptr;
+  }
+  const bool direct = o.plugin == PDOS_DIRECT;
+  double lnz = 0;
+  if (direct) {
+    F.resize(1);
+    if (!build_factor(S[0], {}, F[0], err)) {
+      g_err = err;
+      return nullptr;
+    }
+    lnz = (double)F[0].Lp.back();
+  }
+  double t1 = now_ms();
+  std::vector<const Params*> par{d->p};
+  std::vector<DevResult> res;
+  double up = 0, sv = 0;
+  int engine = PDOS_ENGINE_WG;
+  bool ok;
+  if (o.engine == PDOS_ENGINE_GRID ||
+      (o.engine == PDOS_ENGINE_AUTO && !direct && grid_preferred(S[0], o.precision == PDOS_FP64))) {
+    engine = PDOS_ENGINE_GRID;
+    if (direct) {
+      g_err = "the grid engine runs the indirect plugin only";
+      return nullptr;
+    }
+    ok = solve_grid(S[0], *d->p, o.precision == PDOS_FP64, o.device, res, up, sv, err);
+  } else {
+    ok = solve_wg(S, F, par, o.precision == PDOS_FP64, direct, o.device, res, up, sv, err);
+  }
+  if (!ok) {
+    g_err = err;
+    if (d->p->VERBOSE) std::fprintf(stderr, "pdos: %s\n", err.c_str());
+    return nullptr;
+  }
+  Sol* sol = make_sol(S[0], res[0].status);
+  PdosInfo inf;
+  fill_info(&inf, res[0], S[0], engine, (t1 - t0) + up, sv, lnz);
+  g_info = inf;
+  g_info_valid = true;
+  if (info) *info = inf;
+  print_verbose(d->p, res[0], S[0].lambda, sv * 1e-3);
+  return sol;
+}
+
+Sol* pdos(const Data* d, const Cone* k) {
+  if (d == nullptr || k == nullptr) return nullptr;
+  long tot = k->f + k->l;
+  for (long i = 0; i < k->qsize; ++i) tot += k->q[i];
+  if (d->m != tot) {  // the reference asserts (pdos.c:65-71)
+    std::fprintf(stderr, "pdos: cone sizes (%ld) do not match m (%ld)\n", tot, (long)d->m);
+    std::abort();
+  }
+  PdosOptions o;
+  pdos_default_options(&o);
+  o.precision = env_precision();
+  return pdos_ex(d, k, &o, nullptr);
+}
+
+}  // extern "C"
+
+struct PdosBatchHandle {
+  pdos_amd::WgBatch B;
+  double setup_ms = 0;
+};
+
+static PdosBatchHandle* batch_prepare(const Data* const* ds, const Cone* const* ks, int count,
+                                      const PdosOptions* opt, bool borrow) {
+  g_err.clear();
+  if (count <= 0 || !ds || !ks) {
+    g_err = "bad batch arguments";
+    return nullptr;
+  }
+  PdosOptions o;
+  if (opt) o = *opt;
+  else pdos_default_options(&o);
+  const bool direct = o.plugin == PDOS_DIRECT;
+  const double t0 = now_ms();
+  PdosBatchHandle* h = new PdosBatchHandle;
+  h->B.S.resize(count);
+  if (direct) h->B.F.resize(count);
+  h->B.par.resize(count);
+  std::string err;
+  for (int i = 0; i < count; ++i) {
+    if (!ds[i] || !ks[i] || !build_setup(ds[i], ks[i], h->B.S[i], err) ||
+        (direct && !build_factor(h->B.S[i], {}, h->B.F[i], err))) {
+      g_err = "instance " + std::to_string(i) + ": " + (err.empty() ? "NULL input" : err);
+      delete h;
+      return nullptr;
+    }
+    h->B.par[i] = *ds[i]->p;
+  }
+  if (!h->B.prepare(o.device, o.precision == PDOS_FP64, direct, borrow, err)) {
+    g_err = err;
+    delete h;
+    return nullptr;
+  }
+  h->setup_ms = now_ms() - t0;
+  return h;
+}
+
+static int batch_fetch(PdosBatchHandle* h, Sol** sols, PdosInfo* infos) {
+  std::vector<DevResult> res;
+  std::string err;
+  if (!h->B.fetch(res, err)) {
+    g_err = err;
+    return -1;
+  }
+  for (size_t i = 0; i < res.size(); ++i) {
+    sols[i] = make_sol(h->B.S[i], res[i].status);
+    if (infos)
+      fill_info(&infos[i], res[i], h->B.S[i], PDOS_ENGINE_WG, h->setup_ms, h->B.wall_ms,
+                h->B.direct ? (double)h->B.F[i].Lp.back() : 0.0);
+  }
+  return 0;
+}
+
+extern "C" {
+
+PdosBatchHandle* pdos_batch_prepare(const Data* const* ds, const Cone* const* ks, int count,
+                                    const PdosOptions* opt) {
+  return batch_prepare(ds, ks, count, opt, false);
+}
+
+int pdos_batch_run(PdosBatchHandle* h, double* kernel_ms) {
+  if (!h) return -1;
+  std::string err;
+  if (!h->B.run(err)) {
+    g_err = err;
+    return -1;
+  }
+  if (kernel_ms) *kernel_ms = h->B.kernel_ms;
+  return 0;
+}
+
+int pdos_batch_fetch(PdosBatchHandle* h, Sol** sols, PdosInfo* infos) {
+  if (!h || !sols) return -1;
+  return batch_fetch(h, sols, infos);
+}
+
+void pdos_batch_free(PdosBatchHandle* h) { delete h; }
+
+int pdos_batch(const Data* const* ds, const Cone* const* ks, int count, const PdosOptions* opt,
+               Sol** sols, PdosInfo* infos) {
+  g_err.clear();
+  if (count < 0 || (count > 0 && (!ds || !ks || !sols))) {
+    g_err = "bad batch arguments";
+    return -1;
+  }
+  if (count == 0) return 0;
+  for (int i = 0; i < count; ++i) sols[i] = nullptr;
+  PdosBatchHandle* h = batch_prepare(ds, ks, count, opt, true);
+  if (!h) return -1;
+  std::string err;
+  int rc = h->B.run(err) ? batch_fetch(h, sols, infos) : (g_err = err, -1);
+  delete h;
+  return rc;
+}
+
+void freeData(Data** d, Cone** k) {
+  if (d && *d) {
+    Data* p = *d;
+    std::free(p->b);
+    std::free(p->c);
+    std::free(p->Ax);
+    std::free(p->Ai);
+    std::free(p->Ap);
+    std::free(p->p);
+    std::free(p->x);
+    std::free(p->y);
+    std::free(p->s);
+    std::free(p);
+  }
+  if (k && *k) {
+    std::free((*k)->q);
+    std::free(*k);
+  }
+  if (d) *d = nullptr;
+  if (k) *k = nullptr;
+}
+
+void freeSol(Sol** sol) {
+  if (sol && *sol) {
+    std::free((*sol)->x);
+    std::free((*sol)->y);
+    std::free((*sol)->s);
+    std::free(*sol);
+  }
+  if (sol) *sol = nullptr;
+}
+
+void tic(timer* t) { clock_gettime(CLOCK_MONOTONIC, &t->tic); }
+
+double tocq(timer* t) {
+  clock_gettime(CLOCK_MONOTONIC, &t->toc);
+  return double(t->toc.tv_sec - t->tic.tv_sec) + 1e-9 * double(t->toc.tv_nsec - t->tic.tv_nsec);
+}
+
+double toc(timer* t) {
+  const double v = tocq(t);
+  std::printf("time: %8.4f seconds.\n", v);
+  return v;
+}
+
+}  // extern "C"

@@ -1,0 +1,72 @@
+// device.hpp -- records shared by host and device code.
+#pragma once
+
+#include <cstdint>
+
+namespace pdos_amd {
+
+// Result of one solve, written by the device.
+struct DevResult {
+  int iters;      // ADMM updates performed
+  int status;     // 0 Solved, 1 Indeterminate
+  int cg_iters;   // total CG iterations
+  int pad;
+  double p_res, d_res, p_obj, d_obj, eta;
+};
+
+// One problem instance for the workgroup engine.  All pointers are device
+// pointers; matrix values have the engine's bulk type (double or float).
+struct WgInst {
+  int n, m, f, l, nsoc, N;
+  int ga, gat;             // lanes per row for A rows / A' rows (power of 2, <= 64)
+  int max_iters, cg_max_its;
+  int nfl, nbl;            // direct: forward / backward level counts
+  double eps, alpha, cg_tol_sq, lambda, pscale, dscale;
+  const int* soc_ptr;      // nsoc + 1 absolute row offsets
+  const int* rp;           // A rows (CSR), m + 1
+  const int* ri;
+  const void* rx;
+  const int* cp;           // A' rows = A columns (CSC), n + 1
+  const int* ci;
+  const void* cx;
+  const double* b;         // scaled b, c and the scalings
+  const double* c;
+  const double* D;
+  const double* E;
+  const double* x0;        // scaled warm start (read once)
+  const double* s0;
+  const double* y0;
+  double* x;               // final scaled iterate (written once)
+  double* s;
+  double* y;
+  // direct plugin
+  const int* perm;         // N
+  const int* lrp;          // L rows (forward)
+  const int* lri;
+  const double* lrx;
+  const int* lcp;          // L columns (backward)
+  const int* lci;
+  const double* lcx;
+  const double* ld;        // D of L D L'
+  const int* fl_ptr;
+  const int* fl_rows;
+  const int* bl_ptr;
+  const int* bl_rows;
+  DevResult* res;
+  int lds_bytes;           // dynamic LDS this instance needs
+  int pad2;
+};
+
+constexpr int kWgThreads = 256;
+
+// Dynamic LDS bytes for one instance in the workgroup engine.
+inline int wg_lds_bytes(int n, int m, int N, bool direct, int tsize) {
+  auto al = [](long v) { return (long)((v + 15) & ~15L); };
+  long bytes;
+  if (!direct) bytes = al((long)(4 * (long)n + 5 * (long)m) * tsize);
+  else bytes = al((long)(n + 4 * (long)m) * tsize) + al((long)N * 8);
+  bytes += 2 * 16 * 4 * 8;  // reduction slots
+  return (int)bytes;
+}
+
+}  // namespace pdos_amd

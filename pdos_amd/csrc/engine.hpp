@@ -1,0 +1,74 @@
+// engine.hpp -- per-thread device context and the engine entry points.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/pdos.h"
+#include "device.hpp"
+#include "host.hpp"
+
+namespace pdos_amd {
+
+// Per-thread device state: one non-blocking stream and grow-only device and
+// pinned-host arenas, so a call allocates nothing once warm.
+struct Ctx {
+  int dev = -1;
+  hipStream_t stream = nullptr;
+  void* dbuf = nullptr;
+  size_t dcap = 0;
+  void* hbuf = nullptr;
+  size_t hcap = 0;
+  int max_lds = 65536;
+  int num_cu = 256;
+  bool ensure(int device, std::string& err);
+  bool reserve(size_t dbytes, size_t hbytes, std::string& err);
+};
+Ctx& ctx();
+
+// Host staging buffer: arrays at 256-byte aligned offsets.
+struct Packer {
+  std::vector<unsigned char> buf;
+  size_t put(const void* src, size_t bytes);
+  size_t zeros(size_t bytes);
+};
+
+int lanes_for(long nnz, long rows);
+double now_ms();
+
+hipError_t wg_launch(const WgInst* d_insts, int count, int lds_bytes, bool fp64, bool direct,
+                     hipStream_t st);
+
+// A prepared batch for the workgroup engine: inputs resident on the device,
+// re-runnable from the same start (the kernel reads x0/s0/y0, writes x/s/y).
+struct WgBatch {
+  std::vector<Setup> S;
+  std::vector<Factor> F;
+  std::vector<Params> par;
+  int device = 0;
+  bool fp64 = true, direct = false, own_mem = true;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  void* dbuf = nullptr;
+  size_t bytes = 0, res_off = 0, inst_off = 0;
+  std::vector<size_t> xo, so, yo;
+  int lds = 0;
+  double upload_ms = 0, kernel_ms = 0, wall_ms = 0;
+  ~WgBatch();
+  bool prepare(int device, bool fp64, bool direct, bool borrow_ctx_arena, std::string& err);
+  bool run(std::string& err);
+  bool fetch(std::vector<DevResult>& res, std::string& err);
+};
+
+bool solve_wg(std::vector<Setup>& S, std::vector<Factor>& F, const std::vector<const Params*>& par,
+              bool fp64, bool direct, int device, std::vector<DevResult>& res, double& upload_ms,
+              double& solve_ms, std::string& err);
+
+// Whole-GPU engine for one large instance (indirect plugin).
+bool grid_preferred(const Setup& S, bool fp64);
+bool solve_grid(Setup& S, const Params& par, bool fp64, int device, std::vector<DevResult>& res,
+                double& upload_ms, double& solve_ms, std::string& err);
+
+}  // namespace pdos_amd

@@ -1,0 +1,355 @@
+// host_setup.cpp -- problem setup (scaling, lambda, transpose) and the
+// direct plugin's host factorization (ordering, LDL', level sets).
+//
+// Setup follows common.h:35-218 step for step in binary64, so the scaled data
+// the GPU iterates on are bit-identical to the reference's.  Unlike the
+// reference it never writes to the caller's arrays.
+#include "host.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace pdos_amd {
+
+namespace {
+constexpr double kSqrtRatio = 1e3;  // common.h:16
+constexpr double kRangeMax = 1e6;   // common.h:17
+constexpr double kRangeMin = 1e-6;  // common.h:18
+
+double norm2(const double* v, long n) {
+  double a = 0.0;
+  for (long i = 0; i < n; ++i) a += v[i] * v[i];
+  return std::sqrt(a);
+}
+
+double clamp_rsqrt(double v) {
+  v = std::fabs(v);
+  v = v > kRangeMin ? v : kRangeMin;
+  v = v > kRangeMax ? kRangeMax : v;
+  return 1.0 / std::sqrt(v);
+}
+}  // namespace
+
+bool build_setup(const Data* d, const Cone* k, Setup& S, std::string& err) {
+  if (!d || !k) { err = "NULL data or cone"; return false; }
+  if (!d->p) { err = "NULL params"; return false; }
+  if (d->n < 1 || d->m < 1) { err = "n and m must be positive"; return false; }
+  if (!d->Ap || !d->b || !d->c) { err = "NULL Ap, b or c"; return false; }
+  const long n = d->n, m = d->m, nz = d->Ap[n];
+  if (nz < 0 || nz > 0x7fffffffL || n > 0x3fffffffL || m > 0x3fffffffL) {
+    err = "problem too large for 32-bit device indices"; return false;
+  }
+  if (nz > 0 && (!d->Ai || !d->Ax)) { err = "NULL Ai or Ax"; return false; }
+  long tot = k->f + k->l;
+  if (k->f < 0 || k->l < 0 || k->qsize < 0 || (k->qsize > 0 && !k->q)) { err = "bad cone"; return false; }
+  for (long i = 0; i < k->qsize; ++i) {
+    if (k->q[i] < 1) { err = "SOC size < 1"; return false; }
+    tot += k->q[i];
+  }
+  if (tot != m) { err = "cone sizes do not sum to m"; return false; }
+  for (long j = 0; j < n; ++j)
+    if (d->Ap[j + 1] < d->Ap[j]) { err = "Ap not monotone"; return false; }
+  for (long p = 0; p < nz; ++p)
+    if (d->Ai[p] < 0 || d->Ai[p] >= m) { err = "row index out of range"; return false; }
+
+  S.n = (int)n; S.m = (int)m; S.f = (int)k->f; S.l = (int)k->l;
+  S.soc_ptr.assign(k->qsize + 1, 0);
+  S.soc_ptr[0] = S.f + S.l;
+  for (long i = 0; i < k->qsize; ++i) S.soc_ptr[i + 1] = S.soc_ptr[i] + (int)k->q[i];
+
+  S.cp.resize(n + 1);
+  S.ci.resize(nz);
+  for (long j = 0; j <= n; ++j) S.cp[j] = (int)d->Ap[j];
+  for (long p = 0; p < nz; ++p) S.ci[p] = (int)d->Ai[p];
+  S.cx.assign(d->Ax, d->Ax + nz);
+  S.b.assign(d->b, d->b + m);
+  S.c.assign(d->c, d->c + n);
+  S.x.assign(n, 0.0);
+  S.s.assign(m, 0.0);
+  S.y.assign(m, 0.0);
+  if (d->x) std::memcpy(S.x.data(), d->x, sizeof(double) * n);
+  if (d->s) std::memcpy(S.s.data(), d->s, sizeof(double) * m);
+  if (d->y) std::memcpy(S.y.data(), d->y, sizeof(double) * m);
+  S.D.assign(m, 0.0);
+  S.E.assign(n, 0.0);
+
+  // stop-test scales use the caller's data (pdos.c:76-77)
+  S.pscale = 1.0 + norm2(d->b, m);
+  S.dscale = 1.0 + norm2(d->c, n);
+
+  if (d->p->NORMALIZE) {
+    for (long j = 0; j < n; ++j)
+      for (long p = d->Ap[j]; p < d->Ap[j + 1]; ++p) S.D[d->Ai[p]] += d->Ax[p] * d->Ax[p];
+    for (int c = 0; c < S.nsoc(); ++c) {
+      const int r0 = S.soc_ptr[c], q = S.soc_ptr[c + 1] - r0;
+      double acc = 0.0;
+      for (int i = 0; i < q; ++i) acc += S.D[r0 + i];
+      for (int i = 0; i < q; ++i) S.D[r0 + i] = acc / (long)q;
+    }
+    for (long i = 0; i < m; ++i) S.D[i] = clamp_rsqrt(S.D[i]);
+    for (long j = 0; j < n; ++j)
+      for (long p = d->Ap[j]; p < d->Ap[j + 1]; ++p) S.cx[p] *= S.D[d->Ai[p]];
+    for (long j = 0; j < n; ++j)
+      for (long p = d->Ap[j]; p < d->Ap[j + 1]; ++p) S.E[j] += S.cx[p] * S.cx[p];
+    for (long j = 0; j < n; ++j) S.E[j] = clamp_rsqrt(S.E[j]);
+    for (long j = 0; j < n; ++j)
+      for (long p = d->Ap[j]; p < d->Ap[j + 1]; ++p) S.cx[p] *= S.E[j];
+    for (long j = 0; j < n; ++j) S.c[j] = d->c[j] * S.E[j];
+    for (long i = 0; i < m; ++i) S.b[i] = d->b[i] * S.D[i];
+    for (long i = 0; i < m; ++i) S.y[i] /= S.D[i];
+    for (long j = 0; j < n; ++j) S.x[j] /= S.E[j];
+    for (long i = 0; i < m; ++i) S.s[i] *= S.D[i];
+  } else {
+    std::fill(S.D.begin(), S.D.end(), 1.0);
+    std::fill(S.E.begin(), S.E.end(), 1.0);
+  }
+  S.lambda = (1e-15 + norm2(S.b.data(), m)) / (1e-15 + norm2(S.c.data(), n));
+  for (long j = 0; j < n; ++j) {
+    S.E[j] *= kSqrtRatio;
+    S.c[j] *= kSqrtRatio;
+    S.x[j] /= kSqrtRatio;
+  }
+  for (long p = 0; p < nz; ++p) S.cx[p] *= kSqrtRatio;
+
+  // rows of A: stable counting sort of the CSC entries by row
+  S.rp.assign(m + 1, 0);
+  S.ri.resize(nz);
+  S.rx.resize(nz);
+  std::vector<int> next(m + 1, 0);
+  for (long p = 0; p < nz; ++p) next[S.ci[p] + 1]++;
+  for (long i = 0; i < m; ++i) next[i + 1] += next[i];
+  std::copy(next.begin(), next.end(), S.rp.begin());
+  for (long j = 0; j < n; ++j)
+    for (int p = S.cp[j]; p < S.cp[j + 1]; ++p) {
+      const int q = next[S.ci[p]]++;
+      S.ri[q] = (int)j;
+      S.rx[q] = S.cx[p];
+    }
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// Direct plugin: KKT assembly, ordering, LDL', level sets.
+
+namespace {
+
+// Upper triangle of K = [[-I_n, A'], [A, I_m]] in CSC with the entries of each
+// column in the order direct/private.c:77-120 produces them: column n+r holds
+// A(r, j) for increasing j, then the +1 diagonal.
+void form_kkt(const Setup& S, std::vector<int>& Kp, std::vector<int>& Ki, std::vector<double>& Kx) {
+  const int n = S.n, m = S.m, N = n + m;
+  Kp.assign(N + 1, 0);
+  for (int j = 0; j < n; ++j) Kp[j + 1] = 1;
+  for (int r = 0; r < m; ++r) Kp[n + r + 1] = (S.rp[r + 1] - S.rp[r]) + 1;
+  for (int j = 0; j < N; ++j) Kp[j + 1] += Kp[j];
+  Ki.resize(Kp[N]);
+  Kx.resize(Kp[N]);
+  for (int j = 0; j < n; ++j) { Ki[Kp[j]] = j; Kx[Kp[j]] = -1.0; }
+  for (int r = 0; r < m; ++r) {
+    int q = Kp[n + r];
+    for (int p = S.rp[r]; p < S.rp[r + 1]; ++p) { Ki[q] = S.ri[p]; Kx[q] = S.rx[p]; ++q; }
+    Ki[q] = n + r;
+    Kx[q] = 1.0;
+  }
+}
+
+// Exact minimum degree on a bitset elimination graph; ties go to the lowest
+// index.  O(N^2/64) memory, so limited to N <= kMaxMdN.
+constexpr int kMaxMdN = 32768;
+
+bool min_degree(int N, const std::vector<int>& Kp, const std::vector<int>& Ki, std::vector<int>& perm,
+                std::string& err) {
+  if (N > kMaxMdN) { err = "direct plugin: KKT too large for the host ordering"; return false; }
+  const int W = (N + 63) / 64;
+  std::vector<uint64_t> adj((size_t)N * W, 0);
+  for (int j = 0; j < N; ++j)
+    for (int p = Kp[j]; p < Kp[j + 1]; ++p) {
+      const int i = Ki[p];
+      if (i == j) continue;
+      adj[(size_t)i * W + j / 64] |= 1ull << (j % 64);
+      adj[(size_t)j * W + i / 64] |= 1ull << (i % 64);
+    }
+  std::vector<int> deg(N);
+  std::vector<char> dead(N, 0);
+  for (int v = 0; v < N; ++v) {
+    int c = 0;
+    for (int t = 0; t < W; ++t) c += __builtin_popcountll(adj[(size_t)v * W + t]);
+    deg[v] = c;
+  }
+  perm.resize(N);
+  for (int step = 0; step < N; ++step) {
+    int v = -1;
+    for (int j = 0; j < N; ++j)
+      if (!dead[j] && (v < 0 || deg[j] < deg[v])) v = j;
+    perm[step] = v;
+    dead[v] = 1;
+    const uint64_t* av = &adj[(size_t)v * W];
+    for (int t = 0; t < W; ++t) {
+      uint64_t bits = av[t];
+      while (bits) {
+        const int u = t * 64 + __builtin_ctzll(bits);
+        bits &= bits - 1;
+        uint64_t* au = &adj[(size_t)u * W];
+        for (int s = 0; s < W; ++s) au[s] |= av[s];
+        au[u / 64] &= ~(1ull << (u % 64));
+        au[v / 64] &= ~(1ull << (v % 64));
+        int c = 0;
+        for (int s = 0; s < W; ++s) c += __builtin_popcountll(au[s]);
+        deg[u] = c;
+      }
+    }
+  }
+  return true;
+}
+
+}  // namespace
+
+bool kkt_min_degree(const Setup& S, std::vector<int>& perm, std::string& err) {
+  std::vector<int> Kp, Ki;
+  std::vector<double> Kx;
+  form_kkt(S, Kp, Ki, Kx);
+  return min_degree(S.n + S.m, Kp, Ki, perm, err);
+}
+
+bool build_factor(const Setup& S, const std::vector<int>& perm_in, Factor& F, std::string& err) {
+  const int N = S.n + S.m;
+  F.N = N;
+  std::vector<int> Kp, Ki;
+  std::vector<double> Kx;
+  form_kkt(S, Kp, Ki, Kx);
+  if (!perm_in.empty()) {
+    if ((int)perm_in.size() != N) { err = "permutation has wrong length"; return false; }
+    F.P = perm_in;
+  } else if (!min_degree(N, Kp, Ki, F.P, err)) {
+    return false;
+  }
+  std::vector<int> pinv(N, -1);
+  for (int k = 0; k < N; ++k) {
+    if (F.P[k] < 0 || F.P[k] >= N || pinv[F.P[k]] >= 0) { err = "invalid permutation"; return false; }
+    pinv[F.P[k]] = k;
+  }
+  // C = upper triangle of P K P' (column max(i2, j2) gets row min(i2, j2)),
+  // entries in the visiting order of K's columns (cs.c:109-144).
+  std::vector<int> Cp(N + 1, 0), Ci(Kp[N]);
+  std::vector<double> Cx(Kp[N]);
+  for (int j = 0; j < N; ++j)
+    for (int p = Kp[j]; p < Kp[j + 1]; ++p) {
+      if (Ki[p] > j) continue;
+      Cp[std::max(pinv[Ki[p]], pinv[j]) + 1]++;
+    }
+  for (int j = 0; j < N; ++j) Cp[j + 1] += Cp[j];
+  {
+    std::vector<int> nxt(Cp.begin(), Cp.end() - 1);
+    for (int j = 0; j < N; ++j)
+      for (int p = Kp[j]; p < Kp[j + 1]; ++p) {
+        if (Ki[p] > j) continue;
+        const int i2 = pinv[Ki[p]], j2 = pinv[j];
+        const int q = nxt[std::max(i2, j2)]++;
+        Ci[q] = std::min(i2, j2);
+        Cx[q] = Kx[p];
+      }
+  }
+  // Up-looking LDL' (the algorithm of ldl.c:187-319): the etree and column
+  // counts first, then row k of L as a sparse triangular solve over the
+  // etree reach of column k.
+  std::vector<int> parent(N), lnz(N), flag(N), pattern(N);
+  F.Lp.assign(N + 1, 0);
+  for (int k = 0; k < N; ++k) {
+    parent[k] = -1;
+    flag[k] = k;
+    lnz[k] = 0;
+    for (int p = Cp[k]; p < Cp[k + 1]; ++p) {
+      int i = Ci[p];
+      if (i >= k) continue;
+      for (; flag[i] != k; i = parent[i]) {
+        if (parent[i] == -1) parent[i] = k;
+        lnz[i]++;
+        flag[i] = k;
+      }
+    }
+  }
+  for (int k = 0; k < N; ++k) F.Lp[k + 1] = F.Lp[k] + lnz[k];
+  F.Li.resize(F.Lp[N]);
+  F.Lx.resize(F.Lp[N]);
+  F.Ld.resize(N);
+  std::vector<double> Y(N, 0.0);
+  for (int k = 0; k < N; ++k) {
+    int top = N;
+    Y[k] = 0.0;
+    flag[k] = k;
+    lnz[k] = 0;
+    for (int p = Cp[k]; p < Cp[k + 1]; ++p) {
+      int i = Ci[p];
+      if (i > k) continue;
+      Y[i] += Cx[p];
+      int len = 0;
+      for (; flag[i] != k; i = parent[i]) {
+        pattern[len++] = i;
+        flag[i] = k;
+      }
+      while (len > 0) pattern[--top] = pattern[--len];
+    }
+    F.Ld[k] = Y[k];
+    Y[k] = 0.0;
+    for (; top < N; ++top) {
+      const int i = pattern[top];
+      const double yi = Y[i];
+      Y[i] = 0.0;
+      const int pend = F.Lp[i] + lnz[i];
+      int q = F.Lp[i];
+      for (; q < pend; ++q) Y[F.Li[q]] -= F.Lx[q] * yi;
+      const double lki = yi / F.Ld[i];
+      F.Ld[k] -= lki * yi;
+      F.Li[q] = k;
+      F.Lx[q] = lki;
+      lnz[i]++;
+    }
+    if (F.Ld[k] == 0.0) { err = "zero pivot in LDL'"; return false; }
+  }
+  // Rows of L (CSR) for the forward sweep, columns ascending within a row.
+  const int lnz_tot = F.Lp[N];
+  F.Rp.assign(N + 1, 0);
+  F.Ri.resize(lnz_tot);
+  F.Rx.resize(lnz_tot);
+  for (int p = 0; p < lnz_tot; ++p) F.Rp[F.Li[p] + 1]++;
+  for (int i = 0; i < N; ++i) F.Rp[i + 1] += F.Rp[i];
+  {
+    std::vector<int> nxt(F.Rp.begin(), F.Rp.end() - 1);
+    for (int j = 0; j < N; ++j)
+      for (int p = F.Lp[j]; p < F.Lp[j + 1]; ++p) {
+        const int q = nxt[F.Li[p]]++;
+        F.Ri[q] = j;
+        F.Rx[q] = F.Lx[p];
+      }
+  }
+  // Level sets: forward level of row i = 1 + max level of the columns it
+  // reads; backward level of column j = 1 + max level of the rows it reads.
+  std::vector<int> lev(N, 0);
+  int nlev = 0;
+  for (int j = 0; j < N; ++j) {
+    for (int p = F.Lp[j]; p < F.Lp[j + 1]; ++p) lev[F.Li[p]] = std::max(lev[F.Li[p]], lev[j] + 1);
+    nlev = std::max(nlev, lev[j] + 1);
+  }
+  auto bucket = [&](const std::vector<int>& L, int nl, std::vector<int>& ptr, std::vector<int>& rows) {
+    ptr.assign(nl + 1, 0);
+    for (int i = 0; i < N; ++i) ptr[L[i] + 1]++;
+    for (int t = 0; t < nl; ++t) ptr[t + 1] += ptr[t];
+    rows.resize(N);
+    std::vector<int> nxt(ptr.begin(), ptr.end() - 1);
+    for (int i = 0; i < N; ++i) rows[nxt[L[i]]++] = i;
+  };
+  bucket(lev, nlev, F.fl_ptr, F.fl_rows);
+  std::fill(lev.begin(), lev.end(), 0);
+  nlev = 0;
+  for (int j = N - 1; j >= 0; --j) {
+    int lv = 0;
+    for (int p = F.Lp[j]; p < F.Lp[j + 1]; ++p) lv = std::max(lv, lev[F.Li[p]] + 1);
+    lev[j] = lv;
+    nlev = std::max(nlev, lv + 1);
+  }
+  bucket(lev, nlev, F.bl_ptr, F.bl_rows);
+  return true;
+}
+
+}  // namespace pdos_amd

@@ -1,0 +1,347 @@
+// wg_engine.hip -- the whole ADMM solve of one cone program in one workgroup.
+//
+// One workgroup (256 threads, 4 wave64) owns one instance for its whole solve:
+// every iterate lives in LDS, matrices are read from global memory (L2/MALL
+// resident for small instances), and the only synchronisation is the
+// workgroup barrier.  A batch is one launch with one workgroup per instance
+// (the hardware dispatcher keeps the CUs busy as instances retire); a single
+// small problem is a launch of one workgroup.
+//
+// Per ADMM iteration (pdos.c:94-120) with k CG steps the workgroup runs
+// k + 1 SpMVs with A and k + 2 with A' (the A x row sums of the final
+// s~ = b - A x are reused by the next iteration's residual and CG start,
+// which is exact: x does not change in between), and 4 + 3k barriers.
+//
+// Reference map:
+//   residual check      linAlg.h:158-201, pdos.c:95-107
+//   CG on (I + A'A)      indirect/private.c:20-117 (p = beta p + q fused into
+//                        the next A' sweep; the A sweep gathers beta p + q)
+//   LDL' solve           direct/private.c:16-56, ldl.c:326-428 (level sets)
+//   relax / cones / dual pdos.c:232-249, 296-303; cones.c:5-57
+#include <hip/hip_runtime.h>
+
+#include "device.hpp"
+
+namespace pdos_amd {
+
+namespace {
+
+constexpr int NT = kWgThreads;
+constexpr int NW = NT / 64;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T group_sum(T v, int G) {
+  for (int o = G >> 1; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Sums NV doubles over the workgroup; every thread receives the totals.  One
+// barrier; two alternating slots make back-to-back reductions race free (a
+// slot is rewritten only after every thread has passed the next barrier).
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* red, int& slot) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+  double* buf = red + slot * (16 * 4);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) buf[w * 4 + k] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double a = 0.0;
+#pragma unroll
+    for (int j = 0; j < NW; ++j) a += buf[j * 4 + k];
+    v[k] = a;
+  }
+  slot ^= 1;
+}
+
+// Row-gather SpMV over `nrows` compressed rows with G lanes per row.  The
+// outer loop bound is uniform so every lane reaches the shuffles.
+// gx(col) supplies the gathered operand, emit(row, sum) runs on lane 0 of
+// the row's group.
+template <typename T, typename Gather, typename Emit>
+__device__ __forceinline__ void spmv_rows(int nrows, const int* __restrict__ ptr,
+                                          const int* __restrict__ idx, const T* __restrict__ val,
+                                          int G, Gather gx, Emit emit) {
+  const int g = threadIdx.x & (G - 1);
+  const int grp = threadIdx.x / G, ngrp = NT / G;
+  for (int r0 = 0; r0 < nrows; r0 += ngrp) {
+    const int r = r0 + grp;
+    T acc = T(0);
+    if (r < nrows) {
+      const int pe = ptr[r + 1];
+      for (int p = ptr[r] + g; p < pe; p += G) acc += val[p] * gx(idx[p]);
+    }
+    acc = group_sum(acc, G);
+    if (r < nrows && g == 0) emit(r, acc);
+  }
+}
+
+// One level-scheduled sparse triangular sweep over LDS vector z.
+// Levels with at most NW rows give each row a whole wave; wider levels give
+// each row one thread (sequential in storage order).
+template <bool BACK>
+__device__ __forceinline__ void tri_sweep(int nlev, const int* __restrict__ lptr,
+                                          const int* __restrict__ lrows, const int* __restrict__ ptr,
+                                          const int* __restrict__ idx, const double* __restrict__ val,
+                                          const double* __restrict__ dg, double* z) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int lv = 0; lv < nlev; ++lv) {
+    const int a = lptr[lv], e = lptr[lv + 1];
+    if (e - a <= NW) {
+      if (a + w < e) {
+        const int i = lrows[a + w];
+        double acc = 0.0;
+        for (int p = ptr[i] + lane; p < ptr[i + 1]; p += 64) acc += val[p] * z[idx[p]];
+        acc = wave_sum(acc);
+        if (lane == 0) z[i] = (BACK ? z[i] / dg[i] : z[i]) - acc;
+      }
+    } else {
+      for (int rr = a + (int)threadIdx.x; rr < e; rr += NT) {
+        const int i = lrows[rr];
+        double v = BACK ? z[i] / dg[i] : z[i];
+        for (int p = ptr[i]; p < ptr[i + 1]; ++p) v -= val[p] * z[idx[p]];
+        z[i] = v;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T, bool DIRECT>
+__global__ __launch_bounds__(NT) void k_wg_solve(const WgInst* __restrict__ insts) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  const WgInst& I = insts[blockIdx.x];
+  const int n = I.n, m = I.m, tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6;
+
+  // LDS carve-up (matches wg_lds_bytes)
+  T* x = reinterpret_cast<T*>(smem);
+  T *p = nullptr, *q = nullptr, *u = nullptr, *t = nullptr;
+  T *s, *y, *st, *ax;
+  double* bp = nullptr;
+  unsigned char* tail;
+  if (!DIRECT) {
+    p = x + n; q = p + n; u = q + n;
+    s = u + n; y = s + m; st = y + m; t = st + m; ax = t + m;
+    tail = smem + ((((size_t)(4 * n + 5 * m) * sizeof(T)) + 15) & ~(size_t)15);
+  } else {
+    s = x + n; y = s + m; st = y + m; ax = st + m;
+    unsigned char* after = smem + ((((size_t)(n + 4 * m) * sizeof(T)) + 15) & ~(size_t)15);
+    bp = reinterpret_cast<double*>(after);
+    tail = after + (((size_t)I.N * 8 + 15) & ~(size_t)15);
+  }
+  double* red = reinterpret_cast<double*>(tail);
+
+  const T* __restrict__ rx = reinterpret_cast<const T*>(I.rx);
+  const T* __restrict__ cx = reinterpret_cast<const T*>(I.cx);
+  const int* __restrict__ rp = I.rp;
+  const int* __restrict__ ri = I.ri;
+  const int* __restrict__ cp = I.cp;
+  const int* __restrict__ ci = I.ci;
+  const double* __restrict__ bg = I.b;
+  const double* __restrict__ cg = I.c;
+  const double* __restrict__ Dg = I.D;
+  const double* __restrict__ Eg = I.E;
+  const int f = I.f, lend = I.f + I.l, nsoc = I.nsoc;
+  const int ga = I.ga, gat = I.gat;
+  const T lam = T(I.lambda);
+  const T relax_a = T(I.alpha);
+  const T relax_b = T(1.0) - T(I.alpha);
+
+  for (int i = tid; i < n; i += NT) x[i] = T(I.x0[i]);
+  for (int j = tid; j < m; j += NT) { s[j] = T(I.s0[j]); y[j] = T(I.y0[j]); }
+  __syncthreads();
+  spmv_rows<T>(m, rp, ri, rx, ga, [&](int c) { return x[c]; }, [&](int j, T v) { ax[j] = v; });
+  __syncthreads();
+
+  int slot = 0, it = 0, status = 1, cg_total = 0;
+  double pres = -1.0, dres = -1.0, pobj = -1.0, dobj = -1.0, eta = -1.0;
+
+  for (it = 0; it < I.max_iters; ++it) {
+    // ---- stop check (linAlg.h:158-201); also the CG start argument t ----
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};  // |pres|^2, |dres|^2, c'x, b'y
+    for (int j = tid; j < m; j += NT) {
+      const T sj = s[j], bj = T(bg[j]), yj = y[j], axj = ax[j];
+      const T r = ((sj - bj) + axj) / T(Dg[j]);
+      acc[0] += double(r) * double(r);
+      acc[3] += double(bj) * double(yj);
+      if (!DIRECT) t[j] = ((sj + lam * yj) - bj) + axj;
+    }
+    spmv_rows<T>(n, cp, ci, cx, gat, [&](int c) { return y[c]; }, [&](int i, T v) {
+      const T ci_ = T(cg[i]);
+      const T r = (ci_ + v) / T(Eg[i]);
+      acc[1] += double(r) * double(r);
+      acc[2] += double(ci_) * double(x[i]);
+    });
+    block_sum<4>(acc, red, slot);
+    pres = sqrt(acc[0]);
+    dres = sqrt(acc[1]);
+    pobj = acc[2];
+    dobj = -acc[3];
+    eta = fabs(pobj - dobj);
+    if (pres < I.eps * I.pscale && dres < I.eps * I.dscale &&
+        eta < I.eps * (1.0 + fabs(pobj) + fabs(dobj))) {
+      status = 0;
+      break;
+    }
+
+    if (!DIRECT) {
+      // ---- CG on (I + A'A) x = x - lam c + A'(b - s - lam y) ----
+      double r1[1] = {0.0};
+      spmv_rows<T>(n, cp, ci, cx, gat, [&](int c) { return t[c]; }, [&](int i, T v) {
+        const T qi = (-lam) * T(cg[i]) - v;
+        q[i] = qi;
+        p[i] = T(0);
+        r1[0] += double(qi) * double(qi);
+      });
+      block_sum<1>(r1, red, slot);
+      double rho = r1[0];
+      T beta = T(0);
+      int k = 0;
+      if (rho > I.cg_tol_sq) {
+        while (k < I.cg_max_its) {
+          spmv_rows<T>(m, rp, ri, rx, ga, [&](int c) { return beta * p[c] + q[c]; },
+                       [&](int j, T v) { t[j] = v; });
+          __syncthreads();
+          double r2[1] = {0.0};
+          spmv_rows<T>(n, cp, ci, cx, gat, [&](int c) { return t[c]; }, [&](int i, T v) {
+            const T pi = beta * p[i] + q[i];
+            p[i] = pi;
+            const T ui = pi + v;
+            u[i] = ui;
+            r2[0] += double(pi) * double(ui);
+          });
+          block_sum<1>(r2, red, slot);
+          const T alpha = T(rho / r2[0]);
+          double r3[1] = {0.0};
+          for (int i = tid; i < n; i += NT) {
+            x[i] += alpha * p[i];
+            const T qi = q[i] - alpha * u[i];
+            q[i] = qi;
+            r3[0] += double(qi) * double(qi);
+          }
+          block_sum<1>(r3, red, slot);
+          ++k;
+          if (r3[0] < I.cg_tol_sq) break;
+          beta = T(r3[0] / rho);
+          rho = r3[0];
+        }
+      }
+      cg_total += k;
+    } else {
+      // ---- LDL' solve of the KKT system, permuted into bp ----
+      const int N = I.N;
+      for (int kk = tid; kk < N; kk += NT) {
+        const int src = I.perm[kk];
+        double v;
+        if (src < n) {
+          v = double(lam) * cg[src] - double(x[src]);
+        } else {
+          const int j = src - n;
+          v = bg[j] - (double(s[j]) + double(lam) * double(y[j]));
+        }
+        bp[kk] = v;
+      }
+      __syncthreads();
+      tri_sweep<false>(I.nfl, I.fl_ptr, I.fl_rows, I.lrp, I.lri, I.lrx, nullptr, bp);
+      tri_sweep<true>(I.nbl, I.bl_ptr, I.bl_rows, I.lcp, I.lci, I.lcx, I.ld, bp);
+      for (int kk = tid; kk < N; kk += NT) {
+        const int dst = I.perm[kk];
+        const double v = bp[kk];
+        if (dst < n) {
+          x[dst] = T(v);
+        } else {
+          const int j = dst - n;
+          st[j] = T(v + (double(s[j]) + double(lam) * double(y[j])));
+        }
+      }
+      __syncthreads();
+    }
+
+    // ---- s~ = b - A x, relax, project, dual update (pdos.c:232-249,296-303) ----
+    spmv_rows<T>(m, rp, ri, rx, ga, [&](int c) { return x[c]; }, [&](int j, T v) {
+      ax[j] = v;
+      T stj = DIRECT ? st[j] : T(bg[j]) - v;
+      stj = relax_a * stj + relax_b * s[j];
+      const T z = stj - lam * y[j];
+      if (j < f) {
+        s[j] = T(0);
+        y[j] += (T(0) - stj) / lam;
+      } else if (j < lend) {
+        const T sj = z < T(0) ? T(0) : z;
+        s[j] = sj;
+        y[j] += (sj - stj) / lam;
+      } else {
+        st[j] = stj;
+        s[j] = z;
+      }
+    });
+    __syncthreads();
+    for (int cn = wv; cn < nsoc; cn += NW) {
+      const int r0 = I.soc_ptr[cn], r1 = I.soc_ptr[cn + 1];
+      double a2 = 0.0;
+      for (int j = r0 + 1 + lane; j < r1; j += 64) a2 += double(s[j]) * double(s[j]);
+      a2 = wave_sum(a2);
+      const T ns = T(sqrt(a2));
+      const T t0 = s[r0];
+      // 0: inside the cone, 1: inside the polar cone, 2: onto the boundary
+      const int mode = ns <= t0 ? 0 : (ns <= -t0 ? 1 : 2);
+      const T a = (ns + t0) / T(2);
+      for (int j = r0 + lane; j < r1; j += 64) {
+        const T zj = s[j];
+        const T sj = mode == 0 ? zj : (mode == 1 ? T(0) : (j == r0 ? a : a * zj / ns));
+        s[j] = sj;
+        y[j] += (sj - st[j]) / lam;
+      }
+    }
+    __syncthreads();
+  }
+
+  for (int i = tid; i < n; i += NT) I.x[i] = double(x[i]);
+  for (int j = tid; j < m; j += NT) { I.s[j] = double(s[j]); I.y[j] = double(y[j]); }
+  if (tid == 0) {
+    DevResult* R = I.res;
+    R->iters = it;
+    R->status = status;
+    R->cg_iters = cg_total;
+    R->p_res = pres;
+    R->d_res = dres;
+    R->p_obj = pobj;
+    R->d_obj = dobj;
+    R->eta = eta;
+  }
+}
+
+template <typename T, bool DIRECT>
+hipError_t launch_t(const WgInst* d_insts, int count, int lds_bytes, hipStream_t st) {
+  auto fn = k_wg_solve<T, DIRECT>;
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(fn, dim3(count), dim3(NT), lds_bytes, st, d_insts);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+// Launches the workgroup engine over `count` instances (one workgroup each).
+hipError_t wg_launch(const WgInst* d_insts, int count, int lds_bytes, bool fp64, bool direct,
+                     hipStream_t st) {
+  if (fp64) return direct ? launch_t<double, true>(d_insts, count, lds_bytes, st)
+                          : launch_t<double, false>(d_insts, count, lds_bytes, st);
+  return direct ? launch_t<float, true>(d_insts, count, lds_bytes, st)
+                : launch_t<float, false>(d_insts, count, lds_bytes, st);
+}
+
+}  // namespace pdos_amd

@@ -127,7 +127,11 @@ def main():
     def add_generated(tag, prob):
         path = f"inputs/{tag}.bin"
         write_problem_file(os.path.join(HERE, path), prob)
-        cases.append((tag, path, read_problem_file(os.path.join(HERE, path))))
+        back = read_problem_file(os.path.join(HERE, path))
+        if prob.x0 is not None:  # the demo format has no warm start: keep it as arrays
+            arrays[f"{tag}__x0"], arrays[f"{tag}__y0"], arrays[f"{tag}__s0"] = prob.x0, prob.y0, prob.s0
+            back = back.with_warm(prob.x0, prob.y0, prob.s0)
+        cases.append((tag, path, back))
 
     base = problems.portfolio(seed=0)
     add_generated("portfolio_n101_norm0", base.with_params(NORMALIZE=0))

@@ -35,7 +35,11 @@ def case_names():
 
 
 def problem(tag):
-    return read_problem_file(os.path.join(GOLDEN, manifest()["cases"][tag]["input"]))
+    p = read_problem_file(os.path.join(GOLDEN, manifest()["cases"][tag]["input"]))
+    a = arrays()
+    if f"{tag}__x0" in a:
+        p = p.with_warm(a[f"{tag}__x0"], a[f"{tag}__y0"], a[f"{tag}__s0"])
+    return p
 
 
 def expected(tag, plugin):

@@ -1,0 +1,101 @@
+"""GPU parity: the HIP path against the compiled reference's golden outputs.
+
+FP64 mode: same status and iteration count as the reference, and x, s, y
+within 1e-6 relative (norm-wise; the bar the solver's FP64 mode promises).
+Mixed mode (binary32 bulk, binary64 norms): status Solved and the returned
+point within EPS_ABS-scaled distance of the FP64 optimum.
+"""
+import ctypes as C
+import numpy as np
+import pytest
+
+import golden_data as G
+import pdos_amd
+from pdos_amd import problems
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-6
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12)
+
+
+def _check_fp64(out, exp, tag):
+    assert out["status"] == exp["status"], tag
+    assert out["info"]["iters"] == exp["iters"], (tag, out["info"]["iters"], exp["iters"])
+    for k in ("x", "s", "y"):
+        assert _rel(out[k], exp[k]) <= RTOL, (tag, k, _rel(out[k], exp[k]))
+
+
+@pytest.mark.parametrize("tag", G.case_names())
+@pytest.mark.parametrize("plugin", ["indirect", "direct"])
+def test_wg_fp64_vs_reference(tag, plugin):
+    print(f"case {tag} {plugin}", flush=True)
+    out = pdos_amd.solve_problem(G.problem(tag), plugin=plugin, precision="fp64", engine="wg")
+    _check_fp64(out, G.expected(tag, plugin), f"{tag}/{plugin}")
+
+
+@pytest.mark.parametrize("tag", ["random_n300_m900", "portfolio_n101_norm1", "fixture03",
+                                 "portfolio_n101_steps10", "portfolio_n101_cg1"])
+def test_grid_fp64_vs_reference(tag):
+    print(f"case {tag} grid", flush=True)
+    out = pdos_amd.solve_problem(G.problem(tag), plugin="indirect", precision="fp64", engine="grid")
+    _check_fp64(out, G.expected(tag, "indirect"), f"{tag}/grid")
+
+
+@pytest.mark.parametrize("engine", ["wg", "grid"])
+@pytest.mark.parametrize("tag", ["fixture01", "portfolio_n101_norm0", "random_n300_m900", "uneven_1"])
+def test_mixed_precision_within_eps(tag, engine):
+    print(f"case {tag} mixed {engine}", flush=True)
+    prob = G.problem(tag)
+    exp = G.expected(tag, "indirect")
+    out = pdos_amd.solve_problem(prob, plugin="indirect", precision="mixed", engine=engine)
+    assert out["status"] == "Solved"
+    eps = prob.params["EPS_ABS"]
+    # objective agrees to the stop test's relative gap tolerance
+    pobj, pobj_ref = prob.c @ out["x"], prob.c @ exp["x"]
+    assert abs(pobj - pobj_ref) <= 10 * eps * (1 + abs(pobj_ref))
+    # and the returned point meets the stop test in binary64 arithmetic
+    A = prob.dense_A()
+    assert np.linalg.norm(A @ out["x"] + out["s"] - prob.b) < 1.01 * eps * (1 + np.linalg.norm(prob.b))
+    assert np.linalg.norm(prob.c + A.T @ out["y"]) < 1.01 * eps * (1 + np.linalg.norm(prob.c))
+
+
+def test_batch_matches_single():
+    tags = ["fixture01", "fixture02", "uneven_1", "portfolio_n101_norm1", "random_n300_m900"]
+    probs = [G.problem(t) for t in tags]
+    print("batch of", len(probs), flush=True)
+    outs = pdos_amd.solve_batch(probs, plugin="indirect", precision="fp64")
+    for t, o in zip(tags, outs):
+        _check_fp64(o, G.expected(t, "indirect"), f"batch/{t}")
+    outs = pdos_amd.solve_batch(probs, plugin="direct", precision="fp64")
+    for t, o in zip(tags, outs):
+        _check_fp64(o, G.expected(t, "direct"), f"batch-direct/{t}")
+
+
+def test_batch_uneven_vs_oracle():
+    probs = problems.uneven_portfolios(seed=31, count=24, na_range=(20, 300), k_range=(3, 30),
+                                       EPS_ABS=1e-3, MAX_ITERS=4000)
+    print("uneven batch", flush=True)
+    outs = pdos_amd.solve_batch(probs, plugin="indirect", precision="fp64")
+    for p, o in zip(probs, outs):
+        ref = oracle.solve(p, "indirect")
+        assert o["status"] == ref["status"]
+        assert o["info"]["iters"] == ref["iters"]
+        for k in ("x", "s", "y"):
+            assert _rel(o[k], ref[k]) <= RTOL
+
+
+def test_maxiters_zero_returns_warm_start():
+    base = G.problem("portfolio_n101_norm1")
+    rng = np.random.default_rng(5)
+    prob = base.with_warm(x0=rng.standard_normal(base.n), y0=rng.standard_normal(base.m),
+                          s0=rng.standard_normal(base.m)).with_params(MAX_ITERS=0)
+    out = pdos_amd.solve_problem(prob, engine="wg")
+    assert out["status"] == "Indeterminate"
+    np.testing.assert_allclose(out["x"], prob.x0, rtol=1e-15)
+    np.testing.assert_allclose(out["y"], prob.y0, rtol=1e-15)
+    np.testing.assert_allclose(out["s"], prob.s0, rtol=1e-15)
