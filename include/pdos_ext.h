@@ -20,6 +20,8 @@
 #ifndef PDOS_AMD_PDOS_EXT_H
 #define PDOS_AMD_PDOS_EXT_H
 
+#include <stddef.h>
+
 #include "pdos.h"
 
 #ifdef __cplusplus
@@ -45,6 +47,7 @@ typedef struct PdosInfo {
   long cg_iters;     /* total CG iterations (indirect) */
   long status;       /* 0 Solved, 1 Indeterminate */
   long engine;       /* engine that ran */
+  long fallback;     /* 1 if the warm start was rejected for the cold start */
   double p_res, d_res, p_obj, d_obj, eta;  /* last stop check */
   double lambda;
   double setup_ms;   /* host setup + factorization + upload */
@@ -82,6 +85,51 @@ PdosBatchHandle *pdos_batch_prepare(const Data *const *ds, const Cone *const *ks
 int pdos_batch_run(PdosBatchHandle *h, double *kernel_ms);
 int pdos_batch_fetch(PdosBatchHandle *h, Sol **sols, PdosInfo *infos);
 void pdos_batch_free(PdosBatchHandle *h);
+
+/* Warm start for every instance of a prepared batch from device memory:
+ * instance i reads [x (n), y (m), s (m)] in original units (float) at
+ * dW + i * ld.  With check != 0 the solver compares, at its first stop test,
+ * the warm start's scaled residuals max(p_res/(1+|b|), d_res/(1+|c|), gap)
+ * with the cold start's and falls back to the cold start if they are worse
+ * (PdosInfo.fallback).  dW == NULL restores the prepared start. */
+int pdos_batch_set_warm_dev(PdosBatchHandle *h, const float *dW, long ld, int check);
+
+/* Device memory for callers that keep inputs resident (plain HIP memory on
+ * the library's device; no torch). */
+void *pdos_device_alloc(size_t bytes);
+void pdos_device_free(void *p);
+int pdos_device_upload(void *dst, const void *src, size_t bytes);
+int pdos_device_download(void *dst, const void *src, size_t bytes);
+
+/* Learned warm start: MLP n_in -> n_hidden -> n_hidden -> n_out with GELU,
+ * bf16 weights mirrored from a binary32 master, binary32 accumulation, all
+ * products on BF16 MFMA (hand-written kernels), Adam training.
+ * Parameters (get/set, gradients) use the layout W1 [n_in x n_hidden],
+ * b1, W2 [n_hidden x n_hidden], b2, W3 [n_hidden x n_out], b3, row-major.
+ * Inputs pass through (x - in_mean) / in_std; outputs of predict are
+ * y * out_std + out_mean; training targets are in the standardised space. */
+typedef struct PdosMlp PdosMlp;
+PdosMlp *pdos_mlp_create(int n_in, int n_hidden, int n_out, unsigned long seed, int device);
+void pdos_mlp_free(PdosMlp *P);
+int pdos_mlp_num_params(const PdosMlp *P);
+int pdos_mlp_get_params(PdosMlp *P, float *out);
+int pdos_mlp_set_params(PdosMlp *P, const float *in);
+int pdos_mlp_set_affine(PdosMlp *P, const float *in_mean, const float *in_std, const float *out_std,
+                        const float *out_mean);
+int pdos_mlp_loss_grad(PdosMlp *P, const float *X, const float *T, int rows, double *loss,
+                       float *grads);
+int pdos_mlp_train(PdosMlp *P, const float *X, const float *T, int rows, int epochs, int batch,
+                   float lr, unsigned long seed, double *last_loss);
+int pdos_mlp_predict(PdosMlp *P, const float *X, int rows, float *Y, double *kernel_ms);
+int pdos_mlp_predict_dev(PdosMlp *P, const float *dX, int rows, float *dY, int ldy);
+const char *pdos_mlp_last_error(void);
+
+/* Host-side check of the direct plugin's factorization: builds the scaled
+ * KKT [[-I, A'], [A, I]] of (d, k) exactly as pdos() does, orders and factors
+ * it, and solves K z = rhs (length n+m) on the host with the same level-set
+ * schedule the GPU sweeps use.  stats (optional, 4 longs): N, nnz(L),
+ * forward levels, backward levels.  Returns 0 or -1. */
+int pdos_kkt_solve_host(const Data *d, const Cone *k, const double *rhs, double *z, long *stats);
 
 /* Text of the last error on this thread ("" if none). */
 const char *pdos_last_error(void);

@@ -35,7 +35,7 @@ class PdosOptions(C.Structure):
 
 class PdosInfo(C.Structure):
     _fields_ = [("iters", C.c_long), ("cg_iters", C.c_long), ("status", C.c_long),
-                ("engine", C.c_long), ("p_res", C.c_double), ("d_res", C.c_double),
+                ("engine", C.c_long), ("fallback", C.c_long), ("p_res", C.c_double), ("d_res", C.c_double),
                 ("p_obj", C.c_double), ("d_obj", C.c_double), ("eta", C.c_double),
                 ("lam", C.c_double), ("setup_ms", C.c_double), ("solve_ms", C.c_double),
                 ("total_ms", C.c_double), ("lnz", C.c_double)]
@@ -47,7 +47,12 @@ class PdosInfo(C.Structure):
 # Every symbol include/pdos.h and include/pdos_ext.h declare.
 EXPORTS = ["pdos", "freeData", "freeSol", "tic", "toc", "tocq", "pdos_default_options",
            "pdos_ex", "pdos_last_info", "pdos_batch", "pdos_batch_prepare", "pdos_batch_run",
-           "pdos_batch_fetch", "pdos_batch_free", "pdos_last_error", "pdos_version"]
+           "pdos_batch_fetch", "pdos_batch_free", "pdos_batch_set_warm_dev", "pdos_device_alloc",
+           "pdos_device_free", "pdos_device_upload", "pdos_device_download", "pdos_mlp_create",
+           "pdos_mlp_free", "pdos_mlp_num_params", "pdos_mlp_get_params", "pdos_mlp_set_params",
+           "pdos_mlp_set_affine", "pdos_mlp_loss_grad", "pdos_mlp_train", "pdos_mlp_predict",
+           "pdos_mlp_predict_dev", "pdos_mlp_last_error", "pdos_kkt_solve_host", "pdos_last_error",
+           "pdos_version"]
 
 _libs = {}
 
@@ -78,6 +83,30 @@ def lib(plugin: str = "indirect"):
         L.pdos_batch_fetch.restype = C.c_int
         L.pdos_batch_fetch.argtypes = [C.c_void_p, C.POINTER(C.POINTER(Sol)), C.POINTER(PdosInfo)]
         L.pdos_batch_free.argtypes = [C.c_void_p]
+        L.pdos_kkt_solve_host.restype = C.c_int
+        L.pdos_kkt_solve_host.argtypes = [C.POINTER(Data), C.POINTER(Cone), C.POINTER(C.c_double),
+                                          C.POINTER(C.c_double), C.POINTER(C.c_long)]
+        L.pdos_batch_set_warm_dev.restype = C.c_int
+        L.pdos_batch_set_warm_dev.argtypes = [C.c_void_p, C.c_void_p, C.c_long, C.c_int]
+        L.pdos_device_alloc.restype = C.c_void_p
+        L.pdos_device_alloc.argtypes = [C.c_size_t]
+        L.pdos_device_free.argtypes = [C.c_void_p]
+        L.pdos_device_upload.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+        L.pdos_device_download.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+        fp = C.POINTER(C.c_float)
+        L.pdos_mlp_create.restype = C.c_void_p
+        L.pdos_mlp_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_ulong, C.c_int]
+        L.pdos_mlp_free.argtypes = [C.c_void_p]
+        L.pdos_mlp_num_params.argtypes = [C.c_void_p]
+        L.pdos_mlp_get_params.argtypes = [C.c_void_p, fp]
+        L.pdos_mlp_set_params.argtypes = [C.c_void_p, fp]
+        L.pdos_mlp_set_affine.argtypes = [C.c_void_p, fp, fp, fp, fp]
+        L.pdos_mlp_loss_grad.argtypes = [C.c_void_p, fp, fp, C.c_int, C.POINTER(C.c_double), fp]
+        L.pdos_mlp_train.argtypes = [C.c_void_p, fp, fp, C.c_int, C.c_int, C.c_int, C.c_float,
+                                     C.c_ulong, C.POINTER(C.c_double)]
+        L.pdos_mlp_predict.argtypes = [C.c_void_p, fp, C.c_int, fp, C.POINTER(C.c_double)]
+        L.pdos_mlp_predict_dev.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int]
+        L.pdos_mlp_last_error.restype = C.c_char_p
         L.pdos_batch.restype = C.c_int
         L.pdos_batch.argtypes = [C.POINTER(C.POINTER(Data)), C.POINTER(C.POINTER(Cone)), C.c_int,
                                  C.POINTER(PdosOptions), C.POINTER(C.POINTER(Sol)),
@@ -136,6 +165,22 @@ def solve_batch(problems, plugin="indirect", precision="fp64") -> list:
     return out
 
 
+def kkt_solve_host(p: Problem, rhs):
+    """Solves the scaled KKT system with the library's host factorization;
+    returns (z, stats = [N, nnz(L), forward levels, backward levels])."""
+    L = lib("direct")
+    d, k, keep = p.to_c()
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    z = np.zeros(p.n + p.m)
+    st = np.zeros(4, dtype=np.int64)
+    rc = L.pdos_kkt_solve_host(C.byref(d), C.byref(k), rhs.ctypes.data_as(C.POINTER(C.c_double)),
+                               z.ctypes.data_as(C.POINTER(C.c_double)),
+                               st.ctypes.data_as(C.POINTER(C.c_long)))
+    if rc != 0:
+        raise RuntimeError(L.pdos_last_error().decode())
+    return z, st
+
+
 class BatchHandle:
     """A prepared batch resident on the device (pdos_batch_prepare/run/fetch)."""
 
@@ -169,6 +214,11 @@ class BatchHandle:
             out.append(r)
         return out
 
+    def set_warm_dev(self, dev_ptr, ld: int, check: bool = True):
+        """Warm start from device memory (see pdos_batch_set_warm_dev)."""
+        if self.L.pdos_batch_set_warm_dev(self.h, dev_ptr, ld, int(check)) != 0:
+            raise RuntimeError(self.L.pdos_last_error().decode())
+
     def close(self):
         if self.h:
             self.L.pdos_batch_free(self.h)
@@ -177,6 +227,39 @@ class BatchHandle:
     def __del__(self):
         try:
             self.close()
+        except Exception:
+            pass
+
+
+class DeviceArray:
+    """A float32 buffer in device memory (plain hipMalloc via the library)."""
+
+    def __init__(self, arr=None, shape=None):
+        self.L = lib("indirect")
+        if arr is not None:
+            arr = np.ascontiguousarray(arr, dtype=np.float32)
+            shape = arr.shape
+        self.shape = tuple(shape)
+        self.nbytes = int(np.prod(self.shape)) * 4
+        self.ptr = self.L.pdos_device_alloc(self.nbytes)
+        if not self.ptr:
+            raise RuntimeError("device allocation failed")
+        if arr is not None:
+            self.L.pdos_device_upload(self.ptr, arr.ctypes.data, self.nbytes)
+
+    def numpy(self):
+        out = np.empty(self.shape, dtype=np.float32)
+        self.L.pdos_device_download(out.ctypes.data, self.ptr, self.nbytes)
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.L.pdos_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
         except Exception:
             pass
 

@@ -124,6 +124,60 @@ WgBatch::~WgBatch() {
   if (ev1) (void)hipEventDestroy(ev1);
 }
 
+static uint64_t fnv(uint64_t h, const void* p, size_t bytes) {
+  const unsigned char* c = static_cast<const unsigned char*>(p);
+  for (size_t i = 0; i < bytes; ++i) h = (h ^ c[i]) * 1099511628211ull;
+  return h;
+}
+
+template <typename V>
+static uint64_t fnv_vec(uint64_t h, const std::vector<V>& v) {
+  return fnv(h, v.data(), v.size() * sizeof(V));
+}
+
+static bool same_matrix(const Setup& a, const Setup& b) {
+  return a.n == b.n && a.m == b.m && a.f == b.f && a.l == b.l && a.soc_ptr == b.soc_ptr &&
+         a.cp == b.cp && a.ci == b.ci && a.cx == b.cx && a.D == b.D && a.E == b.E;
+}
+
+bool WgBatch::group_and_factor(std::string& err) {
+  const int count = (int)S.size();
+  grp.assign(count, -1);
+  rep.clear();
+  std::vector<std::pair<uint64_t, int>> keys;  // (hash, group)
+  for (int i = 0; i < count; ++i) {
+    const Setup& s = S[i];
+    uint64_t h = 1469598103934665603ull;
+    const int dims[4] = {s.n, s.m, s.f, s.l};
+    h = fnv(h, dims, sizeof(dims));
+    h = fnv_vec(h, s.soc_ptr);
+    h = fnv_vec(h, s.cp);
+    h = fnv_vec(h, s.ci);
+    h = fnv_vec(h, s.cx);
+    h = fnv_vec(h, s.D);
+    h = fnv_vec(h, s.E);
+    for (const auto& kv : keys)
+      if (kv.first == h && same_matrix(S[rep[kv.second]], s)) { grp[i] = kv.second; break; }
+    if (grp[i] < 0) {
+      grp[i] = (int)rep.size();
+      keys.emplace_back(h, grp[i]);
+      rep.push_back(i);
+    }
+  }
+  for (int i = 0; i < count; ++i) {
+    if (rep[grp[i]] == i) continue;
+    Setup& s = S[i];  // drop the duplicate matrix copies (the device shares the group's)
+    std::vector<int>().swap(s.cp); std::vector<int>().swap(s.ci); std::vector<double>().swap(s.cx);
+    std::vector<int>().swap(s.rp); std::vector<int>().swap(s.ri); std::vector<double>().swap(s.rx);
+  }
+  if (direct) {
+    F.assign(rep.size(), Factor());
+    for (size_t g = 0; g < rep.size(); ++g)
+      if (!build_factor(S[rep[g]], {}, F[g], err)) return false;
+  }
+  return true;
+}
+
 bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::string& err) {
   device = device_; fp64 = fp64_; direct = direct_; own_mem = !borrow;
   Ctx& C = ctx();
@@ -137,29 +191,39 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
                size_t perm, lrp, lri, lrx, lcp, lci, lcx, ld, flp, flr, blp, blr; };
   std::vector<Off> pk(count);
   xo.resize(count); so.resize(count); yo.resize(count);
+  if (grp.size() != (size_t)count) {  // no grouping requested: every instance is its own group
+    grp.resize(count);
+    rep.resize(count);
+    for (int i = 0; i < count; ++i) grp[i] = rep[i] = i;
+  }
   lds = 0;
   for (int i = 0; i < count; ++i) {
     const Setup& s = S[i];
     Off& k = pk[i];
-    k.soc = put_vec(P, s.soc_ptr);
-    k.rp = put_vec(P, s.rp);
-    k.ri = put_vec(P, s.ri);
-    k.rx = put_vals(P, s.rx, fp64);
-    k.cp = put_vec(P, s.cp);
-    k.ci = put_vec(P, s.ci);
-    k.cx = put_vals(P, s.cx, fp64);
+    if (rep[grp[i]] != i) {  // shares the representative's matrix, scalings and factor
+      const Off& r = pk[rep[grp[i]]];
+      k = r;
+    } else {
+      k.soc = put_vec(P, s.soc_ptr);
+      k.rp = put_vec(P, s.rp);
+      k.ri = put_vec(P, s.ri);
+      k.rx = put_vals(P, s.rx, fp64);
+      k.cp = put_vec(P, s.cp);
+      k.ci = put_vec(P, s.ci);
+      k.cx = put_vals(P, s.cx, fp64);
+      k.D = put_vec(P, s.D);
+      k.E = put_vec(P, s.E);
+    }
     k.b = put_vec(P, s.b);
     k.c = put_vec(P, s.c);
-    k.D = put_vec(P, s.D);
-    k.E = put_vec(P, s.E);
     k.x0 = put_vec(P, s.x);
     k.s0 = put_vec(P, s.s);
     k.y0 = put_vec(P, s.y);
     xo[i] = P.zeros(sizeof(double) * s.n);
     so[i] = P.zeros(sizeof(double) * s.m);
     yo[i] = P.zeros(sizeof(double) * s.m);
-    if (direct) {
-      const Factor& f = F[i];
+    if (direct && rep[grp[i]] == i) {
+      const Factor& f = F[grp[i]];
       k.perm = put_vec(P, f.P);
       k.lrp = put_vec(P, f.Rp);
       k.lri = put_vec(P, f.Ri);
@@ -191,15 +255,16 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
     if (e != hipSuccess) { dbuf = nullptr; err = std::string("hipMalloc: ") + hipGetErrorString(e); return false; }
   }
   unsigned char* base = static_cast<unsigned char*>(dbuf);
-  std::vector<WgInst> insts(count);
+  insts.assign(count, WgInst());
   for (int i = 0; i < count; ++i) {
     const Setup& s = S[i];
     const Off& k = pk[i];
+    const Setup& sr = S[rep[grp[i]]];  // holds the matrix arrays
     WgInst& w = insts[i];
     std::memset(&w, 0, sizeof(w));
-    w.n = s.n; w.m = s.m; w.f = s.f; w.l = s.l; w.nsoc = s.nsoc(); w.N = s.n + s.m;
-    w.ga = lanes_for(s.nnz(), s.m);
-    w.gat = lanes_for(s.nnz(), s.n);
+    w.n = s.n; w.m = s.m; w.f = s.f; w.l = s.l; w.nsoc = sr.nsoc(); w.N = s.n + s.m;
+    w.ga = lanes_for(sr.nnz(), s.m);
+    w.gat = lanes_for(sr.nnz(), s.n);
     w.max_iters = (int)std::min<long>(std::max<long>(par[i].MAX_ITERS, 0), 0x7fffffff);
     w.cg_max_its = (int)std::min<long>(std::max<long>(par[i].CG_MAX_ITS, 0), 0x7fffffff);
     w.eps = par[i].EPS_ABS;
@@ -226,7 +291,7 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
     w.s = reinterpret_cast<double*>(base + so[i]);
     w.y = reinterpret_cast<double*>(base + yo[i]);
     if (direct) {
-      const Factor& f = F[i];
+      const Factor& f = F[grp[i]];
       w.perm = reinterpret_cast<const int*>(base + k.perm);
       w.lrp = reinterpret_cast<const int*>(base + k.lrp);
       w.lri = reinterpret_cast<const int*>(base + k.lri);
@@ -258,6 +323,19 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
   if (e == hipSuccess && !ev1) e = hipEventCreate(&ev1);
   if (e != hipSuccess) { err = std::string("upload: ") + hipGetErrorString(e); return false; }
   upload_ms = now_ms() - t0;
+  return true;
+}
+
+bool WgBatch::set_warm(const float* dW, long ld, int check, std::string& err) {
+  for (size_t i = 0; i < insts.size(); ++i) {
+    insts[i].wsrc = dW ? dW + (size_t)i * ld : nullptr;
+    insts[i].warm_check = dW ? check : 0;
+  }
+  unsigned char* base = static_cast<unsigned char*>(dbuf);
+  hipError_t e = hipMemcpyAsync(base + inst_off, insts.data(), sizeof(WgInst) * insts.size(),
+                                hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) { err = std::string("set_warm: ") + hipGetErrorString(e); return false; }
   return true;
 }
 
@@ -334,6 +412,7 @@ void fill_info(PdosInfo* info, const DevResult& r, const Setup& s, int engine, d
   info->cg_iters = r.cg_iters;
   info->status = r.status;
   info->engine = engine;
+  info->fallback = r.fallback;
   info->p_res = r.p_res;
   info->d_res = r.d_res;
   info->p_obj = r.p_obj;
@@ -486,17 +565,21 @@ static PdosBatchHandle* batch_prepare(const Data* const* ds, const Cone* const* 
   const double t0 = now_ms();
   PdosBatchHandle* h = new PdosBatchHandle;
   h->B.S.resize(count);
-  if (direct) h->B.F.resize(count);
   h->B.par.resize(count);
+  h->B.direct = direct;
   std::string err;
   for (int i = 0; i < count; ++i) {
-    if (!ds[i] || !ks[i] || !build_setup(ds[i], ks[i], h->B.S[i], err) ||
-        (direct && !build_factor(h->B.S[i], {}, h->B.F[i], err))) {
+    if (!ds[i] || !ks[i] || !build_setup(ds[i], ks[i], h->B.S[i], err)) {
       g_err = "instance " + std::to_string(i) + ": " + (err.empty() ? "NULL input" : err);
       delete h;
       return nullptr;
     }
     h->B.par[i] = *ds[i]->p;
+  }
+  if (!h->B.group_and_factor(err)) {
+    g_err = err;
+    delete h;
+    return nullptr;
   }
   if (!h->B.prepare(o.device, o.precision == PDOS_FP64, direct, borrow, err)) {
     g_err = err;
@@ -518,7 +601,7 @@ static int batch_fetch(PdosBatchHandle* h, Sol** sols, PdosInfo* infos) {
     sols[i] = make_sol(h->B.S[i], res[i].status);
     if (infos)
       fill_info(&infos[i], res[i], h->B.S[i], PDOS_ENGINE_WG, h->setup_ms, h->B.wall_ms,
-                h->B.direct ? (double)h->B.F[i].Lp.back() : 0.0);
+                h->B.direct ? (double)h->B.F[h->B.grp[i]].Lp.back() : 0.0);
   }
   return 0;
 }
@@ -548,6 +631,37 @@ int pdos_batch_fetch(PdosBatchHandle* h, Sol** sols, PdosInfo* infos) {
 
 void pdos_batch_free(PdosBatchHandle* h) { delete h; }
 
+int pdos_batch_set_warm_dev(PdosBatchHandle* h, const float* dW, long ld, int check) {
+  if (!h) return -1;
+  std::string err;
+  if (!h->B.set_warm(dW, ld, check, err)) {
+    g_err = err;
+    return -1;
+  }
+  return 0;
+}
+
+void* pdos_device_alloc(size_t bytes) {
+  void* p = nullptr;
+  if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) {
+    g_err = "hipMalloc failed";
+    return nullptr;
+  }
+  return p;
+}
+
+void pdos_device_free(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+int pdos_device_upload(void* dst, const void* src, size_t bytes) {
+  return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? 0 : -1;
+}
+
+int pdos_device_download(void* dst, const void* src, size_t bytes) {
+  return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? 0 : -1;
+}
+
 int pdos_batch(const Data* const* ds, const Cone* const* ks, int count, const PdosOptions* opt,
                Sol** sols, PdosInfo* infos) {
   g_err.clear();
@@ -563,6 +677,42 @@ int pdos_batch(const Data* const* ds, const Cone* const* ks, int count, const Pd
   int rc = h->B.run(err) ? batch_fetch(h, sols, infos) : (g_err = err, -1);
   delete h;
   return rc;
+}
+
+int pdos_kkt_solve_host(const Data* d, const Cone* k, const double* rhs, double* z, long* stats) {
+  g_err.clear();
+  Setup S;
+  Factor F;
+  std::string err;
+  if (!build_setup(d, k, S, err) || !build_factor(S, {}, F, err)) {
+    g_err = err;
+    return -1;
+  }
+  const int N = F.N;
+  std::vector<double> bp(N);
+  for (int j = 0; j < N; ++j) bp[j] = rhs[F.P[j]];
+  for (int lv = 0; lv < F.nfl(); ++lv)
+    for (int r = F.fl_ptr[lv]; r < F.fl_ptr[lv + 1]; ++r) {
+      const int i = F.fl_rows[r];
+      double v = bp[i];
+      for (int p = F.Rp[i]; p < F.Rp[i + 1]; ++p) v -= F.Rx[p] * bp[F.Ri[p]];
+      bp[i] = v;
+    }
+  for (int lv = 0; lv < F.nbl(); ++lv)
+    for (int r = F.bl_ptr[lv]; r < F.bl_ptr[lv + 1]; ++r) {
+      const int j = F.bl_rows[r];
+      double v = bp[j] / F.Ld[j];
+      for (int p = F.Lp[j]; p < F.Lp[j + 1]; ++p) v -= F.Lx[p] * bp[F.Li[p]];
+      bp[j] = v;
+    }
+  for (int j = 0; j < N; ++j) z[F.P[j]] = bp[j];
+  if (stats) {
+    stats[0] = N;
+    stats[1] = F.Lp[N];
+    stats[2] = F.nfl();
+    stats[3] = F.nbl();
+  }
+  return 0;
 }
 
 void freeData(Data** d, Cone** k) {

@@ -10,7 +10,7 @@ struct DevResult {
   int iters;      // ADMM updates performed
   int status;     // 0 Solved, 1 Indeterminate
   int cg_iters;   // total CG iterations
-  int pad;
+  int fallback;   // 1 if a warm start was rejected for the cold start
   double p_res, d_res, p_obj, d_obj, eta;
 };
 
@@ -36,6 +36,9 @@ struct WgInst {
   const double* x0;        // scaled warm start (read once)
   const double* s0;
   const double* y0;
+  const float* wsrc;        // optional warm start in original units: [x (n), y (m), s (m)]
+  int warm_check;          // 1: compare the warm start's residuals with the cold start's
+  int pad3;
   double* x;               // final scaled iterate (written once)
   double* s;
   double* y;

@@ -45,7 +45,9 @@ hipError_t wg_launch(const WgInst* d_insts, int count, int lds_bytes, bool fp64,
 // re-runnable from the same start (the kernel reads x0/s0/y0, writes x/s/y).
 struct WgBatch {
   std::vector<Setup> S;
-  std::vector<Factor> F;
+  std::vector<int> grp;       // instance -> matrix group (instances with identical scaled A)
+  std::vector<int> rep;       // group -> representative instance (keeps the matrix arrays)
+  std::vector<Factor> F;      // per group (direct plugin)
   std::vector<Params> par;
   int device = 0;
   bool fp64 = true, direct = false, own_mem = true;
@@ -54,12 +56,17 @@ struct WgBatch {
   void* dbuf = nullptr;
   size_t bytes = 0, res_off = 0, inst_off = 0;
   std::vector<size_t> xo, so, yo;
+  std::vector<WgInst> insts;  // host copy of the device records
   int lds = 0;
   double upload_ms = 0, kernel_ms = 0, wall_ms = 0;
   ~WgBatch();
+  // Groups instances by their scaled matrix (and scalings, cones) and drops
+  // the duplicate host copies; with `direct`, factors once per group.
+  bool group_and_factor(std::string& err);
   bool prepare(int device, bool fp64, bool direct, bool borrow_ctx_arena, std::string& err);
   bool run(std::string& err);
   bool fetch(std::vector<DevResult>& res, std::string& err);
+  bool set_warm(const float* dW, long ld, int check, std::string& err);
 };
 
 bool solve_wg(std::vector<Setup>& S, std::vector<Factor>& F, const std::vector<const Params*>& par,

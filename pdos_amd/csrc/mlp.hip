@@ -1,0 +1,553 @@
+// mlp.hip -- the learned warm start: a 3-layer MLP (in -> H -> H -> out, GELU)
+// trained and evaluated with hand-written CDNA4 kernels.
+//
+// Weights are kept in binary32 (master copy, Adam state) and mirrored in
+// bfloat16 for the products; every product is one BF16 MFMA GEMM
+// (v_mfma_f32_16x16x32_bf16, binary32 accumulation) with a fused epilogue:
+//
+//   forward   Z1 = X W1 + b1, H1 = gelu(Z1)      [bias+GELU epilogue]
+//             Z2 = H1 W2 + b2, H2 = gelu(Z2)
+//             Y  = H2 W3 + b3                    [bias epilogue]
+//   backward  dY = 2 (Y - T) / count (masked to valid rows / columns)
+//             dW3 = H2' dY, db3 = colsum dY      [A-transposed GEMM]
+//             dZ2 = (dY W3') .* gelu'(Z2)         [B-transposed GEMM, GELU' epilogue]
+//             dW2 = H1' dZ2, dZ1 = (dZ2 W2') .* gelu'(Z1), dW1 = X' dZ1
+//   update    Adam on the binary32 master, then refresh the bf16 mirror
+//
+// All dimensions are padded to multiples of 64 (zero rows/columns, masked in
+// the loss), so the GEMM tiles never need bounds checks.  Inputs and outputs
+// pass through affine maps (standardisation) folded into the first layer's
+// input kernel and the last layer's epilogue.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/pdos_ext.h"
+
+namespace pdos_amd {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int BM = 64, BN = 64, BK = 32, KP = BK + 8;  // LDS rows padded against bank conflicts
+constexpr int GT = 256;
+
+enum Epi { EPI_F32 = 0, EPI_BIAS_GELU = 1, EPI_BIAS_AFFINE = 2, EPI_GELU_GRAD = 3 };
+
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad(float z) {
+  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+}
+
+// C[M x N] = op(A) op(B) with bf16 operands and fp32 accumulation.
+//   TA = false: A is M x K row-major (lda);  TA = true: A is stored K x M (lda)
+//   TB = false: B is K x N row-major (ldb);  TB = true: B is stored N x K (ldb)
+// M % 64 == 0, N % 64 == 0, K % 32 == 0.  Block: 64 x 64 tile, 4 waves in a
+// 2 x 2 layout, each wave 32 x 32 = 2 x 2 MFMA tiles of 16 x 16.
+template <bool TA, bool TB, int EPI>
+__global__ __launch_bounds__(GT) void k_gemm(int M, int N, int K, const __bf16* __restrict__ A, int lda,
+                                             const __bf16* __restrict__ B, int ldb, float* __restrict__ Cf,
+                                             __bf16* __restrict__ Cb, int ldc, const float* __restrict__ bias,
+                                             const float* __restrict__ aux, const float* __restrict__ omul,
+                                             const float* __restrict__ oadd) {
+  __shared__ __attribute__((aligned(16))) __bf16 As[BM][KP];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[BN][KP];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int wm = (wid >> 1) * 32, wn = (wid & 1) * 32;
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    // stage A tile (64 x 32) as As[row][k]
+    if (!TA) {
+      const int r = tid >> 2, kc = (tid & 3) * 8;
+      *reinterpret_cast<bf16x8*>(&As[r][kc]) =
+          *reinterpret_cast<const bf16x8*>(&A[(size_t)(m0 + r) * lda + k0 + kc]);
+    } else {
+      const int kk = tid >> 3, rc = (tid & 7) * 8;
+      const bf16x8 v = *reinterpret_cast<const bf16x8*>(&A[(size_t)(k0 + kk) * lda + m0 + rc]);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) As[rc + i][kk] = v[i];
+    }
+    // stage B tile (32 x 64) as Bs[col][k]
+    if (!TB) {
+      const int kk = tid >> 3, cc = (tid & 7) * 8;
+      const bf16x8 v = *reinterpret_cast<const bf16x8*>(&B[(size_t)(k0 + kk) * ldb + n0 + cc]);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) Bs[cc + i][kk] = v[i];
+    } else {
+      const int c = tid >> 2, kc = (tid & 3) * 8;
+      *reinterpret_cast<bf16x8*>(&Bs[c][kc]) =
+          *reinterpret_cast<const bf16x8*>(&B[(size_t)(n0 + c) * ldb + k0 + kc]);
+    }
+    __syncthreads();
+    // lane l: A rows (l % 16), k = (l / 16) * 8 .. +7; B cols (l % 16), same k
+    const int fr = lane & 15, fk = (lane >> 4) * 8;
+    bf16x8 af[2], bfv[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) af[i] = *reinterpret_cast<const bf16x8*>(&As[wm + i * 16 + fr][fk]);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) bfv[j] = *reinterpret_cast<const bf16x8*>(&Bs[wn + j * 16 + fr][fk]);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfv[j], acc[i][j], 0, 0, 0);
+    __syncthreads();
+  }
+  // accumulator lane l holds rows (l / 16) * 4 + r, column l % 16
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
+        const int col = n0 + wn + j * 16 + (lane & 15);
+        const size_t o = (size_t)row * ldc + col;
+        float v = acc[i][j][r];
+        if (EPI == EPI_F32) {
+          Cf[o] = v;
+        } else if (EPI == EPI_BIAS_GELU) {
+          v += bias[col];
+          Cf[o] = v;                 // pre-activation, kept for the backward pass
+          Cb[o] = (__bf16)gelu(v);   // activation, the next layer's operand
+        } else if (EPI == EPI_BIAS_AFFINE) {
+          v += bias[col];
+          Cf[o] = omul ? v * omul[col] + oadd[col] : v;
+        } else {  // EPI_GELU_GRAD: v is dH; aux holds Z
+          Cb[o] = (__bf16)(v * gelu_grad(aux[o]));
+        }
+      }
+}
+
+// Standardised, zero-padded bf16 input rows: X'[r][c] = (X[r][c] - mu[c]) / sd[c].
+__global__ void k_prep_input(int rows, int cols, int ld_in, const float* __restrict__ X,
+                             const float* __restrict__ mu, const float* __restrict__ sd, int ld_out,
+                             int rows_pad, __bf16* __restrict__ Xb) {
+  const long total = (long)rows_pad * ld_out;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    const int r = (int)(e / ld_out), c = (int)(e % ld_out);
+    float v = 0.f;
+    if (r < rows && c < cols) v = (X[(size_t)r * ld_in + c] - mu[c]) / sd[c];
+    Xb[e] = (__bf16)v;
+  }
+}
+
+// dY = 2 (Y - T) / (rows * cols) on valid entries, 0 on padding; per-block
+// partial sums of (Y - T)^2.  T is standardised (cols valid of ld).
+__global__ void k_loss_grad(int rows, int cols, int ld, const float* __restrict__ Y,
+                            const float* __restrict__ T, int ldt, int rows_pad, float* __restrict__ dYf,
+                            __bf16* __restrict__ dYb, double* __restrict__ partial) {
+  __shared__ double red[GT / 64];
+  const long total = (long)rows_pad * ld;
+  const float scale = 2.0f / (float)((double)rows * cols);
+  double acc = 0.0;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    const int r = (int)(e / ld), c = (int)(e % ld);
+    float g = 0.f;
+    if (r < rows && c < cols) {
+      const float d = Y[e] - T[(size_t)r * ldt + c];
+      acc += (double)d * d;
+      g = scale * d;
+    }
+    dYf[e] = g;
+    dYb[e] = (__bf16)g;
+  }
+  for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < GT / 64; ++w) s += red[w];
+    partial[blockIdx.x] = s;
+  }
+}
+
+// db[c] = sum_r G[r][c] over rows (fp32 or bf16 source).
+template <typename S>
+__global__ void k_colsum(int rows, int cols, int ld, const S* __restrict__ G, float* __restrict__ db) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cols) return;
+  float s = 0.f;
+  for (int r = 0; r < rows; ++r) s += (float)G[(size_t)r * ld + c];
+  db[c] = s;
+}
+
+// Adam (binary32 master, bias-corrected), then the bf16 mirror.
+__global__ void k_adam(long n, float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, __bf16* __restrict__ wb, float lr, float b1, float b2,
+                       float c1, float c2, float eps) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    w[i] = wi;
+    if (wb) wb[i] = (__bf16)wi;
+  }
+}
+
+// dst[i][:] = src[idx[i]][:] for i < rows (minibatch gather).
+__global__ void k_gather_rows(int rows, int cols, const int* __restrict__ idx, const float* __restrict__ src,
+                              float* __restrict__ dst) {
+  const long total = (long)rows * cols;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    const int r = (int)(e / cols), c = (int)(e % cols);
+    dst[e] = src[(size_t)idx[r] * cols + c];
+  }
+}
+
+__global__ void k_to_bf16(long n, const float* __restrict__ a, __bf16* __restrict__ b) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    b[i] = (__bf16)a[i];
+}
+
+int pad64(int v) { return (v + 63) & ~63; }
+
+template <bool TA, bool TB, int EPI>
+void gemm(hipStream_t st, int M, int N, int K, const __bf16* A, int lda, const __bf16* B, int ldb, float* Cf,
+          __bf16* Cb, int ldc, const float* bias = nullptr, const float* aux = nullptr,
+          const float* omul = nullptr, const float* oadd = nullptr) {
+  hipLaunchKernelGGL((k_gemm<TA, TB, EPI>), dim3(N / BN, M / BM), dim3(GT), 0, st, M, N, K, A, lda, B, ldb,
+                     Cf, Cb, ldc, bias, aux, omul, oadd);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+
+struct Mlp {
+  int nin = 0, nh = 0, nout = 0;        // logical sizes
+  int pin = 0, ph = 0, pout = 0;        // padded sizes
+  int cap = 0;                          // padded rows the work buffers hold
+  hipStream_t st = nullptr;
+  // parameters: fp32 master, Adam state, bf16 mirror (weights only)
+  float *W1 = nullptr, *b1 = nullptr, *W2 = nullptr, *b2 = nullptr, *W3 = nullptr, *b3 = nullptr;
+  float *g = nullptr, *mom = nullptr, *vel = nullptr;  // one block each, same layout as the master
+  float* master = nullptr;
+  __bf16 *W1b = nullptr, *W2b = nullptr, *W3b = nullptr;
+  float *in_mu = nullptr, *in_sd = nullptr, *out_mul = nullptr, *out_add = nullptr;
+  // activations
+  __bf16 *Xb = nullptr, *H1b = nullptr, *H2b = nullptr, *dYb = nullptr, *dZ2b = nullptr, *dZ1b = nullptr;
+  float *Z1 = nullptr, *Z2 = nullptr, *Y = nullptr, *dYf = nullptr, *T = nullptr, *Xf = nullptr;
+  double* part = nullptr;
+  long step = 0;
+  long nparam = 0;
+  size_t off_W1 = 0, off_b1 = 0, off_W2 = 0, off_b2 = 0, off_W3 = 0, off_b3 = 0;
+  std::vector<void*> allocs;
+
+  template <typename P>
+  bool alloc(P*& p, size_t count, std::string& err) {
+    void* q = nullptr;
+    if (hipMalloc(&q, count * sizeof(P) + 256) != hipSuccess) { err = "hipMalloc failed"; return false; }
+    (void)hipMemset(q, 0, count * sizeof(P) + 256);
+    allocs.push_back(q);
+    p = static_cast<P*>(q);
+    return true;
+  }
+  ~Mlp() {
+    for (void* p : allocs) (void)hipFree(p);
+    if (st) (void)hipStreamDestroy(st);
+  }
+  bool ensure_rows(int rows, std::string& err) {
+    const int need = pad64(rows);
+    if (need <= cap) return true;
+    cap = need;
+    return alloc(Xb, (size_t)cap * pin, err) && alloc(H1b, (size_t)cap * ph, err) &&
+           alloc(H2b, (size_t)cap * ph, err) && alloc(dYb, (size_t)cap * pout, err) &&
+           alloc(dZ2b, (size_t)cap * ph, err) && alloc(dZ1b, (size_t)cap * ph, err) &&
+           alloc(Z1, (size_t)cap * ph, err) && alloc(Z2, (size_t)cap * ph, err) &&
+           alloc(Y, (size_t)cap * pout, err) && alloc(dYf, (size_t)cap * pout, err) &&
+           alloc(T, (size_t)cap * nout, err) && alloc(Xf, (size_t)cap * nin, err) &&
+           alloc(part, 1024, err);
+  }
+  void refresh_bf16() {
+    hipLaunchKernelGGL(k_to_bf16, dim3(256), dim3(256), 0, st, (long)pin * ph, W1, W1b);
+    hipLaunchKernelGGL(k_to_bf16, dim3(256), dim3(256), 0, st, (long)ph * ph, W2, W2b);
+    hipLaunchKernelGGL(k_to_bf16, dim3(256), dim3(256), 0, st, (long)ph * pout, W3, W3b);
+  }
+  // forward for `rows` rows whose raw inputs are in Xf (device, rows x nin)
+  void forward(int rows, bool affine_out) {
+    const int rp = pad64(rows);
+    hipLaunchKernelGGL(k_prep_input, dim3(512), dim3(256), 0, st, rows, nin, nin, Xf, in_mu, in_sd, pin, rp, Xb);
+    gemm<false, false, EPI_BIAS_GELU>(st, rp, ph, pin, Xb, pin, W1b, ph, Z1, H1b, ph, b1);
+    gemm<false, false, EPI_BIAS_GELU>(st, rp, ph, ph, H1b, ph, W2b, ph, Z2, H2b, ph, b2);
+    gemm<false, false, EPI_BIAS_AFFINE>(st, rp, pout, ph, H2b, ph, W3b, pout, Y, nullptr, pout, b3, nullptr,
+                                        affine_out ? out_mul : nullptr, affine_out ? out_add : nullptr);
+  }
+  // loss and parameter gradients into g for the rows in Xf / T
+  double backward(int rows) {
+    const int rp = pad64(rows);
+    const int nb = 256;
+    hipLaunchKernelGGL(k_loss_grad, dim3(nb), dim3(GT), 0, st, rows, nout, pout, Y, T, nout, rp, dYf, dYb, part);
+    float* gW1 = g + off_W1; float* gb1 = g + off_b1;
+    float* gW2 = g + off_W2; float* gb2 = g + off_b2;
+    float* gW3 = g + off_W3; float* gb3 = g + off_b3;
+    gemm<true, false, EPI_F32>(st, ph, pout, rp, H2b, ph, dYb, pout, gW3, nullptr, pout);
+    hipLaunchKernelGGL(k_colsum<float>, dim3((pout + 255) / 256), dim3(256), 0, st, rp, pout, pout, dYf, gb3);
+    gemm<false, true, EPI_GELU_GRAD>(st, rp, ph, pout, dYb, pout, W3b, pout, nullptr, dZ2b, ph, nullptr, Z2);
+    gemm<true, false, EPI_F32>(st, ph, ph, rp, H1b, ph, dZ2b, ph, gW2, nullptr, ph);
+    hipLaunchKernelGGL(k_colsum<__bf16>, dim3((ph + 255) / 256), dim3(256), 0, st, rp, ph, ph, dZ2b, gb2);
+    gemm<false, true, EPI_GELU_GRAD>(st, rp, ph, ph, dZ2b, ph, W2b, ph, nullptr, dZ1b, ph, nullptr, Z1);
+    gemm<true, false, EPI_F32>(st, pin, ph, rp, Xb, pin, dZ1b, ph, gW1, nullptr, ph);
+    hipLaunchKernelGGL(k_colsum<__bf16>, dim3((ph + 255) / 256), dim3(256), 0, st, rp, ph, ph, dZ1b, gb1);
+    std::vector<double> hp(nb);
+    (void)hipMemcpyAsync(hp.data(), part, nb * sizeof(double), hipMemcpyDeviceToHost, st);
+    (void)hipStreamSynchronize(st);
+    double s = 0.0;
+    for (double v : hp) s += v;
+    return s / ((double)rows * nout);
+  }
+  void adam(float lr) {
+    ++step;
+    const float b1c = 0.9f, b2c = 0.999f;
+    const float c1 = 1.f - std::pow(b1c, (float)step), c2 = 1.f - std::pow(b2c, (float)step);
+    hipLaunchKernelGGL(k_adam, dim3(1024), dim3(256), 0, st, nparam, master, g, mom, vel, (__bf16*)nullptr, lr,
+                       b1c, b2c, c1, c2, 1e-8f);
+    refresh_bf16();
+  }
+};
+
+}  // namespace pdos_amd
+
+using pdos_amd::Mlp;
+
+struct PdosMlp {
+  Mlp net;
+};
+
+namespace {
+thread_local std::string g_mlp_err;
+}
+
+extern "C" {
+
+const char* pdos_mlp_last_error(void) { return g_mlp_err.c_str(); }
+
+PdosMlp* pdos_mlp_create(int n_in, int n_hidden, int n_out, unsigned long seed, int device) {
+  using namespace pdos_amd;
+  if (n_in < 1 || n_hidden < 1 || n_out < 1) { g_mlp_err = "bad sizes"; return nullptr; }
+  if (hipSetDevice(device) != hipSuccess) { g_mlp_err = "hipSetDevice failed"; return nullptr; }
+  PdosMlp* P = new PdosMlp;
+  Mlp& M = P->net;
+  std::string err;
+  M.nin = n_in; M.nh = n_hidden; M.nout = n_out;
+  M.pin = pad64(n_in); M.ph = pad64(n_hidden); M.pout = pad64(n_out);
+  M.off_W1 = 0;
+  M.off_b1 = M.off_W1 + (size_t)M.pin * M.ph;
+  M.off_W2 = M.off_b1 + M.ph;
+  M.off_b2 = M.off_W2 + (size_t)M.ph * M.ph;
+  M.off_W3 = M.off_b2 + M.ph;
+  M.off_b3 = M.off_W3 + (size_t)M.ph * M.pout;
+  M.nparam = (long)(M.off_b3 + M.pout);
+  if (hipStreamCreateWithFlags(&M.st, hipStreamNonBlocking) != hipSuccess ||
+      !M.alloc(M.master, M.nparam, err) || !M.alloc(M.g, M.nparam, err) || !M.alloc(M.mom, M.nparam, err) ||
+      !M.alloc(M.vel, M.nparam, err) || !M.alloc(M.W1b, (size_t)M.pin * M.ph, err) ||
+      !M.alloc(M.W2b, (size_t)M.ph * M.ph, err) || !M.alloc(M.W3b, (size_t)M.ph * M.pout, err) ||
+      !M.alloc(M.in_mu, M.nin, err) || !M.alloc(M.in_sd, M.nin, err) || !M.alloc(M.out_mul, M.pout, err) ||
+      !M.alloc(M.out_add, M.pout, err)) {
+    g_mlp_err = err.empty() ? "stream create failed" : err;
+    delete P;
+    return nullptr;
+  }
+  M.W1 = M.master + M.off_W1; M.b1 = M.master + M.off_b1;
+  M.W2 = M.master + M.off_W2; M.b2 = M.master + M.off_b2;
+  M.W3 = M.master + M.off_W3; M.b3 = M.master + M.off_b3;
+  // He-style init on the logical block, zeros on the padding
+  std::vector<float> h(M.nparam, 0.f);
+  std::mt19937_64 rng(seed);
+  auto fill = [&](size_t off, int rows, int cols, int ld, float scale) {
+    std::normal_distribution<float> nd(0.f, scale);
+    for (int r = 0; r < rows; ++r)
+      for (int c = 0; c < cols; ++c) h[off + (size_t)r * ld + c] = nd(rng);
+  };
+  fill(M.off_W1, M.nin, M.nh, M.ph, std::sqrt(2.f / M.nin));
+  fill(M.off_W2, M.nh, M.nh, M.ph, std::sqrt(2.f / M.nh));
+  fill(M.off_W3, M.nh, M.nout, M.pout, std::sqrt(1.f / M.nh));
+  std::vector<float> ones(M.pout, 1.f), zeros(M.pout, 0.f);
+  (void)hipMemcpy(M.master, h.data(), h.size() * 4, hipMemcpyHostToDevice);
+  (void)hipMemcpy(M.in_mu, zeros.data(), M.nin * 4, hipMemcpyHostToDevice);
+  (void)hipMemcpy(M.in_sd, ones.data(), M.nin * 4, hipMemcpyHostToDevice);
+  (void)hipMemcpy(M.out_mul, ones.data(), M.pout * 4, hipMemcpyHostToDevice);
+  (void)hipMemcpy(M.out_add, zeros.data(), M.pout * 4, hipMemcpyHostToDevice);
+  M.refresh_bf16();
+  (void)hipStreamSynchronize(M.st);
+  return P;
+}
+
+void pdos_mlp_free(PdosMlp* P) { delete P; }
+
+int pdos_mlp_num_params(const PdosMlp* P) {
+  const Mlp& M = P->net;
+  return M.nin * M.nh + M.nh + M.nh * M.nh + M.nh + M.nh * M.nout + M.nout;
+}
+
+// Parameters in logical layout W1 (in x h), b1, W2 (h x h), b2, W3 (h x out), b3.
+static void copy_params(Mlp& M, float* host, bool to_device) {
+  size_t o = 0;
+  auto blk = [&](size_t off, int rows, int cols, int ld) {
+    for (int r = 0; r < rows; ++r) {
+      float* dev = M.master + off + (size_t)r * ld;
+      if (to_device) (void)hipMemcpy(dev, host + o, cols * 4, hipMemcpyHostToDevice);
+      else (void)hipMemcpy(host + o, dev, cols * 4, hipMemcpyDeviceToHost);
+      o += cols;
+    }
+  };
+  blk(M.off_W1, M.nin, M.nh, M.ph);
+  blk(M.off_b1, 1, M.nh, M.ph);
+  blk(M.off_W2, M.nh, M.nh, M.ph);
+  blk(M.off_b2, 1, M.nh, M.ph);
+  blk(M.off_W3, M.nh, M.nout, M.pout);
+  blk(M.off_b3, 1, M.nout, M.pout);
+}
+
+int pdos_mlp_get_params(PdosMlp* P, float* out) {
+  copy_params(P->net, out, false);
+  return 0;
+}
+
+int pdos_mlp_set_params(PdosMlp* P, const float* in) {
+  copy_params(P->net, const_cast<float*>(in), true);
+  P->net.refresh_bf16();
+  return hipStreamSynchronize(P->net.st) == hipSuccess ? 0 : -1;
+}
+
+int pdos_mlp_set_affine(PdosMlp* P, const float* in_mean, const float* in_std, const float* out_std,
+                        const float* out_mean) {
+  Mlp& M = P->net;
+  if (in_mean) (void)hipMemcpy(M.in_mu, in_mean, M.nin * 4, hipMemcpyHostToDevice);
+  if (in_std) (void)hipMemcpy(M.in_sd, in_std, M.nin * 4, hipMemcpyHostToDevice);
+  if (out_std) (void)hipMemcpy(M.out_mul, out_std, M.nout * 4, hipMemcpyHostToDevice);
+  if (out_mean) (void)hipMemcpy(M.out_add, out_mean, M.nout * 4, hipMemcpyHostToDevice);
+  return 0;
+}
+
+// Loss and gradients (logical layout, same as get_params) of one batch;
+// targets T are in the standardised output space.
+int pdos_mlp_loss_grad(PdosMlp* P, const float* X, const float* T, int rows, double* loss, float* grads) {
+  Mlp& M = P->net;
+  std::string err;
+  if (!M.ensure_rows(rows, err)) { g_mlp_err = err; return -1; }
+  (void)hipMemcpy(M.Xf, X, (size_t)rows * M.nin * 4, hipMemcpyHostToDevice);
+  (void)hipMemcpy(M.T, T, (size_t)rows * M.nout * 4, hipMemcpyHostToDevice);
+  M.forward(rows, false);
+  const double l = M.backward(rows);
+  if (loss) *loss = l;
+  if (grads) {
+    float* save = M.master;
+    M.master = M.g;  // reuse the layout walker on the gradient block
+    copy_params(M, grads, false);
+    M.master = save;
+  }
+  return hipStreamSynchronize(M.st) == hipSuccess ? 0 : -1;
+}
+
+// Trains `epochs` passes over (X, T) (host arrays, rows x nin / rows x nout,
+// T standardised) in shuffled minibatches; returns the last epoch's mean loss.
+int pdos_mlp_train(PdosMlp* P, const float* X, const float* T, int rows, int epochs, int batch, float lr,
+                   unsigned long seed, double* last_loss) {
+  Mlp& M = P->net;
+  std::string err;
+  if (batch < 1 || rows < 1) { g_mlp_err = "bad batch"; return -1; }
+  if (!M.ensure_rows(batch, err)) { g_mlp_err = err; return -1; }
+  // the whole training set lives on the device; minibatches are gathered by index
+  float *dX = nullptr, *dT = nullptr;
+  if (hipMalloc(&dX, (size_t)rows * M.nin * 4) != hipSuccess ||
+      hipMalloc(&dT, (size_t)rows * M.nout * 4) != hipSuccess) {
+    g_mlp_err = "hipMalloc failed";
+    return -1;
+  }
+  (void)hipMemcpy(dX, X, (size_t)rows * M.nin * 4, hipMemcpyHostToDevice);
+  (void)hipMemcpy(dT, T, (size_t)rows * M.nout * 4, hipMemcpyHostToDevice);
+  std::vector<int> order(rows);
+  for (int i = 0; i < rows; ++i) order[i] = i;
+  int* dIdx = nullptr;
+  if (hipMalloc(&dIdx, (size_t)rows * 4) != hipSuccess) { g_mlp_err = "hipMalloc failed"; return -1; }
+  std::mt19937_64 rng(seed);
+  double epoch_loss = 0.0;
+  for (int e = 0; e < epochs; ++e) {
+    std::shuffle(order.begin(), order.end(), rng);
+    (void)hipMemcpyAsync(dIdx, order.data(), (size_t)rows * 4, hipMemcpyHostToDevice, M.st);
+    double acc = 0.0;
+    int nb = 0;
+    for (int s = 0; s + batch <= rows; s += batch) {
+      hipLaunchKernelGGL(pdos_amd::k_gather_rows, dim3(512), dim3(256), 0, M.st, batch, M.nin, dIdx + s, dX, M.Xf);
+      hipLaunchKernelGGL(pdos_amd::k_gather_rows, dim3(1024), dim3(256), 0, M.st, batch, M.nout, dIdx + s, dT, M.T);
+      M.forward(batch, false);
+      acc += M.backward(batch);
+      M.adam(lr);
+      ++nb;
+    }
+    epoch_loss = nb ? acc / nb : 0.0;
+  }
+  (void)hipStreamSynchronize(M.st);
+  (void)hipFree(dX);
+  (void)hipFree(dT);
+  (void)hipFree(dIdx);
+  if (last_loss) *last_loss = epoch_loss;
+  return hipStreamSynchronize(M.st) == hipSuccess ? 0 : -1;
+}
+
+// Predictions in the original output units (out_std * y + out_mean) for
+// `rows` host input rows; Y is rows x nout.
+int pdos_mlp_predict(PdosMlp* P, const float* X, int rows, float* Y, double* kernel_ms) {
+  Mlp& M = P->net;
+  std::string err;
+  const int chunk = 8192;
+  if (!M.ensure_rows(std::min(rows, chunk), err)) { g_mlp_err = err; return -1; }
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  float total = 0.f;
+  for (int s = 0; s < rows; s += chunk) {
+    const int r = std::min(chunk, rows - s);
+    (void)hipMemcpyAsync(M.Xf, X + (size_t)s * M.nin, (size_t)r * M.nin * 4, hipMemcpyHostToDevice, M.st);
+    (void)hipEventRecord(e0, M.st);
+    M.forward(r, true);
+    (void)hipEventRecord(e1, M.st);
+    (void)hipMemcpy2DAsync(Y + (size_t)s * M.nout, M.nout * 4, M.Y, M.pout * 4, M.nout * 4, r,
+                           hipMemcpyDeviceToHost, M.st);
+    (void)hipStreamSynchronize(M.st);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    total += ms;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (kernel_ms) *kernel_ms = total;
+  return hipStreamSynchronize(M.st) == hipSuccess ? 0 : -1;
+}
+
+// Device-to-device prediction: dX (device, rows x nin raw features) to dY
+// (device, rows x ldy, original output units).  The timed warm-start path.
+int pdos_mlp_predict_dev(PdosMlp* P, const float* dX, int rows, float* dY, int ldy) {
+  Mlp& M = P->net;
+  std::string err;
+  const int chunk = 8192;
+  if (!M.ensure_rows(std::min(rows, chunk), err)) { g_mlp_err = err; return -1; }
+  for (int s = 0; s < rows; s += chunk) {
+    const int r = std::min(chunk, rows - s);
+    (void)hipMemcpyAsync(M.Xf, dX + (size_t)s * M.nin, (size_t)r * M.nin * 4, hipMemcpyDeviceToDevice, M.st);
+    M.forward(r, true);
+    (void)hipMemcpy2DAsync(dY + (size_t)s * ldy, (size_t)ldy * 4, M.Y, M.pout * 4, M.nout * 4, r,
+                           hipMemcpyDeviceToDevice, M.st);
+  }
+  return hipStreamSynchronize(M.st) == hipSuccess ? 0 : -1;
+}
+
+}  // extern "C"

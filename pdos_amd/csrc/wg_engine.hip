@@ -159,13 +159,23 @@ __global__ __launch_bounds__(NT) void k_wg_solve(const WgInst* __restrict__ inst
   const T relax_a = T(I.alpha);
   const T relax_b = T(1.0) - T(I.alpha);
 
-  for (int i = tid; i < n; i += NT) x[i] = T(I.x0[i]);
-  for (int j = tid; j < m; j += NT) { s[j] = T(I.s0[j]); y[j] = T(I.y0[j]); }
+  if (I.wsrc) {  // warm start in original units (common.h:148-161, 188): x/E, y/D, s*D
+    const float* w = I.wsrc;
+    for (int i = tid; i < n; i += NT) x[i] = T(double(w[i]) / Eg[i]);
+    for (int j = tid; j < m; j += NT) {
+      y[j] = T(double(w[n + j]) / Dg[j]);
+      s[j] = T(double(w[n + m + j]) * Dg[j]);
+    }
+  } else {
+    for (int i = tid; i < n; i += NT) x[i] = T(I.x0[i]);
+    for (int j = tid; j < m; j += NT) { s[j] = T(I.s0[j]); y[j] = T(I.y0[j]); }
+  }
   __syncthreads();
   spmv_rows<T>(m, rp, ri, rx, ga, [&](int c) { return x[c]; }, [&](int j, T v) { ax[j] = v; });
   __syncthreads();
 
-  int slot = 0, it = 0, status = 1, cg_total = 0;
+  int slot = 0, it = 0, status = 1, cg_total = 0, fallback = 0;
+  bool judge = I.warm_check != 0;
   double pres = -1.0, dres = -1.0, pobj = -1.0, dobj = -1.0, eta = -1.0;
 
   for (it = 0; it < I.max_iters; ++it) {
@@ -190,6 +200,21 @@ __global__ __launch_bounds__(NT) void k_wg_solve(const WgInst* __restrict__ inst
     pobj = acc[2];
     dobj = -acc[3];
     eta = fabs(pobj - dobj);
+    if (judge) {
+      // ADMM judges the warm start: keep it only if its scaled residuals are
+      // no worse than the cold start's, ||b|| / pscale and ||c|| / dscale.
+      judge = false;
+      const double mw = fmax(fmax(pres / I.pscale, dres / I.dscale), eta / (1.0 + fabs(pobj) + fabs(dobj)));
+      const double mc = fmax((I.pscale - 1.0) / I.pscale, (I.dscale - 1.0) / I.dscale);
+      if (!(mw <= mc)) {
+        fallback = 1;
+        for (int i = tid; i < n; i += NT) x[i] = T(0);
+        for (int j = tid; j < m; j += NT) { s[j] = T(0); y[j] = T(0); ax[j] = T(0); }
+        __syncthreads();
+        it = -1;
+        continue;
+      }
+    }
     if (pres < I.eps * I.pscale && dres < I.eps * I.dscale &&
         eta < I.eps * (1.0 + fabs(pobj) + fabs(dobj))) {
       status = 0;
@@ -315,6 +340,7 @@ __global__ __launch_bounds__(NT) void k_wg_solve(const WgInst* __restrict__ inst
     R->iters = it;
     R->status = status;
     R->cg_iters = cg_total;
+    R->fallback = fallback;
     R->p_res = pres;
     R->d_res = dres;
     R->p_obj = pobj;
