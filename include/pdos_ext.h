@@ -101,6 +101,31 @@ void pdos_device_free(void *p);
 int pdos_device_upload(void *dst, const void *src, size_t bytes);
 int pdos_device_download(void *dst, const void *src, size_t bytes);
 
+/* A family of cone programs sharing A, the cones and the parameters of the
+ * template data d (its b is the default b, its c is ignored), differing in c
+ * (and optionally b).  Solved in lockstep with G = (I + A'A)^-1 formed once:
+ * the x-update is one GEMM for the whole batch (PDOS_FP64: rocBLAS dgemm;
+ * PDOS_MIXED: split-bf16 MFMA, three bf16 products accumulated in binary32).
+ *   load   C: count x n, B: count x m or NULL (original units, row-major)
+ *   run    warm_dev: optional device array count x ld of [x, y, s] in original
+ *          units; judge != 0 compares it with the cold start per instance
+ *   fetch  X: count x n, Y, S: count x m (any may be NULL), iters, status
+ *          (0 Solved, 1 Indeterminate), fallback, resid (count x 5: p_res,
+ *          d_res, c'x, -b'y, gap of the last stop check) */
+typedef struct PdosFamily PdosFamily;
+PdosFamily *pdos_family_create(const Data *d, const Cone *k, const PdosOptions *opt);
+int pdos_family_load(PdosFamily *h, int count, const double *C, const double *B);
+int pdos_family_run(PdosFamily *h, const float *warm_dev, long ld, int judge, double *kernel_ms);
+int pdos_family_fetch(PdosFamily *h, double *X, double *Y, double *S, long *iters, long *status,
+                      long *fallback, double *resid);
+void pdos_family_free(PdosFamily *h);
+const char *pdos_family_last_error(void);
+/* Per-kernel HIP-event timing of the solve loop.  Returns (and resets) the
+ * totals since the last call in ms[6] / calls[6] for the kinds: phase A,
+ * A' product, phase B, x-update GEMM, A product, phase C; then turns the
+ * timing on or off for later runs (events add a little launch overhead). */
+int pdos_family_timing(PdosFamily *h, int enable, double *ms, long *calls);
+
 /* Learned warm start: MLP n_in -> n_hidden -> n_hidden -> n_out with GELU,
  * bf16 weights mirrored from a binary32 master, binary32 accumulation, all
  * products on BF16 MFMA (hand-written kernels), Adam training.

@@ -51,7 +51,10 @@ EXPORTS = ["pdos", "freeData", "freeSol", "tic", "toc", "tocq", "pdos_default_op
            "pdos_device_free", "pdos_device_upload", "pdos_device_download", "pdos_mlp_create",
            "pdos_mlp_free", "pdos_mlp_num_params", "pdos_mlp_get_params", "pdos_mlp_set_params",
            "pdos_mlp_set_affine", "pdos_mlp_loss_grad", "pdos_mlp_train", "pdos_mlp_predict",
-           "pdos_mlp_predict_dev", "pdos_mlp_last_error", "pdos_kkt_solve_host", "pdos_last_error",
+           "pdos_mlp_predict_dev", "pdos_mlp_last_error", "pdos_family_create", "pdos_family_load",
+           "pdos_family_run", "pdos_family_fetch", "pdos_family_free", "pdos_family_last_error",
+           "pdos_family_timing",
+           "pdos_kkt_solve_host", "pdos_last_error",
            "pdos_version"]
 
 _libs = {}

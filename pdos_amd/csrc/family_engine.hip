@@ -1,0 +1,980 @@
+// family_engine.hip -- many cone programs that share A (and the cones), each
+// with its own c (and optionally b), solved in lockstep.
+//
+// Every per-instance vector is stored feature-major, X[row * ldb + instance],
+// so a wave of 64 lanes holds 64 instances: the shared matrix entries are
+// wave-uniform (scalar loads, broadcast) and each instance's values are one
+// coalesced 256-byte (fp32) row segment.  With G = (I + A'A)^-1 formed once,
+// the x-update of the direct plugin (direct/private.c:16-56) is one GEMM for
+// the whole batch:
+//
+//   phase A   W = b - S - lam Y; |(S - b + AX) ./ D|^2, b'Y      (per instance)
+//   SpMM      U = A'Y, V = A'W                                   (one pass over A')
+//   phase B   |(c + U) ./ E|^2, c'X, stop test, R = X - lam c + V
+//   GEMM      XN = G R     (fp64: rocBLAS dgemm)
+//             XN = X + G (V - A'AX - lam c)  (mixed: split-bf16 MFMA, 3 products; the
+//             correction form keeps the bf16 error proportional to the step)
+//   SpMM      AX = A XN
+//   phase C   X = XN; S~ = b - AX; relax; S = Pi_K(S~ - lam Y); Y += (S - S~)/lam
+//
+// An instance that meets the stop test (pdos.c:95-107) or MAX_ITERS is frozen:
+// phases A-C skip it, so its iterate is exactly the one the stop test saw.
+// Warm starts (original units) are scaled in on entry; with the judge on, an
+// extra A/B pass compares the warm start's residuals with the cold start's and
+// resets rejected instances to zero.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/pdos_ext.h"
+#include "engine.hpp"
+
+namespace pdos_amd {
+
+hipError_t bf16_gemm_f32(hipStream_t st, int M, int N, int K, const __bf16* A, int lda, const __bf16* B,
+                         int ldb, float* C, int ldc, const float* addend);
+
+namespace {
+
+constexpr int LANES = 64;  // instances per block
+constexpr int NWV = 8;     // waves per block, splitting rows
+constexpr int FT = LANES * NWV;
+
+template <typename T>
+struct FamArgs {
+  int n, m, f, lend, nsoc, ldb, count, np, b_inst;
+  int max_iters, judge;
+  double eps, relax;
+  const int* soc_ptr;
+  const int* rp; const int* ri; const T* rx;   // A rows
+  const int* cp; const int* ci; const T* cx;   // A' rows
+  const double* D; const double* E;
+  T* b;                // m (shared) or m x ldb
+  T* c;                // n x ldb, scaled (x1e3)
+  double* lam; double* pscale; double* dscale;
+  int* orig;           // column -> instance of the loaded batch
+  T* X; T* XN; T* S; T* Y; T* AX; T* W; T* U; T* V; T* R; T* Z;
+  __bf16* Rcat;        // mixed: [D_hi; D_lo; D_hi] of the correction D, 3 np x ldb
+  int* done; int* status; int* iters; int* fallback;
+  double* red;         // per instance: pres2, by, pres, dres, pobj, dobj, eta, spare
+};
+
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
+// Sums one value per (wave, lane) over the waves of the block: every thread
+// gets its lane's total.  Two barriers; `buf` is NWV x LANES doubles.
+__device__ __forceinline__ double waves_sum(double v, double* buf) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  buf[w * LANES + lane] = v;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < NWV; ++k) s += buf[k * LANES + lane];
+  __syncthreads();
+  return s;
+}
+
+template <typename T>
+__device__ __forceinline__ T bval(const FamArgs<T>& a, int j, int inst) {
+  return a.b_inst ? a.b[(size_t)j * a.ldb + inst] : a.b[j];
+}
+
+// Initial iterate from a warm start in original units (x/E, y/D, s*D) or zero.
+template <typename T>
+__global__ __launch_bounds__(FT) void k_fam_init(FamArgs<T> a, const float* __restrict__ warm, long wld) {
+  const int lane = threadIdx.x & 63, w = wave_id();
+  const int inst = blockIdx.x * LANES + lane;
+  const bool live = inst < a.count;
+  for (int i = w; i < a.n; i += NWV) {
+    T v = T(0);
+    if (live && warm) v = T(double(warm[(size_t)inst * wld + i]) / a.E[i]);
+    a.X[(size_t)i * a.ldb + inst] = v;
+  }
+  for (int r = a.n + w; r < a.np; r += NWV) {  // padding rows of the GEMM operands
+    a.X[(size_t)r * a.ldb + inst] = T(0);
+    if (a.Rcat)
+      for (int k = 0; k < 3; ++k) a.Rcat[(size_t)(k * a.np + r) * a.ldb + inst] = (__bf16)0.f;
+  }
+  for (int j = w; j < a.m; j += NWV) {
+    T yv = T(0), sv = T(0);
+    if (live && warm) {
+      yv = T(double(warm[(size_t)inst * wld + a.n + j]) / a.D[j]);
+      sv = T(double(warm[(size_t)inst * wld + a.n + a.m + j]) * a.D[j]);
+    }
+    a.Y[(size_t)j * a.ldb + inst] = yv;
+    a.S[(size_t)j * a.ldb + inst] = sv;
+  }
+  if (w == 0) {
+    a.done[inst] = live ? 0 : 1;
+    a.status[inst] = 1;
+    a.iters[inst] = 0;
+    a.fallback[inst] = 0;
+    for (int k = 0; k < 8; ++k) a.red[(size_t)inst * 8 + k] = -1.0;
+  }
+}
+
+// Y_k = M X_k (k < NR) for a shared sparse M (rows ptr/idx/val) and
+// feature-major X_k: up to three right-hand sides per pass over M.
+template <typename T, int NR>
+__global__ __launch_bounds__(FT) void k_fam_spmm(int nrows, const int* __restrict__ ptr,
+                                                 const int* __restrict__ idx, const T* __restrict__ val,
+                                                 const T* __restrict__ X1, const T* __restrict__ X2,
+                                                 const T* __restrict__ X3, T* __restrict__ Y1,
+                                                 T* __restrict__ Y2, T* __restrict__ Y3, int ldb) {
+  const int lane = threadIdx.x & 63, w = wave_id();
+  const int inst = blockIdx.x * LANES + lane;
+  for (int r = w; r < nrows; r += NWV) {
+    T a1 = T(0), a2 = T(0), a3 = T(0);
+    const int pe = ptr[r + 1];
+    for (int p = ptr[r]; p < pe; ++p) {
+      const T v = val[p];
+      const size_t o = (size_t)idx[p] * ldb + inst;
+      a1 += v * X1[o];
+      if (NR > 1) a2 += v * X2[o];
+      if (NR > 2) a3 += v * X3[o];
+    }
+    const size_t q = (size_t)r * ldb + inst;
+    Y1[q] = a1;
+    if (NR > 1) Y2[q] = a2;
+    if (NR > 2) Y3[q] = a3;
+  }
+}
+
+// Phase A: W = b - S - lam Y and the m-row parts of the stop test.
+template <typename T>
+__global__ __launch_bounds__(FT) void k_fam_a(FamArgs<T> a) {
+  __shared__ double buf[NWV * LANES];
+  const int lane = threadIdx.x & 63, w = wave_id();
+  const int inst = blockIdx.x * LANES + lane;
+  const bool act = inst < a.count && !a.done[inst];
+  double p2 = 0.0, by = 0.0;
+  if (act) {
+    const T lam = T(a.lam[inst]);
+    for (int j = w; j < a.m; j += NWV) {
+      const size_t o = (size_t)j * a.ldb + inst;
+      const T s = a.S[o], y = a.Y[o], bj = bval(a, j, inst);
+      const T r = ((s - bj) + a.AX[o]) / T(a.D[j]);
+      p2 += double(r) * double(r);
+      by += double(bj) * double(y);
+      a.W[o] = bj - (s + lam * y);
+    }
+  }
+  p2 = waves_sum(p2, buf);
+  by = waves_sum(by, buf);
+  if (act && w == 0) {
+    a.red[(size_t)inst * 8 + 0] = p2;
+    a.red[(size_t)inst * 8 + 1] = by;
+  }
+}
+
+// Phase B: the n-row parts of the stop test, the test itself, and R.
+// With a.judge the pass instead compares the warm start with the cold start.
+template <typename T, bool MIXED>
+__global__ __launch_bounds__(FT) void k_fam_b(FamArgs<T> a) {
+  __shared__ double buf[NWV * LANES];
+  __shared__ int s_reset[LANES];
+  const int lane = threadIdx.x & 63, w = wave_id();
+  const int inst = blockIdx.x * LANES + lane;
+  const bool act = inst < a.count && !a.done[inst];
+  const int it0 = act ? a.iters[inst] : 0;  // read before wave 0 advances it
+  double d2 = 0.0, cx = 0.0;
+  if (act) {
+    for (int i = w; i < a.n; i += NWV) {
+      const size_t o = (size_t)i * a.ldb + inst;
+      const T ci = a.c[o];
+      const T r = (ci + a.U[o]) / T(a.E[i]);
+      d2 += double(r) * double(r);
+      cx += double(ci) * double(a.X[o]);
+    }
+  }
+  d2 = waves_sum(d2, buf);
+  cx = waves_sum(cx, buf);
+  bool upd = false;
+  if (act) {
+    double* rd = a.red + (size_t)inst * 8;
+    const double pres = sqrt(rd[0]), dres = sqrt(d2), pobj = cx, dobj = -rd[1];
+    const double eta = fabs(pobj - dobj);
+    const double ps = a.pscale[inst], ds = a.dscale[inst];
+    if (a.judge) {
+      const double mw = fmax(fmax(pres / ps, dres / ds), eta / (1.0 + fabs(pobj) + fabs(dobj)));
+      const double mc = fmax((ps - 1.0) / ps, (ds - 1.0) / ds);
+      if (w == 0) s_reset[lane] = !(mw <= mc);
+    } else if (it0 >= a.max_iters) {
+      if (w == 0) a.done[inst] = 1;  // Indeterminate; residuals of the last check stay
+    } else {
+      if (w == 0) {
+        rd[2] = pres; rd[3] = dres; rd[4] = pobj; rd[5] = dobj; rd[6] = eta;
+      }
+      if (pres < a.eps * ps && dres < a.eps * ds && eta < a.eps * (1.0 + fabs(pobj) + fabs(dobj))) {
+        if (w == 0) { a.done[inst] = 1; a.status[inst] = 0; }
+      } else {
+        upd = true;
+        if (w == 0) a.iters[inst] = it0 + 1;
+      }
+    }
+  }
+  if (a.judge) {
+    __syncthreads();
+    if (act && s_reset[lane]) {  // rejected warm start: cold start for this instance
+      for (int i = w; i < a.n; i += NWV) a.X[(size_t)i * a.ldb + inst] = T(0);
+      for (int j = w; j < a.m; j += NWV) {
+        const size_t o = (size_t)j * a.ldb + inst;
+        a.S[o] = T(0); a.Y[o] = T(0); a.AX[o] = T(0);
+      }
+      if (w == 0) a.fallback[inst] = 1;
+    }
+    return;
+  }
+  if (upd) {
+    const T lam = T(a.lam[inst]);
+    for (int i = w; i < a.n; i += NWV) {
+      const size_t o = (size_t)i * a.ldb + inst;
+      // fp64: R = x - lam c + A'(b - s - lam y), then x+ = G R.
+      // mixed: the correction Delta = R - (I + A'A) x = A'(b - s - lam y) - lam c - A'(A x),
+      // then x+ = x + G Delta, so the bf16 error of the product shrinks with Delta.
+      const T rv = MIXED ? (a.V[o] - a.Z[o]) - lam * a.c[o] : a.X[o] - lam * a.c[o] + a.V[o];
+      if (MIXED) {
+        const __bf16 hi = (__bf16)float(rv);
+        const __bf16 lo = (__bf16)(float(rv) - float(hi));
+        a.Rcat[(size_t)i * a.ldb + inst] = hi;
+        a.Rcat[(size_t)(a.np + i) * a.ldb + inst] = lo;
+        a.Rcat[(size_t)(2 * a.np + i) * a.ldb + inst] = hi;
+      } else {
+        a.R[o] = rv;
+      }
+    }
+  }
+}
+
+// Phase C: X = XN, s~ = b - AX, relax, cone projection, dual update.
+template <typename T>
+__global__ __launch_bounds__(FT) void k_fam_c(FamArgs<T> a) {
+  __shared__ double buf[NWV * LANES];
+  __shared__ double s_t0[LANES];
+  const int lane = threadIdx.x & 63, w = wave_id();
+  const int inst = blockIdx.x * LANES + lane;
+  const bool act = inst < a.count && !a.done[inst];
+  const T lam = act ? T(a.lam[inst]) : T(1);
+  const T ra = T(a.relax), rb = T(1.0) - T(a.relax);
+  if (act) {
+    for (int i = w; i < a.n; i += NWV) {
+      const size_t o = (size_t)i * a.ldb + inst;
+      a.X[o] = a.XN[o];
+    }
+    for (int j = w; j < a.lend; j += NWV) {
+      const size_t o = (size_t)j * a.ldb + inst;
+      T st = bval(a, j, inst) - a.AX[o];
+      st = ra * st + rb * a.S[o];
+      const T z = st - lam * a.Y[o];
+      const T s = j < a.f ? T(0) : (z < T(0) ? T(0) : z);
+      a.S[o] = s;
+      a.Y[o] += (s - st) / lam;
+    }
+  }
+  for (int cn = 0; cn < a.nsoc; ++cn) {
+    const int r0 = a.soc_ptr[cn], r1 = a.soc_ptr[cn + 1];
+    double n2 = 0.0;
+    if (act) {
+      for (int j = r0 + w; j < r1; j += NWV) {
+        const size_t o = (size_t)j * a.ldb + inst;
+        T st = bval(a, j, inst) - a.AX[o];
+        st = ra * st + rb * a.S[o];
+        const T z = st - lam * a.Y[o];
+        a.W[o] = st;   // keep s~ for the dual update
+        a.S[o] = z;
+        if (j > r0) n2 += double(z) * double(z);
+        else s_t0[lane] = double(z);  // row r0 belongs to wave 0
+      }
+    }
+    n2 = waves_sum(n2, buf);  // also orders the z writes before the reads below
+    if (act) {
+      const T ns = T(sqrt(n2));
+      const T t0 = T(s_t0[lane]);
+      const int mode = ns <= t0 ? 0 : (ns <= -t0 ? 1 : 2);
+      const T al = (ns + t0) / T(2);
+      for (int j = r0 + w; j < r1; j += NWV) {
+        const size_t o = (size_t)j * a.ldb + inst;
+        const T z = a.S[o];
+        const T s = mode == 0 ? z : (mode == 1 ? T(0) : (j == r0 ? al : al * z / ns));
+        a.Y[o] += (s - a.W[o]) / lam;
+        a.S[o] = s;
+      }
+    }
+    __syncthreads();  // s_t0 is rewritten by the next cone
+  }
+}
+
+// Row-major m x n dense copy with the lower triangle mirrored into the upper.
+template <typename T>
+__global__ void k_symmetrize_convert(int n, const double* __restrict__ G, T* __restrict__ out, int ld_out) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < (long)n * n; e += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    // column-major lower triangle from potri: element (r, c) with r >= c at c * n + r
+    const int r = i > j ? i : j, c = i > j ? j : i;
+    out[(size_t)i * ld_out + j] = T(G[(size_t)c * n + r]);
+  }
+}
+
+// Gcat = [G_hi, G_hi, G_lo] (np x 3np, bf16) from the fp64 symmetric inverse,
+// so Gcat [R_hi; R_lo; R_hi] = G_hi R_hi + G_hi R_lo + G_lo R_hi.
+__global__ void k_split_g(int n, int np, const double* __restrict__ G, __bf16* __restrict__ Gcat) {
+  const long total = (long)np * np;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(e / np), j = (int)(e % np);
+    float v = 0.f;
+    if (i < n && j < n) {
+      const int r = i > j ? i : j, c = i > j ? j : i;
+      v = float(G[(size_t)c * n + r]);
+    }
+    const __bf16 hi = (__bf16)v;
+    const __bf16 lo = (__bf16)(v - float(hi));
+    const size_t row = (size_t)i * 3 * np;  // pairs with [R_hi; R_lo; R_hi]
+    Gcat[row + j] = hi;
+    Gcat[row + np + j] = hi;
+    Gcat[row + 2 * np + j] = lo;
+  }
+}
+
+template <typename T>
+__global__ void k_gather_out(int rows, int count, int ldb, const T* __restrict__ Z, double* __restrict__ out) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < (long)rows * count; e += (long)gridDim.x * blockDim.x) {
+    const int inst = (int)(e / rows), r = (int)(e % rows);
+    out[e] = double(Z[(size_t)r * ldb + inst]);
+  }
+}
+
+// Writes the finished columns of a working set to the output store
+// (feature-major, ld = ldo, indexed by the original instance).
+template <typename T>
+__global__ __launch_bounds__(FT) void k_fam_scatter(FamArgs<T> a, int ldo, T* __restrict__ Xo, T* __restrict__ Yo,
+                                                    T* __restrict__ So, int* __restrict__ Io,
+                                                    double* __restrict__ Ro) {
+  const int lane = threadIdx.x & 63, w = wave_id();
+  const int col = blockIdx.x * LANES + lane;
+  if (col >= a.count || !a.done[col]) return;
+  const int t = a.orig[col];
+  for (int i = w; i < a.n; i += NWV) Xo[(size_t)i * ldo + t] = a.X[(size_t)i * a.ldb + col];
+  for (int j = w; j < a.m; j += NWV) {
+    Yo[(size_t)j * ldo + t] = a.Y[(size_t)j * a.ldb + col];
+    So[(size_t)j * ldo + t] = a.S[(size_t)j * a.ldb + col];
+  }
+  if (w == 0) {
+    Io[t] = a.iters[col];
+    Io[ldo + t] = a.status[col];
+    Io[2 * ldo + t] = a.fallback[col];
+    for (int k = 0; k < 8; ++k) Ro[(size_t)k * ldo + t] = a.red[(size_t)col * 8 + k];
+  }
+}
+
+// Gathers the columns `list` of working set s into the dense prefix of d
+// (d.count live columns, d.ldb - d.count padding columns left inactive).
+template <typename T>
+__global__ __launch_bounds__(FT) void k_fam_move(FamArgs<T> s, FamArgs<T> d, const int* __restrict__ list) {
+  const int lane = threadIdx.x & 63, w = wave_id();
+  const int col = blockIdx.x * LANES + lane;
+  const bool live = col < d.count;
+  const int sc = live ? list[col] : 0;
+  auto mv = [&](const T* src, T* dst, int rows) {
+    for (int r = w; r < rows; r += NWV)
+      dst[(size_t)r * d.ldb + col] = live ? src[(size_t)r * s.ldb + sc] : T(0);
+  };
+  mv(s.X, d.X, d.n);
+  mv(s.S, d.S, d.m);
+  mv(s.Y, d.Y, d.m);
+  mv(s.AX, d.AX, d.m);
+  mv(s.c, d.c, d.n);
+  if (d.b_inst) mv(s.b, d.b, d.m);
+  for (int r = d.n + w; r < d.np; r += NWV) {  // padding rows of the GEMM operands
+    d.X[(size_t)r * d.ldb + col] = T(0);
+    if (d.Rcat)
+      for (int k = 0; k < 3; ++k) d.Rcat[(size_t)(k * d.np + r) * d.ldb + col] = (__bf16)0.f;
+  }
+  if (w == 0) {
+    d.lam[col] = live ? s.lam[sc] : 1.0;
+    d.pscale[col] = live ? s.pscale[sc] : 1.0;
+    d.dscale[col] = live ? s.dscale[sc] : 1.0;
+    d.done[col] = live ? 0 : 1;
+    d.status[col] = live ? s.status[sc] : 1;
+    d.iters[col] = live ? s.iters[sc] : 0;
+    d.fallback[col] = live ? s.fallback[sc] : 0;
+    d.orig[col] = live ? s.orig[sc] : 0;
+    for (int k = 0; k < 8; ++k) d.red[(size_t)col * 8 + k] = live ? s.red[(size_t)sc * 8 + k] : -1.0;
+  }
+}
+
+__global__ void k_iota(int n, int* __restrict__ v) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) v[i] = i;
+}
+
+int pad64(int v) { return (v + 63) & ~63; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+
+struct Family {
+  Setup S;              // shared matrix, scalings, cones (b, c of the template data)
+  Params par;
+  bool fp64 = false;
+  int device = 0;
+  hipStream_t st = nullptr;
+  rocblas_handle blas = nullptr;
+  int n = 0, m = 0, np = 0;
+  // shared device data
+  void *rp = nullptr, *ri = nullptr, *rx = nullptr, *cp = nullptr, *ci = nullptr, *cx = nullptr;
+  void *soc = nullptr, *D = nullptr, *E = nullptr, *bshared = nullptr;
+  void* Ad = nullptr;      // dense scaled A (m x n row-major, bulk type) when A is dense
+  bool dense = false;
+  void* Gd = nullptr;      // fp64 G (n x n row-major, symmetric) for dgemm
+  void* Gcat = nullptr;    // mixed: bf16 [G_hi, G_lo, G_hi]
+  // per-batch device data: the loaded constants, two working sets (the
+  // second receives the compacted active columns), and the output store
+  int count = 0, ldb = 0, b_inst = 0;
+  std::vector<void*> batch_allocs;
+  struct Set {
+    void *X, *XN, *S, *Y, *AX, *W, *U, *V, *Z, *R, *Rcat, *c, *b, *lam, *ps, *ds, *ints, *red, *orig;
+  } set[2];
+  void *base_c = nullptr, *base_b = nullptr, *base_lam = nullptr, *base_ps = nullptr, *base_ds = nullptr;
+  void *Xo = nullptr, *Yo = nullptr, *So = nullptr, *Io = nullptr, *Ro = nullptr;
+  int* dlist = nullptr;
+  int compactions = 0;
+  double last_ms = 0;
+  // optional per-kernel timing: HIP events around every launch of the loop
+  bool timing = false;
+  enum { K_A = 0, K_AT, K_B, K_GEMM, K_AX, K_C, K_KINDS };
+  double kind_ms[K_KINDS] = {0};
+  long kind_calls[K_KINDS] = {0};
+  std::vector<hipEvent_t> evs;
+  std::vector<int> ev_kind;
+  void mark(int kind) {  // event before launching `kind` (kind < 0: closing event)
+    if (!timing) return;
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    (void)hipEventRecord(e, st);
+    evs.push_back(e);
+    ev_kind.push_back(kind);
+  }
+  void collect() {
+    if (!timing) return;
+    (void)hipStreamSynchronize(st);
+    for (size_t i = 0; i + 1 < evs.size(); ++i) {
+      if (ev_kind[i] < 0) continue;
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, evs[i], evs[i + 1]);
+      kind_ms[ev_kind[i]] += ms;
+      kind_calls[ev_kind[i]] += 1;
+    }
+    for (hipEvent_t e : evs) (void)hipEventDestroy(e);
+    evs.clear();
+    ev_kind.clear();
+  }
+  std::vector<void*> allocs;
+
+  ~Family() {
+    free_batch();
+    for (void* p : allocs) (void)hipFree(p);
+    if (blas) rocblas_destroy_handle(blas);
+    if (st) (void)hipStreamDestroy(st);
+  }
+  void* dalloc(size_t bytes, std::vector<void*>& list, std::string& err) {
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) { err = "hipMalloc failed"; return nullptr; }
+    (void)hipMemsetAsync(p, 0, bytes ? bytes : 1, st);
+    list.push_back(p);
+    return p;
+  }
+  template <typename V>
+  void* upload(const std::vector<V>& v, std::string& err) {
+    void* p = dalloc(v.size() * sizeof(V), allocs, err);
+    if (p) (void)hipMemcpyAsync(p, v.data(), v.size() * sizeof(V), hipMemcpyHostToDevice, st);
+    return p;
+  }
+  void* upload_vals(const std::vector<double>& v, std::string& err) {
+    if (fp64) return upload(v, err);
+    std::vector<float> f(v.begin(), v.end());
+    return upload(f, err);
+  }
+  void free_batch() {
+    for (void* p : batch_allocs) (void)hipFree(p);
+    batch_allocs.clear();
+    count = 0;
+  }
+
+  bool create(const Data* d, const Cone* k, const PdosOptions& o, std::string& err);
+  bool load(int cnt, const double* C, const double* B, std::string& err);
+  bool run(const float* warm, long wld, int judge, std::string& err);
+  bool fetch(double* X, double* Y, double* Sx, long* iters, long* status, long* fallback, double* resid,
+             std::string& err);
+  template <typename T>
+  void fill_args(FamArgs<T>& a);
+  template <typename T>
+  FamArgs<T> args(int set_index, int ld, int cnt);
+  template <typename T>
+  bool run_t(const float* warm, long wld, int judge, std::string& err);
+  template <typename T>
+  bool iterate(FamArgs<T>& a, int steps, std::string& err);
+  template <typename T>
+  bool apply_A(FamArgs<T>& a, const T* Xin, T* Z);
+  template <typename T>
+  bool apply_At(FamArgs<T>& a, const T* X1, const T* X2, const T* X3, T* Z1, T* Z2, T* Z3);
+};
+
+bool Family::create(const Data* d, const Cone* k, const PdosOptions& o, std::string& err) {
+  fp64 = o.precision == PDOS_FP64;
+  device = o.device;
+  if (!build_setup(d, k, S, err)) return false;
+  par = *d->p;
+  n = S.n;
+  m = S.m;
+  np = pad64(n);
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+    err = "device/stream setup failed";
+    return false;
+  }
+  if (rocblas_create_handle(&blas) != rocblas_status_success ||
+      rocblas_set_stream(blas, st) != rocblas_status_success) {
+    err = "rocblas handle";
+    return false;
+  }
+  rp = upload(S.rp, err); ri = upload(S.ri, err); rx = upload_vals(S.rx, err);
+  cp = upload(S.cp, err); ci = upload(S.ci, err); cx = upload_vals(S.cx, err);
+  soc = upload(S.soc_ptr, err); D = upload(S.D, err); E = upload(S.E, err);
+  bshared = upload_vals(S.b, err);
+  if (!err.empty()) return false;
+  // M = I + A'A of the scaled A on the device (rocBLAS syrk), then
+  // G = M^-1 by Cholesky (rocSOLVER potrf + potri), once per family.
+  std::vector<double> Acol((size_t)m * n, 0.0);  // column-major m x n
+  for (int j = 0; j < n; ++j)
+    for (int p = S.cp[j]; p < S.cp[j + 1]; ++p) Acol[(size_t)j * m + S.ci[p]] += S.cx[p];
+  std::vector<double> M((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) M[(size_t)i * n + i] = 1.0;
+  double* dA = static_cast<double*>(dalloc(Acol.size() * 8, allocs, err));
+  double* dM = static_cast<double*>(dalloc(M.size() * 8, allocs, err));
+  rocblas_int* dinfo = static_cast<rocblas_int*>(dalloc(sizeof(rocblas_int), allocs, err));
+  if (!dA || !dM || !dinfo) return false;
+  (void)hipMemcpyAsync(dA, Acol.data(), Acol.size() * 8, hipMemcpyHostToDevice, st);
+  (void)hipMemcpyAsync(dM, M.data(), M.size() * 8, hipMemcpyHostToDevice, st);
+  const double one = 1.0;
+  if (rocblas_dsyrk(blas, rocblas_fill_lower, rocblas_operation_transpose, n, m, &one, dA, m, &one, dM, n) !=
+          rocblas_status_success ||
+      rocsolver_dpotrf(blas, rocblas_fill_lower, n, dM, n, dinfo) != rocblas_status_success ||
+      rocsolver_dpotri(blas, rocblas_fill_lower, n, dM, n, dinfo) != rocblas_status_success) {
+    err = "rocblas syrk / rocsolver potrf, potri failed";
+    return false;
+  }
+  // dense A: the products run as GEMMs on a row-major copy (column-major n x m)
+  dense = S.nnz() > (long)m * n / 4;
+  if (dense) {
+    std::vector<double> Arow((size_t)m * n, 0.0);
+    for (int r = 0; r < m; ++r)
+      for (int p = S.rp[r]; p < S.rp[r + 1]; ++p) Arow[(size_t)r * n + S.ri[p]] += S.rx[p];
+    Ad = upload_vals(Arow, err);
+    if (!Ad) return false;
+  }
+  rocblas_int info = 0;
+  (void)hipMemcpyAsync(&info, dinfo, sizeof(info), hipMemcpyDeviceToHost, st);
+  (void)hipStreamSynchronize(st);
+  if (info != 0) { err = "I + A'A is not positive definite"; return false; }
+  if (fp64) {
+    Gd = dalloc((size_t)n * n * 8, allocs, err);
+    if (!Gd) return false;
+    hipLaunchKernelGGL(k_symmetrize_convert<double>, dim3(1024), dim3(256), 0, st, n, dM,
+                       static_cast<double*>(Gd), n);
+  } else {
+    Gcat = dalloc((size_t)np * 3 * np * 2, allocs, err);
+    if (!Gcat) return false;
+    hipLaunchKernelGGL(k_split_g, dim3(1024), dim3(256), 0, st, n, np, dM, static_cast<__bf16*>(Gcat));
+  }
+  return hipStreamSynchronize(st) == hipSuccess;
+}
+
+// Per-instance c (count x n) and optional b (count x m), original units:
+// scaled exactly as build_setup does (common.h:140-192) and uploaded
+// feature-major together with lambda, pscale, dscale.
+bool Family::load(int cnt, const double* C, const double* B, std::string& err) {
+  free_batch();
+  count = cnt;
+  ldb = pad64(cnt);
+  b_inst = B != nullptr;
+  const size_t tsz = fp64 ? 8 : 4;
+  std::vector<double> lam(ldb, 1.0), ps(ldb, 1.0), ds(ldb, 1.0);
+  std::vector<double> cf((size_t)n * ldb, 0.0), bf(b_inst ? (size_t)m * ldb : 0, 0.0);
+  const double Esc = 1e3;
+  for (int t = 0; t < cnt; ++t) {
+    const double* c = C + (size_t)t * n;
+    const double* b = B ? B + (size_t)t * m : nullptr;
+    double nb = 0, nc = 0, nbs = 0, ncs = 0;
+    for (int j = 0; j < m; ++j) {
+      const double bj = b ? b[j] : 0.0;
+      nb += bj * bj;
+      const double bs = b ? bj * S.D[j] : S.b[j];
+      nbs += bs * bs;
+      if (b_inst) bf[(size_t)j * ldb + t] = bs;
+    }
+    if (!b) nb = (S.pscale - 1.0) * (S.pscale - 1.0);
+    for (int i = 0; i < n; ++i) {
+      nc += c[i] * c[i];
+      const double cs = c[i] * S.E0[i];  // E0: the column scaling before the x1e3
+      ncs += cs * cs;
+      cf[(size_t)i * ldb + t] = cs * Esc;
+    }
+    ps[t] = 1.0 + std::sqrt(nb);
+    ds[t] = 1.0 + std::sqrt(nc);
+    lam[t] = (1e-15 + std::sqrt(nbs)) / (1e-15 + std::sqrt(ncs));
+  }
+  auto up = [&](const std::vector<double>& v, bool as_t) -> void* {
+    if (!as_t || fp64) {
+      void* p = dalloc(v.size() * 8, batch_allocs, err);
+      if (p) (void)hipMemcpyAsync(p, v.data(), v.size() * 8, hipMemcpyHostToDevice, st);
+      (void)hipStreamSynchronize(st);
+      return p;
+    }
+    std::vector<float> f(v.begin(), v.end());
+    void* p = dalloc(f.size() * 4, batch_allocs, err);
+    if (p) (void)hipMemcpyAsync(p, f.data(), f.size() * 4, hipMemcpyHostToDevice, st);
+    (void)hipStreamSynchronize(st);
+    return p;
+  };
+  base_c = up(cf, true);
+  base_b = b_inst ? up(bf, true) : nullptr;
+  base_lam = up(lam, false);
+  base_ps = up(ps, false);
+  base_ds = up(ds, false);
+  auto vec = [&](int rows) { return dalloc((size_t)rows * ldb * tsz, batch_allocs, err); };
+  for (int k = 0; k < 2; ++k) {
+    Set& q = set[k];
+    q.X = vec(np); q.XN = vec(np); q.S = vec(m); q.Y = vec(m); q.AX = vec(m); q.W = vec(m);
+    q.U = vec(n); q.V = vec(n); q.Z = vec(n); q.R = vec(np);
+    q.Rcat = fp64 ? nullptr : dalloc((size_t)3 * np * ldb * 2, batch_allocs, err);
+    q.c = vec(n);
+    q.b = b_inst ? vec(m) : nullptr;
+    q.lam = dalloc((size_t)ldb * 8, batch_allocs, err);
+    q.ps = dalloc((size_t)ldb * 8, batch_allocs, err);
+    q.ds = dalloc((size_t)ldb * 8, batch_allocs, err);
+    q.ints = dalloc((size_t)4 * ldb * 4, batch_allocs, err);
+    q.red = dalloc((size_t)8 * ldb * 8, batch_allocs, err);
+    q.orig = dalloc((size_t)ldb * 4, batch_allocs, err);
+  }
+  Xo = vec(n); Yo = vec(m); So = vec(m);
+  Io = dalloc((size_t)3 * ldb * 4, batch_allocs, err);
+  Ro = dalloc((size_t)8 * ldb * 8, batch_allocs, err);
+  dlist = static_cast<int*>(dalloc((size_t)ldb * 4, batch_allocs, err));
+  if (!err.empty()) return false;
+  return hipStreamSynchronize(st) == hipSuccess;
+}
+
+template <typename T>
+FamArgs<T> Family::args(int k, int ld, int cnt) {
+  FamArgs<T> a;
+  fill_args(a);
+  const Set& q = set[k];
+  a.ldb = ld;
+  a.count = cnt;
+  a.b = b_inst ? static_cast<T*>(q.b) : static_cast<T*>(bshared);
+  a.c = static_cast<T*>(q.c);
+  a.lam = static_cast<double*>(q.lam);
+  a.pscale = static_cast<double*>(q.ps);
+  a.dscale = static_cast<double*>(q.ds);
+  a.X = static_cast<T*>(q.X); a.XN = static_cast<T*>(q.XN); a.S = static_cast<T*>(q.S);
+  a.Y = static_cast<T*>(q.Y); a.AX = static_cast<T*>(q.AX); a.W = static_cast<T*>(q.W);
+  a.U = static_cast<T*>(q.U); a.V = static_cast<T*>(q.V); a.Z = static_cast<T*>(q.Z);
+  a.R = static_cast<T*>(q.R);
+  a.Rcat = static_cast<__bf16*>(q.Rcat);
+  int* ip = static_cast<int*>(q.ints);
+  a.done = ip; a.status = ip + ld; a.iters = ip + 2 * ld; a.fallback = ip + 3 * ld;
+  a.red = static_cast<double*>(q.red);
+  a.orig = static_cast<int*>(q.orig);
+  return a;
+}
+
+template <typename T>
+void Family::fill_args(FamArgs<T>& a) {
+  std::memset(&a, 0, sizeof(a));
+  a.n = n; a.m = m; a.f = S.f; a.lend = S.f + S.l; a.nsoc = S.nsoc();
+  a.ldb = ldb; a.count = count; a.np = np; a.b_inst = b_inst;
+  a.max_iters = (int)std::min<long>(std::max<long>(par.MAX_ITERS, 0), 0x7fffffff);
+  a.eps = par.EPS_ABS; a.relax = par.ALPHA;
+  a.soc_ptr = static_cast<const int*>(soc);
+  a.rp = static_cast<const int*>(rp); a.ri = static_cast<const int*>(ri); a.rx = static_cast<const T*>(rx);
+  a.cp = static_cast<const int*>(cp); a.ci = static_cast<const int*>(ci); a.cx = static_cast<const T*>(cx);
+  a.D = static_cast<const double*>(D); a.E = static_cast<const double*>(E);
+}
+
+template <typename T>
+bool Family::iterate(FamArgs<T>& a, int steps, std::string& err) {
+  const dim3 grid(a.ldb / LANES), block(FT);
+  for (int s = 0; s < steps; ++s) {
+    mark(K_A);
+    hipLaunchKernelGGL(k_fam_a<T>, grid, block, 0, st, a);
+    mark(K_AT);
+    if (!apply_At(a, a.Y, a.W, fp64 ? nullptr : a.AX, a.U, a.V, a.Z)) { err = "A' product failed"; return false; }
+    mark(K_B);
+    if (fp64) {
+      hipLaunchKernelGGL((k_fam_b<T, false>), grid, block, 0, st, a);
+      mark(K_GEMM);
+      const double one = 1.0, zero = 0.0;
+      // row-major XN (n x ldb) = G (n x n) R (n x ldb): column-major XN' = R' G
+      if (rocblas_dgemm(blas, rocblas_operation_none, rocblas_operation_none, a.ldb, n, n, &one,
+                        reinterpret_cast<const double*>(a.R), a.ldb, static_cast<const double*>(Gd), n, &zero,
+                        reinterpret_cast<double*>(a.XN), a.ldb) != rocblas_status_success) {
+        err = "rocblas_dgemm failed";
+        return false;
+      }
+    } else {
+      hipLaunchKernelGGL((k_fam_b<T, true>), grid, block, 0, st, a);
+      mark(K_GEMM);
+      if (bf16_gemm_f32(st, np, a.ldb, 3 * np, static_cast<const __bf16*>(Gcat), 3 * np, a.Rcat, a.ldb,
+                        reinterpret_cast<float*>(a.XN), a.ldb, reinterpret_cast<const float*>(a.X)) != hipSuccess) {
+        err = "bf16 gemm launch failed";
+        return false;
+      }
+    }
+    mark(K_AX);
+    if (!apply_A(a, a.XN, a.AX)) { err = "A product failed"; return false; }
+    mark(K_C);
+    hipLaunchKernelGGL(k_fam_c<T>, grid, block, 0, st, a);
+    mark(-1);
+  }
+  return hipGetLastError() == hipSuccess;
+}
+
+// Z = A X (m x ldb) for the batch: sparse rows or a dense GEMM.
+template <typename T>
+bool Family::apply_A(FamArgs<T>& a, const T* Xin, T* Z) {
+  if (!dense) {
+    hipLaunchKernelGGL((k_fam_spmm<T, 1>), dim3(a.ldb / LANES), dim3(FT), 0, st, m, a.rp, a.ri, a.rx, Xin, Xin,
+                       Xin, Z, Z, Z, a.ldb);
+    return hipGetLastError() == hipSuccess;
+  }
+  const T one = T(1), zero = T(0);
+  // column-major Z' (ldb x m) = X' (ldb x n) * A^T, A^T stored column-major n x m (ld n)
+  if constexpr (std::is_same<T, double>::value)
+    return rocblas_dgemm(blas, rocblas_operation_none, rocblas_operation_none, a.ldb, m, n, &one, Xin, a.ldb,
+                         static_cast<const double*>(Ad), n, &zero, Z, a.ldb) == rocblas_status_success;
+  else
+    return rocblas_sgemm(blas, rocblas_operation_none, rocblas_operation_none, a.ldb, m, n, &one, Xin, a.ldb,
+                         static_cast<const float*>(Ad), n, &zero, Z, a.ldb) == rocblas_status_success;
+}
+
+// Z1 = A' X1, Z2 = A' X2 and, if X3, Z3 = A' X3 (n x ldb): one sparse pass
+// or one dense GEMM per right-hand side.
+template <typename T>
+bool Family::apply_At(FamArgs<T>& a, const T* X1, const T* X2, const T* X3, T* Z1, T* Z2, T* Z3) {
+  if (!dense) {
+    if (X3)
+      hipLaunchKernelGGL((k_fam_spmm<T, 3>), dim3(a.ldb / LANES), dim3(FT), 0, st, n, a.cp, a.ci, a.cx, X1, X2,
+                         X3, Z1, Z2, Z3, a.ldb);
+    else
+      hipLaunchKernelGGL((k_fam_spmm<T, 2>), dim3(a.ldb / LANES), dim3(FT), 0, st, n, a.cp, a.ci, a.cx, X1, X2,
+                         X2, Z1, Z2, Z2, a.ldb);
+    return hipGetLastError() == hipSuccess;
+  }
+  const T one = T(1), zero = T(0);
+  // column-major Z' (ldb x n) = X' (ldb x m) * A, A = (stored n x m)^T
+  for (int k = 0; k < (X3 ? 3 : 2); ++k) {
+    const T* Xk = k == 0 ? X1 : (k == 1 ? X2 : X3);
+    T* Zk = k == 0 ? Z1 : (k == 1 ? Z2 : Z3);
+    rocblas_status r;
+    if constexpr (std::is_same<T, double>::value)
+      r = rocblas_dgemm(blas, rocblas_operation_none, rocblas_operation_transpose, a.ldb, n, m, &one, Xk, a.ldb,
+                        static_cast<const double*>(Ad), n, &zero, Zk, a.ldb);
+    else
+      r = rocblas_sgemm(blas, rocblas_operation_none, rocblas_operation_transpose, a.ldb, n, m, &one, Xk, a.ldb,
+                        static_cast<const float*>(Ad), n, &zero, Zk, a.ldb);
+    if (r != rocblas_status_success) return false;
+  }
+  return true;
+}
+
+template <typename T>
+bool Family::run_t(const float* warm, long wld, int judge, std::string& err) {
+  const size_t tsz = sizeof(T);
+  int cur = 0, ld = ldb, cnt = count;
+  FamArgs<T> a = args<T>(0, ld, cnt);
+  // constants of the loaded batch into working set 0, identity column map
+  (void)hipMemcpyAsync(a.c, base_c, (size_t)n * ld * tsz, hipMemcpyDeviceToDevice, st);
+  if (b_inst) (void)hipMemcpyAsync(a.b, base_b, (size_t)m * ld * tsz, hipMemcpyDeviceToDevice, st);
+  (void)hipMemcpyAsync(a.lam, base_lam, (size_t)ld * 8, hipMemcpyDeviceToDevice, st);
+  (void)hipMemcpyAsync(a.pscale, base_ps, (size_t)ld * 8, hipMemcpyDeviceToDevice, st);
+  (void)hipMemcpyAsync(a.dscale, base_ds, (size_t)ld * 8, hipMemcpyDeviceToDevice, st);
+  hipLaunchKernelGGL(k_iota, dim3(64), dim3(256), 0, st, ld, a.orig);
+  const dim3 block(FT);
+  hipLaunchKernelGGL(k_fam_init<T>, dim3(ld / LANES), block, 0, st, a, warm, wld);
+  if (!apply_A(a, a.X, a.AX)) { err = "A product failed"; return false; }
+  if (warm && judge) {
+    a.judge = 1;
+    hipLaunchKernelGGL(k_fam_a<T>, dim3(ld / LANES), block, 0, st, a);
+    if (!apply_At(a, a.Y, a.W, (const T*)nullptr, a.U, a.V, a.Z)) { err = "A' product failed"; return false; }
+    if (fp64) hipLaunchKernelGGL((k_fam_b<T, false>), dim3(ld / LANES), block, 0, st, a);
+    else hipLaunchKernelGGL((k_fam_b<T, true>), dim3(ld / LANES), block, 0, st, a);
+    a.judge = 0;
+  }
+  std::vector<int> hdone(ldb), list;
+  int chunk = 8;
+  compactions = 0;
+  for (long total = 0; total <= (long)a.max_iters + 1;) {
+    const int steps = (int)std::min<long>(chunk, a.max_iters + 2 - total);
+    if (!iterate(a, steps, err)) return false;
+    total += steps;
+    (void)hipMemcpyAsync(hdone.data(), a.done, (size_t)cnt * 4, hipMemcpyDeviceToHost, st);
+    if (hipStreamSynchronize(st) != hipSuccess) { err = "family engine failed"; return false; }
+    collect();
+    list.clear();
+    for (int t = 0; t < cnt; ++t)
+      if (!hdone[t]) list.push_back(t);
+    const int active = (int)list.size();
+    if (active == 0) break;
+    chunk = std::min(chunk * 2, 32);
+    if (active <= (cnt * 5) / 8 && ld > LANES) {
+      // compaction: finished columns to the output store, live ones to the other set
+      hipLaunchKernelGGL(k_fam_scatter<T>, dim3(ld / LANES), block, 0, st, a, ldb, static_cast<T*>(Xo),
+                         static_cast<T*>(Yo), static_cast<T*>(So), static_cast<int*>(Io), static_cast<double*>(Ro));
+      (void)hipMemcpyAsync(dlist, list.data(), (size_t)active * 4, hipMemcpyHostToDevice, st);
+      const int nld = pad64(active);
+      FamArgs<T> b = args<T>(cur ^ 1, nld, active);
+      hipLaunchKernelGGL(k_fam_move<T>, dim3(nld / LANES), block, 0, st, a, b, dlist);
+      (void)hipStreamSynchronize(st);  // dlist / hdone are reused
+      cur ^= 1;
+      ld = nld;
+      cnt = active;
+      a = b;
+      ++compactions;
+    }
+  }
+  hipLaunchKernelGGL(k_fam_scatter<T>, dim3(ld / LANES), block, 0, st, a, ldb, static_cast<T*>(Xo),
+                     static_cast<T*>(Yo), static_cast<T*>(So), static_cast<int*>(Io), static_cast<double*>(Ro));
+  return hipGetLastError() == hipSuccess;
+}
+
+bool Family::run(const float* warm, long wld, int judge, std::string& err) {
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  (void)hipEventRecord(e0, st);
+  const bool ok = fp64 ? run_t<double>(warm, wld, judge, err) : run_t<float>(warm, wld, judge, err);
+  (void)hipEventRecord(e1, st);
+  (void)hipEventSynchronize(e1);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  last_ms = ms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ok;
+}
+
+bool Family::fetch(double* X, double* Y, double* Sx, long* iters, long* status, long* fallback,
+                   double* resid, std::string& err) {
+  const int cnt = count;
+  std::vector<int> ip((size_t)3 * ldb);
+  std::vector<double> red((size_t)8 * ldb);
+  std::vector<double> hx((size_t)n * cnt), hy((size_t)m * cnt), hs((size_t)m * cnt);
+  auto gather = [&](const void* Zp, int rows, double* out) {
+    double* dout = static_cast<double*>(dalloc((size_t)rows * cnt * 8, batch_allocs, err));
+    if (!dout) return;
+    if (fp64) hipLaunchKernelGGL(k_gather_out<double>, dim3(1024), dim3(256), 0, st, rows, cnt, ldb,
+                                 static_cast<const double*>(Zp), dout);
+    else hipLaunchKernelGGL(k_gather_out<float>, dim3(1024), dim3(256), 0, st, rows, cnt, ldb,
+                            static_cast<const float*>(Zp), dout);
+    (void)hipMemcpyAsync(out, dout, (size_t)rows * cnt * 8, hipMemcpyDeviceToHost, st);
+  };
+  gather(Xo, n, hx.data());
+  gather(Yo, m, hy.data());
+  gather(So, m, hs.data());
+  (void)hipMemcpyAsync(ip.data(), Io, (size_t)3 * ldb * 4, hipMemcpyDeviceToHost, st);
+  (void)hipMemcpyAsync(red.data(), Ro, (size_t)8 * ldb * 8, hipMemcpyDeviceToHost, st);
+  if (hipStreamSynchronize(st) != hipSuccess || !err.empty()) {
+    if (err.empty()) err = "fetch failed";
+    return false;
+  }
+  // unscale (pdos.c:251-294): x = E x, y = D y, s = s / D
+  for (int t = 0; t < cnt; ++t) {
+    if (X)
+      for (int i = 0; i < n; ++i) X[(size_t)t * n + i] = S.E[i] * hx[(size_t)t * n + i];
+    if (Y)
+      for (int j = 0; j < m; ++j) Y[(size_t)t * m + j] = S.D[j] * hy[(size_t)t * m + j];
+    if (Sx)
+      for (int j = 0; j < m; ++j) Sx[(size_t)t * m + j] = hs[(size_t)t * m + j] / S.D[j];
+    if (iters) iters[t] = ip[t];
+    if (status) status[t] = ip[ldb + t];
+    if (fallback) fallback[t] = ip[2 * ldb + t];
+    if (resid)
+      for (int q = 0; q < 5; ++q) resid[(size_t)t * 5 + q] = red[(size_t)(2 + q) * ldb + t];
+  }
+  // the temporary gather buffers are released with the next load
+  return true;
+}
+
+}  // namespace pdos_amd
+
+struct PdosFamily {
+  pdos_amd::Family F;
+};
+
+namespace {
+thread_local std::string g_fam_err;
+}
+
+extern "C" {
+
+const char* pdos_family_last_error(void) { return g_fam_err.c_str(); }
+
+PdosFamily* pdos_family_create(const Data* d, const Cone* k, const PdosOptions* opt) {
+  PdosOptions o;
+  if (opt) o = *opt;
+  else pdos_default_options(&o);
+  PdosFamily* h = new PdosFamily;
+  std::string err;
+  if (!h->F.create(d, k, o, err)) {
+    g_fam_err = err;
+    delete h;
+    return nullptr;
+  }
+  return h;
+}
+
+int pdos_family_load(PdosFamily* h, int count, const double* C, const double* B) {
+  std::string err;
+  if (!h || count < 1 || !C) { g_fam_err = "bad arguments"; return -1; }
+  if (!h->F.load(count, C, B, err)) { g_fam_err = err; return -1; }
+  return 0;
+}
+
+int pdos_family_run(PdosFamily* h, const float* warm_dev, long ld, int judge, double* kernel_ms) {
+  std::string err;
+  if (!h || h->F.count < 1) { g_fam_err = "nothing loaded"; return -1; }
+  if (!h->F.run(warm_dev, ld, judge, err)) { g_fam_err = err; return -1; }
+  if (kernel_ms) *kernel_ms = h->F.last_ms;
+  return 0;
+}
+
+int pdos_family_fetch(PdosFamily* h, double* X, double* Y, double* S, long* iters, long* status,
+                      long* fallback, double* resid) {
+  std::string err;
+  if (!h) return -1;
+  if (!h->F.fetch(X, Y, S, iters, status, fallback, resid, err)) { g_fam_err = err; return -1; }
+  return 0;
+}
+
+void pdos_family_free(PdosFamily* h) { delete h; }
+
+int pdos_family_timing(PdosFamily* h, int enable, double* ms, long* calls) {
+  if (!h) return -1;
+  pdos_amd::Family& F = h->F;
+  for (int k = 0; k < pdos_amd::Family::K_KINDS; ++k) {
+    if (ms) ms[k] = F.kind_ms[k];
+    if (calls) calls[k] = F.kind_calls[k];
+    F.kind_ms[k] = 0;
+    F.kind_calls[k] = 0;
+  }
+  F.timing = enable != 0;
+  return 0;
+}
+
+}  // extern "C"

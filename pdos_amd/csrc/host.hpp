@@ -20,6 +20,7 @@ struct Setup {
   std::vector<int> rp, ri;
   std::vector<double> rx;
   std::vector<double> b, c, D, E;  // scaled b, c; D[m], E[n] (E includes x1e3)
+  std::vector<double> E0;          // E before the x1e3 (1 without NORMALIZE)
   double lambda = 0, pscale = 0, dscale = 0;
   std::vector<double> x, s, y;     // scaled warm start (zeros if absent)
   int nsoc() const { return (int)soc_ptr.size() - 1; }

@@ -104,6 +104,7 @@ bool build_setup(const Data* d, const Cone* k, Setup& S, std::string& err) {
     std::fill(S.D.begin(), S.D.end(), 1.0);
     std::fill(S.E.begin(), S.E.end(), 1.0);
   }
+  S.E0 = S.E;
   S.lambda = (1e-15 + norm2(S.b.data(), m)) / (1e-15 + norm2(S.c.data(), n));
   for (long j = 0; j < n; ++j) {
     S.E[j] *= kSqrtRatio;

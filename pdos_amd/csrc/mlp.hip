@@ -40,7 +40,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int BM = 64, BN = 64, BK = 32, KP = BK + 8;  // LDS rows padded against bank conflicts
 constexpr int GT = 256;
 
-enum Epi { EPI_F32 = 0, EPI_BIAS_GELU = 1, EPI_BIAS_AFFINE = 2, EPI_GELU_GRAD = 3 };
+enum Epi { EPI_F32 = 0, EPI_BIAS_GELU = 1, EPI_BIAS_AFFINE = 2, EPI_GELU_GRAD = 3, EPI_ADD = 4 };
 
 __device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
 __device__ __forceinline__ float gelu_grad(float z) {
@@ -120,6 +120,8 @@ __global__ __launch_bounds__(GT) void k_gemm(int M, int N, int K, const __bf16* 
         float v = acc[i][j][r];
         if (EPI == EPI_F32) {
           Cf[o] = v;
+        } else if (EPI == EPI_ADD) {  // aux is an addend with the layout of C
+          Cf[o] = aux[o] + v;
         } else if (EPI == EPI_BIAS_GELU) {
           v += bias[col];
           Cf[o] = v;                 // pre-activation, kept for the backward pass
@@ -228,6 +230,16 @@ void gemm(hipStream_t st, int M, int N, int K, const __bf16* A, int lda, const _
 }
 
 }  // namespace
+
+// C (fp32, M x N) = A (bf16, M x K) B (bf16, K x N), row-major; M, N multiples
+// of 64 and K of 32; with `addend` (M x N, ldc), C = addend + A B.  Used by
+// the family engine's split-bf16 x-update.
+hipError_t bf16_gemm_f32(hipStream_t st, int M, int N, int K, const __bf16* A, int lda, const __bf16* B,
+                         int ldb, float* C, int ldc, const float* addend) {
+  if (addend) gemm<false, false, EPI_ADD>(st, M, N, K, A, lda, B, ldb, C, nullptr, ldc, nullptr, addend);
+  else gemm<false, false, EPI_F32>(st, M, N, K, A, lda, B, ldb, C, nullptr, ldc);
+  return hipGetLastError();
+}
 
 // ---------------------------------------------------------------------------
 

@@ -1,0 +1,102 @@
+"""Python handle on the shared-A family engine (pdos_family_* in include/pdos_ext.h)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import PdosOptions, lib, options
+from .abi import Problem
+
+_dp = C.POINTER(C.c_double)
+_lp = C.POINTER(C.c_long)
+
+
+def _bind(L):
+    if getattr(L, "_fam_bound", False):
+        return
+    L.pdos_family_create.restype = C.c_void_p
+    L.pdos_family_create.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(PdosOptions)]
+    L.pdos_family_load.restype = C.c_int
+    L.pdos_family_load.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+    L.pdos_family_run.restype = C.c_int
+    L.pdos_family_run.argtypes = [C.c_void_p, C.c_void_p, C.c_long, C.c_int, _dp]
+    L.pdos_family_fetch.restype = C.c_int
+    L.pdos_family_fetch.argtypes = [C.c_void_p, _dp, _dp, _dp, _lp, _lp, _lp, _dp]
+    L.pdos_family_free.argtypes = [C.c_void_p]
+    L.pdos_family_last_error.restype = C.c_char_p
+    L.pdos_family_timing.restype = C.c_int
+    L.pdos_family_timing.argtypes = [C.c_void_p, C.c_int, _dp, _lp]
+    L._fam_bound = True
+
+
+class Family:
+    """Instances sharing the template's A, cones and parameters; each has its
+    own c (and optionally b)."""
+
+    def __init__(self, template: Problem, precision: str = "fp64", device: int = 0):
+        self.L = lib("direct")
+        _bind(self.L)
+        self.n, self.m = template.n, template.m
+        d, k, self._keep = template.to_c()
+        o = options("direct", precision, "auto", device)
+        self.h = self.L.pdos_family_create(C.byref(d), C.byref(k), C.byref(o))
+        if not self.h:
+            raise RuntimeError(self.L.pdos_family_last_error().decode())
+        self.count = 0
+
+    def load(self, Cm, Bm=None) -> None:
+        Cm = np.ascontiguousarray(Cm, dtype=np.float64)
+        self._C = Cm
+        self._B = None if Bm is None else np.ascontiguousarray(Bm, dtype=np.float64)
+        self.count = Cm.shape[0]
+        rc = self.L.pdos_family_load(self.h, self.count, Cm.ctypes.data_as(_dp),
+                                     self._B.ctypes.data_as(_dp) if self._B is not None else None)
+        if rc != 0:
+            raise RuntimeError(self.L.pdos_family_last_error().decode())
+
+    def run(self, warm=None, ld: int = 0, judge: bool = True) -> float:
+        """Solves the loaded batch; warm is a DeviceArray [count x ld] of
+        (x, y, s) in original units.  Returns the device time in ms."""
+        ms = C.c_double()
+        rc = self.L.pdos_family_run(self.h, warm.ptr if warm is not None else None, ld, int(judge),
+                                    C.byref(ms))
+        if rc != 0:
+            raise RuntimeError(self.L.pdos_family_last_error().decode())
+        return ms.value
+
+    def fetch(self) -> dict:
+        c = self.count
+        X = np.zeros((c, self.n))
+        Y = np.zeros((c, self.m))
+        S = np.zeros((c, self.m))
+        it = np.zeros(c, dtype=np.int64)
+        st = np.zeros(c, dtype=np.int64)
+        fb = np.zeros(c, dtype=np.int64)
+        rs = np.zeros((c, 5))
+        rc = self.L.pdos_family_fetch(self.h, X.ctypes.data_as(_dp), Y.ctypes.data_as(_dp),
+                                      S.ctypes.data_as(_dp), it.ctypes.data_as(_lp),
+                                      st.ctypes.data_as(_lp), fb.ctypes.data_as(_lp),
+                                      rs.ctypes.data_as(_dp))
+        if rc != 0:
+            raise RuntimeError(self.L.pdos_family_last_error().decode())
+        return dict(x=X, y=Y, s=S, iters=it, status=np.where(st == 0, "Solved", "Indeterminate"),
+                    fallback=fb, resid=rs)
+
+    KINDS = ["phase_a", "At_product", "phase_b", "xupdate_gemm", "A_product", "phase_c"]
+
+    def timing(self, enable: bool) -> dict:
+        """Returns per-kernel {kind: (total_ms, calls)} since the last call and
+        switches event timing on or off for later runs."""
+        ms = np.zeros(6)
+        calls = np.zeros(6, dtype=np.int64)
+        self.L.pdos_family_timing(self.h, int(enable), ms.ctypes.data_as(_dp), calls.ctypes.data_as(_lp))
+        return {k: (float(ms[i]), int(calls[i])) for i, k in enumerate(self.KINDS)}
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.L.pdos_family_free(self.h)
+                self.h = None
+        except Exception:
+            pass

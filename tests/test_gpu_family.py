@@ -1,0 +1,150 @@
+"""Shared-A family engine against the oracle, instance by instance.
+
+FP64 mode: the x-update is G (x - lam c + A'(b - s - lam y)) with
+G = (I + A'A)^-1 -- the direct plugin's solve up to roundoff -- so status,
+iteration count and x, s, y (1e-6 relative) match the oracle's direct plugin.
+Mixed mode: every Solved instance meets the stop test in binary64.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import golden_data as G
+from oracle import oracle
+from pdos_amd import DeviceArray, problems
+from pdos_amd.family import Family
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12)
+
+
+def _fam(na=60, k=6, **kw):
+    return problems.PortfolioFamily(seed=4, na=na, k=k, EPS_ABS=1e-3, **kw)
+
+
+@pytest.mark.parametrize("normalize", [0, 1])
+def test_family_fp64_matches_oracle(normalize):
+    fam = _fam(NORMALIZE=normalize)
+    _, Cm = fam.sample(seed=7, count=40)
+    print("family fp64", normalize, flush=True)
+    F = Family(fam.instance(Cm[0]), precision="fp64")
+    F.load(Cm)
+    F.run()
+    out = F.fetch()
+    for t in range(len(Cm)):
+        ref = oracle.solve(fam.instance(Cm[t]), "direct")
+        assert out["status"][t] == ref["status"], t
+        assert out["iters"][t] == ref["iters"], (t, out["iters"][t], ref["iters"])
+        for key in ("x", "s", "y"):
+            assert _rel(out[key][t], ref[key]) <= 1e-6, (t, key)
+
+
+def test_dense_family_fp64_matches_oracle():
+    dense = problems.DenseFamily(seed=2, m=96, n=24, l=64, nsoc=4, qdim=8, EPS_ABS=1e-3)
+    B, Cm, _ = dense.sample(5, 10)
+    print("dense family", flush=True)
+    F = Family(dense.instance(B[0], Cm[0]), precision="fp64")
+    F.load(Cm, B)
+    F.run()
+    out = F.fetch()
+    for t in range(len(Cm)):
+        ref = oracle.solve(dense.instance(B[t], Cm[t]), "direct")
+        assert out["status"][t] == ref["status"]
+        assert out["iters"][t] == ref["iters"]
+        for key in ("x", "s", "y"):
+            assert _rel(out[key][t], ref[key]) <= 1e-6
+    # golden case of the same family: the reference's own output
+    exp = G.expected("dense_m96_n24", "direct")
+    gp = G.problem("dense_m96_n24")
+    F2 = Family(gp, precision="fp64")
+    F2.load(gp.c[None, :], gp.b[None, :])
+    F2.run()
+    o2 = F2.fetch()
+    assert o2["iters"][0] == exp["iters"] and o2["status"][0] == exp["status"]
+    for key in ("x", "s", "y"):
+        assert _rel(o2[key][0], exp[key]) <= 1e-6
+
+
+def _verify(fam, Cm, out, slack):
+    A = sp.csc_matrix((fam.Ax, fam.Ai, fam.Ap), shape=(fam.m, fam.n))
+    eps = fam.params["EPS_ABS"]
+    nb = np.linalg.norm(fam.b)
+    ok = 0
+    for t in range(len(Cm)):
+        if out["status"][t] != "Solved":
+            continue
+        x, s, y, c = out["x"][t], out["s"][t], out["y"][t], Cm[t]
+        pres = np.linalg.norm(A @ x + s - fam.b)
+        dres = np.linalg.norm(c + A.T @ y)
+        po, do = c @ x, -fam.b @ y
+        ok += (pres < slack * eps * (1 + nb) and dres < slack * eps * (1 + np.linalg.norm(c))
+               and abs(po - do) < slack * eps * (1 + abs(po) + abs(do)))
+    return ok
+
+
+def test_family_mixed_solutions_verify_in_fp64():
+    fam = _fam(na=200, k=15)
+    _, Cm = fam.sample(seed=8, count=300)
+    print("family mixed", flush=True)
+    F = Family(fam.instance(Cm[0]), precision="mixed")
+    F.load(Cm)
+    F.run()
+    out = F.fetch()
+    solved = int((out["status"] == "Solved").sum())
+    assert solved >= 0.8 * len(Cm)
+    assert _verify(fam, Cm, out, 1.05) == solved
+    # objectives agree with the FP64 engine within the stop tolerance
+    F64 = Family(fam.instance(Cm[0]), precision="fp64")
+    F64.load(Cm)
+    F64.run()
+    o64 = F64.fetch()
+    both = (out["status"] == "Solved") & (o64["status"] == "Solved")
+    p32 = np.einsum("ij,ij->i", Cm, out["x"])[both]
+    p64 = np.einsum("ij,ij->i", Cm, o64["x"])[both]
+    assert np.all(np.abs(p32 - p64) <= 20 * 1e-3 * (1 + np.abs(p64)))
+
+
+def test_warm_start_judge():
+    fam = _fam()
+    _, Cm = fam.sample(seed=9, count=16)
+    print("judge", flush=True)
+    F = Family(fam.instance(Cm[0]), precision="fp64")
+    F.load(Cm)
+    F.run()
+    cold = F.fetch()
+    n, m = fam.n, fam.m
+    # the converged solution as warm start: accepted, stops at once
+    W = np.concatenate([cold["x"], cold["y"], cold["s"]], axis=1).astype(np.float32)
+    dW = DeviceArray(W)
+    F.run(dW, W.shape[1], judge=True)
+    hot = F.fetch()
+    assert np.all(hot["fallback"] == 0)
+    solved = cold["status"] == "Solved"
+    assert np.all(hot["iters"][solved] <= 2)
+    # a wild warm start: rejected, identical to the cold solve
+    Wb = (1e6 * np.random.default_rng(0).standard_normal(W.shape)).astype(np.float32)
+    dWb = DeviceArray(Wb)
+    F.run(dWb, W.shape[1], judge=True)
+    bad = F.fetch()
+    assert np.all(bad["fallback"] == 1)
+    np.testing.assert_array_equal(bad["iters"], cold["iters"])
+    for key in ("x", "s", "y"):
+        np.testing.assert_array_equal(bad[key], cold[key])
+
+
+def test_max_iters_indeterminate():
+    fam = _fam(MAX_ITERS=3)
+    _, Cm = fam.sample(seed=10, count=5)
+    F = Family(fam.instance(Cm[0]), precision="fp64")
+    F.load(Cm)
+    F.run()
+    out = F.fetch()
+    for t in range(5):
+        ref = oracle.solve(fam.instance(Cm[t]), "direct")
+        assert out["status"][t] == ref["status"] == "Indeterminate"
+        assert out["iters"][t] == 3
+        for key in ("x", "s", "y"):
+            assert _rel(out[key][t], ref[key]) <= 1e-6
