@@ -1,22 +1,29 @@
 """Benchmark: SOCPs solved/sec at EPS_ABS=1e-3 on one MI355X.
 
-  python bench.py [--steps K] [--warmup W] [--workload family|uneven|random|portfolio]
+  python bench.py [--steps K] [--warmup W] [--workload family|dense|uneven|random|portfolio]
 
-A step is one pass of the hot path over one batch of synthetic input, with
-every input resident in device memory when the timed region starts (setup and
-upload happen once, before warm-up).  The step is a synchronous library call
-(the C API blocks until its HIP stream is idle), timed with the host clock;
-the dominant kernel is also timed with HIP events on its own stream.
+A step is one pass of the hot path over one batch of synthetic input with
+every input resident in device memory when the timed region starts (setup,
+factorization, training and upload happen before warm-up).  The library calls
+are synchronous (they return once their HIP stream is idle), so the host
+clock around them brackets the device work; the dominant kernel is also
+timed live with HIP events on the stream it is launched on.
 
-Workloads (seeded, see pdos_amd/problems.py):
-  family     16,384 held-out instances of the parametric portfolio family
-             (n_a=500, k=20; A and b shared; mu, gamma vary), batched
-  uneven     4,096 portfolios with n_a in [50,1000], k in [5,50], batched
+Workloads (seeded, pdos_amd/problems.py):
+  family     (default, the headline) portfolio family n_a=500, k=20: A and b
+             shared, mu and gamma vary.  65,536 training instances are solved
+             by the batched GPU solver, the MLP (501 -> 512 -> 512 -> 2545,
+             GELU, BF16 MFMA kernels) is trained on them on the GPU, and the
+             timed step is: MLP inference for 16,384 held-out instances +
+             batched warm-started ADMM with the in-solver fallback judge.
+             Also reported: the same batch cold, and the plain single-instance
+             GPU path (pdos_ex, one call per instance).
+  dense      1,024 instances sharing one dense 8,192 x 2,048 A (per-instance b, c)
+  uneven     4,096 portfolios, n_a in [50,1000], k in [5,50] (one workgroup each)
   random     one random feasible SOCP, n=100,000, m=300,000, nnz~5M (grid engine)
   portfolio  the n=101, m=212 portfolio, one instance
 
-Printed: one JSON line (metric, value, ..., roofline, cpu_baseline, plain_gpu,
-accuracy).  --gpus N > 1 exits: this build targets one GPU.
+Prints one JSON line.  --gpus N > 1 exits: this build targets one GPU.
 """
 from __future__ import annotations
 
@@ -38,86 +45,43 @@ from pdos_amd import problems  # noqa: E402
 
 METRIC = "SOCPs solved/sec at EPS_ABS=1e-3: plain GPU vs batched+learned warm start"
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
+FP64_PEAK_TFLOPS = 78.6
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-# ----------------------------------------------------------------- workloads
-
-def make_workload(name, count):
-    if name == "family":
-        fam = problems.PortfolioFamily(seed=4, na=500, k=20, EPS_ABS=1e-3, MAX_ITERS=2000)
-        _, Cm = fam.sample(seed=1004, count=count or 16384)
-        probs = [fam.instance(c) for c in Cm]
-        return probs, {"workload": "portfolio family, held-out batch", "n_a": 500, "k": 20,
-                       "n": fam.n, "m": fam.m, "nnz": int(fam.Ap[-1]), "count": len(probs),
-                       "eps_abs": 1e-3}
-    if name == "uneven":
-        probs = problems.uneven_portfolios(seed=3, count=count or 4096, EPS_ABS=1e-3,
-                                           MAX_ITERS=2000)
-        return probs, {"workload": "uneven portfolio batch", "count": len(probs),
-                       "n_a": [50, 1000], "k": [5, 50], "eps_abs": 1e-3,
-                       "nnz_total": int(sum(p.nnz for p in probs))}
-    if name == "random":
-        p = problems.random_feasible(seed=1, EPS_ABS=1e-3, MAX_ITERS=2000)
-        return [p], {"workload": "random feasible SOCP", "n": p.n, "m": p.m, "nnz": p.nnz,
-                     "count": 1, "eps_abs": 1e-3}
-    if name == "portfolio":
-        p = problems.portfolio(seed=0, EPS_ABS=1e-3)
-        return [p], {"workload": "portfolio n=101 m=212", "n": p.n, "m": p.m, "nnz": p.nnz,
-                     "count": 1, "eps_abs": 1e-3}
-    raise ValueError(name)
-
-
 # ---------------------------------------------------------- FP64 verification
 
-def verify_fp64(probs, outs):
-    """Recomputes the stop test of every returned solution in binary64 on the
-    host (pdos.c:95-107 in original units); returns (verified, solved)."""
-    ok = solved = 0
+def verify_batch(A, B, Cm, X, Y, S, status, eps):
+    """Recomputes the stop test (pdos.c:95-107, original units) of every
+    Solved instance in binary64 on the host.  A is shared (sparse or dense),
+    B is m-vector (shared) or count x m.  Returns the number that pass."""
+    Bm = np.broadcast_to(B, (X.shape[0], A.shape[0])) if B.ndim == 1 else B
+    P = (A @ X.T).T + S - Bm
+    Dd = (A.T @ Y.T).T + Cm
+    pres = np.linalg.norm(P, axis=1)
+    dres = np.linalg.norm(Dd, axis=1)
+    po = np.einsum("ij,ij->i", Cm, X)
+    do = -np.einsum("ij,ij->i", Bm, Y)
+    slack = 1.001
+    ok = ((pres < slack * eps * (1 + np.linalg.norm(Bm, axis=1))) &
+          (dres < slack * eps * (1 + np.linalg.norm(Cm, axis=1))) &
+          (np.abs(po - do) < slack * eps * (1 + np.abs(po) + np.abs(do))) & (status == "Solved"))
+    return int(ok.sum())
+
+
+def verify_list(probs, outs):
+    ok = 0
     for p, o in zip(probs, outs):
         if o["status"] != "Solved":
             continue
-        solved += 1
         A = sp.csc_matrix((p.Ax, p.Ai, p.Ap), shape=(p.m, p.n))
-        eps = p.params["EPS_ABS"]
-        pres = np.linalg.norm(A @ o["x"] + o["s"] - p.b)
-        dres = np.linalg.norm(p.c + A.T @ o["y"])
-        pobj, dobj = p.c @ o["x"], -p.b @ o["y"]
-        slack = 1.001
-        if (pres < slack * eps * (1 + np.linalg.norm(p.b)) and
-                dres < slack * eps * (1 + np.linalg.norm(p.c)) and
-                abs(pobj - dobj) < slack * eps * (1 + abs(pobj) + abs(dobj))):
-            ok += 1
-    return ok, solved
-
-
-# ------------------------------------------------------------------ roofline
-
-def algorithmic_bytes(probs, outs, plugin, sv):
-    """SpMV bytes the launch executed (B_A / B_At per SpMV with s_v = value
-    size, s_i = 4), plus the direct plugin's two triangular sweeps per
-    iteration.  Counts: initial A x, then per update k+1 SpMVs with A and k+2
-    with A' (indirect) or 1 and 1 (direct), plus the stop check's A' SpMV."""
-    si = 4
-    tot = 0.0
-    for p, o in zip(probs, outs):
-        it, cg = o["info"]["iters"], o["info"]["cg_iters"]
-        solved = 1 if o["status"] == "Solved" else 0
-        nnz, n, m = p.nnz, p.n, p.m
-        BA = nnz * (sv + si) + (m + 1) * si + n * sv + m * sv
-        BAt = nnz * (sv + si) + (n + 1) * si + m * sv + n * sv
-        if plugin == "indirect":
-            NA, NAt = 1 + it + cg, 2 * it + cg + solved
-            tot += NA * BA + NAt * BAt
-        else:
-            NA, NAt = 1 + it, it + solved
-            N, lnz = n + m, o["info"]["lnz"]
-            BL = lnz * (8 + si) + (N + 1) * si + 2 * N * 8
-            tot += NA * BA + NAt * BAt + it * 2 * BL
-    return tot
+        ok += verify_batch(A, p.b, p.c[None, :], o["x"][None, :], o["y"][None, :], o["s"][None, :],
+                           np.array([o["status"]]), p.params["EPS_ABS"])
+    return ok
 
 
 # -------------------------------------------------------------- CPU baseline
@@ -136,7 +100,6 @@ def _cpu_worker(args):
         solve = lambda d, k: L.freeSol(C.byref(L.pdos(C.byref(d), C.byref(k))))  # noqa: E731
     else:
         from oracle import oracle as O
-        O.lib()
         solve = lambda d, k: O.lib().oracle_free_sol(  # noqa: E731
             O.lib().oracle_pdos(C.byref(d), C.byref(k), int(plugin == "direct"), None, None))
     probs = [Problem(**b) for b in blobs]
@@ -147,27 +110,269 @@ def _cpu_worker(args):
     return time.perf_counter() - t0
 
 
-def cpu_baseline(probs, plugin, sample, procs):
-    """Times the reference C library (oracle/_ref, built from the reference
-    sources) -- or the oracle port if it is absent -- on `sample` instances,
-    split over `procs` worker processes, one per core."""
+def cpu_baseline(probs, plugin, procs):
+    """The reference C library (oracle/_ref, built from the reference sources)
+    -- or the oracle port if absent -- on the given instances, split over
+    `procs` worker processes (one per core), factorization included."""
     kind = "reference" if os.path.exists(os.path.join(ROOT, "oracle", "_ref", "libpdosdir_ref.so")) else "port"
-    sel = probs[:sample]
     blobs = [dict(n=p.n, m=p.m, Ap=p.Ap, Ai=p.Ai, Ax=p.Ax, b=p.b, c=p.c, f=p.f, l=p.l, q=p.q,
-                  params=dict(p.params, VERBOSE=0)) for p in sel]
+                  params=dict(p.params, VERBOSE=0)) for p in probs]
     shards = [blobs[i::procs] for i in range(procs)]
     ctx = mp.get_context("fork")
-    t0 = time.perf_counter()
     with ctx.Pool(procs) as pool:
+        pool.map(_cpu_worker, [(kind, plugin, s[:1]) for s in shards if s])  # warm the workers
+        t0 = time.perf_counter()
         pool.map(_cpu_worker, [(kind, plugin, s) for s in shards if s])
-    wall = time.perf_counter() - t0
-    return {"value": len(sel) / wall, "unit": "SOCPs/s", "cores": procs, "kind": kind,
+        wall = time.perf_counter() - t0
+    return {"value": round(len(probs) / wall, 3), "unit": "SOCPs/s", "cores": procs, "kind": kind,
             "plugin": plugin,
-            "sample": f"first {len(sel)} instances of the same batch, {procs} processes, "
-                      f"{wall:.1f} s wall (factorization included)"}
+            "sample": f"{len(probs)} instances of the same workload, {procs} processes, {wall:.1f} s wall, "
+                      "factorization included, cold start"}
 
 
-# --------------------------------------------------------------------- main
+def plain_gpu(probs, plugin, precision):
+    """The plain single-instance GPU path: one pdos_ex call per instance
+    (host setup, upload, solve, download), cold start."""
+    pdos_amd.solve_problem(probs[0], plugin=plugin, precision=precision)
+    t0 = time.perf_counter()
+    ns = sum(pdos_amd.solve_problem(p, plugin=plugin, precision=precision)["status"] == "Solved"
+             for p in probs)
+    dt = time.perf_counter() - t0
+    return {"value": round(ns / dt, 3), "unit": "SOCPs/s", "plugin": plugin, "precision": precision,
+            "sample": f"{len(probs)} instances, one pdos_ex call each (setup, upload, solve, download)"}
+
+
+# ------------------------------------------------------------------ roofline
+
+def family_roofline(kt, fam_info, nnz, fp64):
+    """Roofline of the dominant kernel of a family run, from per-kernel HIP
+    event totals {kind: (ms, calls, columns)}.  Algorithmic work per column
+    (instance): A' product  nR right-hand sides read (m rows) and written
+    (n rows) + the shared matrix once per launch; A product  n read, m
+    written; x-update GEMM  2 * np * 3np flops (split-bf16) or 2 n^2 (fp64)."""
+    n, m, npad = fam_info["n"], fam_info["m"], fam_info["np"]
+    sv = 8 if fp64 else 4
+    kind = max(kt, key=lambda k: kt[k][0])
+    ms, calls, cols = kt[kind]
+    if kind == "xupdate_gemm":
+        flops = (2.0 * n * n if fp64 else 2.0 * npad * fam_info.get("products", 3) * npad) * cols
+        ach = flops / (ms * 1e-3) / 1e12
+        peak = FP64_PEAK_TFLOPS if fp64 else BF16_PEAK_TFLOPS
+        return {"bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(ach / peak, 4), "traffic": None, "kernel": kind,
+                "algorithmic_flops_per_launch": flops / max(calls, 1), "avg_launch_us": round(1e3 * ms / max(calls, 1), 2)}
+    if kind == "At_product":
+        nr = 2 if fp64 else 3
+        byt = cols * nr * (m + n) * sv + calls * nnz * (sv + 4)
+    elif kind == "A_product":
+        byt = cols * (m + n) * sv + calls * nnz * (sv + 4)
+    elif kind == "phase_a":
+        byt = cols * m * sv * 4
+    elif kind == "phase_b":
+        byt = cols * n * sv * 5 + cols * n * (2 * 3 if not fp64 else sv)
+    else:
+        byt = cols * (2 * n + 6 * m) * sv
+    ach = byt / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kind,
+            "algorithmic_bytes_per_launch": byt / max(calls, 1), "avg_launch_us": round(1e3 * ms / max(calls, 1), 2)}
+
+
+def wg_roofline(probs, outs, plugin, sv, kernel_ms):
+    """SpMV bytes one k_wg_solve launch executed (B_A / B_At per SpMV, s_i = 4)
+    plus the direct plugin's two triangular sweeps per iteration."""
+    si, tot = 4, 0.0
+    for p, o in zip(probs, outs):
+        it, cg = o["info"]["iters"], o["info"]["cg_iters"]
+        solved = 1 if o["status"] == "Solved" else 0
+        nnz, n, m = p.nnz, p.n, p.m
+        BA = nnz * (sv + si) + (m + 1) * si + n * sv + m * sv
+        BAt = nnz * (sv + si) + (n + 1) * si + m * sv + n * sv
+        if plugin == "indirect":
+            tot += (1 + it + cg) * BA + (2 * it + cg + solved) * BAt
+        else:
+            N, lnz = n + m, o["info"]["lnz"]
+            tot += (1 + it) * BA + (it + solved) * BAt + it * 2 * (lnz * (8 + si) + (N + 1) * si + 2 * N * 8)
+    ach = tot / (kernel_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_wg_solve",
+            "algorithmic_bytes_per_launch": tot, "avg_launch_us": round(kernel_ms * 1e3, 1)}
+
+
+# ------------------------------------------------------------------ workloads
+
+def run_family(args):
+    from pdos_amd import DeviceArray
+    from pdos_amd.family import Family
+    from pdos_amd.warmstart import LearnedWarmStart, portfolio_features
+
+    fam = problems.PortfolioFamily(seed=4, na=500, k=20, EPS_ABS=1e-3, MAX_ITERS=2000)
+    n_tr, n_te = args.train or 65536, args.count or 16384
+    th_tr, C_tr = fam.sample(seed=1001, count=n_tr)
+    th_te, C_te = fam.sample(seed=1004, count=n_te)
+    prec = args.precision or "mixed"
+    F = Family(fam.instance(C_te[0]), precision=prec, bf16_products=args.products)
+    info = F.info()
+    # 1. training data: the solver's own solutions
+    t0 = time.perf_counter()
+    F.load(C_tr)
+    F.run()
+    tr = F.fetch()
+    t_data = time.perf_counter() - t0
+    keep = tr["status"] == "Solved"
+    T = np.concatenate([tr["x"], tr["y"], tr["s"]], axis=1)[keep]
+    log(f"training data: {n_tr} solved on the GPU in {t_data:.1f} s, {keep.sum()} Solved, "
+        f"mean iters {tr['iters'].mean():.1f}")
+    # 2. training on the GPU (own BF16 kernels)
+    t0 = time.perf_counter()
+    ws = LearnedWarmStart(th_tr.shape[1], T.shape[1], hidden=512, seed=0)
+    loss = ws.fit(portfolio_features(th_tr[keep]), T, epochs=args.epochs, batch=512, lr=1e-3)
+    t_train = time.perf_counter() - t0
+    log(f"MLP trained: {args.epochs} epochs, loss {loss:.4f}, {t_train:.1f} s")
+    # 3. held-out batch resident on the device
+    F.load(C_te)
+    dF = DeviceArray(portfolio_features(th_te))
+    ld = T.shape[1]
+    dW = DeviceArray(shape=(n_te, ld))
+    # cold batched
+    for _ in range(args.warmup):
+        F.run()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        F.run()
+    cold_wall = (time.perf_counter() - t0) / args.steps
+    cold = F.fetch()
+
+    def step():
+        ws.predict_dev(dF, n_te, dW, ld)
+        F.run(dW, ld, judge=True)
+
+    for _ in range(args.warmup):
+        step()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+    wall = (time.perf_counter() - t0) / args.steps
+    warm = F.fetch()
+    comp = F.info()["compactions"]
+    # one more step with per-kernel events for the roofline
+    F.timing(True)
+    step()
+    kt = F.timing(False)
+    A = sp.csc_matrix((fam.Ax, fam.Ai, fam.Ap), shape=(fam.m, fam.n))
+    v_warm = verify_batch(A, fam.b, C_te, warm["x"], warm["y"], warm["s"], warm["status"], 1e-3)
+    v_cold = verify_batch(A, fam.b, C_te, cold["x"], cold["y"], cold["s"], cold["status"], 1e-3)
+    solved = int((warm["status"] == "Solved").sum())
+    solved_cold = int((cold["status"] == "Solved").sum())
+    value = solved / wall
+    log(f"warm: {solved}/{n_te} Solved ({v_warm} verified), {value:.0f} SOCPs/s, mean iters "
+        f"{warm['iters'].mean():.1f}, fallbacks {int(warm['fallback'].sum())}; cold: {solved_cold / cold_wall:.0f} "
+        f"SOCPs/s, mean iters {cold['iters'].mean():.1f}")
+    probs = [fam.instance(C_te[i]) for i in range(max(args.plain_sample, args.cpu_sample))]
+    plain = plain_gpu(probs[:args.plain_sample], "direct", "fp64") if args.plain_sample else None
+    if plain:
+        log(f"plain GPU: {plain['value']} SOCPs/s")
+    config = {"workload": "portfolio family, learned warm start, held-out batch", "n_a": 500, "k": 20,
+              "n": fam.n, "m": fam.m, "nnz": int(fam.Ap[-1]), "count": n_te, "train_count": n_tr,
+              "eps_abs": 1e-3, "max_iters": 2000, "precision": prec, "engine": "family (shared A)",
+              "mlp": "501-512-512-2545 GELU bf16"}
+    extra = {
+        "batched_cold": {"value": round(solved_cold / cold_wall, 3), "unit": "SOCPs/s",
+                         "ms_per_step": round(cold_wall * 1e3, 3), "mean_iters": float(cold["iters"].mean()),
+                         "solved": solved_cold, "verified_fp64": v_cold},
+        "warm_start": {"mean_iters": float(warm["iters"].mean()), "fallbacks": int(warm["fallback"].sum()),
+                       "train_loss": loss, "train_s": round(t_train, 2), "data_s": round(t_data, 2),
+                       "compactions": comp},
+        "kernels": {k: {"ms": round(v[0], 3), "calls": v[1]} for k, v in kt.items()},
+        "plain_gpu": plain,
+        "ratio_vs_plain_gpu": round(value / plain["value"], 1) if plain else None,
+    }
+    roof = family_roofline(kt, info, int(fam.Ap[-1]), prec == "fp64")
+    acc = {"solved": solved, "count": n_te, "verified_fp64": v_warm,
+           "mean_iters": float(warm["iters"].mean()), "max_iters": int(warm["iters"].max())}
+    return value, wall, "f32+bf16" if prec == "mixed" else "f64", config, roof, acc, extra, probs[:args.cpu_sample], "direct"
+
+
+def run_dense(args):
+    from pdos_amd.family import Family
+    dense = problems.DenseFamily(seed=2, EPS_ABS=1e-3, MAX_ITERS=2000)
+    cnt = args.count or 1024
+    B, Cm, opt = dense.sample(seed=1002, count=cnt)
+    prec = args.precision or "mixed"
+    F = Family(dense.instance(B[0], Cm[0]), precision=prec)
+    F.load(Cm, B)
+    for _ in range(args.warmup):
+        F.run()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        F.run()
+    wall = (time.perf_counter() - t0) / args.steps
+    out = F.fetch()
+    F.timing(True)
+    F.run()
+    kt = F.timing(False)
+    solved = int((out["status"] == "Solved").sum())
+    ver = verify_batch(dense.A, B, Cm, out["x"], out["y"], out["s"], out["status"], 1e-3)
+    gap = np.abs(np.einsum("ij,ij->i", Cm, out["x"]) - opt) / (1 + np.abs(opt))
+    config = {"workload": "dense shared-A family", "m": dense.m, "n": dense.n, "count": cnt, "eps_abs": 1e-3,
+              "precision": prec, "engine": "family (shared dense A)"}
+    roof = family_roofline(kt, F.info(), dense.m * dense.n, prec == "fp64")
+    acc = {"solved": solved, "count": cnt, "verified_fp64": ver, "mean_iters": float(out["iters"].mean()),
+           "max_iters": int(out["iters"].max()), "max_rel_obj_err_vs_planted": float(gap.max())}
+    extra = {"kernels": {k: {"ms": round(v[0], 3), "calls": v[1]} for k, v in kt.items()}}
+    probs = [dense.instance(B[i], Cm[i]) for i in range(min(args.cpu_sample, 16))]
+    return solved / wall, wall, "f32+bf16" if prec == "mixed" else "f64", config, roof, acc, extra, probs, "direct"
+
+
+def run_wg(args, probs, config, plugin):
+    prec = args.precision or "fp64"
+    t0 = time.perf_counter()
+    h = pdos_amd.BatchHandle(probs, plugin=plugin, precision=prec)
+    log(f"prepare (host setup + factor + upload): {time.perf_counter() - t0:.2f} s")
+    for _ in range(args.warmup):
+        h.run()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        kms.append(h.run())
+    wall = (time.perf_counter() - t0) / args.steps
+    outs = h.fetch()
+    h.close()
+    solved = sum(o["status"] == "Solved" for o in outs)
+    iters = np.array([o["info"]["iters"] for o in outs])
+    roof = wg_roofline(probs, outs, plugin, 8 if prec == "fp64" else 4, float(np.mean(kms)))
+    acc = {"solved": int(solved), "count": len(outs), "verified_fp64": verify_list(probs, outs),
+           "mean_iters": float(iters.mean()), "max_iters": int(iters.max())}
+    extra = {"plain_gpu": plain_gpu(probs[:args.plain_sample], plugin, prec) if args.plain_sample else None}
+    return solved / wall, wall, "f64" if prec == "fp64" else "f32", config, roof, acc, extra, \
+        probs[:args.cpu_sample], plugin
+
+
+def run_single(args, p, config, plugin):
+    prec = args.precision or "fp64"
+    for _ in range(args.warmup):
+        pdos_amd.solve_problem(p, plugin=plugin, precision=prec)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = pdos_amd.solve_problem(p, plugin=plugin, precision=prec)
+    wall = (time.perf_counter() - t0) / args.steps
+    solved = int(out["status"] == "Solved")
+    if plugin == "indirect" and out["info"]["engine"] == 2:
+        it, cg = out["info"]["iters"], out["info"]["cg_iters"]
+        sv, si = (8 if prec == "fp64" else 4), 4
+        BA = p.nnz * (sv + si) + (p.m + 1) * si + (p.n + p.m) * sv
+        BAt = p.nnz * (sv + si) + (p.n + 1) * si + (p.n + p.m) * sv
+        byt = (1 + it + cg) * BA + (2 * it + cg + solved) * BAt
+        ach = byt / (out["info"]["solve_ms"] * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "grid engine (all kernels)",
+                "algorithmic_bytes_per_solve": byt, "spmv_count": 1 + it + cg + 2 * it + cg + solved}
+    else:
+        roof = wg_roofline([p], [out], plugin, 8 if prec == "fp64" else 4, out["info"]["solve_ms"])
+    acc = {"solved": solved, "count": 1, "verified_fp64": verify_list([p], [out]),
+           "mean_iters": float(out["info"]["iters"]), "max_iters": int(out["info"]["iters"])}
+    return solved / wall, wall, "f64" if prec == "fp64" else "f32", config, roof, acc, {}, [p], plugin
+
 
 def main():
     ap = argparse.ArgumentParser()
@@ -176,94 +381,60 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="family")
     ap.add_argument("--count", type=int, default=0)
+    ap.add_argument("--train", type=int, default=0)
+    ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--plugin", default="direct")
-    ap.add_argument("--precision", default="fp64")
-    ap.add_argument("--cpu-sample", type=int, default=256)
-    ap.add_argument("--plain-sample", type=int, default=24)
+    ap.add_argument("--precision", default="")
+    ap.add_argument("--cpu-sample", type=int, default=1024)
+    ap.add_argument("--plain-sample", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--products", type=int, default=0, help="family, mixed: bf16 products (1-3)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; --gpus must be 1", file=sys.stderr)
         sys.exit(2)
 
-    probs, config = make_workload(args.workload, args.count)
-    plugin, precision = args.plugin, args.precision
-    if args.workload == "random":
-        plugin = "indirect"
-    config.update(plugin=plugin, precision=precision)
-    log(f"workload {args.workload}: {len(probs)} instances, plugin {plugin}, {precision}")
-
-    sv = 8 if precision == "fp64" else 4
-    if args.workload in ("random", "portfolio"):
-        # single instance: the step is one full pdos_ex call on prepared data
-        run = lambda: pdos_amd.solve_problem(probs[0], plugin=plugin, precision=precision)  # noqa
-        for _ in range(args.warmup):
-            out = run()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            out = run()
-        wall = (time.perf_counter() - t0) / args.steps
-        outs = [out]
-        kernel_ms = out["info"]["solve_ms"]
+    w = args.workload
+    if w == "family":
+        res = run_family(args)
+    elif w == "dense":
+        res = run_dense(args)
+    elif w == "uneven":
+        probs = problems.uneven_portfolios(seed=3, count=args.count or 4096, EPS_ABS=1e-3, MAX_ITERS=2000)
+        cfg = {"workload": "uneven portfolio batch", "count": len(probs), "n_a": [50, 1000], "k": [5, 50],
+               "eps_abs": 1e-3, "nnz_total": int(sum(p.nnz for p in probs)), "plugin": args.plugin,
+               "engine": "workgroup per instance"}
+        res = run_wg(args, probs, cfg, args.plugin)
+    elif w == "random":
+        p = problems.random_feasible(seed=1, EPS_ABS=1e-3, MAX_ITERS=2000)
+        cfg = {"workload": "random feasible SOCP", "n": p.n, "m": p.m, "nnz": p.nnz, "count": 1,
+               "eps_abs": 1e-3, "plugin": "indirect", "engine": "grid"}
+        res = run_single(args, p, cfg, "indirect")
+    elif w == "portfolio":
+        p = problems.portfolio(seed=0, EPS_ABS=1e-3)
+        cfg = {"workload": "portfolio n=101 m=212", "n": p.n, "m": p.m, "nnz": p.nnz, "count": 1,
+               "eps_abs": 1e-3, "plugin": args.plugin, "engine": "workgroup"}
+        res = run_single(args, p, cfg, args.plugin)
     else:
-        t0 = time.perf_counter()
-        h = pdos_amd.BatchHandle(probs, plugin=plugin, precision=precision)
-        log(f"prepare (host setup + factor + upload): {time.perf_counter() - t0:.2f} s")
-        for _ in range(args.warmup):
-            h.run()
-        kms = []
-        t0 = time.perf_counter()
-        for i in range(args.steps):
-            kms.append(h.run())
-            log(f"step {i}: {kms[-1]:.2f} ms")
-        wall = (time.perf_counter() - t0) / args.steps
-        kernel_ms = float(np.mean(kms))
-        outs = h.fetch()
-        h.close()
-
-    solved = sum(o["status"] == "Solved" for o in outs)
-    verified, _ = verify_fp64(probs, outs)
-    iters = np.array([o["info"]["iters"] for o in outs])
-    value = solved / wall
-    log(f"{solved}/{len(outs)} solved, {verified} verified in FP64, {value:.1f} SOCPs/s, "
-        f"mean iters {iters.mean():.1f}")
-
-    abytes = algorithmic_bytes(probs, outs, plugin, sv)
-    achieved = abytes / (kernel_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_wg_solve" if len(outs) > 1 or args.workload == "portfolio" else "grid",
-                "algorithmic_bytes_per_launch": abytes, "kernel_ms": kernel_ms}
-
-    plain = None
-    if len(probs) > 1 and args.plain_sample > 0:
-        sel = probs[:args.plain_sample]
-        pdos_amd.solve_problem(sel[0], plugin=plugin, precision=precision)
-        t0 = time.perf_counter()
-        ns = sum(pdos_amd.solve_problem(p, plugin=plugin, precision=precision)["status"] == "Solved"
-                 for p in sel)
-        dt = time.perf_counter() - t0
-        plain = {"value": ns / dt, "unit": "SOCPs/s",
-                 "sample": f"first {len(sel)} instances, one pdos_ex call each (setup, upload, "
-                           f"solve, download)"}
-        log(f"plain single-instance GPU path: {plain['value']:.1f} SOCPs/s")
+        raise SystemExit(f"unknown workload {w}")
+    value, wall, dtype, config, roof, acc, extra, cpu_probs, cpu_plugin = res
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and cpu_probs:
         procs = max(1, min(16, os.cpu_count() or 1))
-        cpu = cpu_baseline(probs, "direct" if args.workload != "random" else "indirect",
-                           min(args.cpu_sample, len(probs)), procs)
-        log(f"cpu baseline: {cpu['value']:.1f} SOCPs/s on {cpu['cores']} cores")
+        cpu = cpu_baseline(cpu_probs, "indirect" if w == "random" else cpu_plugin, procs)
+        log(f"cpu baseline: {cpu['value']} SOCPs/s on {cpu['cores']} cores ({cpu['kind']})")
 
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "SOCPs/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if precision == "fp64" else "f32", "data": "synthetic",
-        "config": config, "roofline": roofline, "cpu_baseline": cpu, "plain_gpu": plain,
-        "accuracy": {"solved": int(solved), "count": len(outs), "verified_fp64": int(verified),
-                     "mean_iters": float(iters.mean()), "max_iters": int(iters.max())},
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+        "data": "synthetic", "config": config, "roofline": roof, "cpu_baseline": cpu,
+        "accuracy": acc,
     }
+    line.update(extra)
+    if cpu:
+        line["ratio_vs_cpu_baseline"] = round(value / cpu["value"], 1)
     print(json.dumps(line), flush=True)
 
 

@@ -39,7 +39,11 @@ typedef struct PdosOptions {
                       GRID = whole-GPU kernels for one large instance */
   int device;      /* HIP device ordinal */
   int block;       /* WG engine threads per workgroup (0 = autotuned) */
-  int reserved[3];
+  int bf16_products; /* family engine, mixed: products of the split-bf16 x-update
+                        GEMM, 1 (hi*hi), 2 (+hi*lo) or 3 (+lo*hi); 0 = 1.  The
+                        GEMM computes the correction x+ - x, whose error shrinks
+                        with the step, so one product converges like three. */
+  int reserved[2];
 } PdosOptions;
 
 typedef struct PdosInfo {
@@ -121,10 +125,14 @@ int pdos_family_fetch(PdosFamily *h, double *X, double *Y, double *S, long *iter
 void pdos_family_free(PdosFamily *h);
 const char *pdos_family_last_error(void);
 /* Per-kernel HIP-event timing of the solve loop.  Returns (and resets) the
- * totals since the last call in ms[6] / calls[6] for the kinds: phase A,
- * A' product, phase B, x-update GEMM, A product, phase C; then turns the
- * timing on or off for later runs (events add a little launch overhead). */
-int pdos_family_timing(PdosFamily *h, int enable, double *ms, long *calls);
+ * totals since the last call in ms[6] / calls[6] / cols[6] (instance columns
+ * the launches covered, padding included) for the kinds: phase A, A' product,
+ * phase B, x-update GEMM, A product, phase C; then turns the timing on or off
+ * for later runs (events add a little launch overhead). */
+int pdos_family_timing(PdosFamily *h, int enable, double *ms, long *calls, double *cols);
+/* out[7]: n, m, padded n of the GEMM, compactions in the last run, dense A,
+ * fp64, bf16 products of the mixed x-update. */
+int pdos_family_info(PdosFamily *h, long *out);
 
 /* Learned warm start: MLP n_in -> n_hidden -> n_hidden -> n_out with GELU,
  * bf16 weights mirrored from a binary32 master, binary32 accumulation, all

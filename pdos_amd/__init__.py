@@ -30,7 +30,8 @@ ENGINE = {"auto": 0, "wg": 1, "grid": 2}
 
 class PdosOptions(C.Structure):
     _fields_ = [("plugin", C.c_int), ("precision", C.c_int), ("engine", C.c_int),
-                ("device", C.c_int), ("block", C.c_int), ("reserved", C.c_int * 3)]
+                ("device", C.c_int), ("block", C.c_int), ("bf16_products", C.c_int),
+                ("reserved", C.c_int * 2)]
 
 
 class PdosInfo(C.Structure):
@@ -53,7 +54,7 @@ EXPORTS = ["pdos", "freeData", "freeSol", "tic", "toc", "tocq", "pdos_default_op
            "pdos_mlp_set_affine", "pdos_mlp_loss_grad", "pdos_mlp_train", "pdos_mlp_predict",
            "pdos_mlp_predict_dev", "pdos_mlp_last_error", "pdos_family_create", "pdos_family_load",
            "pdos_family_run", "pdos_family_fetch", "pdos_family_free", "pdos_family_last_error",
-           "pdos_family_timing",
+           "pdos_family_timing", "pdos_family_info",
            "pdos_kkt_solve_host", "pdos_last_error",
            "pdos_version"]
 
@@ -118,8 +119,9 @@ def lib(plugin: str = "indirect"):
     return _libs[plugin]
 
 
-def options(plugin="indirect", precision="fp64", engine="auto", device=0) -> PdosOptions:
+def options(plugin="indirect", precision="fp64", engine="auto", device=0, bf16_products=0) -> PdosOptions:
     o = PdosOptions()
+    o.bf16_products = bf16_products
     o.plugin = PLUGIN[plugin]
     o.precision = PRECISION[precision]
     o.engine = ENGINE[engine]

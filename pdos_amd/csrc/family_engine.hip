@@ -28,6 +28,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <type_traits>
@@ -60,6 +61,11 @@ struct FamArgs {
   T* c;                // n x ldb, scaled (x1e3)
   double* lam; double* pscale; double* dscale;
   int* orig;           // column -> instance of the loaded batch
+  double* pa;          // phase A partials [chunk][ldb][2]
+  double* pb;          // phase B partials [chunk][ldb][2]
+  double* pc;          // phase C cone-chunk partial norms [item][ldb]
+  int rpc;             // rows per chunk of phases A and B (coarse RPC or fine RPC_FINE)
+  double* pt0;         // phase C first entry of each cone [cone][ldb]
   T* X; T* XN; T* S; T* Y; T* AX; T* W; T* U; T* V; T* R; T* Z;
   __bf16* Rcat;        // mixed: [D_hi; D_lo; D_hi] of the correction D, 3 np x ldb
   int* done; int* status; int* iters; int* fallback;
@@ -120,44 +126,87 @@ __global__ __launch_bounds__(FT) void k_fam_init(FamArgs<T> a, const float* __re
   }
 }
 
+// Rows [r0, r1) of chunk blockIdx.y: chunks of RPC rows give every launch
+// (instance blocks) x (row chunks) workgroups, so a compacted batch of a few
+// hundred instances still fills the GPU.
+constexpr int RPC = 64;
+constexpr int RPC_FINE = 8;  // one row per wave: small (compacted) batches
+
+__device__ __forceinline__ void chunk_rows(int nrows, int rpc, int& r0, int& r1) {
+  r0 = blockIdx.y * rpc;
+  r1 = min(nrows, r0 + rpc);
+}
+
 // Y_k = M X_k (k < NR) for a shared sparse M (rows ptr/idx/val) and
-// feature-major X_k: up to three right-hand sides per pass over M.
+// feature-major X_k, up to three right-hand sides per pass over M.  Work
+// items (host-built, one per blockIdx.y) balance the rows: item (r0, r1, 0)
+// is a run of short rows, one row per wave; item (r, r + 1, 1) is one long
+// row whose nonzeros the block's waves split, reduced in LDS.
 template <typename T, int NR>
-__global__ __launch_bounds__(FT) void k_fam_spmm(int nrows, const int* __restrict__ ptr,
+__global__ __launch_bounds__(FT) void k_fam_spmm(const int4* __restrict__ items, const int* __restrict__ ptr,
                                                  const int* __restrict__ idx, const T* __restrict__ val,
                                                  const T* __restrict__ X1, const T* __restrict__ X2,
                                                  const T* __restrict__ X3, T* __restrict__ Y1,
                                                  T* __restrict__ Y2, T* __restrict__ Y3, int ldb) {
+  __shared__ double buf[NWV * LANES];
   const int lane = threadIdx.x & 63, w = wave_id();
   const int inst = blockIdx.x * LANES + lane;
-  for (int r = w; r < nrows; r += NWV) {
-    T a1 = T(0), a2 = T(0), a3 = T(0);
-    const int pe = ptr[r + 1];
-    for (int p = ptr[r]; p < pe; ++p) {
-      const T v = val[p];
-      const size_t o = (size_t)idx[p] * ldb + inst;
-      a1 += v * X1[o];
-      if (NR > 1) a2 += v * X2[o];
-      if (NR > 2) a3 += v * X3[o];
+  const int4 it = items[blockIdx.y];
+  if (it.z == 0) {
+    for (int r = it.x + w; r < it.y; r += NWV) {
+      T a1 = T(0), a2 = T(0), a3 = T(0);
+      const int pe = ptr[r + 1];
+      for (int p = ptr[r]; p < pe; ++p) {
+        const T v = val[p];
+        const size_t o = (size_t)idx[p] * ldb + inst;
+        a1 += v * X1[o];
+        if (NR > 1) a2 += v * X2[o];
+        if (NR > 2) a3 += v * X3[o];
+      }
+      const size_t q = (size_t)r * ldb + inst;
+      Y1[q] = a1;
+      if (NR > 1) Y2[q] = a2;
+      if (NR > 2) Y3[q] = a3;
     }
-    const size_t q = (size_t)r * ldb + inst;
-    Y1[q] = a1;
-    if (NR > 1) Y2[q] = a2;
-    if (NR > 2) Y3[q] = a3;
+    return;
+  }
+  const int r = it.x;
+  T a1 = T(0), a2 = T(0), a3 = T(0);
+  const int pe = ptr[r + 1];
+  for (int p = ptr[r] + w; p < pe; p += NWV) {
+    const T v = val[p];
+    const size_t o = (size_t)idx[p] * ldb + inst;
+    a1 += v * X1[o];
+    if (NR > 1) a2 += v * X2[o];
+    if (NR > 2) a3 += v * X3[o];
+  }
+  const size_t q = (size_t)r * ldb + inst;
+  const T t1 = T(waves_sum(double(a1), buf));
+  if (w == 0) Y1[q] = t1;
+  if (NR > 1) {
+    const T t2 = T(waves_sum(double(a2), buf));
+    if (w == 0) Y2[q] = t2;
+  }
+  if (NR > 2) {
+    const T t3 = T(waves_sum(double(a3), buf));
+    if (w == 0) Y3[q] = t3;
   }
 }
 
-// Phase A: W = b - S - lam Y and the m-row parts of the stop test.
+// Phase A (one row chunk of the m rows): W = b - S - lam Y, and the chunk's
+// partial |(S - b + AX) ./ D|^2 and b'Y per instance.
 template <typename T>
 __global__ __launch_bounds__(FT) void k_fam_a(FamArgs<T> a) {
   __shared__ double buf[NWV * LANES];
   const int lane = threadIdx.x & 63, w = wave_id();
   const int inst = blockIdx.x * LANES + lane;
   const bool act = inst < a.count && !a.done[inst];
+  int r0, r1;
+  chunk_rows(a.m, a.rpc, r0, r1);
   double p2 = 0.0, by = 0.0;
   if (act) {
     const T lam = T(a.lam[inst]);
-    for (int j = w; j < a.m; j += NWV) {
+    for (int j = r0 + w; j < r1; j += NWV) {
       const size_t o = (size_t)j * a.ldb + inst;
       const T s = a.S[o], y = a.Y[o], bj = bval(a, j, inst);
       const T r = ((s - bj) + a.AX[o]) / T(a.D[j]);
@@ -169,77 +218,36 @@ __global__ __launch_bounds__(FT) void k_fam_a(FamArgs<T> a) {
   p2 = waves_sum(p2, buf);
   by = waves_sum(by, buf);
   if (act && w == 0) {
-    a.red[(size_t)inst * 8 + 0] = p2;
-    a.red[(size_t)inst * 8 + 1] = by;
+    double* pa = a.pa + ((size_t)blockIdx.y * a.ldb + inst) * 2;
+    pa[0] = p2;
+    pa[1] = by;
   }
 }
 
-// Phase B: the n-row parts of the stop test, the test itself, and R.
-// With a.judge the pass instead compares the warm start with the cold start.
+// Phase B (one row chunk of the n rows): partial |(c + U) ./ E|^2 and c'X,
+// and the x-update right-hand side of every active instance:
+//   fp64: R = x - lam c + A'(b - s - lam y), then x+ = G R;
+//   mixed: the correction D = R - (I + A'A) x = A'(b - s - lam y) - lam c - A'(A x),
+//          then x+ = x + G D, so the bf16 error of the product shrinks with D.
 template <typename T, bool MIXED>
 __global__ __launch_bounds__(FT) void k_fam_b(FamArgs<T> a) {
   __shared__ double buf[NWV * LANES];
-  __shared__ int s_reset[LANES];
   const int lane = threadIdx.x & 63, w = wave_id();
   const int inst = blockIdx.x * LANES + lane;
   const bool act = inst < a.count && !a.done[inst];
-  const int it0 = act ? a.iters[inst] : 0;  // read before wave 0 advances it
+  int r0, r1;
+  chunk_rows(a.n, a.rpc, r0, r1);
   double d2 = 0.0, cx = 0.0;
   if (act) {
-    for (int i = w; i < a.n; i += NWV) {
+    const T lam = T(a.lam[inst]);
+    for (int i = r0 + w; i < r1; i += NWV) {
       const size_t o = (size_t)i * a.ldb + inst;
-      const T ci = a.c[o];
+      const T ci = a.c[o], xi = a.X[o];
       const T r = (ci + a.U[o]) / T(a.E[i]);
       d2 += double(r) * double(r);
-      cx += double(ci) * double(a.X[o]);
-    }
-  }
-  d2 = waves_sum(d2, buf);
-  cx = waves_sum(cx, buf);
-  bool upd = false;
-  if (act) {
-    double* rd = a.red + (size_t)inst * 8;
-    const double pres = sqrt(rd[0]), dres = sqrt(d2), pobj = cx, dobj = -rd[1];
-    const double eta = fabs(pobj - dobj);
-    const double ps = a.pscale[inst], ds = a.dscale[inst];
-    if (a.judge) {
-      const double mw = fmax(fmax(pres / ps, dres / ds), eta / (1.0 + fabs(pobj) + fabs(dobj)));
-      const double mc = fmax((ps - 1.0) / ps, (ds - 1.0) / ds);
-      if (w == 0) s_reset[lane] = !(mw <= mc);
-    } else if (it0 >= a.max_iters) {
-      if (w == 0) a.done[inst] = 1;  // Indeterminate; residuals of the last check stay
-    } else {
-      if (w == 0) {
-        rd[2] = pres; rd[3] = dres; rd[4] = pobj; rd[5] = dobj; rd[6] = eta;
-      }
-      if (pres < a.eps * ps && dres < a.eps * ds && eta < a.eps * (1.0 + fabs(pobj) + fabs(dobj))) {
-        if (w == 0) { a.done[inst] = 1; a.status[inst] = 0; }
-      } else {
-        upd = true;
-        if (w == 0) a.iters[inst] = it0 + 1;
-      }
-    }
-  }
-  if (a.judge) {
-    __syncthreads();
-    if (act && s_reset[lane]) {  // rejected warm start: cold start for this instance
-      for (int i = w; i < a.n; i += NWV) a.X[(size_t)i * a.ldb + inst] = T(0);
-      for (int j = w; j < a.m; j += NWV) {
-        const size_t o = (size_t)j * a.ldb + inst;
-        a.S[o] = T(0); a.Y[o] = T(0); a.AX[o] = T(0);
-      }
-      if (w == 0) a.fallback[inst] = 1;
-    }
-    return;
-  }
-  if (upd) {
-    const T lam = T(a.lam[inst]);
-    for (int i = w; i < a.n; i += NWV) {
-      const size_t o = (size_t)i * a.ldb + inst;
-      // fp64: R = x - lam c + A'(b - s - lam y), then x+ = G R.
-      // mixed: the correction Delta = R - (I + A'A) x = A'(b - s - lam y) - lam c - A'(A x),
-      // then x+ = x + G Delta, so the bf16 error of the product shrinks with Delta.
-      const T rv = MIXED ? (a.V[o] - a.Z[o]) - lam * a.c[o] : a.X[o] - lam * a.c[o] + a.V[o];
+      cx += double(ci) * double(xi);
+      if (a.judge) continue;
+      const T rv = MIXED ? (a.V[o] - a.Z[o]) - lam * ci : xi - lam * ci + a.V[o];
       if (MIXED) {
         const __bf16 hi = (__bf16)float(rv);
         const __bf16 lo = (__bf16)(float(rv) - float(hi));
@@ -251,24 +259,91 @@ __global__ __launch_bounds__(FT) void k_fam_b(FamArgs<T> a) {
       }
     }
   }
+  d2 = waves_sum(d2, buf);
+  cx = waves_sum(cx, buf);
+  if (act && w == 0) {
+    double* pb = a.pb + ((size_t)blockIdx.y * a.ldb + inst) * 2;
+    pb[0] = d2;
+    pb[1] = cx;
+  }
 }
 
-// Phase C: X = XN, s~ = b - AX, relax, cone projection, dual update.
+// The stop test of every active instance (pdos.c:95-107) from the chunk
+// partials, in a fixed order; one thread per instance.  In judge mode it
+// compares a warm start's scaled residuals with the cold start's and resets
+// rejected instances to zero instead.
 template <typename T>
-__global__ __launch_bounds__(FT) void k_fam_c(FamArgs<T> a) {
+__global__ __launch_bounds__(256) void k_fam_decide(FamArgs<T> a, int chunks_m, int chunks_n) {
+  const int inst = blockIdx.x * blockDim.x + threadIdx.x;
+  if (inst >= a.count || a.done[inst]) return;
+  double p2 = 0.0, by = 0.0, d2 = 0.0, cx = 0.0;
+  for (int c = 0; c < chunks_m; ++c) {
+    const double* pa = a.pa + ((size_t)c * a.ldb + inst) * 2;
+    p2 += pa[0];
+    by += pa[1];
+  }
+  for (int c = 0; c < chunks_n; ++c) {
+    const double* pb = a.pb + ((size_t)c * a.ldb + inst) * 2;
+    d2 += pb[0];
+    cx += pb[1];
+  }
+  const double pres = sqrt(p2), dres = sqrt(d2), pobj = cx, dobj = -by;
+  const double eta = fabs(pobj - dobj);
+  const double ps = a.pscale[inst], ds = a.dscale[inst];
+  double* rd = a.red + (size_t)inst * 8;
+  if (a.judge) {
+    const double mw = fmax(fmax(pres / ps, dres / ds), eta / (1.0 + fabs(pobj) + fabs(dobj)));
+    const double mc = fmax((ps - 1.0) / ps, (ds - 1.0) / ds);
+    if (!(mw <= mc)) {  // rejected warm start: cold start for this instance
+      for (int i = 0; i < a.n; ++i) a.X[(size_t)i * a.ldb + inst] = T(0);
+      for (int j = 0; j < a.m; ++j) {
+        const size_t o = (size_t)j * a.ldb + inst;
+        a.S[o] = T(0); a.Y[o] = T(0); a.AX[o] = T(0);
+      }
+      a.fallback[inst] = 1;
+    }
+    return;
+  }
+  const int it = a.iters[inst];
+  if (it >= a.max_iters) {  // Indeterminate; the residuals of the last check stay
+    a.done[inst] = 1;
+    return;
+  }
+  rd[2] = pres; rd[3] = dres; rd[4] = pobj; rd[5] = dobj; rd[6] = eta;
+  if (pres < a.eps * ps && dres < a.eps * ds && eta < a.eps * (1.0 + fabs(pobj) + fabs(dobj))) {
+    a.done[inst] = 1;
+    a.status[inst] = 0;
+  } else {
+    a.iters[inst] = it + 1;
+  }
+}
+
+// Phase C, first pass.  Work items (one per blockIdx.y, host-built):
+//   (i0, i1, 0, -)  rows of X = XN
+//   (j0, j1, 1, -)  zero / nonneg rows: s~ = b - AX, relax, project, dual update
+//   (j0, j1, 2, g)  a chunk of second-order cone rows (inside one cone):
+//                   s~ and z = s~ - lam y stored, partial |z|^2 (without the
+//                   cone's first row) into pc[g]; the cone's first row keeps t0
+template <typename T>
+__global__ __launch_bounds__(FT) void k_fam_c1(FamArgs<T> a, const int4* __restrict__ items) {
   __shared__ double buf[NWV * LANES];
-  __shared__ double s_t0[LANES];
   const int lane = threadIdx.x & 63, w = wave_id();
   const int inst = blockIdx.x * LANES + lane;
   const bool act = inst < a.count && !a.done[inst];
   const T lam = act ? T(a.lam[inst]) : T(1);
   const T ra = T(a.relax), rb = T(1.0) - T(a.relax);
-  if (act) {
-    for (int i = w; i < a.n; i += NWV) {
+  const int4 it = items[blockIdx.y];
+  if (it.z == 0) {
+    if (!act) return;
+    for (int i = it.x + w; i < it.y; i += NWV) {
       const size_t o = (size_t)i * a.ldb + inst;
       a.X[o] = a.XN[o];
     }
-    for (int j = w; j < a.lend; j += NWV) {
+    return;
+  }
+  if (it.z == 1) {
+    if (!act) return;
+    for (int j = it.x + w; j < it.y; j += NWV) {
       const size_t o = (size_t)j * a.ldb + inst;
       T st = bval(a, j, inst) - a.AX[o];
       st = ra * st + rb * a.S[o];
@@ -277,37 +352,53 @@ __global__ __launch_bounds__(FT) void k_fam_c(FamArgs<T> a) {
       a.S[o] = s;
       a.Y[o] += (s - st) / lam;
     }
+    return;
   }
-  for (int cn = 0; cn < a.nsoc; ++cn) {
-    const int r0 = a.soc_ptr[cn], r1 = a.soc_ptr[cn + 1];
-    double n2 = 0.0;
-    if (act) {
-      for (int j = r0 + w; j < r1; j += NWV) {
-        const size_t o = (size_t)j * a.ldb + inst;
-        T st = bval(a, j, inst) - a.AX[o];
-        st = ra * st + rb * a.S[o];
-        const T z = st - lam * a.Y[o];
-        a.W[o] = st;   // keep s~ for the dual update
-        a.S[o] = z;
-        if (j > r0) n2 += double(z) * double(z);
-        else s_t0[lane] = double(z);  // row r0 belongs to wave 0
-      }
+  const int cone_r0 = a.soc_ptr[it.w];
+  double n2 = 0.0;
+  if (act) {
+    for (int j = it.x + w; j < it.y; j += NWV) {
+      const size_t o = (size_t)j * a.ldb + inst;
+      T st = bval(a, j, inst) - a.AX[o];
+      st = ra * st + rb * a.S[o];
+      const T z = st - lam * a.Y[o];
+      a.W[o] = st;  // s~ for the dual update
+      a.S[o] = z;
+      if (j > cone_r0) n2 += double(z) * double(z);
+      else a.pt0[(size_t)it.w * a.ldb + inst] = double(z);  // t0, read by every chunk of the cone
     }
-    n2 = waves_sum(n2, buf);  // also orders the z writes before the reads below
-    if (act) {
-      const T ns = T(sqrt(n2));
-      const T t0 = T(s_t0[lane]);
-      const int mode = ns <= t0 ? 0 : (ns <= -t0 ? 1 : 2);
-      const T al = (ns + t0) / T(2);
-      for (int j = r0 + w; j < r1; j += NWV) {
-        const size_t o = (size_t)j * a.ldb + inst;
-        const T z = a.S[o];
-        const T s = mode == 0 ? z : (mode == 1 ? T(0) : (j == r0 ? al : al * z / ns));
-        a.Y[o] += (s - a.W[o]) / lam;
-        a.S[o] = s;
-      }
-    }
-    __syncthreads();  // s_t0 is rewritten by the next cone
+  }
+  n2 = waves_sum(n2, buf);
+  if (act && w == 0) a.pc[(size_t)blockIdx.y * a.ldb + inst] = n2;
+}
+
+// Phase C, second pass over the cone chunks: the cone's norm is the sum of
+// its chunks' partials (fixed order) and t0 comes from the first pass, so no
+// block depends on another block's writes; project (cones.c:26-47), dual
+// update.  Item (j0, j1, first chunk of the cone, chunks in the cone);
+// cone[item] is the cone index.
+template <typename T>
+__global__ __launch_bounds__(FT) void k_fam_c2(FamArgs<T> a, const int4* __restrict__ items,
+                                               const int* __restrict__ cone) {
+  const int lane = threadIdx.x & 63, w = wave_id();
+  const int inst = blockIdx.x * LANES + lane;
+  if (inst >= a.count || a.done[inst]) return;
+  const int4 it = items[blockIdx.y];
+  const int cn = cone[blockIdx.y];
+  double n2 = 0.0;
+  for (int g = it.z; g < it.z + it.w; ++g) n2 += a.pc[(size_t)g * a.ldb + inst];
+  const int r0 = a.soc_ptr[cn];
+  const T lam = T(a.lam[inst]);
+  const T ns = T(sqrt(n2));
+  const T t0 = T(a.pt0[(size_t)cn * a.ldb + inst]);
+  const int mode = ns <= t0 ? 0 : (ns <= -t0 ? 1 : 2);
+  const T al = (ns + t0) / T(2);
+  for (int j = it.x + w; j < it.y; j += NWV) {
+    const size_t o = (size_t)j * a.ldb + inst;
+    const T z = a.S[o];
+    const T sv = mode == 0 ? z : (mode == 1 ? T(0) : (j == r0 ? al : al * z / ns));
+    a.Y[o] += (sv - a.W[o]) / lam;
+    a.S[o] = sv;
   }
 }
 
@@ -439,20 +530,45 @@ struct Family {
   int count = 0, ldb = 0, b_inst = 0;
   std::vector<void*> batch_allocs;
   struct Set {
-    void *X, *XN, *S, *Y, *AX, *W, *U, *V, *Z, *R, *Rcat, *c, *b, *lam, *ps, *ds, *ints, *red, *orig;
+    void *X, *XN, *S, *Y, *AX, *W, *U, *V, *Z, *R, *Rcat, *c, *b, *lam, *ps, *ds, *ints, *red, *orig, *pa, *pb, *pc, *pt0;
   } set[2];
   void *base_c = nullptr, *base_b = nullptr, *base_lam = nullptr, *base_ps = nullptr, *base_ds = nullptr;
   void *Xo = nullptr, *Yo = nullptr, *So = nullptr, *Io = nullptr, *Ro = nullptr;
   int* dlist = nullptr;
   int compactions = 0;
+  int nprod = 1;          // mixed: bf16 products of the split GEMM (1, 2 or 3)
+  // Graph replay of the loop (mixed mode) is opt-in (PDOS_FAMILY_GRAPH=1): it
+  // measured no faster than direct launches, and its capture crashed inside
+  // the HIP runtime under the rocprofv3 kernel tracer.
+  bool use_graph = false;
+  // mixed: one captured graph of `gsteps` iterations for the current layout
+  hipGraphExec_t gexec = nullptr;
+  int g_ld = -1, g_steps = -1;
+  const void* g_x = nullptr;
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    gexec = nullptr;
+    g_ld = g_steps = -1;
+    g_x = nullptr;
+  }
+  // work-item tables (device), built once per family
+  struct Items {
+    int4 *A = nullptr, *At = nullptr, *c1 = nullptr, *c2 = nullptr;
+    int* c2_cone = nullptr;
+    int nA = 0, nAt = 0, nc1 = 0, nc2 = 0;
+    int rpc = RPC;
+  } items[2];  // [0] coarse, [1] fine
+  const Items* cur_items = &items[0];
   double last_ms = 0;
   // optional per-kernel timing: HIP events around every launch of the loop
   bool timing = false;
   enum { K_A = 0, K_AT, K_B, K_GEMM, K_AX, K_C, K_KINDS };
   double kind_ms[K_KINDS] = {0};
   long kind_calls[K_KINDS] = {0};
+  double kind_cols[K_KINDS] = {0};  // columns (instances incl. padding) the launches covered
+  int cur_ld = 0;
   std::vector<hipEvent_t> evs;
-  std::vector<int> ev_kind;
+  std::vector<int> ev_kind, ev_ld;
   void mark(int kind) {  // event before launching `kind` (kind < 0: closing event)
     if (!timing) return;
     hipEvent_t e;
@@ -460,6 +576,7 @@ struct Family {
     (void)hipEventRecord(e, st);
     evs.push_back(e);
     ev_kind.push_back(kind);
+    ev_ld.push_back(cur_ld);
   }
   void collect() {
     if (!timing) return;
@@ -470,14 +587,17 @@ struct Family {
       (void)hipEventElapsedTime(&ms, evs[i], evs[i + 1]);
       kind_ms[ev_kind[i]] += ms;
       kind_calls[ev_kind[i]] += 1;
+      kind_cols[ev_kind[i]] += ev_ld[i];
     }
     for (hipEvent_t e : evs) (void)hipEventDestroy(e);
     evs.clear();
     ev_kind.clear();
+    ev_ld.clear();
   }
   std::vector<void*> allocs;
 
   ~Family() {
+    drop_graph();
     free_batch();
     for (void* p : allocs) (void)hipFree(p);
     if (blas) rocblas_destroy_handle(blas);
@@ -526,8 +646,43 @@ struct Family {
   bool apply_At(FamArgs<T>& a, const T* X1, const T* X2, const T* X3, T* Z1, T* Z2, T* Z3);
 };
 
+// Rows of a compressed matrix into SpMM work items: a row with more than
+// kLong nonzeros is an item of its own (its block's waves split the row);
+// other rows are grouped in runs of at most RPC rows and kRunNnz nonzeros.
+static std::vector<int4> spmm_items(const std::vector<int>& ptr, int rows_per_run) {
+  constexpr int kLong = 64;
+  const int kRunNnz = rows_per_run * 8;
+  std::vector<int4> items;
+  const int nrows = (int)ptr.size() - 1;
+  int r = 0;
+  while (r < nrows) {
+    const int len = ptr[r + 1] - ptr[r];
+    if (len > kLong) {
+      items.push_back(int4{r, r + 1, 1, 0});
+      ++r;
+      continue;
+    }
+    int e = r, nz = 0;
+    while (e < nrows && e - r < rows_per_run) {
+      const int l = ptr[e + 1] - ptr[e];
+      if (l > kLong || (e > r && nz + l > kRunNnz)) break;
+      nz += l;
+      ++e;
+    }
+    items.push_back(int4{r, e, 0, 0});
+    r = e;
+  }
+  if (items.empty()) items.push_back(int4{0, 0, 0, 0});
+  return items;
+}
+
 bool Family::create(const Data* d, const Cone* k, const PdosOptions& o, std::string& err) {
   fp64 = o.precision == PDOS_FP64;
+  if (o.bf16_products >= 1 && o.bf16_products <= 3) nprod = o.bf16_products;
+  {
+    const char* g = std::getenv("PDOS_FAMILY_GRAPH");
+    use_graph = g && g[0] == '1';
+  }
   device = o.device;
   if (!build_setup(d, k, S, err)) return false;
   par = *d->p;
@@ -548,6 +703,35 @@ bool Family::create(const Data* d, const Cone* k, const PdosOptions& o, std::str
   soc = upload(S.soc_ptr, err); D = upload(S.D, err); E = upload(S.E, err);
   bshared = upload_vals(S.b, err);
   if (!err.empty()) return false;
+  for (int gi = 0; gi < 2; ++gi) {
+    Items& I = items[gi];
+    const int R = gi == 0 ? RPC : RPC_FINE;
+    I.rpc = R;
+    const std::vector<int4> iA = spmm_items(S.rp, R), iAt = spmm_items(S.cp, R);
+    std::vector<int4> c1, c2;
+    std::vector<int> c2c;
+    for (int i = 0; i < n; i += R) c1.push_back(int4{i, std::min(n, i + R), 0, 0});
+    for (int j = 0; j < S.f + S.l; j += R) c1.push_back(int4{j, std::min(S.f + S.l, j + R), 1, 0});
+    for (int cn = 0; cn < S.nsoc(); ++cn) {
+      const int r0 = S.soc_ptr[cn], r1 = S.soc_ptr[cn + 1];
+      const int g0 = (int)c1.size();
+      for (int j = r0; j < r1; j += R) c1.push_back(int4{j, std::min(r1, j + R), 2, cn});
+      const int ng = (int)c1.size() - g0;
+      for (int g = g0; g < g0 + ng; ++g) {
+        c2.push_back(int4{c1[g].x, c1[g].y, g0, ng});
+        c2c.push_back(cn);
+      }
+    }
+    I.nA = (int)iA.size(); I.nAt = (int)iAt.size(); I.nc1 = (int)c1.size(); I.nc2 = (int)c2.size();
+    I.A = static_cast<int4*>(upload(iA, err));
+    I.At = static_cast<int4*>(upload(iAt, err));
+    I.c1 = static_cast<int4*>(upload(c1, err));
+    if (!c2.empty()) {
+      I.c2 = static_cast<int4*>(upload(c2, err));
+      I.c2_cone = static_cast<int*>(upload(c2c, err));
+    }
+    if (!err.empty()) return false;
+  }
   // M = I + A'A of the scaled A on the device (rocBLAS syrk), then
   // G = M^-1 by Cholesky (rocSOLVER potrf + potri), once per family.
   std::vector<double> Acol((size_t)m * n, 0.0);  // column-major m x n
@@ -661,6 +845,11 @@ bool Family::load(int cnt, const double* C, const double* B, std::string& err) {
     q.ints = dalloc((size_t)4 * ldb * 4, batch_allocs, err);
     q.red = dalloc((size_t)8 * ldb * 8, batch_allocs, err);
     q.orig = dalloc((size_t)ldb * 4, batch_allocs, err);
+    // partials sized for the fine granularity (the most chunks)
+    q.pa = dalloc((size_t)((m + RPC_FINE - 1) / RPC_FINE) * ldb * 2 * 8, batch_allocs, err);
+    q.pb = dalloc((size_t)((n + RPC_FINE - 1) / RPC_FINE) * ldb * 2 * 8, batch_allocs, err);
+    q.pc = dalloc((size_t)std::max(items[0].nc1, items[1].nc1) * ldb * 8, batch_allocs, err);
+    q.pt0 = dalloc((size_t)std::max(1, S.nsoc()) * ldb * 8, batch_allocs, err);
   }
   Xo = vec(n); Yo = vec(m); So = vec(m);
   Io = dalloc((size_t)3 * ldb * 4, batch_allocs, err);
@@ -691,6 +880,10 @@ FamArgs<T> Family::args(int k, int ld, int cnt) {
   a.done = ip; a.status = ip + ld; a.iters = ip + 2 * ld; a.fallback = ip + 3 * ld;
   a.red = static_cast<double*>(q.red);
   a.orig = static_cast<int*>(q.orig);
+  a.pa = static_cast<double*>(q.pa);
+  a.pb = static_cast<double*>(q.pb);
+  a.pc = static_cast<double*>(q.pc);
+  a.pt0 = static_cast<double*>(q.pt0);
   return a;
 }
 
@@ -709,16 +902,26 @@ void Family::fill_args(FamArgs<T>& a) {
 
 template <typename T>
 bool Family::iterate(FamArgs<T>& a, int steps, std::string& err) {
-  const dim3 grid(a.ldb / LANES), block(FT);
+  const int cols = a.ldb / LANES;
+  // SpMM work items: coarse runs while the batch is large, one row per wave
+  // once it is small; the phase kernels keep coarse chunks (measured faster).
+  const Items& I = items[0];
+  cur_items = &items[(long)cols * items[0].nA < 2048 ? 1 : 0];
+  a.rpc = I.rpc;
+  const int cm = (m + I.rpc - 1) / I.rpc, cn = (n + I.rpc - 1) / I.rpc;
+  const dim3 gm(cols, cm), gn(cols, cn), gd((a.ldb + 255) / 256), block(FT);
+  cur_ld = a.ldb;
   for (int s = 0; s < steps; ++s) {
     mark(K_A);
-    hipLaunchKernelGGL(k_fam_a<T>, grid, block, 0, st, a);
+    hipLaunchKernelGGL(k_fam_a<T>, gm, block, 0, st, a);
     mark(K_AT);
     if (!apply_At(a, a.Y, a.W, fp64 ? nullptr : a.AX, a.U, a.V, a.Z)) { err = "A' product failed"; return false; }
     mark(K_B);
+    if (fp64) hipLaunchKernelGGL((k_fam_b<T, false>), gn, block, 0, st, a);
+    else hipLaunchKernelGGL((k_fam_b<T, true>), gn, block, 0, st, a);
+    hipLaunchKernelGGL(k_fam_decide<T>, gd, dim3(256), 0, st, a, cm, cn);
+    mark(K_GEMM);
     if (fp64) {
-      hipLaunchKernelGGL((k_fam_b<T, false>), grid, block, 0, st, a);
-      mark(K_GEMM);
       const double one = 1.0, zero = 0.0;
       // row-major XN (n x ldb) = G (n x n) R (n x ldb): column-major XN' = R' G
       if (rocblas_dgemm(blas, rocblas_operation_none, rocblas_operation_none, a.ldb, n, n, &one,
@@ -727,19 +930,17 @@ bool Family::iterate(FamArgs<T>& a, int steps, std::string& err) {
         err = "rocblas_dgemm failed";
         return false;
       }
-    } else {
-      hipLaunchKernelGGL((k_fam_b<T, true>), grid, block, 0, st, a);
-      mark(K_GEMM);
-      if (bf16_gemm_f32(st, np, a.ldb, 3 * np, static_cast<const __bf16*>(Gcat), 3 * np, a.Rcat, a.ldb,
-                        reinterpret_cast<float*>(a.XN), a.ldb, reinterpret_cast<const float*>(a.X)) != hipSuccess) {
-        err = "bf16 gemm launch failed";
-        return false;
-      }
+    } else if (bf16_gemm_f32(st, np, a.ldb, nprod * np, static_cast<const __bf16*>(Gcat), 3 * np, a.Rcat, a.ldb,
+                             reinterpret_cast<float*>(a.XN), a.ldb, reinterpret_cast<const float*>(a.X)) !=
+               hipSuccess) {
+      err = "bf16 gemm launch failed";
+      return false;
     }
     mark(K_AX);
     if (!apply_A(a, a.XN, a.AX)) { err = "A product failed"; return false; }
     mark(K_C);
-    hipLaunchKernelGGL(k_fam_c<T>, grid, block, 0, st, a);
+    hipLaunchKernelGGL(k_fam_c1<T>, dim3(cols, I.nc1), block, 0, st, a, I.c1);
+    if (I.nc2) hipLaunchKernelGGL(k_fam_c2<T>, dim3(cols, I.nc2), block, 0, st, a, I.c2, I.c2_cone);
     mark(-1);
   }
   return hipGetLastError() == hipSuccess;
@@ -749,7 +950,8 @@ bool Family::iterate(FamArgs<T>& a, int steps, std::string& err) {
 template <typename T>
 bool Family::apply_A(FamArgs<T>& a, const T* Xin, T* Z) {
   if (!dense) {
-    hipLaunchKernelGGL((k_fam_spmm<T, 1>), dim3(a.ldb / LANES), dim3(FT), 0, st, m, a.rp, a.ri, a.rx, Xin, Xin,
+    hipLaunchKernelGGL((k_fam_spmm<T, 1>), dim3(a.ldb / LANES, cur_items->nA), dim3(FT), 0, st, cur_items->A, a.rp,
+                       a.ri, a.rx, Xin, Xin,
                        Xin, Z, Z, Z, a.ldb);
     return hipGetLastError() == hipSuccess;
   }
@@ -769,10 +971,12 @@ template <typename T>
 bool Family::apply_At(FamArgs<T>& a, const T* X1, const T* X2, const T* X3, T* Z1, T* Z2, T* Z3) {
   if (!dense) {
     if (X3)
-      hipLaunchKernelGGL((k_fam_spmm<T, 3>), dim3(a.ldb / LANES), dim3(FT), 0, st, n, a.cp, a.ci, a.cx, X1, X2,
+      hipLaunchKernelGGL((k_fam_spmm<T, 3>), dim3(a.ldb / LANES, cur_items->nAt), dim3(FT), 0, st, cur_items->At, a.cp,
+                         a.ci, a.cx, X1, X2,
                          X3, Z1, Z2, Z3, a.ldb);
     else
-      hipLaunchKernelGGL((k_fam_spmm<T, 2>), dim3(a.ldb / LANES), dim3(FT), 0, st, n, a.cp, a.ci, a.cx, X1, X2,
+      hipLaunchKernelGGL((k_fam_spmm<T, 2>), dim3(a.ldb / LANES, cur_items->nAt), dim3(FT), 0, st, cur_items->At, a.cp,
+                         a.ci, a.cx, X1, X2,
                          X2, Z1, Z2, Z2, a.ldb);
     return hipGetLastError() == hipSuccess;
   }
@@ -806,22 +1010,53 @@ bool Family::run_t(const float* warm, long wld, int judge, std::string& err) {
   (void)hipMemcpyAsync(a.dscale, base_ds, (size_t)ld * 8, hipMemcpyDeviceToDevice, st);
   hipLaunchKernelGGL(k_iota, dim3(64), dim3(256), 0, st, ld, a.orig);
   const dim3 block(FT);
+  cur_items = &items[0];
+  a.rpc = RPC;
   hipLaunchKernelGGL(k_fam_init<T>, dim3(ld / LANES), block, 0, st, a, warm, wld);
   if (!apply_A(a, a.X, a.AX)) { err = "A product failed"; return false; }
   if (warm && judge) {
+    cur_items = &items[0];
+    a.rpc = RPC;
+    const int cm = (m + RPC - 1) / RPC, cn = (n + RPC - 1) / RPC;
     a.judge = 1;
-    hipLaunchKernelGGL(k_fam_a<T>, dim3(ld / LANES), block, 0, st, a);
+    hipLaunchKernelGGL(k_fam_a<T>, dim3(ld / LANES, cm), block, 0, st, a);
     if (!apply_At(a, a.Y, a.W, (const T*)nullptr, a.U, a.V, a.Z)) { err = "A' product failed"; return false; }
-    if (fp64) hipLaunchKernelGGL((k_fam_b<T, false>), dim3(ld / LANES), block, 0, st, a);
-    else hipLaunchKernelGGL((k_fam_b<T, true>), dim3(ld / LANES), block, 0, st, a);
+    if (fp64) hipLaunchKernelGGL((k_fam_b<T, false>), dim3(ld / LANES, cn), block, 0, st, a);
+    else hipLaunchKernelGGL((k_fam_b<T, true>), dim3(ld / LANES, cn), block, 0, st, a);
+    hipLaunchKernelGGL(k_fam_decide<T>, dim3((ld + 255) / 256), dim3(256), 0, st, a, cm, cn);
     a.judge = 0;
   }
   std::vector<int> hdone(ldb), list;
-  int chunk = 8;
+  int chunk = 16;
   compactions = 0;
   for (long total = 0; total <= (long)a.max_iters + 1;) {
     const int steps = (int)std::min<long>(chunk, a.max_iters + 2 - total);
-    if (!iterate(a, steps, err)) return false;
+    if (use_graph && !fp64 && !timing) {
+      // replay a captured graph of `steps` iterations for this layout
+      if (!gexec || g_ld != a.ldb || g_steps != steps || g_x != (const void*)a.X) {
+        drop_graph();
+        hipGraph_t g = nullptr;
+        if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+          err = "stream capture failed";
+          return false;
+        }
+        const bool ok = iterate(a, steps, err);
+        if (hipStreamEndCapture(st, &g) != hipSuccess || !ok ||
+            hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0) != hipSuccess) {
+          if (g) (void)hipGraphDestroy(g);
+          gexec = nullptr;
+          if (err.empty()) err = "graph instantiation failed";
+          return false;
+        }
+        (void)hipGraphDestroy(g);
+        g_ld = a.ldb;
+        g_steps = steps;
+        g_x = a.X;
+      }
+      if (hipGraphLaunch(gexec, st) != hipSuccess) { err = "graph launch failed"; return false; }
+    } else if (!iterate(a, steps, err)) {
+      return false;
+    }
     total += steps;
     (void)hipMemcpyAsync(hdone.data(), a.done, (size_t)cnt * 4, hipMemcpyDeviceToHost, st);
     if (hipStreamSynchronize(st) != hipSuccess) { err = "family engine failed"; return false; }
@@ -831,7 +1066,7 @@ bool Family::run_t(const float* warm, long wld, int judge, std::string& err) {
       if (!hdone[t]) list.push_back(t);
     const int active = (int)list.size();
     if (active == 0) break;
-    chunk = std::min(chunk * 2, 32);
+    chunk = 16;
     if (active <= (cnt * 5) / 8 && ld > LANES) {
       // compaction: finished columns to the output store, live ones to the other set
       hipLaunchKernelGGL(k_fam_scatter<T>, dim3(ld / LANES), block, 0, st, a, ldb, static_cast<T*>(Xo),
@@ -954,6 +1189,18 @@ int pdos_family_run(PdosFamily* h, const float* warm_dev, long ld, int judge, do
   return 0;
 }
 
+int pdos_family_info(PdosFamily* h, long* out) {
+  if (!h) return -1;
+  out[0] = h->F.n;
+  out[1] = h->F.m;
+  out[2] = h->F.np;
+  out[3] = h->F.compactions;
+  out[4] = h->F.dense ? 1 : 0;
+  out[5] = h->F.fp64 ? 1 : 0;
+  out[6] = h->F.nprod;
+  return 0;
+}
+
 int pdos_family_fetch(PdosFamily* h, double* X, double* Y, double* S, long* iters, long* status,
                       long* fallback, double* resid) {
   std::string err;
@@ -964,14 +1211,16 @@ int pdos_family_fetch(PdosFamily* h, double* X, double* Y, double* S, long* iter
 
 void pdos_family_free(PdosFamily* h) { delete h; }
 
-int pdos_family_timing(PdosFamily* h, int enable, double* ms, long* calls) {
+int pdos_family_timing(PdosFamily* h, int enable, double* ms, long* calls, double* cols) {
   if (!h) return -1;
   pdos_amd::Family& F = h->F;
   for (int k = 0; k < pdos_amd::Family::K_KINDS; ++k) {
     if (ms) ms[k] = F.kind_ms[k];
     if (calls) calls[k] = F.kind_calls[k];
+    if (cols) cols[k] = F.kind_cols[k];
     F.kind_ms[k] = 0;
     F.kind_calls[k] = 0;
+    F.kind_cols[k] = 0;
   }
   F.timing = enable != 0;
   return 0;

@@ -26,7 +26,9 @@ def _bind(L):
     L.pdos_family_free.argtypes = [C.c_void_p]
     L.pdos_family_last_error.restype = C.c_char_p
     L.pdos_family_timing.restype = C.c_int
-    L.pdos_family_timing.argtypes = [C.c_void_p, C.c_int, _dp, _lp]
+    L.pdos_family_timing.argtypes = [C.c_void_p, C.c_int, _dp, _lp, _dp]
+    L.pdos_family_info.restype = C.c_int
+    L.pdos_family_info.argtypes = [C.c_void_p, _lp]
     L._fam_bound = True
 
 
@@ -34,12 +36,12 @@ class Family:
     """Instances sharing the template's A, cones and parameters; each has its
     own c (and optionally b)."""
 
-    def __init__(self, template: Problem, precision: str = "fp64", device: int = 0):
+    def __init__(self, template: Problem, precision: str = "fp64", device: int = 0, bf16_products: int = 0):
         self.L = lib("direct")
         _bind(self.L)
         self.n, self.m = template.n, template.m
         d, k, self._keep = template.to_c()
-        o = options("direct", precision, "auto", device)
+        o = options("direct", precision, "auto", device, bf16_products)
         self.h = self.L.pdos_family_create(C.byref(d), C.byref(k), C.byref(o))
         if not self.h:
             raise RuntimeError(self.L.pdos_family_last_error().decode())
@@ -86,12 +88,20 @@ class Family:
     KINDS = ["phase_a", "At_product", "phase_b", "xupdate_gemm", "A_product", "phase_c"]
 
     def timing(self, enable: bool) -> dict:
-        """Returns per-kernel {kind: (total_ms, calls)} since the last call and
-        switches event timing on or off for later runs."""
+        """Returns per-kernel {kind: (total_ms, calls, columns)} since the last
+        call and switches event timing on or off for later runs."""
         ms = np.zeros(6)
         calls = np.zeros(6, dtype=np.int64)
-        self.L.pdos_family_timing(self.h, int(enable), ms.ctypes.data_as(_dp), calls.ctypes.data_as(_lp))
-        return {k: (float(ms[i]), int(calls[i])) for i, k in enumerate(self.KINDS)}
+        cols = np.zeros(6)
+        self.L.pdos_family_timing(self.h, int(enable), ms.ctypes.data_as(_dp), calls.ctypes.data_as(_lp),
+                                  cols.ctypes.data_as(_dp))
+        return {k: (float(ms[i]), int(calls[i]), float(cols[i])) for i, k in enumerate(self.KINDS)}
+
+    def info(self) -> dict:
+        out = np.zeros(7, dtype=np.int64)
+        self.L.pdos_family_info(self.h, out.ctypes.data_as(_lp))
+        return dict(n=int(out[0]), m=int(out[1]), np=int(out[2]), compactions=int(out[3]),
+                    dense=bool(out[4]), fp64=bool(out[5]), products=int(out[6]))
 
     def __del__(self):
         try:
