@@ -99,3 +99,17 @@ def test_maxiters_zero_returns_warm_start():
     np.testing.assert_allclose(out["x"], prob.x0, rtol=1e-15)
     np.testing.assert_allclose(out["y"], prob.y0, rtol=1e-15)
     np.testing.assert_allclose(out["s"], prob.s0, rtol=1e-15)
+
+
+@pytest.mark.parametrize("plugin", ["indirect", "direct"])
+def test_demo_cli_on_reference_fixture(plugin):
+    """The demo CLI (run_pdos.c semantics) on the reference's own data file."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(pdos_amd.__file__), f"demo_{plugin}_hip")
+    path = os.path.join(G.GOLDEN, "data", "portfolio_test_01_10x100")
+    r = subprocess.run([exe, path], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "status: Solved" in r.stdout
+    # VERBOSE is 1 in the file: the summary row carries the iteration count
+    assert any(line.strip().startswith("20 |") for line in r.stdout.splitlines()), r.stdout
