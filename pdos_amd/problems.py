@@ -172,16 +172,26 @@ def soc_sizes(rng, total: int, count: int, lo: int, hi: int) -> list:
 
 def random_feasible(seed: int = 1, n: int = 100_000, m: int = 300_000, nnz: int = 5_000_000,
                     f: int = 1_000, l: int = 149_000, nsoc: int = 2_000, soc_lo: int = 10,
-                    soc_hi: int = 140, **params) -> Problem:
+                    soc_hi: int = 140, dense_rows: int = 0, dense_cols: int = 0, **params) -> Problem:
     """Random sparse SOCP with a planted optimal pair.  Entries N(0,1) at
-    uniform random (i, j) (duplicates summed).  x* ~ N(0,1), z ~ N(0,1),
-    s* = Pi_K(z), y* = Pi_K(z) - z, b = A x* + s*, c = -A' y*; then (x*, s*, y*)
-    is primal-dual optimal with c'x* = -b'y*."""
+    uniform random (i, j) (duplicates summed), plus `dense_rows` full rows
+    (entries N(0,1)/sqrt(n)) and `dense_cols` full columns (N(0,1)/sqrt(m)) at
+    random positions.
+    x* ~ N(0,1), z ~ N(0,1), s* = Pi_K(z), y* = Pi_K(z) - z, b = A x* + s*,
+    c = -A' y*; then (x*, s*, y*) is primal-dual optimal with c'x* = -b'y*."""
     rng = np.random.default_rng(seed)
     q = soc_sizes(rng, m - f - l, nsoc, soc_lo, soc_hi) if nsoc > 0 else []
     rows = rng.integers(0, m, size=nnz)
     cols = rng.integers(0, n, size=nnz)
     vals = rng.standard_normal(nnz)
+    if dense_rows or dense_cols:
+        rd = np.random.default_rng(seed + 7919)
+        dr = rd.choice(m, size=dense_rows, replace=False)
+        dc = rd.choice(n, size=dense_cols, replace=False)
+        rows = np.concatenate([rows, np.repeat(dr, n), np.tile(np.arange(m), dense_cols)])
+        cols = np.concatenate([cols, np.tile(np.arange(n), dense_rows), np.repeat(dc, m)])
+        vals = np.concatenate([vals, rd.standard_normal(dense_rows * n) / np.sqrt(n),
+                               rd.standard_normal(dense_cols * m) / np.sqrt(m)])
     A = sp.csc_matrix((vals, (rows, cols)), shape=(m, n))
     A.sum_duplicates()
     A.sort_indices()

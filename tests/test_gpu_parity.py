@@ -46,6 +46,22 @@ def test_grid_fp64_vs_reference(tag):
     _check_fp64(out, G.expected(tag, "indirect"), f"{tag}/grid")
 
 
+def test_grid_long_rows_vs_oracle():
+    """A full row (6,000 entries) and a full column (9,000 entries): both
+    exceed one SpMV row block and take the whole-workgroup row path.  This
+    instance is well conditioned for the bar: solving it in the oracle with
+    the columns permuted (other summation orders) moves x by ~2e-8 and keeps
+    the iteration count."""
+    prob = problems.random_feasible(seed=6, n=6000, m=9000, nnz=30000, f=100, l=4900, nsoc=40,
+                                    soc_lo=10, soc_hi=190, dense_rows=1, dense_cols=1,
+                                    EPS_ABS=1e-3, MAX_ITERS=2000)
+    print("case random_long_rows grid", flush=True)
+    exp = oracle.solve(prob, "indirect")
+    out = pdos_amd.solve_problem(prob, plugin="indirect", precision="fp64", engine="grid")
+    _check_fp64(out, exp, "long_rows/grid")
+    assert out["info"]["cg_iters"] == exp["cg_iters"]
+
+
 @pytest.mark.parametrize("engine", ["wg", "grid"])
 @pytest.mark.parametrize("tag", ["fixture01", "portfolio_n101_norm0", "random_n300_m900", "uneven_1"])
 def test_mixed_precision_within_eps(tag, engine):
