@@ -64,7 +64,6 @@ struct FamArgs {
   double* pa;          // phase A partials [chunk][ldb][2]
   double* pb;          // phase B partials [chunk][ldb][2]
   double* pc;          // phase C cone-chunk partial norms [item][ldb]
-  int rpc;             // rows per chunk of phases A and B (coarse RPC or fine RPC_FINE)
   double* pt0;         // phase C first entry of each cone [cone][ldb]
   T* X; T* XN; T* S; T* Y; T* AX; T* W; T* U; T* V; T* R; T* Z;
   __bf16* Rcat;        // mixed: [D_hi; D_lo; D_hi] of the correction D, 3 np x ldb
@@ -126,24 +125,31 @@ __global__ __launch_bounds__(FT) void k_fam_init(FamArgs<T> a, const float* __re
   }
 }
 
-// Rows [r0, r1) of chunk blockIdx.y: chunks of RPC rows give every launch
-// (instance blocks) x (row chunks) workgroups, so a compacted batch of a few
-// hundred instances still fills the GPU.
-constexpr int RPC = 64;
-constexpr int RPC_FINE = 8;  // one row per wave: small (compacted) batches
-
-__device__ __forceinline__ void chunk_rows(int nrows, int rpc, int& r0, int& r1) {
-  r0 = blockIdx.y * rpc;
-  r1 = min(nrows, r0 + rpc);
+// Wave-uniform read of lane l's value (l uniform).
+__device__ __forceinline__ int lane_of(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ float lane_of(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double lane_of(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+  const long long hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double((hi << 32) | lo);
 }
 
-// Y_k = M X_k (k < NR) for a shared sparse M (rows ptr/idx/val) and
-// feature-major X_k, up to three right-hand sides per pass over M.  Work
-// items (host-built, one per blockIdx.y) balance the rows: item (r0, r1, 0)
-// is a run of short rows, one row per wave; item (r, r + 1, 1) is one long
-// row whose nonzeros the block's waves split, reduced in LDS.
+// Y_k = M X_k (k < NR) for a shared sparse M in CSR (idx, val, row of each
+// entry) and feature-major X_k, up to three right-hand sides per pass.
+// Work is host-built per wave: task {r0, r1, q0, q1} computes the complete
+// rows [r0, r1) from the entries [q0, q1) (at most 64 of them); a long row is
+// split over the 8 waves of its block, every task {r, -1, q0, q1}, and the
+// block sums the waves' partials.  A wave loads up to 64 entries' (column,
+// value, row) in one coalesced load per array, then issues the gathers of
+// kUnroll entries before using any of them, so a task costs a few memory
+// latencies however short its rows are.  Each row is summed in entry order.
+constexpr int kUnroll = 16;
+
 template <typename T, int NR>
-__global__ __launch_bounds__(FT) void k_fam_spmm(const int4* __restrict__ items, const int* __restrict__ ptr,
+__global__ __launch_bounds__(FT) void k_fam_spmm(const int4* __restrict__ tasks, const int* __restrict__ rowof,
                                                  const int* __restrict__ idx, const T* __restrict__ val,
                                                  const T* __restrict__ X1, const T* __restrict__ X2,
                                                  const T* __restrict__ X3, T* __restrict__ Y1,
@@ -151,47 +157,82 @@ __global__ __launch_bounds__(FT) void k_fam_spmm(const int4* __restrict__ items,
   __shared__ double buf[NWV * LANES];
   const int lane = threadIdx.x & 63, w = wave_id();
   const int inst = blockIdx.x * LANES + lane;
-  const int4 it = items[blockIdx.y];
-  if (it.z == 0) {
-    for (int r = it.x + w; r < it.y; r += NWV) {
-      T a1 = T(0), a2 = T(0), a3 = T(0);
-      const int pe = ptr[r + 1];
-      for (int p = ptr[r]; p < pe; ++p) {
-        const T v = val[p];
-        const size_t o = (size_t)idx[p] * ldb + inst;
-        a1 += v * X1[o];
-        if (NR > 1) a2 += v * X2[o];
-        if (NR > 2) a3 += v * X3[o];
+  const int4 tk = tasks[blockIdx.y * NWV + w];
+  const bool split = tasks[blockIdx.y * NWV].y < 0;  // block-uniform
+  T acc[NR];
+#pragma unroll
+  for (int k = 0; k < NR; ++k) acc[k] = T(0);
+  int cur = tk.x;
+  auto put = [&](int row, const T (&v)[NR]) {
+    const size_t o = (size_t)row * ldb + inst;
+    Y1[o] = v[0];
+    if (NR > 1) Y2[o] = v[1];
+    if (NR > 2) Y3[o] = v[2];
+  };
+  auto zero_rows = [&](int from, int to) {
+    const T z[NR] = {};
+    for (int r = from; r < to; ++r) put(r, z);
+  };
+  for (int q = tk.z; q < tk.w; q += 64) {
+    const int cnt = min(64, tk.w - q);
+    int ci = 0, ri = 0;
+    T vi = T(0);
+    if (lane < cnt) {
+      ci = idx[q + lane];
+      vi = val[q + lane];
+      if (!split) ri = rowof[q + lane];
+    }
+    for (int e0 = 0; e0 < cnt; e0 += kUnroll) {
+      // entries past cnt repeat entry cnt - 1: in bounds, not accumulated
+      T xv[kUnroll][NR], vv[kUnroll];
+      int cc[kUnroll], rr[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const int e = min(e0 + u, cnt - 1);
+        cc[u] = lane_of(ci, e);
+        vv[u] = lane_of(vi, e);
+        rr[u] = split ? cur : lane_of(ri, e);
       }
-      const size_t q = (size_t)r * ldb + inst;
-      Y1[q] = a1;
-      if (NR > 1) Y2[q] = a2;
-      if (NR > 2) Y3[q] = a3;
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const size_t o = (size_t)cc[u] * ldb + inst;
+        xv[u][0] = X1[o];
+        if (NR > 1) xv[u][1] = X2[o];
+        if (NR > 2) xv[u][2] = X3[o];
+      }
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        if (e0 + u < cnt) {
+          if (rr[u] != cur) {  // wave-uniform: the previous row is complete
+            put(cur, acc);
+            zero_rows(cur + 1, rr[u]);
+#pragma unroll
+            for (int k = 0; k < NR; ++k) acc[k] = T(0);
+            cur = rr[u];
+          }
+#pragma unroll
+          for (int k = 0; k < NR; ++k) acc[k] += vv[u] * xv[u][k];
+        }
+      }
+    }
+  }
+  if (!split) {
+    if (tk.x < tk.y) {
+      put(cur, acc);
+      zero_rows(cur + 1, tk.y);
     }
     return;
   }
-  const int r = it.x;
-  T a1 = T(0), a2 = T(0), a3 = T(0);
-  const int pe = ptr[r + 1];
-  for (int p = ptr[r] + w; p < pe; p += NWV) {
-    const T v = val[p];
-    const size_t o = (size_t)idx[p] * ldb + inst;
-    a1 += v * X1[o];
-    if (NR > 1) a2 += v * X2[o];
-    if (NR > 2) a3 += v * X3[o];
-  }
-  const size_t q = (size_t)r * ldb + inst;
-  const T t1 = T(waves_sum(double(a1), buf));
-  if (w == 0) Y1[q] = t1;
-  if (NR > 1) {
-    const T t2 = T(waves_sum(double(a2), buf));
-    if (w == 0) Y2[q] = t2;
-  }
-  if (NR > 2) {
-    const T t3 = T(waves_sum(double(a3), buf));
-    if (w == 0) Y3[q] = t3;
-  }
+  T tot[NR];
+#pragma unroll
+  for (int k = 0; k < NR; ++k) tot[k] = T(waves_sum(double(acc[k]), buf));
+  if (w == 0) put(tk.x, tot);
 }
+
+// Phase kernels work on chunks of RPC rows, RW rows per wave: every wave
+// issues the loads of all its rows before it computes or stores anything.
+constexpr int RPC = 64;
+constexpr int RW = RPC / NWV;
 
 // Phase A (one row chunk of the m rows): W = b - S - lam Y, and the chunk's
 // partial |(S - b + AX) ./ D|^2 and b'Y per instance.
@@ -201,18 +242,28 @@ __global__ __launch_bounds__(FT) void k_fam_a(FamArgs<T> a) {
   const int lane = threadIdx.x & 63, w = wave_id();
   const int inst = blockIdx.x * LANES + lane;
   const bool act = inst < a.count && !a.done[inst];
-  int r0, r1;
-  chunk_rows(a.m, a.rpc, r0, r1);
+  const int r0 = blockIdx.y * RPC, r1 = min(a.m, r0 + RPC);
   double p2 = 0.0, by = 0.0;
   if (act) {
+    T sv[RW], yv[RW], av[RW], bv[RW];
+#pragma unroll
+    for (int k = 0; k < RW; ++k) {
+      const int j = r0 + w + k * NWV;
+      if (j < r1) {
+        const size_t o = (size_t)j * a.ldb + inst;
+        sv[k] = a.S[o]; yv[k] = a.Y[o]; av[k] = a.AX[o]; bv[k] = bval(a, j, inst);
+      }
+    }
     const T lam = T(a.lam[inst]);
-    for (int j = r0 + w; j < r1; j += NWV) {
-      const size_t o = (size_t)j * a.ldb + inst;
-      const T s = a.S[o], y = a.Y[o], bj = bval(a, j, inst);
-      const T r = ((s - bj) + a.AX[o]) / T(a.D[j]);
-      p2 += double(r) * double(r);
-      by += double(bj) * double(y);
-      a.W[o] = bj - (s + lam * y);
+#pragma unroll
+    for (int k = 0; k < RW; ++k) {
+      const int j = r0 + w + k * NWV;
+      if (j < r1) {
+        const T r = ((sv[k] - bv[k]) + av[k]) / T(a.D[j]);
+        p2 += double(r) * double(r);
+        by += double(bv[k]) * double(yv[k]);
+        a.W[(size_t)j * a.ldb + inst] = bv[k] - (sv[k] + lam * yv[k]);
+      }
     }
   }
   p2 = waves_sum(p2, buf);
@@ -235,27 +286,41 @@ __global__ __launch_bounds__(FT) void k_fam_b(FamArgs<T> a) {
   const int lane = threadIdx.x & 63, w = wave_id();
   const int inst = blockIdx.x * LANES + lane;
   const bool act = inst < a.count && !a.done[inst];
-  int r0, r1;
-  chunk_rows(a.n, a.rpc, r0, r1);
+  const int r0 = blockIdx.y * RPC, r1 = min(a.n, r0 + RPC);
   double d2 = 0.0, cx = 0.0;
   if (act) {
+    T cv[RW], xv[RW], uv[RW], vv[RW], zv[RW];
+#pragma unroll
+    for (int k = 0; k < RW; ++k) {
+      const int i = r0 + w + k * NWV;
+      if (i < r1) {
+        const size_t o = (size_t)i * a.ldb + inst;
+        cv[k] = a.c[o]; xv[k] = a.X[o]; uv[k] = a.U[o];
+        if (!a.judge) {
+          vv[k] = a.V[o];
+          if (MIXED) zv[k] = a.Z[o];
+        }
+      }
+    }
     const T lam = T(a.lam[inst]);
-    for (int i = r0 + w; i < r1; i += NWV) {
-      const size_t o = (size_t)i * a.ldb + inst;
-      const T ci = a.c[o], xi = a.X[o];
-      const T r = (ci + a.U[o]) / T(a.E[i]);
-      d2 += double(r) * double(r);
-      cx += double(ci) * double(xi);
-      if (a.judge) continue;
-      const T rv = MIXED ? (a.V[o] - a.Z[o]) - lam * ci : xi - lam * ci + a.V[o];
-      if (MIXED) {
-        const __bf16 hi = (__bf16)float(rv);
-        const __bf16 lo = (__bf16)(float(rv) - float(hi));
-        a.Rcat[(size_t)i * a.ldb + inst] = hi;
-        a.Rcat[(size_t)(a.np + i) * a.ldb + inst] = lo;
-        a.Rcat[(size_t)(2 * a.np + i) * a.ldb + inst] = hi;
-      } else {
-        a.R[o] = rv;
+#pragma unroll
+    for (int k = 0; k < RW; ++k) {
+      const int i = r0 + w + k * NWV;
+      if (i < r1) {
+        const T r = (cv[k] + uv[k]) / T(a.E[i]);
+        d2 += double(r) * double(r);
+        cx += double(cv[k]) * double(xv[k]);
+        if (a.judge) continue;
+        const T rv = MIXED ? (vv[k] - zv[k]) - lam * cv[k] : xv[k] - lam * cv[k] + vv[k];
+        if (MIXED) {
+          const __bf16 hi = (__bf16)float(rv);
+          const __bf16 lo = (__bf16)(float(rv) - float(hi));
+          a.Rcat[(size_t)i * a.ldb + inst] = hi;
+          a.Rcat[(size_t)(a.np + i) * a.ldb + inst] = lo;
+          a.Rcat[(size_t)(2 * a.np + i) * a.ldb + inst] = hi;
+        } else {
+          a.R[(size_t)i * a.ldb + inst] = rv;
+        }
       }
     }
   }
@@ -277,15 +342,17 @@ __global__ __launch_bounds__(256) void k_fam_decide(FamArgs<T> a, int chunks_m, 
   const int inst = blockIdx.x * blockDim.x + threadIdx.x;
   if (inst >= a.count || a.done[inst]) return;
   double p2 = 0.0, by = 0.0, d2 = 0.0, cx = 0.0;
+#pragma unroll 16
   for (int c = 0; c < chunks_m; ++c) {
-    const double* pa = a.pa + ((size_t)c * a.ldb + inst) * 2;
-    p2 += pa[0];
-    by += pa[1];
+    const double2 pa = *reinterpret_cast<const double2*>(a.pa + ((size_t)c * a.ldb + inst) * 2);
+    p2 += pa.x;
+    by += pa.y;
   }
+#pragma unroll 16
   for (int c = 0; c < chunks_n; ++c) {
-    const double* pb = a.pb + ((size_t)c * a.ldb + inst) * 2;
-    d2 += pb[0];
-    cx += pb[1];
+    const double2 pb = *reinterpret_cast<const double2*>(a.pb + ((size_t)c * a.ldb + inst) * 2);
+    d2 += pb.x;
+    cx += pb.y;
   }
   const double pres = sqrt(p2), dres = sqrt(d2), pobj = cx, dobj = -by;
   const double eta = fabs(pobj - dobj);
@@ -318,7 +385,8 @@ __global__ __launch_bounds__(256) void k_fam_decide(FamArgs<T> a, int chunks_m, 
   }
 }
 
-// Phase C, first pass.  Work items (one per blockIdx.y, host-built):
+// Phase C, first pass.  Work items (one per blockIdx.y, host-built, at most
+// RPC rows each):
 //   (i0, i1, 0, -)  rows of X = XN
 //   (j0, j1, 1, -)  zero / nonneg rows: s~ = b - AX, relax, project, dual update
 //   (j0, j1, 2, g)  a chunk of second-order cone rows (inside one cone):
@@ -330,44 +398,58 @@ __global__ __launch_bounds__(FT) void k_fam_c1(FamArgs<T> a, const int4* __restr
   const int lane = threadIdx.x & 63, w = wave_id();
   const int inst = blockIdx.x * LANES + lane;
   const bool act = inst < a.count && !a.done[inst];
-  const T lam = act ? T(a.lam[inst]) : T(1);
-  const T ra = T(a.relax), rb = T(1.0) - T(a.relax);
   const int4 it = items[blockIdx.y];
   if (it.z == 0) {
     if (!act) return;
-    for (int i = it.x + w; i < it.y; i += NWV) {
-      const size_t o = (size_t)i * a.ldb + inst;
-      a.X[o] = a.XN[o];
+    T xv[RW];
+#pragma unroll
+    for (int k = 0; k < RW; ++k) {
+      const int i = it.x + w + k * NWV;
+      if (i < it.y) xv[k] = a.XN[(size_t)i * a.ldb + inst];
+    }
+#pragma unroll
+    for (int k = 0; k < RW; ++k) {
+      const int i = it.x + w + k * NWV;
+      if (i < it.y) a.X[(size_t)i * a.ldb + inst] = xv[k];
     }
     return;
   }
-  if (it.z == 1) {
-    if (!act) return;
-    for (int j = it.x + w; j < it.y; j += NWV) {
-      const size_t o = (size_t)j * a.ldb + inst;
-      T st = bval(a, j, inst) - a.AX[o];
-      st = ra * st + rb * a.S[o];
-      const T z = st - lam * a.Y[o];
-      const T s = j < a.f ? T(0) : (z < T(0) ? T(0) : z);
-      a.S[o] = s;
-      a.Y[o] += (s - st) / lam;
-    }
-    return;
-  }
-  const int cone_r0 = a.soc_ptr[it.w];
+  const T ra = T(a.relax), rb = T(1.0) - T(a.relax);
+  const int cone = it.w, cone_r0 = it.z == 2 ? a.soc_ptr[cone] : 0;
   double n2 = 0.0;
   if (act) {
-    for (int j = it.x + w; j < it.y; j += NWV) {
-      const size_t o = (size_t)j * a.ldb + inst;
-      T st = bval(a, j, inst) - a.AX[o];
-      st = ra * st + rb * a.S[o];
-      const T z = st - lam * a.Y[o];
-      a.W[o] = st;  // s~ for the dual update
-      a.S[o] = z;
-      if (j > cone_r0) n2 += double(z) * double(z);
-      else a.pt0[(size_t)it.w * a.ldb + inst] = double(z);  // t0, read by every chunk of the cone
+    T bv[RW], av[RW], sv[RW], yv[RW];
+#pragma unroll
+    for (int k = 0; k < RW; ++k) {
+      const int j = it.x + w + k * NWV;
+      if (j < it.y) {
+        const size_t o = (size_t)j * a.ldb + inst;
+        bv[k] = bval(a, j, inst); av[k] = a.AX[o]; sv[k] = a.S[o]; yv[k] = a.Y[o];
+      }
+    }
+    const T lam = T(a.lam[inst]);
+#pragma unroll
+    for (int k = 0; k < RW; ++k) {
+      const int j = it.x + w + k * NWV;
+      if (j < it.y) {
+        const size_t o = (size_t)j * a.ldb + inst;
+        T st = bv[k] - av[k];
+        st = ra * st + rb * sv[k];
+        const T z = st - lam * yv[k];
+        if (it.z == 1) {
+          const T s = j < a.f ? T(0) : (z < T(0) ? T(0) : z);
+          a.S[o] = s;
+          a.Y[o] = yv[k] + (s - st) / lam;
+        } else {
+          a.W[o] = st;  // s~ for the dual update
+          a.S[o] = z;
+          if (j > cone_r0) n2 += double(z) * double(z);
+          else a.pt0[(size_t)cone * a.ldb + inst] = double(z);  // t0, read by every chunk of the cone
+        }
+      }
     }
   }
+  if (it.z == 1) return;
   n2 = waves_sum(n2, buf);
   if (act && w == 0) a.pc[(size_t)blockIdx.y * a.ldb + inst] = n2;
 }
@@ -385,7 +467,17 @@ __global__ __launch_bounds__(FT) void k_fam_c2(FamArgs<T> a, const int4* __restr
   if (inst >= a.count || a.done[inst]) return;
   const int4 it = items[blockIdx.y];
   const int cn = cone[blockIdx.y];
+  T zv[RW], wv[RW], yv[RW];
+#pragma unroll
+  for (int k = 0; k < RW; ++k) {
+    const int j = it.x + w + k * NWV;
+    if (j < it.y) {
+      const size_t o = (size_t)j * a.ldb + inst;
+      zv[k] = a.S[o]; wv[k] = a.W[o]; yv[k] = a.Y[o];
+    }
+  }
   double n2 = 0.0;
+#pragma unroll 8
   for (int g = it.z; g < it.z + it.w; ++g) n2 += a.pc[(size_t)g * a.ldb + inst];
   const int r0 = a.soc_ptr[cn];
   const T lam = T(a.lam[inst]);
@@ -393,12 +485,16 @@ __global__ __launch_bounds__(FT) void k_fam_c2(FamArgs<T> a, const int4* __restr
   const T t0 = T(a.pt0[(size_t)cn * a.ldb + inst]);
   const int mode = ns <= t0 ? 0 : (ns <= -t0 ? 1 : 2);
   const T al = (ns + t0) / T(2);
-  for (int j = it.x + w; j < it.y; j += NWV) {
-    const size_t o = (size_t)j * a.ldb + inst;
-    const T z = a.S[o];
-    const T sv = mode == 0 ? z : (mode == 1 ? T(0) : (j == r0 ? al : al * z / ns));
-    a.Y[o] += (sv - a.W[o]) / lam;
-    a.S[o] = sv;
+#pragma unroll
+  for (int k = 0; k < RW; ++k) {
+    const int j = it.x + w + k * NWV;
+    if (j < it.y) {
+      const size_t o = (size_t)j * a.ldb + inst;
+      const T z = zv[k];
+      const T sv = mode == 0 ? z : (mode == 1 ? T(0) : (j == r0 ? al : al * z / ns));
+      a.Y[o] = yv[k] + (sv - wv[k]) / lam;
+      a.S[o] = sv;
+    }
   }
 }
 
@@ -551,14 +647,18 @@ struct Family {
     g_ld = g_steps = -1;
     g_x = nullptr;
   }
-  // work-item tables (device), built once per family
-  struct Items {
-    int4 *A = nullptr, *At = nullptr, *c1 = nullptr, *c2 = nullptr;
-    int* c2_cone = nullptr;
-    int nA = 0, nAt = 0, nc1 = 0, nc2 = 0;
-    int rpc = RPC;
-  } items[2];  // [0] coarse, [1] fine
-  const Items* cur_items = &items[0];
+  // work tables (device), built once per family: SpMM wave tasks for A and
+  // A' at two granularities ([0] up to 64 entries per task, [1] up to 16, for
+  // small compacted batches) and the phase C items (RPC rows each)
+  struct Tasks {
+    int4 *A = nullptr, *At = nullptr;
+    int nA = 0, nAt = 0;  // blocks (NWV tasks each)
+  } tasks[2];
+  const Tasks* cur_tasks = &tasks[0];
+  int *rowA = nullptr, *rowAt = nullptr;  // row of every CSR entry of A, of A'
+  int4 *c1 = nullptr, *c2 = nullptr;
+  int* c2_cone = nullptr;
+  int nc1 = 0, nc2 = 0;
   double last_ms = 0;
   // optional per-kernel timing: HIP events around every launch of the loop
   bool timing = false;
@@ -646,34 +746,48 @@ struct Family {
   bool apply_At(FamArgs<T>& a, const T* X1, const T* X2, const T* X3, T* Z1, T* Z2, T* Z3);
 };
 
-// Rows of a compressed matrix into SpMM work items: a row with more than
-// kLong nonzeros is an item of its own (its block's waves split the row);
-// other rows are grouped in runs of at most RPC rows and kRunNnz nonzeros.
-static std::vector<int4> spmm_items(const std::vector<int>& ptr, int rows_per_run) {
-  constexpr int kLong = 64;
-  const int kRunNnz = rows_per_run * 8;
-  std::vector<int4> items;
+// Rows of a CSR matrix into SpMM wave tasks, NWV per block (k_fam_spmm):
+// runs of complete rows of at most `te` entries (and 256 rows), or, for a
+// row longer than `te`, one block whose NWV tasks split the row's entries.
+static std::vector<int4> spmm_tasks(const std::vector<int>& ptr, int te) {
+  std::vector<int4> out, cur;
+  auto flush = [&]() {
+    if (cur.empty()) return;
+    while (cur.size() < (size_t)NWV) cur.push_back(int4{0, 0, 0, 0});
+    out.insert(out.end(), cur.begin(), cur.end());
+    cur.clear();
+  };
   const int nrows = (int)ptr.size() - 1;
   int r = 0;
   while (r < nrows) {
     const int len = ptr[r + 1] - ptr[r];
-    if (len > kLong) {
-      items.push_back(int4{r, r + 1, 1, 0});
+    if (len > te) {
+      flush();
+      const int part = (len + NWV - 1) / NWV;
+      for (int k = 0; k < NWV; ++k) {
+        const int q0 = std::min(ptr[r + 1], ptr[r] + k * part), q1 = std::min(ptr[r + 1], q0 + part);
+        out.push_back(int4{r, -1, q0, q1});
+      }
       ++r;
       continue;
     }
-    int e = r, nz = 0;
-    while (e < nrows && e - r < rows_per_run) {
-      const int l = ptr[e + 1] - ptr[e];
-      if (l > kLong || (e > r && nz + l > kRunNnz)) break;
-      nz += l;
-      ++e;
-    }
-    items.push_back(int4{r, e, 0, 0});
+    int e = r;
+    while (e < nrows && e - r < 256 && ptr[e + 1] - ptr[r] <= te) ++e;
+    cur.push_back(int4{r, e, ptr[r], ptr[e]});
+    if (cur.size() == (size_t)NWV) flush();
     r = e;
   }
-  if (items.empty()) items.push_back(int4{0, 0, 0, 0});
-  return items;
+  flush();
+  if (out.empty()) out.assign(NWV, int4{0, 0, 0, 0});
+  return out;
+}
+
+static std::vector<int> entry_rows(const std::vector<int>& ptr) {
+  std::vector<int> rows(ptr.empty() ? 0 : ptr.back());
+  for (size_t r = 0; r + 1 < ptr.size(); ++r)
+    for (int q = ptr[r]; q < ptr[r + 1]; ++q) rows[q] = (int)r;
+  if (rows.empty()) rows.push_back(0);
+  return rows;
 }
 
 bool Family::create(const Data* d, const Cone* k, const PdosOptions& o, std::string& err) {
@@ -704,34 +818,40 @@ bool Family::create(const Data* d, const Cone* k, const PdosOptions& o, std::str
   bshared = upload_vals(S.b, err);
   if (!err.empty()) return false;
   for (int gi = 0; gi < 2; ++gi) {
-    Items& I = items[gi];
-    const int R = gi == 0 ? RPC : RPC_FINE;
-    I.rpc = R;
-    const std::vector<int4> iA = spmm_items(S.rp, R), iAt = spmm_items(S.cp, R);
-    std::vector<int4> c1, c2;
-    std::vector<int> c2c;
-    for (int i = 0; i < n; i += R) c1.push_back(int4{i, std::min(n, i + R), 0, 0});
-    for (int j = 0; j < S.f + S.l; j += R) c1.push_back(int4{j, std::min(S.f + S.l, j + R), 1, 0});
+    Tasks& T = tasks[gi];
+    const int te = gi == 0 ? 64 : 16;
+    const std::vector<int4> tA = spmm_tasks(S.rp, te), tAt = spmm_tasks(S.cp, te);
+    T.nA = (int)tA.size() / NWV;
+    T.nAt = (int)tAt.size() / NWV;
+    T.A = static_cast<int4*>(upload(tA, err));
+    T.At = static_cast<int4*>(upload(tAt, err));
+  }
+  rowA = static_cast<int*>(upload(entry_rows(S.rp), err));
+  rowAt = static_cast<int*>(upload(entry_rows(S.cp), err));
+  {
+    std::vector<int4> v1, v2;
+    std::vector<int> vc;
+    for (int i = 0; i < n; i += RPC) v1.push_back(int4{i, std::min(n, i + RPC), 0, 0});
+    for (int j = 0; j < S.f + S.l; j += RPC) v1.push_back(int4{j, std::min(S.f + S.l, j + RPC), 1, 0});
     for (int cn = 0; cn < S.nsoc(); ++cn) {
       const int r0 = S.soc_ptr[cn], r1 = S.soc_ptr[cn + 1];
-      const int g0 = (int)c1.size();
-      for (int j = r0; j < r1; j += R) c1.push_back(int4{j, std::min(r1, j + R), 2, cn});
-      const int ng = (int)c1.size() - g0;
+      const int g0 = (int)v1.size();
+      for (int j = r0; j < r1; j += RPC) v1.push_back(int4{j, std::min(r1, j + RPC), 2, cn});
+      const int ng = (int)v1.size() - g0;
       for (int g = g0; g < g0 + ng; ++g) {
-        c2.push_back(int4{c1[g].x, c1[g].y, g0, ng});
-        c2c.push_back(cn);
+        v2.push_back(int4{v1[g].x, v1[g].y, g0, ng});
+        vc.push_back(cn);
       }
     }
-    I.nA = (int)iA.size(); I.nAt = (int)iAt.size(); I.nc1 = (int)c1.size(); I.nc2 = (int)c2.size();
-    I.A = static_cast<int4*>(upload(iA, err));
-    I.At = static_cast<int4*>(upload(iAt, err));
-    I.c1 = static_cast<int4*>(upload(c1, err));
-    if (!c2.empty()) {
-      I.c2 = static_cast<int4*>(upload(c2, err));
-      I.c2_cone = static_cast<int*>(upload(c2c, err));
+    nc1 = (int)v1.size();
+    nc2 = (int)v2.size();
+    c1 = static_cast<int4*>(upload(v1, err));
+    if (!v2.empty()) {
+      c2 = static_cast<int4*>(upload(v2, err));
+      c2_cone = static_cast<int*>(upload(vc, err));
     }
-    if (!err.empty()) return false;
   }
+  if (!err.empty()) return false;
   // M = I + A'A of the scaled A on the device (rocBLAS syrk), then
   // G = M^-1 by Cholesky (rocSOLVER potrf + potri), once per family.
   std::vector<double> Acol((size_t)m * n, 0.0);  // column-major m x n
@@ -845,10 +965,9 @@ bool Family::load(int cnt, const double* C, const double* B, std::string& err) {
     q.ints = dalloc((size_t)4 * ldb * 4, batch_allocs, err);
     q.red = dalloc((size_t)8 * ldb * 8, batch_allocs, err);
     q.orig = dalloc((size_t)ldb * 4, batch_allocs, err);
-    // partials sized for the fine granularity (the most chunks)
-    q.pa = dalloc((size_t)((m + RPC_FINE - 1) / RPC_FINE) * ldb * 2 * 8, batch_allocs, err);
-    q.pb = dalloc((size_t)((n + RPC_FINE - 1) / RPC_FINE) * ldb * 2 * 8, batch_allocs, err);
-    q.pc = dalloc((size_t)std::max(items[0].nc1, items[1].nc1) * ldb * 8, batch_allocs, err);
+    q.pa = dalloc((size_t)((m + RPC - 1) / RPC) * ldb * 2 * 8, batch_allocs, err);
+    q.pb = dalloc((size_t)((n + RPC - 1) / RPC) * ldb * 2 * 8, batch_allocs, err);
+    q.pc = dalloc((size_t)nc1 * ldb * 8, batch_allocs, err);
     q.pt0 = dalloc((size_t)std::max(1, S.nsoc()) * ldb * 8, batch_allocs, err);
   }
   Xo = vec(n); Yo = vec(m); So = vec(m);
@@ -903,12 +1022,9 @@ void Family::fill_args(FamArgs<T>& a) {
 template <typename T>
 bool Family::iterate(FamArgs<T>& a, int steps, std::string& err) {
   const int cols = a.ldb / LANES;
-  // SpMM work items: coarse runs while the batch is large, one row per wave
-  // once it is small; the phase kernels keep coarse chunks (measured faster).
-  const Items& I = items[0];
-  cur_items = &items[(long)cols * items[0].nA < 2048 ? 1 : 0];
-  a.rpc = I.rpc;
-  const int cm = (m + I.rpc - 1) / I.rpc, cn = (n + I.rpc - 1) / I.rpc;
+  // SpMM tasks: 64 entries while the batch is large, 16 once it is small
+  cur_tasks = &tasks[(long)cols * tasks[0].nA < 2048 ? 1 : 0];
+  const int cm = (m + RPC - 1) / RPC, cn = (n + RPC - 1) / RPC;
   const dim3 gm(cols, cm), gn(cols, cn), gd((a.ldb + 255) / 256), block(FT);
   cur_ld = a.ldb;
   for (int s = 0; s < steps; ++s) {
@@ -939,8 +1055,8 @@ bool Family::iterate(FamArgs<T>& a, int steps, std::string& err) {
     mark(K_AX);
     if (!apply_A(a, a.XN, a.AX)) { err = "A product failed"; return false; }
     mark(K_C);
-    hipLaunchKernelGGL(k_fam_c1<T>, dim3(cols, I.nc1), block, 0, st, a, I.c1);
-    if (I.nc2) hipLaunchKernelGGL(k_fam_c2<T>, dim3(cols, I.nc2), block, 0, st, a, I.c2, I.c2_cone);
+    hipLaunchKernelGGL(k_fam_c1<T>, dim3(cols, nc1), block, 0, st, a, c1);
+    if (nc2) hipLaunchKernelGGL(k_fam_c2<T>, dim3(cols, nc2), block, 0, st, a, c2, c2_cone);
     mark(-1);
   }
   return hipGetLastError() == hipSuccess;
@@ -950,9 +1066,8 @@ bool Family::iterate(FamArgs<T>& a, int steps, std::string& err) {
 template <typename T>
 bool Family::apply_A(FamArgs<T>& a, const T* Xin, T* Z) {
   if (!dense) {
-    hipLaunchKernelGGL((k_fam_spmm<T, 1>), dim3(a.ldb / LANES, cur_items->nA), dim3(FT), 0, st, cur_items->A, a.rp,
-                       a.ri, a.rx, Xin, Xin,
-                       Xin, Z, Z, Z, a.ldb);
+    hipLaunchKernelGGL((k_fam_spmm<T, 1>), dim3(a.ldb / LANES, cur_tasks->nA), dim3(FT), 0, st, cur_tasks->A, rowA,
+                       a.ri, a.rx, Xin, Xin, Xin, Z, Z, Z, a.ldb);
     return hipGetLastError() == hipSuccess;
   }
   const T one = T(1), zero = T(0);
@@ -971,13 +1086,11 @@ template <typename T>
 bool Family::apply_At(FamArgs<T>& a, const T* X1, const T* X2, const T* X3, T* Z1, T* Z2, T* Z3) {
   if (!dense) {
     if (X3)
-      hipLaunchKernelGGL((k_fam_spmm<T, 3>), dim3(a.ldb / LANES, cur_items->nAt), dim3(FT), 0, st, cur_items->At, a.cp,
-                         a.ci, a.cx, X1, X2,
-                         X3, Z1, Z2, Z3, a.ldb);
+      hipLaunchKernelGGL((k_fam_spmm<T, 3>), dim3(a.ldb / LANES, cur_tasks->nAt), dim3(FT), 0, st, cur_tasks->At,
+                         rowAt, a.ci, a.cx, X1, X2, X3, Z1, Z2, Z3, a.ldb);
     else
-      hipLaunchKernelGGL((k_fam_spmm<T, 2>), dim3(a.ldb / LANES, cur_items->nAt), dim3(FT), 0, st, cur_items->At, a.cp,
-                         a.ci, a.cx, X1, X2,
-                         X2, Z1, Z2, Z2, a.ldb);
+      hipLaunchKernelGGL((k_fam_spmm<T, 2>), dim3(a.ldb / LANES, cur_tasks->nAt), dim3(FT), 0, st, cur_tasks->At,
+                         rowAt, a.ci, a.cx, X1, X2, X2, Z1, Z2, Z2, a.ldb);
     return hipGetLastError() == hipSuccess;
   }
   const T one = T(1), zero = T(0);
@@ -1010,13 +1123,10 @@ bool Family::run_t(const float* warm, long wld, int judge, std::string& err) {
   (void)hipMemcpyAsync(a.dscale, base_ds, (size_t)ld * 8, hipMemcpyDeviceToDevice, st);
   hipLaunchKernelGGL(k_iota, dim3(64), dim3(256), 0, st, ld, a.orig);
   const dim3 block(FT);
-  cur_items = &items[0];
-  a.rpc = RPC;
+  cur_tasks = &tasks[0];
   hipLaunchKernelGGL(k_fam_init<T>, dim3(ld / LANES), block, 0, st, a, warm, wld);
   if (!apply_A(a, a.X, a.AX)) { err = "A product failed"; return false; }
   if (warm && judge) {
-    cur_items = &items[0];
-    a.rpc = RPC;
     const int cm = (m + RPC - 1) / RPC, cn = (n + RPC - 1) / RPC;
     a.judge = 1;
     hipLaunchKernelGGL(k_fam_a<T>, dim3(ld / LANES, cm), block, 0, st, a);
