@@ -144,34 +144,43 @@ def plain_gpu(probs, plugin, precision):
 
 # ------------------------------------------------------------------ roofline
 
-def family_roofline(kt, fam_info, nnz, fp64):
+def family_roofline(kt, fam_info, nnz, fp64, lin_rows, b_inst):
     """Roofline of the dominant kernel of a family run, from per-kernel HIP
-    event totals {kind: (ms, calls, columns)}.  Algorithmic work per column
-    (instance): A' product  nR right-hand sides read (m rows) and written
-    (n rows) + the shared matrix once per launch; A product  n read, m
-    written; x-update GEMM  2 * np * 3np flops (split-bf16) or 2 n^2 (fp64)."""
+    event totals {kind: (ms, calls, columns)}.  Algorithmic bytes per column
+    (instance), values of sv bytes:
+      A' product  2 right-hand sides: 2m read, 2n written (+ the shared
+                  matrix once per launch: nnz (sv + 4))
+      A product   n read, m written (+ the matrix)
+      phase A     S, Y, AX (+ b) read, W written: 4m (5m)
+      phase B     c, X, U, V read: 4n; R written (3 bf16 planes or sv)
+      phase C     X = XN: 2n; zero/nonneg rows: b, AX, S, Y read, S, Y, W
+                  written (7); cone rows: pass 1 reads b, AX, S, Y, writes
+                  W, S; pass 2 reads S, W, Y, AX, b, writes Y, S, W (14);
+                  b reads drop out when b is shared
+    x-update GEMM: 2 n np flops per product (bf16) or 2 n^2 (fp64)."""
     n, m, npad = fam_info["n"], fam_info["m"], fam_info["np"]
     sv = 8 if fp64 else 4
     kind = max(kt, key=lambda k: kt[k][0])
     ms, calls, cols = kt[kind]
+    cone_rows = m - lin_rows
     if kind == "xupdate_gemm":
-        flops = (2.0 * n * n if fp64 else 2.0 * npad * fam_info.get("products", 3) * npad) * cols
+        flops = (2.0 * n * n if fp64 else 2.0 * npad * fam_info.get("products", 1) * npad) * cols
         ach = flops / (ms * 1e-3) / 1e12
         peak = FP64_PEAK_TFLOPS if fp64 else BF16_PEAK_TFLOPS
         return {"bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
                 "frac": round(ach / peak, 4), "traffic": None, "kernel": kind,
                 "algorithmic_flops_per_launch": flops / max(calls, 1), "avg_launch_us": round(1e3 * ms / max(calls, 1), 2)}
     if kind == "At_product":
-        nr = 2 if fp64 else 3
-        byt = cols * nr * (m + n) * sv + calls * nnz * (sv + 4)
+        byt = cols * 2 * (m + n) * sv + calls * nnz * (sv + 4)
     elif kind == "A_product":
         byt = cols * (m + n) * sv + calls * nnz * (sv + 4)
     elif kind == "phase_a":
-        byt = cols * m * sv * 4
+        byt = cols * m * sv * (5 if b_inst else 4)
     elif kind == "phase_b":
-        byt = cols * n * sv * 5 + cols * n * (2 * 3 if not fp64 else sv)
+        byt = cols * n * sv * 4 + cols * n * (2 * 3 if not fp64 else sv)
     else:
-        byt = cols * (2 * n + 6 * m) * sv
+        rows = 2 * n + (7 if b_inst else 6) * lin_rows + (14 if b_inst else 12) * cone_rows
+        byt = cols * rows * sv
     ach = byt / (ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kind,
@@ -287,7 +296,7 @@ def run_family(args):
         "plain_gpu": plain,
         "ratio_vs_plain_gpu": round(value / plain["value"], 1) if plain else None,
     }
-    roof = family_roofline(kt, info, int(fam.Ap[-1]), prec == "fp64")
+    roof = family_roofline(kt, info, int(fam.Ap[-1]), prec == "fp64", fam.f + fam.l, False)
     acc = {"solved": solved, "count": n_te, "verified_fp64": v_warm,
            "mean_iters": float(warm["iters"].mean()), "max_iters": int(warm["iters"].max())}
     return value, wall, "f32+bf16" if prec == "mixed" else "f64", config, roof, acc, extra, probs[:args.cpu_sample], "direct"
@@ -316,7 +325,7 @@ def run_dense(args):
     gap = np.abs(np.einsum("ij,ij->i", Cm, out["x"]) - opt) / (1 + np.abs(opt))
     config = {"workload": "dense shared-A family", "m": dense.m, "n": dense.n, "count": cnt, "eps_abs": 1e-3,
               "precision": prec, "engine": "family (shared dense A)"}
-    roof = family_roofline(kt, F.info(), dense.m * dense.n, prec == "fp64")
+    roof = family_roofline(kt, F.info(), dense.m * dense.n, prec == "fp64", dense.f + dense.l, True)
     acc = {"solved": solved, "count": cnt, "verified_fp64": ver, "mean_iters": float(out["iters"].mean()),
            "max_iters": int(out["iters"].max()), "max_rel_obj_err_vs_planted": float(gap.max())}
     extra = {"kernels": {k: {"ms": round(v[0], 3), "calls": v[1]} for k, v in kt.items()}}

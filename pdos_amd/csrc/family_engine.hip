@@ -8,14 +8,21 @@
 // the x-update of the direct plugin (direct/private.c:16-56) is one GEMM for
 // the whole batch:
 //
-//   phase A   W = b - S - lam Y; |(S - b + AX) ./ D|^2, b'Y      (per instance)
 //   SpMM      U = A'Y, V = A'W                                   (one pass over A')
-//   phase B   |(c + U) ./ E|^2, c'X, stop test, R = X - lam c + V
+//   phase B   |(c + U) ./ E|^2, c'X, R = X - lam c + V (fp64) or V - lam c (mixed)
+//   decide    stop test per instance
 //   GEMM      XN = G R     (fp64: rocBLAS dgemm)
-//             XN = X + G (V - A'AX - lam c)  (mixed: split-bf16 MFMA, 3 products; the
-//             correction form keeps the bf16 error proportional to the step)
+//             XN = X + G R (mixed: bf16 MFMA; the correction form keeps the bf16
+//             error proportional to the step)
 //   SpMM      AX = A XN
-//   phase C   X = XN; S~ = b - AX; relax; S = Pi_K(S~ - lam Y); Y += (S - S~)/lam
+//   phase C   X = XN; S~ = b - AX; relax; S = Pi_K(S~ - lam Y); Y += (S - S~)/lam;
+//             then, for the next iteration's A' product and stop test, W and
+//             the partials |(S - b + AX) ./ D|^2, b'Y per instance, where
+//             W = b - S - lam Y (fp64) or b - AX - S - lam Y (mixed)
+//
+// Phase A forms W and those partials from scratch; it runs only when the
+// iterate or the layout changed outside the loop (start, warm-start judge,
+// compaction).
 //
 // An instance that meets the stop test (pdos.c:95-107) or MAX_ITERS is frozen:
 // phases A-C skip it, so its iterate is exactly the one the stop test saw.
@@ -52,6 +59,7 @@ template <typename T>
 struct FamArgs {
   int n, m, f, lend, nsoc, ldb, count, np, b_inst;
   int max_iters, judge;
+  int nprod;           // mixed: planes of Rcat the x-update GEMM reads (1-3)
   double eps, relax;
   const int* soc_ptr;
   const int* rp; const int* ri; const T* rx;   // A rows
@@ -65,7 +73,9 @@ struct FamArgs {
   double* pb;          // phase B partials [chunk][ldb][2]
   double* pc;          // phase C cone-chunk partial norms [item][ldb]
   double* pt0;         // phase C first entry of each cone [cone][ldb]
-  T* X; T* XN; T* S; T* Y; T* AX; T* W; T* U; T* V; T* R; T* Z;
+  int nx_items;        // phase C: items of x rows (come first); pa index = item - nx_items
+  int nlin_items;      // phase C: items of zero / nonneg rows (then the cone chunks)
+  T* X; T* XN; T* S; T* Y; T* AX; T* W; T* U; T* V; T* R;
   __bf16* Rcat;        // mixed: [D_hi; D_lo; D_hi] of the correction D, 3 np x ldb
   int* done; int* status; int* iters; int* fallback;
   double* red;         // per instance: pres2, by, pres, dres, pobj, dobj, eta, spare
@@ -84,6 +94,18 @@ __device__ __forceinline__ double waves_sum(double v, double* buf) {
   for (int k = 0; k < NWV; ++k) s += buf[k * LANES + lane];
   __syncthreads();
   return s;
+}
+
+// Mixed precision (binary32 bulk) uses the correction form of the x-update.
+template <typename T>
+constexpr bool kCorr = std::is_same<T, float>::value;
+
+// The next A' right-hand side of row j from the finished (s, y):
+//   fp64:  W = b - (s + lam y)
+//   mixed: W = (b - A x) - (s + lam y), so A'W = A'(b - s - lam y) - A'A x.
+template <typename T>
+__device__ __forceinline__ T next_w(T b, T ax, T s, T y, T lam) {
+  return kCorr<T> ? (b - ax) - (s + lam * y) : b - (s + lam * y);
 }
 
 template <typename T>
@@ -234,15 +256,19 @@ __global__ __launch_bounds__(FT) void k_fam_spmm(const int4* __restrict__ tasks,
 constexpr int RPC = 64;
 constexpr int RW = RPC / NWV;
 
-// Phase A (one row chunk of the m rows): W = b - S - lam Y, and the chunk's
-// partial |(S - b + AX) ./ D|^2 and b'Y per instance.
+// Phase A over the m-row items of phase C (item blockIdx.y of `items`):
+// W (next_w) and the item's partial |(S - b + AX) ./ D|^2 and b'Y per
+// instance.  Inside the loop phase C produces these for the rows it
+// finishes; this kernel runs only after the layout or the iterate changed
+// outside the loop (start, judge, compaction).
 template <typename T>
-__global__ __launch_bounds__(FT) void k_fam_a(FamArgs<T> a) {
+__global__ __launch_bounds__(FT) void k_fam_a(FamArgs<T> a, const int4* __restrict__ items) {
   __shared__ double buf[NWV * LANES];
   const int lane = threadIdx.x & 63, w = wave_id();
   const int inst = blockIdx.x * LANES + lane;
   const bool act = inst < a.count && !a.done[inst];
-  const int r0 = blockIdx.y * RPC, r1 = min(a.m, r0 + RPC);
+  const int4 it = items[blockIdx.y];
+  const int r0 = it.x, r1 = it.y;
   double p2 = 0.0, by = 0.0;
   if (act) {
     T sv[RW], yv[RW], av[RW], bv[RW];
@@ -262,7 +288,7 @@ __global__ __launch_bounds__(FT) void k_fam_a(FamArgs<T> a) {
         const T r = ((sv[k] - bv[k]) + av[k]) / T(a.D[j]);
         p2 += double(r) * double(r);
         by += double(bv[k]) * double(yv[k]);
-        a.W[(size_t)j * a.ldb + inst] = bv[k] - (sv[k] + lam * yv[k]);
+        a.W[(size_t)j * a.ldb + inst] = next_w(bv[k], av[k], sv[k], yv[k], lam);
       }
     }
   }
@@ -276,9 +302,9 @@ __global__ __launch_bounds__(FT) void k_fam_a(FamArgs<T> a) {
 }
 
 // Phase B (one row chunk of the n rows): partial |(c + U) ./ E|^2 and c'X,
-// and the x-update right-hand side of every active instance:
+// and the x-update right-hand side of every active instance (V = A'W):
 //   fp64: R = x - lam c + A'(b - s - lam y), then x+ = G R;
-//   mixed: the correction D = R - (I + A'A) x = A'(b - s - lam y) - lam c - A'(A x),
+//   mixed: the correction D = R - (I + A'A) x = A'(b - s - lam y - A x) - lam c,
 //          then x+ = x + G D, so the bf16 error of the product shrinks with D.
 template <typename T, bool MIXED>
 __global__ __launch_bounds__(FT) void k_fam_b(FamArgs<T> a) {
@@ -289,17 +315,14 @@ __global__ __launch_bounds__(FT) void k_fam_b(FamArgs<T> a) {
   const int r0 = blockIdx.y * RPC, r1 = min(a.n, r0 + RPC);
   double d2 = 0.0, cx = 0.0;
   if (act) {
-    T cv[RW], xv[RW], uv[RW], vv[RW], zv[RW];
+    T cv[RW], xv[RW], uv[RW], vv[RW];
 #pragma unroll
     for (int k = 0; k < RW; ++k) {
       const int i = r0 + w + k * NWV;
       if (i < r1) {
         const size_t o = (size_t)i * a.ldb + inst;
         cv[k] = a.c[o]; xv[k] = a.X[o]; uv[k] = a.U[o];
-        if (!a.judge) {
-          vv[k] = a.V[o];
-          if (MIXED) zv[k] = a.Z[o];
-        }
+        if (!a.judge) vv[k] = a.V[o];
       }
     }
     const T lam = T(a.lam[inst]);
@@ -311,13 +334,12 @@ __global__ __launch_bounds__(FT) void k_fam_b(FamArgs<T> a) {
         d2 += double(r) * double(r);
         cx += double(cv[k]) * double(xv[k]);
         if (a.judge) continue;
-        const T rv = MIXED ? (vv[k] - zv[k]) - lam * cv[k] : xv[k] - lam * cv[k] + vv[k];
+        const T rv = MIXED ? vv[k] - lam * cv[k] : xv[k] - lam * cv[k] + vv[k];
         if (MIXED) {
           const __bf16 hi = (__bf16)float(rv);
-          const __bf16 lo = (__bf16)(float(rv) - float(hi));
           a.Rcat[(size_t)i * a.ldb + inst] = hi;
-          a.Rcat[(size_t)(a.np + i) * a.ldb + inst] = lo;
-          a.Rcat[(size_t)(2 * a.np + i) * a.ldb + inst] = hi;
+          if (a.nprod > 1) a.Rcat[(size_t)(a.np + i) * a.ldb + inst] = (__bf16)(float(rv) - float(hi));
+          if (a.nprod > 2) a.Rcat[(size_t)(2 * a.np + i) * a.ldb + inst] = hi;
         } else {
           a.R[(size_t)i * a.ldb + inst] = rv;
         }
@@ -388,7 +410,9 @@ __global__ __launch_bounds__(256) void k_fam_decide(FamArgs<T> a, int chunks_m, 
 // Phase C, first pass.  Work items (one per blockIdx.y, host-built, at most
 // RPC rows each):
 //   (i0, i1, 0, -)  rows of X = XN
-//   (j0, j1, 1, -)  zero / nonneg rows: s~ = b - AX, relax, project, dual update
+//   (j0, j1, 1, -)  zero / nonneg rows: s~ = b - AX, relax, project, dual
+//                   update; then, for the next iteration, W (next_w) and the
+//                   item's stop-test partials (as phase A)
 //   (j0, j1, 2, g)  a chunk of second-order cone rows (inside one cone):
 //                   s~ and z = s~ - lam y stored, partial |z|^2 (without the
 //                   cone's first row) into pc[g]; the cone's first row keeps t0
@@ -416,7 +440,7 @@ __global__ __launch_bounds__(FT) void k_fam_c1(FamArgs<T> a, const int4* __restr
   }
   const T ra = T(a.relax), rb = T(1.0) - T(a.relax);
   const int cone = it.w, cone_r0 = it.z == 2 ? a.soc_ptr[cone] : 0;
-  double n2 = 0.0;
+  double n2 = 0.0, p2 = 0.0, by = 0.0;
   if (act) {
     T bv[RW], av[RW], sv[RW], yv[RW];
 #pragma unroll
@@ -438,8 +462,13 @@ __global__ __launch_bounds__(FT) void k_fam_c1(FamArgs<T> a, const int4* __restr
         const T z = st - lam * yv[k];
         if (it.z == 1) {
           const T s = j < a.f ? T(0) : (z < T(0) ? T(0) : z);
+          const T y = yv[k] + (s - st) / lam;
           a.S[o] = s;
-          a.Y[o] = yv[k] + (s - st) / lam;
+          a.Y[o] = y;
+          const T r = ((s - bv[k]) + av[k]) / T(a.D[j]);
+          p2 += double(r) * double(r);
+          by += double(bv[k]) * double(y);
+          a.W[o] = next_w(bv[k], av[k], s, y, lam);
         } else {
           a.W[o] = st;  // s~ for the dual update
           a.S[o] = z;
@@ -449,7 +478,16 @@ __global__ __launch_bounds__(FT) void k_fam_c1(FamArgs<T> a, const int4* __restr
       }
     }
   }
-  if (it.z == 1) return;
+  if (it.z == 1) {
+    p2 = waves_sum(p2, buf);
+    by = waves_sum(by, buf);
+    if (act && w == 0) {
+      double* pa = a.pa + ((size_t)(blockIdx.y - a.nx_items) * a.ldb + inst) * 2;
+      pa[0] = p2;
+      pa[1] = by;
+    }
+    return;
+  }
   n2 = waves_sum(n2, buf);
   if (act && w == 0) a.pc[(size_t)blockIdx.y * a.ldb + inst] = n2;
 }
@@ -457,44 +495,61 @@ __global__ __launch_bounds__(FT) void k_fam_c1(FamArgs<T> a, const int4* __restr
 // Phase C, second pass over the cone chunks: the cone's norm is the sum of
 // its chunks' partials (fixed order) and t0 comes from the first pass, so no
 // block depends on another block's writes; project (cones.c:26-47), dual
-// update.  Item (j0, j1, first chunk of the cone, chunks in the cone);
+// update; then W (next_w) and the chunk's stop-test partials for the next
+// iteration.  Item (j0, j1, first chunk of the cone, chunks in the cone);
 // cone[item] is the cone index.
 template <typename T>
 __global__ __launch_bounds__(FT) void k_fam_c2(FamArgs<T> a, const int4* __restrict__ items,
                                                const int* __restrict__ cone) {
+  __shared__ double buf[NWV * LANES];
   const int lane = threadIdx.x & 63, w = wave_id();
   const int inst = blockIdx.x * LANES + lane;
-  if (inst >= a.count || a.done[inst]) return;
+  const bool act = inst < a.count && !a.done[inst];
   const int4 it = items[blockIdx.y];
   const int cn = cone[blockIdx.y];
-  T zv[RW], wv[RW], yv[RW];
+  double p2 = 0.0, by = 0.0;
+  if (act) {
+    T zv[RW], wv[RW], yv[RW], av[RW], bv[RW];
 #pragma unroll
-  for (int k = 0; k < RW; ++k) {
-    const int j = it.x + w + k * NWV;
-    if (j < it.y) {
-      const size_t o = (size_t)j * a.ldb + inst;
-      zv[k] = a.S[o]; wv[k] = a.W[o]; yv[k] = a.Y[o];
+    for (int k = 0; k < RW; ++k) {
+      const int j = it.x + w + k * NWV;
+      if (j < it.y) {
+        const size_t o = (size_t)j * a.ldb + inst;
+        zv[k] = a.S[o]; wv[k] = a.W[o]; yv[k] = a.Y[o]; av[k] = a.AX[o]; bv[k] = bval(a, j, inst);
+      }
+    }
+    double n2 = 0.0;
+#pragma unroll 8
+    for (int g = it.z; g < it.z + it.w; ++g) n2 += a.pc[(size_t)g * a.ldb + inst];
+    const int r0 = a.soc_ptr[cn];
+    const T lam = T(a.lam[inst]);
+    const T ns = T(sqrt(n2));
+    const T t0 = T(a.pt0[(size_t)cn * a.ldb + inst]);
+    const int mode = ns <= t0 ? 0 : (ns <= -t0 ? 1 : 2);
+    const T al = (ns + t0) / T(2);
+#pragma unroll
+    for (int k = 0; k < RW; ++k) {
+      const int j = it.x + w + k * NWV;
+      if (j < it.y) {
+        const size_t o = (size_t)j * a.ldb + inst;
+        const T z = zv[k];
+        const T sv = mode == 0 ? z : (mode == 1 ? T(0) : (j == r0 ? al : al * z / ns));
+        const T y = yv[k] + (sv - wv[k]) / lam;
+        a.Y[o] = y;
+        a.S[o] = sv;
+        const T r = ((sv - bv[k]) + av[k]) / T(a.D[j]);
+        p2 += double(r) * double(r);
+        by += double(bv[k]) * double(y);
+        a.W[o] = next_w(bv[k], av[k], sv, y, lam);
+      }
     }
   }
-  double n2 = 0.0;
-#pragma unroll 8
-  for (int g = it.z; g < it.z + it.w; ++g) n2 += a.pc[(size_t)g * a.ldb + inst];
-  const int r0 = a.soc_ptr[cn];
-  const T lam = T(a.lam[inst]);
-  const T ns = T(sqrt(n2));
-  const T t0 = T(a.pt0[(size_t)cn * a.ldb + inst]);
-  const int mode = ns <= t0 ? 0 : (ns <= -t0 ? 1 : 2);
-  const T al = (ns + t0) / T(2);
-#pragma unroll
-  for (int k = 0; k < RW; ++k) {
-    const int j = it.x + w + k * NWV;
-    if (j < it.y) {
-      const size_t o = (size_t)j * a.ldb + inst;
-      const T z = zv[k];
-      const T sv = mode == 0 ? z : (mode == 1 ? T(0) : (j == r0 ? al : al * z / ns));
-      a.Y[o] = yv[k] + (sv - wv[k]) / lam;
-      a.S[o] = sv;
-    }
+  p2 = waves_sum(p2, buf);
+  by = waves_sum(by, buf);
+  if (act && w == 0) {
+    double* pa = a.pa + ((size_t)(a.nlin_items + blockIdx.y) * a.ldb + inst) * 2;
+    pa[0] = p2;
+    pa[1] = by;
   }
 }
 
@@ -626,7 +681,7 @@ struct Family {
   int count = 0, ldb = 0, b_inst = 0;
   std::vector<void*> batch_allocs;
   struct Set {
-    void *X, *XN, *S, *Y, *AX, *W, *U, *V, *Z, *R, *Rcat, *c, *b, *lam, *ps, *ds, *ints, *red, *orig, *pa, *pb, *pc, *pt0;
+    void *X, *XN, *S, *Y, *AX, *W, *U, *V, *R, *Rcat, *c, *b, *lam, *ps, *ds, *ints, *red, *orig, *pa, *pb, *pc, *pt0;
   } set[2];
   void *base_c = nullptr, *base_b = nullptr, *base_lam = nullptr, *base_ps = nullptr, *base_ds = nullptr;
   void *Xo = nullptr, *Yo = nullptr, *So = nullptr, *Io = nullptr, *Ro = nullptr;
@@ -658,7 +713,9 @@ struct Family {
   int *rowA = nullptr, *rowAt = nullptr;  // row of every CSR entry of A, of A'
   int4 *c1 = nullptr, *c2 = nullptr;
   int* c2_cone = nullptr;
-  int nc1 = 0, nc2 = 0;
+  int nc1 = 0, nc2 = 0, nx_items = 0, nlin_items = 0;  // c1: x items, then zero/nonneg items, then cone chunks
+  int m_items() const { return nc1 - nx_items; }       // stop-test partials over the m rows
+  bool need_prologue = true;                           // W and the partials must be formed by k_fam_a
   double last_ms = 0;
   // optional per-kernel timing: HIP events around every launch of the loop
   bool timing = false;
@@ -743,7 +800,9 @@ struct Family {
   template <typename T>
   bool apply_A(FamArgs<T>& a, const T* Xin, T* Z);
   template <typename T>
-  bool apply_At(FamArgs<T>& a, const T* X1, const T* X2, const T* X3, T* Z1, T* Z2, T* Z3);
+  bool apply_At(FamArgs<T>& a, const T* X1, const T* X2, T* Z1, T* Z2);
+  template <typename T>
+  void prologue(FamArgs<T>& a);
 };
 
 // Rows of a CSR matrix into SpMM wave tasks, NWV per block (k_fam_spmm):
@@ -832,7 +891,9 @@ bool Family::create(const Data* d, const Cone* k, const PdosOptions& o, std::str
     std::vector<int4> v1, v2;
     std::vector<int> vc;
     for (int i = 0; i < n; i += RPC) v1.push_back(int4{i, std::min(n, i + RPC), 0, 0});
+    nx_items = (int)v1.size();
     for (int j = 0; j < S.f + S.l; j += RPC) v1.push_back(int4{j, std::min(S.f + S.l, j + RPC), 1, 0});
+    nlin_items = (int)v1.size() - nx_items;
     for (int cn = 0; cn < S.nsoc(); ++cn) {
       const int r0 = S.soc_ptr[cn], r1 = S.soc_ptr[cn + 1];
       const int g0 = (int)v1.size();
@@ -955,7 +1016,7 @@ bool Family::load(int cnt, const double* C, const double* B, std::string& err) {
   for (int k = 0; k < 2; ++k) {
     Set& q = set[k];
     q.X = vec(np); q.XN = vec(np); q.S = vec(m); q.Y = vec(m); q.AX = vec(m); q.W = vec(m);
-    q.U = vec(n); q.V = vec(n); q.Z = vec(n); q.R = vec(np);
+    q.U = vec(n); q.V = vec(n); q.R = vec(np);
     q.Rcat = fp64 ? nullptr : dalloc((size_t)3 * np * ldb * 2, batch_allocs, err);
     q.c = vec(n);
     q.b = b_inst ? vec(m) : nullptr;
@@ -965,7 +1026,7 @@ bool Family::load(int cnt, const double* C, const double* B, std::string& err) {
     q.ints = dalloc((size_t)4 * ldb * 4, batch_allocs, err);
     q.red = dalloc((size_t)8 * ldb * 8, batch_allocs, err);
     q.orig = dalloc((size_t)ldb * 4, batch_allocs, err);
-    q.pa = dalloc((size_t)((m + RPC - 1) / RPC) * ldb * 2 * 8, batch_allocs, err);
+    q.pa = dalloc((size_t)std::max(1, m_items()) * ldb * 2 * 8, batch_allocs, err);
     q.pb = dalloc((size_t)((n + RPC - 1) / RPC) * ldb * 2 * 8, batch_allocs, err);
     q.pc = dalloc((size_t)nc1 * ldb * 8, batch_allocs, err);
     q.pt0 = dalloc((size_t)std::max(1, S.nsoc()) * ldb * 8, batch_allocs, err);
@@ -992,7 +1053,7 @@ FamArgs<T> Family::args(int k, int ld, int cnt) {
   a.dscale = static_cast<double*>(q.ds);
   a.X = static_cast<T*>(q.X); a.XN = static_cast<T*>(q.XN); a.S = static_cast<T*>(q.S);
   a.Y = static_cast<T*>(q.Y); a.AX = static_cast<T*>(q.AX); a.W = static_cast<T*>(q.W);
-  a.U = static_cast<T*>(q.U); a.V = static_cast<T*>(q.V); a.Z = static_cast<T*>(q.Z);
+  a.U = static_cast<T*>(q.U); a.V = static_cast<T*>(q.V);
   a.R = static_cast<T*>(q.R);
   a.Rcat = static_cast<__bf16*>(q.Rcat);
   int* ip = static_cast<int*>(q.ints);
@@ -1017,6 +1078,9 @@ void Family::fill_args(FamArgs<T>& a) {
   a.rp = static_cast<const int*>(rp); a.ri = static_cast<const int*>(ri); a.rx = static_cast<const T*>(rx);
   a.cp = static_cast<const int*>(cp); a.ci = static_cast<const int*>(ci); a.cx = static_cast<const T*>(cx);
   a.D = static_cast<const double*>(D); a.E = static_cast<const double*>(E);
+  a.nx_items = nx_items;
+  a.nlin_items = nlin_items;
+  a.nprod = nprod;
 }
 
 template <typename T>
@@ -1024,14 +1088,12 @@ bool Family::iterate(FamArgs<T>& a, int steps, std::string& err) {
   const int cols = a.ldb / LANES;
   // SpMM tasks: 64 entries while the batch is large, 16 once it is small
   cur_tasks = &tasks[(long)cols * tasks[0].nA < 2048 ? 1 : 0];
-  const int cm = (m + RPC - 1) / RPC, cn = (n + RPC - 1) / RPC;
-  const dim3 gm(cols, cm), gn(cols, cn), gd((a.ldb + 255) / 256), block(FT);
+  const int cm = m_items(), cn = (n + RPC - 1) / RPC;
+  const dim3 gn(cols, cn), gd((a.ldb + 255) / 256), block(FT);
   cur_ld = a.ldb;
   for (int s = 0; s < steps; ++s) {
-    mark(K_A);
-    hipLaunchKernelGGL(k_fam_a<T>, gm, block, 0, st, a);
     mark(K_AT);
-    if (!apply_At(a, a.Y, a.W, fp64 ? nullptr : a.AX, a.U, a.V, a.Z)) { err = "A' product failed"; return false; }
+    if (!apply_At(a, a.Y, a.W, a.U, a.V)) { err = "A' product failed"; return false; }
     mark(K_B);
     if (fp64) hipLaunchKernelGGL((k_fam_b<T, false>), gn, block, 0, st, a);
     else hipLaunchKernelGGL((k_fam_b<T, true>), gn, block, 0, st, a);
@@ -1080,24 +1142,20 @@ bool Family::apply_A(FamArgs<T>& a, const T* Xin, T* Z) {
                          static_cast<const float*>(Ad), n, &zero, Z, a.ldb) == rocblas_status_success;
 }
 
-// Z1 = A' X1, Z2 = A' X2 and, if X3, Z3 = A' X3 (n x ldb): one sparse pass
-// or one dense GEMM per right-hand side.
+// Z1 = A' X1 and Z2 = A' X2 (n x ldb): one sparse pass or one dense GEMM
+// per right-hand side.
 template <typename T>
-bool Family::apply_At(FamArgs<T>& a, const T* X1, const T* X2, const T* X3, T* Z1, T* Z2, T* Z3) {
+bool Family::apply_At(FamArgs<T>& a, const T* X1, const T* X2, T* Z1, T* Z2) {
   if (!dense) {
-    if (X3)
-      hipLaunchKernelGGL((k_fam_spmm<T, 3>), dim3(a.ldb / LANES, cur_tasks->nAt), dim3(FT), 0, st, cur_tasks->At,
-                         rowAt, a.ci, a.cx, X1, X2, X3, Z1, Z2, Z3, a.ldb);
-    else
-      hipLaunchKernelGGL((k_fam_spmm<T, 2>), dim3(a.ldb / LANES, cur_tasks->nAt), dim3(FT), 0, st, cur_tasks->At,
-                         rowAt, a.ci, a.cx, X1, X2, X2, Z1, Z2, Z2, a.ldb);
+    hipLaunchKernelGGL((k_fam_spmm<T, 2>), dim3(a.ldb / LANES, cur_tasks->nAt), dim3(FT), 0, st, cur_tasks->At,
+                       rowAt, a.ci, a.cx, X1, X2, X2, Z1, Z2, Z2, a.ldb);
     return hipGetLastError() == hipSuccess;
   }
   const T one = T(1), zero = T(0);
   // column-major Z' (ldb x n) = X' (ldb x m) * A, A = (stored n x m)^T
-  for (int k = 0; k < (X3 ? 3 : 2); ++k) {
-    const T* Xk = k == 0 ? X1 : (k == 1 ? X2 : X3);
-    T* Zk = k == 0 ? Z1 : (k == 1 ? Z2 : Z3);
+  for (int k = 0; k < 2; ++k) {
+    const T* Xk = k == 0 ? X1 : X2;
+    T* Zk = k == 0 ? Z1 : Z2;
     rocblas_status r;
     if constexpr (std::is_same<T, double>::value)
       r = rocblas_dgemm(blas, rocblas_operation_none, rocblas_operation_transpose, a.ldb, n, m, &one, Xk, a.ldb,
@@ -1108,6 +1166,17 @@ bool Family::apply_At(FamArgs<T>& a, const T* X1, const T* X2, const T* X3, T* Z
     if (r != rocblas_status_success) return false;
   }
   return true;
+}
+
+// W and the stop-test partials of the current iterate (phase C forms them
+// inside the loop).
+template <typename T>
+void Family::prologue(FamArgs<T>& a) {
+  cur_ld = a.ldb;
+  mark(K_A);
+  hipLaunchKernelGGL(k_fam_a<T>, dim3(a.ldb / LANES, m_items()), dim3(FT), 0, st, a, c1 + nx_items);
+  mark(-1);
+  need_prologue = false;
 }
 
 template <typename T>
@@ -1126,21 +1195,24 @@ bool Family::run_t(const float* warm, long wld, int judge, std::string& err) {
   cur_tasks = &tasks[0];
   hipLaunchKernelGGL(k_fam_init<T>, dim3(ld / LANES), block, 0, st, a, warm, wld);
   if (!apply_A(a, a.X, a.AX)) { err = "A product failed"; return false; }
+  need_prologue = true;
   if (warm && judge) {
-    const int cm = (m + RPC - 1) / RPC, cn = (n + RPC - 1) / RPC;
+    const int cm = m_items(), cn = (n + RPC - 1) / RPC;
     a.judge = 1;
-    hipLaunchKernelGGL(k_fam_a<T>, dim3(ld / LANES, cm), block, 0, st, a);
-    if (!apply_At(a, a.Y, a.W, (const T*)nullptr, a.U, a.V, a.Z)) { err = "A' product failed"; return false; }
+    prologue(a);
+    if (!apply_At(a, a.Y, a.W, a.U, a.V)) { err = "A' product failed"; return false; }
     if (fp64) hipLaunchKernelGGL((k_fam_b<T, false>), dim3(ld / LANES, cn), block, 0, st, a);
     else hipLaunchKernelGGL((k_fam_b<T, true>), dim3(ld / LANES, cn), block, 0, st, a);
     hipLaunchKernelGGL(k_fam_decide<T>, dim3((ld + 255) / 256), dim3(256), 0, st, a, cm, cn);
     a.judge = 0;
+    need_prologue = true;  // rejected instances were reset
   }
   std::vector<int> hdone(ldb), list;
   int chunk = 16;
   compactions = 0;
   for (long total = 0; total <= (long)a.max_iters + 1;) {
     const int steps = (int)std::min<long>(chunk, a.max_iters + 2 - total);
+    if (need_prologue) prologue(a);
     if (use_graph && !fp64 && !timing) {
       // replay a captured graph of `steps` iterations for this layout
       if (!gexec || g_ld != a.ldb || g_steps != steps || g_x != (const void*)a.X) {
@@ -1177,7 +1249,9 @@ bool Family::run_t(const float* warm, long wld, int judge, std::string& err) {
     const int active = (int)list.size();
     if (active == 0) break;
     chunk = 16;
-    if (active <= (cnt * 5) / 8 && ld > LANES) {
+    // compact once it drops at least 1/16 of the width: a move costs a few
+    // microseconds, a dead column costs its share of every later iteration
+    if (pad64(active) <= ld - std::max(LANES, ld / 16)) {
       // compaction: finished columns to the output store, live ones to the other set
       hipLaunchKernelGGL(k_fam_scatter<T>, dim3(ld / LANES), block, 0, st, a, ldb, static_cast<T*>(Xo),
                          static_cast<T*>(Yo), static_cast<T*>(So), static_cast<int*>(Io), static_cast<double*>(Ro));
@@ -1191,6 +1265,7 @@ bool Family::run_t(const float* warm, long wld, int judge, std::string& err) {
       cnt = active;
       a = b;
       ++compactions;
+      need_prologue = true;
     }
   }
   hipLaunchKernelGGL(k_fam_scatter<T>, dim3(ld / LANES), block, 0, st, a, ldb, static_cast<T*>(Xo),

@@ -37,7 +37,7 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int BM = 64, BN = 64, BK = 32, KP = BK + 8;  // LDS rows padded against bank conflicts
+constexpr int BM = 64, BN = 64, BK = 64, KP = BK + 8;  // LDS rows padded against bank conflicts
 constexpr int GT = 256;
 
 enum Epi { EPI_F32 = 0, EPI_BIAS_GELU = 1, EPI_BIAS_AFFINE = 2, EPI_GELU_GRAD = 3, EPI_ADD = 4 };
@@ -50,8 +50,10 @@ __device__ __forceinline__ float gelu_grad(float z) {
 // C[M x N] = op(A) op(B) with bf16 operands and fp32 accumulation.
 //   TA = false: A is M x K row-major (lda);  TA = true: A is stored K x M (lda)
 //   TB = false: B is K x N row-major (ldb);  TB = true: B is stored N x K (ldb)
-// M % 64 == 0, N % 64 == 0, K % 32 == 0.  Block: 64 x 64 tile, 4 waves in a
-// 2 x 2 layout, each wave 32 x 32 = 2 x 2 MFMA tiles of 16 x 16.
+// M % 64 == 0, N % 64 == 0, K % 64 == 0.  Block: 64 x 64 tile, 4 waves in a
+// 2 x 2 layout, each wave 32 x 32 = 2 x 2 MFMA tiles of 16 x 16.  The K loop
+// is software-pipelined: the global loads of tile k + 1 are in flight while
+// tile k is multiplied out of LDS.
 template <bool TA, bool TB, int EPI>
 __global__ __launch_bounds__(GT) void k_gemm(int M, int N, int K, const __bf16* __restrict__ A, int lda,
                                              const __bf16* __restrict__ B, int ldb, float* __restrict__ Cf,
@@ -69,42 +71,56 @@ __global__ __launch_bounds__(GT) void k_gemm(int M, int N, int K, const __bf16* 
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
+  // Each thread moves two 16-byte pieces of each tile per K step.
+  //   row-major source (A: !TA, B: TB):  row t / 4, k (t % 4) * 8 + {0, 32}
+  //   transposed source (A: TA, B: !TB): k t / 8 + {0, 32}, row (t % 8) * 8 .. + 7
+  bf16x8 ra[2], rb[2];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if (!TA) ra[h] = *reinterpret_cast<const bf16x8*>(&A[(size_t)(m0 + (tid >> 2)) * lda + k0 + (tid & 3) * 8 + h * 32]);
+      else ra[h] = *reinterpret_cast<const bf16x8*>(&A[(size_t)(k0 + (tid >> 3) + h * 32) * lda + m0 + (tid & 7) * 8]);
+      if (TB) rb[h] = *reinterpret_cast<const bf16x8*>(&B[(size_t)(n0 + (tid >> 2)) * ldb + k0 + (tid & 3) * 8 + h * 32]);
+      else rb[h] = *reinterpret_cast<const bf16x8*>(&B[(size_t)(k0 + (tid >> 3) + h * 32) * ldb + n0 + (tid & 7) * 8]);
+    }
+  };
+  auto sstore = [&]() {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if (!TA) {
+        *reinterpret_cast<bf16x8*>(&As[tid >> 2][(tid & 3) * 8 + h * 32]) = ra[h];
+      } else {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) As[(tid & 7) * 8 + i][(tid >> 3) + h * 32] = ra[h][i];
+      }
+      if (TB) {
+        *reinterpret_cast<bf16x8*>(&Bs[tid >> 2][(tid & 3) * 8 + h * 32]) = rb[h];
+      } else {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) Bs[(tid & 7) * 8 + i][(tid >> 3) + h * 32] = rb[h][i];
+      }
+    }
+  };
+  gload(0);
   for (int k0 = 0; k0 < K; k0 += BK) {
-    // stage A tile (64 x 32) as As[row][k]
-    if (!TA) {
-      const int r = tid >> 2, kc = (tid & 3) * 8;
-      *reinterpret_cast<bf16x8*>(&As[r][kc]) =
-          *reinterpret_cast<const bf16x8*>(&A[(size_t)(m0 + r) * lda + k0 + kc]);
-    } else {
-      const int kk = tid >> 3, rc = (tid & 7) * 8;
-      const bf16x8 v = *reinterpret_cast<const bf16x8*>(&A[(size_t)(k0 + kk) * lda + m0 + rc]);
-#pragma unroll
-      for (int i = 0; i < 8; ++i) As[rc + i][kk] = v[i];
-    }
-    // stage B tile (32 x 64) as Bs[col][k]
-    if (!TB) {
-      const int kk = tid >> 3, cc = (tid & 7) * 8;
-      const bf16x8 v = *reinterpret_cast<const bf16x8*>(&B[(size_t)(k0 + kk) * ldb + n0 + cc]);
-#pragma unroll
-      for (int i = 0; i < 8; ++i) Bs[cc + i][kk] = v[i];
-    } else {
-      const int c = tid >> 2, kc = (tid & 3) * 8;
-      *reinterpret_cast<bf16x8*>(&Bs[c][kc]) =
-          *reinterpret_cast<const bf16x8*>(&B[(size_t)(n0 + c) * ldb + k0 + kc]);
-    }
+    sstore();
     __syncthreads();
+    if (k0 + BK < K) gload(k0 + BK);
     // lane l: A rows (l % 16), k = (l / 16) * 8 .. +7; B cols (l % 16), same k
     const int fr = lane & 15, fk = (lane >> 4) * 8;
-    bf16x8 af[2], bfv[2];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) af[i] = *reinterpret_cast<const bf16x8*>(&As[wm + i * 16 + fr][fk]);
+    for (int ks = 0; ks < BK; ks += 32) {
+      bf16x8 af[2], bfv[2];
 #pragma unroll
-    for (int j = 0; j < 2; ++j) bfv[j] = *reinterpret_cast<const bf16x8*>(&Bs[wn + j * 16 + fr][fk]);
+      for (int i = 0; i < 2; ++i) af[i] = *reinterpret_cast<const bf16x8*>(&As[wm + i * 16 + fr][ks + fk]);
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 2; ++j) bfv[j] = *reinterpret_cast<const bf16x8*>(&Bs[wn + j * 16 + fr][ks + fk]);
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfv[j], acc[i][j], 0, 0, 0);
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfv[j], acc[i][j], 0, 0, 0);
+    }
     __syncthreads();
   }
   // accumulator lane l holds rows (l / 16) * 4 + r, column l % 16
@@ -221,24 +237,27 @@ __global__ void k_to_bf16(long n, const float* __restrict__ a, __bf16* __restric
 
 int pad64(int v) { return (v + 63) & ~63; }
 
+// Launches k_gemm; shapes off the 64-multiple grid are refused (the kernel
+// has no bounds checks).
 template <bool TA, bool TB, int EPI>
-void gemm(hipStream_t st, int M, int N, int K, const __bf16* A, int lda, const __bf16* B, int ldb, float* Cf,
-          __bf16* Cb, int ldc, const float* bias = nullptr, const float* aux = nullptr,
-          const float* omul = nullptr, const float* oadd = nullptr) {
+hipError_t gemm(hipStream_t st, int M, int N, int K, const __bf16* A, int lda, const __bf16* B, int ldb,
+                float* Cf, __bf16* Cb, int ldc, const float* bias = nullptr, const float* aux = nullptr,
+                const float* omul = nullptr, const float* oadd = nullptr) {
+  if (M <= 0 || N <= 0 || K <= 0 || M % BM || N % BN || K % BK) return hipErrorInvalidValue;
   hipLaunchKernelGGL((k_gemm<TA, TB, EPI>), dim3(N / BN, M / BM), dim3(GT), 0, st, M, N, K, A, lda, B, ldb,
                      Cf, Cb, ldc, bias, aux, omul, oadd);
+  return hipGetLastError();
 }
 
 }  // namespace
 
-// C (fp32, M x N) = A (bf16, M x K) B (bf16, K x N), row-major; M, N multiples
-// of 64 and K of 32; with `addend` (M x N, ldc), C = addend + A B.  Used by
-// the family engine's split-bf16 x-update.
+// C (fp32, M x N) = A (bf16, M x K) B (bf16, K x N), row-major; M, N and K
+// multiples of 64; with `addend` (M x N, ldc), C = addend + A B.  Used by
+// the family engine's x-update.
 hipError_t bf16_gemm_f32(hipStream_t st, int M, int N, int K, const __bf16* A, int lda, const __bf16* B,
                          int ldb, float* C, int ldc, const float* addend) {
-  if (addend) gemm<false, false, EPI_ADD>(st, M, N, K, A, lda, B, ldb, C, nullptr, ldc, nullptr, addend);
-  else gemm<false, false, EPI_F32>(st, M, N, K, A, lda, B, ldb, C, nullptr, ldc);
-  return hipGetLastError();
+  if (addend) return gemm<false, false, EPI_ADD>(st, M, N, K, A, lda, B, ldb, C, nullptr, ldc, nullptr, addend);
+  return gemm<false, false, EPI_F32>(st, M, N, K, A, lda, B, ldb, C, nullptr, ldc);
 }
 
 // ---------------------------------------------------------------------------
@@ -288,6 +307,12 @@ struct Mlp {
            alloc(T, (size_t)cap * nout, err) && alloc(Xf, (size_t)cap * nin, err) &&
            alloc(part, 1024, err);
   }
+  // first refused GEMM launch since the last API call (none expected: every
+  // dimension is padded to a multiple of 64)
+  hipError_t gerr = hipSuccess;
+  void note(hipError_t e) {
+    if (gerr == hipSuccess && e != hipSuccess) gerr = e;
+  }
   void refresh_bf16() {
     hipLaunchKernelGGL(k_to_bf16, dim3(256), dim3(256), 0, st, (long)pin * ph, W1, W1b);
     hipLaunchKernelGGL(k_to_bf16, dim3(256), dim3(256), 0, st, (long)ph * ph, W2, W2b);
@@ -297,10 +322,10 @@ struct Mlp {
   void forward(int rows, bool affine_out) {
     const int rp = pad64(rows);
     hipLaunchKernelGGL(k_prep_input, dim3(512), dim3(256), 0, st, rows, nin, nin, Xf, in_mu, in_sd, pin, rp, Xb);
-    gemm<false, false, EPI_BIAS_GELU>(st, rp, ph, pin, Xb, pin, W1b, ph, Z1, H1b, ph, b1);
-    gemm<false, false, EPI_BIAS_GELU>(st, rp, ph, ph, H1b, ph, W2b, ph, Z2, H2b, ph, b2);
-    gemm<false, false, EPI_BIAS_AFFINE>(st, rp, pout, ph, H2b, ph, W3b, pout, Y, nullptr, pout, b3, nullptr,
-                                        affine_out ? out_mul : nullptr, affine_out ? out_add : nullptr);
+    note(gemm<false, false, EPI_BIAS_GELU>(st, rp, ph, pin, Xb, pin, W1b, ph, Z1, H1b, ph, b1));
+    note(gemm<false, false, EPI_BIAS_GELU>(st, rp, ph, ph, H1b, ph, W2b, ph, Z2, H2b, ph, b2));
+    note(gemm<false, false, EPI_BIAS_AFFINE>(st, rp, pout, ph, H2b, ph, W3b, pout, Y, nullptr, pout, b3, nullptr,
+                                        affine_out ? out_mul : nullptr, affine_out ? out_add : nullptr));
   }
   // loss and parameter gradients into g for the rows in Xf / T
   double backward(int rows) {
@@ -310,13 +335,13 @@ struct Mlp {
     float* gW1 = g + off_W1; float* gb1 = g + off_b1;
     float* gW2 = g + off_W2; float* gb2 = g + off_b2;
     float* gW3 = g + off_W3; float* gb3 = g + off_b3;
-    gemm<true, false, EPI_F32>(st, ph, pout, rp, H2b, ph, dYb, pout, gW3, nullptr, pout);
+    note(gemm<true, false, EPI_F32>(st, ph, pout, rp, H2b, ph, dYb, pout, gW3, nullptr, pout));
     hipLaunchKernelGGL(k_colsum<float>, dim3((pout + 255) / 256), dim3(256), 0, st, rp, pout, pout, dYf, gb3);
-    gemm<false, true, EPI_GELU_GRAD>(st, rp, ph, pout, dYb, pout, W3b, pout, nullptr, dZ2b, ph, nullptr, Z2);
-    gemm<true, false, EPI_F32>(st, ph, ph, rp, H1b, ph, dZ2b, ph, gW2, nullptr, ph);
+    note(gemm<false, true, EPI_GELU_GRAD>(st, rp, ph, pout, dYb, pout, W3b, pout, nullptr, dZ2b, ph, nullptr, Z2));
+    note(gemm<true, false, EPI_F32>(st, ph, ph, rp, H1b, ph, dZ2b, ph, gW2, nullptr, ph));
     hipLaunchKernelGGL(k_colsum<__bf16>, dim3((ph + 255) / 256), dim3(256), 0, st, rp, ph, ph, dZ2b, gb2);
-    gemm<false, true, EPI_GELU_GRAD>(st, rp, ph, ph, dZ2b, ph, W2b, ph, nullptr, dZ1b, ph, nullptr, Z1);
-    gemm<true, false, EPI_F32>(st, pin, ph, rp, Xb, pin, dZ1b, ph, gW1, nullptr, ph);
+    note(gemm<false, true, EPI_GELU_GRAD>(st, rp, ph, ph, dZ2b, ph, W2b, ph, nullptr, dZ1b, ph, nullptr, Z1));
+    note(gemm<true, false, EPI_F32>(st, pin, ph, rp, Xb, pin, dZ1b, ph, gW1, nullptr, ph));
     hipLaunchKernelGGL(k_colsum<__bf16>, dim3((ph + 255) / 256), dim3(256), 0, st, rp, ph, ph, dZ1b, gb1);
     std::vector<double> hp(nb);
     (void)hipMemcpyAsync(hp.data(), part, nb * sizeof(double), hipMemcpyDeviceToHost, st);
@@ -449,6 +474,14 @@ int pdos_mlp_set_affine(PdosMlp* P, const float* in_mean, const float* in_std, c
   return 0;
 }
 
+// 0 when the stream is idle without error and no GEMM launch was refused.
+static int mlp_status(Mlp& M) {
+  const bool ok = hipStreamSynchronize(M.st) == hipSuccess && M.gerr == hipSuccess;
+  if (M.gerr != hipSuccess) g_mlp_err = "GEMM shape off the 64-multiple grid";
+  M.gerr = hipSuccess;
+  return ok ? 0 : -1;
+}
+
 // Loss and gradients (logical layout, same as get_params) of one batch;
 // targets T are in the standardised output space.
 int pdos_mlp_loss_grad(PdosMlp* P, const float* X, const float* T, int rows, double* loss, float* grads) {
@@ -466,7 +499,7 @@ int pdos_mlp_loss_grad(PdosMlp* P, const float* X, const float* T, int rows, dou
     copy_params(M, grads, false);
     M.master = save;
   }
-  return hipStreamSynchronize(M.st) == hipSuccess ? 0 : -1;
+  return mlp_status(M);
 }
 
 // Trains `epochs` passes over (X, T) (host arrays, rows x nin / rows x nout,
@@ -512,7 +545,7 @@ int pdos_mlp_train(PdosMlp* P, const float* X, const float* T, int rows, int epo
   (void)hipFree(dT);
   (void)hipFree(dIdx);
   if (last_loss) *last_loss = epoch_loss;
-  return hipStreamSynchronize(M.st) == hipSuccess ? 0 : -1;
+  return mlp_status(M);
 }
 
 // Predictions in the original output units (out_std * y + out_mean) for
@@ -542,7 +575,7 @@ int pdos_mlp_predict(PdosMlp* P, const float* X, int rows, float* Y, double* ker
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (kernel_ms) *kernel_ms = total;
-  return hipStreamSynchronize(M.st) == hipSuccess ? 0 : -1;
+  return mlp_status(M);
 }
 
 // Device-to-device prediction: dX (device, rows x nin raw features) to dY
@@ -559,7 +592,7 @@ int pdos_mlp_predict_dev(PdosMlp* P, const float* dX, int rows, float* dY, int l
     (void)hipMemcpy2DAsync(dY + (size_t)s * ldy, (size_t)ldy * 4, M.Y, M.pout * 4, M.nout * 4, r,
                            hipMemcpyDeviceToDevice, M.st);
   }
-  return hipStreamSynchronize(M.st) == hipSuccess ? 0 : -1;
+  return mlp_status(M);
 }
 
 }  // extern "C"
