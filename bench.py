@@ -148,38 +148,47 @@ def family_roofline(kt, fam_info, nnz, fp64, lin_rows, b_inst):
     """Roofline of the dominant kernel of a family run, from per-kernel HIP
     event totals {kind: (ms, calls, columns)}.  Algorithmic bytes per column
     (instance), values of sv bytes:
-      A' product  2 right-hand sides: 2m read, 2n written (+ the shared
-                  matrix once per launch: nnz (sv + 4))
+      A' product  sparse A, fused with phase B: Y, W read (2m), c, X read
+                  (2n), the x-update right-hand side written (bf16 planes
+                  or sv); dense A: Y, W read, U, V written (2m + 2n)
+                  (+ the shared matrix once per launch: nnz (sv + 4))
+      phase B     dense A only: c, X, U, V read (4n), R written; sparse A:
+                  the stop test, 16 bytes of partials per chunk
       A product   n read, m written (+ the matrix)
       phase A     S, Y, AX (+ b) read, W written: 4m (5m)
-      phase B     c, X, U, V read: 4n; R written (3 bf16 planes or sv)
-      phase C     X = XN: 2n; zero/nonneg rows: b, AX, S, Y read, S, Y, W
-                  written (7); cone rows: pass 1 reads b, AX, S, Y, writes
-                  W, S; pass 2 reads S, W, Y, AX, b, writes Y, S, W (14);
-                  b reads drop out when b is shared
+      phase C     zero/nonneg rows: AX, S, Y (+ b) read, S, Y, W written
+                  (6, 7); cone rows: pass 1 AX, S, Y (+ b) read, W, S
+                  written; pass 2 S, W, Y (+ AX, + b) read, Y, S, W written
+                  (11 in mixed mode, where s~ stands in for b - AX; 12 in
+                  fp64; + 2 with per-instance b)
     x-update GEMM: 2 n np flops per product (bf16) or 2 n^2 (fp64)."""
     n, m, npad = fam_info["n"], fam_info["m"], fam_info["np"]
+    dense = bool(fam_info.get("dense", 0))
+    nprod = fam_info.get("products", 1)
     sv = 8 if fp64 else 4
     kind = max(kt, key=lambda k: kt[k][0])
     ms, calls, cols = kt[kind]
     cone_rows = m - lin_rows
+    rhs = n * sv if fp64 else n * 2 * nprod
     if kind == "xupdate_gemm":
-        flops = (2.0 * n * n if fp64 else 2.0 * npad * fam_info.get("products", 1) * npad) * cols
+        flops = (2.0 * n * n if fp64 else 2.0 * npad * nprod * npad) * cols
         ach = flops / (ms * 1e-3) / 1e12
         peak = FP64_PEAK_TFLOPS if fp64 else BF16_PEAK_TFLOPS
         return {"bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
                 "frac": round(ach / peak, 4), "traffic": None, "kernel": kind,
                 "algorithmic_flops_per_launch": flops / max(calls, 1), "avg_launch_us": round(1e3 * ms / max(calls, 1), 2)}
     if kind == "At_product":
-        byt = cols * 2 * (m + n) * sv + calls * nnz * (sv + 4)
+        per_col = 2 * (m + n) * sv + (0 if dense else rhs)
+        byt = cols * per_col + calls * nnz * (sv + 4)
     elif kind == "A_product":
         byt = cols * (m + n) * sv + calls * nnz * (sv + 4)
     elif kind == "phase_a":
         byt = cols * m * sv * (5 if b_inst else 4)
     elif kind == "phase_b":
-        byt = cols * n * sv * 4 + cols * n * (2 * 3 if not fp64 else sv)
+        byt = cols * (n * sv * 4 + rhs) if dense else cols * 16 * ((n + 63) // 64 + (m + 63) // 64)
     else:
-        rows = 2 * n + (7 if b_inst else 6) * lin_rows + (14 if b_inst else 12) * cone_rows
+        cone = (12 if fp64 else 11) + (2 if b_inst else 0)
+        rows = (7 if b_inst else 6) * lin_rows + cone * cone_rows
         byt = cols * rows * sv
     ach = byt / (ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

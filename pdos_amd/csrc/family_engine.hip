@@ -159,50 +159,39 @@ __device__ __forceinline__ double lane_of(double v, int l) {
   return __longlong_as_double((hi << 32) | lo);
 }
 
-// Y_k = M X_k (k < NR) for a shared sparse M in CSR (idx, val, row of each
-// entry) and feature-major X_k, up to three right-hand sides per pass.
-// Work is host-built per wave: task {r0, r1, q0, q1} computes the complete
-// rows [r0, r1) from the entries [q0, q1) (at most 64 of them); a long row is
-// split over the 8 waves of its block, every task {r, -1, q0, q1}, and the
-// block sums the waves' partials.  A wave loads up to 64 entries' (column,
-// value, row) in one coalesced load per array, then issues the gathers of
-// kUnroll entries before using any of them, so a task costs a few memory
-// latencies however short its rows are.  Each row is summed in entry order.
-constexpr int kUnroll = 16;
+constexpr int kUnroll = 16;  // gathers in flight per wave
 
 template <typename T, int NR>
-__global__ __launch_bounds__(FT) void k_fam_spmm(const int4* __restrict__ tasks, const int* __restrict__ rowof,
-                                                 const int* __restrict__ idx, const T* __restrict__ val,
-                                                 const T* __restrict__ X1, const T* __restrict__ X2,
-                                                 const T* __restrict__ X3, T* __restrict__ Y1,
-                                                 T* __restrict__ Y2, T* __restrict__ Y3, int ldb) {
-  __shared__ double buf[NWV * LANES];
+struct SpmmIn {
+  const int4* tasks; const int* rowof; const int* idx; const T* val;
+  const T* X1; const T* X2; const T* X3;
+  int ldb;
+};
+
+// The wave's task of block blockIdx.y.  begin(row) is called when the wave
+// starts a row (before its entries are gathered: loads issued there overlap
+// the row's products), done(row, acc) when the row is complete; for a split
+// row, wave 0 gets done(row, block total).  Rows without entries inside a
+// task are begun and done with zero sums.
+template <typename T, int NR, typename Begin, typename Done>
+__device__ __forceinline__ void spmm_core(const SpmmIn<T, NR>& in, double* buf, Begin begin, Done done) {
   const int lane = threadIdx.x & 63, w = wave_id();
   const int inst = blockIdx.x * LANES + lane;
-  const int4 tk = tasks[blockIdx.y * NWV + w];
-  const bool split = tasks[blockIdx.y * NWV].y < 0;  // block-uniform
+  const int4 tk = in.tasks[blockIdx.y * NWV + w];
+  const bool split = in.tasks[blockIdx.y * NWV].y < 0;  // block-uniform
   T acc[NR];
 #pragma unroll
   for (int k = 0; k < NR; ++k) acc[k] = T(0);
   int cur = tk.x;
-  auto put = [&](int row, const T (&v)[NR]) {
-    const size_t o = (size_t)row * ldb + inst;
-    Y1[o] = v[0];
-    if (NR > 1) Y2[o] = v[1];
-    if (NR > 2) Y3[o] = v[2];
-  };
-  auto zero_rows = [&](int from, int to) {
-    const T z[NR] = {};
-    for (int r = from; r < to; ++r) put(r, z);
-  };
+  if (split ? w == 0 : tk.x < tk.y) begin(cur);
   for (int q = tk.z; q < tk.w; q += 64) {
     const int cnt = min(64, tk.w - q);
     int ci = 0, ri = 0;
     T vi = T(0);
     if (lane < cnt) {
-      ci = idx[q + lane];
-      vi = val[q + lane];
-      if (!split) ri = rowof[q + lane];
+      ci = in.idx[q + lane];
+      vi = in.val[q + lane];
+      if (!split) ri = in.rowof[q + lane];
     }
     for (int e0 = 0; e0 < cnt; e0 += kUnroll) {
       // entries past cnt repeat entry cnt - 1: in bounds, not accumulated
@@ -217,20 +206,20 @@ __global__ __launch_bounds__(FT) void k_fam_spmm(const int4* __restrict__ tasks,
       }
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u) {
-        const size_t o = (size_t)cc[u] * ldb + inst;
-        xv[u][0] = X1[o];
-        if (NR > 1) xv[u][1] = X2[o];
-        if (NR > 2) xv[u][2] = X3[o];
+        const size_t o = (size_t)cc[u] * in.ldb + inst;
+        xv[u][0] = in.X1[o];
+        if (NR > 1) xv[u][1] = in.X2[o];
+        if (NR > 2) xv[u][2] = in.X3[o];
       }
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u) {
         if (e0 + u < cnt) {
-          if (rr[u] != cur) {  // wave-uniform: the previous row is complete
-            put(cur, acc);
-            zero_rows(cur + 1, rr[u]);
+          while (rr[u] != cur) {  // wave-uniform: rows up to rr[u] are complete
+            done(cur, acc);
 #pragma unroll
             for (int k = 0; k < NR; ++k) acc[k] = T(0);
-            cur = rr[u];
+            ++cur;
+            begin(cur);
           }
 #pragma unroll
           for (int k = 0; k < NR; ++k) acc[k] += vv[u] * xv[u][k];
@@ -240,15 +229,87 @@ __global__ __launch_bounds__(FT) void k_fam_spmm(const int4* __restrict__ tasks,
   }
   if (!split) {
     if (tk.x < tk.y) {
-      put(cur, acc);
-      zero_rows(cur + 1, tk.y);
+      for (;;) {
+        done(cur, acc);
+        if (++cur >= tk.y) break;
+#pragma unroll
+        for (int k = 0; k < NR; ++k) acc[k] = T(0);
+        begin(cur);
+      }
     }
     return;
   }
   T tot[NR];
 #pragma unroll
   for (int k = 0; k < NR; ++k) tot[k] = T(waves_sum(double(acc[k]), buf));
-  if (w == 0) put(tk.x, tot);
+  if (w == 0) done(tk.x, tot);
+}
+
+// Y_k = M X_k (k < NR) for a shared sparse M in CSR (idx, val, row of each
+// entry) and feature-major X_k, up to three right-hand sides per pass.
+// Work is host-built per wave: task {r0, r1, q0, q1} computes the complete
+// rows [r0, r1) from the entries [q0, q1) (at most 64 of them); a long row is
+// split over the 8 waves of its block, every task {r, -1, q0, q1}, and the
+// block sums the waves' partials.  A wave loads up to 64 entries' (column,
+// value, row) in one coalesced load per array, then issues the gathers of
+// kUnroll entries before using any of them, so a task costs a few memory
+// latencies however short its rows are.  Each row is summed in entry order.
+template <typename T, int NR>
+__global__ __launch_bounds__(FT) void k_fam_spmm(SpmmIn<T, NR> in, T* __restrict__ Y1, T* __restrict__ Y2,
+                                                 T* __restrict__ Y3) {
+  __shared__ double buf[NWV * LANES];
+  const int inst = blockIdx.x * LANES + (threadIdx.x & 63);
+  spmm_core<T, NR>(in, buf, [](int) {}, [&](int row, const T (&v)[NR]) {
+    const size_t o = (size_t)row * in.ldb + inst;
+    Y1[o] = v[0];
+    if (NR > 1) Y2[o] = v[1];
+    if (NR > 2) Y3[o] = v[2];
+  });
+}
+
+// The A' product with phase B as its epilogue (sparse A): per row i of A',
+// U_i = (A'Y)_i and V_i = (A'W)_i stay in registers and give the partial
+// |(c + U) ./ E|^2 and c'X and the x-update right-hand side (as k_fam_b).
+// c_i and X_i are loaded when the row begins.  Partials per block into
+// pb[blockIdx.y].
+template <typename T>
+__global__ __launch_bounds__(FT) void k_fam_at_b(FamArgs<T> a, SpmmIn<T, 2> in) {
+  __shared__ double buf[NWV * LANES];
+  const int lane = threadIdx.x & 63, w = wave_id();
+  const int inst = blockIdx.x * LANES + lane;
+  const bool act = inst < a.count && !a.done[inst];
+  const T lam = T(a.lam[inst]);
+  T c_i = T(0), x_i = T(0);
+  double d2 = 0.0, cx = 0.0;
+  spmm_core<T, 2>(in, buf,
+                  [&](int i) {
+                    const size_t o = (size_t)i * a.ldb + inst;
+                    c_i = a.c[o];
+                    x_i = a.X[o];
+                  },
+                  [&](int i, const T (&v)[2]) {
+                    if (!act) return;
+                    const T r = (c_i + v[0]) / T(a.E[i]);
+                    d2 += double(r) * double(r);
+                    cx += double(c_i) * double(x_i);
+                    if (a.judge) return;
+                    const T rv = kCorr<T> ? v[1] - lam * c_i : x_i - lam * c_i + v[1];
+                    if (kCorr<T>) {
+                      const __bf16 hi = (__bf16)float(rv);
+                      a.Rcat[(size_t)i * a.ldb + inst] = hi;
+                      if (a.nprod > 1) a.Rcat[(size_t)(a.np + i) * a.ldb + inst] = (__bf16)(float(rv) - float(hi));
+                      if (a.nprod > 2) a.Rcat[(size_t)(2 * a.np + i) * a.ldb + inst] = hi;
+                    } else {
+                      a.R[(size_t)i * a.ldb + inst] = rv;
+                    }
+                  });
+  d2 = waves_sum(d2, buf);
+  cx = waves_sum(cx, buf);
+  if (act && w == 0) {
+    double* pb = a.pb + ((size_t)blockIdx.y * a.ldb + inst) * 2;
+    pb[0] = d2;
+    pb[1] = cx;
+  }
 }
 
 // Phase kernels work on chunks of RPC rows, RW rows per wave: every wave
@@ -356,25 +417,43 @@ __global__ __launch_bounds__(FT) void k_fam_b(FamArgs<T> a) {
 }
 
 // The stop test of every active instance (pdos.c:95-107) from the chunk
-// partials, in a fixed order; one thread per instance.  In judge mode it
-// compares a warm start's scaled residuals with the cold start's and resets
-// rejected instances to zero instead.
+// partials.  A block holds 64 instances; wave w sums chunks w, w + NWV, ...
+// of both partial sets (its loads all in flight), and the waves' sums are
+// added in wave order, so the totals do not depend on timing.  Wave 0 then
+// decides.  In judge mode it compares a warm start's scaled residuals with
+// the cold start's and resets rejected instances to zero instead.
 template <typename T>
-__global__ __launch_bounds__(256) void k_fam_decide(FamArgs<T> a, int chunks_m, int chunks_n) {
-  const int inst = blockIdx.x * blockDim.x + threadIdx.x;
-  if (inst >= a.count || a.done[inst]) return;
-  double p2 = 0.0, by = 0.0, d2 = 0.0, cx = 0.0;
-#pragma unroll 16
-  for (int c = 0; c < chunks_m; ++c) {
-    const double2 pa = *reinterpret_cast<const double2*>(a.pa + ((size_t)c * a.ldb + inst) * 2);
-    p2 += pa.x;
-    by += pa.y;
+__global__ __launch_bounds__(FT) void k_fam_decide(FamArgs<T> a, int chunks_m, int chunks_n) {
+  __shared__ double4 part[NWV][LANES];
+  const int lane = threadIdx.x & 63, w = wave_id();
+  const int inst = blockIdx.x * LANES + lane;
+  const bool act = inst < a.count && !a.done[inst];
+  double4 v = {0.0, 0.0, 0.0, 0.0};  // |pres|^2, b'y, |dres|^2, c'x
+  if (act) {
+#pragma unroll 8
+    for (int c = w; c < chunks_m; c += NWV) {
+      const double2 pa = *reinterpret_cast<const double2*>(a.pa + ((size_t)c * a.ldb + inst) * 2);
+      v.x += pa.x;
+      v.y += pa.y;
+    }
+#pragma unroll 8
+    for (int c = w; c < chunks_n; c += NWV) {
+      const double2 pb = *reinterpret_cast<const double2*>(a.pb + ((size_t)c * a.ldb + inst) * 2);
+      v.z += pb.x;
+      v.w += pb.y;
+    }
   }
-#pragma unroll 16
-  for (int c = 0; c < chunks_n; ++c) {
-    const double2 pb = *reinterpret_cast<const double2*>(a.pb + ((size_t)c * a.ldb + inst) * 2);
-    d2 += pb.x;
-    cx += pb.y;
+  part[w][lane] = v;
+  __syncthreads();
+  if (w != 0 || !act) return;
+  double p2 = 0.0, by = 0.0, d2 = 0.0, cx = 0.0;
+#pragma unroll
+  for (int k = 0; k < NWV; ++k) {
+    const double4 q = part[k][lane];
+    p2 += q.x;
+    by += q.y;
+    d2 += q.z;
+    cx += q.w;
   }
   const double pres = sqrt(p2), dres = sqrt(d2), pobj = cx, dobj = -by;
   const double eta = fabs(pobj - dobj);
@@ -409,7 +488,9 @@ __global__ __launch_bounds__(256) void k_fam_decide(FamArgs<T> a, int chunks_m, 
 
 // Phase C, first pass.  Work items (one per blockIdx.y, host-built, at most
 // RPC rows each):
-//   (i0, i1, 0, -)  rows of X = XN
+//   (i0, i1, 0, -)  rows of x of finished instances: XN = X (the host swaps
+//                   X and XN after every iteration, so active instances take
+//                   the GEMM's XN as their new x without a copy)
 //   (j0, j1, 1, -)  zero / nonneg rows: s~ = b - AX, relax, project, dual
 //                   update; then, for the next iteration, W (next_w) and the
 //                   item's stop-test partials (as phase A)
@@ -424,17 +505,17 @@ __global__ __launch_bounds__(FT) void k_fam_c1(FamArgs<T> a, const int4* __restr
   const bool act = inst < a.count && !a.done[inst];
   const int4 it = items[blockIdx.y];
   if (it.z == 0) {
-    if (!act) return;
+    if (act || inst >= a.count) return;
     T xv[RW];
 #pragma unroll
     for (int k = 0; k < RW; ++k) {
       const int i = it.x + w + k * NWV;
-      if (i < it.y) xv[k] = a.XN[(size_t)i * a.ldb + inst];
+      if (i < it.y) xv[k] = a.X[(size_t)i * a.ldb + inst];
     }
 #pragma unroll
     for (int k = 0; k < RW; ++k) {
       const int i = it.x + w + k * NWV;
-      if (i < it.y) a.X[(size_t)i * a.ldb + inst] = xv[k];
+      if (i < it.y) a.XN[(size_t)i * a.ldb + inst] = xv[k];
     }
     return;
   }
@@ -507,6 +588,9 @@ __global__ __launch_bounds__(FT) void k_fam_c2(FamArgs<T> a, const int4* __restr
   const bool act = inst < a.count && !a.done[inst];
   const int4 it = items[blockIdx.y];
   const int cn = cone[blockIdx.y];
+  // mixed mode without relaxation: s~ = b - AX exactly, so the residual and
+  // W come from s~ and AX, b are not read again
+  const bool from_st = kCorr<T> && a.relax == 1.0;
   double p2 = 0.0, by = 0.0;
   if (act) {
     T zv[RW], wv[RW], yv[RW], av[RW], bv[RW];
@@ -515,7 +599,9 @@ __global__ __launch_bounds__(FT) void k_fam_c2(FamArgs<T> a, const int4* __restr
       const int j = it.x + w + k * NWV;
       if (j < it.y) {
         const size_t o = (size_t)j * a.ldb + inst;
-        zv[k] = a.S[o]; wv[k] = a.W[o]; yv[k] = a.Y[o]; av[k] = a.AX[o]; bv[k] = bval(a, j, inst);
+        zv[k] = a.S[o]; wv[k] = a.W[o]; yv[k] = a.Y[o];
+        bv[k] = bval(a, j, inst);
+        if (!from_st) av[k] = a.AX[o];
       }
     }
     double n2 = 0.0;
@@ -537,10 +623,10 @@ __global__ __launch_bounds__(FT) void k_fam_c2(FamArgs<T> a, const int4* __restr
         const T y = yv[k] + (sv - wv[k]) / lam;
         a.Y[o] = y;
         a.S[o] = sv;
-        const T r = ((sv - bv[k]) + av[k]) / T(a.D[j]);
+        const T r = from_st ? (sv - wv[k]) / T(a.D[j]) : ((sv - bv[k]) + av[k]) / T(a.D[j]);
         p2 += double(r) * double(r);
         by += double(bv[k]) * double(y);
-        a.W[o] = next_w(bv[k], av[k], sv, y, lam);
+        a.W[o] = from_st ? wv[k] - (sv + lam * y) : next_w(bv[k], av[k], sv, y, lam);
       }
     }
   }
@@ -803,6 +889,11 @@ struct Family {
   bool apply_At(FamArgs<T>& a, const T* X1, const T* X2, T* Z1, T* Z2);
   template <typename T>
   void prologue(FamArgs<T>& a);
+  template <typename T>
+  bool at_phase_b(FamArgs<T>& a);
+  // chunks of phase B partials: one per A' task block (sparse, fused into the
+  // A' product) or per RPC rows (dense)
+  int b_chunks() const { return dense ? (n + RPC - 1) / RPC : cur_tasks->nAt; }
 };
 
 // Rows of a CSR matrix into SpMM wave tasks, NWV per block (k_fam_spmm):
@@ -1027,7 +1118,8 @@ bool Family::load(int cnt, const double* C, const double* B, std::string& err) {
     q.red = dalloc((size_t)8 * ldb * 8, batch_allocs, err);
     q.orig = dalloc((size_t)ldb * 4, batch_allocs, err);
     q.pa = dalloc((size_t)std::max(1, m_items()) * ldb * 2 * 8, batch_allocs, err);
-    q.pb = dalloc((size_t)((n + RPC - 1) / RPC) * ldb * 2 * 8, batch_allocs, err);
+    const int nb = std::max({(n + RPC - 1) / RPC, tasks[0].nAt, tasks[1].nAt});
+    q.pb = dalloc((size_t)nb * ldb * 2 * 8, batch_allocs, err);
     q.pc = dalloc((size_t)nc1 * ldb * 8, batch_allocs, err);
     q.pt0 = dalloc((size_t)std::max(1, S.nsoc()) * ldb * 8, batch_allocs, err);
   }
@@ -1088,16 +1180,14 @@ bool Family::iterate(FamArgs<T>& a, int steps, std::string& err) {
   const int cols = a.ldb / LANES;
   // SpMM tasks: 64 entries while the batch is large, 16 once it is small
   cur_tasks = &tasks[(long)cols * tasks[0].nA < 2048 ? 1 : 0];
-  const int cm = m_items(), cn = (n + RPC - 1) / RPC;
-  const dim3 gn(cols, cn), gd((a.ldb + 255) / 256), block(FT);
+  const int cm = m_items(), cn = b_chunks();
+  const dim3 block(FT);
   cur_ld = a.ldb;
   for (int s = 0; s < steps; ++s) {
     mark(K_AT);
-    if (!apply_At(a, a.Y, a.W, a.U, a.V)) { err = "A' product failed"; return false; }
+    if (!at_phase_b(a)) { err = "A' product failed"; return false; }
     mark(K_B);
-    if (fp64) hipLaunchKernelGGL((k_fam_b<T, false>), gn, block, 0, st, a);
-    else hipLaunchKernelGGL((k_fam_b<T, true>), gn, block, 0, st, a);
-    hipLaunchKernelGGL(k_fam_decide<T>, gd, dim3(256), 0, st, a, cm, cn);
+    hipLaunchKernelGGL(k_fam_decide<T>, dim3(cols), block, 0, st, a, cm, cn);
     mark(K_GEMM);
     if (fp64) {
       const double one = 1.0, zero = 0.0;
@@ -1119,6 +1209,7 @@ bool Family::iterate(FamArgs<T>& a, int steps, std::string& err) {
     mark(K_C);
     hipLaunchKernelGGL(k_fam_c1<T>, dim3(cols, nc1), block, 0, st, a, c1);
     if (nc2) hipLaunchKernelGGL(k_fam_c2<T>, dim3(cols, nc2), block, 0, st, a, c2, c2_cone);
+    std::swap(a.X, a.XN);
     mark(-1);
   }
   return hipGetLastError() == hipSuccess;
@@ -1128,8 +1219,8 @@ bool Family::iterate(FamArgs<T>& a, int steps, std::string& err) {
 template <typename T>
 bool Family::apply_A(FamArgs<T>& a, const T* Xin, T* Z) {
   if (!dense) {
-    hipLaunchKernelGGL((k_fam_spmm<T, 1>), dim3(a.ldb / LANES, cur_tasks->nA), dim3(FT), 0, st, cur_tasks->A, rowA,
-                       a.ri, a.rx, Xin, Xin, Xin, Z, Z, Z, a.ldb);
+    const SpmmIn<T, 1> in{cur_tasks->A, rowA, a.ri, a.rx, Xin, Xin, Xin, a.ldb};
+    hipLaunchKernelGGL((k_fam_spmm<T, 1>), dim3(a.ldb / LANES, cur_tasks->nA), dim3(FT), 0, st, in, Z, Z, Z);
     return hipGetLastError() == hipSuccess;
   }
   const T one = T(1), zero = T(0);
@@ -1147,8 +1238,8 @@ bool Family::apply_A(FamArgs<T>& a, const T* Xin, T* Z) {
 template <typename T>
 bool Family::apply_At(FamArgs<T>& a, const T* X1, const T* X2, T* Z1, T* Z2) {
   if (!dense) {
-    hipLaunchKernelGGL((k_fam_spmm<T, 2>), dim3(a.ldb / LANES, cur_tasks->nAt), dim3(FT), 0, st, cur_tasks->At,
-                       rowAt, a.ci, a.cx, X1, X2, X2, Z1, Z2, Z2, a.ldb);
+    const SpmmIn<T, 2> in{cur_tasks->At, rowAt, a.ci, a.cx, X1, X2, X2, a.ldb};
+    hipLaunchKernelGGL((k_fam_spmm<T, 2>), dim3(a.ldb / LANES, cur_tasks->nAt), dim3(FT), 0, st, in, Z1, Z2, Z2);
     return hipGetLastError() == hipSuccess;
   }
   const T one = T(1), zero = T(0);
@@ -1166,6 +1257,22 @@ bool Family::apply_At(FamArgs<T>& a, const T* X1, const T* X2, T* Z1, T* Z2) {
     if (r != rocblas_status_success) return false;
   }
   return true;
+}
+
+// A' products and phase B: for a sparse A one fused pass (k_fam_at_b), for a
+// dense A the GEMMs then k_fam_b.
+template <typename T>
+bool Family::at_phase_b(FamArgs<T>& a) {
+  if (!dense) {
+    const SpmmIn<T, 2> in{cur_tasks->At, rowAt, a.ci, a.cx, a.Y, a.W, a.W, a.ldb};
+    hipLaunchKernelGGL(k_fam_at_b<T>, dim3(a.ldb / LANES, cur_tasks->nAt), dim3(FT), 0, st, a, in);
+    return hipGetLastError() == hipSuccess;
+  }
+  if (!apply_At(a, a.Y, a.W, a.U, a.V)) return false;
+  const dim3 gn(a.ldb / LANES, (n + RPC - 1) / RPC);
+  if (fp64) hipLaunchKernelGGL((k_fam_b<T, false>), gn, dim3(FT), 0, st, a);
+  else hipLaunchKernelGGL((k_fam_b<T, true>), gn, dim3(FT), 0, st, a);
+  return hipGetLastError() == hipSuccess;
 }
 
 // W and the stop-test partials of the current iterate (phase C forms them
@@ -1197,13 +1304,11 @@ bool Family::run_t(const float* warm, long wld, int judge, std::string& err) {
   if (!apply_A(a, a.X, a.AX)) { err = "A product failed"; return false; }
   need_prologue = true;
   if (warm && judge) {
-    const int cm = m_items(), cn = (n + RPC - 1) / RPC;
+    const int cm = m_items(), cn = b_chunks();
     a.judge = 1;
     prologue(a);
-    if (!apply_At(a, a.Y, a.W, a.U, a.V)) { err = "A' product failed"; return false; }
-    if (fp64) hipLaunchKernelGGL((k_fam_b<T, false>), dim3(ld / LANES, cn), block, 0, st, a);
-    else hipLaunchKernelGGL((k_fam_b<T, true>), dim3(ld / LANES, cn), block, 0, st, a);
-    hipLaunchKernelGGL(k_fam_decide<T>, dim3((ld + 255) / 256), dim3(256), 0, st, a, cm, cn);
+    if (!at_phase_b(a)) { err = "A' product failed"; return false; }
+    hipLaunchKernelGGL(k_fam_decide<T>, dim3(ld / LANES), block, 0, st, a, cm, cn);
     a.judge = 0;
     need_prologue = true;  // rejected instances were reset
   }
@@ -1217,6 +1322,7 @@ bool Family::run_t(const float* warm, long wld, int judge, std::string& err) {
       // replay a captured graph of `steps` iterations for this layout
       if (!gexec || g_ld != a.ldb || g_steps != steps || g_x != (const void*)a.X) {
         drop_graph();
+        const void* x0 = a.X;
         hipGraph_t g = nullptr;
         if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
           err = "stream capture failed";
@@ -1233,9 +1339,12 @@ bool Family::run_t(const float* warm, long wld, int judge, std::string& err) {
         (void)hipGraphDestroy(g);
         g_ld = a.ldb;
         g_steps = steps;
-        g_x = a.X;
+        g_x = x0;
+        // capture left a.X / a.XN as after `steps` iterations; undo, the launch redoes it
+        if (steps % 2) std::swap(a.X, a.XN);
       }
       if (hipGraphLaunch(gexec, st) != hipSuccess) { err = "graph launch failed"; return false; }
+      if (steps % 2) std::swap(a.X, a.XN);
     } else if (!iterate(a, steps, err)) {
       return false;
     }
