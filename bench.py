@@ -144,7 +144,30 @@ def plain_gpu(probs, plugin, precision):
 
 # ------------------------------------------------------------------ roofline
 
-def family_roofline(kt, fam_info, nnz, fp64, lin_rows, b_inst):
+# kernel names behind each timed kind of the family engine (sparse A)
+FAMILY_KERNELS = {"At_product": ["k_fam_at_b<{t}>"], "A_product": ["k_fam_spmm<{t}, 1>"],
+                  "xupdate_gemm": ["k_gemm<false, false, 4>"], "phase_b": ["k_fam_decide<{t}>"],
+                  "phase_c": ["k_fam_c1<{t}>", "k_fam_c2<{t}>"], "phase_a": ["k_fam_a<{t}>"]}
+
+
+def measured_traffic(profile, kind, fp64):
+    """HBM bytes per launch of `kind` from a committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE runs scaled by a calibration copy; the JSON
+    block of profiles/<profile>.md), or None when it does not cover the kind."""
+    path = os.path.join(ROOT, "profiles", profile + ".md")
+    if not os.path.exists(path):
+        return None
+    text = open(path).read()
+    if "```json" not in text:
+        return None
+    data = json.loads(text.split("```json", 1)[1].split("```", 1)[0])["kernels"]
+    names = [k.format(t="double" if fp64 else "float") for k in FAMILY_KERNELS.get(kind, [])]
+    if not names or any(k not in data for k in names):
+        return None
+    return sum(data[k]["read_bytes"] + data[k]["write_bytes"] for k in names)
+
+
+def family_roofline(kt, fam_info, nnz, fp64, lin_rows, b_inst, traffic_profile=None):
     """Roofline of the dominant kernel of a family run, from per-kernel HIP
     event totals {kind: (ms, calls, columns)}.  Algorithmic bytes per column
     (instance), values of sv bytes:
@@ -191,9 +214,13 @@ def family_roofline(kt, fam_info, nnz, fp64, lin_rows, b_inst):
         rows = (7 if b_inst else 6) * lin_rows + cone * cone_rows
         byt = cols * rows * sv
     ach = byt / (ms * 1e-3) / 1e9
-    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kind,
-            "algorithmic_bytes_per_launch": byt / max(calls, 1), "avg_launch_us": round(1e3 * ms / max(calls, 1), 2)}
+    traffic = measured_traffic(traffic_profile, kind, fp64) if traffic_profile and not dense else None
+    out = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kind,
+           "algorithmic_bytes_per_launch": byt / max(calls, 1), "avg_launch_us": round(1e3 * ms / max(calls, 1), 2)}
+    if traffic is not None:
+        out["traffic_unit"] = "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/%s.md)" % traffic_profile
+    return out
 
 
 def wg_roofline(probs, outs, plugin, sv, kernel_ms):
@@ -305,7 +332,8 @@ def run_family(args):
         "plain_gpu": plain,
         "ratio_vs_plain_gpu": round(value / plain["value"], 1) if plain else None,
     }
-    roof = family_roofline(kt, info, int(fam.Ap[-1]), prec == "fp64", fam.f + fam.l, False)
+    prof = "family_portfolio_16384_pmc_traffic" if (args.count or 16384) == 16384 else None
+    roof = family_roofline(kt, info, int(fam.Ap[-1]), prec == "fp64", fam.f + fam.l, False, prof)
     acc = {"solved": solved, "count": n_te, "verified_fp64": v_warm,
            "mean_iters": float(warm["iters"].mean()), "max_iters": int(warm["iters"].max())}
     return value, wall, "f32+bf16" if prec == "mixed" else "f64", config, roof, acc, extra, probs[:args.cpu_sample], "direct"
@@ -392,6 +420,17 @@ def run_single(args, p, config, plugin):
     return solved / wall, wall, "f64" if prec == "fp64" else "f32", config, roof, acc, {}, [p], plugin
 
 
+def calibration_copy(mib=256):
+    """One plain copy of `mib` MiB between two device buffers by the library's
+    copy kernel (k_copy): under rocprofv3 --pmc its FETCH_SIZE / WRITE_SIZE
+    per dispatch, against the known mib * 1024 KiB, gives the counters'
+    scale."""
+    src = pdos_amd.DeviceArray(np.ones(mib << 18, dtype=np.float32))
+    dst = pdos_amd.DeviceArray(shape=(mib << 18,))
+    dst.copy_from(src)
+    log(f"calibration copy: {mib} MiB device to device (k_copy)")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -407,6 +446,9 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--products", type=int, default=0, help="family, mixed: bf16 products (1-3)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="end with one 256 MiB device-to-device copy (the reference kernel for "
+                         "scaling FETCH_SIZE / WRITE_SIZE counters in profiled runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; --gpus must be 1", file=sys.stderr)
@@ -454,6 +496,8 @@ def main():
     if cpu:
         line["ratio_vs_cpu_baseline"] = round(value / cpu["value"], 1)
     print(json.dumps(line), flush=True)
+    if args.calibrate_copy:
+        calibration_copy()
 
 
 if __name__ == "__main__":

@@ -104,6 +104,9 @@ void *pdos_device_alloc(size_t bytes);
 void pdos_device_free(void *p);
 int pdos_device_upload(void *dst, const void *src, size_t bytes);
 int pdos_device_download(void *dst, const void *src, size_t bytes);
+/* dst = src for device memory, by one plain copy kernel of this library (the
+ * reference kernel for scaling memory-traffic counters); synchronous. */
+int pdos_device_copy(void *dst, const void *src, size_t bytes);
 
 /* A family of cone programs sharing A, the cones and the parameters of the
  * template data d (its b is the default b, its c is ignored), differing in c

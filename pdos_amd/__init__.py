@@ -49,7 +49,8 @@ class PdosInfo(C.Structure):
 EXPORTS = ["pdos", "freeData", "freeSol", "tic", "toc", "tocq", "pdos_default_options",
            "pdos_ex", "pdos_last_info", "pdos_batch", "pdos_batch_prepare", "pdos_batch_run",
            "pdos_batch_fetch", "pdos_batch_free", "pdos_batch_set_warm_dev", "pdos_device_alloc",
-           "pdos_device_free", "pdos_device_upload", "pdos_device_download", "pdos_mlp_create",
+           "pdos_device_free", "pdos_device_upload", "pdos_device_download", "pdos_device_copy",
+           "pdos_mlp_create",
            "pdos_mlp_free", "pdos_mlp_num_params", "pdos_mlp_get_params", "pdos_mlp_set_params",
            "pdos_mlp_set_affine", "pdos_mlp_loss_grad", "pdos_mlp_train", "pdos_mlp_predict",
            "pdos_mlp_predict_dev", "pdos_mlp_last_error", "pdos_family_create", "pdos_family_load",
@@ -97,6 +98,8 @@ def lib(plugin: str = "indirect"):
         L.pdos_device_free.argtypes = [C.c_void_p]
         L.pdos_device_upload.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
         L.pdos_device_download.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+        L.pdos_device_copy.restype = C.c_int
+        L.pdos_device_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
         fp = C.POINTER(C.c_float)
         L.pdos_mlp_create.restype = C.c_void_p
         L.pdos_mlp_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_ulong, C.c_int]
@@ -256,6 +259,13 @@ class DeviceArray:
         out = np.empty(self.shape, dtype=np.float32)
         self.L.pdos_device_download(out.ctypes.data, self.ptr, self.nbytes)
         return out
+
+    def copy_from(self, other: "DeviceArray"):
+        """Device-to-device copy by the library's copy kernel."""
+        if other.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        if self.L.pdos_device_copy(self.ptr, other.ptr, self.nbytes) != 0:
+            raise RuntimeError("device copy failed")
 
     def free(self):
         if self.ptr:

@@ -662,6 +662,13 @@ int pdos_device_download(void* dst, const void* src, size_t bytes) {
   return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? 0 : -1;
 }
 
+int pdos_device_copy(void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return 0;
+  if (!dst || !src) return -1;
+  if (pdos_amd::device_copy(dst, src, bytes, nullptr) != hipSuccess) return -1;
+  return hipDeviceSynchronize() == hipSuccess ? 0 : -1;
+}
+
 int pdos_batch(const Data* const* ds, const Cone* const* ks, int count, const PdosOptions* opt,
                Sol** sols, PdosInfo* infos) {
   g_err.clear();

@@ -38,6 +38,7 @@ struct Packer {
 int lanes_for(long nnz, long rows);
 double now_ms();
 
+hipError_t device_copy(void* dst, const void* src, size_t bytes, hipStream_t st);
 hipError_t wg_launch(const WgInst* d_insts, int count, int lds_bytes, bool fp64, bool direct,
                      hipStream_t st);
 

@@ -166,19 +166,38 @@ struct SpmmIn {
   const int4* tasks; const int* rowof; const int* idx; const T* val;
   const T* X1; const T* X2; const T* X3;
   int ldb;
+  int cols, nitems;  // grid: cols column blocks x nitems task blocks, 1-D
 };
 
-// The wave's task of block blockIdx.y.  begin(row) is called when the wave
+// (column block bx, task block by) of this workgroup in a 1-D grid of
+// cols * nitems.  The hardware deals workgroups to the 8 XCDs round robin;
+// with cols % 8 == 0 each XCD's consecutive workgroups take all task blocks
+// of one column block before the next, so the rows they gather for those 64
+// instances stay in that XCD's L2.
+__device__ __forceinline__ void spmm_block(int cols, int nitems, int& bx, int& by) {
+  const int L = blockIdx.x;
+  if ((cols & 7) == 0) {
+    const int seq = L >> 3;
+    bx = (seq / nitems) * 8 + (L & 7);
+    by = seq % nitems;
+  } else {
+    bx = L % cols;
+    by = L / cols;
+  }
+}
+
+// The wave's task of task block by.  begin(row) is called when the wave
 // starts a row (before its entries are gathered: loads issued there overlap
 // the row's products), done(row, acc) when the row is complete; for a split
 // row, wave 0 gets done(row, block total).  Rows without entries inside a
 // task are begun and done with zero sums.
 template <typename T, int NR, typename Begin, typename Done>
-__device__ __forceinline__ void spmm_core(const SpmmIn<T, NR>& in, double* buf, Begin begin, Done done) {
+__device__ __forceinline__ void spmm_core(const SpmmIn<T, NR>& in, int bx, int by, double* buf, Begin begin,
+                                          Done done) {
   const int lane = threadIdx.x & 63, w = wave_id();
-  const int inst = blockIdx.x * LANES + lane;
-  const int4 tk = in.tasks[blockIdx.y * NWV + w];
-  const bool split = in.tasks[blockIdx.y * NWV].y < 0;  // block-uniform
+  const int inst = bx * LANES + lane;
+  const int4 tk = in.tasks[by * NWV + w];
+  const bool split = in.tasks[by * NWV].y < 0;  // block-uniform
   T acc[NR];
 #pragma unroll
   for (int k = 0; k < NR; ++k) acc[k] = T(0);
@@ -258,8 +277,10 @@ template <typename T, int NR>
 __global__ __launch_bounds__(FT) void k_fam_spmm(SpmmIn<T, NR> in, T* __restrict__ Y1, T* __restrict__ Y2,
                                                  T* __restrict__ Y3) {
   __shared__ double buf[NWV * LANES];
-  const int inst = blockIdx.x * LANES + (threadIdx.x & 63);
-  spmm_core<T, NR>(in, buf, [](int) {}, [&](int row, const T (&v)[NR]) {
+  int bx, by;
+  spmm_block(in.cols, in.nitems, bx, by);
+  const int inst = bx * LANES + (threadIdx.x & 63);
+  spmm_core<T, NR>(in, bx, by, buf, [](int) {}, [&](int row, const T (&v)[NR]) {
     const size_t o = (size_t)row * in.ldb + inst;
     Y1[o] = v[0];
     if (NR > 1) Y2[o] = v[1];
@@ -271,17 +292,19 @@ __global__ __launch_bounds__(FT) void k_fam_spmm(SpmmIn<T, NR> in, T* __restrict
 // U_i = (A'Y)_i and V_i = (A'W)_i stay in registers and give the partial
 // |(c + U) ./ E|^2 and c'X and the x-update right-hand side (as k_fam_b).
 // c_i and X_i are loaded when the row begins.  Partials per block into
-// pb[blockIdx.y].
+// pb[task block].
 template <typename T>
 __global__ __launch_bounds__(FT) void k_fam_at_b(FamArgs<T> a, SpmmIn<T, 2> in) {
   __shared__ double buf[NWV * LANES];
   const int lane = threadIdx.x & 63, w = wave_id();
-  const int inst = blockIdx.x * LANES + lane;
+  int bx, by;
+  spmm_block(in.cols, in.nitems, bx, by);
+  const int inst = bx * LANES + lane;
   const bool act = inst < a.count && !a.done[inst];
   const T lam = T(a.lam[inst]);
   T c_i = T(0), x_i = T(0);
   double d2 = 0.0, cx = 0.0;
-  spmm_core<T, 2>(in, buf,
+  spmm_core<T, 2>(in, bx, by, buf,
                   [&](int i) {
                     const size_t o = (size_t)i * a.ldb + inst;
                     c_i = a.c[o];
@@ -306,7 +329,7 @@ __global__ __launch_bounds__(FT) void k_fam_at_b(FamArgs<T> a, SpmmIn<T, 2> in) 
   d2 = waves_sum(d2, buf);
   cx = waves_sum(cx, buf);
   if (act && w == 0) {
-    double* pb = a.pb + ((size_t)blockIdx.y * a.ldb + inst) * 2;
+    double* pb = a.pb + ((size_t)by * a.ldb + inst) * 2;
     pb[0] = d2;
     pb[1] = cx;
   }
@@ -1219,8 +1242,9 @@ bool Family::iterate(FamArgs<T>& a, int steps, std::string& err) {
 template <typename T>
 bool Family::apply_A(FamArgs<T>& a, const T* Xin, T* Z) {
   if (!dense) {
-    const SpmmIn<T, 1> in{cur_tasks->A, rowA, a.ri, a.rx, Xin, Xin, Xin, a.ldb};
-    hipLaunchKernelGGL((k_fam_spmm<T, 1>), dim3(a.ldb / LANES, cur_tasks->nA), dim3(FT), 0, st, in, Z, Z, Z);
+    const int cols = a.ldb / LANES;
+    const SpmmIn<T, 1> in{cur_tasks->A, rowA, a.ri, a.rx, Xin, Xin, Xin, a.ldb, cols, cur_tasks->nA};
+    hipLaunchKernelGGL((k_fam_spmm<T, 1>), dim3(cols * cur_tasks->nA), dim3(FT), 0, st, in, Z, Z, Z);
     return hipGetLastError() == hipSuccess;
   }
   const T one = T(1), zero = T(0);
@@ -1238,8 +1262,9 @@ bool Family::apply_A(FamArgs<T>& a, const T* Xin, T* Z) {
 template <typename T>
 bool Family::apply_At(FamArgs<T>& a, const T* X1, const T* X2, T* Z1, T* Z2) {
   if (!dense) {
-    const SpmmIn<T, 2> in{cur_tasks->At, rowAt, a.ci, a.cx, X1, X2, X2, a.ldb};
-    hipLaunchKernelGGL((k_fam_spmm<T, 2>), dim3(a.ldb / LANES, cur_tasks->nAt), dim3(FT), 0, st, in, Z1, Z2, Z2);
+    const int cols = a.ldb / LANES;
+    const SpmmIn<T, 2> in{cur_tasks->At, rowAt, a.ci, a.cx, X1, X2, X2, a.ldb, cols, cur_tasks->nAt};
+    hipLaunchKernelGGL((k_fam_spmm<T, 2>), dim3(cols * cur_tasks->nAt), dim3(FT), 0, st, in, Z1, Z2, Z2);
     return hipGetLastError() == hipSuccess;
   }
   const T one = T(1), zero = T(0);
@@ -1264,8 +1289,9 @@ bool Family::apply_At(FamArgs<T>& a, const T* X1, const T* X2, T* Z1, T* Z2) {
 template <typename T>
 bool Family::at_phase_b(FamArgs<T>& a) {
   if (!dense) {
-    const SpmmIn<T, 2> in{cur_tasks->At, rowAt, a.ci, a.cx, a.Y, a.W, a.W, a.ldb};
-    hipLaunchKernelGGL(k_fam_at_b<T>, dim3(a.ldb / LANES, cur_tasks->nAt), dim3(FT), 0, st, a, in);
+    const int cols = a.ldb / LANES;
+    const SpmmIn<T, 2> in{cur_tasks->At, rowAt, a.ci, a.cx, a.Y, a.W, a.W, a.ldb, cols, cur_tasks->nAt};
+    hipLaunchKernelGGL(k_fam_at_b<T>, dim3(cols * cur_tasks->nAt), dim3(FT), 0, st, a, in);
     return hipGetLastError() == hipSuccess;
   }
   if (!apply_At(a, a.Y, a.W, a.U, a.V)) return false;

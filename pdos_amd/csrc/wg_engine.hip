@@ -359,7 +359,26 @@ hipError_t launch_t(const WgInst* d_insts, int count, int lds_bytes, hipStream_t
   return hipGetLastError();
 }
 
+// Plain device copy, 16 bytes per thread and step (the tail bytewise).
+__global__ void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n16,
+                       const unsigned char* __restrict__ src_b, unsigned char* __restrict__ dst_b, size_t tail) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tail; i += stride) dst_b[i] = src_b[i];
+}
+
 }  // namespace
+
+// dst = src (device memory, `bytes`), by a kernel of this library: the
+// reference kernel for scaling memory-traffic counters.
+hipError_t device_copy(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  const size_t n16 = bytes / 16, tail = bytes % 16;
+  const unsigned char* sb = static_cast<const unsigned char*>(src) + n16 * 16;
+  unsigned char* db = static_cast<unsigned char*>(dst) + n16 * 16;
+  hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, st, static_cast<const uint4*>(src), static_cast<uint4*>(dst),
+                     n16, sb, db, tail);
+  return hipGetLastError();
+}
 
 // Launches the workgroup engine over `count` instances (one workgroup each).
 hipError_t wg_launch(const WgInst* d_insts, int count, int lds_bytes, bool fp64, bool direct,

@@ -129,3 +129,13 @@ def test_demo_cli_on_reference_fixture(plugin):
     assert "status: Solved" in r.stdout
     # VERBOSE is 1 in the file: the summary row carries the iteration count
     assert any(line.strip().startswith("20 |") for line in r.stdout.splitlines()), r.stdout
+
+
+def test_device_copy_kernel():
+    """pdos_device_copy (the counter-calibration copy): bulk 16-byte path and
+    a 12-byte tail."""
+    src = np.random.default_rng(3).standard_normal(1003).astype(np.float32)
+    a = pdos_amd.DeviceArray(src)
+    b = pdos_amd.DeviceArray(shape=(1003,))
+    b.copy_from(a)
+    np.testing.assert_array_equal(b.numpy(), src)
