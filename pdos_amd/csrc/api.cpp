@@ -95,11 +95,31 @@ size_t Packer::zeros(size_t bytes) {
   return off;
 }
 
-int lanes_for(long nnz, long rows) {
-  const double avg = rows > 0 ? double(nnz) / double(rows) : 1.0;
-  int g = 1;
-  while (g < 64 && g < avg) g <<= 1;
-  return g;
+// Row blocks {r0, r1, p0, p1} (flattened) of a CSR matrix for the workgroup
+// engine's SpMV: runs of whole rows of one class, at most kWgChunk entries
+// and kWgRowsPerBlock rows per block.  Rows shorter than kWgLongRow entries
+// form blocks {r0, r1, ..}; longer rows form blocks {r0, -r1, ..} (a wave
+// per row); a row longer than kWgChunk is a block of its own.
+std::vector<int> wg_row_blocks(const std::vector<int>& ptr) {
+  std::vector<int> out;
+  const int nrows = (int)ptr.size() - 1;
+  auto len = [&](int r) { return ptr[r + 1] - ptr[r]; };
+  int r = 0;
+  while (r < nrows) {
+    if (len(r) > kWgChunk) {
+      out.insert(out.end(), {r, r + 1, ptr[r], ptr[r + 1]});
+      ++r;
+      continue;
+    }
+    const bool lng = len(r) >= kWgLongRow;
+    int e = r;
+    while (e < nrows && e - r < kWgRowsPerBlock && len(e) <= kWgChunk && (len(e) >= kWgLongRow) == lng &&
+           ptr[e + 1] - ptr[r] <= kWgChunk)
+      ++e;
+    out.insert(out.end(), {r, lng ? -e : e, ptr[r], ptr[e]});
+    r = e;
+  }
+  return out;
 }
 
 double now_ms() {
@@ -187,7 +207,8 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
   const int count = (int)S.size();
   const int tsz = fp64 ? 8 : 4;
   Packer P;
-  struct Off { size_t soc, rp, ri, rx, cp, ci, cx, b, c, D, E, x0, s0, y0;
+  struct Off { size_t soc, rp, ri, rx, cp, ci, cx, rb, cb, b, c, D, E, x0, s0, y0;
+               int nrb, ncb;
                size_t perm, lrp, lri, lrx, lcp, lci, lcx, ld, flp, flr, blp, blr; };
   std::vector<Off> pk(count);
   xo.resize(count); so.resize(count); yo.resize(count);
@@ -211,6 +232,11 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
       k.cp = put_vec(P, s.cp);
       k.ci = put_vec(P, s.ci);
       k.cx = put_vals(P, s.cx, fp64);
+      const std::vector<int> rb = wg_row_blocks(s.rp), cb = wg_row_blocks(s.cp);
+      k.rb = put_vec(P, rb);
+      k.cb = put_vec(P, cb);
+      k.nrb = (int)rb.size() / 4;
+      k.ncb = (int)cb.size() / 4;
       k.D = put_vec(P, s.D);
       k.E = put_vec(P, s.E);
     }
@@ -263,8 +289,8 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
     WgInst& w = insts[i];
     std::memset(&w, 0, sizeof(w));
     w.n = s.n; w.m = s.m; w.f = s.f; w.l = s.l; w.nsoc = sr.nsoc(); w.N = s.n + s.m;
-    w.ga = lanes_for(sr.nnz(), s.m);
-    w.gat = lanes_for(sr.nnz(), s.n);
+    w.nrb = k.nrb;
+    w.ncb = k.ncb;
     w.max_iters = (int)std::min<long>(std::max<long>(par[i].MAX_ITERS, 0), 0x7fffffff);
     w.cg_max_its = (int)std::min<long>(std::max<long>(par[i].CG_MAX_ITS, 0), 0x7fffffff);
     w.eps = par[i].EPS_ABS;
@@ -280,6 +306,8 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
     w.cp = reinterpret_cast<const int*>(base + k.cp);
     w.ci = reinterpret_cast<const int*>(base + k.ci);
     w.cx = base + k.cx;
+    w.rblk = reinterpret_cast<const int*>(base + k.rb);
+    w.cblk = reinterpret_cast<const int*>(base + k.cb);
     w.b = reinterpret_cast<const double*>(base + k.b);
     w.c = reinterpret_cast<const double*>(base + k.c);
     w.D = reinterpret_cast<const double*>(base + k.D);

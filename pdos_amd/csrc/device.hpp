@@ -18,7 +18,7 @@ struct DevResult {
 // pointers; matrix values have the engine's bulk type (double or float).
 struct WgInst {
   int n, m, f, l, nsoc, N;
-  int ga, gat;             // lanes per row for A rows / A' rows (power of 2, <= 64)
+  int nrb, ncb;            // row blocks of A rows / A' rows (spmv_blocks)
   int max_iters, cg_max_its;
   int nfl, nbl;            // direct: forward / backward level counts
   double eps, alpha, cg_tol_sq, lambda, pscale, dscale;
@@ -29,6 +29,8 @@ struct WgInst {
   const int* cp;           // A' rows = A columns (CSC), n + 1
   const int* ci;
   const void* cx;
+  const int* rblk;         // A row blocks {r0, r1, p0, p1}, nrb of them (int4)
+  const int* cblk;         // A' row blocks, ncb of them
   const double* b;         // scaled b, c and the scalings
   const double* c;
   const double* D;
@@ -61,6 +63,9 @@ struct WgInst {
 };
 
 constexpr int kWgThreads = 256;
+constexpr int kWgChunk = 2048;        // entries per SpMV row block (products staged in LDS)
+constexpr int kWgRowsPerBlock = 1024;  // rows per SpMV row block
+constexpr int kWgLongRow = 64;        // rows with this many entries are summed by a wave
 
 // Dynamic LDS bytes for one instance in the workgroup engine.
 inline int wg_lds_bytes(int n, int m, int N, bool direct, int tsize) {
@@ -68,6 +73,7 @@ inline int wg_lds_bytes(int n, int m, int N, bool direct, int tsize) {
   long bytes;
   if (!direct) bytes = al((long)(4 * (long)n + 5 * (long)m) * tsize);
   else bytes = al((long)(n + 4 * (long)m) * tsize) + al((long)N * 8);
+  bytes += al((long)kWgChunk * tsize);  // SpMV products
   bytes += 2 * 16 * 4 * 8;  // reduction slots
   return (int)bytes;
 }

@@ -35,7 +35,6 @@ struct Packer {
   size_t zeros(size_t bytes);
 };
 
-int lanes_for(long nnz, long rows);
 double now_ms();
 
 hipError_t device_copy(void* dst, const void* src, size_t bytes, hipStream_t st);

@@ -35,12 +35,6 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-template <typename T>
-__device__ __forceinline__ T group_sum(T v, int G) {
-  for (int o = G >> 1; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
 // Sums NV doubles over the workgroup; every thread receives the totals.  One
 // barrier; two alternating slots make back-to-back reductions race free (a
 // slot is rewritten only after every thread has passed the next barrier).
@@ -65,25 +59,116 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* red, int& slo
   slot ^= 1;
 }
 
-// Row-gather SpMV over `nrows` compressed rows with G lanes per row.  The
-// outer loop bound is uniform so every lane reaches the shuffles.
-// gx(col) supplies the gathered operand, emit(row, sum) runs on lane 0 of
-// the row's group.
+// SpMV over host-built row blocks {r0, r1, p0, p1} of whole rows (at most
+// kWgChunk entries and kWgRowsPerBlock rows).  Per block every thread first
+// loads its kWgChunk / NT entries and the row pointers it needs (coalesced,
+// all in flight), forms the products val * gx(idx) into LDS, then the rows
+// are summed from LDS and emit(row, sum) is called:
+//   r1 > 0: rows shorter than kWgLongRow entries, one thread per row, in
+//           entry order;
+//   r1 < 0: rows [r0, -r1) of kWgLongRow entries or more, one wave per row
+//           (lanes stride the row, then a wave reduction; emit on lane 0);
+//   a single row longer than kWgChunk (p1 - p0 > kWgChunk) is summed by the
+//   whole workgroup (emit on thread 0).
+// Starts with a barrier: `prod` and the gathered vector may still be in use
+// by the caller's previous phase.
 template <typename T, typename Gather, typename Emit>
-__device__ __forceinline__ void spmv_rows(int nrows, const int* __restrict__ ptr,
-                                          const int* __restrict__ idx, const T* __restrict__ val,
-                                          int G, Gather gx, Emit emit) {
-  const int g = threadIdx.x & (G - 1);
-  const int grp = threadIdx.x / G, ngrp = NT / G;
-  for (int r0 = 0; r0 < nrows; r0 += ngrp) {
-    const int r = r0 + grp;
-    T acc = T(0);
-    if (r < nrows) {
-      const int pe = ptr[r + 1];
-      for (int p = ptr[r] + g; p < pe; p += G) acc += val[p] * gx(idx[p]);
+__device__ __forceinline__ void spmv_blocks(const int4* __restrict__ blk, int nblk, const int* __restrict__ ptr,
+                                            const int* __restrict__ idx, const T* __restrict__ val, T* prod,
+                                            double* red, int& slot, Gather gx, Emit emit) {
+  constexpr int PT = kWgChunk / NT;
+  constexpr int RT = kWgRowsPerBlock / NT;
+  constexpr int LT = kWgChunk / kWgLongRow / NW;  // long rows per wave and block
+  constexpr int QT = RT > LT ? RT : LT;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  int q0[QT], q1[QT], ci[PT];
+  T cv[PT];
+  // entries of block dd (skipped for a single row longer than kWgChunk)
+  auto load_entries = [&](const int4& dd) {
+    const int len = dd.w - dd.z;
+    if (len > kWgChunk) return;
+#pragma unroll
+    for (int u = 0; u < PT; ++u) {
+      const int e = tid + u * NT;
+      if (e < len) { ci[u] = idx[dd.z + e]; cv[u] = val[dd.z + e]; }
     }
-    acc = group_sum(acc, G);
-    if (r < nrows && g == 0) emit(r, acc);
+  };
+  // the row pointers this thread (short rows) or wave (long rows) needs
+  auto load_rows = [&](const int4& dd) {
+    if (dd.w - dd.z > kWgChunk) return;
+    if (dd.y >= 0) {
+#pragma unroll
+      for (int k = 0; k < RT; ++k) {
+        const int r = dd.x + tid + k * NT;
+        if (r < dd.y) { q0[k] = ptr[r] - dd.z; q1[k] = ptr[r + 1] - dd.z; }
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < LT; ++k) {
+        const int r = dd.x + w + k * NW;
+        if (r < -dd.y) { q0[k] = ptr[r] - dd.z; q1[k] = ptr[r + 1] - dd.z; }
+      }
+    }
+  };
+  if (nblk == 0) return;
+  int4 d = blk[0];
+  load_entries(d);
+  load_rows(d);
+  for (int b = 0; b < nblk; ++b) {
+    const bool more = b + 1 < nblk;
+    const int len = d.w - d.z;
+    if (len > kWgChunk) {
+      double acc[1] = {0.0};
+      for (int e = d.z + tid; e < d.w; e += NT) acc[0] += double(val[e] * gx(idx[e]));
+      block_sum<1>(acc, red, slot);
+      if (tid == 0) emit(d.x, T(acc[0]));
+      if (more) {
+        d = blk[b + 1];
+        load_entries(d);
+        load_rows(d);
+      }
+      continue;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < PT; ++u) {
+      const int e = tid + u * NT;
+      if (e < len) prod[e] = cv[u] * gx(ci[u]);
+    }
+    __syncthreads();
+    // the next block's entries are in flight while this block's rows are summed
+    int4 dn = d;
+    if (more) {
+      dn = blk[b + 1];
+      load_entries(dn);
+    }
+    if (d.y >= 0) {
+#pragma unroll
+      for (int k = 0; k < RT; ++k) {
+        const int r = d.x + tid + k * NT;
+        if (r < d.y) {
+          T acc = T(0);
+#pragma unroll 4
+          for (int e = q0[k]; e < q1[k]; ++e) acc += prod[e];
+          emit(r, acc);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < LT; ++k) {
+        const int r = d.x + w + k * NW;
+        if (r < -d.y) {
+          double acc = 0.0;
+          for (int e = q0[k] + lane; e < q1[k]; e += 64) acc += double(prod[e]);
+          acc = wave_sum(acc);
+          if (lane == 0) emit(r, T(acc));
+        }
+      }
+    }
+    if (more) {
+      load_rows(dn);
+      d = dn;
+    }
   }
 }
 
@@ -130,7 +215,7 @@ __global__ __launch_bounds__(NT) void k_wg_solve(const WgInst* __restrict__ inst
   T *p = nullptr, *q = nullptr, *u = nullptr, *t = nullptr;
   T *s, *y, *st, *ax;
   double* bp = nullptr;
-  unsigned char* tail;
+  unsigned char* tail;  // SpMV products, then the reduction slots
   if (!DIRECT) {
     p = x + n; q = p + n; u = q + n;
     s = u + n; y = s + m; st = y + m; t = st + m; ax = t + m;
@@ -141,7 +226,8 @@ __global__ __launch_bounds__(NT) void k_wg_solve(const WgInst* __restrict__ inst
     bp = reinterpret_cast<double*>(after);
     tail = after + (((size_t)I.N * 8 + 15) & ~(size_t)15);
   }
-  double* red = reinterpret_cast<double*>(tail);
+  T* prod = reinterpret_cast<T*>(tail);
+  double* red = reinterpret_cast<double*>(tail + (((size_t)kWgChunk * sizeof(T) + 15) & ~(size_t)15));
 
   const T* __restrict__ rx = reinterpret_cast<const T*>(I.rx);
   const T* __restrict__ cx = reinterpret_cast<const T*>(I.cx);
@@ -154,7 +240,10 @@ __global__ __launch_bounds__(NT) void k_wg_solve(const WgInst* __restrict__ inst
   const double* __restrict__ Dg = I.D;
   const double* __restrict__ Eg = I.E;
   const int f = I.f, lend = I.f + I.l, nsoc = I.nsoc;
-  const int ga = I.ga, gat = I.gat;
+  const int4* __restrict__ rblk = reinterpret_cast<const int4*>(I.rblk);
+  const int4* __restrict__ cblk = reinterpret_cast<const int4*>(I.cblk);
+  const int nrb = I.nrb, ncb = I.ncb;
+  int slot = 0;
   const T lam = T(I.lambda);
   const T relax_a = T(I.alpha);
   const T relax_b = T(1.0) - T(I.alpha);
@@ -171,10 +260,11 @@ __global__ __launch_bounds__(NT) void k_wg_solve(const WgInst* __restrict__ inst
     for (int j = tid; j < m; j += NT) { s[j] = T(I.s0[j]); y[j] = T(I.y0[j]); }
   }
   __syncthreads();
-  spmv_rows<T>(m, rp, ri, rx, ga, [&](int c) { return x[c]; }, [&](int j, T v) { ax[j] = v; });
+  spmv_blocks<T>(rblk, nrb, rp, ri, rx, prod, red, slot, [&](int c) { return x[c]; },
+                 [&](int j, T v) { ax[j] = v; });
   __syncthreads();
 
-  int slot = 0, it = 0, status = 1, cg_total = 0, fallback = 0;
+  int it = 0, status = 1, cg_total = 0, fallback = 0;
   bool judge = I.warm_check != 0;
   double pres = -1.0, dres = -1.0, pobj = -1.0, dobj = -1.0, eta = -1.0;
 
@@ -188,7 +278,7 @@ __global__ __launch_bounds__(NT) void k_wg_solve(const WgInst* __restrict__ inst
       acc[3] += double(bj) * double(yj);
       if (!DIRECT) t[j] = ((sj + lam * yj) - bj) + axj;
     }
-    spmv_rows<T>(n, cp, ci, cx, gat, [&](int c) { return y[c]; }, [&](int i, T v) {
+    spmv_blocks<T>(cblk, ncb, cp, ci, cx, prod, red, slot, [&](int c) { return y[c]; }, [&](int i, T v) {
       const T ci_ = T(cg[i]);
       const T r = (ci_ + v) / T(Eg[i]);
       acc[1] += double(r) * double(r);
@@ -224,7 +314,7 @@ __global__ __launch_bounds__(NT) void k_wg_solve(const WgInst* __restrict__ inst
     if (!DIRECT) {
       // ---- CG on (I + A'A) x = x - lam c + A'(b - s - lam y) ----
       double r1[1] = {0.0};
-      spmv_rows<T>(n, cp, ci, cx, gat, [&](int c) { return t[c]; }, [&](int i, T v) {
+      spmv_blocks<T>(cblk, ncb, cp, ci, cx, prod, red, slot, [&](int c) { return t[c]; }, [&](int i, T v) {
         const T qi = (-lam) * T(cg[i]) - v;
         q[i] = qi;
         p[i] = T(0);
@@ -236,11 +326,11 @@ __global__ __launch_bounds__(NT) void k_wg_solve(const WgInst* __restrict__ inst
       int k = 0;
       if (rho > I.cg_tol_sq) {
         while (k < I.cg_max_its) {
-          spmv_rows<T>(m, rp, ri, rx, ga, [&](int c) { return beta * p[c] + q[c]; },
-                       [&](int j, T v) { t[j] = v; });
+          spmv_blocks<T>(rblk, nrb, rp, ri, rx, prod, red, slot, [&](int c) { return beta * p[c] + q[c]; },
+                         [&](int j, T v) { t[j] = v; });
           __syncthreads();
           double r2[1] = {0.0};
-          spmv_rows<T>(n, cp, ci, cx, gat, [&](int c) { return t[c]; }, [&](int i, T v) {
+          spmv_blocks<T>(cblk, ncb, cp, ci, cx, prod, red, slot, [&](int c) { return t[c]; }, [&](int i, T v) {
             const T pi = beta * p[i] + q[i];
             p[i] = pi;
             const T ui = pi + v;
@@ -295,7 +385,7 @@ __global__ __launch_bounds__(NT) void k_wg_solve(const WgInst* __restrict__ inst
     }
 
     // ---- s~ = b - A x, relax, project, dual update (pdos.c:232-249,296-303) ----
-    spmv_rows<T>(m, rp, ri, rx, ga, [&](int c) { return x[c]; }, [&](int j, T v) {
+    spmv_blocks<T>(rblk, nrb, rp, ri, rx, prod, red, slot, [&](int c) { return x[c]; }, [&](int j, T v) {
       ax[j] = v;
       T stj = DIRECT ? st[j] : T(bg[j]) - v;
       stj = relax_a * stj + relax_b * s[j];
