@@ -142,6 +142,10 @@ WgBatch::~WgBatch() {
   if (own_mem && dbuf) (void)hipFree(dbuf);
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
+  for (int c = 0; c < kClasses - 1; ++c) {
+    if (xev[c]) (void)hipEventDestroy(xev[c]);
+    if (xs[c]) (void)hipStreamDestroy(xs[c]);
+  }
 }
 
 static uint64_t fnv(uint64_t h, const void* p, size_t bytes) {
@@ -272,6 +276,7 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
   }
   res_off = P.zeros(sizeof(DevResult) * count);
   inst_off = P.zeros(sizeof(WgInst) * count);
+  cinst_off = P.zeros(sizeof(WgInst) * count);
   bytes = P.buf.size();
   if (borrow) {
     if (!C.reserve(bytes, bytes, err)) return false;
@@ -338,7 +343,33 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
     w.res = reinterpret_cast<DevResult*>(base + res_off) + i;
     w.lds_bytes = wg_lds_bytes(s.n, s.m, s.n + s.m, direct, tsz);
   }
+  // launch classes by workgroups per CU that the LDS allows (at most
+  // kClasses; the engine's registers allow kClasses), biggest first, and
+  // within a class the instances with the most nonzeros first
+  {
+    const long cu_lds = 160 * 1024;
+    auto occ = [&](int i) { return (int)std::max<long>(1, std::min<long>(kClasses, cu_lds / insts[i].lds_bytes)); };
+    order.resize(count);
+    for (int i = 0; i < count; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+      const int oa = occ(a), ob = occ(b);
+      if (oa != ob) return oa < ob;
+      return S[rep[grp[a]]].nnz() > S[rep[grp[b]]].nnz();
+    });
+    ncls = 0;
+    for (int k = 0; k < count; ++k) {
+      if (k == 0 || occ(order[k]) != occ(order[k - 1])) {
+        cls_start[ncls] = k;
+        cls_lds[ncls] = 0;
+        ++ncls;
+      }
+      cls_lds[ncls - 1] = std::max(cls_lds[ncls - 1], insts[order[k]].lds_bytes);
+    }
+    cls_start[ncls] = count;
+  }
   std::memcpy(P.buf.data() + inst_off, insts.data(), sizeof(WgInst) * count);
+  for (int k = 0; k < count; ++k)
+    std::memcpy(P.buf.data() + cinst_off + sizeof(WgInst) * k, &insts[order[k]], sizeof(WgInst));
   hipError_t e;
   if (borrow) {
     std::memcpy(C.hbuf, P.buf.data(), bytes);
@@ -349,8 +380,27 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
   if (e == hipSuccess && !ev0) e = hipEventCreate(&ev0);
   if (e == hipSuccess && !ev1) e = hipEventCreate(&ev1);
+  for (int c = 0; c + 1 < ncls && e == hipSuccess; ++c) {
+    if (!xs[c]) e = hipStreamCreateWithFlags(&xs[c], hipStreamNonBlocking);
+    if (e == hipSuccess && !xev[c]) e = hipEventCreateWithFlags(&xev[c], hipEventDisableTiming);
+  }
   if (e != hipSuccess) { err = std::string("upload: ") + hipGetErrorString(e); return false; }
   upload_ms = now_ms() - t0;
+  return true;
+}
+
+// Device records in instance order and in class order, from `insts`.
+bool WgBatch::upload_records(std::string& err) {
+  unsigned char* base = static_cast<unsigned char*>(dbuf);
+  const size_t count = insts.size();
+  std::vector<WgInst> byc(count);
+  for (size_t k = 0; k < count; ++k) byc[k] = insts[order[k]];
+  hipError_t e = hipMemcpyAsync(base + inst_off, insts.data(), sizeof(WgInst) * count, hipMemcpyHostToDevice,
+                                stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(base + cinst_off, byc.data(), sizeof(WgInst) * count, hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) { err = std::string("records: ") + hipGetErrorString(e); return false; }
   return true;
 }
 
@@ -359,20 +409,28 @@ bool WgBatch::set_warm(const float* dW, long ld, int check, std::string& err) {
     insts[i].wsrc = dW ? dW + (size_t)i * ld : nullptr;
     insts[i].warm_check = dW ? check : 0;
   }
-  unsigned char* base = static_cast<unsigned char*>(dbuf);
-  hipError_t e = hipMemcpyAsync(base + inst_off, insts.data(), sizeof(WgInst) * insts.size(),
-                                hipMemcpyHostToDevice, stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(stream);
-  if (e != hipSuccess) { err = std::string("set_warm: ") + hipGetErrorString(e); return false; }
+  if (!upload_records(err)) {
+    err = "set_warm: " + err;
+    return false;
+  }
   return true;
 }
 
+// One launch per class, class c > 0 on its own stream forked from and joined
+// back into `stream` (ev0 .. ev1 covers all of them).
 bool WgBatch::run(std::string& err) {
   unsigned char* base = static_cast<unsigned char*>(dbuf);
+  const WgInst* recs = reinterpret_cast<const WgInst*>(base + cinst_off);
   const double t0 = now_ms();
   hipError_t e = hipEventRecord(ev0, stream);
-  if (e == hipSuccess)
-    e = wg_launch(reinterpret_cast<const WgInst*>(base + inst_off), (int)S.size(), lds, fp64, direct, stream);
+  for (int c = 0; c < ncls && e == hipSuccess; ++c) {
+    hipStream_t sc = c == 0 ? stream : xs[c - 1];
+    if (c > 0) e = hipStreamWaitEvent(sc, ev0, 0);
+    if (e == hipSuccess)
+      e = wg_launch(recs + cls_start[c], cls_start[c + 1] - cls_start[c], cls_lds[c], fp64, direct, sc);
+    if (e == hipSuccess && c > 0) e = hipEventRecord(xev[c - 1], sc);
+    if (e == hipSuccess && c > 0) e = hipStreamWaitEvent(stream, xev[c - 1], 0);
+  }
   if (e == hipSuccess) e = hipEventRecord(ev1, stream);
   if (e == hipSuccess) e = hipEventSynchronize(ev1);
   float ms = 0.f;

@@ -58,6 +58,16 @@ struct WgBatch {
   std::vector<size_t> xo, so, yo;
   std::vector<WgInst> insts;  // host copy of the device records
   int lds = 0;
+  // Launch classes: instances grouped by how many of their workgroups fit
+  // in a CU's LDS, each class launched with its own LDS size on its own
+  // stream; the device holds a second copy of the records in class order.
+  static constexpr int kClasses = 4;
+  std::vector<int> order;                 // class order -> instance
+  int ncls = 0, cls_start[kClasses + 1] = {0}, cls_lds[kClasses] = {0};
+  size_t cinst_off = 0;
+  hipStream_t xs[kClasses - 1] = {nullptr, nullptr, nullptr};
+  hipEvent_t xev[kClasses - 1] = {nullptr, nullptr, nullptr};
+  bool upload_records(std::string& err);
   double upload_ms = 0, kernel_ms = 0, wall_ms = 0;
   ~WgBatch();
   // Groups instances by their scaled matrix (and scalings, cones) and drops
