@@ -227,8 +227,8 @@ __device__ __forceinline__ void spmm_core(const SpmmIn<T, NR>& in, int bx, int b
       for (int u = 0; u < kUnroll; ++u) {
         const size_t o = (size_t)cc[u] * in.ldb + inst;
         xv[u][0] = in.X1[o];
-        if (NR > 1) xv[u][1] = in.X2[o];
-        if (NR > 2) xv[u][2] = in.X3[o];
+        if constexpr (NR > 1) xv[u][1] = in.X2[o];
+        if constexpr (NR > 2) xv[u][2] = in.X3[o];
       }
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u) {
@@ -283,8 +283,8 @@ __global__ __launch_bounds__(FT) void k_fam_spmm(SpmmIn<T, NR> in, T* __restrict
   spmm_core<T, NR>(in, bx, by, buf, [](int) {}, [&](int row, const T (&v)[NR]) {
     const size_t o = (size_t)row * in.ldb + inst;
     Y1[o] = v[0];
-    if (NR > 1) Y2[o] = v[1];
-    if (NR > 2) Y3[o] = v[2];
+    if constexpr (NR > 1) Y2[o] = v[1];
+    if constexpr (NR > 2) Y3[o] = v[2];
   });
 }
 
@@ -1129,8 +1129,12 @@ bool Family::load(int cnt, const double* C, const double* B, std::string& err) {
   auto vec = [&](int rows) { return dalloc((size_t)rows * ldb * tsz, batch_allocs, err); };
   for (int k = 0; k < 2; ++k) {
     Set& q = set[k];
-    q.X = vec(np); q.XN = vec(np); q.S = vec(m); q.Y = vec(m); q.AX = vec(m); q.W = vec(m);
-    q.U = vec(n); q.V = vec(n); q.R = vec(np);
+    q.X = vec(np); q.XN = vec(np); q.S = vec(m); q.AX = vec(m);
+    q.Y = vec(2 * m);
+    q.W = q.Y ? static_cast<unsigned char*>(q.Y) + (size_t)m * ldb * tsz : nullptr;
+    q.U = vec(2 * n);
+    q.V = q.U ? static_cast<unsigned char*>(q.U) + (size_t)n * ldb * tsz : nullptr;
+    q.R = vec(np);
     q.Rcat = fp64 ? nullptr : dalloc((size_t)3 * np * ldb * 2, batch_allocs, err);
     q.c = vec(n);
     q.b = b_inst ? vec(m) : nullptr;
@@ -1268,20 +1272,20 @@ bool Family::apply_At(FamArgs<T>& a, const T* X1, const T* X2, T* Z1, T* Z2) {
     return hipGetLastError() == hipSuccess;
   }
   const T one = T(1), zero = T(0);
-  // column-major Z' (ldb x n) = X' (ldb x m) * A, A = (stored n x m)^T
-  for (int k = 0; k < 2; ++k) {
+  // column-major Z_k' (ldb x n) = X_k' (ldb x m) * A, A = (stored n x m)^T:
+  // NT GEMMs (measured 3x faster here than NN on a column-major copy of A)
+  rocblas_status r = rocblas_status_success;
+  for (int k = 0; k < 2 && r == rocblas_status_success; ++k) {
     const T* Xk = k == 0 ? X1 : X2;
     T* Zk = k == 0 ? Z1 : Z2;
-    rocblas_status r;
     if constexpr (std::is_same<T, double>::value)
       r = rocblas_dgemm(blas, rocblas_operation_none, rocblas_operation_transpose, a.ldb, n, m, &one, Xk, a.ldb,
                         static_cast<const double*>(Ad), n, &zero, Zk, a.ldb);
     else
       r = rocblas_sgemm(blas, rocblas_operation_none, rocblas_operation_transpose, a.ldb, n, m, &one, Xk, a.ldb,
                         static_cast<const float*>(Ad), n, &zero, Zk, a.ldb);
-    if (r != rocblas_status_success) return false;
   }
-  return true;
+  return r == rocblas_status_success;
 }
 
 // A' products and phase B: for a sparse A one fused pass (k_fam_at_b), for a
