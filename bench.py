@@ -47,6 +47,7 @@ METRIC = "SOCPs solved/sec at EPS_ABS=1e-3: plain GPU vs batched+learned warm st
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 
 
 def log(*a):
@@ -193,6 +194,18 @@ def family_roofline(kt, fam_info, nnz, fp64, lin_rows, b_inst, traffic_profile=N
     ms, calls, cols = kt[kind]
     cone_rows = m - lin_rows
     rhs = n * sv if fp64 else n * 2 * nprod
+    if dense and kind in ("At_product", "A_product"):
+        # dense A: GEMMs.  A' products: two fp32 (fp64) GEMMs of 2 m n flops per
+        # column; the A product in mixed mode: one bf16 GEMM over split operands,
+        # three products of 2 m n flops per column
+        if kind == "At_product":
+            flops, peak = 2 * 2.0 * m * n * cols, FP64_PEAK_TFLOPS if fp64 else FP32_PEAK_TFLOPS
+        else:
+            flops, peak = (2.0 * m * n * cols, FP64_PEAK_TFLOPS) if fp64 else (3 * 2.0 * m * n * cols, BF16_PEAK_TFLOPS)
+        ach = flops / (ms * 1e-3) / 1e12
+        return {"bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(ach / peak, 4), "traffic": None, "kernel": kind,
+                "algorithmic_flops_per_launch": flops / max(calls, 1), "avg_launch_us": round(1e3 * ms / max(calls, 1), 2)}
     if kind == "xupdate_gemm":
         flops = (2.0 * n * n if fp64 else 2.0 * npad * nprod * npad) * cols
         ach = flops / (ms * 1e-3) / 1e12

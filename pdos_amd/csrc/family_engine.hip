@@ -693,6 +693,18 @@ __global__ void k_split_g(int n, int np, const double* __restrict__ G, __bf16* _
   }
 }
 
+// dst = [hi(src); lo(src); hi(src)]: three bf16 planes of `total` values
+// (rows x ld, plane stride total), src = hi + lo to about 2^-16 relative.
+__global__ void k_split3(long total, const float* __restrict__ src, __bf16* __restrict__ dst) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    const float v = src[e];
+    const __bf16 hi = (__bf16)v;
+    dst[e] = hi;
+    dst[total + e] = (__bf16)(v - float(hi));
+    dst[2 * total + e] = hi;
+  }
+}
+
 template <typename T>
 __global__ void k_gather_out(int rows, int count, int ldb, const T* __restrict__ Z, double* __restrict__ out) {
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < (long)rows * count; e += (long)gridDim.x * blockDim.x) {
@@ -782,6 +794,14 @@ struct Family {
   void *rp = nullptr, *ri = nullptr, *rx = nullptr, *cp = nullptr, *ci = nullptr, *cx = nullptr;
   void *soc = nullptr, *D = nullptr, *E = nullptr, *bshared = nullptr;
   void* Ad = nullptr;      // dense scaled A (m x n row-major, bulk type) when A is dense
+  // mixed precision, dense A: the A product runs as one bf16 GEMM with fp32
+  // accumulation on split operands (A = A_hi + A_lo, X = X_hi + X_lo, three
+  // products hi.hi + lo.hi + hi.lo; the A' products stay fp32 GEMMs, measured
+  // faster for that shape):
+  //   Ncat  rows [A_hi | A_hi | A_lo] row-major m x 3n
+  //   Ssp   [X_hi; X_lo; X_hi] planes of the multiplied x, 3n x ldb
+  void *Ncat = nullptr, *Ssp = nullptr;
+  bool split_dense = false;
   bool dense = false;
   void* Gd = nullptr;      // fp64 G (n x n row-major, symmetric) for dgemm
   void* Gcat = nullptr;    // mixed: bf16 [G_hi, G_lo, G_hi]
@@ -1056,6 +1076,20 @@ bool Family::create(const Data* d, const Cone* k, const PdosOptions& o, std::str
       for (int p = S.rp[r]; p < S.rp[r + 1]; ++p) Arow[(size_t)r * n + S.ri[p]] += S.rx[p];
     Ad = upload_vals(Arow, err);
     if (!Ad) return false;
+    split_dense = !fp64;
+    if (split_dense) {
+      std::vector<__bf16> nc((size_t)3 * m * n);
+      for (int r = 0; r < m; ++r)
+        for (int c = 0; c < n; ++c) {
+          const float v = float(Arow[(size_t)r * n + c]);
+          const __bf16 hi = (__bf16)v, lo = (__bf16)(v - float(hi));
+          nc[(size_t)r * 3 * n + c] = hi;
+          nc[(size_t)r * 3 * n + n + c] = hi;
+          nc[(size_t)r * 3 * n + 2 * n + c] = lo;
+        }
+      Ncat = upload(nc, err);
+      if (!Ncat) return false;
+    }
   }
   rocblas_int info = 0;
   (void)hipMemcpyAsync(&info, dinfo, sizeof(info), hipMemcpyDeviceToHost, st);
@@ -1151,6 +1185,7 @@ bool Family::load(int cnt, const double* C, const double* B, std::string& err) {
     q.pt0 = dalloc((size_t)std::max(1, S.nsoc()) * ldb * 8, batch_allocs, err);
   }
   Xo = vec(n); Yo = vec(m); So = vec(m);
+  if (split_dense) Ssp = dalloc((size_t)3 * n * ldb * 2, batch_allocs, err);
   Io = dalloc((size_t)3 * ldb * 4, batch_allocs, err);
   Ro = dalloc((size_t)8 * ldb * 8, batch_allocs, err);
   dlist = static_cast<int*>(dalloc((size_t)ldb * 4, batch_allocs, err));
@@ -1252,6 +1287,17 @@ bool Family::apply_A(FamArgs<T>& a, const T* Xin, T* Z) {
     return hipGetLastError() == hipSuccess;
   }
   const T one = T(1), zero = T(0);
+  if (split_dense) {
+    // Z' (ldb x m) = [X_hi; X_lo; X_hi]' (ldb x 3n) * Ncat' (Ncat row-major m x 3n = column-major 3n x m)
+    const long tot = (long)n * a.ldb;
+    hipLaunchKernelGGL(k_split3, dim3(2048), dim3(256), 0, st, tot, reinterpret_cast<const float*>(Xin),
+                       static_cast<__bf16*>(Ssp));
+    const float fone = 1.f, fzero = 0.f;
+    return rocblas_gemm_ex(blas, rocblas_operation_none, rocblas_operation_none, a.ldb, m, 3 * n, &fone, Ssp,
+                           rocblas_datatype_bf16_r, a.ldb, Ncat, rocblas_datatype_bf16_r, 3 * n, &fzero, Z,
+                           rocblas_datatype_f32_r, a.ldb, Z, rocblas_datatype_f32_r, a.ldb, rocblas_datatype_f32_r,
+                           rocblas_gemm_algo_standard, 0, 0) == rocblas_status_success;
+  }
   // column-major Z' (ldb x m) = X' (ldb x n) * A^T, A^T stored column-major n x m (ld n)
   if constexpr (std::is_same<T, double>::value)
     return rocblas_dgemm(blas, rocblas_operation_none, rocblas_operation_none, a.ldb, m, n, &one, Xin, a.ldb,
@@ -1272,9 +1318,9 @@ bool Family::apply_At(FamArgs<T>& a, const T* X1, const T* X2, T* Z1, T* Z2) {
     return hipGetLastError() == hipSuccess;
   }
   const T one = T(1), zero = T(0);
+  rocblas_status r = rocblas_status_success;
   // column-major Z_k' (ldb x n) = X_k' (ldb x m) * A, A = (stored n x m)^T:
   // NT GEMMs (measured 3x faster here than NN on a column-major copy of A)
-  rocblas_status r = rocblas_status_success;
   for (int k = 0; k < 2 && r == rocblas_status_success; ++k) {
     const T* Xk = k == 0 ? X1 : X2;
     T* Zk = k == 0 ? Z1 : Z2;

@@ -148,3 +148,24 @@ def test_max_iters_indeterminate():
         assert out["iters"][t] == 3
         for key in ("x", "s", "y"):
             assert _rel(out[key][t], ref[key]) <= 1e-6
+
+
+def test_dense_family_mixed_verifies_in_fp64():
+    """Mixed precision with a dense A: the A and A' products are split-bf16
+    GEMMs (three products, fp32 accumulation); every Solved instance meets the
+    stop test in binary64 and the objective matches the planted optimum."""
+    dense = problems.DenseFamily(seed=2, m=256, n=64, l=192, nsoc=8, qdim=8, EPS_ABS=1e-3)
+    B, Cm, opt = dense.sample(6, 64)
+    print("dense family mixed", flush=True)
+    F = Family(dense.instance(B[0], Cm[0]), precision="mixed")
+    F.load(Cm, B)
+    F.run()
+    out = F.fetch()
+    solved = out["status"] == "Solved"
+    assert solved.sum() >= 60
+    A = dense.A
+    for t in np.nonzero(solved)[0]:
+        x, s, y, b, c = out["x"][t], out["s"][t], out["y"][t], B[t], Cm[t]
+        assert np.linalg.norm(A @ x + s - b) < 1.01e-3 * (1 + np.linalg.norm(b))
+        assert np.linalg.norm(c + A.T @ y) < 1.01e-3 * (1 + np.linalg.norm(c))
+        assert abs(c @ x - opt[t]) <= 2e-2 * (1 + abs(opt[t]))
