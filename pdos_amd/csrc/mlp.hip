@@ -194,14 +194,28 @@ __global__ void k_loss_grad(int rows, int cols, int ld, const float* __restrict_
   }
 }
 
-// db[c] = sum_r G[r][c] over rows (fp32 or bf16 source).
+// db[c] = sum_r G[r][c] over rows (fp32 or bf16 source).  A block of 512
+// threads takes 64 columns (lane = column, coalesced rows); wave w sums rows
+// w, w + 8, ... and the 8 wave sums are added in wave order (deterministic).
 template <typename S>
-__global__ void k_colsum(int rows, int cols, int ld, const S* __restrict__ G, float* __restrict__ db) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= cols) return;
+__global__ __launch_bounds__(512) void k_colsum(int rows, int cols, int ld, const S* __restrict__ G,
+                                                float* __restrict__ db) {
+  __shared__ float part[8][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int c = blockIdx.x * 64 + lane;
   float s = 0.f;
-  for (int r = 0; r < rows; ++r) s += (float)G[(size_t)r * ld + c];
-  db[c] = s;
+  if (c < cols) {
+#pragma unroll 8
+    for (int r = w; r < rows; r += 8) s += (float)G[(size_t)r * ld + c];
+  }
+  part[w][lane] = s;
+  __syncthreads();
+  if (w == 0 && c < cols) {
+    float t = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) t += part[k][lane];
+    db[c] = t;
+  }
 }
 
 // Adam (binary32 master, bias-corrected), then the bf16 mirror.
@@ -336,13 +350,13 @@ struct Mlp {
     float* gW2 = g + off_W2; float* gb2 = g + off_b2;
     float* gW3 = g + off_W3; float* gb3 = g + off_b3;
     note(gemm<true, false, EPI_F32>(st, ph, pout, rp, H2b, ph, dYb, pout, gW3, nullptr, pout));
-    hipLaunchKernelGGL(k_colsum<float>, dim3((pout + 255) / 256), dim3(256), 0, st, rp, pout, pout, dYf, gb3);
+    hipLaunchKernelGGL(k_colsum<float>, dim3((pout + 63) / 64), dim3(512), 0, st, rp, pout, pout, dYf, gb3);
     note(gemm<false, true, EPI_GELU_GRAD>(st, rp, ph, pout, dYb, pout, W3b, pout, nullptr, dZ2b, ph, nullptr, Z2));
     note(gemm<true, false, EPI_F32>(st, ph, ph, rp, H1b, ph, dZ2b, ph, gW2, nullptr, ph));
-    hipLaunchKernelGGL(k_colsum<__bf16>, dim3((ph + 255) / 256), dim3(256), 0, st, rp, ph, ph, dZ2b, gb2);
+    hipLaunchKernelGGL(k_colsum<__bf16>, dim3((ph + 63) / 64), dim3(512), 0, st, rp, ph, ph, dZ2b, gb2);
     note(gemm<false, true, EPI_GELU_GRAD>(st, rp, ph, ph, dZ2b, ph, W2b, ph, nullptr, dZ1b, ph, nullptr, Z1));
     note(gemm<true, false, EPI_F32>(st, pin, ph, rp, Xb, pin, dZ1b, ph, gW1, nullptr, ph));
-    hipLaunchKernelGGL(k_colsum<__bf16>, dim3((ph + 255) / 256), dim3(256), 0, st, rp, ph, ph, dZ1b, gb1);
+    hipLaunchKernelGGL(k_colsum<__bf16>, dim3((ph + 63) / 64), dim3(512), 0, st, rp, ph, ph, dZ1b, gb1);
     std::vector<double> hp(nb);
     (void)hipMemcpyAsync(hp.data(), part, nb * sizeof(double), hipMemcpyDeviceToHost, st);
     (void)hipStreamSynchronize(st);

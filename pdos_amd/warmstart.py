@@ -24,7 +24,13 @@ class LearnedWarmStart:
         self.net = Mlp(n_in, hidden, n_out, seed=seed)
         self.n_in, self.n_out = n_in, n_out
 
+    # (fraction of the epochs, learning-rate factor): a step decay; Adam's
+    # moments carry over from one stage to the next
+    SCHEDULE = ((0.6, 1.0), (0.25, 1.0 / 3.0), (0.15, 0.1))
+
     def fit(self, F, T, epochs: int = 30, batch: int = 512, lr: float = 1e-3, seed: int = 0) -> float:
+        """Trains on features F and targets T (rows); returns the last epoch's
+        mean loss in the standardized target space."""
         F = np.asarray(F, np.float64)
         T = np.asarray(T, np.float64)
         self.fmu, self.fsd = F.mean(0), F.std(0)
@@ -33,7 +39,14 @@ class LearnedWarmStart:
         self.tsd[self.tsd < 1e-8] = 1.0
         self.net.set_affine(self.fmu, self.fsd, self.tsd, self.tmu)
         Tn = ((T - self.tmu) / self.tsd).astype(np.float32)
-        return self.net.train(F.astype(np.float32), Tn, epochs, batch, lr, seed)
+        Ff = F.astype(np.float32)
+        loss, done = float("nan"), 0
+        for k, (frac, f) in enumerate(self.SCHEDULE):
+            e = epochs - done if k == len(self.SCHEDULE) - 1 else int(round(frac * epochs))
+            if e > 0:
+                loss = self.net.train(Ff, Tn, e, batch, lr * f, seed + k)
+                done += e
+        return loss
 
     def predict(self, F) -> np.ndarray:
         return self.net.predict(np.asarray(F, np.float32))
