@@ -838,7 +838,19 @@ struct Family {
     int4 *A = nullptr, *At = nullptr;
     int nA = 0, nAt = 0;  // blocks (NWV tasks each)
   } tasks[2];
-  const Tasks* cur_tasks = &tasks[0];
+  const Tasks* curA = &tasks[0];   // task table of the A product for the current width
+  const Tasks* curAt = &tasks[0];  // and of the A' product
+  int num_cu = 256;
+  // Task granularity of a product over `cols` column blocks: the coarse table
+  // (64 entries per task: an index load and four gather rounds per wave) or
+  // the fine one (16 entries: one of each), whichever needs fewer sequential
+  // memory round trips, counted as rounds of resident workgroups (4 of 512
+  // threads per CU) times round trips per task.
+  int pick_tasks(int cols, int n_coarse, int n_fine) const {
+    const long slots = 4L * num_cu;
+    const long rc = (cols * (long)n_coarse + slots - 1) / slots, rf = (cols * (long)n_fine + slots - 1) / slots;
+    return rc * 5 <= rf * 2 ? 0 : 1;
+  }
   int *rowA = nullptr, *rowAt = nullptr;  // row of every CSR entry of A, of A'
   int4 *c1 = nullptr, *c2 = nullptr;
   int* c2_cone = nullptr;
@@ -936,7 +948,7 @@ struct Family {
   bool at_phase_b(FamArgs<T>& a);
   // chunks of phase B partials: one per A' task block (sparse, fused into the
   // A' product) or per RPC rows (dense)
-  int b_chunks() const { return dense ? (n + RPC - 1) / RPC : cur_tasks->nAt; }
+  int b_chunks() const { return dense ? (n + RPC - 1) / RPC : curAt->nAt; }
 };
 
 // Rows of a CSR matrix into SpMM wave tasks, NWV per block (k_fam_spmm):
@@ -996,6 +1008,11 @@ bool Family::create(const Data* d, const Cone* k, const PdosOptions& o, std::str
   n = S.n;
   m = S.m;
   np = pad64(n);
+  {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+      num_cu = prop.multiProcessorCount;
+  }
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
     err = "device/stream setup failed";
     return false;
@@ -1241,7 +1258,8 @@ template <typename T>
 bool Family::iterate(FamArgs<T>& a, int steps, std::string& err) {
   const int cols = a.ldb / LANES;
   // SpMM tasks: 64 entries while the batch is large, 16 once it is small
-  cur_tasks = &tasks[(long)cols * tasks[0].nA < 2048 ? 1 : 0];
+  curA = &tasks[pick_tasks(cols, tasks[0].nA, tasks[1].nA)];
+  curAt = &tasks[pick_tasks(cols, tasks[0].nAt, tasks[1].nAt)];
   const int cm = m_items(), cn = b_chunks();
   const dim3 block(FT);
   cur_ld = a.ldb;
@@ -1282,8 +1300,8 @@ template <typename T>
 bool Family::apply_A(FamArgs<T>& a, const T* Xin, T* Z) {
   if (!dense) {
     const int cols = a.ldb / LANES;
-    const SpmmIn<T, 1> in{cur_tasks->A, rowA, a.ri, a.rx, Xin, Xin, Xin, a.ldb, cols, cur_tasks->nA};
-    hipLaunchKernelGGL((k_fam_spmm<T, 1>), dim3(cols * cur_tasks->nA), dim3(FT), 0, st, in, Z, Z, Z);
+    const SpmmIn<T, 1> in{curA->A, rowA, a.ri, a.rx, Xin, Xin, Xin, a.ldb, cols, curA->nA};
+    hipLaunchKernelGGL((k_fam_spmm<T, 1>), dim3(cols * curA->nA), dim3(FT), 0, st, in, Z, Z, Z);
     return hipGetLastError() == hipSuccess;
   }
   const T one = T(1), zero = T(0);
@@ -1313,8 +1331,8 @@ template <typename T>
 bool Family::apply_At(FamArgs<T>& a, const T* X1, const T* X2, T* Z1, T* Z2) {
   if (!dense) {
     const int cols = a.ldb / LANES;
-    const SpmmIn<T, 2> in{cur_tasks->At, rowAt, a.ci, a.cx, X1, X2, X2, a.ldb, cols, cur_tasks->nAt};
-    hipLaunchKernelGGL((k_fam_spmm<T, 2>), dim3(cols * cur_tasks->nAt), dim3(FT), 0, st, in, Z1, Z2, Z2);
+    const SpmmIn<T, 2> in{curAt->At, rowAt, a.ci, a.cx, X1, X2, X2, a.ldb, cols, curAt->nAt};
+    hipLaunchKernelGGL((k_fam_spmm<T, 2>), dim3(cols * curAt->nAt), dim3(FT), 0, st, in, Z1, Z2, Z2);
     return hipGetLastError() == hipSuccess;
   }
   const T one = T(1), zero = T(0);
@@ -1340,8 +1358,8 @@ template <typename T>
 bool Family::at_phase_b(FamArgs<T>& a) {
   if (!dense) {
     const int cols = a.ldb / LANES;
-    const SpmmIn<T, 2> in{cur_tasks->At, rowAt, a.ci, a.cx, a.Y, a.W, a.W, a.ldb, cols, cur_tasks->nAt};
-    hipLaunchKernelGGL(k_fam_at_b<T>, dim3(cols * cur_tasks->nAt), dim3(FT), 0, st, a, in);
+    const SpmmIn<T, 2> in{curAt->At, rowAt, a.ci, a.cx, a.Y, a.W, a.W, a.ldb, cols, curAt->nAt};
+    hipLaunchKernelGGL(k_fam_at_b<T>, dim3(cols * curAt->nAt), dim3(FT), 0, st, a, in);
     return hipGetLastError() == hipSuccess;
   }
   if (!apply_At(a, a.Y, a.W, a.U, a.V)) return false;
@@ -1375,7 +1393,8 @@ bool Family::run_t(const float* warm, long wld, int judge, std::string& err) {
   (void)hipMemcpyAsync(a.dscale, base_ds, (size_t)ld * 8, hipMemcpyDeviceToDevice, st);
   hipLaunchKernelGGL(k_iota, dim3(64), dim3(256), 0, st, ld, a.orig);
   const dim3 block(FT);
-  cur_tasks = &tasks[0];
+  curA = &tasks[pick_tasks(ld / LANES, tasks[0].nA, tasks[1].nA)];
+  curAt = &tasks[pick_tasks(ld / LANES, tasks[0].nAt, tasks[1].nAt)];
   hipLaunchKernelGGL(k_fam_init<T>, dim3(ld / LANES), block, 0, st, a, warm, wld);
   if (!apply_A(a, a.X, a.AX)) { err = "A product failed"; return false; }
   need_prologue = true;
