@@ -37,7 +37,7 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int BM = 64, BN = 64, BK = 64, KP = BK + 8;  // LDS rows padded against bank conflicts
+constexpr int BM = 64, BN = 64, BK = 64;  // K step of the MLP products (the family's x-update uses 128)
 constexpr int GT = 256;
 
 enum Epi { EPI_F32 = 0, EPI_BIAS_GELU = 1, EPI_BIAS_AFFINE = 2, EPI_GELU_GRAD = 3, EPI_ADD = 4 };
@@ -50,16 +50,18 @@ __device__ __forceinline__ float gelu_grad(float z) {
 // C[M x N] = op(A) op(B) with bf16 operands and fp32 accumulation.
 //   TA = false: A is M x K row-major (lda);  TA = true: A is stored K x M (lda)
 //   TB = false: B is K x N row-major (ldb);  TB = true: B is stored N x K (ldb)
-// M % 64 == 0, N % 64 == 0, K % 64 == 0.  Block: 64 x 64 tile, 4 waves in a
-// 2 x 2 layout, each wave 32 x 32 = 2 x 2 MFMA tiles of 16 x 16.  The K loop
-// is software-pipelined: the global loads of tile k + 1 are in flight while
-// tile k is multiplied out of LDS.
-template <bool TA, bool TB, int EPI>
+// M % 64 == 0, N % 64 == 0, K % KB == 0 (KB = 64 or 128).  Block: 64 x 64
+// tile, 4 waves in a 2 x 2 layout, each wave 32 x 32 = 2 x 2 MFMA tiles of
+// 16 x 16.  The K loop is software-pipelined: the global loads of step k + 1
+// are in flight while step k is multiplied out of LDS.
+template <bool TA, bool TB, int EPI, int KB = BK>
 __global__ __launch_bounds__(GT) void k_gemm(int M, int N, int K, const __bf16* __restrict__ A, int lda,
                                              const __bf16* __restrict__ B, int ldb, float* __restrict__ Cf,
                                              __bf16* __restrict__ Cb, int ldc, const float* __restrict__ bias,
                                              const float* __restrict__ aux, const float* __restrict__ omul,
                                              const float* __restrict__ oadd) {
+  constexpr int KP = KB + 8;  // LDS rows padded against bank conflicts
+  constexpr int H = KB / 32;   // 16-byte pieces per thread, tile and K step
   __shared__ __attribute__((aligned(16))) __bf16 As[BM][KP];
   __shared__ __attribute__((aligned(16))) __bf16 Bs[BN][KP];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -71,13 +73,13 @@ __global__ __launch_bounds__(GT) void k_gemm(int M, int N, int K, const __bf16* 
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-  // Each thread moves two 16-byte pieces of each tile per K step.
-  //   row-major source (A: !TA, B: TB):  row t / 4, k (t % 4) * 8 + {0, 32}
-  //   transposed source (A: TA, B: !TB): k t / 8 + {0, 32}, row (t % 8) * 8 .. + 7
-  bf16x8 ra[2], rb[2];
+  // Each thread moves H 16-byte pieces of each tile per K step.
+  //   row-major source (A: !TA, B: TB):  row t / 4, k (t % 4) * 8 + 32 h
+  //   transposed source (A: TA, B: !TB): k t / 8 + 32 h, row (t % 8) * 8 .. + 7
+  bf16x8 ra[H], rb[H];
   auto gload = [&](int k0) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < H; ++h) {
       if (!TA) ra[h] = *reinterpret_cast<const bf16x8*>(&A[(size_t)(m0 + (tid >> 2)) * lda + k0 + (tid & 3) * 8 + h * 32]);
       else ra[h] = *reinterpret_cast<const bf16x8*>(&A[(size_t)(k0 + (tid >> 3) + h * 32) * lda + m0 + (tid & 7) * 8]);
       if (TB) rb[h] = *reinterpret_cast<const bf16x8*>(&B[(size_t)(n0 + (tid >> 2)) * ldb + k0 + (tid & 3) * 8 + h * 32]);
@@ -86,7 +88,7 @@ __global__ __launch_bounds__(GT) void k_gemm(int M, int N, int K, const __bf16* 
   };
   auto sstore = [&]() {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < H; ++h) {
       if (!TA) {
         *reinterpret_cast<bf16x8*>(&As[tid >> 2][(tid & 3) * 8 + h * 32]) = ra[h];
       } else {
@@ -102,14 +104,28 @@ __global__ __launch_bounds__(GT) void k_gemm(int M, int N, int K, const __bf16* 
     }
   };
   gload(0);
-  for (int k0 = 0; k0 < K; k0 += BK) {
+  // EPI_ADD: the addend tile, loaded now so it arrives during the K loop
+  float addv[2][2][4];
+  if (EPI == EPI_ADD) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
+          const int col = n0 + wn + j * 16 + (lane & 15);
+          addv[i][j][r] = aux[(size_t)row * ldc + col];
+        }
+  }
+  for (int k0 = 0; k0 < K; k0 += KB) {
     sstore();
     __syncthreads();
-    if (k0 + BK < K) gload(k0 + BK);
+    if (k0 + KB < K) gload(k0 + KB);
     // lane l: A rows (l % 16), k = (l / 16) * 8 .. +7; B cols (l % 16), same k
     const int fr = lane & 15, fk = (lane >> 4) * 8;
 #pragma unroll
-    for (int ks = 0; ks < BK; ks += 32) {
+    for (int ks = 0; ks < KB; ks += 32) {
       bf16x8 af[2], bfv[2];
 #pragma unroll
       for (int i = 0; i < 2; ++i) af[i] = *reinterpret_cast<const bf16x8*>(&As[wm + i * 16 + fr][ks + fk]);
@@ -137,7 +153,7 @@ __global__ __launch_bounds__(GT) void k_gemm(int M, int N, int K, const __bf16* 
         if (EPI == EPI_F32) {
           Cf[o] = v;
         } else if (EPI == EPI_ADD) {  // aux is an addend with the layout of C
-          Cf[o] = aux[o] + v;
+          Cf[o] = addv[i][j][r] + v;
         } else if (EPI == EPI_BIAS_GELU) {
           v += bias[col];
           Cf[o] = v;                 // pre-activation, kept for the backward pass
@@ -253,12 +269,12 @@ int pad64(int v) { return (v + 63) & ~63; }
 
 // Launches k_gemm; shapes off the 64-multiple grid are refused (the kernel
 // has no bounds checks).
-template <bool TA, bool TB, int EPI>
+template <bool TA, bool TB, int EPI, int KB = BK>
 hipError_t gemm(hipStream_t st, int M, int N, int K, const __bf16* A, int lda, const __bf16* B, int ldb,
                 float* Cf, __bf16* Cb, int ldc, const float* bias = nullptr, const float* aux = nullptr,
                 const float* omul = nullptr, const float* oadd = nullptr) {
-  if (M <= 0 || N <= 0 || K <= 0 || M % BM || N % BN || K % BK) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((k_gemm<TA, TB, EPI>), dim3(N / BN, M / BM), dim3(GT), 0, st, M, N, K, A, lda, B, ldb,
+  if (M <= 0 || N <= 0 || K <= 0 || M % BM || N % BN || K % KB) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_gemm<TA, TB, EPI, KB>), dim3(N / BN, M / BM), dim3(GT), 0, st, M, N, K, A, lda, B, ldb,
                      Cf, Cb, ldc, bias, aux, omul, oadd);
   return hipGetLastError();
 }
@@ -270,6 +286,12 @@ hipError_t gemm(hipStream_t st, int M, int N, int K, const __bf16* A, int lda, c
 // the family engine's x-update.
 hipError_t bf16_gemm_f32(hipStream_t st, int M, int N, int K, const __bf16* A, int lda, const __bf16* B,
                          int ldb, float* C, int ldc, const float* addend) {
+  // 128-wide K steps when K allows: half the dependent load round trips
+  if (K % 128 == 0) {
+    if (addend)
+      return gemm<false, false, EPI_ADD, 128>(st, M, N, K, A, lda, B, ldb, C, nullptr, ldc, nullptr, addend);
+    return gemm<false, false, EPI_F32, 128>(st, M, N, K, A, lda, B, ldb, C, nullptr, ldc);
+  }
   if (addend) return gemm<false, false, EPI_ADD>(st, M, N, K, A, lda, B, ldb, C, nullptr, ldc, nullptr, addend);
   return gemm<false, false, EPI_F32>(st, M, N, K, A, lda, B, ldb, C, nullptr, ldc);
 }
