@@ -161,6 +161,27 @@ __device__ __forceinline__ double lane_of(double v, int l) {
 
 constexpr int kUnroll = 16;  // gathers in flight per wave
 
+// Element of a feature-major row at a lane's byte offset: the row start is
+// wave-uniform (computed on the scalar unit), the 32-bit offset per lane, so
+// the access needs no 64-bit vector address arithmetic.
+template <typename P>
+__device__ __forceinline__ P& at_lane(P* row, unsigned boff) {
+  return *reinterpret_cast<P*>(reinterpret_cast<char*>(row) + boff);
+}
+template <typename P>
+__device__ __forceinline__ const P& at_lane(const P* row, unsigned boff) {
+  return *reinterpret_cast<const P*>(reinterpret_cast<const char*>(row) + boff);
+}
+
+// Wave-uniform read of data no kernel writes while it runs (per-row scales,
+// index tables): through the constant address space, so it is a scalar load
+// waited on by itself rather than behind the vector loads in flight.
+template <typename P>
+__device__ __forceinline__ P ld_uniform(const P* p) {
+  typedef const __attribute__((address_space(4))) P* cptr;
+  return *(cptr)p;
+}
+
 template <typename T, int NR>
 struct SpmmIn {
   const int4* tasks; const int* rowof; const int* idx; const T* val;
@@ -191,11 +212,14 @@ __device__ __forceinline__ void spmm_block(int cols, int nitems, int& bx, int& b
 // the row's products), done(row, acc) when the row is complete; for a split
 // row, wave 0 gets done(row, block total).  Rows without entries inside a
 // task are begun and done with zero sums.
-template <typename T, int NR, typename Begin, typename Done>
+//
+// With FROM_LDS the entries' columns are offsets into `stage` (LDS): the
+// value of right-hand side k for this lane is stage[idx + k * 64 + lane].
+template <typename T, int NR, bool FROM_LDS = false, typename Begin, typename Done>
 __device__ __forceinline__ void spmm_core(const SpmmIn<T, NR>& in, int bx, int by, double* buf, Begin begin,
-                                          Done done) {
+                                          Done done, const T* stage = nullptr) {
   const int lane = threadIdx.x & 63, w = wave_id();
-  const int inst = bx * LANES + lane;
+  const unsigned boff = (unsigned)(bx * LANES + lane) * (unsigned)sizeof(T);
   const int4 tk = in.tasks[by * NWV + w];
   const bool split = in.tasks[by * NWV].y < 0;  // block-uniform
   T acc[NR];
@@ -210,7 +234,7 @@ __device__ __forceinline__ void spmm_core(const SpmmIn<T, NR>& in, int bx, int b
     if (lane < cnt) {
       ci = in.idx[q + lane];
       vi = in.val[q + lane];
-      if (!split) ri = in.rowof[q + lane];
+      ri = split ? tk.x : in.rowof[q + lane];
     }
     for (int e0 = 0; e0 < cnt; e0 += kUnroll) {
       // entries past cnt repeat entry cnt - 1: in bounds, not accumulated
@@ -221,14 +245,21 @@ __device__ __forceinline__ void spmm_core(const SpmmIn<T, NR>& in, int bx, int b
         const int e = min(e0 + u, cnt - 1);
         cc[u] = lane_of(ci, e);
         vv[u] = lane_of(vi, e);
-        rr[u] = split ? cur : lane_of(ri, e);
+        rr[u] = lane_of(ri, e);
       }
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u) {
-        const size_t o = (size_t)cc[u] * in.ldb + inst;
-        xv[u][0] = in.X1[o];
-        if constexpr (NR > 1) xv[u][1] = in.X2[o];
-        if constexpr (NR > 2) xv[u][2] = in.X3[o];
+        if constexpr (FROM_LDS) {
+          const T* sp = stage + cc[u] + lane;
+          xv[u][0] = sp[0];
+          if constexpr (NR > 1) xv[u][1] = sp[LANES];
+          if constexpr (NR > 2) xv[u][2] = sp[2 * LANES];
+        } else {
+          const size_t ro = (size_t)cc[u] * in.ldb;
+          xv[u][0] = at_lane(in.X1 + ro, boff);
+          if constexpr (NR > 1) xv[u][1] = at_lane(in.X2 + ro, boff);
+          if constexpr (NR > 2) xv[u][2] = at_lane(in.X3 + ro, boff);
+        }
       }
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u) {
@@ -279,12 +310,12 @@ __global__ __launch_bounds__(FT) void k_fam_spmm(SpmmIn<T, NR> in, T* __restrict
   __shared__ double buf[NWV * LANES];
   int bx, by;
   spmm_block(in.cols, in.nitems, bx, by);
-  const int inst = bx * LANES + (threadIdx.x & 63);
+  const unsigned boff = (unsigned)(bx * LANES + (threadIdx.x & 63)) * (unsigned)sizeof(T);
   spmm_core<T, NR>(in, bx, by, buf, [](int) {}, [&](int row, const T (&v)[NR]) {
-    const size_t o = (size_t)row * in.ldb + inst;
-    Y1[o] = v[0];
-    if constexpr (NR > 1) Y2[o] = v[1];
-    if constexpr (NR > 2) Y3[o] = v[2];
+    const size_t ro = (size_t)row * in.ldb;
+    at_lane(Y1 + ro, boff) = v[0];
+    if constexpr (NR > 1) at_lane(Y2 + ro, boff) = v[1];
+    if constexpr (NR > 2) at_lane(Y3 + ro, boff) = v[2];
   });
 }
 
@@ -293,46 +324,120 @@ __global__ __launch_bounds__(FT) void k_fam_spmm(SpmmIn<T, NR> in, T* __restrict
 // |(c + U) ./ E|^2 and c'X and the x-update right-hand side (as k_fam_b).
 // c_i and X_i are loaded when the row begins.  Partials per block into
 // pb[task block].
+// Phase B for row i of instance inst, given U_i = u and V_i = v: the
+// partials |(c + U) ./ E|^2 and c'X, and the x-update right-hand side.
+template <typename T>
+__device__ __forceinline__ void at_b_row(const FamArgs<T>& a, int i, int inst, T lam, T c_i, T x_i, double e_i,
+                                         T u, T v, double& d2, double& cx) {
+  const T r = (c_i + u) / T(e_i);
+  d2 += double(r) * double(r);
+  cx += double(c_i) * double(x_i);
+  if (a.judge) return;
+  const T rv = kCorr<T> ? v - lam * c_i : x_i - lam * c_i + v;
+  if (kCorr<T>) {
+    const unsigned o2 = (unsigned)inst * 2u;
+    const __bf16 hi = (__bf16)float(rv);
+    at_lane(a.Rcat + (size_t)i * a.ldb, o2) = hi;
+    if (a.nprod > 1) at_lane(a.Rcat + (size_t)(a.np + i) * a.ldb, o2) = (__bf16)(float(rv) - float(hi));
+    if (a.nprod > 2) at_lane(a.Rcat + (size_t)(2 * a.np + i) * a.ldb, o2) = hi;
+  } else {
+    at_lane(a.R + (size_t)i * a.ldb, (unsigned)inst * (unsigned)sizeof(T)) = rv;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void at_b_partials(const FamArgs<T>& a, int by, int inst, bool act, double d2,
+                                              double cx, double* buf) {
+  d2 = waves_sum(d2, buf);
+  cx = waves_sum(cx, buf);
+  if (act && wave_id() == 0) {
+    double* pb = a.pb + ((size_t)by * a.ldb + inst) * 2;
+    pb[0] = d2;
+    pb[1] = cx;
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(FT) void k_fam_at_b(FamArgs<T> a, SpmmIn<T, 2> in) {
   __shared__ double buf[NWV * LANES];
-  const int lane = threadIdx.x & 63, w = wave_id();
+  const int lane = threadIdx.x & 63;
   int bx, by;
   spmm_block(in.cols, in.nitems, bx, by);
   const int inst = bx * LANES + lane;
   const bool act = inst < a.count && !a.done[inst];
   const T lam = T(a.lam[inst]);
   T c_i = T(0), x_i = T(0);
+  double e_i = 1.0;  // converted where used: the scalar load is waited for there
   double d2 = 0.0, cx = 0.0;
+  const unsigned boff = (unsigned)inst * (unsigned)sizeof(T);
   spmm_core<T, 2>(in, bx, by, buf,
                   [&](int i) {
-                    const size_t o = (size_t)i * a.ldb + inst;
-                    c_i = a.c[o];
-                    x_i = a.X[o];
+                    const size_t ro = (size_t)i * a.ldb;
+                    c_i = at_lane(a.c + ro, boff);
+                    x_i = at_lane(a.X + ro, boff);
+                    e_i = ld_uniform(a.E + i);
                   },
                   [&](int i, const T (&v)[2]) {
-                    if (!act) return;
-                    const T r = (c_i + v[0]) / T(a.E[i]);
-                    d2 += double(r) * double(r);
-                    cx += double(c_i) * double(x_i);
-                    if (a.judge) return;
-                    const T rv = kCorr<T> ? v[1] - lam * c_i : x_i - lam * c_i + v[1];
-                    if (kCorr<T>) {
-                      const __bf16 hi = (__bf16)float(rv);
-                      a.Rcat[(size_t)i * a.ldb + inst] = hi;
-                      if (a.nprod > 1) a.Rcat[(size_t)(a.np + i) * a.ldb + inst] = (__bf16)(float(rv) - float(hi));
-                      if (a.nprod > 2) a.Rcat[(size_t)(2 * a.np + i) * a.ldb + inst] = hi;
-                    } else {
-                      a.R[(size_t)i * a.ldb + inst] = rv;
-                    }
+                    if (act) at_b_row(a, i, inst, lam, c_i, x_i, e_i, v[0], v[1], d2, cx);
                   });
-  d2 = waves_sum(d2, buf);
-  cx = waves_sum(cx, buf);
-  if (act && w == 0) {
-    double* pb = a.pb + ((size_t)by * a.ldb + inst) * 2;
-    pb[0] = d2;
-    pb[1] = cx;
+  at_b_partials(a, by, inst, act, d2, cx, buf);
+}
+
+// Columns of A' staged in LDS (the densest ones: rows of A that many rows of
+// A' share), at most kStageCols, and rows per task of the staged variant.
+constexpr int kStageCols = 32;
+constexpr int kStageRows = 8;
+
+// k_fam_at_b with the staged columns' rows of Y and W loaded once per block
+// into LDS (kStageCols x 2 x 64 values): each row's staged entries are summed
+// from LDS first (`hin`: idx = LDS offset of the column's Y value, W 64
+// values later) into per-wave row partials, then the other entries (`in`)
+// are gathered and summed as in k_fam_at_b and the partials added.  Both
+// task tables cover the same rows, at most kStageRows per task.
+template <typename T>
+__global__ __launch_bounds__(FT) void k_fam_at_bs(FamArgs<T> a, SpmmIn<T, 2> in, SpmmIn<T, 2> hin,
+                                                  const int* __restrict__ hcol, int nh) {
+  __shared__ double buf[NWV * LANES];
+  __shared__ T stage[kStageCols * 2 * LANES];
+  __shared__ T part[NWV][kStageRows][2 * LANES];
+  const int lane = threadIdx.x & 63, w = wave_id();
+  int bx, by;
+  spmm_block(in.cols, in.nitems, bx, by);
+  const int inst = bx * LANES + lane;
+  for (int j = w; j < 2 * nh; j += NWV) {
+    const T* src = (j & 1) ? a.W : a.Y;
+    stage[j * LANES + lane] = src[(size_t)hcol[j >> 1] * a.ldb + inst];
   }
+  const bool act = inst < a.count && !a.done[inst];
+  const T lam = T(a.lam[inst]);
+  const int base = in.tasks[by * NWV + w].x;  // first row of the wave's task
+  __syncthreads();
+  spmm_core<T, 2, true>(
+      hin, bx, by, buf, [](int) {},
+      [&](int i, const T (&v)[2]) {
+        T* p = part[w][i - base];
+        p[lane] = v[0];
+        p[LANES + lane] = v[1];
+      },
+      stage);
+  T c_i = T(0), x_i = T(0), h0 = T(0), h1 = T(0);
+  double e_i = 1.0;
+  double d2 = 0.0, cx = 0.0;
+  const unsigned boff = (unsigned)inst * (unsigned)sizeof(T);
+  spmm_core<T, 2>(in, bx, by, buf,
+                  [&](int i) {
+                    const size_t ro = (size_t)i * a.ldb;
+                    c_i = at_lane(a.c + ro, boff);
+                    x_i = at_lane(a.X + ro, boff);
+                    e_i = ld_uniform(a.E + i);
+                    const T* p = part[w][i - base];
+                    h0 = p[lane];
+                    h1 = p[LANES + lane];
+                  },
+                  [&](int i, const T (&v)[2]) {
+                    if (act) at_b_row(a, i, inst, lam, c_i, x_i, e_i, v[0] + h0, v[1] + h1, d2, cx);
+                  });
+  at_b_partials(a, by, inst, act, d2, cx, buf);
 }
 
 // Phase kernels work on chunks of RPC rows, RW rows per wave: every wave
@@ -445,6 +550,27 @@ __global__ __launch_bounds__(FT) void k_fam_b(FamArgs<T> a) {
 // added in wave order, so the totals do not depend on timing.  Wave 0 then
 // decides.  In judge mode it compares a warm start's scaled residuals with
 // the cold start's and resets rejected instances to zero instead.
+// Wave w's share of a stop-test partial table (double2 per chunk and
+// instance): chunks w, w + NWV, ... < nc summed in that order, with the loads
+// of 8 chunks issued before any is used.
+__device__ __forceinline__ void wave_parts(const double* p, int ldb, int inst, int w, int nc, double& s0,
+                                           double& s1) {
+  for (int c0 = w; c0 < nc; c0 += 8 * NWV) {
+    double2 q[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int c = min(c0 + k * NWV, nc - 1);
+      q[k] = *reinterpret_cast<const double2*>(p + ((size_t)c * ldb + inst) * 2);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (c0 + k * NWV < nc) {
+        s0 += q[k].x;
+        s1 += q[k].y;
+      }
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(FT) void k_fam_decide(FamArgs<T> a, int chunks_m, int chunks_n) {
   __shared__ double4 part[NWV][LANES];
@@ -452,19 +578,16 @@ __global__ __launch_bounds__(FT) void k_fam_decide(FamArgs<T> a, int chunks_m, i
   const int inst = blockIdx.x * LANES + lane;
   const bool act = inst < a.count && !a.done[inst];
   double4 v = {0.0, 0.0, 0.0, 0.0};  // |pres|^2, b'y, |dres|^2, c'x
+  double ps = 1.0, ds = 1.0;
+  int it = 0;
   if (act) {
-#pragma unroll 8
-    for (int c = w; c < chunks_m; c += NWV) {
-      const double2 pa = *reinterpret_cast<const double2*>(a.pa + ((size_t)c * a.ldb + inst) * 2);
-      v.x += pa.x;
-      v.y += pa.y;
+    if (w == 0) {
+      ps = a.pscale[inst];
+      ds = a.dscale[inst];
+      it = a.iters[inst];
     }
-#pragma unroll 8
-    for (int c = w; c < chunks_n; c += NWV) {
-      const double2 pb = *reinterpret_cast<const double2*>(a.pb + ((size_t)c * a.ldb + inst) * 2);
-      v.z += pb.x;
-      v.w += pb.y;
-    }
+    wave_parts(a.pa, a.ldb, inst, w, chunks_m, v.x, v.y);
+    wave_parts(a.pb, a.ldb, inst, w, chunks_n, v.z, v.w);
   }
   part[w][lane] = v;
   __syncthreads();
@@ -480,7 +603,6 @@ __global__ __launch_bounds__(FT) void k_fam_decide(FamArgs<T> a, int chunks_m, i
   }
   const double pres = sqrt(p2), dres = sqrt(d2), pobj = cx, dobj = -by;
   const double eta = fabs(pobj - dobj);
-  const double ps = a.pscale[inst], ds = a.dscale[inst];
   double* rd = a.red + (size_t)inst * 8;
   if (a.judge) {
     const double mw = fmax(fmax(pres / ps, dres / ds), eta / (1.0 + fabs(pobj) + fabs(dobj)));
@@ -495,7 +617,6 @@ __global__ __launch_bounds__(FT) void k_fam_decide(FamArgs<T> a, int chunks_m, i
     }
     return;
   }
-  const int it = a.iters[inst];
   if (it >= a.max_iters) {  // Indeterminate; the residuals of the last check stay
     a.done[inst] = 1;
     return;
@@ -616,6 +737,8 @@ __global__ __launch_bounds__(FT) void k_fam_c2(FamArgs<T> a, const int4* __restr
   const bool from_st = kCorr<T> && a.relax == 1.0;
   double p2 = 0.0, by = 0.0;
   if (act) {
+    const T lam = T(a.lam[inst]);
+    const T t0 = T(a.pt0[(size_t)cn * a.ldb + inst]);
     T zv[RW], wv[RW], yv[RW], av[RW], bv[RW];
 #pragma unroll
     for (int k = 0; k < RW; ++k) {
@@ -627,13 +750,19 @@ __global__ __launch_bounds__(FT) void k_fam_c2(FamArgs<T> a, const int4* __restr
         if (!from_st) av[k] = a.AX[o];
       }
     }
+    // the cone's chunk partials in order, 16 loads issued at a time
     double n2 = 0.0;
-#pragma unroll 8
-    for (int g = it.z; g < it.z + it.w; ++g) n2 += a.pc[(size_t)g * a.ldb + inst];
+    const int gend = it.z + it.w;
+    for (int g0 = it.z; g0 < gend; g0 += 16) {
+      double pv[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) pv[u] = a.pc[(size_t)min(g0 + u, gend - 1) * a.ldb + inst];
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+        if (g0 + u < gend) n2 += pv[u];
+    }
     const int r0 = a.soc_ptr[cn];
-    const T lam = T(a.lam[inst]);
     const T ns = T(sqrt(n2));
-    const T t0 = T(a.pt0[(size_t)cn * a.ldb + inst]);
     const int mode = ns <= t0 ? 0 : (ns <= -t0 ? 1 : 2);
     const T al = (ns + t0) / T(2);
 #pragma unroll
@@ -837,7 +966,18 @@ struct Family {
   struct Tasks {
     int4 *A = nullptr, *At = nullptr;
     int nA = 0, nAt = 0;  // blocks (NWV tasks each)
+    int4 *AtN = nullptr, *AtH = nullptr;  // staged A' product: gathered and staged entries
+    int nAtS = 0;
   } tasks[2];
+  // staged A' product (binary32, sparse A): the nh densest columns of A'
+  // (hcol) are read from LDS; their entries (sH*) and the others (sN*) are
+  // separate CSR lists of the rows of A'
+  int nh = 0;
+  int* hcol = nullptr;
+  int *sNi = nullptr, *sNrow = nullptr, *sHi = nullptr, *sHrow = nullptr;
+  void *sNx = nullptr, *sHx = nullptr;
+  bool staged() const { return nh > 0 && !fp64 && !dense; }
+  int at_items(const Tasks* t) const { return staged() ? t->nAtS : t->nAt; }
   const Tasks* curA = &tasks[0];   // task table of the A product for the current width
   const Tasks* curAt = &tasks[0];  // and of the A' product
   int num_cu = 256;
@@ -948,7 +1088,7 @@ struct Family {
   bool at_phase_b(FamArgs<T>& a);
   // chunks of phase B partials: one per A' task block (sparse, fused into the
   // A' product) or per RPC rows (dense)
-  int b_chunks() const { return dense ? (n + RPC - 1) / RPC : curAt->nAt; }
+  int b_chunks() const { return dense ? (n + RPC - 1) / RPC : at_items(curAt); }
 };
 
 // Rows of a CSR matrix into SpMM wave tasks, NWV per block (k_fam_spmm):
@@ -985,6 +1125,55 @@ static std::vector<int4> spmm_tasks(const std::vector<int>& ptr, int te) {
   flush();
   if (out.empty()) out.assign(NWV, int4{0, 0, 0, 0});
   return out;
+}
+
+// Tasks over two entry lists of the same rows (staged A' product): `pn`
+// (gathered, weight 4) and `ph` (read from LDS, weight 1).  Runs of at most
+// `rt` complete rows weighing at most 4 te, or, for a heavier row, one block
+// whose NWV tasks split both of the row's lists.  tn[t] and th[t] cover the
+// same rows.
+static void staged_tasks(const std::vector<int>& pn, const std::vector<int>& ph, int te, int rt,
+                         std::vector<int4>& tn, std::vector<int4>& th) {
+  std::vector<int4> cn, ch;
+  auto flush = [&]() {
+    if (cn.empty()) return;
+    while (cn.size() < (size_t)NWV) {
+      cn.push_back(int4{0, 0, 0, 0});
+      ch.push_back(int4{0, 0, 0, 0});
+    }
+    tn.insert(tn.end(), cn.begin(), cn.end());
+    th.insert(th.end(), ch.begin(), ch.end());
+    cn.clear();
+    ch.clear();
+  };
+  auto wt = [&](int r0, int r1) { return 4L * (pn[r1] - pn[r0]) + (ph[r1] - ph[r0]); };
+  const int nrows = (int)pn.size() - 1;
+  int r = 0;
+  while (r < nrows) {
+    if (wt(r, r + 1) > 4L * te) {
+      flush();
+      const int partn = (pn[r + 1] - pn[r] + NWV - 1) / NWV, parth = (ph[r + 1] - ph[r] + NWV - 1) / NWV;
+      for (int k = 0; k < NWV; ++k) {
+        const int q0 = std::min(pn[r + 1], pn[r] + k * partn), q1 = std::min(pn[r + 1], q0 + partn);
+        const int h0 = std::min(ph[r + 1], ph[r] + k * parth), h1 = std::min(ph[r + 1], h0 + parth);
+        tn.push_back(int4{r, -1, q0, q1});
+        th.push_back(int4{r, -1, h0, h1});
+      }
+      ++r;
+      continue;
+    }
+    int e = r;
+    while (e < nrows && e - r < rt && wt(r, e + 1) <= 4L * te) ++e;
+    cn.push_back(int4{r, e, pn[r], pn[e]});
+    ch.push_back(int4{r, e, ph[r], ph[e]});
+    if (cn.size() == (size_t)NWV) flush();
+    r = e;
+  }
+  flush();
+  if (tn.empty()) {
+    tn.assign(NWV, int4{0, 0, 0, 0});
+    th.assign(NWV, int4{0, 0, 0, 0});
+  }
 }
 
 static std::vector<int> entry_rows(const std::vector<int>& ptr) {
@@ -1038,6 +1227,56 @@ bool Family::create(const Data* d, const Cone* k, const PdosOptions& o, std::str
   }
   rowA = static_cast<int*>(upload(entry_rows(S.rp), err));
   rowAt = static_cast<int*>(upload(entry_rows(S.cp), err));
+  std::vector<int> hc;  // staged columns of A' (rows of A), ascending
+  const char* stage_env = std::getenv("PDOS_FAMILY_STAGE");  // TEMP: A/B switch
+  if (!fp64 && S.nnz() <= (long)m * n / 4 && stage_env && stage_env[0] == '1') {
+    // staged A' product: columns of A' (rows of A) with at least 64 entries,
+    // the kStageCols longest; each saves its entry count in gathers per
+    // column block and costs one row load per block
+    for (int r = 0; r < m; ++r)
+      if (S.rp[r + 1] - S.rp[r] >= 64) hc.push_back(r);
+    std::stable_sort(hc.begin(), hc.end(),
+                     [&](int x, int y) { return S.rp[x + 1] - S.rp[x] > S.rp[y + 1] - S.rp[y]; });
+    if (hc.size() > (size_t)kStageCols) hc.resize(kStageCols);
+    std::sort(hc.begin(), hc.end());
+    nh = (int)hc.size();
+  }
+  if (nh > 0) {
+    std::vector<int> slot(m, -1);
+    for (int s2 = 0; s2 < nh; ++s2) slot[hc[s2]] = s2;
+    std::vector<int> pn{0}, ph{0}, ni, hi;
+    std::vector<double> nx, hx;
+    for (int i = 0; i < n; ++i) {
+      for (int p = S.cp[i]; p < S.cp[i + 1]; ++p) {
+        const int sl = slot[S.ci[p]];
+        if (sl < 0) {
+          ni.push_back(S.ci[p]);
+          nx.push_back(S.cx[p]);
+        } else {
+          hi.push_back(sl * 2 * LANES);
+          hx.push_back(S.cx[p]);
+        }
+      }
+      pn.push_back((int)ni.size());
+      ph.push_back((int)hi.size());
+    }
+    if (ni.empty()) { ni.push_back(0); nx.push_back(0.0); }
+    if (hi.empty()) { hi.push_back(0); hx.push_back(0.0); }
+    hcol = static_cast<int*>(upload(hc, err));
+    sNi = static_cast<int*>(upload(ni, err));
+    sNx = upload_vals(nx, err);
+    sHi = static_cast<int*>(upload(hi, err));
+    sHx = upload_vals(hx, err);
+    sNrow = static_cast<int*>(upload(entry_rows(pn), err));
+    sHrow = static_cast<int*>(upload(entry_rows(ph), err));
+    for (int gi = 0; gi < 2; ++gi) {
+      std::vector<int4> tn, th;
+      staged_tasks(pn, ph, gi == 0 ? 64 : 16, kStageRows, tn, th);
+      tasks[gi].nAtS = (int)tn.size() / NWV;
+      tasks[gi].AtN = static_cast<int4*>(upload(tn, err));
+      tasks[gi].AtH = static_cast<int4*>(upload(th, err));
+    }
+  }
   {
     std::vector<int4> v1, v2;
     std::vector<int> vc;
@@ -1196,7 +1435,7 @@ bool Family::load(int cnt, const double* C, const double* B, std::string& err) {
     q.red = dalloc((size_t)8 * ldb * 8, batch_allocs, err);
     q.orig = dalloc((size_t)ldb * 4, batch_allocs, err);
     q.pa = dalloc((size_t)std::max(1, m_items()) * ldb * 2 * 8, batch_allocs, err);
-    const int nb = std::max({(n + RPC - 1) / RPC, tasks[0].nAt, tasks[1].nAt});
+    const int nb = std::max({(n + RPC - 1) / RPC, tasks[0].nAt, tasks[1].nAt, tasks[0].nAtS, tasks[1].nAtS});
     q.pb = dalloc((size_t)nb * ldb * 2 * 8, batch_allocs, err);
     q.pc = dalloc((size_t)nc1 * ldb * 8, batch_allocs, err);
     q.pt0 = dalloc((size_t)std::max(1, S.nsoc()) * ldb * 8, batch_allocs, err);
@@ -1259,7 +1498,7 @@ bool Family::iterate(FamArgs<T>& a, int steps, std::string& err) {
   const int cols = a.ldb / LANES;
   // SpMM tasks: 64 entries while the batch is large, 16 once it is small
   curA = &tasks[pick_tasks(cols, tasks[0].nA, tasks[1].nA)];
-  curAt = &tasks[pick_tasks(cols, tasks[0].nAt, tasks[1].nAt)];
+  curAt = &tasks[pick_tasks(cols, at_items(&tasks[0]), at_items(&tasks[1]))];
   const int cm = m_items(), cn = b_chunks();
   const dim3 block(FT);
   cur_ld = a.ldb;
@@ -1358,6 +1597,15 @@ template <typename T>
 bool Family::at_phase_b(FamArgs<T>& a) {
   if (!dense) {
     const int cols = a.ldb / LANES;
+    if constexpr (std::is_same<T, float>::value) {
+      if (staged()) {
+        const int nb = curAt->nAtS;
+        const SpmmIn<T, 2> in{curAt->AtN, sNrow, sNi, static_cast<const T*>(sNx), a.Y, a.W, a.W, a.ldb, cols, nb};
+        const SpmmIn<T, 2> hin{curAt->AtH, sHrow, sHi, static_cast<const T*>(sHx), a.Y, a.W, a.W, a.ldb, cols, nb};
+        hipLaunchKernelGGL(k_fam_at_bs<T>, dim3(cols * nb), dim3(FT), 0, st, a, in, hin, hcol, nh);
+        return hipGetLastError() == hipSuccess;
+      }
+    }
     const SpmmIn<T, 2> in{curAt->At, rowAt, a.ci, a.cx, a.Y, a.W, a.W, a.ldb, cols, curAt->nAt};
     hipLaunchKernelGGL(k_fam_at_b<T>, dim3(cols * curAt->nAt), dim3(FT), 0, st, a, in);
     return hipGetLastError() == hipSuccess;
@@ -1394,7 +1642,7 @@ bool Family::run_t(const float* warm, long wld, int judge, std::string& err) {
   hipLaunchKernelGGL(k_iota, dim3(64), dim3(256), 0, st, ld, a.orig);
   const dim3 block(FT);
   curA = &tasks[pick_tasks(ld / LANES, tasks[0].nA, tasks[1].nA)];
-  curAt = &tasks[pick_tasks(ld / LANES, tasks[0].nAt, tasks[1].nAt)];
+  curAt = &tasks[pick_tasks(ld / LANES, at_items(&tasks[0]), at_items(&tasks[1]))];
   hipLaunchKernelGGL(k_fam_init<T>, dim3(ld / LANES), block, 0, st, a, warm, wld);
   if (!apply_A(a, a.X, a.AX)) { err = "A product failed"; return false; }
   need_prologue = true;
