@@ -212,12 +212,9 @@ __device__ __forceinline__ void spmm_block(int cols, int nitems, int& bx, int& b
 // the row's products), done(row, acc) when the row is complete; for a split
 // row, wave 0 gets done(row, block total).  Rows without entries inside a
 // task are begun and done with zero sums.
-//
-// With FROM_LDS the entries' columns are offsets into `stage` (LDS): the
-// value of right-hand side k for this lane is stage[idx + k * 64 + lane].
-template <typename T, int NR, bool FROM_LDS = false, typename Begin, typename Done>
+template <typename T, int NR, typename Begin, typename Done>
 __device__ __forceinline__ void spmm_core(const SpmmIn<T, NR>& in, int bx, int by, double* buf, Begin begin,
-                                          Done done, const T* stage = nullptr) {
+                                          Done done) {
   const int lane = threadIdx.x & 63, w = wave_id();
   const unsigned boff = (unsigned)(bx * LANES + lane) * (unsigned)sizeof(T);
   const int4 tk = in.tasks[by * NWV + w];
@@ -249,17 +246,10 @@ __device__ __forceinline__ void spmm_core(const SpmmIn<T, NR>& in, int bx, int b
       }
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u) {
-        if constexpr (FROM_LDS) {
-          const T* sp = stage + cc[u] + lane;
-          xv[u][0] = sp[0];
-          if constexpr (NR > 1) xv[u][1] = sp[LANES];
-          if constexpr (NR > 2) xv[u][2] = sp[2 * LANES];
-        } else {
-          const size_t ro = (size_t)cc[u] * in.ldb;
-          xv[u][0] = at_lane(in.X1 + ro, boff);
-          if constexpr (NR > 1) xv[u][1] = at_lane(in.X2 + ro, boff);
-          if constexpr (NR > 2) xv[u][2] = at_lane(in.X3 + ro, boff);
-        }
+        const size_t ro = (size_t)cc[u] * in.ldb;
+        xv[u][0] = at_lane(in.X1 + ro, boff);
+        if constexpr (NR > 1) xv[u][1] = at_lane(in.X2 + ro, boff);
+        if constexpr (NR > 2) xv[u][2] = at_lane(in.X3 + ro, boff);
       }
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u) {
@@ -379,63 +369,6 @@ __global__ __launch_bounds__(FT) void k_fam_at_b(FamArgs<T> a, SpmmIn<T, 2> in) 
                   },
                   [&](int i, const T (&v)[2]) {
                     if (act) at_b_row(a, i, inst, lam, c_i, x_i, e_i, v[0], v[1], d2, cx);
-                  });
-  at_b_partials(a, by, inst, act, d2, cx, buf);
-}
-
-// Columns of A' staged in LDS (the densest ones: rows of A that many rows of
-// A' share), at most kStageCols, and rows per task of the staged variant.
-constexpr int kStageCols = 32;
-constexpr int kStageRows = 8;
-
-// k_fam_at_b with the staged columns' rows of Y and W loaded once per block
-// into LDS (kStageCols x 2 x 64 values): each row's staged entries are summed
-// from LDS first (`hin`: idx = LDS offset of the column's Y value, W 64
-// values later) into per-wave row partials, then the other entries (`in`)
-// are gathered and summed as in k_fam_at_b and the partials added.  Both
-// task tables cover the same rows, at most kStageRows per task.
-template <typename T>
-__global__ __launch_bounds__(FT) void k_fam_at_bs(FamArgs<T> a, SpmmIn<T, 2> in, SpmmIn<T, 2> hin,
-                                                  const int* __restrict__ hcol, int nh) {
-  __shared__ double buf[NWV * LANES];
-  __shared__ T stage[kStageCols * 2 * LANES];
-  __shared__ T part[NWV][kStageRows][2 * LANES];
-  const int lane = threadIdx.x & 63, w = wave_id();
-  int bx, by;
-  spmm_block(in.cols, in.nitems, bx, by);
-  const int inst = bx * LANES + lane;
-  for (int j = w; j < 2 * nh; j += NWV) {
-    const T* src = (j & 1) ? a.W : a.Y;
-    stage[j * LANES + lane] = src[(size_t)hcol[j >> 1] * a.ldb + inst];
-  }
-  const bool act = inst < a.count && !a.done[inst];
-  const T lam = T(a.lam[inst]);
-  const int base = in.tasks[by * NWV + w].x;  // first row of the wave's task
-  __syncthreads();
-  spmm_core<T, 2, true>(
-      hin, bx, by, buf, [](int) {},
-      [&](int i, const T (&v)[2]) {
-        T* p = part[w][i - base];
-        p[lane] = v[0];
-        p[LANES + lane] = v[1];
-      },
-      stage);
-  T c_i = T(0), x_i = T(0), h0 = T(0), h1 = T(0);
-  double e_i = 1.0;
-  double d2 = 0.0, cx = 0.0;
-  const unsigned boff = (unsigned)inst * (unsigned)sizeof(T);
-  spmm_core<T, 2>(in, bx, by, buf,
-                  [&](int i) {
-                    const size_t ro = (size_t)i * a.ldb;
-                    c_i = at_lane(a.c + ro, boff);
-                    x_i = at_lane(a.X + ro, boff);
-                    e_i = ld_uniform(a.E + i);
-                    const T* p = part[w][i - base];
-                    h0 = p[lane];
-                    h1 = p[LANES + lane];
-                  },
-                  [&](int i, const T (&v)[2]) {
-                    if (act) at_b_row(a, i, inst, lam, c_i, x_i, e_i, v[0] + h0, v[1] + h1, d2, cx);
                   });
   at_b_partials(a, by, inst, act, d2, cx, buf);
 }
@@ -966,18 +899,7 @@ struct Family {
   struct Tasks {
     int4 *A = nullptr, *At = nullptr;
     int nA = 0, nAt = 0;  // blocks (NWV tasks each)
-    int4 *AtN = nullptr, *AtH = nullptr;  // staged A' product: gathered and staged entries
-    int nAtS = 0;
   } tasks[2];
-  // staged A' product (binary32, sparse A): the nh densest columns of A'
-  // (hcol) are read from LDS; their entries (sH*) and the others (sN*) are
-  // separate CSR lists of the rows of A'
-  int nh = 0;
-  int* hcol = nullptr;
-  int *sNi = nullptr, *sNrow = nullptr, *sHi = nullptr, *sHrow = nullptr;
-  void *sNx = nullptr, *sHx = nullptr;
-  bool staged() const { return nh > 0 && !fp64 && !dense; }
-  int at_items(const Tasks* t) const { return staged() ? t->nAtS : t->nAt; }
   const Tasks* curA = &tasks[0];   // task table of the A product for the current width
   const Tasks* curAt = &tasks[0];  // and of the A' product
   int num_cu = 256;
@@ -1088,7 +1010,7 @@ struct Family {
   bool at_phase_b(FamArgs<T>& a);
   // chunks of phase B partials: one per A' task block (sparse, fused into the
   // A' product) or per RPC rows (dense)
-  int b_chunks() const { return dense ? (n + RPC - 1) / RPC : at_items(curAt); }
+  int b_chunks() const { return dense ? (n + RPC - 1) / RPC : curAt->nAt; }
 };
 
 // Rows of a CSR matrix into SpMM wave tasks, NWV per block (k_fam_spmm):
@@ -1125,55 +1047,6 @@ static std::vector<int4> spmm_tasks(const std::vector<int>& ptr, int te) {
   flush();
   if (out.empty()) out.assign(NWV, int4{0, 0, 0, 0});
   return out;
-}
-
-// Tasks over two entry lists of the same rows (staged A' product): `pn`
-// (gathered, weight 4) and `ph` (read from LDS, weight 1).  Runs of at most
-// `rt` complete rows weighing at most 4 te, or, for a heavier row, one block
-// whose NWV tasks split both of the row's lists.  tn[t] and th[t] cover the
-// same rows.
-static void staged_tasks(const std::vector<int>& pn, const std::vector<int>& ph, int te, int rt,
-                         std::vector<int4>& tn, std::vector<int4>& th) {
-  std::vector<int4> cn, ch;
-  auto flush = [&]() {
-    if (cn.empty()) return;
-    while (cn.size() < (size_t)NWV) {
-      cn.push_back(int4{0, 0, 0, 0});
-      ch.push_back(int4{0, 0, 0, 0});
-    }
-    tn.insert(tn.end(), cn.begin(), cn.end());
-    th.insert(th.end(), ch.begin(), ch.end());
-    cn.clear();
-    ch.clear();
-  };
-  auto wt = [&](int r0, int r1) { return 4L * (pn[r1] - pn[r0]) + (ph[r1] - ph[r0]); };
-  const int nrows = (int)pn.size() - 1;
-  int r = 0;
-  while (r < nrows) {
-    if (wt(r, r + 1) > 4L * te) {
-      flush();
-      const int partn = (pn[r + 1] - pn[r] + NWV - 1) / NWV, parth = (ph[r + 1] - ph[r] + NWV - 1) / NWV;
-      for (int k = 0; k < NWV; ++k) {
-        const int q0 = std::min(pn[r + 1], pn[r] + k * partn), q1 = std::min(pn[r + 1], q0 + partn);
-        const int h0 = std::min(ph[r + 1], ph[r] + k * parth), h1 = std::min(ph[r + 1], h0 + parth);
-        tn.push_back(int4{r, -1, q0, q1});
-        th.push_back(int4{r, -1, h0, h1});
-      }
-      ++r;
-      continue;
-    }
-    int e = r;
-    while (e < nrows && e - r < rt && wt(r, e + 1) <= 4L * te) ++e;
-    cn.push_back(int4{r, e, pn[r], pn[e]});
-    ch.push_back(int4{r, e, ph[r], ph[e]});
-    if (cn.size() == (size_t)NWV) flush();
-    r = e;
-  }
-  flush();
-  if (tn.empty()) {
-    tn.assign(NWV, int4{0, 0, 0, 0});
-    th.assign(NWV, int4{0, 0, 0, 0});
-  }
 }
 
 static std::vector<int> entry_rows(const std::vector<int>& ptr) {
@@ -1227,56 +1100,6 @@ bool Family::create(const Data* d, const Cone* k, const PdosOptions& o, std::str
   }
   rowA = static_cast<int*>(upload(entry_rows(S.rp), err));
   rowAt = static_cast<int*>(upload(entry_rows(S.cp), err));
-  std::vector<int> hc;  // staged columns of A' (rows of A), ascending
-  const char* stage_env = std::getenv("PDOS_FAMILY_STAGE");  // TEMP: A/B switch
-  if (!fp64 && S.nnz() <= (long)m * n / 4 && stage_env && stage_env[0] == '1') {
-    // staged A' product: columns of A' (rows of A) with at least 64 entries,
-    // the kStageCols longest; each saves its entry count in gathers per
-    // column block and costs one row load per block
-    for (int r = 0; r < m; ++r)
-      if (S.rp[r + 1] - S.rp[r] >= 64) hc.push_back(r);
-    std::stable_sort(hc.begin(), hc.end(),
-                     [&](int x, int y) { return S.rp[x + 1] - S.rp[x] > S.rp[y + 1] - S.rp[y]; });
-    if (hc.size() > (size_t)kStageCols) hc.resize(kStageCols);
-    std::sort(hc.begin(), hc.end());
-    nh = (int)hc.size();
-  }
-  if (nh > 0) {
-    std::vector<int> slot(m, -1);
-    for (int s2 = 0; s2 < nh; ++s2) slot[hc[s2]] = s2;
-    std::vector<int> pn{0}, ph{0}, ni, hi;
-    std::vector<double> nx, hx;
-    for (int i = 0; i < n; ++i) {
-      for (int p = S.cp[i]; p < S.cp[i + 1]; ++p) {
-        const int sl = slot[S.ci[p]];
-        if (sl < 0) {
-          ni.push_back(S.ci[p]);
-          nx.push_back(S.cx[p]);
-        } else {
-          hi.push_back(sl * 2 * LANES);
-          hx.push_back(S.cx[p]);
-        }
-      }
-      pn.push_back((int)ni.size());
-      ph.push_back((int)hi.size());
-    }
-    if (ni.empty()) { ni.push_back(0); nx.push_back(0.0); }
-    if (hi.empty()) { hi.push_back(0); hx.push_back(0.0); }
-    hcol = static_cast<int*>(upload(hc, err));
-    sNi = static_cast<int*>(upload(ni, err));
-    sNx = upload_vals(nx, err);
-    sHi = static_cast<int*>(upload(hi, err));
-    sHx = upload_vals(hx, err);
-    sNrow = static_cast<int*>(upload(entry_rows(pn), err));
-    sHrow = static_cast<int*>(upload(entry_rows(ph), err));
-    for (int gi = 0; gi < 2; ++gi) {
-      std::vector<int4> tn, th;
-      staged_tasks(pn, ph, gi == 0 ? 64 : 16, kStageRows, tn, th);
-      tasks[gi].nAtS = (int)tn.size() / NWV;
-      tasks[gi].AtN = static_cast<int4*>(upload(tn, err));
-      tasks[gi].AtH = static_cast<int4*>(upload(th, err));
-    }
-  }
   {
     std::vector<int4> v1, v2;
     std::vector<int> vc;
@@ -1435,7 +1258,7 @@ bool Family::load(int cnt, const double* C, const double* B, std::string& err) {
     q.red = dalloc((size_t)8 * ldb * 8, batch_allocs, err);
     q.orig = dalloc((size_t)ldb * 4, batch_allocs, err);
     q.pa = dalloc((size_t)std::max(1, m_items()) * ldb * 2 * 8, batch_allocs, err);
-    const int nb = std::max({(n + RPC - 1) / RPC, tasks[0].nAt, tasks[1].nAt, tasks[0].nAtS, tasks[1].nAtS});
+    const int nb = std::max({(n + RPC - 1) / RPC, tasks[0].nAt, tasks[1].nAt});
     q.pb = dalloc((size_t)nb * ldb * 2 * 8, batch_allocs, err);
     q.pc = dalloc((size_t)nc1 * ldb * 8, batch_allocs, err);
     q.pt0 = dalloc((size_t)std::max(1, S.nsoc()) * ldb * 8, batch_allocs, err);
@@ -1498,7 +1321,7 @@ bool Family::iterate(FamArgs<T>& a, int steps, std::string& err) {
   const int cols = a.ldb / LANES;
   // SpMM tasks: 64 entries while the batch is large, 16 once it is small
   curA = &tasks[pick_tasks(cols, tasks[0].nA, tasks[1].nA)];
-  curAt = &tasks[pick_tasks(cols, at_items(&tasks[0]), at_items(&tasks[1]))];
+  curAt = &tasks[pick_tasks(cols, tasks[0].nAt, tasks[1].nAt)];
   const int cm = m_items(), cn = b_chunks();
   const dim3 block(FT);
   cur_ld = a.ldb;
@@ -1597,15 +1420,6 @@ template <typename T>
 bool Family::at_phase_b(FamArgs<T>& a) {
   if (!dense) {
     const int cols = a.ldb / LANES;
-    if constexpr (std::is_same<T, float>::value) {
-      if (staged()) {
-        const int nb = curAt->nAtS;
-        const SpmmIn<T, 2> in{curAt->AtN, sNrow, sNi, static_cast<const T*>(sNx), a.Y, a.W, a.W, a.ldb, cols, nb};
-        const SpmmIn<T, 2> hin{curAt->AtH, sHrow, sHi, static_cast<const T*>(sHx), a.Y, a.W, a.W, a.ldb, cols, nb};
-        hipLaunchKernelGGL(k_fam_at_bs<T>, dim3(cols * nb), dim3(FT), 0, st, a, in, hin, hcol, nh);
-        return hipGetLastError() == hipSuccess;
-      }
-    }
     const SpmmIn<T, 2> in{curAt->At, rowAt, a.ci, a.cx, a.Y, a.W, a.W, a.ldb, cols, curAt->nAt};
     hipLaunchKernelGGL(k_fam_at_b<T>, dim3(cols * curAt->nAt), dim3(FT), 0, st, a, in);
     return hipGetLastError() == hipSuccess;
@@ -1642,7 +1456,7 @@ bool Family::run_t(const float* warm, long wld, int judge, std::string& err) {
   hipLaunchKernelGGL(k_iota, dim3(64), dim3(256), 0, st, ld, a.orig);
   const dim3 block(FT);
   curA = &tasks[pick_tasks(ld / LANES, tasks[0].nA, tasks[1].nA)];
-  curAt = &tasks[pick_tasks(ld / LANES, at_items(&tasks[0]), at_items(&tasks[1]))];
+  curAt = &tasks[pick_tasks(ld / LANES, tasks[0].nAt, tasks[1].nAt)];
   hipLaunchKernelGGL(k_fam_init<T>, dim3(ld / LANES), block, 0, st, a, warm, wld);
   if (!apply_A(a, a.X, a.AX)) { err = "A product failed"; return false; }
   need_prologue = true;
