@@ -195,8 +195,7 @@ struct SpmmIn {
 // with cols % 8 == 0 each XCD's consecutive workgroups take all task blocks
 // of one column block before the next, so the rows they gather for those 64
 // instances stay in that XCD's L2.
-__device__ __forceinline__ void spmm_block(int cols, int nitems, int& bx, int& by) {
-  const int L = blockIdx.x;
+__device__ __forceinline__ void spmm_block(int cols, int nitems, int& bx, int& by, int L = blockIdx.x) {
   if ((cols & 7) == 0) {
     const int seq = L >> 3;
     bx = (seq / nitems) * 8 + (L & 7);
@@ -504,11 +503,13 @@ __device__ __forceinline__ void wave_parts(const double* p, int ldb, int inst, i
   }
 }
 
+// The stop test of column block col (one workgroup of NWV waves; `part` is
+// NWV x LANES double4 of LDS).
 template <typename T>
-__global__ __launch_bounds__(FT) void k_fam_decide(FamArgs<T> a, int chunks_m, int chunks_n) {
-  __shared__ double4 part[NWV][LANES];
+__device__ __forceinline__ void decide_block(const FamArgs<T>& a, int col, int chunks_m, int chunks_n,
+                                             double4 (*part)[LANES]) {
   const int lane = threadIdx.x & 63, w = wave_id();
-  const int inst = blockIdx.x * LANES + lane;
+  const int inst = col * LANES + lane;
   const bool act = inst < a.count && !a.done[inst];
   double4 v = {0.0, 0.0, 0.0, 0.0};  // |pres|^2, b'y, |dres|^2, c'x
   double ps = 1.0, ds = 1.0;
@@ -561,6 +562,35 @@ __global__ __launch_bounds__(FT) void k_fam_decide(FamArgs<T> a, int chunks_m, i
   } else {
     a.iters[inst] = it + 1;
   }
+}
+
+
+template <typename T>
+__global__ __launch_bounds__(FT) void k_fam_decide(FamArgs<T> a, int chunks_m, int chunks_n) {
+  __shared__ double4 part[NWV][LANES];
+  decide_block(a, blockIdx.x, chunks_m, chunks_n, part);
+}
+
+// The A product (as k_fam_spmm<T, 1>: AX = A XN) with the stop test of the
+// iteration in the same launch: workgroups 0 .. cols-1 run decide_block for
+// their column block, the others the product.  Neither reads what the other
+// writes (the product reads XN, the stop test the partials of phase C and the
+// A' product), so the iteration saves a kernel boundary.
+template <typename T>
+__global__ __launch_bounds__(FT) void k_fam_ax_decide(FamArgs<T> a, SpmmIn<T, 1> in, int chunks_m, int chunks_n) {
+  __shared__ double4 part[NWV][LANES];
+  if ((int)blockIdx.x < in.cols) {
+    decide_block(a, blockIdx.x, chunks_m, chunks_n, part);
+    return;
+  }
+  double* buf = reinterpret_cast<double*>(&part[0][0]);
+  int bx, by;
+  spmm_block(in.cols, in.nitems, bx, by, (int)blockIdx.x - in.cols);
+  const unsigned boff = (unsigned)(bx * LANES + (threadIdx.x & 63)) * (unsigned)sizeof(T);
+  T* Z = a.AX;
+  spmm_core<T, 1>(in, bx, by, buf, [](int) {}, [&](int row, const T (&v)[1]) {
+    at_lane(Z + (size_t)row * in.ldb, boff) = v[0];
+  });
 }
 
 // Phase C, first pass.  Work items (one per blockIdx.y, host-built, at most
@@ -1328,8 +1358,10 @@ bool Family::iterate(FamArgs<T>& a, int steps, std::string& err) {
   for (int s = 0; s < steps; ++s) {
     mark(K_AT);
     if (!at_phase_b(a)) { err = "A' product failed"; return false; }
-    mark(K_B);
-    hipLaunchKernelGGL(k_fam_decide<T>, dim3(cols), block, 0, st, a, cm, cn);
+    if (dense) {
+      mark(K_B);
+      hipLaunchKernelGGL(k_fam_decide<T>, dim3(cols), block, 0, st, a, cm, cn);
+    }
     mark(K_GEMM);
     if (fp64) {
       const double one = 1.0, zero = 0.0;
@@ -1347,7 +1379,12 @@ bool Family::iterate(FamArgs<T>& a, int steps, std::string& err) {
       return false;
     }
     mark(K_AX);
-    if (!apply_A(a, a.XN, a.AX)) { err = "A product failed"; return false; }
+    if (dense) {
+      if (!apply_A(a, a.XN, a.AX)) { err = "A product failed"; return false; }
+    } else {
+      const SpmmIn<T, 1> in{curA->A, rowA, a.ri, a.rx, a.XN, a.XN, a.XN, a.ldb, cols, curA->nA};
+      hipLaunchKernelGGL(k_fam_ax_decide<T>, dim3(cols + cols * curA->nA), block, 0, st, a, in, cm, cn);
+    }
     mark(K_C);
     hipLaunchKernelGGL(k_fam_c1<T>, dim3(cols, nc1), block, 0, st, a, c1);
     if (nc2) hipLaunchKernelGGL(k_fam_c2<T>, dim3(cols, nc2), block, 0, st, a, c2, c2_cone);
