@@ -46,9 +46,6 @@
 
 namespace pdos_amd {
 
-hipError_t bf16_gemm_f32(hipStream_t st, int M, int N, int K, const __bf16* A, int lda, const __bf16* B,
-                         int ldb, float* C, int ldc, const float* addend);
-
 namespace {
 
 constexpr int LANES = 64;  // instances per block
@@ -82,6 +79,35 @@ struct FamArgs {
 };
 
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
+// Element of a feature-major row at a lane's byte offset: the row start is
+// wave-uniform (computed on the scalar unit), the 32-bit offset per lane, so
+// the access needs no 64-bit vector address arithmetic.
+template <typename P>
+__device__ __forceinline__ P& at_lane(P* row, unsigned boff) {
+  return *reinterpret_cast<P*>(reinterpret_cast<char*>(row) + boff);
+}
+template <typename P>
+__device__ __forceinline__ const P& at_lane(const P* row, unsigned boff) {
+  return *reinterpret_cast<const P*>(reinterpret_cast<const char*>(row) + boff);
+}
+
+// The x-update right-hand side Rcat (3 np rows x ldb instances, bf16) is
+// stored in blocks of 8 rows: element (k, inst) at (k / 8) * 8 ldb + 8 inst +
+// k % 8, so the 8 values of one block for one instance -- an MFMA operand of
+// 16x16x32 -- are 16 contiguous bytes.
+__device__ __forceinline__ __bf16& rcat_at(__bf16* R, int ldb, int k, int inst) {
+  return at_lane(R + ((size_t)(k >> 3) * ldb) * 8 + (k & 7), (unsigned)inst * 16u);
+}
+
+// Wave-uniform read of data no kernel writes while it runs (per-row scales,
+// index tables): through the constant address space, so it is a scalar load
+// waited on by itself rather than behind the vector loads in flight.
+template <typename P>
+__device__ __forceinline__ P ld_uniform(const P* p) {
+  typedef const __attribute__((address_space(4))) P* cptr;
+  return *(cptr)p;
+}
 
 // Sums one value per (wave, lane) over the waves of the block: every thread
 // gets its lane's total.  Two barriers; `buf` is NWV x LANES doubles.
@@ -127,7 +153,7 @@ __global__ __launch_bounds__(FT) void k_fam_init(FamArgs<T> a, const float* __re
   for (int r = a.n + w; r < a.np; r += NWV) {  // padding rows of the GEMM operands
     a.X[(size_t)r * a.ldb + inst] = T(0);
     if (a.Rcat)
-      for (int k = 0; k < 3; ++k) a.Rcat[(size_t)(k * a.np + r) * a.ldb + inst] = (__bf16)0.f;
+      for (int k = 0; k < 3; ++k) rcat_at(a.Rcat, a.ldb, k * a.np + r, inst) = (__bf16)0.f;
   }
   for (int j = w; j < a.m; j += NWV) {
     T yv = T(0), sv = T(0);
@@ -160,27 +186,6 @@ __device__ __forceinline__ double lane_of(double v, int l) {
 }
 
 constexpr int kUnroll = 16;  // gathers in flight per wave
-
-// Element of a feature-major row at a lane's byte offset: the row start is
-// wave-uniform (computed on the scalar unit), the 32-bit offset per lane, so
-// the access needs no 64-bit vector address arithmetic.
-template <typename P>
-__device__ __forceinline__ P& at_lane(P* row, unsigned boff) {
-  return *reinterpret_cast<P*>(reinterpret_cast<char*>(row) + boff);
-}
-template <typename P>
-__device__ __forceinline__ const P& at_lane(const P* row, unsigned boff) {
-  return *reinterpret_cast<const P*>(reinterpret_cast<const char*>(row) + boff);
-}
-
-// Wave-uniform read of data no kernel writes while it runs (per-row scales,
-// index tables): through the constant address space, so it is a scalar load
-// waited on by itself rather than behind the vector loads in flight.
-template <typename P>
-__device__ __forceinline__ P ld_uniform(const P* p) {
-  typedef const __attribute__((address_space(4))) P* cptr;
-  return *(cptr)p;
-}
 
 template <typename T, int NR>
 struct SpmmIn {
@@ -324,11 +329,10 @@ __device__ __forceinline__ void at_b_row(const FamArgs<T>& a, int i, int inst, T
   if (a.judge) return;
   const T rv = kCorr<T> ? v - lam * c_i : x_i - lam * c_i + v;
   if (kCorr<T>) {
-    const unsigned o2 = (unsigned)inst * 2u;
     const __bf16 hi = (__bf16)float(rv);
-    at_lane(a.Rcat + (size_t)i * a.ldb, o2) = hi;
-    if (a.nprod > 1) at_lane(a.Rcat + (size_t)(a.np + i) * a.ldb, o2) = (__bf16)(float(rv) - float(hi));
-    if (a.nprod > 2) at_lane(a.Rcat + (size_t)(2 * a.np + i) * a.ldb, o2) = hi;
+    rcat_at(a.Rcat, a.ldb, i, inst) = hi;
+    if (a.nprod > 1) rcat_at(a.Rcat, a.ldb, a.np + i, inst) = (__bf16)(float(rv) - float(hi));
+    if (a.nprod > 2) rcat_at(a.Rcat, a.ldb, 2 * a.np + i, inst) = hi;
   } else {
     at_lane(a.R + (size_t)i * a.ldb, (unsigned)inst * (unsigned)sizeof(T)) = rv;
   }
@@ -458,9 +462,9 @@ __global__ __launch_bounds__(FT) void k_fam_b(FamArgs<T> a) {
         const T rv = MIXED ? vv[k] - lam * cv[k] : xv[k] - lam * cv[k] + vv[k];
         if (MIXED) {
           const __bf16 hi = (__bf16)float(rv);
-          a.Rcat[(size_t)i * a.ldb + inst] = hi;
-          if (a.nprod > 1) a.Rcat[(size_t)(a.np + i) * a.ldb + inst] = (__bf16)(float(rv) - float(hi));
-          if (a.nprod > 2) a.Rcat[(size_t)(2 * a.np + i) * a.ldb + inst] = hi;
+          rcat_at(a.Rcat, a.ldb, i, inst) = hi;
+          if (a.nprod > 1) rcat_at(a.Rcat, a.ldb, a.np + i, inst) = (__bf16)(float(rv) - float(hi));
+          if (a.nprod > 2) rcat_at(a.Rcat, a.ldb, 2 * a.np + i, inst) = hi;
         } else {
           a.R[(size_t)i * a.ldb + inst] = rv;
         }
@@ -785,6 +789,90 @@ __global__ void k_split_g(int n, int np, const double* __restrict__ G, __bf16* _
   }
 }
 
+typedef __bf16 xbf16x8 __attribute__((ext_vector_type(8)));
+typedef float xf32x4 __attribute__((ext_vector_type(4)));
+
+// MFMA 16x16x32 K steps of one wave and batch of the x-update: all their
+// operand loads are issued before the first product.
+constexpr int kXuSteps = 4;
+
+// The mixed-precision x-update XN = X + Gcat[:, :K] Rcat (K = nprod np):
+// Gcat row-major np x ldg bf16, Rcat in 8-row blocks (rcat_at), X and XN
+// row-major np x ldb binary32.  A workgroup of 4 waves computes a 64 x 64
+// tile (rows m0.., instances n0..): wave q multiplies the K steps of 32
+// q, q + 4, ... for the whole tile, loading its operands straight from
+// memory (A: 16 contiguous bytes of a row of G; B: one 16-byte Rcat block),
+// so a tile costs one round trip per kXuSteps steps of a wave instead of a
+// pass through LDS per step.  The four partial tiles meet in LDS and are
+// summed in wave order (deterministic) with the addend X.  Grid 1-D over
+// (row tile, column tile); with ntile % 8 == 0 the row tiles of one column
+// tile go to one XCD, consecutively, so their Rcat reads share its L2.
+__global__ __launch_bounds__(256) void k_fam_xupdate(int mt, int nt, int ksteps, const __bf16* __restrict__ G,
+                                                     int ldg, const __bf16* __restrict__ R, int ldb,
+                                                     const float* __restrict__ X, float* __restrict__ XN) {
+  __shared__ float red[4][64][65];
+  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int L = blockIdx.x;
+  int mi, ni;
+  if ((nt & 7) == 0) {
+    const int seq = L >> 3;
+    ni = (seq / mt) * 8 + (L & 7);
+    mi = seq % mt;
+  } else {
+    ni = L / mt;
+    mi = L % mt;
+  }
+  const int m0 = mi * 64, n0 = ni * 64;
+  // addend rows 16 q .. 16 q + 15 of the tile, column `lane`, for the epilogue
+  float xa[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) xa[r] = X[(size_t)(m0 + 16 * q + r) * ldb + n0 + lane];
+  xf32x4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = xf32x4{0.f, 0.f, 0.f, 0.f};
+  const int fr = lane & 15, fk = lane >> 4;  // operand row / column, 8-wide K group
+  for (int s0 = q; s0 < ksteps; s0 += 4 * kXuSteps) {
+    xbf16x8 af[kXuSteps][4], bv[kXuSteps][4];
+#pragma unroll
+    for (int t = 0; t < kXuSteps; ++t) {
+      const int st = min(s0 + 4 * t, ksteps - 1);  // past the end: repeat the last step, not used
+      const int k = st * 32 + fk * 8;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        af[t][i] = *reinterpret_cast<const xbf16x8*>(G + (size_t)(m0 + i * 16 + fr) * ldg + k);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        bv[t][j] = *reinterpret_cast<const xbf16x8*>(R + ((size_t)(k >> 3) * ldb + n0 + j * 16 + fr) * 8);
+    }
+#pragma unroll
+    for (int t = 0; t < kXuSteps; ++t) {
+      if (s0 + 4 * t < ksteps) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[t][i], bv[t][j], acc[i][j], 0, 0, 0);
+      }
+    }
+  }
+  // accumulator lane l: rows (l / 16) * 4 + r, column l % 16 of each 16 x 16 tile
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) red[q][i * 16 + fk * 4 + r][j * 16 + fr] = acc[i][j][r];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int row = 16 * q + r;
+    const float v = ((red[0][row][lane] + red[1][row][lane]) + red[2][row][lane]) + red[3][row][lane];
+    XN[(size_t)(m0 + row) * ldb + n0 + lane] = xa[r] + v;
+  }
+}
+
 // dst = [hi(src); lo(src); hi(src)]: three bf16 planes of `total` values
 // (rows x ld, plane stride total), src = hi + lo to about 2^-16 relative.
 __global__ void k_split3(long total, const float* __restrict__ src, __bf16* __restrict__ dst) {
@@ -849,7 +937,7 @@ __global__ __launch_bounds__(FT) void k_fam_move(FamArgs<T> s, FamArgs<T> d, con
   for (int r = d.n + w; r < d.np; r += NWV) {  // padding rows of the GEMM operands
     d.X[(size_t)r * d.ldb + col] = T(0);
     if (d.Rcat)
-      for (int k = 0; k < 3; ++k) d.Rcat[(size_t)(k * d.np + r) * d.ldb + col] = (__bf16)0.f;
+      for (int k = 0; k < 3; ++k) rcat_at(d.Rcat, d.ldb, k * d.np + r, col) = (__bf16)0.f;
   }
   if (w == 0) {
     d.lam[col] = live ? s.lam[sc] : 1.0;
@@ -1372,11 +1460,11 @@ bool Family::iterate(FamArgs<T>& a, int steps, std::string& err) {
         err = "rocblas_dgemm failed";
         return false;
       }
-    } else if (bf16_gemm_f32(st, np, a.ldb, nprod * np, static_cast<const __bf16*>(Gcat), 3 * np, a.Rcat, a.ldb,
-                             reinterpret_cast<float*>(a.XN), a.ldb, reinterpret_cast<const float*>(a.X)) !=
-               hipSuccess) {
-      err = "bf16 gemm launch failed";
-      return false;
+    } else {
+      const int mt = np / 64, nt = a.ldb / 64;
+      hipLaunchKernelGGL(k_fam_xupdate, dim3(mt * nt), dim3(256), 0, st, mt, nt, nprod * np / 32,
+                         static_cast<const __bf16*>(Gcat), 3 * np, a.Rcat, a.ldb,
+                         reinterpret_cast<const float*>(a.X), reinterpret_cast<float*>(a.XN));
     }
     mark(K_AX);
     if (dense) {
