@@ -995,6 +995,8 @@ struct Family {
   void *base_c = nullptr, *base_b = nullptr, *base_lam = nullptr, *base_ps = nullptr, *base_ds = nullptr;
   void *Xo = nullptr, *Yo = nullptr, *So = nullptr, *Io = nullptr, *Ro = nullptr;
   int* dlist = nullptr;
+  int* hflags = nullptr;    // pinned: done flags of the last two chunks (2 x ldb)
+  hipEvent_t fev[2] = {nullptr, nullptr};
   int compactions = 0;
   int nprod = 1;          // mixed: bf16 products of the split GEMM (1, 2 or 3)
   // Graph replay of the loop (mixed mode) is opt-in (PDOS_FAMILY_GRAPH=1): it
@@ -1077,6 +1079,8 @@ struct Family {
   ~Family() {
     drop_graph();
     free_batch();
+    for (hipEvent_t& e : fev)
+      if (e) (void)hipEventDestroy(e);
     for (void* p : allocs) (void)hipFree(p);
     if (blas) rocblas_destroy_handle(blas);
     if (st) (void)hipStreamDestroy(st);
@@ -1102,6 +1106,8 @@ struct Family {
   void free_batch() {
     for (void* p : batch_allocs) (void)hipFree(p);
     batch_allocs.clear();
+    if (hflags) (void)hipHostFree(hflags);
+    hflags = nullptr;
     count = 0;
   }
 
@@ -1193,7 +1199,9 @@ bool Family::create(const Data* d, const Cone* k, const PdosOptions& o, std::str
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
       num_cu = prop.multiProcessorCount;
   }
-  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&fev[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&fev[1], hipEventDisableTiming) != hipSuccess) {
     err = "device/stream setup failed";
     return false;
   }
@@ -1386,6 +1394,10 @@ bool Family::load(int cnt, const double* C, const double* B, std::string& err) {
   Io = dalloc((size_t)3 * ldb * 4, batch_allocs, err);
   Ro = dalloc((size_t)8 * ldb * 8, batch_allocs, err);
   dlist = static_cast<int*>(dalloc((size_t)ldb * 4, batch_allocs, err));
+  if (hipHostMalloc(reinterpret_cast<void**>(&hflags), (size_t)2 * ldb * 4, hipHostMallocDefault) != hipSuccess) {
+    hflags = nullptr;
+    err = "hipHostMalloc failed";
+  }
   if (!err.empty()) return false;
   return hipStreamSynchronize(st) == hipSuccess;
 }
@@ -1594,11 +1606,19 @@ bool Family::run_t(const float* warm, long wld, int judge, std::string& err) {
     a.judge = 0;
     need_prologue = true;  // rejected instances were reset
   }
-  std::vector<int> hdone(ldb), list;
-  int chunk = 16;
+  // Chunks of 16 iterations; after each, its done flags are copied to pinned
+  // host memory.  Outside timing runs the next chunk is launched before the
+  // host reads the flags (one chunk ahead: the GPU does not wait for the
+  // check); the run-ahead chunk only repeats finished columns' frozen state.
+  std::vector<int> list;
   compactions = 0;
-  for (long total = 0; total <= (long)a.max_iters + 1;) {
-    const int steps = (int)std::min<long>(chunk, a.max_iters + 2 - total);
+  long total = 0;
+  const bool ahead = !timing;
+  // launches one chunk into flag slot `slot`: iterations launched, 0 when the
+  // iteration budget is spent, -1 on error
+  auto launch = [&](int slot) -> int {
+    const int steps = (int)std::min<long>(16, a.max_iters + 2 - total);
+    if (steps <= 0) return 0;
     if (need_prologue) prologue(a);
     if (use_graph && !fp64 && !timing) {
       // replay a captured graph of `steps` iterations for this layout
@@ -1608,7 +1628,7 @@ bool Family::run_t(const float* warm, long wld, int judge, std::string& err) {
         hipGraph_t g = nullptr;
         if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
           err = "stream capture failed";
-          return false;
+          return -1;
         }
         const bool ok = iterate(a, steps, err);
         if (hipStreamEndCapture(st, &g) != hipSuccess || !ok ||
@@ -1616,7 +1636,7 @@ bool Family::run_t(const float* warm, long wld, int judge, std::string& err) {
           if (g) (void)hipGraphDestroy(g);
           gexec = nullptr;
           if (err.empty()) err = "graph instantiation failed";
-          return false;
+          return -1;
         }
         (void)hipGraphDestroy(g);
         g_ld = a.ldb;
@@ -1625,24 +1645,42 @@ bool Family::run_t(const float* warm, long wld, int judge, std::string& err) {
         // capture left a.X / a.XN as after `steps` iterations; undo, the launch redoes it
         if (steps % 2) std::swap(a.X, a.XN);
       }
-      if (hipGraphLaunch(gexec, st) != hipSuccess) { err = "graph launch failed"; return false; }
+      if (hipGraphLaunch(gexec, st) != hipSuccess) {
+        err = "graph launch failed";
+        return -1;
+      }
       if (steps % 2) std::swap(a.X, a.XN);
     } else if (!iterate(a, steps, err)) {
-      return false;
+      return -1;
     }
     total += steps;
-    (void)hipMemcpyAsync(hdone.data(), a.done, (size_t)cnt * 4, hipMemcpyDeviceToHost, st);
-    if (hipStreamSynchronize(st) != hipSuccess) { err = "family engine failed"; return false; }
-    collect();
+    (void)hipMemcpyAsync(hflags + (size_t)slot * ldb, a.done, (size_t)cnt * 4, hipMemcpyDeviceToHost, st);
+    (void)hipEventRecord(fev[slot], st);
+    return steps;
+  };
+  auto live = [&](const int* hd) {
     list.clear();
     for (int t = 0; t < cnt; ++t)
-      if (!hdone[t]) list.push_back(t);
-    const int active = (int)list.size();
+      if (!hd[t]) list.push_back(t);
+    return (int)list.size();
+  };
+  int slot = 0;
+  int r = launch(slot);
+  while (r > 0) {
+    const int nxt = ahead ? launch(slot ^ 1) : 0;
+    if (nxt < 0) return false;
+    if (hipEventSynchronize(fev[slot]) != hipSuccess) { err = "family engine failed"; return false; }
+    collect();
+    int active = live(hflags + (size_t)slot * ldb);
     if (active == 0) break;
-    chunk = 16;
     // compact once it drops at least 1/16 of the width: a move costs a few
     // microseconds, a dead column costs its share of every later iteration
     if (pad64(active) <= ld - std::max(LANES, ld / 16)) {
+      if (nxt > 0) {  // the run-ahead chunk has moved on: compact its state
+        if (hipStreamSynchronize(st) != hipSuccess) { err = "family engine failed"; return false; }
+        active = live(hflags + (size_t)(slot ^ 1) * ldb);
+        if (active == 0) break;
+      }
       // compaction: finished columns to the output store, live ones to the other set
       hipLaunchKernelGGL(k_fam_scatter<T>, dim3(ld / LANES), block, 0, st, a, ldb, static_cast<T*>(Xo),
                          static_cast<T*>(Yo), static_cast<T*>(So), static_cast<int*>(Io), static_cast<double*>(Ro));
@@ -1650,15 +1688,27 @@ bool Family::run_t(const float* warm, long wld, int judge, std::string& err) {
       const int nld = pad64(active);
       FamArgs<T> b = args<T>(cur ^ 1, nld, active);
       hipLaunchKernelGGL(k_fam_move<T>, dim3(nld / LANES), block, 0, st, a, b, dlist);
-      (void)hipStreamSynchronize(st);  // dlist / hdone are reused
+      (void)hipStreamSynchronize(st);  // dlist and the flag slots are reused
       cur ^= 1;
       ld = nld;
       cnt = active;
       a = b;
       ++compactions;
       need_prologue = true;
+      slot = 0;
+      r = launch(slot);
+      if (r < 0) return false;
+      continue;
+    }
+    if (nxt > 0) {
+      slot ^= 1;
+      r = nxt;
+    } else {
+      r = launch(slot);
+      if (r < 0) return false;
     }
   }
+  if (r < 0) return false;
   hipLaunchKernelGGL(k_fam_scatter<T>, dim3(ld / LANES), block, 0, st, a, ldb, static_cast<T*>(Xo),
                      static_cast<T*>(Yo), static_cast<T*>(So), static_cast<int*>(Io), static_cast<double*>(Ro));
   return hipGetLastError() == hipSuccess;
