@@ -186,6 +186,12 @@ __device__ __forceinline__ double lane_of(double v, int l) {
 }
 
 constexpr int kUnroll = 16;  // gathers in flight per wave
+// Mixed precision, sparse A: rows of A with at least kDenseRow entries (the
+// kDenseMax longest) are the "dense rows" whose A' part is an fp32 MFMA tile
+constexpr int kDenseRow = 64, kDenseMax = 32;
+struct DenseRows {
+  int row[kDenseMax];  // rows of A (unused slots: row 0, with zero coefficients)
+};
 
 template <typename T, int NR>
 struct SpmmIn {
@@ -374,6 +380,86 @@ __global__ __launch_bounds__(FT) void k_fam_at_b(FamArgs<T> a, SpmmIn<T, 2> in) 
                     if (act) at_b_row(a, i, inst, lam, c_i, x_i, e_i, v[0], v[1], d2, cx);
                   });
   at_b_partials(a, by, inst, act, d2, cx, buf);
+}
+
+typedef float hf32x4 __attribute__((ext_vector_type(4)));
+
+// The A' product with phase B (mixed precision) for an A with dense rows.
+// Workgroup: a 64 x 64 tile (rows m0.. of A', instances n0..).  The dense
+// rows' part U_h = A'_h Y_h, V_h = A'_h W_h (K = kDenseMax rows of Y and W)
+// runs on the fp32 MFMA 16x16x4, each wave two 16 x 16 sub-tiles of each,
+// and the tiles meet in LDS; then wave w takes rows m0 + 8 w .. + 7: their
+// other entries (the light CSR `in`, one task per wave and tile) are gathered
+// and added, and the row epilogue is k_fam_at_b's.  Both parts are summed in
+// binary32 before the right-hand side is rounded to bf16.  Ah holds A'_h in
+// MFMA operand order: 64 floats per (16-row group g, K quad s) at
+// (g * 8 + s) * 64 + lane, lane holding A'_h[16 g + lane % 16][4 s + lane / 16].
+template <typename T>
+__global__ __launch_bounds__(FT) void k_fam_at_hyb(FamArgs<T> a, SpmmIn<T, 2> in, const float* __restrict__ Ah,
+                                                   DenseRows dr) {
+  __shared__ double buf[NWV * LANES];
+  __shared__ float Ut[64][65], Vt[64][65];
+  const int lane = threadIdx.x & 63, w = wave_id();
+  int bx, ti;
+  spmm_block(in.cols, in.nitems, bx, ti);
+  const int m0 = ti * 64, n0 = bx * LANES;
+  {
+    const int rg = w >> 1, cg0 = (w & 1) * 2, kq = lane >> 4, col = lane & 15;
+    float av[8], yb[2][8], wb[2][8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      av[s] = Ah[((size_t)(ti * 4 + rg) * 8 + s) * 64 + lane];
+      const int r0 = dr.row[4 * s], r1 = dr.row[4 * s + 1], r2 = dr.row[4 * s + 2], r3 = dr.row[4 * s + 3];
+      const int r = kq == 0 ? r0 : (kq == 1 ? r1 : (kq == 2 ? r2 : r3));
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const size_t o = (size_t)r * a.ldb + n0 + (cg0 + c) * 16 + col;
+        yb[c][s] = float(a.Y[o]);
+        wb[c][s] = float(a.W[o]);
+      }
+    }
+    hf32x4 au[2], aw[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      au[c] = hf32x4{0.f, 0.f, 0.f, 0.f};
+      aw[c] = hf32x4{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int s = 0; s < 8; ++s)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        au[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s], yb[c][s], au[c], 0, 0, 0);
+        aw[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s], wb[c][s], aw[c], 0, 0, 0);
+      }
+    // accumulator lane l: rows (l / 16) * 4 + r, column l % 16
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        Ut[rg * 16 + kq * 4 + r][(cg0 + c) * 16 + col] = au[c][r];
+        Vt[rg * 16 + kq * 4 + r][(cg0 + c) * 16 + col] = aw[c][r];
+      }
+  }
+  __syncthreads();
+  const int inst = n0 + lane;
+  const bool act = inst < a.count && !a.done[inst];
+  const T lam = T(a.lam[inst]);
+  T c_i = T(0), x_i = T(0), uh = T(0), vh = T(0);
+  double e_i = 1.0, d2 = 0.0, cx = 0.0;
+  const unsigned boff = (unsigned)inst * (unsigned)sizeof(T);
+  spmm_core<T, 2>(in, bx, ti, buf,
+                  [&](int i) {
+                    const size_t ro = (size_t)i * a.ldb;
+                    c_i = at_lane(a.c + ro, boff);
+                    x_i = at_lane(a.X + ro, boff);
+                    e_i = ld_uniform(a.E + i);
+                    uh = T(Ut[i - m0][lane]);
+                    vh = T(Vt[i - m0][lane]);
+                  },
+                  [&](int i, const T (&v)[2]) {
+                    if (act) at_b_row(a, i, inst, lam, c_i, x_i, e_i, v[0] + uh, v[1] + vh, d2, cx);
+                  });
+  at_b_partials(a, ti, inst, act, d2, cx, buf);
 }
 
 // Phase kernels work on chunks of RPC rows, RW rows per wave: every wave
@@ -1020,6 +1106,15 @@ struct Family {
     int4 *A = nullptr, *At = nullptr;
     int nA = 0, nAt = 0;  // blocks (NWV tasks each)
   } tasks[2];
+  // mixed precision, A with dense rows: k_fam_at_hyb's operands (nh dense
+  // rows; the light CSR of A' and one task per wave and 64-row tile)
+  int nh = 0;
+  DenseRows drows{};
+  float* Ah = nullptr;
+  int *li = nullptr, *lrow = nullptr;
+  void* lx = nullptr;
+  int4* ltasks = nullptr;
+  bool hybrid() const { return nh > 0 && !fp64 && !dense; }
   const Tasks* curA = &tasks[0];   // task table of the A product for the current width
   const Tasks* curAt = &tasks[0];  // and of the A' product
   int num_cu = 256;
@@ -1134,7 +1229,7 @@ struct Family {
   bool at_phase_b(FamArgs<T>& a);
   // chunks of phase B partials: one per A' task block (sparse, fused into the
   // A' product) or per RPC rows (dense)
-  int b_chunks() const { return dense ? (n + RPC - 1) / RPC : curAt->nAt; }
+  int b_chunks() const { return dense ? (n + RPC - 1) / RPC : hybrid() ? np / 64 : curAt->nAt; }
 };
 
 // Rows of a CSR matrix into SpMM wave tasks, NWV per block (k_fam_spmm):
@@ -1226,6 +1321,57 @@ bool Family::create(const Data* d, const Cone* k, const PdosOptions& o, std::str
   }
   rowA = static_cast<int*>(upload(entry_rows(S.rp), err));
   rowAt = static_cast<int*>(upload(entry_rows(S.cp), err));
+  std::vector<int> hc;  // dense rows of A (mixed precision, sparse A), ascending
+  if (!fp64 && S.nnz() <= (long)m * n / 4) {
+    for (int r = 0; r < m; ++r)
+      if (S.rp[r + 1] - S.rp[r] >= kDenseRow) hc.push_back(r);
+    std::stable_sort(hc.begin(), hc.end(),
+                     [&](int x, int y) { return S.rp[x + 1] - S.rp[x] > S.rp[y + 1] - S.rp[y]; });
+    if (hc.size() > (size_t)kDenseMax) hc.resize(kDenseMax);
+    std::sort(hc.begin(), hc.end());
+  }
+  nh = (int)hc.size();
+  if (nh > 0) {
+    std::vector<int> sl(m, -1);
+    for (int k = 0; k < nh; ++k) {
+      sl[hc[k]] = k;
+      drows.row[k] = hc[k];
+    }
+    // A'_h (np x kDenseMax, A'_h[i][k] = A[hc[k]][i]) in MFMA operand order
+    std::vector<float> ah((size_t)np * kDenseMax, 0.f);
+    for (int k = 0; k < nh; ++k)
+      for (int q = S.rp[hc[k]]; q < S.rp[hc[k] + 1]; ++q) {
+        const int i = S.ri[q];
+        const int g = i / 16, il = i % 16, sq = k / 4, kq = k % 4;
+        ah[((size_t)g * 8 + sq) * 64 + kq * 16 + il] += float(S.rx[q]);
+      }
+    std::vector<int> lp{0}, lidx;
+    std::vector<double> lval;
+    for (int i = 0; i < n; ++i) {
+      for (int q = S.cp[i]; q < S.cp[i + 1]; ++q)
+        if (sl[S.ci[q]] < 0) {
+          lidx.push_back(S.ci[q]);
+          lval.push_back(S.cx[q]);
+        }
+      lp.push_back((int)lidx.size());
+    }
+    const std::vector<int> lr = entry_rows(lp);
+    std::vector<int4> lt;
+    for (int t = 0; t < np / 64; ++t)
+      for (int w = 0; w < NWV; ++w) {
+        const int r0 = std::min(n, 64 * t + 8 * w), r1 = std::min(n, r0 + 8);
+        lt.push_back(int4{r0, r1, lp[r0], lp[r1]});
+      }
+    if (lidx.empty()) {
+      lidx.push_back(0);
+      lval.push_back(0.0);
+    }
+    Ah = static_cast<float*>(upload(ah, err));
+    li = static_cast<int*>(upload(lidx, err));
+    lx = upload_vals(lval, err);
+    lrow = static_cast<int*>(upload(lr, err));
+    ltasks = static_cast<int4*>(upload(lt, err));
+  }
   {
     std::vector<int4> v1, v2;
     std::vector<int> vc;
@@ -1557,6 +1703,13 @@ template <typename T>
 bool Family::at_phase_b(FamArgs<T>& a) {
   if (!dense) {
     const int cols = a.ldb / LANES;
+    if constexpr (std::is_same<T, float>::value) {
+      if (hybrid()) {
+        const SpmmIn<T, 2> in{ltasks, lrow, li, static_cast<const T*>(lx), a.Y, a.W, a.W, a.ldb, cols, np / 64};
+        hipLaunchKernelGGL(k_fam_at_hyb<T>, dim3(cols * (np / 64)), dim3(FT), 0, st, a, in, Ah, drows);
+        return hipGetLastError() == hipSuccess;
+      }
+    }
     const SpmmIn<T, 2> in{curAt->At, rowAt, a.ci, a.cx, a.Y, a.W, a.W, a.ldb, cols, curAt->nAt};
     hipLaunchKernelGGL(k_fam_at_b<T>, dim3(cols * curAt->nAt), dim3(FT), 0, st, a, in);
     return hipGetLastError() == hipSuccess;
