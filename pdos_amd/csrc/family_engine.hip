@@ -74,6 +74,7 @@ struct FamArgs {
   int nlin_items;      // phase C: items of zero / nonneg rows (then the cone chunks)
   T* X; T* XN; T* S; T* Y; T* AX; T* W; T* U; T* V; T* R;
   __bf16* Rcat;        // mixed: [D_hi; D_lo; D_hi] of the correction D, 3 np x ldb
+  const int* hslot;    // mixed, A with dense rows: slot of row j of A among them, or -1
   int* done; int* status; int* iters; int* fallback;
   double* red;         // per instance: pres2, by, pres, dres, pobj, dobj, eta, spare
 };
@@ -683,6 +684,111 @@ __global__ __launch_bounds__(FT) void k_fam_ax_decide(FamArgs<T> a, SpmmIn<T, 1>
   });
 }
 
+// The A product with the stop test (as k_fam_ax_decide) for an A with dense
+// rows (mixed precision).  Workgroups 0 .. cols-1: the stop test; cols ..
+// 2 cols-1: the dense rows of column block bx, AX_h = A_h XN (kDenseMax
+// rows, K = np) on the fp32 MFMA 16x16x4: wave w multiplies the K quads
+// w * np / 32 .. for the whole 32 x 64 tile, in batches of 8 quads whose
+// loads are issued together, and the eight partial tiles are added in LDS in a
+// fixed tree order; the rest: the other rows (`in`: the light CSR, the dense
+// rows left empty and skipped) as k_fam_ax_decide.  Ahr: A_h in MFMA operand
+// order, 64 floats per (K quad kk, row group rg) at (kk * 2 + rg) * 64 + lane,
+// lane holding A[dense row rg * 16 + lane % 16][4 kk + lane / 16].
+template <typename T>
+__global__ __launch_bounds__(FT) void k_fam_ax_hyb(FamArgs<T> a, SpmmIn<T, 1> in, const float* __restrict__ Ahr,
+                                                   const int* __restrict__ hrows, int chunks_m, int chunks_n) {
+  __shared__ float red[4][32][65];  // also the stop test's double4 [NWV][LANES] and the core's buffer
+  const int cols = in.cols;
+  int L = blockIdx.x;
+  if (L < cols) {
+    decide_block(a, L, chunks_m, chunks_n, reinterpret_cast<double4(*)[LANES]>(&red[0][0][0]));
+    return;
+  }
+  L -= cols;
+  const int lane = threadIdx.x & 63, w = wave_id();
+  if (L < cols) {
+    const int n0 = L * LANES, kq = lane >> 4, col = lane & 15;
+    const int qpw = a.np / 32;  // K quads per wave
+    hf32x4 acc[2][4];
+#pragma unroll
+    for (int rg = 0; rg < 2; ++rg)
+#pragma unroll
+      for (int cg = 0; cg < 4; ++cg) acc[rg][cg] = hf32x4{0.f, 0.f, 0.f, 0.f};
+    for (int b0 = 0; b0 < qpw; b0 += 8) {
+      float av[8][2], xv[8][4];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const int kk = w * qpw + min(b0 + t, qpw - 1);  // past the slice: repeat its last quad, not used
+#pragma unroll
+        for (int rg = 0; rg < 2; ++rg) av[t][rg] = Ahr[((size_t)kk * 2 + rg) * 64 + lane];
+        const T* xr = a.XN + (size_t)(4 * kk + kq) * a.ldb + n0 + col;
+#pragma unroll
+        for (int cg = 0; cg < 4; ++cg) xv[t][cg] = float(xr[cg * 16]);
+      }
+#pragma unroll
+      for (int t = 0; t < 8; ++t)
+        if (b0 + t < qpw) {
+#pragma unroll
+          for (int rg = 0; rg < 2; ++rg)
+#pragma unroll
+            for (int cg = 0; cg < 4; ++cg)
+              acc[rg][cg] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[t][rg], xv[t][cg], acc[rg][cg], 0, 0, 0);
+        }
+    }
+    // accumulator lane l: rows (l / 16) * 4 + r, column l % 16 of each 16 x 16 tile
+    auto put = [&](int slot) {
+#pragma unroll
+      for (int rg = 0; rg < 2; ++rg)
+#pragma unroll
+        for (int cg = 0; cg < 4; ++cg)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) red[slot][rg * 16 + kq * 4 + r][cg * 16 + col] = acc[rg][cg][r];
+    };
+    auto add = [&](int slot) {
+#pragma unroll
+      for (int rg = 0; rg < 2; ++rg)
+#pragma unroll
+        for (int cg = 0; cg < 4; ++cg)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc[rg][cg][r] += red[slot][rg * 16 + kq * 4 + r][cg * 16 + col];
+    };
+    if (w >= 4) put(w - 4);
+    __syncthreads();
+    if (w < 4) add(w);
+    __syncthreads();
+    if (w == 2 || w == 3) put(w - 2);
+    __syncthreads();
+    if (w < 2) add(w);
+    __syncthreads();
+    if (w == 1) put(0);
+    __syncthreads();
+    if (w == 0) {
+      add(0);
+#pragma unroll
+      for (int rg = 0; rg < 2; ++rg)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int j = hrows[rg * 16 + kq * 4 + r];
+          if (j >= 0) {
+            T* zr = a.AX + (size_t)j * a.ldb + n0 + col;
+#pragma unroll
+            for (int cg = 0; cg < 4; ++cg) zr[cg * 16] = T(acc[rg][cg][r]);
+          }
+        }
+    }
+    return;
+  }
+  L -= cols;
+  double* buf = reinterpret_cast<double*>(&red[0][0][0]);
+  int bx, by;
+  spmm_block(cols, in.nitems, bx, by, L);
+  const unsigned boff = (unsigned)(bx * LANES + lane) * (unsigned)sizeof(T);
+  T* Z = a.AX;
+  spmm_core<T, 1>(in, bx, by, buf, [](int) {}, [&](int row, const T (&v)[1]) {
+    if (ld_uniform(a.hslot + row) < 0) at_lane(Z + (size_t)row * in.ldb, boff) = v[0];
+  });
+}
+
 // Phase C, first pass.  Work items (one per blockIdx.y, host-built, at most
 // RPC rows each):
 //   (i0, i1, 0, -)  rows of x of finished instances: XN = X (the host swaps
@@ -1114,6 +1220,15 @@ struct Family {
   int *li = nullptr, *lrow = nullptr;
   void* lx = nullptr;
   int4* ltasks = nullptr;
+  // ... and for the A product: A_h (MFMA order), the dense rows (-1 padded),
+  // the slot of every row of A, the light CSR of A (dense rows empty) and its
+  // task tables (coarse, fine)
+  float* Ahr = nullptr;
+  int *hrows = nullptr, *hslot = nullptr;
+  int *lri = nullptr, *lrrow = nullptr;
+  void* lrx = nullptr;
+  int4* lAt[2] = {nullptr, nullptr};
+  int nLA[2] = {0, 0};
   bool hybrid() const { return nh > 0 && !fp64 && !dense; }
   const Tasks* curA = &tasks[0];   // task table of the A product for the current width
   const Tasks* curAt = &tasks[0];  // and of the A' product
@@ -1366,6 +1481,41 @@ bool Family::create(const Data* d, const Cone* k, const PdosOptions& o, std::str
       lidx.push_back(0);
       lval.push_back(0.0);
     }
+    // A_h for the A product (32 x np, MFMA order) and the light CSR of A
+    std::vector<float> ahr((size_t)np * kDenseMax, 0.f);
+    for (int k = 0; k < nh; ++k)
+      for (int q = S.rp[hc[k]]; q < S.rp[hc[k] + 1]; ++q) {
+        const int j = S.ri[q];
+        ahr[((size_t)(j / 4) * 2 + k / 16) * 64 + (j % 4) * 16 + k % 16] += float(S.rx[q]);
+      }
+    std::vector<int> hr(kDenseMax, -1);
+    for (int k = 0; k < nh; ++k) hr[k] = hc[k];
+    std::vector<int> lrp{0}, lridx;
+    std::vector<double> lrval;
+    for (int r = 0; r < m; ++r) {
+      if (sl[r] < 0)
+        for (int q = S.rp[r]; q < S.rp[r + 1]; ++q) {
+          lridx.push_back(S.ri[q]);
+          lrval.push_back(S.rx[q]);
+        }
+      lrp.push_back((int)lridx.size());
+    }
+    const std::vector<int> lrr = entry_rows(lrp);
+    for (int gi = 0; gi < 2; ++gi) {
+      const std::vector<int4> t = spmm_tasks(lrp, gi == 0 ? 64 : 16);
+      nLA[gi] = (int)t.size() / NWV;
+      lAt[gi] = static_cast<int4*>(upload(t, err));
+    }
+    if (lridx.empty()) {
+      lridx.push_back(0);
+      lrval.push_back(0.0);
+    }
+    Ahr = static_cast<float*>(upload(ahr, err));
+    hrows = static_cast<int*>(upload(hr, err));
+    hslot = static_cast<int*>(upload(sl, err));
+    lri = static_cast<int*>(upload(lridx, err));
+    lrx = upload_vals(lrval, err);
+    lrrow = static_cast<int*>(upload(lrr, err));
     Ah = static_cast<float*>(upload(ah, err));
     li = static_cast<int*>(upload(lidx, err));
     lx = upload_vals(lval, err);
@@ -1590,6 +1740,7 @@ void Family::fill_args(FamArgs<T>& a) {
   a.nx_items = nx_items;
   a.nlin_items = nlin_items;
   a.nprod = nprod;
+  a.hslot = hslot;
 }
 
 template <typename T>
@@ -1627,6 +1778,10 @@ bool Family::iterate(FamArgs<T>& a, int steps, std::string& err) {
     mark(K_AX);
     if (dense) {
       if (!apply_A(a, a.XN, a.AX)) { err = "A product failed"; return false; }
+    } else if (hybrid()) {
+      const int gi = pick_tasks(cols, nLA[0], nLA[1]);
+      const SpmmIn<T, 1> in{lAt[gi], lrrow, lri, static_cast<const T*>(lrx), a.XN, a.XN, a.XN, a.ldb, cols, nLA[gi]};
+      hipLaunchKernelGGL(k_fam_ax_hyb<T>, dim3(2 * cols + cols * nLA[gi]), block, 0, st, a, in, Ahr, hrows, cm, cn);
     } else {
       const SpmmIn<T, 1> in{curA->A, rowA, a.ri, a.rx, a.XN, a.XN, a.XN, a.ldb, cols, curA->nA};
       hipLaunchKernelGGL(k_fam_ax_decide<T>, dim3(cols + cols * curA->nA), block, 0, st, a, in, cm, cn);
