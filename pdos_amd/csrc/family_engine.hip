@@ -686,16 +686,12 @@ __global__ __launch_bounds__(FT) void k_fam_ax_decide(FamArgs<T> a, SpmmIn<T, 1>
 
 // The A product with the stop test (as k_fam_ax_decide) for an A with dense
 // rows (mixed precision).  Workgroups 0 .. cols-1: the stop test; cols ..
-// 2 cols-1: the dense rows of column block bx, AX_h = A_h XN (kDenseMax
-// rows, K = np) on the fp32 MFMA 16x16x4: wave w multiplies the K quads
-// w * np / 32 .. for the whole 32 x 64 tile, in batches of 8 quads whose
-// loads are issued together, and the eight partial tiles are added in LDS in a
-// fixed tree order; the rest: the other rows (`in`: the light CSR, the dense
-// rows left empty and skipped) as k_fam_ax_decide.  Ahr: A_h in MFMA operand
-// order, 64 floats per (K quad kk, row group rg) at (kk * 2 + rg) * 64 + lane,
-// lane holding A[dense row rg * 16 + lane % 16][4 kk + lane / 16].
+// 2 cols-1: the dense rows of column block bx, AX_j = the sum, in row-tile
+// order, of the planes AXp the x-update wrote (each tile's part of A_h XN);
+// the rest: the other rows (`in`: the light CSR, the dense rows left empty and
+// skipped) as k_fam_ax_decide.
 template <typename T>
-__global__ __launch_bounds__(FT) void k_fam_ax_hyb(FamArgs<T> a, SpmmIn<T, 1> in, const float* __restrict__ Ahr,
+__global__ __launch_bounds__(FT) void k_fam_ax_hyb(FamArgs<T> a, SpmmIn<T, 1> in, const float* __restrict__ AXp,
                                                    const int* __restrict__ hrows, int chunks_m, int chunks_n) {
   __shared__ float red[4][32][65];  // also the stop test's double4 [NWV][LANES] and the core's buffer
   const int cols = in.cols;
@@ -707,74 +703,18 @@ __global__ __launch_bounds__(FT) void k_fam_ax_hyb(FamArgs<T> a, SpmmIn<T, 1> in
   L -= cols;
   const int lane = threadIdx.x & 63, w = wave_id();
   if (L < cols) {
-    const int n0 = L * LANES, kq = lane >> 4, col = lane & 15;
-    const int qpw = a.np / 32;  // K quads per wave
-    hf32x4 acc[2][4];
+    // dense rows: the sum of the x-update's row-tile planes, in order
+    const int n0 = L * LANES, mt = a.np / 64;
+    for (int sl = w; sl < kDenseMax; sl += NWV) {
+      const int j = hrows[sl];
+      if (j < 0) continue;
+      float v[16];
 #pragma unroll
-    for (int rg = 0; rg < 2; ++rg)
-#pragma unroll
-      for (int cg = 0; cg < 4; ++cg) acc[rg][cg] = hf32x4{0.f, 0.f, 0.f, 0.f};
-    for (int b0 = 0; b0 < qpw; b0 += 8) {
-      float av[8][2], xv[8][4];
-#pragma unroll
-      for (int t = 0; t < 8; ++t) {
-        const int kk = w * qpw + min(b0 + t, qpw - 1);  // past the slice: repeat its last quad, not used
-#pragma unroll
-        for (int rg = 0; rg < 2; ++rg) av[t][rg] = Ahr[((size_t)kk * 2 + rg) * 64 + lane];
-        const T* xr = a.XN + (size_t)(4 * kk + kq) * a.ldb + n0 + col;
-#pragma unroll
-        for (int cg = 0; cg < 4; ++cg) xv[t][cg] = float(xr[cg * 16]);
-      }
-#pragma unroll
-      for (int t = 0; t < 8; ++t)
-        if (b0 + t < qpw) {
-#pragma unroll
-          for (int rg = 0; rg < 2; ++rg)
-#pragma unroll
-            for (int cg = 0; cg < 4; ++cg)
-              acc[rg][cg] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[t][rg], xv[t][cg], acc[rg][cg], 0, 0, 0);
-        }
-    }
-    // accumulator lane l: rows (l / 16) * 4 + r, column l % 16 of each 16 x 16 tile
-    auto put = [&](int slot) {
-#pragma unroll
-      for (int rg = 0; rg < 2; ++rg)
-#pragma unroll
-        for (int cg = 0; cg < 4; ++cg)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) red[slot][rg * 16 + kq * 4 + r][cg * 16 + col] = acc[rg][cg][r];
-    };
-    auto add = [&](int slot) {
-#pragma unroll
-      for (int rg = 0; rg < 2; ++rg)
-#pragma unroll
-        for (int cg = 0; cg < 4; ++cg)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) acc[rg][cg][r] += red[slot][rg * 16 + kq * 4 + r][cg * 16 + col];
-    };
-    if (w >= 4) put(w - 4);
-    __syncthreads();
-    if (w < 4) add(w);
-    __syncthreads();
-    if (w == 2 || w == 3) put(w - 2);
-    __syncthreads();
-    if (w < 2) add(w);
-    __syncthreads();
-    if (w == 1) put(0);
-    __syncthreads();
-    if (w == 0) {
-      add(0);
-#pragma unroll
-      for (int rg = 0; rg < 2; ++rg)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int j = hrows[rg * 16 + kq * 4 + r];
-          if (j >= 0) {
-            T* zr = a.AX + (size_t)j * a.ldb + n0 + col;
-#pragma unroll
-            for (int cg = 0; cg < 4; ++cg) zr[cg * 16] = T(acc[rg][cg][r]);
-          }
-        }
+      for (int mi = 0; mi < 16; ++mi) v[mi] = mi < mt ? AXp[((size_t)mi * 32 + sl) * a.ldb + n0 + lane] : 0.f;
+      float t = 0.f;
+      for (int mi = 0; mi < mt; ++mi) t += v[mi];
+      for (int mi = 16; mi < mt; ++mi) t += AXp[((size_t)mi * 32 + sl) * a.ldb + n0 + lane];
+      a.AX[(size_t)j * a.ldb + n0 + lane] = T(t);
     }
     return;
   }
@@ -1001,7 +941,8 @@ constexpr int kXuSteps = 4;
 // tile go to one XCD, consecutively, so their Rcat reads share its L2.
 __global__ __launch_bounds__(256) void k_fam_xupdate(int mt, int nt, int ksteps, const __bf16* __restrict__ G,
                                                      int ldg, const __bf16* __restrict__ R, int ldb,
-                                                     const float* __restrict__ X, float* __restrict__ XN) {
+                                                     const float* __restrict__ X, float* __restrict__ XN,
+                                                     const float* __restrict__ Ahr, float* __restrict__ AXp) {
   __shared__ float red[4][64][65];
   const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
   const int L = blockIdx.x;
@@ -1057,11 +998,57 @@ __global__ __launch_bounds__(256) void k_fam_xupdate(int mt, int nt, int ksteps,
 #pragma unroll
       for (int r = 0; r < 4; ++r) red[q][i * 16 + fk * 4 + r][j * 16 + fr] = acc[i][j][r];
   __syncthreads();
+  float xn[16];
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     const int row = 16 * q + r;
     const float v = ((red[0][row][lane] + red[1][row][lane]) + red[2][row][lane]) + red[3][row][lane];
-    XN[(size_t)(m0 + row) * ldb + n0 + lane] = xa[r] + v;
+    xn[r] = xa[r] + v;
+    XN[(size_t)(m0 + row) * ldb + n0 + lane] = xn[r];
+  }
+  if (!AXp) return;
+  // A with dense rows: this tile's part of AX_h = A_h XN, the 32 dense rows
+  // over K rows m0 .. m0 + 63, on the fp32 MFMA 16x16x4 (wave q: K rows
+  // m0 + 16 q ..); the four wave partials meet in LDS and each wave adds 8
+  // rows of them in wave order into plane mi of AXp (32 x ldb per row tile;
+  // the A product sums the planes).  Each wave rewrites only the rows of
+  // red[0..3] it read above, so the XN tile can go to red[0] at once.
+#pragma unroll
+  for (int r = 0; r < 16; ++r) red[0][16 * q + r][lane] = xn[r];
+  __syncthreads();
+  hf32x4 ah[2][4];
+#pragma unroll
+  for (int rg = 0; rg < 2; ++rg)
+#pragma unroll
+    for (int cg = 0; cg < 4; ++cg) ah[rg][cg] = hf32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int kk = (m0 + 16 * q) / 4 + s;  // K quad
+    float av[2];
+#pragma unroll
+    for (int rg = 0; rg < 2; ++rg) av[rg] = Ahr[((size_t)kk * 2 + rg) * 64 + lane];
+#pragma unroll
+    for (int cg = 0; cg < 4; ++cg) {
+      const float xb = red[0][16 * q + 4 * s + fk][cg * 16 + fr];
+#pragma unroll
+      for (int rg = 0; rg < 2; ++rg)
+        ah[rg][cg] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rg], xb, ah[rg][cg], 0, 0, 0);
+    }
+  }
+  // partial of wave q: rows (q & 1) * 32 .. of red[1 + q / 2]
+  float(*pq)[65] = &red[1 + (q >> 1)][(q & 1) * 32];
+#pragma unroll
+  for (int rg = 0; rg < 2; ++rg)
+#pragma unroll
+    for (int cg = 0; cg < 4; ++cg)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) pq[rg * 16 + fk * 4 + r][cg * 16 + fr] = ah[rg][cg][r];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int row = 8 * q + r;
+    const float v = ((red[1][row][lane] + red[1][32 + row][lane]) + red[2][row][lane]) + red[2][32 + row][lane];
+    AXp[((size_t)mi * 32 + row) * ldb + n0 + lane] = v;
   }
 }
 
@@ -1228,6 +1215,7 @@ struct Family {
   int *lri = nullptr, *lrrow = nullptr;
   void* lrx = nullptr;
   int4* lAt[2] = {nullptr, nullptr};
+  float* AXp = nullptr;  // per batch: the x-update's row-tile planes of A_h XN (np / 64 x 32 x ldb)
   int nLA[2] = {0, 0};
   bool hybrid() const { return nh > 0 && !fp64 && !dense; }
   const Tasks* curA = &tasks[0];   // task table of the A product for the current width
@@ -1690,6 +1678,7 @@ bool Family::load(int cnt, const double* C, const double* B, std::string& err) {
   Io = dalloc((size_t)3 * ldb * 4, batch_allocs, err);
   Ro = dalloc((size_t)8 * ldb * 8, batch_allocs, err);
   dlist = static_cast<int*>(dalloc((size_t)ldb * 4, batch_allocs, err));
+  AXp = hybrid() ? static_cast<float*>(dalloc((size_t)(np / 64) * 32 * ldb * 4, batch_allocs, err)) : nullptr;
   if (hipHostMalloc(reinterpret_cast<void**>(&hflags), (size_t)2 * ldb * 4, hipHostMallocDefault) != hipSuccess) {
     hflags = nullptr;
     err = "hipHostMalloc failed";
@@ -1773,7 +1762,8 @@ bool Family::iterate(FamArgs<T>& a, int steps, std::string& err) {
       const int mt = np / 64, nt = a.ldb / 64;
       hipLaunchKernelGGL(k_fam_xupdate, dim3(mt * nt), dim3(256), 0, st, mt, nt, nprod * np / 32,
                          static_cast<const __bf16*>(Gcat), 3 * np, a.Rcat, a.ldb,
-                         reinterpret_cast<const float*>(a.X), reinterpret_cast<float*>(a.XN));
+                         reinterpret_cast<const float*>(a.X), reinterpret_cast<float*>(a.XN),
+                         hybrid() ? Ahr : nullptr, hybrid() ? AXp : nullptr);
     }
     mark(K_AX);
     if (dense) {
@@ -1781,7 +1771,7 @@ bool Family::iterate(FamArgs<T>& a, int steps, std::string& err) {
     } else if (hybrid()) {
       const int gi = pick_tasks(cols, nLA[0], nLA[1]);
       const SpmmIn<T, 1> in{lAt[gi], lrrow, lri, static_cast<const T*>(lrx), a.XN, a.XN, a.XN, a.ldb, cols, nLA[gi]};
-      hipLaunchKernelGGL(k_fam_ax_hyb<T>, dim3(2 * cols + cols * nLA[gi]), block, 0, st, a, in, Ahr, hrows, cm, cn);
+      hipLaunchKernelGGL(k_fam_ax_hyb<T>, dim3(2 * cols + cols * nLA[gi]), block, 0, st, a, in, AXp, hrows, cm, cn);
     } else {
       const SpmmIn<T, 1> in{curA->A, rowA, a.ri, a.rx, a.XN, a.XN, a.XN, a.ldb, cols, curA->nA};
       hipLaunchKernelGGL(k_fam_ax_decide<T>, dim3(cols + cols * curA->nA), block, 0, st, a, in, cm, cn);
