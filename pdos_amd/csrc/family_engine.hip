@@ -37,6 +37,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -75,6 +76,12 @@ struct FamArgs {
   T* X; T* XN; T* S; T* Y; T* AX; T* W; T* U; T* V; T* R;
   __bf16* Rcat;        // mixed: [D_hi; D_lo; D_hi] of the correction D, 3 np x ldb
   const int* hslot;    // mixed, A with dense rows: slot of row j of A among them, or -1
+  // ... and phase C forms AX itself: per (item, wave, k) the row's record
+  // {column of its only entry, bits of its value} or {-1 dense row (the sum
+  // of the x-update's planes AXp, mt of them), 0} / {-2 no entries} / {-3:
+  // several entries, the light CSR lrp, lri, lrx}
+  const int2* rec8; const float* AXp; int mt;
+  const int* lrp; const int* lri; const float* lrx;
   int* done; int* status; int* iters; int* fallback;
   double* red;         // per instance: pres2, by, pres, dres, pobj, dobj, eta, spare
 };
@@ -108,6 +115,10 @@ template <typename P>
 __device__ __forceinline__ P ld_uniform(const P* p) {
   typedef const __attribute__((address_space(4))) P* cptr;
   return *(cptr)p;
+}
+__device__ __forceinline__ int2 ld_uniform(const int2* p) {
+  const int* q = reinterpret_cast<const int*>(p);
+  return int2{ld_uniform(q), ld_uniform(q + 1)};
 }
 
 // Sums one value per (wave, lane) over the waves of the block: every thread
@@ -574,29 +585,30 @@ __global__ __launch_bounds__(FT) void k_fam_b(FamArgs<T> a) {
 // decides.  In judge mode it compares a warm start's scaled residuals with
 // the cold start's and resets rejected instances to zero instead.
 // Wave w's share of a stop-test partial table (double2 per chunk and
-// instance): chunks w, w + NWV, ... < nc summed in that order, with the loads
+// instance): chunks w, w + NW, ... < nc summed in that order, with the loads
 // of 8 chunks issued before any is used.
+template <int NW>
 __device__ __forceinline__ void wave_parts(const double* p, int ldb, int inst, int w, int nc, double& s0,
                                            double& s1) {
-  for (int c0 = w; c0 < nc; c0 += 8 * NWV) {
+  for (int c0 = w; c0 < nc; c0 += 8 * NW) {
     double2 q[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      const int c = min(c0 + k * NWV, nc - 1);
+      const int c = min(c0 + k * NW, nc - 1);
       q[k] = *reinterpret_cast<const double2*>(p + ((size_t)c * ldb + inst) * 2);
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k)
-      if (c0 + k * NWV < nc) {
+      if (c0 + k * NW < nc) {
         s0 += q[k].x;
         s1 += q[k].y;
       }
   }
 }
 
-// The stop test of column block col (one workgroup of NWV waves; `part` is
-// NWV x LANES double4 of LDS).
-template <typename T>
+// The stop test of column block col (one workgroup of NW waves; `part` is
+// NW x LANES double4 of LDS).
+template <typename T, int NW = NWV>
 __device__ __forceinline__ void decide_block(const FamArgs<T>& a, int col, int chunks_m, int chunks_n,
                                              double4 (*part)[LANES]) {
   const int lane = threadIdx.x & 63, w = wave_id();
@@ -611,15 +623,15 @@ __device__ __forceinline__ void decide_block(const FamArgs<T>& a, int col, int c
       ds = a.dscale[inst];
       it = a.iters[inst];
     }
-    wave_parts(a.pa, a.ldb, inst, w, chunks_m, v.x, v.y);
-    wave_parts(a.pb, a.ldb, inst, w, chunks_n, v.z, v.w);
+    wave_parts<NW>(a.pa, a.ldb, inst, w, chunks_m, v.x, v.y);
+    wave_parts<NW>(a.pb, a.ldb, inst, w, chunks_n, v.z, v.w);
   }
   part[w][lane] = v;
   __syncthreads();
   if (w != 0 || !act) return;
   double p2 = 0.0, by = 0.0, d2 = 0.0, cx = 0.0;
 #pragma unroll
-  for (int k = 0; k < NWV; ++k) {
+  for (int k = 0; k < NW; ++k) {
     const double4 q = part[k][lane];
     p2 += q.x;
     by += q.y;
@@ -684,51 +696,6 @@ __global__ __launch_bounds__(FT) void k_fam_ax_decide(FamArgs<T> a, SpmmIn<T, 1>
   });
 }
 
-// The A product with the stop test (as k_fam_ax_decide) for an A with dense
-// rows (mixed precision).  Workgroups 0 .. cols-1: the stop test; cols ..
-// 2 cols-1: the dense rows of column block bx, AX_j = the sum, in row-tile
-// order, of the planes AXp the x-update wrote (each tile's part of A_h XN);
-// the rest: the other rows (`in`: the light CSR, the dense rows left empty and
-// skipped) as k_fam_ax_decide.
-template <typename T>
-__global__ __launch_bounds__(FT) void k_fam_ax_hyb(FamArgs<T> a, SpmmIn<T, 1> in, const float* __restrict__ AXp,
-                                                   const int* __restrict__ hrows, int chunks_m, int chunks_n) {
-  __shared__ float red[4][32][65];  // also the stop test's double4 [NWV][LANES] and the core's buffer
-  const int cols = in.cols;
-  int L = blockIdx.x;
-  if (L < cols) {
-    decide_block(a, L, chunks_m, chunks_n, reinterpret_cast<double4(*)[LANES]>(&red[0][0][0]));
-    return;
-  }
-  L -= cols;
-  const int lane = threadIdx.x & 63, w = wave_id();
-  if (L < cols) {
-    // dense rows: the sum of the x-update's row-tile planes, in order
-    const int n0 = L * LANES, mt = a.np / 64;
-    for (int sl = w; sl < kDenseMax; sl += NWV) {
-      const int j = hrows[sl];
-      if (j < 0) continue;
-      float v[16];
-#pragma unroll
-      for (int mi = 0; mi < 16; ++mi) v[mi] = mi < mt ? AXp[((size_t)mi * 32 + sl) * a.ldb + n0 + lane] : 0.f;
-      float t = 0.f;
-      for (int mi = 0; mi < mt; ++mi) t += v[mi];
-      for (int mi = 16; mi < mt; ++mi) t += AXp[((size_t)mi * 32 + sl) * a.ldb + n0 + lane];
-      a.AX[(size_t)j * a.ldb + n0 + lane] = T(t);
-    }
-    return;
-  }
-  L -= cols;
-  double* buf = reinterpret_cast<double*>(&red[0][0][0]);
-  int bx, by;
-  spmm_block(cols, in.nitems, bx, by, L);
-  const unsigned boff = (unsigned)(bx * LANES + lane) * (unsigned)sizeof(T);
-  T* Z = a.AX;
-  spmm_core<T, 1>(in, bx, by, buf, [](int) {}, [&](int row, const T (&v)[1]) {
-    if (ld_uniform(a.hslot + row) < 0) at_lane(Z + (size_t)row * in.ldb, boff) = v[0];
-  });
-}
-
 // Phase C, first pass.  Work items (one per blockIdx.y, host-built, at most
 // RPC rows each):
 //   (i0, i1, 0, -)  rows of x of finished instances: XN = X (the host swaps
@@ -767,12 +734,44 @@ __global__ __launch_bounds__(FT) void k_fam_c1(FamArgs<T> a, const int4* __restr
   double n2 = 0.0, p2 = 0.0, by = 0.0;
   if (act) {
     T bv[RW], av[RW], sv[RW], yv[RW];
+    int2 rc[RW];
 #pragma unroll
     for (int k = 0; k < RW; ++k) {
       const int j = it.x + w + k * NWV;
       if (j < it.y) {
         const size_t o = (size_t)j * a.ldb + inst;
-        bv[k] = bval(a, j, inst); av[k] = a.AX[o]; sv[k] = a.S[o]; yv[k] = a.Y[o];
+        bv[k] = bval(a, j, inst); sv[k] = a.S[o]; yv[k] = a.Y[o];
+        if constexpr (kCorr<T>) {
+          if (a.rec8) {  // AX_j from XN (one entry) or later (dense row, several entries)
+            rc[k] = ld_uniform(a.rec8 + ((size_t)blockIdx.y * NWV + w) * RW + k);
+            av[k] = rc[k].x >= 0 ? a.XN[(size_t)rc[k].x * a.ldb + inst] : T(0);
+            continue;
+          }
+        }
+        av[k] = a.AX[o];
+      }
+    }
+    if constexpr (kCorr<T>) {
+      if (a.rec8) {
+#pragma unroll
+        for (int k = 0; k < RW; ++k) {
+          const int j = it.x + w + k * NWV;
+          if (j >= it.y) continue;
+          T v;
+          if (rc[k].x >= 0) {
+            v = T(__int_as_float(rc[k].y)) * av[k];
+          } else if (rc[k].x == -1) {  // dense row: the x-update's planes, in order
+            v = T(0);
+            const int sl = ld_uniform(a.hslot + j);
+            for (int mi = 0; mi < a.mt; ++mi) v += T(a.AXp[((size_t)mi * 32 + sl) * a.ldb + inst]);
+          } else {
+            v = T(0);
+            for (int q = ld_uniform(a.lrp + j); q < ld_uniform(a.lrp + j + 1); ++q)
+              v += T(ld_uniform(a.lrx + q)) * a.XN[(size_t)ld_uniform(a.lri + q) * a.ldb + inst];
+          }
+          av[k] = v;
+          a.AX[(size_t)j * a.ldb + inst] = v;
+        }
       }
     }
     const T lam = T(a.lam[inst]);
@@ -938,14 +937,21 @@ constexpr int kXuSteps = 4;
 // pass through LDS per step.  The four partial tiles meet in LDS and are
 // summed in wave order (deterministic) with the addend X.  Grid 1-D over
 // (row tile, column tile); with ntile % 8 == 0 the row tiles of one column
-// tile go to one XCD, consecutively, so their Rcat reads share its L2.
+// tile go to one XCD, consecutively, so their Rcat reads share its L2.  For an
+// A with dense rows the launch also carries the iteration's stop test
+// (workgroups past the tiles, one per column block).
 __global__ __launch_bounds__(256) void k_fam_xupdate(int mt, int nt, int ksteps, const __bf16* __restrict__ G,
                                                      int ldg, const __bf16* __restrict__ R, int ldb,
                                                      const float* __restrict__ X, float* __restrict__ XN,
-                                                     const float* __restrict__ Ahr, float* __restrict__ AXp) {
+                                                     const float* __restrict__ Ahr, float* __restrict__ AXp,
+                                                     FamArgs<float> fa, int chunks_m, int chunks_n) {
   __shared__ float red[4][64][65];
   const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
   const int L = blockIdx.x;
+  if (L >= mt * nt) {  // the iteration's stop test for column block L - mt nt (A with dense rows)
+    decide_block<float, 4>(fa, L - mt * nt, chunks_m, chunks_n, reinterpret_cast<double4(*)[LANES]>(&red[0][0][0]));
+    return;
+  }
   int mi, ni;
   if ((nt & 7) == 0) {
     const int seq = L >> 3;
@@ -1207,16 +1213,15 @@ struct Family {
   int *li = nullptr, *lrow = nullptr;
   void* lx = nullptr;
   int4* ltasks = nullptr;
-  // ... and for the A product: A_h (MFMA order), the dense rows (-1 padded),
-  // the slot of every row of A, the light CSR of A (dense rows empty) and its
-  // task tables (coarse, fine)
+  // ... and for the A product, which the x-update (dense rows: its planes
+  // AXp) and phase C (the other rows, from XN) form: A_h (MFMA order), the
+  // slot of every row of A, the light CSR of A (dense rows empty) and phase
+  // C's per-row records
   float* Ahr = nullptr;
-  int *hrows = nullptr, *hslot = nullptr;
-  int *lri = nullptr, *lrrow = nullptr;
-  void* lrx = nullptr;
-  int4* lAt[2] = {nullptr, nullptr};
+  int *hslot = nullptr, *lrpd = nullptr, *lri = nullptr;
+  float* lrxf = nullptr;
+  int2* rec8d = nullptr;
   float* AXp = nullptr;  // per batch: the x-update's row-tile planes of A_h XN (np / 64 x 32 x ldb)
-  int nLA[2] = {0, 0};
   bool hybrid() const { return nh > 0 && !fp64 && !dense; }
   const Tasks* curA = &tasks[0];   // task table of the A product for the current width
   const Tasks* curAt = &tasks[0];  // and of the A' product
@@ -1424,6 +1429,7 @@ bool Family::create(const Data* d, const Cone* k, const PdosOptions& o, std::str
   }
   rowA = static_cast<int*>(upload(entry_rows(S.rp), err));
   rowAt = static_cast<int*>(upload(entry_rows(S.cp), err));
+  std::function<int2(int)> rec_rows;  // set when A has dense rows
   std::vector<int> hc;  // dense rows of A (mixed precision, sparse A), ascending
   if (!fp64 && S.nnz() <= (long)m * n / 4) {
     for (int r = 0; r < m; ++r)
@@ -1476,8 +1482,6 @@ bool Family::create(const Data* d, const Cone* k, const PdosOptions& o, std::str
         const int j = S.ri[q];
         ahr[((size_t)(j / 4) * 2 + k / 16) * 64 + (j % 4) * 16 + k % 16] += float(S.rx[q]);
       }
-    std::vector<int> hr(kDenseMax, -1);
-    for (int k = 0; k < nh; ++k) hr[k] = hc[k];
     std::vector<int> lrp{0}, lridx;
     std::vector<double> lrval;
     for (int r = 0; r < m; ++r) {
@@ -1488,22 +1492,25 @@ bool Family::create(const Data* d, const Cone* k, const PdosOptions& o, std::str
         }
       lrp.push_back((int)lridx.size());
     }
-    const std::vector<int> lrr = entry_rows(lrp);
-    for (int gi = 0; gi < 2; ++gi) {
-      const std::vector<int4> t = spmm_tasks(lrp, gi == 0 ? 64 : 16);
-      nLA[gi] = (int)t.size() / NWV;
-      lAt[gi] = static_cast<int4*>(upload(t, err));
-    }
+    rec_rows = [&, sl, lrp, lridx, lrval](int j) {  // phase C's record of row j of A
+      if (sl[j] >= 0) return int2{-1, 0};
+      const int q0 = lrp[j], q1 = lrp[j + 1];
+      if (q1 - q0 != 1) return int2{q1 == q0 ? -2 : -3, 0};
+      const float v = float(lrval[q0]);
+      int vb;
+      std::memcpy(&vb, &v, 4);
+      return int2{lridx[q0], vb};
+    };
     if (lridx.empty()) {
       lridx.push_back(0);
       lrval.push_back(0.0);
     }
+    std::vector<float> lrvf(lrval.begin(), lrval.end());
     Ahr = static_cast<float*>(upload(ahr, err));
-    hrows = static_cast<int*>(upload(hr, err));
     hslot = static_cast<int*>(upload(sl, err));
+    lrpd = static_cast<int*>(upload(lrp, err));
     lri = static_cast<int*>(upload(lridx, err));
-    lrx = upload_vals(lrval, err);
-    lrrow = static_cast<int*>(upload(lrr, err));
+    lrxf = static_cast<float*>(upload(lrvf, err));
     Ah = static_cast<float*>(upload(ah, err));
     li = static_cast<int*>(upload(lidx, err));
     lx = upload_vals(lval, err);
@@ -1530,6 +1537,18 @@ bool Family::create(const Data* d, const Cone* k, const PdosOptions& o, std::str
     nc1 = (int)v1.size();
     nc2 = (int)v2.size();
     c1 = static_cast<int4*>(upload(v1, err));
+    if (rec_rows) {
+      // phase C's record for row j = item.x + w + NWV k of every item (RW per wave)
+      std::vector<int2> r8((size_t)nc1 * NWV * RW, int2{-2, 0});
+      for (int t = 0; t < nc1; ++t)
+        if (v1[t].z != 0)
+          for (int w = 0; w < NWV; ++w)
+            for (int k = 0; k < RW; ++k) {
+              const int j = v1[t].x + w + k * NWV;
+              if (j < v1[t].y) r8[((size_t)t * NWV + w) * RW + k] = rec_rows(j);
+            }
+      rec8d = static_cast<int2*>(upload(r8, err));
+    }
     if (!v2.empty()) {
       c2 = static_cast<int4*>(upload(v2, err));
       c2_cone = static_cast<int*>(upload(vc, err));
@@ -1730,6 +1749,14 @@ void Family::fill_args(FamArgs<T>& a) {
   a.nlin_items = nlin_items;
   a.nprod = nprod;
   a.hslot = hslot;
+  if (hybrid()) {
+    a.rec8 = rec8d;
+    a.AXp = AXp;
+    a.mt = np / 64;
+    a.lrp = lrpd;
+    a.lri = lri;
+    a.lrx = lrxf;
+  }
 }
 
 template <typename T>
@@ -1759,20 +1786,24 @@ bool Family::iterate(FamArgs<T>& a, int steps, std::string& err) {
         return false;
       }
     } else {
+      // A with dense rows: the launch also forms their AX planes and runs
+      // the stop test; phase C forms the rest of AX
       const int mt = np / 64, nt = a.ldb / 64;
-      hipLaunchKernelGGL(k_fam_xupdate, dim3(mt * nt), dim3(256), 0, st, mt, nt, nprod * np / 32,
+      const bool fuse = hybrid();
+      FamArgs<float> fa{};
+      if constexpr (std::is_same<T, float>::value) fa = a;
+      hipLaunchKernelGGL(k_fam_xupdate, dim3(mt * nt + (fuse ? nt : 0)), dim3(256), 0, st, mt, nt, nprod * np / 32,
                          static_cast<const __bf16*>(Gcat), 3 * np, a.Rcat, a.ldb,
                          reinterpret_cast<const float*>(a.X), reinterpret_cast<float*>(a.XN),
-                         hybrid() ? Ahr : nullptr, hybrid() ? AXp : nullptr);
+                         fuse ? Ahr : nullptr, fuse ? AXp : nullptr, fa, cm, cn);
     }
-    mark(K_AX);
     if (dense) {
+      mark(K_AX);
       if (!apply_A(a, a.XN, a.AX)) { err = "A product failed"; return false; }
     } else if (hybrid()) {
-      const int gi = pick_tasks(cols, nLA[0], nLA[1]);
-      const SpmmIn<T, 1> in{lAt[gi], lrrow, lri, static_cast<const T*>(lrx), a.XN, a.XN, a.XN, a.ldb, cols, nLA[gi]};
-      hipLaunchKernelGGL(k_fam_ax_hyb<T>, dim3(2 * cols + cols * nLA[gi]), block, 0, st, a, in, AXp, hrows, cm, cn);
+      // formed by the x-update and phase C
     } else {
+      mark(K_AX);
       const SpmmIn<T, 1> in{curA->A, rowA, a.ri, a.rx, a.XN, a.XN, a.XN, a.ldb, cols, curA->nA};
       hipLaunchKernelGGL(k_fam_ax_decide<T>, dim3(cols + cols * curA->nA), block, 0, st, a, in, cm, cn);
     }
