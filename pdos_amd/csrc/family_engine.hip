@@ -40,6 +40,8 @@
 #include <functional>
 #include <string>
 #include <type_traits>
+#include <memory>
+#include <thread>
 #include <vector>
 
 #include "../../include/pdos_ext.h"
@@ -2044,6 +2046,10 @@ bool Family::run_t(const float* warm, long wld, int judge, std::string& err) {
 }
 
 bool Family::run(const float* warm, long wld, int judge, std::string& err) {
+  if (hipSetDevice(device) != hipSuccess) {  // the calling thread's device
+    err = "hipSetDevice failed";
+    return false;
+  }
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
@@ -2103,12 +2109,17 @@ bool Family::fetch(double* X, double* Y, double* Sx, long* iters, long* status, 
 
 }  // namespace pdos_amd
 
+// A family handle: one engine, or for mixed precision two, which run the two
+// halves of a large batch at once on their own streams (host threads): the
+// narrow, latency-bound end of one half overlaps the other's.
 struct PdosFamily {
-  pdos_amd::Family F;
+  std::vector<std::unique_ptr<pdos_amd::Family>> F;
+  std::vector<int> first, count;  // instance range of each engine in use
 };
 
 namespace {
 thread_local std::string g_fam_err;
+constexpr int kSplitMin = 8192;  // smallest batch split over two engines
 }
 
 extern "C" {
@@ -2120,11 +2131,15 @@ PdosFamily* pdos_family_create(const Data* d, const Cone* k, const PdosOptions* 
   if (opt) o = *opt;
   else pdos_default_options(&o);
   PdosFamily* h = new PdosFamily;
-  std::string err;
-  if (!h->F.create(d, k, o, err)) {
-    g_fam_err = err;
-    delete h;
-    return nullptr;
+  const int engines = o.precision == PDOS_FP64 ? 1 : 2;
+  for (int e = 0; e < engines; ++e) {
+    h->F.emplace_back(new pdos_amd::Family);
+    std::string err;
+    if (!h->F.back()->create(d, k, o, err)) {
+      g_fam_err = err;
+      delete h;
+      return nullptr;
+    }
   }
   return h;
 }
@@ -2132,27 +2147,61 @@ PdosFamily* pdos_family_create(const Data* d, const Cone* k, const PdosOptions* 
 int pdos_family_load(PdosFamily* h, int count, const double* C, const double* B) {
   std::string err;
   if (!h || count < 1 || !C) { g_fam_err = "bad arguments"; return -1; }
-  if (!h->F.load(count, C, B, err)) { g_fam_err = err; return -1; }
+  const int parts = h->F.size() > 1 && count >= kSplitMin ? 2 : 1;
+  h->first.clear();
+  h->count.clear();
+  const int n = h->F[0]->n, m = h->F[0]->m;
+  int at = 0;
+  for (int p = 0; p < parts; ++p) {
+    const int c = p + 1 == parts ? count - at : ((count / parts + 63) & ~63);
+    if (!h->F[p]->load(c, C + (size_t)at * n, B ? B + (size_t)at * m : nullptr, err)) {
+      g_fam_err = err;
+      return -1;
+    }
+    h->first.push_back(at);
+    h->count.push_back(c);
+    at += c;
+  }
   return 0;
 }
 
 int pdos_family_run(PdosFamily* h, const float* warm_dev, long ld, int judge, double* kernel_ms) {
-  std::string err;
-  if (!h || h->F.count < 1) { g_fam_err = "nothing loaded"; return -1; }
-  if (!h->F.run(warm_dev, ld, judge, err)) { g_fam_err = err; return -1; }
-  if (kernel_ms) *kernel_ms = h->F.last_ms;
+  if (!h || h->count.empty()) { g_fam_err = "nothing loaded"; return -1; }
+  const int parts = (int)h->count.size();
+  std::vector<std::string> errs(parts);
+  std::vector<char> ok(parts, 0);
+  auto go = [&](int p) {
+    const float* w = warm_dev ? warm_dev + (size_t)h->first[p] * ld : nullptr;
+    ok[p] = h->F[p]->run(w, ld, judge, errs[p]) ? 1 : 0;
+  };
+  if (parts == 1 || h->F[0]->timing) {  // per-kernel timing: one engine at a time
+    for (int p = 0; p < parts; ++p) go(p);
+  } else {
+    std::vector<std::thread> th;
+    for (int p = 1; p < parts; ++p) th.emplace_back(go, p);
+    go(0);
+    for (std::thread& t : th) t.join();
+  }
+  double ms = 0.0;
+  for (int p = 0; p < parts; ++p) {
+    if (!ok[p]) { g_fam_err = errs[p]; return -1; }
+    ms = std::max(ms, (double)h->F[p]->last_ms);
+  }
+  if (kernel_ms) *kernel_ms = ms;
   return 0;
 }
 
 int pdos_family_info(PdosFamily* h, long* out) {
   if (!h) return -1;
-  out[0] = h->F.n;
-  out[1] = h->F.m;
-  out[2] = h->F.np;
-  out[3] = h->F.compactions;
-  out[4] = h->F.dense ? 1 : 0;
-  out[5] = h->F.fp64 ? 1 : 0;
-  out[6] = h->F.nprod;
+  const pdos_amd::Family& F = *h->F[0];
+  out[0] = F.n;
+  out[1] = F.m;
+  out[2] = F.np;
+  out[3] = 0;
+  for (size_t p = 0; p < h->count.size(); ++p) out[3] += h->F[p]->compactions;
+  out[4] = F.dense ? 1 : 0;
+  out[5] = F.fp64 ? 1 : 0;
+  out[6] = F.nprod;
   return 0;
 }
 
@@ -2160,7 +2209,16 @@ int pdos_family_fetch(PdosFamily* h, double* X, double* Y, double* S, long* iter
                       long* fallback, double* resid) {
   std::string err;
   if (!h) return -1;
-  if (!h->F.fetch(X, Y, S, iters, status, fallback, resid, err)) { g_fam_err = err; return -1; }
+  const long n = h->F[0]->n, m = h->F[0]->m;
+  for (size_t p = 0; p < h->count.size(); ++p) {
+    const long f = h->first[p];
+    if (!h->F[p]->fetch(X ? X + f * n : nullptr, Y ? Y + f * m : nullptr, S ? S + f * m : nullptr,
+                        iters ? iters + f : nullptr, status ? status + f : nullptr,
+                        fallback ? fallback + f : nullptr, resid ? resid + f * 5 : nullptr, err)) {
+      g_fam_err = err;
+      return -1;
+    }
+  }
   return 0;
 }
 
@@ -2168,16 +2226,23 @@ void pdos_family_free(PdosFamily* h) { delete h; }
 
 int pdos_family_timing(PdosFamily* h, int enable, double* ms, long* calls, double* cols) {
   if (!h) return -1;
-  pdos_amd::Family& F = h->F;
   for (int k = 0; k < pdos_amd::Family::K_KINDS; ++k) {
-    if (ms) ms[k] = F.kind_ms[k];
-    if (calls) calls[k] = F.kind_calls[k];
-    if (cols) cols[k] = F.kind_cols[k];
-    F.kind_ms[k] = 0;
-    F.kind_calls[k] = 0;
-    F.kind_cols[k] = 0;
+    if (ms) ms[k] = 0;
+    if (calls) calls[k] = 0;
+    if (cols) cols[k] = 0;
   }
-  F.timing = enable != 0;
+  for (auto& Fp : h->F) {
+    pdos_amd::Family& F = *Fp;
+    for (int k = 0; k < pdos_amd::Family::K_KINDS; ++k) {
+      if (ms) ms[k] += F.kind_ms[k];
+      if (calls) calls[k] += F.kind_calls[k];
+      if (cols) cols[k] += F.kind_cols[k];
+      F.kind_ms[k] = 0;
+      F.kind_calls[k] = 0;
+      F.kind_cols[k] = 0;
+    }
+    F.timing = enable != 0;
+  }
   return 0;
 }
 

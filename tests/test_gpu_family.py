@@ -107,6 +107,27 @@ def test_family_mixed_solutions_verify_in_fp64():
     assert np.all(np.abs(p32 - p64) <= 20 * 1e-3 * (1 + np.abs(p64)))
 
 
+def test_family_split_batch_matches_single_engine_runs():
+    """A mixed batch of 8,192 or more runs as two halves on two engines at
+    once.  With dense rows in A no kernel's sums depend on the batch width,
+    so every instance must come out bit for bit as in separate runs of 2,048."""
+    fam = _fam(na=200, k=15, MAX_ITERS=300)
+    _, Cm = fam.sample(seed=12, count=8192)
+    print("family split batch", flush=True)
+    F = Family(fam.instance(Cm[0]), precision="mixed")
+    F.load(Cm)
+    F.run()
+    whole = F.fetch()
+    parts = []
+    for q in range(4):
+        F.load(Cm[q * 2048:(q + 1) * 2048])
+        F.run()
+        parts.append(F.fetch())
+    for key in ("iters", "status", "x", "y", "s"):
+        np.testing.assert_array_equal(whole[key], np.concatenate([p[key] for p in parts]))
+    assert (whole["status"] == "Solved").sum() >= 0.5 * len(Cm)
+
+
 def test_warm_start_judge():
     fam = _fam()
     _, Cm = fam.sample(seed=9, count=16)
