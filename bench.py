@@ -174,9 +174,12 @@ def plain_gpu(probs, plugin, precision):
 # ------------------------------------------------------------------ roofline
 
 # kernel names behind each timed kind of the family engine (sparse A)
-FAMILY_KERNELS = {"At_product": ["k_fam_at_b<{t}>"], "A_product": ["k_fam_spmm<{t}, 1>"],
-                  "xupdate_gemm": ["k_gemm<false, false, 4>"], "phase_b": ["k_fam_decide<{t}>"],
-                  "phase_c": ["k_fam_c1<{t}>", "k_fam_c2<{t}>"], "phase_a": ["k_fam_a<{t}>"]}
+# (sparse A; with dense rows in A the A' product is k_fam_at_hyb, the A
+# product and the stop test ride in the x-update and phase C)
+FAMILY_KERNELS = {"At_product": ["k_fam_at_b<{t}>", "k_fam_at_hyb<{t}>"],
+                  "A_product": ["k_fam_ax_decide<{t}>"], "xupdate_gemm": ["k_fam_xupdate"],
+                  "phase_b": ["k_fam_decide<{t}>"], "phase_c": ["k_fam_c1<{t}>", "k_fam_c2<{t}>"],
+                  "phase_a": ["k_fam_a<{t}>"]}
 
 
 def measured_traffic(profile, kind, fp64):
@@ -191,7 +194,8 @@ def measured_traffic(profile, kind, fp64):
         return None
     data = json.loads(text.split("```json", 1)[1].split("```", 1)[0])["kernels"]
     names = [k.format(t="double" if fp64 else "float") for k in FAMILY_KERNELS.get(kind, [])]
-    if not names or any(k not in data for k in names):
+    names = [k for k in names if k in data]
+    if not names:
         return None
     return sum(data[k]["read_bytes"] + data[k]["write_bytes"] for k in names)
 
@@ -212,8 +216,12 @@ def family_roofline(kt, fam_info, nnz, fp64, lin_rows, b_inst, traffic_profile=N
                   (6, 7); cone rows: pass 1 AX, S, Y (+ b) read, W, S
                   written; pass 2 S, W, Y (+ AX, + b) read, Y, S, W written
                   (11 in mixed mode, where s~ stands in for b - AX; 12 in
-                  fp64; + 2 with per-instance b)
-    x-update GEMM: 2 n np flops per product (bf16) or 2 n^2 (fp64)."""
+                  fp64; + 2 with per-instance b).  With dense rows in A
+                  phase C forms AX: the row of XN its entry names is read in
+                  place of AX and AX is written, one value more per row (7,
+                  12), and each dense row reads the x-update's np / 64 planes
+    x-update GEMM: 2 n np flops per product (bf16) or 2 n^2 (fp64), with
+                  dense rows in A + 2 * 32 np (their A_h XN planes)."""
     n, m, npad = fam_info["n"], fam_info["m"], fam_info["np"]
     dense = bool(fam_info.get("dense", 0))
     nprod = fam_info.get("products", 1)
@@ -234,8 +242,9 @@ def family_roofline(kt, fam_info, nnz, fp64, lin_rows, b_inst, traffic_profile=N
         return {"bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
                 "frac": round(ach / peak, 4), "traffic": None, "kernel": kind,
                 "algorithmic_flops_per_launch": flops / max(calls, 1), "avg_launch_us": round(1e3 * ms / max(calls, 1), 2)}
+    nh = fam_info.get("dense_rows", 0)
     if kind == "xupdate_gemm":
-        flops = (2.0 * n * n if fp64 else 2.0 * npad * nprod * npad) * cols
+        flops = (2.0 * n * n if fp64 else 2.0 * npad * nprod * npad + (2.0 * 32 * npad if nh else 0.0)) * cols
         ach = flops / (ms * 1e-3) / 1e12
         peak = FP64_PEAK_TFLOPS if fp64 else BF16_PEAK_TFLOPS
         return {"bound": "mfma", "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
@@ -251,8 +260,9 @@ def family_roofline(kt, fam_info, nnz, fp64, lin_rows, b_inst, traffic_profile=N
     elif kind == "phase_b":
         byt = cols * (n * sv * 4 + rhs) if dense else cols * 16 * ((n + 63) // 64 + (m + 63) // 64)
     else:
-        cone = (12 if fp64 else 11) + (2 if b_inst else 0)
-        rows = (7 if b_inst else 6) * lin_rows + cone * cone_rows
+        extra = 1 if nh else 0
+        cone = (12 if fp64 else 11) + (2 if b_inst else 0) + extra
+        rows = (7 if b_inst else 6) * lin_rows + cone * cone_rows + extra * lin_rows + nh * (npad // 64)
         byt = cols * rows * sv
     ach = byt / (ms * 1e-3) / 1e9
     traffic = measured_traffic(traffic_profile, kind, fp64) if traffic_profile and not dense else None
@@ -340,11 +350,17 @@ def run_family(args):
         step()
     wall = (time.perf_counter() - t0) / args.steps
     warm = F.fetch()
-    comp = F.info()["compactions"]
-    # one more step with per-kernel events for the roofline
+    info = F.info()
+    comp = info["compactions"]
+    # one more step with per-kernel events for the roofline, on one engine
+    # over the whole batch (the timed steps split it over two running at once)
+    engines = F.engines()
+    F.engines(1)
+    F.load(C_te)
     F.timing(True)
     step()
     kt = F.timing(False)
+    F.engines(engines)
     A = sp.csc_matrix((fam.Ax, fam.Ai, fam.Ap), shape=(fam.m, fam.n))
     v_warm = verify_batch(A, fam.b, C_te, warm["x"], warm["y"], warm["s"], warm["status"], 1e-3)
     v_cold = verify_batch(A, fam.b, C_te, cold["x"], cold["y"], cold["s"], cold["status"], 1e-3)
@@ -361,6 +377,7 @@ def run_family(args):
     config = {"workload": "portfolio family, learned warm start, held-out batch", "n_a": 500, "k": 20,
               "n": fam.n, "m": fam.m, "nnz": int(fam.Ap[-1]), "count": n_te, "train_count": n_tr,
               "eps_abs": 1e-3, "max_iters": 2000, "precision": prec, "engine": "family (shared A)",
+              "engines": info["engines"], "dense_rows": info["dense_rows"],
               "mlp": "501-512-512-2545 GELU bf16"}
     extra = {
         "batched_cold": {"value": round(solved_cold / cold_wall, 3), "unit": "SOCPs/s",

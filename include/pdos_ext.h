@@ -133,8 +133,12 @@ const char *pdos_family_last_error(void);
  * phase B, x-update GEMM, A product, phase C; then turns the timing on or off
  * for later runs (events add a little launch overhead). */
 int pdos_family_timing(PdosFamily *h, int enable, double *ms, long *calls, double *cols);
-/* out[7]: n, m, padded n of the GEMM, compactions in the last run, dense A,
- * fp64, bf16 products of the mixed x-update. */
+/* Engines the next pdos_family_load splits a large mixed-precision batch
+ * over (1 or 2; n <= 0 only queries); returns the setting. */
+int pdos_family_engines(PdosFamily *h, int n);
+/* out[9]: n, m, padded n of the GEMM, compactions in the last run, dense A,
+ * fp64, bf16 products of the mixed x-update, dense rows of A multiplied as
+ * MFMA tiles (0: none), engines the loaded batch is split over (1 or 2). */
 int pdos_family_info(PdosFamily *h, long *out);
 
 /* Learned warm start: MLP n_in -> n_hidden -> n_hidden -> n_out with GELU,

@@ -55,7 +55,7 @@ EXPORTS = ["pdos", "freeData", "freeSol", "tic", "toc", "tocq", "pdos_default_op
            "pdos_mlp_set_affine", "pdos_mlp_loss_grad", "pdos_mlp_train", "pdos_mlp_predict",
            "pdos_mlp_predict_dev", "pdos_mlp_last_error", "pdos_family_create", "pdos_family_load",
            "pdos_family_run", "pdos_family_fetch", "pdos_family_free", "pdos_family_last_error",
-           "pdos_family_timing", "pdos_family_info",
+           "pdos_family_timing", "pdos_family_info", "pdos_family_engines",
            "pdos_kkt_solve_host", "pdos_last_error",
            "pdos_version"]
 

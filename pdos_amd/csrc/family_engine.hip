@@ -2115,6 +2115,7 @@ bool Family::fetch(double* X, double* Y, double* Sx, long* iters, long* status, 
 struct PdosFamily {
   std::vector<std::unique_ptr<pdos_amd::Family>> F;
   std::vector<int> first, count;  // instance range of each engine in use
+  int max_engines = 2;            // engines a batch is split over (pdos_family_engines)
 };
 
 namespace {
@@ -2147,7 +2148,7 @@ PdosFamily* pdos_family_create(const Data* d, const Cone* k, const PdosOptions* 
 int pdos_family_load(PdosFamily* h, int count, const double* C, const double* B) {
   std::string err;
   if (!h || count < 1 || !C) { g_fam_err = "bad arguments"; return -1; }
-  const int parts = h->F.size() > 1 && count >= kSplitMin ? 2 : 1;
+  const int parts = std::min<int>(h->max_engines, (int)h->F.size()) > 1 && count >= kSplitMin ? 2 : 1;
   h->first.clear();
   h->count.clear();
   const int n = h->F[0]->n, m = h->F[0]->m;
@@ -2191,6 +2192,12 @@ int pdos_family_run(PdosFamily* h, const float* warm_dev, long ld, int judge, do
   return 0;
 }
 
+int pdos_family_engines(PdosFamily* h, int n) {
+  if (!h) return -1;
+  if (n > 0) h->max_engines = std::min<int>(n, (int)h->F.size());
+  return std::min<int>(h->max_engines, (int)h->F.size());
+}
+
 int pdos_family_info(PdosFamily* h, long* out) {
   if (!h) return -1;
   const pdos_amd::Family& F = *h->F[0];
@@ -2202,6 +2209,8 @@ int pdos_family_info(PdosFamily* h, long* out) {
   out[4] = F.dense ? 1 : 0;
   out[5] = F.fp64 ? 1 : 0;
   out[6] = F.nprod;
+  out[7] = F.hybrid() ? F.nh : 0;
+  out[8] = h->count.empty() ? 1 : (long)h->count.size();
   return 0;
 }
 

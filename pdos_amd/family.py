@@ -29,6 +29,8 @@ def _bind(L):
     L.pdos_family_timing.argtypes = [C.c_void_p, C.c_int, _dp, _lp, _dp]
     L.pdos_family_info.restype = C.c_int
     L.pdos_family_info.argtypes = [C.c_void_p, _lp]
+    L.pdos_family_engines.restype = C.c_int
+    L.pdos_family_engines.argtypes = [C.c_void_p, C.c_int]
     L._fam_bound = True
 
 
@@ -97,11 +99,17 @@ class Family:
                                   cols.ctypes.data_as(_dp))
         return {k: (float(ms[i]), int(calls[i]), float(cols[i])) for i, k in enumerate(self.KINDS)}
 
+    def engines(self, n: int = 0) -> int:
+        """Engines the next load splits a large mixed batch over (1 or 2; 0
+        only queries)."""
+        return int(self.L.pdos_family_engines(self.h, int(n)))
+
     def info(self) -> dict:
-        out = np.zeros(7, dtype=np.int64)
+        out = np.zeros(9, dtype=np.int64)
         self.L.pdos_family_info(self.h, out.ctypes.data_as(_lp))
         return dict(n=int(out[0]), m=int(out[1]), np=int(out[2]), compactions=int(out[3]),
-                    dense=bool(out[4]), fp64=bool(out[5]), products=int(out[6]))
+                    dense=bool(out[4]), fp64=bool(out[5]), products=int(out[6]),
+                    dense_rows=int(out[7]), engines=int(out[8]))
 
     def __del__(self):
         try:
