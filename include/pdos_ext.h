@@ -112,7 +112,10 @@ int pdos_device_copy(void *dst, const void *src, size_t bytes);
  * template data d (its b is the default b, its c is ignored), differing in c
  * (and optionally b).  Solved in lockstep with G = (I + A'A)^-1 formed once:
  * the x-update is one GEMM for the whole batch (PDOS_FP64: rocBLAS dgemm;
- * PDOS_MIXED: split-bf16 MFMA, three bf16 products accumulated in binary32).
+ * PDOS_MIXED: the correction x + G (A'W - lambda c) with one bf16 MFMA
+ * product accumulated in binary32; rows of A with 64 or more entries are
+ * multiplied as binary32 MFMA tiles).  A mixed load of 8,192 or more
+ * instances is split over two engines that run at once.
  *   load   C: count x n, B: count x m or NULL (original units, row-major)
  *   run    warm_dev: optional device array count x ld of [x, y, s] in original
  *          units; judge != 0 compares it with the cold start per instance
