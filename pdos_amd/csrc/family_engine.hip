@@ -77,11 +77,11 @@ struct FamArgs {
   int nlin_items;      // phase C: items of zero / nonneg rows (then the cone chunks)
   T* X; T* XN; T* S; T* Y; T* AX; T* W; T* U; T* V; T* R;
   __bf16* Rcat;        // mixed: [D_hi; D_lo; D_hi] of the correction D, 3 np x ldb
-  const int* hslot;    // mixed, A with dense rows: slot of row j of A among them, or -1
-  // ... and phase C forms AX itself: per (item, wave, k) the row's record
-  // {column of its only entry, bits of its value} or {-1 dense row (the sum
-  // of the x-update's planes AXp, mt of them), 0} / {-2 no entries} / {-3:
-  // several entries, the light CSR lrp, lri, lrx}
+  // mixed, A with dense rows: phase C forms AX itself: per (item, wave, k)
+  // the row's record {column of its only entry, bits of its value} or {-1,
+  // the row's slot among the dense rows} (the sum of the x-update's planes
+  // AXp, mt of them) / {-2 no entries} / {-3 several entries: the light CSR
+  // lrp, lri, lrx}
   const int2* rec8; const float* AXp; int mt;
   const int* lrp; const int* lri; const float* lrx;
   int* done; int* status; int* iters; int* fallback;
@@ -480,6 +480,8 @@ __global__ __launch_bounds__(FT) void k_fam_at_hyb(FamArgs<T> a, SpmmIn<T, 2> in
 // issues the loads of all its rows before it computes or stores anything.
 constexpr int RPC = 64;
 constexpr int RW = RPC / NWV;
+constexpr int kPlanes = 8;  // loads in flight per dense row (planes) or light row (entries) in phase C
+constexpr int kDenseK = 3;  // phase C: a wave's rows k < kDenseK load their planes with the other rows
 
 // Phase A over the m-row items of phase C (item blockIdx.y of `items`):
 // W (next_w) and the item's partial |(S - b + AX) ./ D|^2 and b'Y per
@@ -736,6 +738,7 @@ __global__ __launch_bounds__(FT) void k_fam_c1(FamArgs<T> a, const int4* __restr
   double n2 = 0.0, p2 = 0.0, by = 0.0;
   if (act) {
     T bv[RW], av[RW], sv[RW], yv[RW];
+    T pv[kDenseK][kPlanes];  // planes of A_h XN of the dense rows among a wave's first kDenseK rows
     int2 rc[RW];
 #pragma unroll
     for (int k = 0; k < RW; ++k) {
@@ -744,9 +747,14 @@ __global__ __launch_bounds__(FT) void k_fam_c1(FamArgs<T> a, const int4* __restr
         const size_t o = (size_t)j * a.ldb + inst;
         bv[k] = bval(a, j, inst); sv[k] = a.S[o]; yv[k] = a.Y[o];
         if constexpr (kCorr<T>) {
-          if (a.rec8) {  // AX_j from XN (one entry) or later (dense row, several entries)
+          if (a.rec8) {  // AX_j from XN (one entry), the planes (dense row) or later (several entries)
             rc[k] = ld_uniform(a.rec8 + ((size_t)blockIdx.y * NWV + w) * RW + k);
             av[k] = rc[k].x >= 0 ? a.XN[(size_t)rc[k].x * a.ldb + inst] : T(0);
+            if (k < kDenseK && rc[k].x == -1) {
+#pragma unroll
+              for (int u = 0; u < kPlanes; ++u)
+                if (u < a.mt) pv[k < kDenseK ? k : 0][u] = T(a.AXp[((size_t)u * 32 + rc[k].y) * a.ldb + inst]);
+            }
             continue;
           }
         }
@@ -764,12 +772,30 @@ __global__ __launch_bounds__(FT) void k_fam_c1(FamArgs<T> a, const int4* __restr
             v = T(__int_as_float(rc[k].y)) * av[k];
           } else if (rc[k].x == -1) {  // dense row: the x-update's planes, in order
             v = T(0);
-            const int sl = ld_uniform(a.hslot + j);
-            for (int mi = 0; mi < a.mt; ++mi) v += T(a.AXp[((size_t)mi * 32 + sl) * a.ldb + inst]);
-          } else {
+            int mi = 0;
+            if (k < kDenseK) {
+#pragma unroll
+              for (int u = 0; u < kPlanes; ++u)
+                if (u < a.mt) v += pv[k < kDenseK ? k : 0][u];
+              mi = kPlanes;
+            }
+            for (; mi < a.mt; ++mi) v += T(a.AXp[((size_t)mi * 32 + rc[k].y) * a.ldb + inst]);
+          } else {  // several light entries: kPlanes gathers in flight, summed in entry order
             v = T(0);
-            for (int q = ld_uniform(a.lrp + j); q < ld_uniform(a.lrp + j + 1); ++q)
-              v += T(ld_uniform(a.lrx + q)) * a.XN[(size_t)ld_uniform(a.lri + q) * a.ldb + inst];
+            const int q1 = ld_uniform(a.lrp + j + 1);
+            for (int q0 = ld_uniform(a.lrp + j); q0 < q1; q0 += kPlanes) {
+              T g[kPlanes];
+              float e[kPlanes];
+#pragma unroll
+              for (int u = 0; u < kPlanes; ++u) {
+                const int q = min(q0 + u, q1 - 1);
+                e[u] = ld_uniform(a.lrx + q);
+                g[u] = a.XN[(size_t)ld_uniform(a.lri + q) * a.ldb + inst];
+              }
+#pragma unroll
+              for (int u = 0; u < kPlanes; ++u)
+                if (q0 + u < q1) v += T(e[u]) * g[u];
+            }
           }
           av[k] = v;
           a.AX[(size_t)j * a.ldb + inst] = v;
@@ -1217,10 +1243,10 @@ struct Family {
   int4* ltasks = nullptr;
   // ... and for the A product, which the x-update (dense rows: its planes
   // AXp) and phase C (the other rows, from XN) form: A_h (MFMA order), the
-  // slot of every row of A, the light CSR of A (dense rows empty) and phase
-  // C's per-row records
+  // light CSR of A (dense rows empty) and phase C's per-row records (a dense
+  // row's record holds its slot among the dense rows)
   float* Ahr = nullptr;
-  int *hslot = nullptr, *lrpd = nullptr, *lri = nullptr;
+  int *lrpd = nullptr, *lri = nullptr;
   float* lrxf = nullptr;
   int2* rec8d = nullptr;
   float* AXp = nullptr;  // per batch: the x-update's row-tile planes of A_h XN (np / 64 x 32 x ldb)
@@ -1495,7 +1521,7 @@ bool Family::create(const Data* d, const Cone* k, const PdosOptions& o, std::str
       lrp.push_back((int)lridx.size());
     }
     rec_rows = [&, sl, lrp, lridx, lrval](int j) {  // phase C's record of row j of A
-      if (sl[j] >= 0) return int2{-1, 0};
+      if (sl[j] >= 0) return int2{-1, sl[j]};
       const int q0 = lrp[j], q1 = lrp[j + 1];
       if (q1 - q0 != 1) return int2{q1 == q0 ? -2 : -3, 0};
       const float v = float(lrval[q0]);
@@ -1509,7 +1535,6 @@ bool Family::create(const Data* d, const Cone* k, const PdosOptions& o, std::str
     }
     std::vector<float> lrvf(lrval.begin(), lrval.end());
     Ahr = static_cast<float*>(upload(ahr, err));
-    hslot = static_cast<int*>(upload(sl, err));
     lrpd = static_cast<int*>(upload(lrp, err));
     lri = static_cast<int*>(upload(lridx, err));
     lrxf = static_cast<float*>(upload(lrvf, err));
@@ -1750,7 +1775,6 @@ void Family::fill_args(FamArgs<T>& a) {
   a.nx_items = nx_items;
   a.nlin_items = nlin_items;
   a.nprod = nprod;
-  a.hslot = hslot;
   if (hybrid()) {
     a.rec8 = rec8d;
     a.AXp = AXp;
