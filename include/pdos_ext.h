@@ -31,6 +31,12 @@ extern "C" {
 enum { PDOS_INDIRECT = 0, PDOS_DIRECT = 1 };
 enum { PDOS_FP64 = 0, PDOS_MIXED = 1 };
 enum { PDOS_ENGINE_AUTO = 0, PDOS_ENGINE_WG = 1, PDOS_ENGINE_GRID = 2 };
+/* Scaling: the reference build's (common.h:35-218), or the alternative header
+ * the reference ships to be compiled in its place (ruiz_equil_common.h:98-322:
+ * with NORMALIZE, three Ruiz-type sweeps over [A b; c' 0] with cone-collapsed
+ * maxima; lambda = (1e-6 + |b|) / (1e-6 + |c|) in both cases).  pdos() takes
+ * it from the environment: PDOS_EQUIL=ruiz. */
+enum { PDOS_EQUIL_REFERENCE = 0, PDOS_EQUIL_RUIZ = 1 };
 
 typedef struct PdosOptions {
   int plugin;      /* PDOS_INDIRECT / PDOS_DIRECT */
@@ -43,7 +49,8 @@ typedef struct PdosOptions {
                         GEMM, 1 (hi*hi), 2 (+hi*lo) or 3 (+lo*hi); 0 = 1.  The
                         GEMM computes the correction x+ - x, whose error shrinks
                         with the step, so one product converges like three. */
-  int reserved[2];
+  int equil;       /* PDOS_EQUIL_* (the family engine: reference only) */
+  int reserved[1];
 } PdosOptions;
 
 typedef struct PdosInfo {
@@ -173,6 +180,12 @@ const char *pdos_mlp_last_error(void);
  * schedule the GPU sweeps use.  stats (optional, 4 longs): N, nnz(L),
  * forward levels, backward levels.  Returns 0 or -1. */
 int pdos_kkt_solve_host(const Data *d, const Cone *k, const double *rhs, double *z, long *stats);
+
+/* Host-side check of the setup: the scaling the solver iterates with, as the
+ * reference's initWork leaves it in Work (common.h:35-218, or with
+ * equil = PDOS_EQUIL_RUIZ ruiz_equil_common.h:98-322): D (m), E (n, with the
+ * x1e3) and lambda.  Returns 0 or -1 (pdos_last_error). */
+int pdos_setup_host(const Data *d, const Cone *k, int equil, double *D, double *E, double *lambda);
 
 /* Text of the last error on this thread ("" if none). */
 const char *pdos_last_error(void);

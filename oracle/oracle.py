@@ -48,6 +48,7 @@ def lib():
         L.oracle_setup.argtypes = [C.POINTER(Data), C.POINTER(Cone), C.POINTER(C.c_double),
                                    C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.oracle_kkt_order.argtypes = [C.POINTER(Data), C.POINTER(Cone), C.POINTER(C.c_long)]
+        L.oracle_set_equil.argtypes = [C.c_int]
         L.oracle_pdos_many.restype = C.c_long
         L.oracle_pdos_many.argtypes = [C.POINTER(C.POINTER(Data)), C.POINTER(C.POINTER(Cone)),
                                        C.c_long, C.c_int, C.POINTER(C.c_long)]
@@ -55,8 +56,20 @@ def lib():
     return _lib
 
 
-def solve(p: Problem, plugin: str = "indirect", perm=None, hist: int = 0) -> dict:
-    """Runs the oracle; returns x, s, y, status, iters, cg_iters, residuals."""
+EQUIL = {"reference": 0, "ruiz": 1}
+
+
+def solve(p: Problem, plugin: str = "indirect", perm=None, hist: int = 0, equil: str = "reference") -> dict:
+    """Runs the oracle; returns x, s, y, status, iters, cg_iters, residuals.
+    equil "ruiz": the reference's alternative scaling (ruiz_equil_common.h)."""
+    lib().oracle_set_equil(EQUIL[equil])
+    try:
+        return _solve(p, plugin, perm, hist)
+    finally:
+        lib().oracle_set_equil(0)
+
+
+def _solve(p, plugin, perm, hist):
     d, k, keep = p.to_c()
     info = OracleInfo()
     hbuf = None
@@ -90,13 +103,17 @@ def proj_cone(z: np.ndarray, f: int, l: int, q) -> np.ndarray:
     return x
 
 
-def setup(p: Problem):
+def setup(p: Problem, equil: str = "reference"):
     d, k, keep = p.to_c()
     D = np.zeros(p.m)
     E = np.zeros(p.n)
     lam = C.c_double()
-    lib().oracle_setup(C.byref(d), C.byref(k), D.ctypes.data_as(C.POINTER(C.c_double)),
-                       E.ctypes.data_as(C.POINTER(C.c_double)), C.byref(lam))
+    lib().oracle_set_equil(EQUIL[equil])
+    try:
+        lib().oracle_setup(C.byref(d), C.byref(k), D.ctypes.data_as(C.POINTER(C.c_double)),
+                           E.ctypes.data_as(C.POINTER(C.c_double)), C.byref(lam))
+    finally:
+        lib().oracle_set_equil(0)
     return D, E, lam.value
 
 

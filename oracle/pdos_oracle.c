@@ -15,6 +15,9 @@
  *
  * Map to the reference (file:line):
  *   o_setup            common.h:35-218        workspace, NORMALIZE, lambda, x1e3, transpose
+ *                      ruiz_equil_common.h:98-322 instead, after oracle_set_equil(1)
+ *   o_ruiz_collapse    ruiz_equil_common.h:17-81   cone-collapsed |A|, |b|
+ *   o_ruiz             ruiz_equil_common.h:130-269 Ruiz-type scaling of [A b; c' 0]
  *   o_transpose        cs.c:146-168 (+83-97)  stable counting-sort transpose
  *   o_rowsum           linAlg.h:70-115        sequential row sums in CSC/CSR order
  *   o_pri_resid        linAlg.h:158-176
@@ -80,6 +83,13 @@ typedef struct {
   long N;
   long *Lp, *Li; double *Lx, *Ld; long *P; double *bp;
 } OWork;
+
+/* Scaling used by o_setup: 0 = common.h (the reference build), 1 = the
+ * reference's alternative header ruiz_equil_common.h, which replaces
+ * common.h when compiled in its place. */
+static int o_equil = 0;
+
+void oracle_set_equil(int e) { o_equil = e; }
 
 static double o_now(void) {
   struct timespec t;
@@ -152,6 +162,124 @@ static void o_spmv(long nvec, const long *ptr, const long *idx, const double *va
 #define O_ACCUM_AT(w, xin, yout, sg, ac) \
   o_spmv((w)->n, (w)->Ap, (w)->Ai, (w)->Ax, (xin), (yout), (sg), (ac))
 
+/* ruiz_equil_common.h:17-81: in a copy of A's values (ax) and row indices
+ * (ai) and of b, every entry of a column that falls in a second-order cone
+ * becomes the largest |value| of that column's entries in the cone, with the
+ * cone's first row as its row; b's cone entries become the cone's largest
+ * |b|.  As in the header, a column's scan for the entries of a cone is bounded
+ * by nnz rather than by the column's end, so when the next column begins
+ * inside the same cone its leading entries are rewritten too (and its own
+ * scan then sees those rewritten values).  A column whose rows are not sorted
+ * stops at the last cone (the header reads past q[] there). */
+static void o_ruiz_collapse(const Data *d, const Cone *k, double *ax, long *ai, double *b) {
+  const long n = d->n, nz = d->Ap[n], top = k->f + k->l;
+  for (long col = 0; col < n; ++col) {
+    long r0 = top, p = d->Ap[col], cone = 0;
+    while (p < d->Ap[col + 1] && d->Ai[p] < r0) ++p;
+    while (p < d->Ap[col + 1] && cone < k->qsize) {
+      const long r1 = r0 + k->q[cone];
+      double mx = 0.0;
+#ifdef ORACLE_MUTANT_RUIZ_SCAN
+      const long lim = d->Ap[col + 1];  /* planted: the scan stops at the column's end */
+#else
+      const long lim = nz;
+#endif
+      for (long t = p; t < lim && d->Ai[t] >= r0 && d->Ai[t] < r1; ++t) {
+        const double v = fabs(ax[t]);
+        mx = mx > v ? mx : v;
+      }
+      for (; p < lim && d->Ai[p] >= r0 && d->Ai[p] < r1; ++p) {
+        ax[p] = mx;
+        ai[p] = r0;
+      }
+      r0 = r1;
+      ++cone;
+    }
+  }
+  for (long cone = 0, r0 = top; cone < k->qsize; r0 += k->q[cone], ++cone) {
+    double mx = 0.0;
+    for (long i = 0; i < k->q[cone]; ++i) {
+      const double v = fabs(b[r0 + i]);
+      mx = mx > v ? mx : v;
+    }
+    for (long i = 0; i < k->q[cone]; ++i) b[r0 + i] = mx;
+  }
+}
+
+/* ruiz_equil_common.h:130-269: three sweeps over [A b; c' 0] (A, b
+ * collapsed as above).  Each sweep takes the row maxima pi_i = max(max_j
+ * |A_ij| E_j, |b_i| e) with the last row's pi = max_j |c_j| E_j, gives every
+ * row of a cone its first row's pi, and sets D_i = sqrt(D_i / pi_i) (1 where
+ * pi_i <= 1e-6); then the column maxima delta_j = max(max_i |A_ij| D_i, |c_j| d)
+ * with the last column's delta = max_i |b_i| D_i, and E_j = sqrt(E_j /
+ * delta_j) (1 where delta_j <= 1e-6); d and e are the last row's D and last
+ * column's E.  Then A, b, c and the warm start are scaled with D and E. */
+static void o_ruiz(const Data *d, const Cone *k, OWork *w) {
+  const long n = d->n, m = d->m, nz = d->Ap[n];
+  double *ax = (double *)malloc(sizeof(double) * (size_t)(nz > 0 ? nz : 1));
+  long *ai = (long *)malloc(sizeof(long) * (size_t)(nz > 0 ? nz : 1));
+  double *bc = (double *)malloc(sizeof(double) * (size_t)m);
+  double *pi = (double *)calloc((size_t)m, sizeof(double));
+  double *delta = (double *)calloc((size_t)n, sizeof(double));
+  memcpy(ax, d->Ax, sizeof(double) * (size_t)nz);
+  memcpy(ai, d->Ai, sizeof(long) * (size_t)nz);
+  memcpy(bc, d->b, sizeof(double) * (size_t)m);
+  if (k->qsize > 0) o_ruiz_collapse(d, k, ax, ai, bc);
+  for (long i = 0; i < m; ++i) w->D[i] = 1.0;
+  for (long j = 0; j < n; ++j) w->E[j] = 1.0;
+  double dl = 1.0, el = 1.0, pil = 0.0, dlt = 0.0;
+  for (int sweep = 0; sweep < 3; ++sweep) {
+    for (long j = 0; j < n; ++j) {
+      for (long p = d->Ap[j]; p < d->Ap[j + 1]; ++p) {
+        const double v = fabs(ax[p]) * w->E[j];
+        pi[ai[p]] = pi[ai[p]] > v ? pi[ai[p]] : v;
+      }
+      const double v = fabs(d->c[j]) * w->E[j];
+      pil = pil > v ? pil : v;
+    }
+    for (long i = 0; i < m; ++i) {
+      const double v = fabs(bc[i]) * el;
+      pi[i] = pi[i] > v ? pi[i] : v;
+    }
+#ifndef ORACLE_MUTANT_RUIZ_CONE  /* planted: cone rows keep their own maxima */
+    for (long cone = 0, r0 = k->f + k->l; cone < k->qsize; r0 += k->q[cone], ++cone)
+      for (long i = 0; i < k->q[cone]; ++i) pi[r0 + i] = pi[r0];
+#endif
+    for (long i = 0; i < m; ++i) {
+      w->D[i] = fabs(pi[i]) > 1e-6 ? sqrt(w->D[i] / pi[i]) : 1.0;
+      pi[i] = 0.0;
+    }
+    dl = fabs(pil) > 1e-6 ? sqrt(dl / pil) : 1.0;
+    pil = 0.0;
+    for (long j = 0; j < n; ++j) {
+      for (long p = d->Ap[j]; p < d->Ap[j + 1]; ++p) {
+        const double v = fabs(ax[p]) * w->D[ai[p]];
+        delta[j] = delta[j] > v ? delta[j] : v;
+      }
+      const double v = fabs(d->c[j]) * dl;
+      delta[j] = delta[j] > v ? delta[j] : v;
+    }
+    for (long i = 0; i < m; ++i) {
+      const double v = fabs(bc[i]) * w->D[i];
+      dlt = dlt > v ? dlt : v;
+    }
+    for (long j = 0; j < n; ++j) {
+      w->E[j] = fabs(delta[j]) > 1e-6 ? sqrt(w->E[j] / delta[j]) : 1.0;
+      delta[j] = 0.0;
+    }
+    el = fabs(dlt) > 1e-6 ? sqrt(el / dlt) : 1.0;
+    dlt = 0.0;
+  }
+  for (long j = 0; j < n; ++j)
+    for (long p = d->Ap[j]; p < d->Ap[j + 1]; ++p) w->Ax[p] = w->D[d->Ai[p]] * d->Ax[p] * w->E[j];
+  for (long j = 0; j < n; ++j) w->c[j] = d->c[j] * w->E[j];
+  for (long i = 0; i < m; ++i) w->b[i] = d->b[i] * w->D[i];
+  for (long i = 0; i < m; ++i) w->y[i] /= w->D[i];
+  for (long j = 0; j < n; ++j) w->x[j] /= w->E[j];
+  for (long i = 0; i < m; ++i) w->s[i] *= w->D[i];
+  free(ax); free(ai); free(bc); free(pi); free(delta);
+}
+
 /* Workspace setup, common.h:35-218.  The caller's arrays are copied, never
  * written (the reference scales d->c / d->Ax in place when NORMALIZE == 0;
  * the values it computes with are identical to these copies). */
@@ -176,7 +304,9 @@ static OWork *o_setup(const Data *d, const Cone *k) {
   memcpy(w->b, d->b, sizeof(double) * (size_t)m);
   memcpy(w->c, d->c, sizeof(double) * (size_t)n);
 
-  if (d->p->NORMALIZE) {
+  if (o_equil == 1 && d->p->NORMALIZE) {
+    o_ruiz(d, k, w);
+  } else if (d->p->NORMALIZE) {
     long row0 = k->f + k->l, cone;
     /* squared row norms of A (common.h:89-93) */
     for (j = 0; j < n; ++j)
@@ -216,7 +346,10 @@ static OWork *o_setup(const Data *d, const Cone *k) {
     for (i = 0; i < m; ++i) w->D[i] = 1.0;
     for (j = 0; j < n; ++j) w->E[j] = 1.0;
   }
-  w->lambda = (1e-15 + sqrt(o_normsq(w->b, m))) / (1e-15 + sqrt(o_normsq(w->c, n)));
+  {
+    const double eps = o_equil == 1 ? 1e-6 : 1e-15;  /* ruiz_equil_common.h:288, common.h:182 */
+    w->lambda = (eps + sqrt(o_normsq(w->b, m))) / (eps + sqrt(o_normsq(w->c, n)));
+  }
   for (j = 0; j < n; ++j) {
     w->E[j] *= O_SQRT_RATIO;
     w->c[j] *= O_SQRT_RATIO;

@@ -31,7 +31,7 @@ ENGINE = {"auto": 0, "wg": 1, "grid": 2}
 class PdosOptions(C.Structure):
     _fields_ = [("plugin", C.c_int), ("precision", C.c_int), ("engine", C.c_int),
                 ("device", C.c_int), ("block", C.c_int), ("bf16_products", C.c_int),
-                ("reserved", C.c_int * 2)]
+                ("equil", C.c_int), ("reserved", C.c_int * 1)]
 
 
 class PdosInfo(C.Structure):
@@ -56,7 +56,7 @@ EXPORTS = ["pdos", "freeData", "freeSol", "tic", "toc", "tocq", "pdos_default_op
            "pdos_mlp_predict_dev", "pdos_mlp_last_error", "pdos_family_create", "pdos_family_load",
            "pdos_family_run", "pdos_family_fetch", "pdos_family_free", "pdos_family_last_error",
            "pdos_family_timing", "pdos_family_info", "pdos_family_engines",
-           "pdos_kkt_solve_host", "pdos_last_error",
+           "pdos_kkt_solve_host", "pdos_setup_host", "pdos_last_error",
            "pdos_version"]
 
 _libs = {}
@@ -88,6 +88,9 @@ def lib(plugin: str = "indirect"):
         L.pdos_batch_fetch.restype = C.c_int
         L.pdos_batch_fetch.argtypes = [C.c_void_p, C.POINTER(C.POINTER(Sol)), C.POINTER(PdosInfo)]
         L.pdos_batch_free.argtypes = [C.c_void_p]
+        L.pdos_setup_host.restype = C.c_int
+        L.pdos_setup_host.argtypes = [C.POINTER(Data), C.POINTER(Cone), C.c_int, C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.pdos_kkt_solve_host.restype = C.c_int
         L.pdos_kkt_solve_host.argtypes = [C.POINTER(Data), C.POINTER(Cone), C.POINTER(C.c_double),
                                           C.POINTER(C.c_double), C.POINTER(C.c_long)]
@@ -122,9 +125,16 @@ def lib(plugin: str = "indirect"):
     return _libs[plugin]
 
 
-def options(plugin="indirect", precision="fp64", engine="auto", device=0, bf16_products=0) -> PdosOptions:
+EQUIL = {"reference": 0, "ruiz": 1}
+
+
+def options(plugin="indirect", precision="fp64", engine="auto", device=0, bf16_products=0,
+            equil="reference") -> PdosOptions:
+    """equil "ruiz": the scaling of the reference's alternative header
+    ruiz_equil_common.h instead of common.h's (not for families)."""
     o = PdosOptions()
     o.bf16_products = bf16_products
+    o.equil = EQUIL[equil]
     o.plugin = PLUGIN[plugin]
     o.precision = PRECISION[precision]
     o.engine = ENGINE[engine]
@@ -138,11 +148,11 @@ def _take_sol(L, sp):
     return out
 
 
-def solve_problem(p: Problem, plugin="indirect", precision="fp64", engine="auto") -> dict:
+def solve_problem(p: Problem, plugin="indirect", precision="fp64", engine="auto", equil="reference") -> dict:
     """Solves one Problem with the HIP library; returns x, s, y, status, info."""
     L = lib(plugin)
     d, k, keep = p.to_c()
-    o = options(plugin, precision, engine)
+    o = options(plugin, precision, engine, equil=equil)
     info = PdosInfo()
     sp = L.pdos_ex(C.byref(d), C.byref(k), C.byref(o), C.byref(info))
     if not sp:
@@ -152,7 +162,7 @@ def solve_problem(p: Problem, plugin="indirect", precision="fp64", engine="auto"
     return out
 
 
-def solve_batch(problems, plugin="indirect", precision="fp64") -> list:
+def solve_batch(problems, plugin="indirect", precision="fp64", equil="reference") -> list:
     """Solves independent problems in one launch (one workgroup each)."""
     L = lib(plugin)
     n = len(problems)
@@ -161,7 +171,7 @@ def solve_batch(problems, plugin="indirect", precision="fp64") -> list:
     KA = (C.POINTER(Cone) * n)(*[C.pointer(kk[1]) for kk in keeps])
     sols = (C.POINTER(Sol) * n)()
     infos = (PdosInfo * n)()
-    o = options(plugin, precision, "wg")
+    o = options(plugin, precision, "wg", equil=equil)
     rc = L.pdos_batch(DA, KA, n, C.byref(o), sols, infos)
     if rc != 0:
         raise RuntimeError(f"pdos_batch failed: {L.pdos_last_error().decode()}")
@@ -187,6 +197,18 @@ def kkt_solve_host(p: Problem, rhs):
     if rc != 0:
         raise RuntimeError(L.pdos_last_error().decode())
     return z, st
+
+
+def setup_host(p: Problem, equil="reference", plugin="indirect"):
+    """The library's scaling of p on the host: (D, E with the x1e3, lambda)."""
+    L = lib(plugin)
+    d, k, keep = p.to_c()
+    D, E, lam = np.zeros(p.m), np.zeros(p.n), C.c_double()
+    rc = L.pdos_setup_host(C.byref(d), C.byref(k), EQUIL[equil], D.ctypes.data_as(C.POINTER(C.c_double)),
+                           E.ctypes.data_as(C.POINTER(C.c_double)), C.byref(lam))
+    if rc != 0:
+        raise RuntimeError(L.pdos_last_error().decode())
+    return D, E, lam.value
 
 
 class BatchHandle:
@@ -280,8 +302,9 @@ class DeviceArray:
 
 
 def solve(c, A, b, dims, opts=None, x0=None, y0=None, s0=None, plugin="indirect",
-          precision="fp64"):
-    """The reference binding's solve() (python/pdosmodule.c:27-344)."""
+          precision="fp64", equil="reference"):
+    """The reference binding's solve() (python/pdosmodule.c:27-344); equil
+    "ruiz" selects the scaling of the reference's ruiz_equil_common.h."""
     import scipy.sparse as sp
     A = sp.csc_matrix(A)
     m, n = A.shape
@@ -296,4 +319,4 @@ def solve(c, A, b, dims, opts=None, x0=None, y0=None, s0=None, plugin="indirect"
     p = Problem(n, m, A.indptr, A.indices, A.data, np.asarray(b, float).ravel(),
                 np.asarray(c, float).ravel(), int(dims.get("f", 0)), int(dims.get("l", 0)), q,
                 params, x0, y0, s0)
-    return solve_problem(p, plugin, precision)
+    return solve_problem(p, plugin, precision, equil=equil)

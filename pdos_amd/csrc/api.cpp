@@ -517,6 +517,11 @@ int env_precision() {
   return PDOS_FP64;
 }
 
+int env_equil() {
+  const char* p = std::getenv("PDOS_EQUIL");
+  return p && !std::strcmp(p, "ruiz") ? PDOS_EQUIL_RUIZ : PDOS_EQUIL_REFERENCE;
+}
+
 void print_verbose(const Params* p, const DevResult& r, double lam, double secs) {
   if (!p->VERBOSE) return;
   // Banner and summary row in the format of pdos.c:79-92,306-335.
@@ -570,7 +575,7 @@ Sol* pdos_ex(const Data* d, const Cone* k, const PdosOptions* opt, PdosInfo* inf
     g_err = err;
     return nullptr;
   }
-  if (!build_setup(d, k, S[0], err)) {
+  if (!build_setup(d, k, S[0], err, o.equil)) {
     g_err = err;
     return nullptr;
   }
@@ -627,6 +632,7 @@ Sol* pdos(const Data* d, const Cone* k) {
   PdosOptions o;
   pdos_default_options(&o);
   o.precision = env_precision();
+  o.equil = env_equil();
   return pdos_ex(d, k, &o, nullptr);
 }
 
@@ -655,7 +661,7 @@ static PdosBatchHandle* batch_prepare(const Data* const* ds, const Cone* const* 
   h->B.direct = direct;
   std::string err;
   for (int i = 0; i < count; ++i) {
-    if (!ds[i] || !ks[i] || !build_setup(ds[i], ks[i], h->B.S[i], err)) {
+    if (!ds[i] || !ks[i] || !build_setup(ds[i], ks[i], h->B.S[i], err, o.equil)) {
       g_err = "instance " + std::to_string(i) + ": " + (err.empty() ? "NULL input" : err);
       delete h;
       return nullptr;
@@ -770,6 +776,20 @@ int pdos_batch(const Data* const* ds, const Cone* const* ks, int count, const Pd
   int rc = h->B.run(err) ? batch_fetch(h, sols, infos) : (g_err = err, -1);
   delete h;
   return rc;
+}
+
+int pdos_setup_host(const Data* d, const Cone* k, int equil, double* D, double* E, double* lambda) {
+  g_err.clear();
+  Setup S;
+  std::string err;
+  if (!build_setup(d, k, S, err, equil)) {
+    g_err = err;
+    return -1;
+  }
+  if (D) std::copy(S.D.begin(), S.D.end(), D);
+  if (E) std::copy(S.E.begin(), S.E.end(), E);
+  if (lambda) *lambda = S.lambda;
+  return 0;
 }
 
 int pdos_kkt_solve_host(const Data* d, const Cone* k, const double* rhs, double* z, long* stats) {

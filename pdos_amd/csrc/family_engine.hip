@@ -1420,6 +1420,10 @@ bool Family::create(const Data* d, const Cone* k, const PdosOptions& o, std::str
     use_graph = g && g[0] == '1';
   }
   device = o.device;
+  if (o.equil != PDOS_EQUIL_REFERENCE) {  // Ruiz scaling depends on each instance's b and c
+    err = "the family engine shares one scaling of A across instances: PDOS_EQUIL_REFERENCE only";
+    return false;
+  }
   if (!build_setup(d, k, S, err)) return false;
   par = *d->p;
   n = S.n;

@@ -29,8 +29,9 @@ struct Setup {
 
 // Fills `S` from the caller's data without writing to it.  Returns false and
 // sets `err` on invalid input (NULL arrays, cone sizes that do not sum to m,
-// indices out of range, sizes beyond int32).
-bool build_setup(const Data* d, const Cone* k, Setup& S, std::string& err);
+// indices out of range, sizes beyond int32).  `equil` selects the scaling:
+// 0 common.h:35-218, 1 ruiz_equil_common.h:98-322.
+bool build_setup(const Data* d, const Cone* k, Setup& S, std::string& err, int equil = 0);
 
 // LDL' factorization of the permuted quasi-definite KKT [[-I, A'], [A, I]].
 struct Factor {

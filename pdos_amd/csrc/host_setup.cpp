@@ -29,9 +29,76 @@ double clamp_rsqrt(double v) {
   v = v > kRangeMax ? kRangeMax : v;
   return 1.0 / std::sqrt(v);
 }
+
+inline double vmax(double a, double b) { return a > b ? a : b; }
+
+// Ruiz-type scaling of ruiz_equil_common.h:130-269 into S.D, S.E (before the
+// x1e3), on a working copy of |A| and b in which every entry of a column
+// inside a second-order cone takes the column's largest |value| in that cone
+// at the cone's first row, and b's cone entries the cone's largest |b|
+// (ruiz_equil_common.h:17-81; as there, the scan of a column's entries in a
+// cone is bounded by nnz, not by the column, so the next column's leading
+// entries in the same cone are rewritten too).  Three sweeps: row maxima pi
+// (the last row: c), cone rows share their first row's pi, D = sqrt(D / pi);
+// column maxima delta (the last column: b), E = sqrt(E / delta); a maximum
+// <= 1e-6 resets the scale to 1.
+void ruiz_scale(const Data* d, const Cone* k, Setup& S) {
+  const long n = d->n, m = d->m, nz = d->Ap[n], top = k->f + k->l;
+  std::vector<double> ax(d->Ax, d->Ax + nz), bc(d->b, d->b + m);
+  std::vector<long> ai(d->Ai, d->Ai + nz);
+  for (long col = 0; col < n && k->qsize > 0; ++col) {
+    long r0 = top, p = d->Ap[col], cone = 0;
+    while (p < d->Ap[col + 1] && d->Ai[p] < r0) ++p;
+    while (p < d->Ap[col + 1] && cone < k->qsize) {  // rows out of order: stop at the last cone
+      const long r1 = r0 + k->q[cone];
+      double mx = 0.0;
+      for (long t = p; t < nz && d->Ai[t] >= r0 && d->Ai[t] < r1; ++t) mx = vmax(mx, std::fabs(ax[t]));
+      for (; p < nz && d->Ai[p] >= r0 && d->Ai[p] < r1; ++p) {
+        ax[p] = mx;
+        ai[p] = r0;
+      }
+      r0 = r1;
+      ++cone;
+    }
+  }
+  for (int c = 0; c < S.nsoc(); ++c) {
+    double mx = 0.0;
+    for (int i = S.soc_ptr[c]; i < S.soc_ptr[c + 1]; ++i) mx = vmax(mx, std::fabs(bc[i]));
+    for (int i = S.soc_ptr[c]; i < S.soc_ptr[c + 1]; ++i) bc[i] = mx;
+  }
+  std::vector<double> pi(m, 0.0), delta(n, 0.0);
+  std::fill(S.D.begin(), S.D.end(), 1.0);
+  std::fill(S.E.begin(), S.E.end(), 1.0);
+  double dl = 1.0, el = 1.0;
+  for (int sweep = 0; sweep < 3; ++sweep) {
+    double pil = 0.0, dlt = 0.0;
+    for (long j = 0; j < n; ++j) {
+      for (long p = d->Ap[j]; p < d->Ap[j + 1]; ++p) pi[ai[p]] = vmax(pi[ai[p]], std::fabs(ax[p]) * S.E[j]);
+      pil = vmax(pil, std::fabs(d->c[j]) * S.E[j]);
+    }
+    for (long i = 0; i < m; ++i) pi[i] = vmax(pi[i], std::fabs(bc[i]) * el);
+    for (int c = 0; c < S.nsoc(); ++c)
+      for (int i = S.soc_ptr[c]; i < S.soc_ptr[c + 1]; ++i) pi[i] = pi[S.soc_ptr[c]];
+    for (long i = 0; i < m; ++i) {
+      S.D[i] = std::fabs(pi[i]) > 1e-6 ? std::sqrt(S.D[i] / pi[i]) : 1.0;
+      pi[i] = 0.0;
+    }
+    dl = std::fabs(pil) > 1e-6 ? std::sqrt(dl / pil) : 1.0;
+    for (long j = 0; j < n; ++j) {
+      for (long p = d->Ap[j]; p < d->Ap[j + 1]; ++p) delta[j] = vmax(delta[j], std::fabs(ax[p]) * S.D[ai[p]]);
+      delta[j] = vmax(delta[j], std::fabs(d->c[j]) * dl);
+    }
+    for (long i = 0; i < m; ++i) dlt = vmax(dlt, std::fabs(bc[i]) * S.D[i]);
+    for (long j = 0; j < n; ++j) {
+      S.E[j] = std::fabs(delta[j]) > 1e-6 ? std::sqrt(S.E[j] / delta[j]) : 1.0;
+      delta[j] = 0.0;
+    }
+    el = std::fabs(dlt) > 1e-6 ? std::sqrt(el / dlt) : 1.0;
+  }
+}
 }  // namespace
 
-bool build_setup(const Data* d, const Cone* k, Setup& S, std::string& err) {
+bool build_setup(const Data* d, const Cone* k, Setup& S, std::string& err, int equil) {
   if (!d || !k) { err = "NULL data or cone"; return false; }
   if (!d->p) { err = "NULL params"; return false; }
   if (d->n < 1 || d->m < 1) { err = "n and m must be positive"; return false; }
@@ -78,7 +145,17 @@ bool build_setup(const Data* d, const Cone* k, Setup& S, std::string& err) {
   S.pscale = 1.0 + norm2(d->b, m);
   S.dscale = 1.0 + norm2(d->c, n);
 
-  if (d->p->NORMALIZE) {
+  if (equil != 0 && equil != 1) { err = "unknown scaling (equil)"; return false; }
+  if (d->p->NORMALIZE && equil == 1) {
+    ruiz_scale(d, k, S);
+    for (long j = 0; j < n; ++j)
+      for (long p = d->Ap[j]; p < d->Ap[j + 1]; ++p) S.cx[p] = S.D[d->Ai[p]] * d->Ax[p] * S.E[j];
+    for (long j = 0; j < n; ++j) S.c[j] = d->c[j] * S.E[j];
+    for (long i = 0; i < m; ++i) S.b[i] = d->b[i] * S.D[i];
+    for (long i = 0; i < m; ++i) S.y[i] /= S.D[i];
+    for (long j = 0; j < n; ++j) S.x[j] /= S.E[j];
+    for (long i = 0; i < m; ++i) S.s[i] *= S.D[i];
+  } else if (d->p->NORMALIZE) {
     for (long j = 0; j < n; ++j)
       for (long p = d->Ap[j]; p < d->Ap[j + 1]; ++p) S.D[d->Ai[p]] += d->Ax[p] * d->Ax[p];
     for (int c = 0; c < S.nsoc(); ++c) {
@@ -105,7 +182,8 @@ bool build_setup(const Data* d, const Cone* k, Setup& S, std::string& err) {
     std::fill(S.E.begin(), S.E.end(), 1.0);
   }
   S.E0 = S.E;
-  S.lambda = (1e-15 + norm2(S.b.data(), m)) / (1e-15 + norm2(S.c.data(), n));
+  const double leps = equil == 1 ? 1e-6 : 1e-15;  // ruiz_equil_common.h:288, common.h:182
+  S.lambda = (leps + norm2(S.b.data(), m)) / (leps + norm2(S.c.data(), n));
   for (long j = 0; j < n; ++j) {
     S.E[j] *= kSqrtRatio;
     S.c[j] *= kSqrtRatio;

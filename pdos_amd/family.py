@@ -38,12 +38,14 @@ class Family:
     """Instances sharing the template's A, cones and parameters; each has its
     own c (and optionally b)."""
 
-    def __init__(self, template: Problem, precision: str = "fp64", device: int = 0, bf16_products: int = 0):
+    def __init__(self, template: Problem, precision: str = "fp64", device: int = 0, bf16_products: int = 0,
+                 equil: str = "reference"):
         self.L = lib("direct")
         _bind(self.L)
         self.n, self.m = template.n, template.m
+        self.h = None
         d, k, self._keep = template.to_c()
-        o = options("direct", precision, "auto", device, bf16_products)
+        o = options("direct", precision, "auto", device, bf16_products, equil)
         self.h = self.L.pdos_family_create(C.byref(d), C.byref(k), C.byref(o))
         if not self.h:
             raise RuntimeError(self.L.pdos_family_last_error().decode())

@@ -55,8 +55,10 @@ class PrivDirect(C.Structure):  # direct/private.h:8-12
     _fields_ = [("L", C.c_void_p), ("D", C.POINTER(C.c_double)), ("P", C.POINTER(C.c_long))]
 
 
-def load_ref(plugin):
-    L = C.CDLL(os.path.join(REF_LIBS, f"libpdos{'dir' if plugin == 'direct' else 'indir'}_ref.so"))
+def load_ref(plugin, variant=""):
+    """The compiled reference; variant "_ruiz" is the build with
+    ruiz_equil_common.h in place of common.h (oracle/Makefile)."""
+    L = C.CDLL(os.path.join(REF_LIBS, f"libpdos{'dir' if plugin == 'direct' else 'indir'}{variant}_ref.so"))
     L.pdos.restype = C.POINTER(Sol)
     L.pdos.argtypes = [C.POINTER(Data), C.POINTER(Cone)]
     L.initWork.restype = C.POINTER(Work)
@@ -106,14 +108,12 @@ def ref_setup(lib, prob, direct):
     return D, E, w.lam, P
 
 
-def main():
-    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
-    libs = {pl: load_ref(pl) for pl in ("indirect", "direct")}
+def golden_cases(arrays):
+    """The parity cases as (tag, input path, problem): the reference's data
+    files (copied under data/) and generated problems (written to inputs/ in
+    the demo format; warm starts, which the format lacks, go into `arrays`)."""
     os.makedirs(os.path.join(HERE, "inputs"), exist_ok=True)
     os.makedirs(os.path.join(HERE, "data"), exist_ok=True)
-    arrays = {}
-    manifest = {"cases": {}, "projcone": {}}
-
     cases = []
     for i in range(1, 11):
         name = f"portfolio_test_{i:02d}_10x100"
@@ -157,6 +157,15 @@ def main():
     fam = problems.PortfolioFamily(seed=4, na=60, k=6, EPS_ABS=1e-3)
     _, Cf = fam.sample(6, 2)
     add_generated("family_na60", fam.instance(Cf[0]))
+    return cases
+
+
+def main():
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
+    libs = {pl: load_ref(pl) for pl in ("indirect", "direct")}
+    arrays = {}
+    manifest = {"cases": {}, "projcone": {}}
+    cases = golden_cases(arrays)
 
     for tag, path, prob in cases:
         entry = {"input": path, "n": prob.n, "m": prob.m, "nnz": prob.nnz}

@@ -46,6 +46,43 @@ def test_grid_fp64_vs_reference(tag):
     _check_fp64(out, G.expected(tag, "indirect"), f"{tag}/grid")
 
 
+@pytest.mark.parametrize("tag", G.case_names("ruiz"))
+@pytest.mark.parametrize("plugin", ["indirect", "direct"])
+def test_wg_fp64_ruiz_vs_reference(tag, plugin):
+    """The Ruiz scaling option against the reference built with its
+    ruiz_equil_common.h in place of common.h."""
+    print(f"case {tag} {plugin} ruiz", flush=True)
+    out = pdos_amd.solve_problem(G.problem(tag, "ruiz"), plugin=plugin, precision="fp64", engine="wg",
+                                 equil="ruiz")
+    _check_fp64(out, G.expected(tag, plugin, "ruiz"), f"{tag}/{plugin}/ruiz")
+
+
+@pytest.mark.parametrize("tag", ["random_n300_m900_norm1", "portfolio_n101_norm1"])
+def test_grid_fp64_ruiz_vs_reference(tag):
+    print(f"case {tag} grid ruiz", flush=True)
+    out = pdos_amd.solve_problem(G.problem(tag, "ruiz"), plugin="indirect", precision="fp64", engine="grid",
+                                 equil="ruiz")
+    _check_fp64(out, G.expected(tag, "indirect", "ruiz"), f"{tag}/grid/ruiz")
+
+
+def test_pdos_abi_takes_ruiz_from_environment(monkeypatch):
+    """pdos() (the reference's own entry point) reads PDOS_EQUIL at each call."""
+    tag = "portfolio_n101_norm1"
+    prob = G.problem(tag, "ruiz")
+    L = pdos_amd.lib("indirect")
+    print(f"case {tag} pdos() ruiz", flush=True)
+    for env, variant in (("ruiz", "ruiz"), ("", "")):
+        monkeypatch.setenv("PDOS_EQUIL", env)
+        d, k, keep = prob.to_c()
+        sp = L.pdos(C.byref(d), C.byref(k))
+        assert sp, L.pdos_last_error().decode()
+        out = pdos_amd._take_sol(L, sp)
+        exp = G.expected(tag, "indirect", variant)
+        assert out["status"] == exp["status"]
+        for key in ("x", "s", "y"):
+            assert _rel(out[key], exp[key]) <= RTOL, (variant, key)
+
+
 def test_grid_long_rows_vs_oracle():
     """A full row (6,000 entries) and a full column (9,000 entries): both
     exceed one SpMV row block and take the whole-workgroup row path.  This

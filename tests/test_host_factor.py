@@ -29,3 +29,28 @@ def test_kkt_solve_matches_dense(name, p):
     zd = np.linalg.solve(K, rhs)
     assert np.linalg.norm(z - zd) <= 1e-9 * np.linalg.norm(zd)
     assert st[0] == N and st[2] >= 1 and st[3] >= 1 and st[1] >= 0
+
+
+@pytest.mark.parametrize("variant", ["", "ruiz"])
+def test_library_setup_bitwise_vs_reference(variant):
+    """The library's host setup (pdos_setup_host) against the reference's
+    initWork on every golden case: D, E (with the x1e3) and lambda bit for bit,
+    for the reference build and for its ruiz_equil_common.h build."""
+    equil = "ruiz" if variant else "reference"
+    for tag in G.case_names(variant):
+        prob = G.problem(tag, variant)
+        D, E, lam, _ = G.setup_expected(tag, variant)
+        d2, e2, l2 = pdos_amd.setup_host(prob, equil)
+        np.testing.assert_array_equal(d2, D, err_msg=f"{tag} D")
+        np.testing.assert_array_equal(e2, E, err_msg=f"{tag} E")
+        assert l2 == lam, tag
+
+
+def test_family_rejects_ruiz():
+    """Ruiz scaling depends on each instance's b and c; a family shares one
+    scaling of A, so the family engine refuses the option."""
+    from pdos_amd.family import Family
+    fam = problems.PortfolioFamily(seed=4, na=20, k=3)
+    _, Cm = fam.sample(seed=1, count=2)
+    with pytest.raises(RuntimeError, match="PDOS_EQUIL_REFERENCE"):
+        Family(fam.instance(Cm[0]), precision="mixed", equil="ruiz")

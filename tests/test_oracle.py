@@ -57,6 +57,61 @@ def test_oracle_setup_bitwise(tag):
     assert l2 == lam
 
 
+RUIZ_CASES = G.case_names("ruiz")
+
+
+@pytest.mark.parametrize("tag", RUIZ_CASES)
+@pytest.mark.parametrize("plugin", ["indirect", "direct"])
+def test_oracle_ruiz_bitwise_vs_reference(tag, plugin):
+    """The Ruiz scaling option against the reference built with its
+    ruiz_equil_common.h in place of common.h."""
+    prob = G.problem(tag, "ruiz")
+    exp = G.expected(tag, plugin, "ruiz")
+    perm = G.setup_expected(tag, "ruiz")[3] if plugin == "direct" else None
+    out = oracle.solve(prob, plugin, perm=perm, equil="ruiz")
+    assert out["status"] == exp["status"]
+    assert out["iters"] == exp["iters"]
+    for k in ("x", "s", "y"):
+        np.testing.assert_array_equal(out[k], exp[k], err_msg=f"{tag}/{plugin}/{k}")
+    if exp["iters"] > 0:
+        _check_last_row(out, exp)
+
+
+@pytest.mark.parametrize("tag", RUIZ_CASES)
+def test_oracle_ruiz_setup_bitwise(tag):
+    prob = G.problem(tag, "ruiz")
+    D, E, lam, _ = G.setup_expected(tag, "ruiz")
+    d2, e2, l2 = oracle.setup(prob, "ruiz")
+    np.testing.assert_array_equal(d2, D)
+    np.testing.assert_array_equal(e2, E)
+    assert l2 == lam
+
+
+def test_ruiz_closed_form_on_a_diagonal():
+    """A diagonal, no cones, b = c = 0: a sweep maps D <- sqrt(D / (|a| E)) and
+    then E <- sqrt(E / (|a| D)), so from D = E = 1 three sweeps give
+    D = |a|^(-21/32), E = |a|^(-21/64) (E then x 1e3), and lambda = 1e-6 / 1e-6."""
+    from pdos_amd.abi import Problem
+    a = np.array([0.01, -0.5, 1.0, 3.0, -250.0])
+    n = len(a)
+    p = Problem(n, n, np.arange(n + 1), np.arange(n), a, np.zeros(n), np.zeros(n), 0, n, []).with_params(NORMALIZE=1)
+    D, E, lam = oracle.setup(p, "ruiz")
+    np.testing.assert_allclose(D, np.abs(a) ** (-21 / 32), rtol=1e-14)
+    np.testing.assert_allclose(E, 1e3 * np.abs(a) ** (-21 / 64), rtol=1e-14)
+    assert lam == 1.0
+
+
+@pytest.mark.parametrize("tag", [t for t in RUIZ_CASES if G.problem(t, "ruiz").q])
+def test_ruiz_cone_rows_share_one_scale(tag):
+    """The scaling must not change a cone's shape: one D for all of a cone's rows."""
+    prob = G.problem(tag, "ruiz").with_params(NORMALIZE=1)
+    D = oracle.setup(prob, "ruiz")[0]
+    r0 = prob.f + prob.l
+    for q in prob.q:
+        assert np.all(D[r0:r0 + q] == D[r0]), (tag, r0, q)
+        r0 += q
+
+
 def test_projcone_bitwise():
     a = G.arrays()
     for tag, meta in G.manifest()["projcone"].items():
@@ -216,3 +271,34 @@ def test_planted_mistake_is_caught(mutant, tmp_path):
             except AssertionError:
                 caught = True
     assert caught, f"planted mistake in {MUTANTS[mutant]} went unnoticed"
+
+
+RUIZ_MUTANTS = {
+    "RUIZ_CONE": "giving every row of a cone its first row's maximum",
+    "RUIZ_SCAN": "the cone-collapse scan that runs on into the next column",
+}
+
+
+@pytest.mark.parametrize("mutant", sorted(RUIZ_MUTANTS))
+def test_planted_ruiz_mistake_is_caught(mutant, tmp_path):
+    src = os.path.join(os.path.dirname(oracle.HERE), "oracle", "pdos_oracle.c")
+    so = tmp_path / f"mut_{mutant}.so"
+    subprocess.run(["gcc", "-O1", "-std=gnu99", "-fPIC", "-shared", "-w", f"-DORACLE_MUTANT_{mutant}",
+                    "-o", str(so), src, "-lm"], check=True)
+    import ctypes as C
+    from pdos_amd.abi import Data, Cone
+    L = C.CDLL(str(so))
+    L.oracle_setup.argtypes = [C.POINTER(Data), C.POINTER(Cone), C.POINTER(C.c_double),
+                               C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.oracle_set_equil(1)
+    caught = []
+    for tag in RUIZ_CASES:
+        prob = G.problem(tag, "ruiz")
+        D, E, lam, _ = G.setup_expected(tag, "ruiz")
+        d, k, keep = prob.to_c()
+        d2, e2, l2 = np.zeros(prob.m), np.zeros(prob.n), C.c_double()
+        L.oracle_setup(C.byref(d), C.byref(k), d2.ctypes.data_as(C.POINTER(C.c_double)),
+                       e2.ctypes.data_as(C.POINTER(C.c_double)), C.byref(l2))
+        if not (np.array_equal(d2, D) and np.array_equal(e2, E) and l2.value == lam):
+            caught.append(tag)
+    assert caught, f"planted mistake in {RUIZ_MUTANTS[mutant]} went unnoticed"
