@@ -432,7 +432,8 @@ def run_wg(args, probs, config, plugin):
     prec = args.precision or "fp64"
     t0 = time.perf_counter()
     h = pdos_amd.BatchHandle(probs, plugin=plugin, precision=prec)
-    log(f"prepare (host setup + factor + upload): {time.perf_counter() - t0:.2f} s")
+    prep = time.perf_counter() - t0
+    log(f"prepare (host setup + factor + upload): {prep:.2f} s")
     for _ in range(args.warmup):
         h.run()
     kms = []
@@ -447,7 +448,8 @@ def run_wg(args, probs, config, plugin):
     roof = wg_roofline(probs, outs, plugin, 8 if prec == "fp64" else 4, float(np.mean(kms)))
     acc = {"solved": int(solved), "count": len(outs), "verified_fp64": verify_list(probs, outs),
            "mean_iters": float(iters.mean()), "max_iters": int(iters.max())}
-    extra = {"plain_gpu": plain_gpu(probs[:args.plain_sample], plugin, prec) if args.plain_sample else None}
+    extra = {"plain_gpu": plain_gpu(probs[:args.plain_sample], plugin, prec) if args.plain_sample else None,
+             "prepare_s": round(prep, 3), "host_threads": int(os.environ.get("PDOS_HOST_THREADS", os.cpu_count()))}
     return solved / wall, wall, "f64" if prec == "fp64" else "f32", config, roof, acc, extra, \
         probs[:args.cpu_sample], plugin
 

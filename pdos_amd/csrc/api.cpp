@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/pdos.h"
@@ -159,6 +160,38 @@ static uint64_t fnv_vec(uint64_t h, const std::vector<V>& v) {
   return fnv(h, v.data(), v.size() * sizeof(V));
 }
 
+// Runs body(i) for i in [0, count) on up to `host_threads()` threads, each
+// taking a contiguous share; returns the lowest index whose body failed
+// (with its error) or -1.
+static int host_threads() {
+  const char* e = std::getenv("PDOS_HOST_THREADS");
+  int t = e ? std::atoi(e) : (int)std::thread::hardware_concurrency();
+  return t < 1 ? 1 : (t > 64 ? 64 : t);
+}
+
+template <typename Body>
+static int parallel_for(int count, Body body, std::string& err) {
+  const int nt = std::min(host_threads(), std::max(1, count / 4));
+  std::vector<int> bad(nt, -1);
+  std::vector<std::string> msg(nt);
+  auto run = [&](int t) {
+    const int i0 = (int)((long)count * t / nt), i1 = (int)((long)count * (t + 1) / nt);
+    for (int i = i0; i < i1; ++i)
+      if (!body(i, msg[t])) { bad[t] = i; return; }
+  };
+  if (nt == 1) {
+    run(0);
+  } else {
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; ++t) th.emplace_back(run, t);
+    run(0);
+    for (auto& x : th) x.join();
+  }
+  for (int t = 0; t < nt; ++t)
+    if (bad[t] >= 0) { err = msg[t]; return bad[t]; }
+  return -1;
+}
+
 static bool same_matrix(const Setup& a, const Setup& b) {
   return a.n == b.n && a.m == b.m && a.f == b.f && a.l == b.l && a.soc_ptr == b.soc_ptr &&
          a.cp == b.cp && a.ci == b.ci && a.cx == b.cx && a.D == b.D && a.E == b.E;
@@ -194,10 +227,11 @@ bool WgBatch::group_and_factor(std::string& err) {
     std::vector<int>().swap(s.cp); std::vector<int>().swap(s.ci); std::vector<double>().swap(s.cx);
     std::vector<int>().swap(s.rp); std::vector<int>().swap(s.ri); std::vector<double>().swap(s.rx);
   }
-  if (direct) {
+  if (direct) {  // one factorization per distinct matrix, on the host threads
     F.assign(rep.size(), Factor());
-    for (size_t g = 0; g < rep.size(); ++g)
-      if (!build_factor(S[rep[g]], {}, F[g], err)) return false;
+    if (parallel_for((int)rep.size(), [&](int g, std::string& e) { return build_factor(S[rep[g]], {}, F[g], e); },
+                     err) >= 0)
+      return false;
   }
   return true;
 }
@@ -660,13 +694,15 @@ static PdosBatchHandle* batch_prepare(const Data* const* ds, const Cone* const* 
   h->B.par.resize(count);
   h->B.direct = direct;
   std::string err;
-  for (int i = 0; i < count; ++i) {
-    if (!ds[i] || !ks[i] || !build_setup(ds[i], ks[i], h->B.S[i], err, o.equil)) {
-      g_err = "instance " + std::to_string(i) + ": " + (err.empty() ? "NULL input" : err);
-      delete h;
-      return nullptr;
-    }
+  const int bad = parallel_for(count, [&](int i, std::string& e) {
+    if (!ds[i] || !ks[i] || !build_setup(ds[i], ks[i], h->B.S[i], e, o.equil)) return false;
     h->B.par[i] = *ds[i]->p;
+    return true;
+  }, err);
+  if (bad >= 0) {
+    g_err = "instance " + std::to_string(bad) + ": " + (err.empty() ? "NULL input" : err);
+    delete h;
+    return nullptr;
   }
   if (!h->B.group_and_factor(err)) {
     g_err = err;
