@@ -700,6 +700,76 @@ __global__ __launch_bounds__(FT) void k_fam_ax_decide(FamArgs<T> a, SpmmIn<T, 1>
   });
 }
 
+// Mixed mode, A with dense rows: phase C forms AX_j for a wave's rows j =
+// it.x + w + NWV k of c1 item `item` from the rows' records (rec8): the
+// x-update's planes for a dense row, the XN value times the entry for a row
+// with one light entry, else the light CSR.  ax_issue puts the loads of all
+// rows in flight (the rows' other operands are issued before it);
+// ax_finish sums them in a fixed order.
+template <typename T>
+struct AxLoads {
+  int2 rc[RW];
+  T av[RW];
+  T pv[kDenseK][kPlanes];  // planes of the dense rows among the wave's first kDenseK rows
+};
+
+template <typename T>
+__device__ __forceinline__ void ax_issue(const FamArgs<T>& a, int item, int w, int4 it, int inst, AxLoads<T>& L) {
+#pragma unroll
+  for (int k = 0; k < RW; ++k) {
+    const int j = it.x + w + k * NWV;
+    if (j >= it.y) continue;
+    L.rc[k] = ld_uniform(a.rec8 + ((size_t)item * NWV + w) * RW + k);
+    L.av[k] = L.rc[k].x >= 0 ? a.XN[(size_t)L.rc[k].x * a.ldb + inst] : T(0);
+    if (k < kDenseK && L.rc[k].x == -1) {
+#pragma unroll
+      for (int u = 0; u < kPlanes; ++u)
+        if (u < a.mt) L.pv[k < kDenseK ? k : 0][u] = T(a.AXp[((size_t)u * 32 + L.rc[k].y) * a.ldb + inst]);
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void ax_finish(const FamArgs<T>& a, int w, int4 it, int inst, const AxLoads<T>& L,
+                                          T (&av)[RW]) {
+#pragma unroll
+  for (int k = 0; k < RW; ++k) {
+    const int j = it.x + w + k * NWV;
+    if (j >= it.y) continue;
+    T v;
+    if (L.rc[k].x >= 0) {
+      v = T(__int_as_float(L.rc[k].y)) * L.av[k];
+    } else if (L.rc[k].x == -1) {  // dense row: the x-update's planes, in order
+      v = T(0);
+      int mi = 0;
+      if (k < kDenseK) {
+#pragma unroll
+        for (int u = 0; u < kPlanes; ++u)
+          if (u < a.mt) v += L.pv[k < kDenseK ? k : 0][u];
+        mi = kPlanes;
+      }
+      for (; mi < a.mt; ++mi) v += T(a.AXp[((size_t)mi * 32 + L.rc[k].y) * a.ldb + inst]);
+    } else {  // several light entries: kPlanes gathers in flight, summed in entry order
+      v = T(0);
+      const int q1 = ld_uniform(a.lrp + j + 1);
+      for (int q0 = ld_uniform(a.lrp + j); q0 < q1; q0 += kPlanes) {
+        T g[kPlanes];
+        float e[kPlanes];
+#pragma unroll
+        for (int u = 0; u < kPlanes; ++u) {
+          const int q = min(q0 + u, q1 - 1);
+          e[u] = ld_uniform(a.lrx + q);
+          g[u] = a.XN[(size_t)ld_uniform(a.lri + q) * a.ldb + inst];
+        }
+#pragma unroll
+        for (int u = 0; u < kPlanes; ++u)
+          if (q0 + u < q1) v += T(e[u]) * g[u];
+      }
+    }
+    av[k] = v;
+  }
+}
+
 // Phase C, first pass.  Work items (one per blockIdx.y, host-built, at most
 // RPC rows each):
 //   (i0, i1, 0, -)  rows of x of finished instances: XN = X (the host swaps
@@ -711,6 +781,10 @@ __global__ __launch_bounds__(FT) void k_fam_ax_decide(FamArgs<T> a, SpmmIn<T, 1>
 //   (j0, j1, 2, g)  a chunk of second-order cone rows (inside one cone):
 //                   s~ and z = s~ - lam y stored, partial |z|^2 (without the
 //                   cone's first row) into pc[g]; the cone's first row keeps t0
+// With dense rows in A (mixed) AX is formed here and not stored (nothing
+// later reads it: compaction moves W and the stop-test partials instead),
+// and cone rows store neither s~ nor z: the second pass forms them again
+// from the same operands, which costs fewer bytes than storing and reloading.
 template <typename T>
 __global__ __launch_bounds__(FT) void k_fam_c1(FamArgs<T> a, const int4* __restrict__ items) {
   __shared__ double buf[NWV * LANES];
@@ -736,70 +810,23 @@ __global__ __launch_bounds__(FT) void k_fam_c1(FamArgs<T> a, const int4* __restr
   const T ra = T(a.relax), rb = T(1.0) - T(a.relax);
   const int cone = it.w, cone_r0 = it.z == 2 ? a.soc_ptr[cone] : 0;
   double n2 = 0.0, p2 = 0.0, by = 0.0;
+  const bool hyb = kCorr<T> && a.rec8;
   if (act) {
     T bv[RW], av[RW], sv[RW], yv[RW];
-    T pv[kDenseK][kPlanes];  // planes of A_h XN of the dense rows among a wave's first kDenseK rows
-    int2 rc[RW];
 #pragma unroll
     for (int k = 0; k < RW; ++k) {
       const int j = it.x + w + k * NWV;
       if (j < it.y) {
         const size_t o = (size_t)j * a.ldb + inst;
         bv[k] = bval(a, j, inst); sv[k] = a.S[o]; yv[k] = a.Y[o];
-        if constexpr (kCorr<T>) {
-          if (a.rec8) {  // AX_j from XN (one entry), the planes (dense row) or later (several entries)
-            rc[k] = ld_uniform(a.rec8 + ((size_t)blockIdx.y * NWV + w) * RW + k);
-            av[k] = rc[k].x >= 0 ? a.XN[(size_t)rc[k].x * a.ldb + inst] : T(0);
-            if (k < kDenseK && rc[k].x == -1) {
-#pragma unroll
-              for (int u = 0; u < kPlanes; ++u)
-                if (u < a.mt) pv[k < kDenseK ? k : 0][u] = T(a.AXp[((size_t)u * 32 + rc[k].y) * a.ldb + inst]);
-            }
-            continue;
-          }
-        }
-        av[k] = a.AX[o];
+        if (!hyb) av[k] = a.AX[o];
       }
     }
     if constexpr (kCorr<T>) {
-      if (a.rec8) {
-#pragma unroll
-        for (int k = 0; k < RW; ++k) {
-          const int j = it.x + w + k * NWV;
-          if (j >= it.y) continue;
-          T v;
-          if (rc[k].x >= 0) {
-            v = T(__int_as_float(rc[k].y)) * av[k];
-          } else if (rc[k].x == -1) {  // dense row: the x-update's planes, in order
-            v = T(0);
-            int mi = 0;
-            if (k < kDenseK) {
-#pragma unroll
-              for (int u = 0; u < kPlanes; ++u)
-                if (u < a.mt) v += pv[k < kDenseK ? k : 0][u];
-              mi = kPlanes;
-            }
-            for (; mi < a.mt; ++mi) v += T(a.AXp[((size_t)mi * 32 + rc[k].y) * a.ldb + inst]);
-          } else {  // several light entries: kPlanes gathers in flight, summed in entry order
-            v = T(0);
-            const int q1 = ld_uniform(a.lrp + j + 1);
-            for (int q0 = ld_uniform(a.lrp + j); q0 < q1; q0 += kPlanes) {
-              T g[kPlanes];
-              float e[kPlanes];
-#pragma unroll
-              for (int u = 0; u < kPlanes; ++u) {
-                const int q = min(q0 + u, q1 - 1);
-                e[u] = ld_uniform(a.lrx + q);
-                g[u] = a.XN[(size_t)ld_uniform(a.lri + q) * a.ldb + inst];
-              }
-#pragma unroll
-              for (int u = 0; u < kPlanes; ++u)
-                if (q0 + u < q1) v += T(e[u]) * g[u];
-            }
-          }
-          av[k] = v;
-          a.AX[(size_t)j * a.ldb + inst] = v;
-        }
+      if (hyb) {
+        AxLoads<T> L;
+        ax_issue(a, blockIdx.y, w, it, inst, L);
+        ax_finish(a, w, it, inst, L, av);
       }
     }
     const T lam = T(a.lam[inst]);
@@ -821,8 +848,10 @@ __global__ __launch_bounds__(FT) void k_fam_c1(FamArgs<T> a, const int4* __restr
           by += double(bv[k]) * double(y);
           a.W[o] = next_w(bv[k], av[k], s, y, lam);
         } else {
-          a.W[o] = st;  // s~ for the dual update
-          a.S[o] = z;
+          if (!hyb) {
+            a.W[o] = st;  // s~ for the dual update
+            a.S[o] = z;
+          }
           if (j > cone_r0) n2 += double(z) * double(z);
           else a.pt0[(size_t)cone * a.ldb + inst] = double(z);  // t0, read by every chunk of the cone
         }
@@ -861,6 +890,8 @@ __global__ __launch_bounds__(FT) void k_fam_c2(FamArgs<T> a, const int4* __restr
   // mixed mode without relaxation: s~ = b - AX exactly, so the residual and
   // W come from s~ and AX, b are not read again
   const bool from_st = kCorr<T> && a.relax == 1.0;
+  // A with dense rows (mixed): s~ and z are formed again as in the first pass
+  const bool hyb = kCorr<T> && a.rec8;
   double p2 = 0.0, by = 0.0;
   if (act) {
     const T lam = T(a.lam[inst]);
@@ -871,9 +902,28 @@ __global__ __launch_bounds__(FT) void k_fam_c2(FamArgs<T> a, const int4* __restr
       const int j = it.x + w + k * NWV;
       if (j < it.y) {
         const size_t o = (size_t)j * a.ldb + inst;
-        zv[k] = a.S[o]; wv[k] = a.W[o]; yv[k] = a.Y[o];
+        zv[k] = a.S[o]; yv[k] = a.Y[o];
         bv[k] = bval(a, j, inst);
-        if (!from_st) av[k] = a.AX[o];
+        if (!hyb) {
+          wv[k] = a.W[o];
+          if (!from_st) av[k] = a.AX[o];
+        }
+      }
+    }
+    if constexpr (kCorr<T>) {
+      if (hyb) {  // zv holds S of the last iteration here
+        AxLoads<T> L;
+        ax_issue(a, a.nx_items + a.nlin_items + blockIdx.y, w, it, inst, L);
+        ax_finish(a, w, it, inst, L, av);
+        const T ra = T(a.relax), rb = T(1.0) - T(a.relax);
+#pragma unroll
+        for (int k = 0; k < RW; ++k) {
+          if (it.x + w + k * NWV >= it.y) continue;
+          T st = bv[k] - av[k];
+          st = ra * st + rb * zv[k];
+          wv[k] = st;
+          zv[k] = st - lam * yv[k];
+        }
       }
     }
     // the cone's chunk partials in order, 16 loads issued at a time
@@ -1131,8 +1181,12 @@ __global__ __launch_bounds__(FT) void k_fam_scatter(FamArgs<T> a, int ldo, T* __
 
 // Gathers the columns `list` of working set s into the dense prefix of d
 // (d.count live columns, d.ldb - d.count padding columns left inactive).
+// With pa_items > 0 (A with dense rows, mixed: phase C does not keep AX) W
+// and the pa_items stop-test partials move instead of AX, so the moved state
+// is complete without a phase A pass.
 template <typename T>
-__global__ __launch_bounds__(FT) void k_fam_move(FamArgs<T> s, FamArgs<T> d, const int* __restrict__ list) {
+__global__ __launch_bounds__(FT) void k_fam_move(FamArgs<T> s, FamArgs<T> d, const int* __restrict__ list,
+                                                 int pa_items) {
   const int lane = threadIdx.x & 63, w = wave_id();
   const int col = blockIdx.x * LANES + lane;
   const bool live = col < d.count;
@@ -1144,7 +1198,16 @@ __global__ __launch_bounds__(FT) void k_fam_move(FamArgs<T> s, FamArgs<T> d, con
   mv(s.X, d.X, d.n);
   mv(s.S, d.S, d.m);
   mv(s.Y, d.Y, d.m);
-  mv(s.AX, d.AX, d.m);
+  if (pa_items > 0) {
+    mv(s.W, d.W, d.m);
+    for (int it = w; it < pa_items; it += NWV) {
+      const double2 v = live ? *reinterpret_cast<const double2*>(s.pa + ((size_t)it * s.ldb + sc) * 2)
+                             : double2{0.0, 0.0};
+      *reinterpret_cast<double2*>(d.pa + ((size_t)it * d.ldb + col) * 2) = v;
+    }
+  } else {
+    mv(s.AX, d.AX, d.m);
+  }
   mv(s.c, d.c, d.n);
   if (d.b_inst) mv(s.b, d.b, d.m);
   for (int r = d.n + w; r < d.np; r += NWV) {  // padding rows of the GEMM operands
@@ -2046,14 +2109,14 @@ bool Family::run_t(const float* warm, long wld, int judge, std::string& err) {
       (void)hipMemcpyAsync(dlist, list.data(), (size_t)active * 4, hipMemcpyHostToDevice, st);
       const int nld = pad64(active);
       FamArgs<T> b = args<T>(cur ^ 1, nld, active);
-      hipLaunchKernelGGL(k_fam_move<T>, dim3(nld / LANES), block, 0, st, a, b, dlist);
+      hipLaunchKernelGGL(k_fam_move<T>, dim3(nld / LANES), block, 0, st, a, b, dlist, hybrid() ? m_items() : 0);
       (void)hipStreamSynchronize(st);  // dlist and the flag slots are reused
       cur ^= 1;
       ld = nld;
       cnt = active;
       a = b;
       ++compactions;
-      need_prologue = true;
+      need_prologue = !hybrid();  // with dense rows W and the partials were moved
       slot = 0;
       r = launch(slot);
       if (r < 0) return false;
