@@ -200,7 +200,7 @@ def measured_traffic(profile, kind, fp64):
     return sum(data[k]["read_bytes"] + data[k]["write_bytes"] for k in names)
 
 
-def family_roofline(kt, fam_info, nnz, fp64, lin_rows, b_inst, traffic_profile=None):
+def family_roofline(kt, fam_info, nnz, fp64, lin_rows, b_inst, traffic_profile=None, dense_cone=None):
     """Roofline of the dominant kernel of a family run, from per-kernel HIP
     event totals {kind: (ms, calls, columns)}.  Algorithmic bytes per column
     (instance), values of sv bytes:
@@ -217,9 +217,10 @@ def family_roofline(kt, fam_info, nnz, fp64, lin_rows, b_inst, traffic_profile=N
                   written; pass 2 S, W, Y (+ AX, + b) read, Y, S, W written
                   (11 in mixed mode, where s~ stands in for b - AX; 12 in
                   fp64; + 2 with per-instance b).  With dense rows in A
-                  phase C forms AX: the row of XN its entry names is read in
-                  place of AX and AX is written, one value more per row (7,
-                  12), and each dense row reads the x-update's np / 64 planes
+                  phase C forms AX and does not store it: zero / nonneg rows
+                  6 values, cone rows 9 (S, Y, XN read in each pass; Y, S, W
+                  written), and each dense row reads the x-update's np / 64
+                  planes in every pass over it (dense_cone: those in cones)
     x-update GEMM: 2 n np flops per product (bf16) or 2 n^2 (fp64), with
                   dense rows in A + 2 * 32 np (their A_h XN planes)."""
     n, m, npad = fam_info["n"], fam_info["m"], fam_info["np"]
@@ -259,10 +260,19 @@ def family_roofline(kt, fam_info, nnz, fp64, lin_rows, b_inst, traffic_profile=N
         byt = cols * m * sv * (5 if b_inst else 4)
     elif kind == "phase_b":
         byt = cols * (n * sv * 4 + rhs) if dense else cols * 16 * ((n + 63) // 64 + (m + 63) // 64)
+    elif nh:
+        # dense rows in A (mixed): AX is formed from XN (or the planes) and
+        # not stored; a zero / nonneg row reads S, Y, XN and writes S, Y, W;
+        # a cone row is read twice (S, Y, XN: once for the norm, once for
+        # the projection) and written once (Y, S, W); a dense row reads the
+        # np / 64 planes in each pass that touches it
+        nh_cone = dense_cone if dense_cone is not None else 0
+        rows = (7 if b_inst else 6) * lin_rows + (9 + (2 if b_inst else 0)) * cone_rows \
+            + ((nh - nh_cone) + 2 * nh_cone) * (npad // 64)
+        byt = cols * rows * sv
     else:
-        extra = 1 if nh else 0
-        cone = (12 if fp64 else 11) + (2 if b_inst else 0) + extra
-        rows = (7 if b_inst else 6) * lin_rows + cone * cone_rows + extra * lin_rows + nh * (npad // 64)
+        cone = (12 if fp64 else 11) + (2 if b_inst else 0)
+        rows = (7 if b_inst else 6) * lin_rows + cone * cone_rows
         byt = cols * rows * sv
     ach = byt / (ms * 1e-3) / 1e9
     traffic = measured_traffic(traffic_profile, kind, fp64) if traffic_profile and not dense else None
@@ -391,7 +401,9 @@ def run_family(args):
         "ratio_vs_plain_gpu": round(value / plain["value"], 1) if plain else None,
     }
     prof = "family_portfolio_16384_pmc_traffic" if (args.count or 16384) == 16384 else None
-    roof = family_roofline(kt, info, int(fam.Ap[-1]), prec == "fp64", fam.f + fam.l, False, prof)
+    row_nnz = np.bincount(fam.Ai, minlength=fam.m)
+    dense_cone = int(((row_nnz >= 64) & (np.arange(fam.m) >= fam.f + fam.l)).sum())
+    roof = family_roofline(kt, info, int(fam.Ap[-1]), prec == "fp64", fam.f + fam.l, False, prof, dense_cone)
     acc = {"solved": solved, "count": n_te, "verified_fp64": v_warm,
            "mean_iters": float(warm["iters"].mean()), "max_iters": int(warm["iters"].max())}
     return value, wall, "f32+bf16" if prec == "mixed" else "f64", config, roof, acc, extra, probs[:args.cpu_sample], "direct"

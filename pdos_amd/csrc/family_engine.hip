@@ -122,6 +122,10 @@ __device__ __forceinline__ int2 ld_uniform(const int2* p) {
   const int* q = reinterpret_cast<const int*>(p);
   return int2{ld_uniform(q), ld_uniform(q + 1)};
 }
+__device__ __forceinline__ int4 ld_uniform(const int4* p) {
+  const int* q = reinterpret_cast<const int*>(p);
+  return int4{ld_uniform(q), ld_uniform(q + 1), ld_uniform(q + 2), ld_uniform(q + 3)};
+}
 
 // Sums one value per (wave, lane) over the waves of the block: every thread
 // gets its lane's total.  Two barriers; `buf` is NWV x LANES doubles.
@@ -482,6 +486,7 @@ constexpr int RPC = 64;
 constexpr int RW = RPC / NWV;
 constexpr int kPlanes = 8;  // loads in flight per dense row (planes) or light row (entries) in phase C
 constexpr int kDenseK = 3;  // phase C: a wave's rows k < kDenseK load their planes with the other rows
+constexpr int kNarrowCols = 64;  // batches of at most this many 64-instance blocks run the latency-first kernels
 
 // Phase A over the m-row items of phase C (item blockIdx.y of `items`):
 // W (next_w) and the item's partial |(S - b + AX) ./ D|^2 and b'Y per
@@ -713,13 +718,22 @@ struct AxLoads {
   T pv[kDenseK][kPlanes];  // planes of the dense rows among the wave's first kDenseK rows
 };
 
+// The wave's records (scalar loads; the table holds one for every (item,
+// wave, k), rows past the item's end included).
 template <typename T>
-__device__ __forceinline__ void ax_issue(const FamArgs<T>& a, int item, int w, int4 it, int inst, AxLoads<T>& L) {
+__device__ __forceinline__ void ax_records(const FamArgs<T>& a, int item, int w, AxLoads<T>& L) {
+#pragma unroll
+  for (int k = 0; k < RW; ++k) L.rc[k] = ld_uniform(a.rec8 + ((size_t)item * NWV + w) * RW + k);
+}
+
+template <typename T>
+__device__ __forceinline__ void ax_issue(const FamArgs<T>& a, int item, int w, int4 it, int inst, AxLoads<T>& L,
+                                         bool have_records = false) {
 #pragma unroll
   for (int k = 0; k < RW; ++k) {
     const int j = it.x + w + k * NWV;
     if (j >= it.y) continue;
-    L.rc[k] = ld_uniform(a.rec8 + ((size_t)item * NWV + w) * RW + k);
+    if (!have_records) L.rc[k] = ld_uniform(a.rec8 + ((size_t)item * NWV + w) * RW + k);
     L.av[k] = L.rc[k].x >= 0 ? a.XN[(size_t)L.rc[k].x * a.ldb + inst] : T(0);
     if (k < kDenseK && L.rc[k].x == -1) {
 #pragma unroll
@@ -938,6 +952,98 @@ __global__ __launch_bounds__(FT) void k_fam_c2(FamArgs<T> a, const int4* __restr
         if (g0 + u < gend) n2 += pv[u];
     }
     const int r0 = a.soc_ptr[cn];
+    const T ns = T(sqrt(n2));
+    const int mode = ns <= t0 ? 0 : (ns <= -t0 ? 1 : 2);
+    const T al = (ns + t0) / T(2);
+#pragma unroll
+    for (int k = 0; k < RW; ++k) {
+      const int j = it.x + w + k * NWV;
+      if (j < it.y) {
+        const size_t o = (size_t)j * a.ldb + inst;
+        const T z = zv[k];
+        const T sv = mode == 0 ? z : (mode == 1 ? T(0) : (j == r0 ? al : al * z / ns));
+        const T y = yv[k] + (sv - wv[k]) / lam;
+        a.Y[o] = y;
+        a.S[o] = sv;
+        const T r = from_st ? (sv - wv[k]) / T(a.D[j]) : ((sv - bv[k]) + av[k]) / T(a.D[j]);
+        p2 += double(r) * double(r);
+        by += double(bv[k]) * double(y);
+        a.W[o] = from_st ? wv[k] - (sv + lam * y) : next_w(bv[k], av[k], sv, y, lam);
+      }
+    }
+  }
+  p2 = waves_sum(p2, buf);
+  by = waves_sum(by, buf);
+  if (act && w == 0) {
+    double* pa = a.pa + ((size_t)(a.nlin_items + blockIdx.y) * a.ldb + inst) * 2;
+    pa[0] = p2;
+    pa[1] = by;
+  }
+}
+
+// Phase C second pass for narrow batches, where each launch is a chain of
+// memory round trips rather than a stream: A with dense rows (mixed) only.
+// The item, cone and row records are read first as one batch of scalar
+// loads (none depends on data), then every operand of the wave's rows (S, Y,
+// b, the XN values and planes of AX, the cone's chunk partials, t0, lambda)
+// as one batch of vector loads issued for every lane, finished or not (all
+// in bounds for inst < ldb); only the stores wait for `done`.  Same
+// arithmetic as k_fam_c2.
+template <typename T>
+__global__ __launch_bounds__(FT) void k_fam_c2_ll(FamArgs<T> a, const int4* __restrict__ items,
+                                                  const int* __restrict__ cone) {
+  __shared__ double buf[NWV * LANES];
+  const int lane = threadIdx.x & 63, w = wave_id();
+  const int inst = blockIdx.x * LANES + lane;
+  const int4 it = ld_uniform(items + blockIdx.y);
+  const int cn = ld_uniform(cone + blockIdx.y);
+  AxLoads<T> L;
+  ax_records(a, a.nx_items + a.nlin_items + blockIdx.y, w, L);
+  const int dn = a.done[inst];
+  const T lam = T(a.lam[inst]);
+  const int r0 = ld_uniform(a.soc_ptr + cn);
+  const T t0 = T(a.pt0[(size_t)cn * a.ldb + inst]);
+  T zv[RW], wv[RW], yv[RW], av[RW], bv[RW];
+#pragma unroll
+  for (int k = 0; k < RW; ++k) {
+    const int j = it.x + w + k * NWV;
+    if (j < it.y) {
+      const size_t o = (size_t)j * a.ldb + inst;
+      zv[k] = a.S[o]; yv[k] = a.Y[o];
+      bv[k] = bval(a, j, inst);
+    }
+  }
+  ax_issue(a, 0, w, it, inst, L, true);
+  const int gend = it.z + it.w;
+  double pv[16];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) pv[u] = a.pc[(size_t)min(it.z + u, gend - 1) * a.ldb + inst];
+  const bool act = inst < a.count && !dn;
+  ax_finish(a, w, it, inst, L, av);
+  const T ra = T(a.relax), rb = T(1.0) - T(a.relax);
+#pragma unroll
+  for (int k = 0; k < RW; ++k) {
+    if (it.x + w + k * NWV >= it.y) continue;
+    T st = bv[k] - av[k];
+    st = ra * st + rb * zv[k];
+    wv[k] = st;
+    zv[k] = st - lam * yv[k];
+  }
+  double n2 = 0.0;
+#pragma unroll
+  for (int u = 0; u < 16; ++u)
+    if (it.z + u < gend) n2 += pv[u];
+  for (int g0 = it.z + 16; g0 < gend; g0 += 16) {  // cones of more than 16 chunks
+    double qv[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) qv[u] = a.pc[(size_t)min(g0 + u, gend - 1) * a.ldb + inst];
+#pragma unroll
+    for (int u = 0; u < 16; ++u)
+      if (g0 + u < gend) n2 += qv[u];
+  }
+  const bool from_st = a.relax == 1.0;
+  double p2 = 0.0, by = 0.0;
+  if (act) {
     const T ns = T(sqrt(n2));
     const int mode = ns <= t0 ? 0 : (ns <= -t0 ? 1 : 2);
     const T al = (ns + t0) / T(2);
@@ -1902,7 +2008,16 @@ bool Family::iterate(FamArgs<T>& a, int steps, std::string& err) {
     }
     mark(K_C);
     hipLaunchKernelGGL(k_fam_c1<T>, dim3(cols, nc1), block, 0, st, a, c1);
-    if (nc2) hipLaunchKernelGGL(k_fam_c2<T>, dim3(cols, nc2), block, 0, st, a, c2, c2_cone);
+    if (nc2) {
+      bool ll = false;
+      if constexpr (std::is_same<T, float>::value) {
+        if (hybrid() && cols <= kNarrowCols) {
+          hipLaunchKernelGGL(k_fam_c2_ll<float>, dim3(cols, nc2), block, 0, st, a, c2, c2_cone);
+          ll = true;
+        }
+      }
+      if (!ll) hipLaunchKernelGGL(k_fam_c2<T>, dim3(cols, nc2), block, 0, st, a, c2, c2_cone);
+    }
     std::swap(a.X, a.XN);
     mark(-1);
   }
