@@ -981,6 +981,90 @@ __global__ __launch_bounds__(FT) void k_fam_c2(FamArgs<T> a, const int4* __restr
   }
 }
 
+// Phase C first pass for narrow batches (A with dense rows, mixed): as
+// k_fam_c1 with its records read first as one batch of scalar loads and
+// every operand as one batch of vector loads for every lane (in bounds for
+// inst < ldb); only the stores wait for `done`.
+template <typename T>
+__global__ __launch_bounds__(FT) void k_fam_c1_ll(FamArgs<T> a, const int4* __restrict__ items) {
+  __shared__ double buf[NWV * LANES];
+  const int lane = threadIdx.x & 63, w = wave_id();
+  const int inst = blockIdx.x * LANES + lane;
+  const int4 it = ld_uniform(items + blockIdx.y);
+  AxLoads<T> L;
+  ax_records(a, blockIdx.y, w, L);
+  const int dn = a.done[inst];
+  const T lam = T(a.lam[inst]);
+  if (it.z == 0) {  // x rows of finished instances: XN = X
+    T xv[RW];
+#pragma unroll
+    for (int k = 0; k < RW; ++k) {
+      const int i = it.x + w + k * NWV;
+      if (i < it.y) xv[k] = a.X[(size_t)i * a.ldb + inst];
+    }
+    if (inst >= a.count || !dn) return;
+#pragma unroll
+    for (int k = 0; k < RW; ++k) {
+      const int i = it.x + w + k * NWV;
+      if (i < it.y) a.XN[(size_t)i * a.ldb + inst] = xv[k];
+    }
+    return;
+  }
+  const int cone = it.w;
+  const int cone_r0 = it.z == 2 ? ld_uniform(a.soc_ptr + cone) : 0;
+  T bv[RW], av[RW], sv[RW], yv[RW];
+#pragma unroll
+  for (int k = 0; k < RW; ++k) {
+    const int j = it.x + w + k * NWV;
+    if (j < it.y) {
+      const size_t o = (size_t)j * a.ldb + inst;
+      bv[k] = bval(a, j, inst); sv[k] = a.S[o]; yv[k] = a.Y[o];
+    }
+  }
+  ax_issue(a, 0, w, it, inst, L, true);
+  const bool act = inst < a.count && !dn;
+  ax_finish(a, w, it, inst, L, av);
+  const T ra = T(a.relax), rb = T(1.0) - T(a.relax);
+  double n2 = 0.0, p2 = 0.0, by = 0.0;
+  if (act) {
+#pragma unroll
+    for (int k = 0; k < RW; ++k) {
+      const int j = it.x + w + k * NWV;
+      if (j < it.y) {
+        const size_t o = (size_t)j * a.ldb + inst;
+        T st = bv[k] - av[k];
+        st = ra * st + rb * sv[k];
+        const T z = st - lam * yv[k];
+        if (it.z == 1) {
+          const T s = j < a.f ? T(0) : (z < T(0) ? T(0) : z);
+          const T y = yv[k] + (s - st) / lam;
+          a.S[o] = s;
+          a.Y[o] = y;
+          const T r = ((s - bv[k]) + av[k]) / T(a.D[j]);
+          p2 += double(r) * double(r);
+          by += double(bv[k]) * double(y);
+          a.W[o] = next_w(bv[k], av[k], s, y, lam);
+        } else {
+          if (j > cone_r0) n2 += double(z) * double(z);
+          else a.pt0[(size_t)cone * a.ldb + inst] = double(z);  // t0, read by every chunk of the cone
+        }
+      }
+    }
+  }
+  if (it.z == 1) {
+    p2 = waves_sum(p2, buf);
+    by = waves_sum(by, buf);
+    if (act && w == 0) {
+      double* pa = a.pa + ((size_t)(blockIdx.y - a.nx_items) * a.ldb + inst) * 2;
+      pa[0] = p2;
+      pa[1] = by;
+    }
+    return;
+  }
+  n2 = waves_sum(n2, buf);
+  if (act && w == 0) a.pc[(size_t)blockIdx.y * a.ldb + inst] = n2;
+}
+
 // Phase C second pass for narrow batches, where each launch is a chain of
 // memory round trips rather than a stream: A with dense rows (mixed) only.
 // The item, cone and row records are read first as one batch of scalar
@@ -2007,7 +2091,14 @@ bool Family::iterate(FamArgs<T>& a, int steps, std::string& err) {
       hipLaunchKernelGGL(k_fam_ax_decide<T>, dim3(cols + cols * curA->nA), block, 0, st, a, in, cm, cn);
     }
     mark(K_C);
-    hipLaunchKernelGGL(k_fam_c1<T>, dim3(cols, nc1), block, 0, st, a, c1);
+    bool ll1 = false;
+    if constexpr (std::is_same<T, float>::value) {
+      if (hybrid() && cols <= kNarrowCols) {
+        hipLaunchKernelGGL(k_fam_c1_ll<float>, dim3(cols, nc1), block, 0, st, a, c1);
+        ll1 = true;
+      }
+    }
+    if (!ll1) hipLaunchKernelGGL(k_fam_c1<T>, dim3(cols, nc1), block, 0, st, a, c1);
     if (nc2) {
       bool ll = false;
       if constexpr (std::is_same<T, float>::value) {
