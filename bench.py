@@ -176,10 +176,13 @@ def plain_gpu(probs, plugin, precision):
 # kernel names behind each timed kind of the family engine (sparse A)
 # (sparse A; with dense rows in A the A' product is k_fam_at_hyb, the A
 # product and the stop test ride in the x-update and phase C)
-FAMILY_KERNELS = {"At_product": ["k_fam_at_b<{t}>", "k_fam_at_hyb<{t}>"],
-                  "A_product": ["k_fam_ax_decide<{t}>"], "xupdate_gemm": ["k_fam_xupdate"],
-                  "phase_b": ["k_fam_decide<{t}>"], "phase_c": ["k_fam_c1<{t}>", "k_fam_c2<{t}>"],
-                  "phase_a": ["k_fam_a<{t}>"]}
+# kernels of each timed kind, in passes: one launch of each pass per
+# iteration; a pass has a kernel for wide batches and one for narrow ones
+FAMILY_KERNELS = {"At_product": [["k_fam_at_b<{t}>", "k_fam_at_hyb<{t}>"]],
+                  "A_product": [["k_fam_ax_decide<{t}>"]], "xupdate_gemm": [["k_fam_xupdate"]],
+                  "phase_b": [["k_fam_decide<{t}>"]],
+                  "phase_c": [["k_fam_c1<{t}>", "k_fam_c1_ll<{t}>"], ["k_fam_c2<{t}>", "k_fam_c2_ll<{t}>"]],
+                  "phase_a": [["k_fam_a<{t}>"]]}
 
 
 def measured_traffic(profile, kind, fp64):
@@ -193,11 +196,14 @@ def measured_traffic(profile, kind, fp64):
     if "```json" not in text:
         return None
     data = json.loads(text.split("```json", 1)[1].split("```", 1)[0])["kernels"]
-    names = [k.format(t="double" if fp64 else "float") for k in FAMILY_KERNELS.get(kind, [])]
-    names = [k for k in names if k in data]
-    if not names:
+    t = "double" if fp64 else "float"
+    passes = [[k.format(t=t) for k in ks if k.format(t=t) in data] for ks in FAMILY_KERNELS.get(kind, [])]
+    if not passes or not passes[0]:
         return None
-    return sum(data[k]["read_bytes"] + data[k]["write_bytes"] for k in names)
+    # per-launch averages in the summary: total bytes over every launch of
+    # the kind's kernels, per launch of its first pass (one per iteration)
+    total = sum((data[k]["read_bytes"] + data[k]["write_bytes"]) * data[k]["launches"] for ks in passes for k in ks)
+    return total / sum(data[k]["launches"] for k in passes[0])
 
 
 def family_roofline(kt, fam_info, nnz, fp64, lin_rows, b_inst, traffic_profile=None, dense_cone=None):
