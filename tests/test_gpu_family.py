@@ -190,3 +190,30 @@ def test_dense_family_mixed_verifies_in_fp64():
         assert np.linalg.norm(A @ x + s - b) < 1.01e-3 * (1 + np.linalg.norm(b))
         assert np.linalg.norm(c + A.T @ y) < 1.01e-3 * (1 + np.linalg.norm(c))
         assert abs(c @ x - opt[t]) <= 2e-2 * (1 + abs(opt[t]))
+
+
+def test_family_wide_batch_uses_both_phase_c_variants():
+    """6,144 instances on one engine: phase C runs its wide kernels while
+    more than 64 blocks of 64 instances are live and its narrow,
+    latency-first kernels after compaction; the same instances in runs of
+    3,072 use only the narrow ones.  Both do the same arithmetic in the same
+    order, so statuses and iteration counts agree, and every Solved point
+    meets the stop test in binary64."""
+    fam = _fam(na=200, k=15, MAX_ITERS=400)
+    _, Cm = fam.sample(seed=13, count=6144)
+    print("family wide batch", flush=True)
+    F = Family(fam.instance(Cm[0]), precision="mixed")
+    F.engines(1)
+    F.load(Cm)
+    F.run()
+    wide = F.fetch()
+    parts = []
+    for q in range(2):
+        F.load(Cm[q * 3072:(q + 1) * 3072])
+        F.run()
+        parts.append(F.fetch())
+    for key in ("iters", "status"):
+        np.testing.assert_array_equal(wide[key], np.concatenate([p[key] for p in parts]))
+    solved = int((wide["status"] == "Solved").sum())
+    assert solved >= 0.5 * len(Cm)
+    assert _verify(fam, Cm, wide, 1.05) == solved
