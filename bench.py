@@ -517,7 +517,7 @@ def main():
     ap.add_argument("--workload", default="family")
     ap.add_argument("--count", type=int, default=0)
     ap.add_argument("--train", type=int, default=0)
-    ap.add_argument("--epochs", type=int, default=300)
+    ap.add_argument("--epochs", type=int, default=600)
     ap.add_argument("--plugin", default="direct")
     ap.add_argument("--precision", default="")
     ap.add_argument("--cpu-sample", type=int, default=1024)
