@@ -1465,20 +1465,6 @@ struct Family {
   hipEvent_t fev[2] = {nullptr, nullptr};
   int compactions = 0;
   int nprod = 1;          // mixed: bf16 products of the split GEMM (1, 2 or 3)
-  // Graph replay of the loop (mixed mode) is opt-in (PDOS_FAMILY_GRAPH=1): it
-  // measured no faster than direct launches, and its capture crashed inside
-  // the HIP runtime under the rocprofv3 kernel tracer.
-  bool use_graph = false;
-  // mixed: one captured graph of `gsteps` iterations for the current layout
-  hipGraphExec_t gexec = nullptr;
-  int g_ld = -1, g_steps = -1;
-  const void* g_x = nullptr;
-  void drop_graph() {
-    if (gexec) (void)hipGraphExecDestroy(gexec);
-    gexec = nullptr;
-    g_ld = g_steps = -1;
-    g_x = nullptr;
-  }
   // work tables (device), built once per family: SpMM wave tasks for A and
   // A' at two granularities ([0] up to 64 entries per task, [1] up to 16, for
   // small compacted batches) and the phase C items (RPC rows each)
@@ -1561,7 +1547,6 @@ struct Family {
   std::vector<void*> allocs;
 
   ~Family() {
-    drop_graph();
     free_batch();
     for (hipEvent_t& e : fev)
       if (e) (void)hipEventDestroy(e);
@@ -1668,10 +1653,6 @@ static std::vector<int> entry_rows(const std::vector<int>& ptr) {
 bool Family::create(const Data* d, const Cone* k, const PdosOptions& o, std::string& err) {
   fp64 = o.precision == PDOS_FP64;
   if (o.bf16_products >= 1 && o.bf16_products <= 3) nprod = o.bf16_products;
-  {
-    const char* g = std::getenv("PDOS_FAMILY_GRAPH");
-    use_graph = g && g[0] == '1';
-  }
   device = o.device;
   if (o.equil != PDOS_EQUIL_REFERENCE) {  // Ruiz scaling depends on each instance's b and c
     err = "the family engine shares one scaling of A across instances: PDOS_EQUIL_REFERENCE only";
@@ -2248,39 +2229,7 @@ bool Family::run_t(const float* warm, long wld, int judge, std::string& err) {
     const int steps = (int)std::min<long>(16, a.max_iters + 2 - total);
     if (steps <= 0) return 0;
     if (need_prologue) prologue(a);
-    if (use_graph && !fp64 && !timing) {
-      // replay a captured graph of `steps` iterations for this layout
-      if (!gexec || g_ld != a.ldb || g_steps != steps || g_x != (const void*)a.X) {
-        drop_graph();
-        const void* x0 = a.X;
-        hipGraph_t g = nullptr;
-        if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-          err = "stream capture failed";
-          return -1;
-        }
-        const bool ok = iterate(a, steps, err);
-        if (hipStreamEndCapture(st, &g) != hipSuccess || !ok ||
-            hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0) != hipSuccess) {
-          if (g) (void)hipGraphDestroy(g);
-          gexec = nullptr;
-          if (err.empty()) err = "graph instantiation failed";
-          return -1;
-        }
-        (void)hipGraphDestroy(g);
-        g_ld = a.ldb;
-        g_steps = steps;
-        g_x = x0;
-        // capture left a.X / a.XN as after `steps` iterations; undo, the launch redoes it
-        if (steps % 2) std::swap(a.X, a.XN);
-      }
-      if (hipGraphLaunch(gexec, st) != hipSuccess) {
-        err = "graph launch failed";
-        return -1;
-      }
-      if (steps % 2) std::swap(a.X, a.XN);
-    } else if (!iterate(a, steps, err)) {
-      return -1;
-    }
+    if (!iterate(a, steps, err)) return -1;
     total += steps;
     (void)hipMemcpyAsync(hflags + (size_t)slot * ldb, a.done, (size_t)cnt * 4, hipMemcpyDeviceToHost, st);
     (void)hipEventRecord(fev[slot], st);
