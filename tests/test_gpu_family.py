@@ -217,3 +217,27 @@ def test_family_wide_batch_uses_both_phase_c_variants():
     solved = int((wide["status"] == "Solved").sum())
     assert solved >= 0.5 * len(Cm)
     assert _verify(fam, Cm, wide, 1.05) == solved
+
+
+def test_family_mixed_relaxed_normalized_verifies_in_fp64():
+    """Over-relaxation (ALPHA 1.8) and NORMALIZE 1 in mixed precision, with
+    dense rows in A: phase C's relaxed branches (W from b, AX, s, y rather
+    than from s~) in both its wide and narrow kernels (5,000 instances on
+    one engine).  Every Solved point meets the stop test in binary64, and the
+    solved fraction matches the FP64 engine's on a subset."""
+    fam = _fam(na=200, k=15, ALPHA=1.8, NORMALIZE=1, MAX_ITERS=600)
+    _, Cm = fam.sample(seed=14, count=5000)
+    print("family mixed relaxed", flush=True)
+    F = Family(fam.instance(Cm[0]), precision="mixed")
+    F.engines(1)
+    F.load(Cm)
+    F.run()
+    out = F.fetch()
+    solved = int((out["status"] == "Solved").sum())
+    assert solved >= 0.5 * len(Cm)
+    assert _verify(fam, Cm, out, 1.05) == solved
+    F64 = Family(fam.instance(Cm[0]), precision="fp64")
+    F64.load(Cm[:200])
+    F64.run()
+    o64 = F64.fetch()
+    assert abs(int((o64["status"] == "Solved").sum()) - int((out["status"][:200] == "Solved").sum())) <= 6
