@@ -486,7 +486,7 @@ constexpr int RPC = 64;
 constexpr int RW = RPC / NWV;
 constexpr int kPlanes = 8;  // loads in flight per dense row (planes) or light row (entries) in phase C
 constexpr int kDenseK = 3;  // phase C: a wave's rows k < kDenseK load their planes with the other rows
-constexpr int kNarrowCols = 64;  // batches of at most this many 64-instance blocks run the latency-first kernels
+constexpr int kNarrowCols = 128;  // batches of at most this many 64-instance blocks run the latency-first kernels
 
 // Phase A over the m-row items of phase C (item blockIdx.y of `items`):
 // W (next_w) and the item's partial |(S - b + AX) ./ D|^2 and b'Y per

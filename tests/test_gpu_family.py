@@ -193,14 +193,14 @@ def test_dense_family_mixed_verifies_in_fp64():
 
 
 def test_family_wide_batch_uses_both_phase_c_variants():
-    """6,144 instances on one engine: phase C runs its wide kernels while
-    more than 64 blocks of 64 instances are live and its narrow,
+    """9,216 instances on one engine: phase C runs its wide kernels while
+    more than 128 blocks of 64 instances are live and its narrow,
     latency-first kernels after compaction; the same instances in runs of
     3,072 use only the narrow ones.  Both do the same arithmetic in the same
     order, so statuses and iteration counts agree, and every Solved point
     meets the stop test in binary64."""
     fam = _fam(na=200, k=15, MAX_ITERS=400)
-    _, Cm = fam.sample(seed=13, count=6144)
+    _, Cm = fam.sample(seed=13, count=9216)
     print("family wide batch", flush=True)
     F = Family(fam.instance(Cm[0]), precision="mixed")
     F.engines(1)
@@ -208,7 +208,7 @@ def test_family_wide_batch_uses_both_phase_c_variants():
     F.run()
     wide = F.fetch()
     parts = []
-    for q in range(2):
+    for q in range(3):
         F.load(Cm[q * 3072:(q + 1) * 3072])
         F.run()
         parts.append(F.fetch())
@@ -222,11 +222,11 @@ def test_family_wide_batch_uses_both_phase_c_variants():
 def test_family_mixed_relaxed_normalized_verifies_in_fp64():
     """Over-relaxation (ALPHA 1.8) and NORMALIZE 1 in mixed precision, with
     dense rows in A: phase C's relaxed branches (W from b, AX, s, y rather
-    than from s~) in both its wide and narrow kernels (5,000 instances on
+    than from s~) in both its wide and narrow kernels (9,000 instances on
     one engine).  Every Solved point meets the stop test in binary64, and the
     solved fraction matches the FP64 engine's on a subset."""
     fam = _fam(na=200, k=15, ALPHA=1.8, NORMALIZE=1, MAX_ITERS=600)
-    _, Cm = fam.sample(seed=14, count=5000)
+    _, Cm = fam.sample(seed=14, count=9000)
     print("family mixed relaxed", flush=True)
     F = Family(fam.instance(Cm[0]), precision="mixed")
     F.engines(1)
