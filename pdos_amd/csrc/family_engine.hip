@@ -203,7 +203,7 @@ __device__ __forceinline__ double lane_of(double v, int l) {
   return __longlong_as_double((hi << 32) | lo);
 }
 
-constexpr int kUnroll = 16;  // gathers in flight per wave
+constexpr int kUnroll = 8;  // gathers in flight per wave (measured 4 / 8 / 12 / 16 / 32: 121.4k / 121.5k / 120.4k / 120.0k / 112.6k SOCPs/s on the portfolio family)
 // Mixed precision, sparse A: rows of A with at least kDenseRow entries (the
 // kDenseMax longest) are the "dense rows" whose A' part is an fp32 MFMA tile
 constexpr int kDenseRow = 64, kDenseMax = 32;
