@@ -22,7 +22,17 @@
 //
 // Phase A forms W and those partials from scratch; it runs only when the
 // iterate or the layout changed outside the loop (start, warm-start judge,
-// compaction).
+// and compaction unless A has dense rows).
+//
+// Mixed precision with dense rows in A (rows of 64 or more entries, at most
+// 32): an iteration is four launches.  The A' product multiplies the dense
+// rows as an fp32 MFMA tile and gathers the rest (k_fam_at_hyb); the
+// x-update also forms the dense rows' part of A XN as one plane per row tile
+// and runs the stop test in extra workgroups (k_fam_xupdate); phase C sums
+// the planes and gathers the other rows' entries from XN itself, without
+// storing AX, in two passes over the cone rows (k_fam_c1 / k_fam_c2, or the
+// latency-first k_fam_c1_ll / k_fam_c2_ll for narrow batches).  Compaction
+// then moves W and the stop-test partials instead of AX.
 //
 // An instance that meets the stop test (pdos.c:95-107) or MAX_ITERS is frozen:
 // phases A-C skip it, so its iterate is exactly the one the stop test saw.
@@ -1285,7 +1295,7 @@ __global__ __launch_bounds__(256) void k_fam_xupdate(int mt, int nt, int ksteps,
   // over K rows m0 .. m0 + 63, on the fp32 MFMA 16x16x4 (wave q: K rows
   // m0 + 16 q ..); the four wave partials meet in LDS and each wave adds 8
   // rows of them in wave order into plane mi of AXp (32 x ldb per row tile;
-  // the A product sums the planes).  Each wave rewrites only the rows of
+  // phase C sums the planes).  Each wave rewrites only the rows of
   // red[0..3] it read above, so the XN tile can go to red[0] at once.
 #pragma unroll
   for (int r = 0; r < 16; ++r) red[0][16 * q + r][lane] = xn[r];
