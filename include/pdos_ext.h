@@ -44,7 +44,7 @@ typedef struct PdosOptions {
   int engine;      /* PDOS_ENGINE_*: WG = one workgroup per instance,
                       GRID = whole-GPU kernels for one large instance */
   int device;      /* HIP device ordinal */
-  int block;       /* WG engine threads per workgroup (0 = autotuned) */
+  int block;       /* reserved, 0 (the WG engine runs 256 threads per workgroup) */
   int bf16_products; /* family engine, mixed: products of the split-bf16 x-update
                         GEMM, 1 (hi*hi), 2 (+hi*lo) or 3 (+lo*hi); 0 = 1.  The
                         GEMM computes the correction x+ - x, whose error shrinks
