@@ -150,6 +150,25 @@ __device__ __forceinline__ double waves_sum(double v, double* buf) {
   return s;
 }
 
+// As waves_sum for two values at once (one pair of barriers, each value
+// summed in the same order); `buf` is 2 x NWV x LANES doubles.
+__device__ __forceinline__ void waves_sum2(double& a, double& b, double* buf) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double2* b2 = reinterpret_cast<double2*>(buf);
+  b2[w * LANES + lane] = double2{a, b};
+  __syncthreads();
+  double sa = 0.0, sb = 0.0;
+#pragma unroll
+  for (int k = 0; k < NWV; ++k) {
+    const double2 v = b2[k * LANES + lane];
+    sa += v.x;
+    sb += v.y;
+  }
+  __syncthreads();
+  a = sa;
+  b = sb;
+}
+
 // Mixed precision (binary32 bulk) uses the correction form of the x-update.
 template <typename T>
 constexpr bool kCorr = std::is_same<T, float>::value;
@@ -335,7 +354,7 @@ __device__ __forceinline__ void spmm_core(const SpmmIn<T, NR>& in, int bx, int b
 template <typename T, int NR>
 __global__ __launch_bounds__(FT) void k_fam_spmm(SpmmIn<T, NR> in, T* __restrict__ Y1, T* __restrict__ Y2,
                                                  T* __restrict__ Y3) {
-  __shared__ double buf[NWV * LANES];
+  __shared__ double buf[2 * NWV * LANES];
   int bx, by;
   spmm_block(in.cols, in.nitems, bx, by);
   const unsigned boff = (unsigned)(bx * LANES + (threadIdx.x & 63)) * (unsigned)sizeof(T);
@@ -375,8 +394,7 @@ __device__ __forceinline__ void at_b_row(const FamArgs<T>& a, int i, int inst, T
 template <typename T>
 __device__ __forceinline__ void at_b_partials(const FamArgs<T>& a, int by, int inst, bool act, double d2,
                                               double cx, double* buf) {
-  d2 = waves_sum(d2, buf);
-  cx = waves_sum(cx, buf);
+  waves_sum2(d2, cx, buf);
   if (act && wave_id() == 0) {
     double* pb = a.pb + ((size_t)by * a.ldb + inst) * 2;
     pb[0] = d2;
@@ -386,7 +404,7 @@ __device__ __forceinline__ void at_b_partials(const FamArgs<T>& a, int by, int i
 
 template <typename T>
 __global__ __launch_bounds__(FT) void k_fam_at_b(FamArgs<T> a, SpmmIn<T, 2> in) {
-  __shared__ double buf[NWV * LANES];
+  __shared__ double buf[2 * NWV * LANES];
   const int lane = threadIdx.x & 63;
   int bx, by;
   spmm_block(in.cols, in.nitems, bx, by);
@@ -425,7 +443,7 @@ typedef float hf32x4 __attribute__((ext_vector_type(4)));
 template <typename T>
 __global__ __launch_bounds__(FT) void k_fam_at_hyb(FamArgs<T> a, SpmmIn<T, 2> in, const float* __restrict__ Ah,
                                                    DenseRows dr) {
-  __shared__ double buf[NWV * LANES];
+  __shared__ double buf[2 * NWV * LANES];
   __shared__ float Ut[64][65], Vt[64][65];
   const int lane = threadIdx.x & 63, w = wave_id();
   int bx, ti;
@@ -505,7 +523,7 @@ constexpr int kNarrowCols = 128;  // batches of at most this many 64-instance bl
 // outside the loop (start, judge, compaction).
 template <typename T>
 __global__ __launch_bounds__(FT) void k_fam_a(FamArgs<T> a, const int4* __restrict__ items) {
-  __shared__ double buf[NWV * LANES];
+  __shared__ double buf[2 * NWV * LANES];
   const int lane = threadIdx.x & 63, w = wave_id();
   const int inst = blockIdx.x * LANES + lane;
   const bool act = inst < a.count && !a.done[inst];
@@ -534,8 +552,7 @@ __global__ __launch_bounds__(FT) void k_fam_a(FamArgs<T> a, const int4* __restri
       }
     }
   }
-  p2 = waves_sum(p2, buf);
-  by = waves_sum(by, buf);
+  waves_sum2(p2, by, buf);
   if (act && w == 0) {
     double* pa = a.pa + ((size_t)blockIdx.y * a.ldb + inst) * 2;
     pa[0] = p2;
@@ -550,7 +567,7 @@ __global__ __launch_bounds__(FT) void k_fam_a(FamArgs<T> a, const int4* __restri
 //          then x+ = x + G D, so the bf16 error of the product shrinks with D.
 template <typename T, bool MIXED>
 __global__ __launch_bounds__(FT) void k_fam_b(FamArgs<T> a) {
-  __shared__ double buf[NWV * LANES];
+  __shared__ double buf[2 * NWV * LANES];
   const int lane = threadIdx.x & 63, w = wave_id();
   const int inst = blockIdx.x * LANES + lane;
   const bool act = inst < a.count && !a.done[inst];
@@ -588,8 +605,7 @@ __global__ __launch_bounds__(FT) void k_fam_b(FamArgs<T> a) {
       }
     }
   }
-  d2 = waves_sum(d2, buf);
-  cx = waves_sum(cx, buf);
+  waves_sum2(d2, cx, buf);
   if (act && w == 0) {
     double* pb = a.pb + ((size_t)blockIdx.y * a.ldb + inst) * 2;
     pb[0] = d2;
@@ -811,7 +827,7 @@ __device__ __forceinline__ void ax_finish(const FamArgs<T>& a, int w, int4 it, i
 // from the same operands, which costs fewer bytes than storing and reloading.
 template <typename T>
 __global__ __launch_bounds__(FT) void k_fam_c1(FamArgs<T> a, const int4* __restrict__ items) {
-  __shared__ double buf[NWV * LANES];
+  __shared__ double buf[2 * NWV * LANES];
   const int lane = threadIdx.x & 63, w = wave_id();
   const int inst = blockIdx.x * LANES + lane;
   const bool act = inst < a.count && !a.done[inst];
@@ -883,8 +899,7 @@ __global__ __launch_bounds__(FT) void k_fam_c1(FamArgs<T> a, const int4* __restr
     }
   }
   if (it.z == 1) {
-    p2 = waves_sum(p2, buf);
-    by = waves_sum(by, buf);
+    waves_sum2(p2, by, buf);
     if (act && w == 0) {
       double* pa = a.pa + ((size_t)(blockIdx.y - a.nx_items) * a.ldb + inst) * 2;
       pa[0] = p2;
@@ -905,7 +920,7 @@ __global__ __launch_bounds__(FT) void k_fam_c1(FamArgs<T> a, const int4* __restr
 template <typename T>
 __global__ __launch_bounds__(FT) void k_fam_c2(FamArgs<T> a, const int4* __restrict__ items,
                                                const int* __restrict__ cone) {
-  __shared__ double buf[NWV * LANES];
+  __shared__ double buf[2 * NWV * LANES];
   const int lane = threadIdx.x & 63, w = wave_id();
   const int inst = blockIdx.x * LANES + lane;
   const bool act = inst < a.count && !a.done[inst];
@@ -982,8 +997,7 @@ __global__ __launch_bounds__(FT) void k_fam_c2(FamArgs<T> a, const int4* __restr
       }
     }
   }
-  p2 = waves_sum(p2, buf);
-  by = waves_sum(by, buf);
+  waves_sum2(p2, by, buf);
   if (act && w == 0) {
     double* pa = a.pa + ((size_t)(a.nlin_items + blockIdx.y) * a.ldb + inst) * 2;
     pa[0] = p2;
@@ -997,7 +1011,7 @@ __global__ __launch_bounds__(FT) void k_fam_c2(FamArgs<T> a, const int4* __restr
 // inst < ldb); only the stores wait for `done`.
 template <typename T>
 __global__ __launch_bounds__(FT) void k_fam_c1_ll(FamArgs<T> a, const int4* __restrict__ items) {
-  __shared__ double buf[NWV * LANES];
+  __shared__ double buf[2 * NWV * LANES];
   const int lane = threadIdx.x & 63, w = wave_id();
   const int inst = blockIdx.x * LANES + lane;
   const int4 it = ld_uniform(items + blockIdx.y);
@@ -1062,8 +1076,7 @@ __global__ __launch_bounds__(FT) void k_fam_c1_ll(FamArgs<T> a, const int4* __re
     }
   }
   if (it.z == 1) {
-    p2 = waves_sum(p2, buf);
-    by = waves_sum(by, buf);
+    waves_sum2(p2, by, buf);
     if (act && w == 0) {
       double* pa = a.pa + ((size_t)(blockIdx.y - a.nx_items) * a.ldb + inst) * 2;
       pa[0] = p2;
@@ -1086,7 +1099,7 @@ __global__ __launch_bounds__(FT) void k_fam_c1_ll(FamArgs<T> a, const int4* __re
 template <typename T>
 __global__ __launch_bounds__(FT) void k_fam_c2_ll(FamArgs<T> a, const int4* __restrict__ items,
                                                   const int* __restrict__ cone) {
-  __shared__ double buf[NWV * LANES];
+  __shared__ double buf[2 * NWV * LANES];
   const int lane = threadIdx.x & 63, w = wave_id();
   const int inst = blockIdx.x * LANES + lane;
   const int4 it = ld_uniform(items + blockIdx.y);
@@ -1158,8 +1171,7 @@ __global__ __launch_bounds__(FT) void k_fam_c2_ll(FamArgs<T> a, const int4* __re
       }
     }
   }
-  p2 = waves_sum(p2, buf);
-  by = waves_sum(by, buf);
+  waves_sum2(p2, by, buf);
   if (act && w == 0) {
     double* pa = a.pa + ((size_t)(a.nlin_items + blockIdx.y) * a.ldb + inst) * 2;
     pa[0] = p2;
