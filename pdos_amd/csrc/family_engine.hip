@@ -391,6 +391,17 @@ __device__ __forceinline__ void at_b_row(const FamArgs<T>& a, int i, int inst, T
   }
 }
 
+// at_b_row's arithmetic for the mixed path, returning the right-hand side
+// value instead of storing it.
+template <typename T>
+__device__ __forceinline__ T at_b_rv(const FamArgs<T>& a, int i, T lam, T c_i, T x_i, double e_i, T u, T v,
+                                     double& d2, double& cx) {
+  const T r = (c_i + u) / T(e_i);
+  d2 += double(r) * double(r);
+  cx += double(c_i) * double(x_i);
+  return v - lam * c_i;
+}
+
 template <typename T>
 __device__ __forceinline__ void at_b_partials(const FamArgs<T>& a, int by, int inst, bool act, double d2,
                                               double cx, double* buf) {
@@ -445,6 +456,7 @@ __global__ __launch_bounds__(FT) void k_fam_at_hyb(FamArgs<T> a, SpmmIn<T, 2> in
                                                    DenseRows dr) {
   __shared__ double buf[2 * NWV * LANES];
   __shared__ float Ut[64][65], Vt[64][65];
+  __shared__ unsigned short rsh[NWV][8][LANES];  // the wave's 8 rows of Rcat (bf16 bits), stored as one block
   const int lane = threadIdx.x & 63, w = wave_id();
   int bx, ti;
   spmm_block(in.cols, in.nitems, bx, ti);
@@ -493,6 +505,10 @@ __global__ __launch_bounds__(FT) void k_fam_at_hyb(FamArgs<T> a, SpmmIn<T, 2> in
   T c_i = T(0), x_i = T(0), uh = T(0), vh = T(0);
   double e_i = 1.0, d2 = 0.0, cx = 0.0;
   const unsigned boff = (unsigned)inst * (unsigned)sizeof(T);
+  // one bf16 product and not the judge's pass: the wave's task is rows
+  // rbase .. rbase + 7 (host-built so), whose right-hand side is stored at once
+  const bool one = !a.judge && a.nprod == 1;
+  const int rbase = m0 + 8 * w;
   spmm_core<T, 2>(in, bx, ti, buf,
                   [&](int i) {
                     const size_t ro = (size_t)i * a.ldb;
@@ -503,8 +519,21 @@ __global__ __launch_bounds__(FT) void k_fam_at_hyb(FamArgs<T> a, SpmmIn<T, 2> in
                     vh = T(Vt[i - m0][lane]);
                   },
                   [&](int i, const T (&v)[2]) {
-                    if (act) at_b_row(a, i, inst, lam, c_i, x_i, e_i, v[0] + uh, v[1] + vh, d2, cx);
+                    if (!act) return;
+                    if (one) {  // this row's bf16 right-hand side, to the wave's block below
+                      const T rv = at_b_rv(a, i, lam, c_i, x_i, e_i, v[0] + uh, v[1] + vh, d2, cx);
+                      rsh[w][i - rbase][lane] = __builtin_bit_cast(unsigned short, (__bf16)float(rv));
+                    } else {
+                      at_b_row(a, i, inst, lam, c_i, x_i, e_i, v[0] + uh, v[1] + vh, d2, cx);
+                    }
                   });
+  if (one && act) {  // rows rbase .. rbase + 7 are one 16-byte block of Rcat per instance
+    unsigned b[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) b[r] = rbase + r < a.n ? rsh[w][r][lane] : 0u;  // rows past n: GEMM padding, 0
+    *reinterpret_cast<uint4*>(&rcat_at(a.Rcat, a.ldb, rbase, inst)) =
+        uint4{b[0] | (b[1] << 16), b[2] | (b[3] << 16), b[4] | (b[5] << 16), b[6] | (b[7] << 16)};
+  }
   at_b_partials(a, ti, inst, act, d2, cx, buf);
 }
 
