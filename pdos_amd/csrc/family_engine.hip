@@ -269,13 +269,40 @@ __device__ __forceinline__ void spmm_block(int cols, int nitems, int& bx, int& b
 // the row's products), done(row, acc) when the row is complete; for a split
 // row, wave 0 gets done(row, block total).  Rows without entries inside a
 // task are begun and done with zero sums.
+// A wave's task and the (column, value, row) of its first 64 entries, loaded
+// ahead by a caller that has other work to overlap them with.
+template <typename T>
+struct SpmmPre {
+  int4 tk;
+  int ci, ri;
+  T vi;
+  bool split;
+};
+
+template <typename T, int NR>
+__device__ __forceinline__ SpmmPre<T> spmm_preload(const SpmmIn<T, NR>& in, int by) {
+  const int lane = threadIdx.x & 63, w = wave_id();
+  SpmmPre<T> p;
+  p.tk = in.tasks[by * NWV + w];
+  p.split = in.tasks[by * NWV].y < 0;
+  p.ci = p.ri = 0;
+  p.vi = T(0);
+  const int q = p.tk.z, cnt = min(64, p.tk.w - q);
+  if (lane < cnt) {
+    p.ci = in.idx[q + lane];
+    p.vi = in.val[q + lane];
+    p.ri = p.split ? p.tk.x : in.rowof[q + lane];
+  }
+  return p;
+}
+
 template <typename T, int NR, typename Begin, typename Done>
 __device__ __forceinline__ void spmm_core(const SpmmIn<T, NR>& in, int bx, int by, double* buf, Begin begin,
-                                          Done done) {
+                                          Done done, const SpmmPre<T>* pre = nullptr) {
   const int lane = threadIdx.x & 63, w = wave_id();
   const unsigned boff = (unsigned)(bx * LANES + lane) * (unsigned)sizeof(T);
-  const int4 tk = in.tasks[by * NWV + w];
-  const bool split = in.tasks[by * NWV].y < 0;  // block-uniform
+  const int4 tk = pre ? pre->tk : in.tasks[by * NWV + w];
+  const bool split = pre ? pre->split : in.tasks[by * NWV].y < 0;  // block-uniform
   T acc[NR];
 #pragma unroll
   for (int k = 0; k < NR; ++k) acc[k] = T(0);
@@ -285,7 +312,11 @@ __device__ __forceinline__ void spmm_core(const SpmmIn<T, NR>& in, int bx, int b
     const int cnt = min(64, tk.w - q);
     int ci = 0, ri = 0;
     T vi = T(0);
-    if (lane < cnt) {
+    if (pre && q == tk.z) {
+      ci = pre->ci;
+      vi = pre->vi;
+      ri = pre->ri;
+    } else if (lane < cnt) {
       ci = in.idx[q + lane];
       vi = in.val[q + lane];
       ri = split ? tk.x : in.rowof[q + lane];
@@ -461,6 +492,7 @@ __global__ __launch_bounds__(FT) void k_fam_at_hyb(FamArgs<T> a, SpmmIn<T, 2> in
   int bx, ti;
   spmm_block(in.cols, in.nitems, bx, ti);
   const int m0 = ti * 64, n0 = bx * LANES;
+  const SpmmPre<T> pre = spmm_preload(in, ti);  // in flight with the dense tile's operands
   {
     const int rg = w >> 1, cg0 = (w & 1) * 2, kq = lane >> 4, col = lane & 15;
     float av[8], yb[2][8], wb[2][8];
@@ -526,7 +558,8 @@ __global__ __launch_bounds__(FT) void k_fam_at_hyb(FamArgs<T> a, SpmmIn<T, 2> in
                     } else {
                       at_b_row(a, i, inst, lam, c_i, x_i, e_i, v[0] + uh, v[1] + vh, d2, cx);
                     }
-                  });
+                  },
+                  &pre);
   if (one && act) {  // rows rbase .. rbase + 7 are one 16-byte block of Rcat per instance
     unsigned b[8];
 #pragma unroll
