@@ -67,7 +67,9 @@ typedef struct PdosInfo {
   double lnz;        /* nnz(L) for the direct plugin */
 } PdosInfo;
 
-/* Default options: indirect, fp64, auto engine, device 0. */
+/* Default options: the library's plugin (indirect in libpdosindir_hip.so,
+ * direct in libpdosdir_hip.so), fp64, auto engine, device 0, the reference
+ * scaling. */
 void pdos_default_options(PdosOptions *opt);
 
 /* pdos() with options and info (info may be NULL). */
@@ -78,7 +80,8 @@ int pdos_last_info(PdosInfo *info);
 
 /* Solves count independent problems; sols[i] receives a malloc'd Sol (free
  * each with freeSol), infos (optional) per-instance info.  Returns 0 or -1
- * (see pdos_last_error). */
+ * (see pdos_last_error).  Setup and the direct plugin's factorizations run on
+ * host threads (PDOS_HOST_THREADS, default the hardware threads, at most 64). */
 int pdos_batch(const Data *const *ds, const Cone *const *ks, int count, const PdosOptions *opt,
                Sol **sols, PdosInfo *infos);
 
