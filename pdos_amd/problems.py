@@ -172,11 +172,14 @@ def soc_sizes(rng, total: int, count: int, lo: int, hi: int) -> list:
 
 def random_feasible(seed: int = 1, n: int = 100_000, m: int = 300_000, nnz: int = 5_000_000,
                     f: int = 1_000, l: int = 149_000, nsoc: int = 2_000, soc_lo: int = 10,
-                    soc_hi: int = 140, dense_rows: int = 0, dense_cols: int = 0, **params) -> Problem:
+                    soc_hi: int = 140, dense_rows: int = 0, dense_cols: int = 0, empty_rows: int = 0,
+                    empty_cols: int = 0, **params) -> Problem:
     """Random sparse SOCP with a planted optimal pair.  Entries N(0,1) at
     uniform random (i, j) (duplicates summed), plus `dense_rows` full rows
     (entries N(0,1)/sqrt(n)) and `dense_cols` full columns (N(0,1)/sqrt(m)) at
-    random positions.
+    random positions.  `empty_rows` rows (0, m/e, 2m/e, ...) and the last
+    `empty_cols` columns then lose all their entries (a row of A that is
+    empty leaves s_i = b_i; an empty column leaves x_j free with c_j = 0).
     x* ~ N(0,1), z ~ N(0,1), s* = Pi_K(z), y* = Pi_K(z) - z, b = A x* + s*,
     c = -A' y*; then (x*, s*, y*) is primal-dual optimal with c'x* = -b'y*."""
     rng = np.random.default_rng(seed)
@@ -192,6 +195,9 @@ def random_feasible(seed: int = 1, n: int = 100_000, m: int = 300_000, nnz: int 
         cols = np.concatenate([cols, np.tile(np.arange(n), dense_rows), np.repeat(dc, m)])
         vals = np.concatenate([vals, rd.standard_normal(dense_rows * n) / np.sqrt(n),
                                rd.standard_normal(dense_cols * m) / np.sqrt(m)])
+    if empty_rows or empty_cols:
+        gone = np.isin(rows, np.arange(empty_rows) * (m // max(empty_rows, 1))) | (cols >= n - empty_cols)
+        rows, cols, vals = rows[~gone], cols[~gone], vals[~gone]
     A = sp.csc_matrix((vals, (rows, cols)), shape=(m, n))
     A.sum_duplicates()
     A.sort_indices()
@@ -203,6 +209,30 @@ def random_feasible(seed: int = 1, n: int = 100_000, m: int = 300_000, nnz: int 
     c = -(A.T @ ys)
     return Problem(n, m, A.indptr, A.indices, A.data, b, c, f, l, q, _params(**params),
                    name=f"random_s{seed}_n{n}_m{m}")
+
+
+def jumbled(p: Problem, seed: int, dup_frac: float = 0.3) -> Problem:
+    """The same problem with A stored as the reference's CSC input allows but
+    a canonical matrix never is: each column's entries in random row order,
+    and a fraction `dup_frac` of them split into two entries of the same row
+    (a - w and w, w ~ N(0,1)), which every product sums.  The split changes A
+    by the rounding of a - w + w, so a planted optimum stays optimal to
+    roundoff."""
+    rng = np.random.default_rng(seed)
+    Ap, Ai, Ax = [0], [], []
+    for j in range(p.n):
+        r = p.Ai[p.Ap[j]:p.Ap[j + 1]]
+        v = p.Ax[p.Ap[j]:p.Ap[j + 1]]
+        split = rng.random(len(r)) < dup_frac
+        w = rng.standard_normal(len(r))
+        rr = np.concatenate([r, r[split]])
+        vv = np.concatenate([np.where(split, v - w, v), w[split]])
+        order = rng.permutation(len(rr))
+        Ai.append(rr[order])
+        Ax.append(vv[order])
+        Ap.append(Ap[-1] + len(rr))
+    return Problem(p.n, p.m, np.array(Ap), np.concatenate(Ai), np.concatenate(Ax), p.b.copy(), p.c.copy(),
+                   p.f, p.l, list(p.q), dict(p.params), name=p.name + "_jumbled")
 
 
 class DenseFamily:

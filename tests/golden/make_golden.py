@@ -32,7 +32,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
-from pdos_amd.abi import Cone, Data, Params, Sol, read_problem_file, sol_to_numpy, write_problem_file  # noqa: E402
+from pdos_amd.abi import Cone, Data, Params, Problem, Sol, read_problem_file, sol_to_numpy, write_problem_file  # noqa: E402
 from pdos_amd import problems  # noqa: E402
 
 REF = "/root/reference"
@@ -151,6 +151,21 @@ def golden_cases(arrays):
     for j, u in enumerate(problems.uneven_portfolios(seed=3, count=3, na_range=(50, 200),
                                                      k_range=(5, 20), EPS_ABS=1e-3)):
         add_generated(f"uneven_{j}", u)
+    # structural edge cases of the input format: CSC columns in random row
+    # order with duplicated entries, an empty column and empty rows (one of
+    # them a zero-cone row); SOCs only; no SOC; b = 0 (lambda = 1e-15 / (1e-15 + |c|))
+    edge = problems.random_feasible(seed=21, n=120, m=300, nnz=1500, f=5, l=150, nsoc=10, soc_lo=3,
+                                    soc_hi=30, empty_rows=2, empty_cols=1, EPS_ABS=1e-3, MAX_ITERS=3000)
+    add_generated("jumbled_n120_m300", problems.jumbled(edge, seed=1))
+    add_generated("jumbled_n120_m300_norm1", problems.jumbled(edge, seed=1).with_params(NORMALIZE=1))
+    add_generated("soc_only_n80_m200", problems.random_feasible(
+        seed=22, n=80, m=200, nnz=1200, f=0, l=0, nsoc=8, soc_lo=5, soc_hi=40, EPS_ABS=1e-3, MAX_ITERS=3000))
+    add_generated("lp_only_n80_m200", problems.random_feasible(
+        seed=23, n=80, m=200, nnz=1200, f=10, l=190, nsoc=0, EPS_ABS=1e-3, MAX_ITERS=3000))
+    bz = problems.random_feasible(seed=24, n=60, m=150, nnz=800, f=0, l=100, nsoc=5, soc_lo=3, soc_hi=20,
+                                  EPS_ABS=1e-3, MAX_ITERS=400)
+    add_generated("b_zero_n60_m150", Problem(bz.n, bz.m, bz.Ap, bz.Ai, bz.Ax, np.zeros(bz.m), bz.c, bz.f,
+                                             bz.l, bz.q, bz.params, name="b_zero"))
     dense = problems.DenseFamily(seed=2, m=96, n=24, l=64, nsoc=4, qdim=8, EPS_ABS=1e-3)
     B, Cm, _ = dense.sample(5, 2)
     add_generated("dense_m96_n24", dense.instance(B[0], Cm[0]))

@@ -12,6 +12,7 @@ import scipy.sparse as sp
 import golden_data as G
 from oracle import oracle
 from pdos_amd import DeviceArray, problems
+from pdos_amd.abi import Problem
 from pdos_amd.family import Family
 
 pytestmark = pytest.mark.gpu
@@ -241,3 +242,42 @@ def test_family_mixed_relaxed_normalized_verifies_in_fp64():
     F64.run()
     o64 = F64.fetch()
     assert abs(int((o64["status"] == "Solved").sum()) - int((out["status"][:200] == "Solved").sum())) <= 6
+
+
+def test_family_jumbled_csc_with_duplicates():
+    """A stored as the input format allows: each column's rows in random order
+    and 30% of the entries split into two of the same row, including the
+    dense (budget and factor) rows that the mixed engine runs as MFMA tiles.
+    FP64: status, iterations and x, s, y (1e-6) as the oracle's direct plugin
+    on the same stored A; mixed: every Solved point meets the stop test in
+    binary64 against the summed A."""
+    fam = _fam(na=200, k=15, MAX_ITERS=2000)
+    _, Cm = fam.sample(seed=15, count=512)
+    t0 = problems.jumbled(fam.instance(Cm[0]), seed=3)
+    assert np.bincount(t0.Ai, minlength=t0.m).max() >= 64
+    print("family jumbled fp64", flush=True)
+    F = Family(t0, precision="fp64")
+    F.load(Cm[:12])
+    F.run()
+    out = F.fetch()
+    for t in range(12):
+        pt = Problem(t0.n, t0.m, t0.Ap, t0.Ai, t0.Ax, t0.b, Cm[t], t0.f, t0.l, t0.q, t0.params)
+        ref = oracle.solve(pt, "direct")
+        assert out["status"][t] == ref["status"], t
+        assert out["iters"][t] == ref["iters"], (t, out["iters"][t], ref["iters"])
+        for key in ("x", "s", "y"):
+            assert _rel(out[key][t], ref[key]) <= 1e-6, (t, key)
+    print("family jumbled mixed", flush=True)
+    F = Family(t0, precision="mixed")
+    assert F.info()["dense_rows"] > 0
+    F.load(Cm)
+    F.run()
+    mix = F.fetch()
+    solved = int((mix["status"] == "Solved").sum())
+    assert solved >= 0.8 * len(Cm)
+    A = sp.csc_matrix((t0.Ax, t0.Ai, t0.Ap), shape=(t0.m, t0.n))
+    nb = np.linalg.norm(t0.b)
+    for t in np.nonzero(mix["status"] == "Solved")[0]:
+        x, s, y, c = mix["x"][t], mix["s"][t], mix["y"][t], Cm[t]
+        assert np.linalg.norm(A @ x + s - t0.b) < 1.05e-3 * (1 + nb)
+        assert np.linalg.norm(c + A.T @ y) < 1.05e-3 * (1 + np.linalg.norm(c))

@@ -39,7 +39,8 @@ def test_wg_fp64_vs_reference(tag, plugin):
 
 
 @pytest.mark.parametrize("tag", ["random_n300_m900", "portfolio_n101_norm1", "fixture03",
-                                 "portfolio_n101_steps10", "portfolio_n101_cg1"])
+                                 "portfolio_n101_steps10", "portfolio_n101_cg1", "jumbled_n120_m300",
+                                 "soc_only_n80_m200", "b_zero_n60_m150"])
 def test_grid_fp64_vs_reference(tag):
     print(f"case {tag} grid", flush=True)
     out = pdos_amd.solve_problem(G.problem(tag), plugin="indirect", precision="fp64", engine="grid")
@@ -100,7 +101,8 @@ def test_grid_long_rows_vs_oracle():
 
 
 @pytest.mark.parametrize("engine", ["wg", "grid"])
-@pytest.mark.parametrize("tag", ["fixture01", "portfolio_n101_norm0", "random_n300_m900", "uneven_1"])
+@pytest.mark.parametrize("tag", ["fixture01", "portfolio_n101_norm0", "random_n300_m900", "uneven_1",
+                                 "jumbled_n120_m300"])
 def test_mixed_precision_within_eps(tag, engine):
     print(f"case {tag} mixed {engine}", flush=True)
     prob = G.problem(tag)

@@ -54,3 +54,20 @@ def test_family_rejects_ruiz():
     _, Cm = fam.sample(seed=1, count=2)
     with pytest.raises(RuntimeError, match="PDOS_EQUIL_REFERENCE"):
         Family(fam.instance(Cm[0]), precision="mixed", equil="ruiz")
+
+
+def test_ruiz_unsorted_columns_library_matches_oracle():
+    """Columns whose row indices do not ascend, with NORMALIZE 1: the
+    reference's cone collapse (ruiz_equil_common.h:29-66) then indexes k->q
+    past qsize, so there is no reference output.  The library and the oracle
+    both stop the walk of a column at its last cone; their scalings agree bit
+    for bit, and with sorted columns they equal the reference's (golden)."""
+    prob = G.problem("jumbled_n120_m300_norm1")
+    assert prob.params["NORMALIZE"] == 1
+    assert any(np.any(np.diff(prob.Ai[prob.Ap[j]:prob.Ap[j + 1]]) < 0) for j in range(prob.n))
+    d2, e2, l2 = pdos_amd.setup_host(prob, "ruiz")
+    d3, e3, l3 = oracle.setup(prob, equil="ruiz")
+    np.testing.assert_array_equal(d2, d3)
+    np.testing.assert_array_equal(e2, e3)
+    assert l2 == l3
+    assert np.all(np.isfinite(d2)) and np.all(np.isfinite(e2))
