@@ -131,6 +131,21 @@ def test_batch_matches_single():
         _check_fp64(o, G.expected(t, "direct"), f"batch-direct/{t}")
 
 
+@pytest.mark.parametrize("variant", ["", "ruiz"])
+@pytest.mark.parametrize("plugin", ["indirect", "direct"])
+def test_batch_of_every_golden_case(plugin, variant):
+    """Every golden case in one pdos_batch call: members with their own
+    parameters (MAX_ITERS 0, EPS 0, ALPHA 1.8, NORMALIZE, CG_MAX_ITS 1), warm
+    starts, cone mixes and stored-A quirks each match the reference's output
+    for that case alone (status, iterations, x, s, y at 1e-6)."""
+    tags = G.case_names(variant)
+    probs = [G.problem(t, variant) for t in tags]
+    print(f"batch of every golden case {plugin} {variant or 'reference'}", flush=True)
+    outs = pdos_amd.solve_batch(probs, plugin=plugin, precision="fp64", equil=variant or "reference")
+    for t, o in zip(tags, outs):
+        _check_fp64(o, G.expected(t, plugin, variant), f"batch-all/{plugin}/{t}")
+
+
 def test_batch_uneven_vs_oracle():
     probs = problems.uneven_portfolios(seed=31, count=24, na_range=(20, 300), k_range=(3, 30),
                                        EPS_ABS=1e-3, MAX_ITERS=4000)
