@@ -308,6 +308,9 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
           " B of LDS, device has " + std::to_string(C.max_lds) + ")";
     return false;
   }
+  ho.assign(count, 0);
+  for (int i = 0; i < count; ++i)  // per row: 5 residuals, then (indirect) 10 x {CG steps, |q|}
+    if (par[i].VERBOSE) ho[i] = P.zeros(sizeof(double) * (direct ? 5 : 25) * verbose_hist_rows(par[i].MAX_ITERS));
   res_off = P.zeros(sizeof(DevResult) * count);
   inst_off = P.zeros(sizeof(WgInst) * count);
   cinst_off = P.zeros(sizeof(WgInst) * count);
@@ -375,6 +378,9 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
       w.nbl = f.nbl();
     }
     w.res = reinterpret_cast<DevResult*>(base + res_off) + i;
+    w.hist_rows = par[i].VERBOSE ? verbose_hist_rows(par[i].MAX_ITERS) : 0;
+    w.hist = w.hist_rows ? reinterpret_cast<double*>(base + ho[i]) : nullptr;
+    w.cg_hist = w.hist_rows && !direct ? w.hist + (size_t)5 * w.hist_rows : nullptr;
     w.lds_bytes = wg_lds_bytes(s.n, s.m, s.n + s.m, direct, tsz);
   }
   // launch classes by workgroups per CU that the LDS allows (at most
@@ -489,6 +495,13 @@ bool WgBatch::fetch(std::vector<DevResult>& res, std::string& err) {
     if (e == hipSuccess)
       e = hipMemcpyAsync(s.y.data(), base + yo[i], sizeof(double) * s.m, hipMemcpyDeviceToHost, stream);
   }
+  hist.assign(count, {});
+  for (int i = 0; i < count && e == hipSuccess; ++i)
+    if (insts[i].hist_rows) {
+      hist[i].assign((size_t)(direct ? 5 : 25) * insts[i].hist_rows, 0.0);
+      e = hipMemcpyAsync(hist[i].data(), base + ho[i], sizeof(double) * hist[i].size(), hipMemcpyDeviceToHost,
+                         stream);
+    }
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
   if (e != hipSuccess) { err = std::string("download: ") + hipGetErrorString(e); return false; }
   return true;
@@ -496,7 +509,7 @@ bool WgBatch::fetch(std::vector<DevResult>& res, std::string& err) {
 
 bool solve_wg(std::vector<Setup>& S, std::vector<Factor>& F, const std::vector<const Params*>& par,
               bool fp64, bool direct, int device, std::vector<DevResult>& res,
-              double& upload_ms, double& solve_ms, std::string& err) {
+              double& upload_ms, double& solve_ms, std::string& err, std::vector<double>* hist0) {
   WgBatch B;
   B.S.swap(S);
   B.F.swap(F);
@@ -506,6 +519,7 @@ bool solve_wg(std::vector<Setup>& S, std::vector<Factor>& F, const std::vector<c
   B.F.swap(F);
   upload_ms = B.upload_ms;
   solve_ms = B.wall_ms;
+  if (ok && hist0 && !B.hist.empty()) hist0->swap(B.hist[0]);
   return ok;
 }
 
@@ -556,18 +570,47 @@ int env_equil() {
   return p && !std::strcmp(p, "ruiz") ? PDOS_EQUIL_RUIZ : PDOS_EQUIL_REFERENCE;
 }
 
-void print_verbose(const Params* p, const DevResult& r, double lam, double secs) {
+// The VERBOSE log in the reference's layout (pdos.c:79-92, 107, 124-127,
+// 306-335; direct/private.c:157): banner, factorization time (direct),
+// lambda, the column header and its rule, a row for every check i with
+// i % 10 == 0 that did not stop the loop (from the engine's history), the
+// final row (updates done, last check's residuals) and the solve time.
+void print_verbose(const Params* p, const DevResult& r, double lam, double secs, const std::vector<double>& hist,
+                   double factor_secs) {
   if (!p->VERBOSE) return;
-  // Banner and summary row in the format of pdos.c:79-92,306-335.
-  std::printf("\nPDOS - A Primal-Dual Operator Splitting for Cone Programming (MI355X).\n\n");
+  static const char* const kHead[6] = {" iter", " ||Ax+s-b||", " ||A'y+c||", "    c'x    ", "   -b'y    ",
+                                       "    eta   "};
+  auto row = [&](long i, const double* v) {
+    std::printf("%*li | ", (int)std::strlen(kHead[0]), i);
+    for (int k = 0; k < 4; ++k) std::printf("%*.3e   ", (int)std::strlen(kHead[k + 1]), v[k]);
+    std::printf("%*.3e   \n", (int)std::strlen(kHead[5]), v[4]);
+  };
+  std::printf("\nPDOS - A Primal-Dual Operator Splitting for Cone Programming.\n");
+  std::printf("       MI355X (gfx950) build: HIP kernels, FP64 or mixed precision.\n\n");
+  if (factor_secs >= 0) std::printf("Factorization time: %4.8fs\n", factor_secs);
   std::printf("lambda: %5.3e\n\n", lam);
-  std::printf(" iter |  ||Ax+s-b||    ||A'y+c||       c'x          -b'y          eta   \n");
-  std::printf("%*i | ", 5, r.iters);
-  std::printf("%*.3e   ", 11, r.p_res);
-  std::printf("%*.3e   ", 10, r.d_res);
-  std::printf("%*.3e   ", 11, r.p_obj);
-  std::printf("%*.3e   ", 11, r.d_obj);
-  std::printf("%*.3e   \n", 9, r.eta);
+  size_t len = 0;
+  for (int k = 0; k < 5; ++k) {
+    std::printf(k == 0 ? "%s | " : "%s   ", kHead[k]);
+    len += std::strlen(kHead[k]) + 3;
+  }
+  std::printf("%s\n", kHead[5]);
+  len += std::strlen(kHead[5]);
+  std::printf("%s\n", std::string(len + 3, '=').c_str());
+  // history: rows x 5 residuals, then (workgroup engine, indirect plugin)
+  // rows x 10 updates' {CG steps, final |q|}, printed after each update as
+  // indirect/private.c:100 does
+  const size_t rows = (size_t)verbose_hist_rows(p->MAX_ITERS);
+  const bool cg = hist.size() == 25 * rows;
+  for (long i = 0; i < r.iters && (size_t)i < 10 * rows && hist.size() >= 5 * rows; ++i) {
+    if (i % 10 == 0) row(i, &hist[(size_t)(i / 10) * 5]);
+    if (cg) {
+      const double* c = &hist[5 * rows + (size_t)i * 2];
+      std::printf("terminating cg residual = %4f, took %li itns\n", c[1], (long)c[0]);
+    }
+  }
+  const double last[5] = {r.p_res, r.d_res, r.p_obj, r.d_obj, r.eta};
+  row(r.iters, last);
   std::printf("Total solve time is %4.8fs\n", secs);
 }
 
@@ -614,10 +657,13 @@ Sol* pdos_ex(const Data* d, const Cone* k, const PdosOptions* opt, PdosInfo* inf
     return nullptr;
   }
   const bool direct = o.plugin == PDOS_DIRECT;
-  double lnz = 0;
+  double lnz = 0, factor_secs = -1.0;
   if (direct) {
     F.resize(1);
-    if (!build_factor(S[0], {}, F[0], err)) {
+    const double tf = now_ms();
+    const bool fok = build_factor(S[0], {}, F[0], err);
+    factor_secs = (now_ms() - tf) * 1e-3;
+    if (!fok) {
       g_err = err;
       return nullptr;
     }
@@ -626,6 +672,7 @@ Sol* pdos_ex(const Data* d, const Cone* k, const PdosOptions* opt, PdosInfo* inf
   double t1 = now_ms();
   std::vector<const Params*> par{d->p};
   std::vector<DevResult> res;
+  std::vector<double> hist;
   double up = 0, sv = 0;
   int engine = PDOS_ENGINE_WG;
   bool ok;
@@ -636,9 +683,9 @@ Sol* pdos_ex(const Data* d, const Cone* k, const PdosOptions* opt, PdosInfo* inf
       g_err = "the grid engine runs the indirect plugin only";
       return nullptr;
     }
-    ok = solve_grid(S[0], *d->p, o.precision == PDOS_FP64, o.device, res, up, sv, err);
+    ok = solve_grid(S[0], *d->p, o.precision == PDOS_FP64, o.device, res, up, sv, err, &hist);
   } else {
-    ok = solve_wg(S, F, par, o.precision == PDOS_FP64, direct, o.device, res, up, sv, err);
+    ok = solve_wg(S, F, par, o.precision == PDOS_FP64, direct, o.device, res, up, sv, err, &hist);
   }
   if (!ok) {
     g_err = err;
@@ -651,7 +698,7 @@ Sol* pdos_ex(const Data* d, const Cone* k, const PdosOptions* opt, PdosInfo* inf
   g_info = inf;
   g_info_valid = true;
   if (info) *info = inf;
-  print_verbose(d->p, res[0], S[0].lambda, sv * 1e-3);
+  print_verbose(d->p, res[0], S[0].lambda, sv * 1e-3, hist, factor_secs);
   return sol;
 }
 

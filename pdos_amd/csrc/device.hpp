@@ -59,8 +59,17 @@ struct WgInst {
   const int* bl_rows;
   DevResult* res;
   int lds_bytes;           // dynamic LDS this instance needs
-  int pad2;
+  int hist_rows;           // rows of hist (0: no history)
+  double* hist;            // VERBOSE: residuals (p_res, d_res, p_obj, d_obj, eta) of checks 0, 10, 20, ...
+  double* cg_hist;         // VERBOSE, indirect: per update {CG steps, final |q|} (hist_rows * 10 rows)
 };
+
+// Rows of the VERBOSE residual history for a MAX_ITERS: one per tenth check
+// (pdos.c:107 prints checks i % 10 == 0), at most 100,000.
+inline int verbose_hist_rows(long max_iters) {
+  const long r = (max_iters + 9) / 10;
+  return (int)(r < 1 ? 1 : (r > 100000 ? 100000 : r));
+}
 
 constexpr int kWgThreads = 256;
 constexpr int kWgChunk = 2048;        // entries per SpMV row block (products staged in LDS)

@@ -56,6 +56,8 @@ struct WgBatch {
   void* dbuf = nullptr;
   size_t bytes = 0, res_off = 0, inst_off = 0;
   std::vector<size_t> xo, so, yo;
+  std::vector<size_t> ho;           // VERBOSE residual history offsets (rows in insts[i].hist_rows)
+  std::vector<std::vector<double>> hist;  // fetched histories (VERBOSE instances)
   std::vector<WgInst> insts;  // host copy of the device records
   int lds = 0;
   // Launch classes: instances grouped by how many of their workgroups fit
@@ -81,11 +83,13 @@ struct WgBatch {
 
 bool solve_wg(std::vector<Setup>& S, std::vector<Factor>& F, const std::vector<const Params*>& par,
               bool fp64, bool direct, int device, std::vector<DevResult>& res, double& upload_ms,
-              double& solve_ms, std::string& err);
+              double& solve_ms, std::string& err, std::vector<double>* hist0 = nullptr);
 
 // Whole-GPU engine for one large instance (indirect plugin).
 bool grid_preferred(const Setup& S, bool fp64);
+// hist (optional): with par.VERBOSE, the residuals of checks 0, 10, 20, ...
+// (5 per row, verbose_hist_rows(MAX_ITERS) rows)
 bool solve_grid(Setup& S, const Params& par, bool fp64, int device, std::vector<DevResult>& res,
-                double& upload_ms, double& solve_ms, std::string& err);
+                double& upload_ms, double& solve_ms, std::string& err, std::vector<double>* hist = nullptr);
 
 }  // namespace pdos_amd

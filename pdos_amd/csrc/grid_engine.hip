@@ -74,6 +74,8 @@ struct GArgs {
   double* p_pu;    // [nb_at]            p'u
   double* p_q;     // [nb_n]             |q|^2 after the update
   GridScal* sc;
+  double* hist;    // VERBOSE: residuals of checks 0, 10, 20, ... (5 each), or null
+  int hist_rows;
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -240,8 +242,16 @@ __global__ __launch_bounds__(GT) SPMV_OCC void k_cginit(GArgs<T> a) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     GridScal* sc = a.sc;
     sc->pres = pres; sc->dres = dres; sc->pobj = pobj; sc->dobj = dobj; sc->eta = eta;
-    if (stop) { sc->done = 1; sc->status = 0; }
-    else sc->iters += 1;
+    if (stop) {
+      sc->done = 1; sc->status = 0;
+    } else {
+      const int i = sc->iters;
+      if (a.hist && i % 10 == 0 && i / 10 < a.hist_rows) {
+        double* h = a.hist + (size_t)(i / 10) * 5;
+        h[0] = pres; h[1] = dres; h[2] = pobj; h[3] = dobj; h[4] = eta;
+      }
+      sc->iters += 1;
+    }
   }
   if (stop) return;
   const T lam = T(a.lambda);
@@ -447,7 +457,7 @@ size_t put_as(Packer& P, const std::vector<double>& v) {
 
 template <typename T>
 bool run_grid(Setup& S, const Params& par, int device, std::vector<DevResult>& res, double& upload_ms,
-              double& solve_ms, std::string& err) {
+              double& solve_ms, std::string& err, std::vector<double>* hist) {
   Ctx& C = ctx();
   if (!C.ensure(device, err)) return false;
   const double t0 = now_ms();
@@ -476,6 +486,8 @@ bool run_grid(Setup& S, const Params& par, int device, std::vector<DevResult>& r
   sc0.status = 1;
   sc0.pres = sc0.dres = sc0.pobj = sc0.dobj = sc0.eta = -1.0;
   const size_t o_sc = P.put(&sc0, sizeof(sc0));
+  const int hrows = par.VERBOSE ? verbose_hist_rows(par.MAX_ITERS) : 0;
+  const size_t o_hist = P.zeros(sizeof(double) * 5 * (size_t)std::max(hrows, 1));
   if (!C.reserve(P.buf.size(), std::max(P.buf.size(), (size_t)(n + 2 * m) * 8 + 4096), err)) return false;
   unsigned char* base = static_cast<unsigned char*>(C.dbuf);
   auto at = [&](size_t off) { return base + off; };
@@ -498,6 +510,8 @@ bool run_grid(Setup& S, const Params& par, int device, std::vector<DevResult>& r
   a.p_res = (double*)at(o_pres); a.p_rho = (double*)at(o_prho);
   a.p_pu = (double*)at(o_ppu); a.p_q = (double*)at(o_pqn);
   a.sc = (GridScal*)at(o_sc);
+  a.hist = hrows ? (double*)at(o_hist) : nullptr;
+  a.hist_rows = hrows;
   std::memcpy(C.hbuf, P.buf.data(), P.buf.size());
   hipError_t e = hipMemcpyAsync(C.dbuf, C.hbuf, P.buf.size(), hipMemcpyHostToDevice, C.stream);
   if (e == hipSuccess) e = hipStreamSynchronize(C.stream);
@@ -539,6 +553,10 @@ bool run_grid(Setup& S, const Params& par, int device, std::vector<DevResult>& r
   e = hipMemcpy(hx.data(), a.x, n * sizeof(T), hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(hs.data(), a.s, m * sizeof(T), hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(hy.data(), a.y, m * sizeof(T), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && hist) {
+    hist->assign((size_t)5 * hrows, 0.0);
+    if (hrows) e = hipMemcpy(hist->data(), a.hist, sizeof(double) * 5 * hrows, hipMemcpyDeviceToHost);
+  }
   if (e != hipSuccess) { err = std::string("download: ") + hipGetErrorString(e); return false; }
   for (int i = 0; i < n; ++i) S.x[i] = double(hx[i]);
   for (int j = 0; j < m; ++j) { S.s[j] = double(hs[j]); S.y[j] = double(hy[j]); }
@@ -561,9 +579,9 @@ bool grid_preferred(const Setup& S, bool fp64) {
 }
 
 bool solve_grid(Setup& S, const Params& par, bool fp64, int device, std::vector<DevResult>& res,
-                double& upload_ms, double& solve_ms, std::string& err) {
-  return fp64 ? run_grid<double>(S, par, device, res, upload_ms, solve_ms, err)
-              : run_grid<float>(S, par, device, res, upload_ms, solve_ms, err);
+                double& upload_ms, double& solve_ms, std::string& err, std::vector<double>* hist) {
+  return fp64 ? run_grid<double>(S, par, device, res, upload_ms, solve_ms, err, hist)
+              : run_grid<float>(S, par, device, res, upload_ms, solve_ms, err, hist);
 }
 
 }  // namespace pdos_amd

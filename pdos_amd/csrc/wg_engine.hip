@@ -310,6 +310,10 @@ __global__ __launch_bounds__(NT) void k_wg_solve(const WgInst* __restrict__ inst
       status = 0;
       break;
     }
+    if (I.hist && tid == 0 && it % 10 == 0 && it / 10 < I.hist_rows) {
+      double* h = I.hist + (size_t)(it / 10) * 5;
+      h[0] = pres; h[1] = dres; h[2] = pobj; h[3] = dobj; h[4] = eta;
+    }
 
     if (!DIRECT) {
       // ---- CG on (I + A'A) x = x - lam c + A'(b - s - lam y) ----
@@ -324,6 +328,7 @@ __global__ __launch_bounds__(NT) void k_wg_solve(const WgInst* __restrict__ inst
       double rho = r1[0];
       T beta = T(0);
       int k = 0;
+      double qlast = 0.0;  // |q|^2 after the last step (indirect/private.c:48,85)
       if (rho > I.cg_tol_sq) {
         while (k < I.cg_max_its) {
           spmv_blocks<T>(rblk, nrb, rp, ri, rx, prod, red, slot, [&](int c) { return beta * p[c] + q[c]; },
@@ -348,12 +353,17 @@ __global__ __launch_bounds__(NT) void k_wg_solve(const WgInst* __restrict__ inst
           }
           block_sum<1>(r3, red, slot);
           ++k;
+          qlast = r3[0];
           if (r3[0] < I.cg_tol_sq) break;
           beta = T(r3[0] / rho);
           rho = r3[0];
         }
       }
       cg_total += k;
+      if (I.cg_hist && tid == 0 && it < I.hist_rows * 10) {
+        I.cg_hist[(size_t)it * 2] = double(k);
+        I.cg_hist[(size_t)it * 2 + 1] = sqrt(qlast);
+      }
     } else {
       // ---- LDL' solve of the KKT system, permuted into bp ----
       const int N = I.N;

@@ -6,6 +6,9 @@ Mixed mode (binary32 bulk, binary64 norms): status Solved and the returned
 point within EPS_ABS-scaled distance of the FP64 optimum.
 """
 import ctypes as C
+import json
+import os
+import re
 import numpy as np
 import pytest
 
@@ -193,3 +196,58 @@ def test_device_copy_kernel():
     b = pdos_amd.DeviceArray(shape=(1003,))
     b.copy_from(a)
     np.testing.assert_array_equal(b.numpy(), src)
+
+
+_VERBOSE_LOGS = os.path.join(G.GOLDEN, "verbose_logs.json")
+_NUMBER = re.compile(r"[-+]?\d+\.\d+(?:e[+-]\d+)?|\d+")
+
+
+def _capture_fd1(fn):
+    import ctypes
+    import tempfile
+    libc = ctypes.CDLL(None)
+    with tempfile.TemporaryFile() as tmp:
+        libc.fflush(None)
+        saved = os.dup(1)
+        os.dup2(tmp.fileno(), 1)
+        try:
+            fn()
+            libc.fflush(None)
+        finally:
+            os.dup2(saved, 1)
+            os.close(saved)
+        tmp.seek(0)
+        return tmp.read().decode()
+
+
+@pytest.mark.parametrize("key", sorted(json.load(open(_VERBOSE_LOGS))))
+def test_verbose_log_matches_reference(key):
+    """pdos() with VERBOSE 1 prints the reference's log (pdos.c:79-127,
+    306-335; direct/private.c:157; indirect/private.c:100): the same lines in
+    the same layout -- banner, factorization time (direct), lambda, header and
+    rule, a row for every tenth check, a CG line per update (indirect), the
+    final row, the solve time.  Fields are fixed width, so every line has the
+    reference's length; numbers agree to the printed digits (one unit of the
+    last digit), counts exactly.  Times and the banner's credit line differ."""
+    tag, plugin = key.split("/")
+    ref = json.load(open(_VERBOSE_LOGS))[key].splitlines()
+    prob = G.problem(tag).with_params(VERBOSE=1)
+    print(f"verbose log {key}", flush=True)
+    got = _capture_fd1(lambda: pdos_amd.solve_problem(prob, plugin=plugin, precision="fp64", engine="wg"))
+    got = got.splitlines()
+    assert len(got) == len(ref), (len(got), len(ref), got[:12])
+    for k, (g, r) in enumerate(zip(got, ref)):
+        if k == 2:  # the banner's second line: the build's own
+            continue
+        if r.startswith(("Factorization time:", "Total solve time is")):
+            assert g.split(":")[0].split(" is")[0] == r.split(":")[0].split(" is")[0], (k, g, r)
+            continue
+        assert len(g) == len(r), (k, g, r)
+        assert _NUMBER.sub("#", g).split() == _NUMBER.sub("#", r).split(), (k, g, r)
+        for a, b in zip(_NUMBER.findall(g), _NUMBER.findall(r)):
+            if "." not in b:
+                assert a == b, (k, g, r)
+            else:
+                digits = 1e-3 if "e" in b else 1e-6  # %.3e: 4 significant digits; %4f: 6 decimals
+                scale = abs(float(b)) if "e" in b else 1.0
+                assert abs(float(a) - float(b)) <= 1.01 * digits * scale + 1e-300, (k, g, r)
