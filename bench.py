@@ -67,10 +67,9 @@ def verify_batch(A, B, Cm, X, Y, S, status, eps):
     dres = np.linalg.norm(Dd, axis=1)
     po = np.einsum("ij,ij->i", Cm, X)
     do = -np.einsum("ij,ij->i", Bm, Y)
-    slack = 1.001
-    ok = ((pres < slack * eps * (1 + np.linalg.norm(Bm, axis=1))) &
-          (dres < slack * eps * (1 + np.linalg.norm(Cm, axis=1))) &
-          (np.abs(po - do) < slack * eps * (1 + np.abs(po) + np.abs(do))) & (status == "Solved"))
+    ok = ((pres < eps * (1 + np.linalg.norm(Bm, axis=1))) &
+          (dres < eps * (1 + np.linalg.norm(Cm, axis=1))) &
+          (np.abs(po - do) < eps * (1 + np.abs(po) + np.abs(do))) & (status == "Solved"))
     return int(ok.sum())
 
 
@@ -88,7 +87,9 @@ def verify_list(probs, outs):
 # -------------------------------------------------------------- CPU baseline
 
 def _cpu_worker(args):
-    kind, plugin, blobs = args
+    kind, plugin, blobs, core = args
+    if core is not None:
+        os.sched_setaffinity(0, {core})  # one worker per core
     import ctypes as C
     sys.path.insert(0, ROOT)
     from pdos_amd.abi import Cone, Data, Problem, Sol
@@ -121,6 +122,25 @@ def _cpu_worker(args):
     return time.perf_counter() - t0, solved
 
 
+# The GPU box allots each GPU a share of its host cores (16 per GPU): the
+# baseline runs one pinned worker process per core of that share; nproc and
+# the CPU model are reported beside it.
+HOST_CORES_PER_GPU = 16
+
+
+def host_cpu_info():
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity_cores": len(os.sched_getaffinity(0)), "cpu_model": model,
+            "cores_per_gpu_share": HOST_CORES_PER_GPU}
+
+
 def cpu_baseline(probs, plugin, procs, cap=0, gpu_mean_iters=0.0, gpu_solved_frac=1.0):
     """The reference C library (oracle/_ref, built from the reference sources)
     -- or the oracle port if absent -- on the given instances, split over
@@ -134,15 +154,18 @@ def cpu_baseline(probs, plugin, procs, cap=0, gpu_mean_iters=0.0, gpu_solved_fra
     blobs = [dict(n=p.n, m=p.m, Ap=p.Ap, Ai=p.Ai, Ax=p.Ax, b=p.b, c=p.c, f=p.f, l=p.l, q=p.q,
                   params=dict(p.params, **over)) for p in probs]
     procs = max(1, min(procs, len(blobs)))
+    cores = sorted(os.sched_getaffinity(0))
+    pin = [cores[i % len(cores)] if len(cores) >= procs else None for i in range(procs)]
     shards = [blobs[i::procs] for i in range(procs)]
     ctx = mp.get_context("fork")
     with ctx.Pool(procs) as pool:
         if not cap:
-            pool.map(_cpu_worker, [(kind, plugin, s[:1]) for s in shards if s])  # warm the workers
+            pool.map(_cpu_worker, [(kind, plugin, s[:1], pin[i]) for i, s in enumerate(shards) if s])  # warm
         t0 = time.perf_counter()
-        res = pool.map(_cpu_worker, [(kind, plugin, s) for s in shards if s])
+        res = pool.map(_cpu_worker, [(kind, plugin, s, pin[i]) for i, s in enumerate(shards) if s])
         wall = time.perf_counter() - t0
-    out = {"unit": "SOCPs/s", "cores": procs, "kind": kind, "plugin": plugin}
+    out = {"unit": "SOCPs/s", "cores": procs, "kind": kind, "plugin": plugin, "pinned": all(c is not None for c in pin)}
+    out.update(host_cpu_info())
     if not cap:
         solved = sum(r[1] for r in res)
         out["value"] = round(solved / wall, 3)
@@ -382,7 +405,7 @@ def run_family(args):
     v_cold = verify_batch(A, fam.b, C_te, cold["x"], cold["y"], cold["s"], cold["status"], 1e-3)
     solved = int((warm["status"] == "Solved").sum())
     solved_cold = int((cold["status"] == "Solved").sum())
-    value = solved / wall
+    value = v_warm / wall  # FP64-verified Solved instances per second
     log(f"warm: {solved}/{n_te} Solved ({v_warm} verified), {value:.0f} SOCPs/s, mean iters "
         f"{warm['iters'].mean():.1f}, fallbacks {int(warm['fallback'].sum())}; cold: {solved_cold / cold_wall:.0f} "
         f"SOCPs/s, mean iters {cold['iters'].mean():.1f}")
@@ -443,7 +466,7 @@ def run_dense(args):
            "max_iters": int(out["iters"].max()), "max_rel_obj_err_vs_planted": float(gap.max())}
     extra = {"kernels": {k: {"ms": round(v[0], 3), "calls": v[1]} for k, v in kt.items()}}
     probs = [dense.instance(B[i], Cm[i]) for i in range(min(args.cpu_sample, 16))]
-    return solved / wall, wall, "f32+bf16" if prec == "mixed" else "f64", config, roof, acc, extra, probs, "direct"
+    return ver / wall, wall, "f32+bf16" if prec == "mixed" else "f64", config, roof, acc, extra, probs, "direct"
 
 
 def run_wg(args, probs, config, plugin):
@@ -468,7 +491,7 @@ def run_wg(args, probs, config, plugin):
            "mean_iters": float(iters.mean()), "max_iters": int(iters.max())}
     extra = {"plain_gpu": plain_gpu(probs[:args.plain_sample], plugin, prec) if args.plain_sample else None,
              "prepare_s": round(prep, 3), "host_threads": int(os.environ.get("PDOS_HOST_THREADS", os.cpu_count()))}
-    return solved / wall, wall, "f64" if prec == "fp64" else "f32", config, roof, acc, extra, \
+    return acc["verified_fp64"] / wall, wall, "f64" if prec == "fp64" else "f32", config, roof, acc, extra, \
         probs[:args.cpu_sample], plugin
 
 
@@ -495,7 +518,7 @@ def run_single(args, p, config, plugin):
         roof = wg_roofline([p], [out], plugin, 8 if prec == "fp64" else 4, out["info"]["solve_ms"])
     acc = {"solved": solved, "count": 1, "verified_fp64": verify_list([p], [out]),
            "mean_iters": float(out["info"]["iters"]), "max_iters": int(out["info"]["iters"])}
-    return solved / wall, wall, "f64" if prec == "fp64" else "f32", config, roof, acc, {}, [p], plugin
+    return acc["verified_fp64"] / wall, wall, "f64" if prec == "fp64" else "f32", config, roof, acc, {}, [p], plugin
 
 
 def calibration_copy(mib=256):
@@ -559,7 +582,7 @@ def main():
 
     cpu = None
     if not args.no_cpu and cpu_probs:
-        procs = max(1, min(16, os.cpu_count() or 1))
+        procs = max(1, min(HOST_CORES_PER_GPU, len(os.sched_getaffinity(0))))
         if w in ("random", "dense"):  # bounded samples: a full CPU solve takes minutes (random) or hours (dense)
             cpu = cpu_baseline(cpu_probs, "indirect", procs, cap=10 if w == "random" else 3,
                                gpu_mean_iters=acc["mean_iters"], gpu_solved_frac=acc["solved"] / acc["count"])

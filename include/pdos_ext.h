@@ -99,6 +99,8 @@ PdosBatchHandle *pdos_batch_prepare(const Data *const *ds, const Cone *const *ks
 int pdos_batch_run(PdosBatchHandle *h, double *kernel_ms);
 int pdos_batch_fetch(PdosBatchHandle *h, Sol **sols, PdosInfo *infos);
 void pdos_batch_free(PdosBatchHandle *h);
+/* Device memory a prepared batch holds (bytes; 0 for NULL). */
+size_t pdos_batch_device_bytes(const PdosBatchHandle *h);
 
 /* Warm start for every instance of a prepared batch from device memory:
  * instance i reads [x (n), y (m), s (m)] in original units (float) at

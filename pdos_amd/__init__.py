@@ -48,7 +48,7 @@ class PdosInfo(C.Structure):
 # Every symbol include/pdos.h and include/pdos_ext.h declare.
 EXPORTS = ["pdos", "freeData", "freeSol", "tic", "toc", "tocq", "pdos_default_options",
            "pdos_ex", "pdos_last_info", "pdos_batch", "pdos_batch_prepare", "pdos_batch_run",
-           "pdos_batch_fetch", "pdos_batch_free", "pdos_batch_set_warm_dev", "pdos_device_alloc",
+           "pdos_batch_fetch", "pdos_batch_free", "pdos_batch_device_bytes", "pdos_batch_set_warm_dev", "pdos_device_alloc",
            "pdos_device_free", "pdos_device_upload", "pdos_device_download", "pdos_device_copy",
            "pdos_mlp_create",
            "pdos_mlp_free", "pdos_mlp_num_params", "pdos_mlp_get_params", "pdos_mlp_set_params",
@@ -88,6 +88,8 @@ def lib(plugin: str = "indirect"):
         L.pdos_batch_fetch.restype = C.c_int
         L.pdos_batch_fetch.argtypes = [C.c_void_p, C.POINTER(C.POINTER(Sol)), C.POINTER(PdosInfo)]
         L.pdos_batch_free.argtypes = [C.c_void_p]
+        L.pdos_batch_device_bytes.restype = C.c_size_t
+        L.pdos_batch_device_bytes.argtypes = [C.c_void_p]
         L.pdos_setup_host.restype = C.c_int
         L.pdos_setup_host.argtypes = [C.POINTER(Data), C.POINTER(Cone), C.c_int, C.POINTER(C.c_double),
                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -249,6 +251,9 @@ class BatchHandle:
         if self.L.pdos_batch_set_warm_dev(self.h, dev_ptr, ld, int(check)) != 0:
             raise RuntimeError(self.L.pdos_last_error().decode())
 
+    def device_bytes(self) -> int:
+        return int(self.L.pdos_batch_device_bytes(self.h))
+
     def close(self):
         if self.h:
             self.L.pdos_batch_free(self.h)
@@ -301,10 +306,11 @@ class DeviceArray:
             pass
 
 
-def solve(c, A, b, dims, opts=None, x0=None, y0=None, s0=None, plugin="indirect",
-          precision="fp64", equil="reference"):
-    """The reference binding's solve() (python/pdosmodule.c:27-344); equil
-    "ruiz" selects the scaling of the reference's ruiz_equil_common.h."""
+def binding_problem(c, A, b, dims, opts=None, x0=None, y0=None, s0=None) -> Problem:
+    """The Problem the reference binding builds from its arguments
+    (python/pdosmodule.c:63-311): defaults MAX_ITERS 2000, EPS_ABS 5e-3,
+    ALPHA 1.0, VERBOSE 1, CG_MAX_ITS 20, CG_TOL 1e-4 (:67-74), NORMALIZE 0
+    (README.md:89; the binding leaves it unset); an unknown option raises."""
     import scipy.sparse as sp
     A = sp.csc_matrix(A)
     m, n = A.shape
@@ -319,4 +325,13 @@ def solve(c, A, b, dims, opts=None, x0=None, y0=None, s0=None, plugin="indirect"
     p = Problem(n, m, A.indptr, A.indices, A.data, np.asarray(b, float).ravel(),
                 np.asarray(c, float).ravel(), int(dims.get("f", 0)), int(dims.get("l", 0)), q,
                 params, x0, y0, s0)
+    return p
+
+
+def solve(c, A, b, dims, opts=None, x0=None, y0=None, s0=None, plugin="indirect",
+          precision="fp64", equil="reference"):
+    """The reference binding's solve() (python/pdosmodule.c:27-344): returns
+    a dict with 'x', 's', 'y', 'status' (and 'info'); equil "ruiz" selects
+    the scaling of the reference's ruiz_equil_common.h."""
+    p = binding_problem(c, A, b, dims, opts, x0, y0, s0)
     return solve_problem(p, plugin, precision, equil=equil)

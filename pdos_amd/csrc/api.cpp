@@ -310,7 +310,7 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
   }
   ho.assign(count, 0);
   for (int i = 0; i < count; ++i)  // per row: 5 residuals, then (indirect) 10 x {CG steps, |q|}
-    if (par[i].VERBOSE) ho[i] = P.zeros(sizeof(double) * (direct ? 5 : 25) * verbose_hist_rows(par[i].MAX_ITERS));
+    if (want_hist && par[i].VERBOSE) ho[i] = P.zeros(sizeof(double) * (direct ? 5 : 25) * verbose_hist_rows(par[i].MAX_ITERS));
   res_off = P.zeros(sizeof(DevResult) * count);
   inst_off = P.zeros(sizeof(WgInst) * count);
   cinst_off = P.zeros(sizeof(WgInst) * count);
@@ -378,7 +378,7 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
       w.nbl = f.nbl();
     }
     w.res = reinterpret_cast<DevResult*>(base + res_off) + i;
-    w.hist_rows = par[i].VERBOSE ? verbose_hist_rows(par[i].MAX_ITERS) : 0;
+    w.hist_rows = want_hist && par[i].VERBOSE ? verbose_hist_rows(par[i].MAX_ITERS) : 0;
     w.hist = w.hist_rows ? reinterpret_cast<double*>(base + ho[i]) : nullptr;
     w.cg_hist = w.hist_rows && !direct ? w.hist + (size_t)5 * w.hist_rows : nullptr;
     w.lds_bytes = wg_lds_bytes(s.n, s.m, s.n + s.m, direct, tsz);
@@ -514,6 +514,7 @@ bool solve_wg(std::vector<Setup>& S, std::vector<Factor>& F, const std::vector<c
   B.S.swap(S);
   B.F.swap(F);
   for (const Params* p : par) B.par.push_back(*p);
+  B.want_hist = hist0 != nullptr && B.S.size() == 1;
   bool ok = B.prepare(device, fp64, direct, true, err) && B.run(err) && B.fetch(res, err);
   B.S.swap(S);
   B.F.swap(F);
@@ -805,6 +806,8 @@ int pdos_batch_fetch(PdosBatchHandle* h, Sol** sols, PdosInfo* infos) {
 }
 
 void pdos_batch_free(PdosBatchHandle* h) { delete h; }
+
+size_t pdos_batch_device_bytes(const PdosBatchHandle* h) { return h ? h->B.bytes : 0; }
 
 int pdos_batch_set_warm_dev(PdosBatchHandle* h, const float* dW, long ld, int check) {
   if (!h) return -1;

@@ -51,6 +51,9 @@ struct WgBatch {
   std::vector<Params> par;
   int device = 0;
   bool fp64 = true, direct = false, own_mem = true;
+  // VERBOSE residual histories are kept only for a single pdos()/pdos_ex()
+  // instance (the only path that prints them)
+  bool want_hist = false;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   void* dbuf = nullptr;

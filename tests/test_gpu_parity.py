@@ -251,3 +251,39 @@ def test_verbose_log_matches_reference(key):
                 digits = 1e-3 if "e" in b else 1e-6  # %.3e: 4 significant digits; %4f: 6 decimals
                 scale = abs(float(b)) if "e" in b else 1.0
                 assert abs(float(a) - float(b)) <= 1.01 * digits * scale + 1e-300, (k, g, r)
+
+
+@pytest.mark.parametrize("plugin", ["indirect", "direct"])
+def test_python_binding_solve(plugin):
+    """pdos_amd.solve (the reference binding's solve(c, A, b, dims, opts),
+    python/pdosmodule.c:27-344): the keys it returns, the fixture's own
+    options against the reference's golden outputs, and the binding's
+    defaults (:67-74, NORMALIZE 0) against the oracle."""
+    import scipy.sparse as sp
+    tag = "fixture01"
+    prob = G.problem(tag)
+    A = sp.csc_matrix((prob.Ax, prob.Ai, prob.Ap), shape=(prob.m, prob.n))
+    dims = {"f": prob.f, "l": prob.l, "q": list(prob.q)}
+    print(f"case {tag} binding {plugin}", flush=True)
+    out = pdos_amd.solve(prob.c, A, prob.b, dims, opts=dict(prob.params, VERBOSE=0), plugin=plugin)
+    assert {"x", "s", "y", "status"} <= set(out)
+    _check_fp64(out, G.expected(tag, plugin), f"{tag}/binding/{plugin}")
+    out = pdos_amd.solve(prob.c, A, prob.b, dims, plugin=plugin)
+    exp = oracle.solve(prob.with_params(MAX_ITERS=2000, CG_MAX_ITS=20, EPS_ABS=5e-3, ALPHA=1.0, CG_TOL=1e-4,
+                                        VERBOSE=0, NORMALIZE=0), plugin)
+    _check_fp64(out, exp, f"{tag}/binding-defaults/{plugin}")
+
+
+def test_batch_verbose_keeps_no_history():
+    """VERBOSE instances in a batch: the batch path prints nothing, so it keeps
+    no residual history -- the device memory of a prepared batch does not
+    depend on MAX_ITERS."""
+    base = problems.portfolio(seed=0, VERBOSE=1)
+    sizes = []
+    for mi in (2000, 2_000_000):
+        probs = [base.with_params(MAX_ITERS=mi, VERBOSE=1) for _ in range(8)]
+        print(f"case verbose batch MAX_ITERS {mi}", flush=True)
+        h = pdos_amd.BatchHandle(probs, plugin="indirect")
+        sizes.append(h.device_bytes())
+        h.close()
+    assert sizes[0] == sizes[1] > 0, sizes
