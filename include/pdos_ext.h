@@ -177,6 +177,16 @@ int pdos_mlp_train(PdosMlp *P, const float *X, const float *T, int rows, int epo
                    float lr, unsigned long seed, double *last_loss);
 int pdos_mlp_predict(PdosMlp *P, const float *X, int rows, float *Y, double *kernel_ms);
 int pdos_mlp_predict_dev(PdosMlp *P, const float *dX, int rows, float *dY, int ldy);
+/* pdos_mlp_predict_dev that allocates nothing (no counterpart in the
+ * reference): for callers that cannot allocate device memory -- code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * The forward pass's activations live in a fixed 32 MiB device array of the
+ * library, one per GPU (held until the process ends); rows are processed in
+ * chunks that fit it (2,176 rows for the 501-512-512-2545 network).  A
+ * network whose 64 rows do not fit is refused (-1, pdos_mlp_last_error).
+ * Results equal pdos_mlp_predict_dev's bit for bit.  Calls that overlap on
+ * one GPU (several threads or networks) race on the array. */
+int pdos_mlp_predict_dev_noalloc(PdosMlp *P, const float *dX, int rows, float *dY, int ldy);
 const char *pdos_mlp_last_error(void);
 
 /* Host-side check of the direct plugin's factorization: builds the scaled

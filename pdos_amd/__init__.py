@@ -53,7 +53,7 @@ EXPORTS = ["pdos", "freeData", "freeSol", "tic", "toc", "tocq", "pdos_default_op
            "pdos_mlp_create",
            "pdos_mlp_free", "pdos_mlp_num_params", "pdos_mlp_get_params", "pdos_mlp_set_params",
            "pdos_mlp_set_affine", "pdos_mlp_loss_grad", "pdos_mlp_train", "pdos_mlp_predict",
-           "pdos_mlp_predict_dev", "pdos_mlp_last_error", "pdos_family_create", "pdos_family_load",
+           "pdos_mlp_predict_dev", "pdos_mlp_predict_dev_noalloc", "pdos_mlp_last_error", "pdos_family_create", "pdos_family_load",
            "pdos_family_run", "pdos_family_fetch", "pdos_family_free", "pdos_family_last_error",
            "pdos_family_timing", "pdos_family_info", "pdos_family_engines",
            "pdos_kkt_solve_host", "pdos_setup_host", "pdos_last_error",
@@ -118,6 +118,7 @@ def lib(plugin: str = "indirect"):
                                      C.c_ulong, C.POINTER(C.c_double)]
         L.pdos_mlp_predict.argtypes = [C.c_void_p, fp, C.c_int, fp, C.POINTER(C.c_double)]
         L.pdos_mlp_predict_dev.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int]
+        L.pdos_mlp_predict_dev_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int]
         L.pdos_mlp_last_error.restype = C.c_char_p
         L.pdos_batch.restype = C.c_int
         L.pdos_batch.argtypes = [C.POINTER(C.POINTER(Data)), C.POINTER(C.POINTER(Cone)), C.c_int,

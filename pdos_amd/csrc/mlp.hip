@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -299,6 +300,7 @@ hipError_t bf16_gemm_f32(hipStream_t st, int M, int N, int K, const __bf16* A, i
 // ---------------------------------------------------------------------------
 
 struct Mlp {
+  int device = 0;
   int nin = 0, nh = 0, nout = 0;        // logical sizes
   int pin = 0, ph = 0, pout = 0;        // padded sizes
   int cap = 0;                          // padded rows the work buffers hold
@@ -355,12 +357,18 @@ struct Mlp {
     hipLaunchKernelGGL(k_to_bf16, dim3(256), dim3(256), 0, st, (long)ph * pout, W3, W3b);
   }
   // forward for `rows` rows whose raw inputs are in Xf (device, rows x nin)
-  void forward(int rows, bool affine_out) {
+  void forward(int rows, bool affine_out) { forward_into(rows, affine_out, Xf, Xb, Z1, H1b, Z2, H2b, Y); }
+  // the same with explicit activation buffers (pad64(rows) rows each): X raw
+  // inputs (rows x nin), xb standardised input, z1 / z2 pre-activations (may
+  // alias each other: the forward pass never reads them), h1 / h2 hidden
+  // layers, y the padded output
+  void forward_into(int rows, bool affine_out, const float* X, __bf16* xb, float* z1, __bf16* h1, float* z2,
+                    __bf16* h2, float* y) {
     const int rp = pad64(rows);
-    hipLaunchKernelGGL(k_prep_input, dim3(512), dim3(256), 0, st, rows, nin, nin, Xf, in_mu, in_sd, pin, rp, Xb);
-    note(gemm<false, false, EPI_BIAS_GELU>(st, rp, ph, pin, Xb, pin, W1b, ph, Z1, H1b, ph, b1));
-    note(gemm<false, false, EPI_BIAS_GELU>(st, rp, ph, ph, H1b, ph, W2b, ph, Z2, H2b, ph, b2));
-    note(gemm<false, false, EPI_BIAS_AFFINE>(st, rp, pout, ph, H2b, ph, W3b, pout, Y, nullptr, pout, b3, nullptr,
+    hipLaunchKernelGGL(k_prep_input, dim3(512), dim3(256), 0, st, rows, nin, nin, X, in_mu, in_sd, pin, rp, xb);
+    note(gemm<false, false, EPI_BIAS_GELU>(st, rp, ph, pin, xb, pin, W1b, ph, z1, h1, ph, b1));
+    note(gemm<false, false, EPI_BIAS_GELU>(st, rp, ph, ph, h1, ph, W2b, ph, z2, h2, ph, b2));
+    note(gemm<false, false, EPI_BIAS_AFFINE>(st, rp, pout, ph, h2, ph, W3b, pout, y, nullptr, pout, b3, nullptr,
                                         affine_out ? out_mul : nullptr, affine_out ? out_add : nullptr));
   }
   // loss and parameter gradients into g for the rows in Xf / T
@@ -400,6 +408,44 @@ struct Mlp {
 
 using pdos_amd::Mlp;
 
+// Scratch of pdos_mlp_predict_dev_noalloc: the forward pass's activations for
+// one chunk of rows (standardised bf16 input, one pre-activation buffer
+// shared by both hidden layers, the two bf16 hidden layers, the padded
+// binary32 output).  One copy per GPU the library runs on, held until the
+// process ends; every no-alloc call on a GPU shares it.
+constexpr size_t kMlpScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[kMlpScratchBytes];
+
+namespace {
+constexpr int kMaxScratchDevices = 256;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxScratchDevices] = {};        // device address of `scratch` per GPU
+hipError_t g_scratch_err[kMaxScratchDevices] = {};
+
+// Once per device, with that device current: the device address of this
+// GPU's copy of `scratch` (the host-side name is not a device address).
+void scratch_setup(int device) {
+  if (device < 0 || device >= kMaxScratchDevices) return;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_dev[device]) return;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  g_scratch_err[device] = e;
+  if (e == hipSuccess) g_scratch_dev[device] = p;
+}
+
+unsigned char* scratch_of(int device, std::string& err) {
+  if (device < 0 || device >= kMaxScratchDevices) { err = "device ordinal out of range"; return nullptr; }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    err = std::string("scratch not set up on this device: hipGetSymbolAddress: ") +
+          hipGetErrorString(g_scratch_err[device]);
+    return nullptr;
+  }
+  return static_cast<unsigned char*>(g_scratch_dev[device]);
+}
+}  // namespace
+
 struct PdosMlp {
   Mlp net;
 };
@@ -416,9 +462,11 @@ PdosMlp* pdos_mlp_create(int n_in, int n_hidden, int n_out, unsigned long seed, 
   using namespace pdos_amd;
   if (n_in < 1 || n_hidden < 1 || n_out < 1) { g_mlp_err = "bad sizes"; return nullptr; }
   if (hipSetDevice(device) != hipSuccess) { g_mlp_err = "hipSetDevice failed"; return nullptr; }
+  scratch_setup(device);
   PdosMlp* P = new PdosMlp;
   Mlp& M = P->net;
   std::string err;
+  M.device = device;
   M.nin = n_in; M.nh = n_hidden; M.nout = n_out;
   M.pin = pad64(n_in); M.ph = pad64(n_hidden); M.pout = pad64(n_out);
   M.off_W1 = 0;
@@ -627,6 +675,46 @@ int pdos_mlp_predict_dev(PdosMlp* P, const float* dX, int rows, float* dY, int l
     M.forward(r, true);
     (void)hipMemcpy2DAsync(dY + (size_t)s * ldy, (size_t)ldy * 4, M.Y, M.pout * 4, M.nout * 4, r,
                            hipMemcpyDeviceToDevice, M.st);
+  }
+  return mlp_status(M);
+}
+
+// pdos_mlp_predict_dev without any allocation: the activations live in the
+// file-scope `scratch` of this GPU, rows are processed in chunks that fit it.
+// Each output row depends only on its own input row, so the results equal
+// pdos_mlp_predict_dev's bit for bit.
+int pdos_mlp_predict_dev_noalloc(PdosMlp* P, const float* dX, int rows, float* dY, int ldy) {
+  using pdos_amd::pad64;
+  if (!P) { g_mlp_err = "null network"; return -1; }
+  Mlp& M = P->net;
+  if (rows < 0 || (rows > 0 && (!dX || !dY)) || ldy < M.nout) { g_mlp_err = "bad arguments"; return -1; }
+  std::string err;
+  unsigned char* base = scratch_of(M.device, err);
+  if (!base) { g_mlp_err = err; return -1; }
+  auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
+  // bytes of one chunk of rp rows: xb, z (shared by both layers), h1, h2, y
+  auto need = [&](size_t rp) {
+    return al(rp * M.pin * 2) + al(rp * M.ph * 4) + 2 * al(rp * M.ph * 2) + al(rp * M.pout * 4);
+  };
+  if (need(64) > kMlpScratchBytes) {
+    g_mlp_err = "network too wide for the no-alloc scratch: 64 rows need " + std::to_string(need(64)) +
+                " bytes, the scratch holds " + std::to_string(kMlpScratchBytes);
+    return -1;
+  }
+  int chunk = 64;
+  while (need((size_t)chunk + 64) <= kMlpScratchBytes && chunk < (1 << 20)) chunk += 64;
+  for (int s = 0; s < rows; s += chunk) {
+    const int r = std::min(chunk, rows - s);
+    const size_t rp = (size_t)pad64(r);
+    __bf16* xb = reinterpret_cast<__bf16*>(base);
+    float* z = reinterpret_cast<float*>(base + al(rp * M.pin * 2));
+    __bf16* h1 = reinterpret_cast<__bf16*>(reinterpret_cast<unsigned char*>(z) + al(rp * M.ph * 4));
+    __bf16* h2 = reinterpret_cast<__bf16*>(reinterpret_cast<unsigned char*>(h1) + al(rp * M.ph * 2));
+    float* y = reinterpret_cast<float*>(reinterpret_cast<unsigned char*>(h2) + al(rp * M.ph * 2));
+    M.forward_into(r, true, dX + (size_t)s * M.nin, xb, z, h1, z, h2, y);
+    const hipError_t e = hipMemcpy2DAsync(dY + (size_t)s * ldy, (size_t)ldy * 4, y, M.pout * 4, M.nout * 4, r,
+                                          hipMemcpyDeviceToDevice, M.st);
+    if (e != hipSuccess) { g_mlp_err = std::string("copy: ") + hipGetErrorString(e); return -1; }
   }
   return mlp_status(M);
 }

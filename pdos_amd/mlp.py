@@ -78,6 +78,12 @@ class Mlp:
         if rc != 0:
             raise RuntimeError(self.L.pdos_mlp_last_error().decode())
 
+    def predict_dev_noalloc(self, dX, rows: int, dY, ldy: int) -> None:
+        """predict_dev without allocation (pdos_mlp_predict_dev_noalloc)."""
+        rc = self.L.pdos_mlp_predict_dev_noalloc(self.h, dX.ptr, rows, dY.ptr, ldy)
+        if rc != 0:
+            raise RuntimeError(self.L.pdos_mlp_last_error().decode())
+
     def __del__(self):
         try:
             if self.h:
