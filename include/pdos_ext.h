@@ -196,6 +196,11 @@ const char *pdos_mlp_last_error(void);
  * forward levels, backward levels.  Returns 0 or -1. */
 int pdos_kkt_solve_host(const Data *d, const Cone *k, const double *rhs, double *z, long *stats);
 
+/* The direct plugin's fill-reducing ordering of the scaled KKT of (d, k):
+ * perm (n+m longs) receives the pivot order (perm[k] = original index of the
+ * k-th pivot).  Returns 0 or -1. */
+int pdos_kkt_order(const Data *d, const Cone *k, long *perm);
+
 /* Host-side check of the setup: the scaling the solver iterates with, as the
  * reference's initWork leaves it in Work (common.h:35-218, or with
  * equil = PDOS_EQUIL_RUIZ ruiz_equil_common.h:98-322): D (m), E (n, with the

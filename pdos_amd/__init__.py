@@ -56,7 +56,7 @@ EXPORTS = ["pdos", "freeData", "freeSol", "tic", "toc", "tocq", "pdos_default_op
            "pdos_mlp_predict_dev", "pdos_mlp_predict_dev_noalloc", "pdos_mlp_last_error", "pdos_family_create", "pdos_family_load",
            "pdos_family_run", "pdos_family_fetch", "pdos_family_free", "pdos_family_last_error",
            "pdos_family_timing", "pdos_family_info", "pdos_family_engines",
-           "pdos_kkt_solve_host", "pdos_setup_host", "pdos_last_error",
+           "pdos_kkt_order", "pdos_kkt_solve_host", "pdos_setup_host", "pdos_last_error",
            "pdos_version"]
 
 _libs = {}
@@ -93,6 +93,8 @@ def lib(plugin: str = "indirect"):
         L.pdos_setup_host.restype = C.c_int
         L.pdos_setup_host.argtypes = [C.POINTER(Data), C.POINTER(Cone), C.c_int, C.POINTER(C.c_double),
                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.pdos_kkt_order.restype = C.c_int
+        L.pdos_kkt_order.argtypes = [C.POINTER(Data), C.POINTER(Cone), C.POINTER(C.c_long)]
         L.pdos_kkt_solve_host.restype = C.c_int
         L.pdos_kkt_solve_host.argtypes = [C.POINTER(Data), C.POINTER(Cone), C.POINTER(C.c_double),
                                           C.POINTER(C.c_double), C.POINTER(C.c_long)]
@@ -200,6 +202,16 @@ def kkt_solve_host(p: Problem, rhs):
     if rc != 0:
         raise RuntimeError(L.pdos_last_error().decode())
     return z, st
+
+
+def kkt_order(p: Problem) -> np.ndarray:
+    """The direct plugin's ordering of p's scaled KKT matrix (pivot order)."""
+    L = lib("direct")
+    d, k, keep = p.to_c()
+    perm = np.zeros(p.n + p.m, dtype=np.int64)
+    if L.pdos_kkt_order(C.byref(d), C.byref(k), perm.ctypes.data_as(C.POINTER(C.c_long))) != 0:
+        raise RuntimeError(L.pdos_last_error().decode())
+    return perm
 
 
 def setup_host(p: Problem, equil="reference", plugin="indirect"):

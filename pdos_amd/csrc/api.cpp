@@ -878,6 +878,19 @@ int pdos_setup_host(const Data* d, const Cone* k, int equil, double* D, double* 
   return 0;
 }
 
+int pdos_kkt_order(const Data* d, const Cone* k, long* perm) {
+  g_err.clear();
+  Setup S;
+  std::vector<int> P;
+  std::string err;
+  if (!build_setup(d, k, S, err) || !kkt_order(S, P, err)) {
+    g_err = err;
+    return -1;
+  }
+  for (size_t i = 0; i < P.size(); ++i) perm[i] = P[i];
+  return 0;
+}
+
 int pdos_kkt_solve_host(const Data* d, const Cone* k, const double* rhs, double* z, long* stats) {
   g_err.clear();
   Setup S;

@@ -47,11 +47,18 @@ struct Factor {
   int nbl() const { return (int)bl_ptr.size() - 1; }
 };
 
-// Builds the KKT of the scaled A, orders it with minimum degree (or uses
-// `perm` if non-empty), factors it, and derives level sets.
+// Builds the KKT of the scaled A, orders it with the approximate minimum
+// degree ordering (or uses `perm` if non-empty), factors it (left-looking
+// LDL', no pivoting: the KKT is quasi-definite), and derives level sets.
 bool build_factor(const Setup& S, const std::vector<int>& perm, Factor& F, std::string& err);
 
-// Minimum-degree ordering of the KKT pattern (exposed for tests).
-bool kkt_min_degree(const Setup& S, std::vector<int>& perm, std::string& err);
+// The fill-reducing ordering build_factor uses for S's KKT matrix.
+bool kkt_order(const Setup& S, std::vector<int>& perm, std::string& err);
+
+// Approximate minimum degree ordering (ordering.cpp) of the symmetric
+// pattern whose upper (or lower, or full) triangle is the CSC (Kp, Ki) of
+// order N; perm[k] = node eliminated k-th.
+bool amd_order(int N, const std::vector<int>& Kp, const std::vector<int>& Ki, std::vector<int>& perm,
+               std::string& err);
 
 }  // namespace pdos_amd

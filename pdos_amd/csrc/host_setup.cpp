@@ -233,62 +233,13 @@ void form_kkt(const Setup& S, std::vector<int>& Kp, std::vector<int>& Ki, std::v
   }
 }
 
-// Exact minimum degree on a bitset elimination graph; ties go to the lowest
-// index.  O(N^2/64) memory, so limited to N <= kMaxMdN.
-constexpr int kMaxMdN = 32768;
-
-bool min_degree(int N, const std::vector<int>& Kp, const std::vector<int>& Ki, std::vector<int>& perm,
-                std::string& err) {
-  if (N > kMaxMdN) { err = "direct plugin: KKT too large for the host ordering"; return false; }
-  const int W = (N + 63) / 64;
-  std::vector<uint64_t> adj((size_t)N * W, 0);
-  for (int j = 0; j < N; ++j)
-    for (int p = Kp[j]; p < Kp[j + 1]; ++p) {
-      const int i = Ki[p];
-      if (i == j) continue;
-      adj[(size_t)i * W + j / 64] |= 1ull << (j % 64);
-      adj[(size_t)j * W + i / 64] |= 1ull << (i % 64);
-    }
-  std::vector<int> deg(N);
-  std::vector<char> dead(N, 0);
-  for (int v = 0; v < N; ++v) {
-    int c = 0;
-    for (int t = 0; t < W; ++t) c += __builtin_popcountll(adj[(size_t)v * W + t]);
-    deg[v] = c;
-  }
-  perm.resize(N);
-  for (int step = 0; step < N; ++step) {
-    int v = -1;
-    for (int j = 0; j < N; ++j)
-      if (!dead[j] && (v < 0 || deg[j] < deg[v])) v = j;
-    perm[step] = v;
-    dead[v] = 1;
-    const uint64_t* av = &adj[(size_t)v * W];
-    for (int t = 0; t < W; ++t) {
-      uint64_t bits = av[t];
-      while (bits) {
-        const int u = t * 64 + __builtin_ctzll(bits);
-        bits &= bits - 1;
-        uint64_t* au = &adj[(size_t)u * W];
-        for (int s = 0; s < W; ++s) au[s] |= av[s];
-        au[u / 64] &= ~(1ull << (u % 64));
-        au[v / 64] &= ~(1ull << (v % 64));
-        int c = 0;
-        for (int s = 0; s < W; ++s) c += __builtin_popcountll(au[s]);
-        deg[u] = c;
-      }
-    }
-  }
-  return true;
-}
-
 }  // namespace
 
-bool kkt_min_degree(const Setup& S, std::vector<int>& perm, std::string& err) {
+bool kkt_order(const Setup& S, std::vector<int>& perm, std::string& err) {
   std::vector<int> Kp, Ki;
   std::vector<double> Kx;
   form_kkt(S, Kp, Ki, Kx);
-  return min_degree(S.n + S.m, Kp, Ki, perm, err);
+  return amd_order(S.n + S.m, Kp, Ki, perm, err);
 }
 
 bool build_factor(const Setup& S, const std::vector<int>& perm_in, Factor& F, std::string& err) {
@@ -300,7 +251,7 @@ bool build_factor(const Setup& S, const std::vector<int>& perm_in, Factor& F, st
   if (!perm_in.empty()) {
     if ((int)perm_in.size() != N) { err = "permutation has wrong length"; return false; }
     F.P = perm_in;
-  } else if (!min_degree(N, Kp, Ki, F.P, err)) {
+  } else if (!amd_order(N, Kp, Ki, F.P, err)) {
     return false;
   }
   std::vector<int> pinv(N, -1);
@@ -308,83 +259,115 @@ bool build_factor(const Setup& S, const std::vector<int>& perm_in, Factor& F, st
     if (F.P[k] < 0 || F.P[k] >= N || pinv[F.P[k]] >= 0) { err = "invalid permutation"; return false; }
     pinv[F.P[k]] = k;
   }
-  // C = upper triangle of P K P' (column max(i2, j2) gets row min(i2, j2)),
-  // entries in the visiting order of K's columns (cs.c:109-144).
-  std::vector<int> Cp(N + 1, 0), Ci(Kp[N]);
-  std::vector<double> Cx(Kp[N]);
+  // M = lower triangle of P K P' by columns: entry (max(i2, j2), min(i2, j2))
+  // of each stored K(i, j), i <= j, in column min(i2, j2).
+  std::vector<int> Mp(N + 1, 0), Mi(Kp[N]);
+  std::vector<double> Mx(Kp[N]);
   for (int j = 0; j < N; ++j)
     for (int p = Kp[j]; p < Kp[j + 1]; ++p) {
       if (Ki[p] > j) continue;
-      Cp[std::max(pinv[Ki[p]], pinv[j]) + 1]++;
+      Mp[std::min(pinv[Ki[p]], pinv[j]) + 1]++;
     }
-  for (int j = 0; j < N; ++j) Cp[j + 1] += Cp[j];
+  for (int j = 0; j < N; ++j) Mp[j + 1] += Mp[j];
   {
-    std::vector<int> nxt(Cp.begin(), Cp.end() - 1);
+    std::vector<int> nxt(Mp.begin(), Mp.end() - 1);
     for (int j = 0; j < N; ++j)
       for (int p = Kp[j]; p < Kp[j + 1]; ++p) {
         if (Ki[p] > j) continue;
         const int i2 = pinv[Ki[p]], j2 = pinv[j];
-        const int q = nxt[std::max(i2, j2)]++;
-        Ci[q] = std::min(i2, j2);
-        Cx[q] = Kx[p];
+        const int q = nxt[std::min(i2, j2)]++;
+        Mi[q] = std::max(i2, j2);
+        Mx[q] = Kx[p];
       }
   }
-  // Up-looking LDL' (the algorithm of ldl.c:187-319): the etree and column
-  // counts first, then row k of L as a sparse triangular solve over the
-  // etree reach of column k.
-  std::vector<int> parent(N), lnz(N), flag(N), pattern(N);
+  // Elimination tree (Liu's algorithm with path-compressed ancestors): row r
+  // of M links each earlier column c it touches to r via c's current root.
+  std::vector<int> parent(N, -1), anc(N, -1);
+  {
+    std::vector<int> rp(N + 1, 0), rc(Mp[N]);  // M by rows: columns c < r of row r
+    for (int p = 0; p < Mp[N]; ++p) rp[Mi[p] + 1]++;
+    for (int r = 0; r < N; ++r) rp[r + 1] += rp[r];
+    std::vector<int> nxt(rp.begin(), rp.end() - 1);
+    for (int c = 0; c < N; ++c)
+      for (int p = Mp[c]; p < Mp[c + 1]; ++p) rc[nxt[Mi[p]]++] = c;
+    for (int r = 0; r < N; ++r)
+      for (int q = rp[r]; q < rp[r + 1]; ++q) {
+        int c = rc[q];
+        if (c >= r) continue;
+        while (c != -1 && c != r) {
+          const int up = anc[c];
+          anc[c] = r;
+          if (up == -1) parent[c] = r;
+          c = up;
+        }
+      }
+  }
+  // Column structures of L: struct(L_j) = rows > j of M(:, j) united with the
+  // structures of j's children in the tree, minus j.
+  std::vector<int> child_head(N, -1), child_next(N, -1);
+  for (int j = N - 1; j >= 0; --j)
+    if (parent[j] >= 0) { child_next[j] = child_head[parent[j]]; child_head[parent[j]] = j; }
+  std::vector<std::vector<int>> cols(N);
+  std::vector<int> seen(N, -1);
   F.Lp.assign(N + 1, 0);
-  for (int k = 0; k < N; ++k) {
-    parent[k] = -1;
-    flag[k] = k;
-    lnz[k] = 0;
-    for (int p = Cp[k]; p < Cp[k + 1]; ++p) {
-      int i = Ci[p];
-      if (i >= k) continue;
-      for (; flag[i] != k; i = parent[i]) {
-        if (parent[i] == -1) parent[i] = k;
-        lnz[i]++;
-        flag[i] = k;
-      }
+  for (int j = 0; j < N; ++j) {
+    auto& cj = cols[j];
+    seen[j] = j;
+    for (int p = Mp[j]; p < Mp[j + 1]; ++p)
+      if (Mi[p] > j && seen[Mi[p]] != j) { seen[Mi[p]] = j; cj.push_back(Mi[p]); }
+    for (int c = child_head[j]; c != -1; c = child_next[c]) {
+      for (int r : cols[c])
+        if (r > j && seen[r] != j) { seen[r] = j; cj.push_back(r); }
     }
+    std::sort(cj.begin(), cj.end());
+    F.Lp[j + 1] = F.Lp[j] + (int)cj.size();
   }
-  for (int k = 0; k < N; ++k) F.Lp[k + 1] = F.Lp[k] + lnz[k];
-  F.Li.resize(F.Lp[N]);
-  F.Lx.resize(F.Lp[N]);
-  F.Ld.resize(N);
-  std::vector<double> Y(N, 0.0);
-  for (int k = 0; k < N; ++k) {
-    int top = N;
-    Y[k] = 0.0;
-    flag[k] = k;
-    lnz[k] = 0;
-    for (int p = Cp[k]; p < Cp[k + 1]; ++p) {
-      int i = Ci[p];
-      if (i > k) continue;
-      Y[i] += Cx[p];
-      int len = 0;
-      for (; flag[i] != k; i = parent[i]) {
-        pattern[len++] = i;
-        flag[i] = k;
+  const long lnz_total = F.Lp[N];
+  F.Li.resize(lnz_total);
+  F.Lx.assign(lnz_total, 0.0);
+  F.Ld.assign(N, 0.0);
+  for (int j = 0; j < N; ++j) {
+    std::copy(cols[j].begin(), cols[j].end(), F.Li.begin() + F.Lp[j]);
+    std::vector<int>().swap(cols[j]);
+  }
+  // Left-looking numeric factorization: column j gathers M(:, j) and
+  // subtracts L(:, k) d_k L(j, k) for every earlier column k with L(j, k) != 0;
+  // the columns waiting for row r are kept in a linked list per row, each at
+  // the position of its next row.
+  std::vector<double> wv(N, 0.0);
+  std::vector<int> head(N, -1), link(N, -1), at(N, 0);
+  for (int j = 0; j < N; ++j) {
+    for (int p = Mp[j]; p < Mp[j + 1]; ++p) wv[Mi[p]] += Mx[p];
+    for (int k = head[j]; k != -1;) {
+      const int knext = link[k];
+      const int q = at[k];
+      const double ljk = F.Lx[q], f = ljk * F.Ld[k];
+      wv[j] -= ljk * f;
+      const int qe = F.Lp[k + 1];
+      for (int t = q + 1; t < qe; ++t) wv[F.Li[t]] -= F.Lx[t] * f;
+      at[k] = q + 1;
+      if (q + 1 < qe) {
+        const int r = F.Li[q + 1];
+        link[k] = head[r];
+        head[r] = k;
       }
-      while (len > 0) pattern[--top] = pattern[--len];
+      k = knext;
     }
-    F.Ld[k] = Y[k];
-    Y[k] = 0.0;
-    for (; top < N; ++top) {
-      const int i = pattern[top];
-      const double yi = Y[i];
-      Y[i] = 0.0;
-      const int pend = F.Lp[i] + lnz[i];
-      int q = F.Lp[i];
-      for (; q < pend; ++q) Y[F.Li[q]] -= F.Lx[q] * yi;
-      const double lki = yi / F.Ld[i];
-      F.Ld[k] -= lki * yi;
-      F.Li[q] = k;
-      F.Lx[q] = lki;
-      lnz[i]++;
+    head[j] = -1;
+    const double dj = wv[j];
+    wv[j] = 0.0;
+    if (dj == 0.0 || !std::isfinite(dj)) { err = "zero pivot in LDL'"; return false; }
+    F.Ld[j] = dj;
+    for (int t = F.Lp[j]; t < F.Lp[j + 1]; ++t) {
+      F.Lx[t] = wv[F.Li[t]] / dj;
+      wv[F.Li[t]] = 0.0;
     }
-    if (F.Ld[k] == 0.0) { err = "zero pivot in LDL'"; return false; }
+    if (F.Lp[j] < F.Lp[j + 1]) {
+      at[j] = F.Lp[j];
+      const int r = F.Li[F.Lp[j]];
+      link[j] = head[r];
+      head[r] = j;
+    }
   }
   // Rows of L (CSR) for the forward sweep, columns ascending within a row.
   const int lnz_tot = F.Lp[N];
