@@ -400,7 +400,21 @@ def run_family(args):
     step()
     kt = F.timing(False)
     F.engines(engines)
+    # end to end: the same step with the per-instance data uploaded (c, and
+    # the features the network reads) and the solutions downloaded and
+    # unscaled, as a caller holding host arrays sees it
+    feats = portfolio_features(th_te)
+    e2e_steps = max(1, min(args.steps, 5))
+    t0 = time.perf_counter()
+    for _ in range(e2e_steps):
+        F.load(C_te)
+        dF2 = DeviceArray(feats)
+        ws.predict_dev(dF2, n_te, dW, ld)
+        F.run(dW, ld, judge=True)
+        e2e = F.fetch()
+    e2e_wall = (time.perf_counter() - t0) / e2e_steps
     A = sp.csc_matrix((fam.Ax, fam.Ai, fam.Ap), shape=(fam.m, fam.n))
+    v_e2e = verify_batch(A, fam.b, C_te, e2e["x"], e2e["y"], e2e["s"], e2e["status"], 1e-3)
     v_warm = verify_batch(A, fam.b, C_te, warm["x"], warm["y"], warm["s"], warm["status"], 1e-3)
     v_cold = verify_batch(A, fam.b, C_te, cold["x"], cold["y"], cold["s"], cold["status"], 1e-3)
     solved = int((warm["status"] == "Solved").sum())
@@ -409,10 +423,15 @@ def run_family(args):
     log(f"warm: {solved}/{n_te} Solved ({v_warm} verified), {value:.0f} SOCPs/s, mean iters "
         f"{warm['iters'].mean():.1f}, fallbacks {int(warm['fallback'].sum())}; cold: {solved_cold / cold_wall:.0f} "
         f"SOCPs/s, mean iters {cold['iters'].mean():.1f}")
+    # error against binary64: every result against the same batch solved by
+    # the family engine in FP64 to EPS_ABS 1e-6; then FP64 polish iterations
+    # warm-started from the mixed results (to EPS_ABS 1e-4, at most 300)
+    fp64_ref = fp64_error_report(fam, C_te, warm, args) if args.fp64_check else None
     probs = [fam.instance(C_te[i]) for i in range(max(args.plain_sample, args.cpu_sample))]
-    plain = plain_gpu(probs[:args.plain_sample], "direct", "fp64") if args.plain_sample else None
-    if plain:
-        log(f"plain GPU: {plain['value']} SOCPs/s")
+    plain = None
+    if args.plain_sample:
+        plain = {prec_: plain_gpu(probs[:args.plain_sample], "direct", prec_) for prec_ in ("fp64", "mixed")}
+        log(f"plain GPU: fp64 {plain['fp64']['value']} / mixed {plain['mixed']['value']} SOCPs/s")
     config = {"workload": "portfolio family, learned warm start, held-out batch", "n_a": 500, "k": 20,
               "n": fam.n, "m": fam.m, "nnz": int(fam.Ap[-1]), "count": n_te, "train_count": n_tr,
               "eps_abs": 1e-3, "max_iters": 2000, "precision": prec, "engine": "family (shared A)",
@@ -426,8 +445,14 @@ def run_family(args):
                        "train_loss": loss, "train_s": round(t_train, 2), "data_s": round(t_data, 2),
                        "compactions": comp},
         "kernels": {k: {"ms": round(v[0], 3), "calls": v[1]} for k, v in kt.items()},
+        "end_to_end": {"value": round(v_e2e / e2e_wall, 3), "unit": "SOCPs/s", "ms_per_step": round(e2e_wall * 1e3, 3),
+                       "verified_fp64": v_e2e,
+                       "includes": "host-to-device copy of c and the features, inference, solve, download and "
+                                   "unscale of x, y, s"},
         "plain_gpu": plain,
-        "ratio_vs_plain_gpu": round(value / plain["value"], 1) if plain else None,
+        # against the faster of the two plain single-instance paths
+        "ratio_vs_plain_gpu": round(value / max(v["value"] for v in plain.values()), 1) if plain else None,
+        "fp64_error": fp64_ref,
     }
     prof = "family_portfolio_16384_pmc_traffic" if (args.count or 16384) == 16384 else None
     row_nnz = np.bincount(fam.Ai, minlength=fam.m)
@@ -436,6 +461,61 @@ def run_family(args):
     acc = {"solved": solved, "count": n_te, "verified_fp64": v_warm,
            "mean_iters": float(warm["iters"].mean()), "max_iters": int(warm["iters"].max())}
     return value, wall, "f32+bf16" if prec == "mixed" else "f64", config, roof, acc, extra, probs[:args.cpu_sample], "direct"
+
+
+def _errors(C, out, ref):
+    """Per instance: relative objective gap and relative distance of (x, y, s)
+    to the reference solution."""
+    po = np.einsum("ij,ij->i", C, out["x"])
+    pr = np.einsum("ij,ij->i", C, ref["x"])
+    gap = np.abs(po - pr) / (1 + np.abs(pr))
+    num = np.sqrt(sum(np.sum((out[k] - ref[k]) ** 2, axis=1) for k in ("x", "y", "s")))
+    den = np.sqrt(sum(np.sum(ref[k] ** 2, axis=1) for k in ("x", "y", "s")))
+    return gap, num / np.maximum(den, 1e-300)
+
+
+def _stats(v):
+    return {"max": float(v.max()), "p99": float(np.quantile(v, 0.99)), "mean": float(v.mean()),
+            "median": float(np.median(v))}
+
+
+def fp64_error_report(fam, C_te, warm, args):
+    from pdos_amd import DeviceArray
+    from pdos_amd.family import Family
+    t0 = time.perf_counter()
+    R = Family(fam.instance(C_te[0]).with_params(EPS_ABS=1e-6, MAX_ITERS=20000), precision="fp64")
+    R.load(C_te)
+    R.run()
+    ref = R.fetch()
+    t_ref = time.perf_counter() - t0
+    keep = (warm["status"] == "Solved") & (ref["status"] == "Solved")
+    gap, dist = _errors(C_te[keep], {k: warm[k][keep] for k in "xys"}, {k: ref[k][keep] for k in "xys"})
+    out = {"reference": "the same batch, family engine in binary64 to EPS_ABS 1e-6 (MAX_ITERS 20,000)",
+           "reference_solved": int((ref["status"] == "Solved").sum()), "compared": int(keep.sum()),
+           "reference_s": round(t_ref, 2), "obj_rel_gap": _stats(gap), "dist_rel": _stats(dist)}
+    # FP64 polish: iterations in binary64 from the mixed results
+    P = Family(fam.instance(C_te[0]).with_params(EPS_ABS=1e-4, MAX_ITERS=300), precision="fp64")
+    W = np.concatenate([warm["x"], warm["y"], warm["s"]], axis=1).astype(np.float32)
+    dW = DeviceArray(W)
+    P.load(C_te)
+    P.run(dW, W.shape[1], judge=False)  # compile / warm
+    P.load(C_te)
+    t0 = time.perf_counter()
+    P.run(dW, W.shape[1], judge=False)
+    t_pol = time.perf_counter() - t0
+    pol = P.fetch()
+    gap2, dist2 = _errors(C_te[keep], {k: pol[k][keep] for k in "xys"}, {k: ref[k][keep] for k in "xys"})
+    out["polish"] = {"eps_abs": 1e-4, "max_iters": 300, "ms": round(t_pol * 1e3, 2),
+                     "mean_iters": float(pol["iters"].mean()),
+                     "solved_at_1e-4": int((pol["status"] == "Solved").sum()),
+                     "obj_rel_gap": _stats(gap2), "dist_rel": _stats(dist2)}
+    if args.errors_out:
+        np.savez_compressed(args.errors_out, keep=keep, obj_gap=gap, dist=dist, obj_gap_polished=gap2,
+                            dist_polished=dist2)
+    log(f"fp64 error: obj gap max {gap.max():.2e} p99 {np.quantile(gap, .99):.2e}; dist max {dist.max():.2e}; "
+        f"after polish ({t_pol * 1e3:.0f} ms, mean {pol['iters'].mean():.1f} iters): obj gap max {gap2.max():.2e}, "
+        f"dist max {dist2.max():.2e}")
+    return out
 
 
 def run_dense(args):
@@ -546,6 +626,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1024)
     ap.add_argument("--plain-sample", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fp64-check", type=int, default=1,
+                    help="family: error of every result against an FP64 solve of the batch, and FP64 polish")
+    ap.add_argument("--errors-out", default="", help="family: write per-instance errors to this .npz")
     ap.add_argument("--products", type=int, default=0, help="family, mixed: bf16 products (1-3)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="end with one 256 MiB device-to-device copy (the reference kernel for "
@@ -572,9 +655,9 @@ def main():
                "eps_abs": 1e-3, "plugin": "indirect", "engine": "grid"}
         res = run_single(args, p, cfg, "indirect")
     elif w == "portfolio":
-        p = problems.portfolio(seed=0, EPS_ABS=1e-3)
+        p = problems.portfolio(seed=0)  # the n=101 portfolio's own options: EPS_ABS 5e-3, ALPHA 1, NORMALIZE 0
         cfg = {"workload": "portfolio n=101 m=212", "n": p.n, "m": p.m, "nnz": p.nnz, "count": 1,
-               "eps_abs": 1e-3, "plugin": args.plugin, "engine": "workgroup"}
+               "eps_abs": p.params["EPS_ABS"], "plugin": args.plugin, "engine": "workgroup"}
         res = run_single(args, p, cfg, args.plugin)
     else:
         raise SystemExit(f"unknown workload {w}")

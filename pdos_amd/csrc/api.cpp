@@ -236,6 +236,190 @@ bool WgBatch::group_and_factor(std::string& err) {
   return true;
 }
 
+// The LDS image of a group for the resident variant (k_wg_res): scalings,
+// both CSR forms of A (and, direct plugin, A's column indices as pivot
+// positions), the rows' wave batches, cone offsets and, direct plugin, the
+// permutation, its inverse and both sweeps of L in level order.  Fills the
+// layout, including the per-instance part after the image.
+static void res_image(const Setup& s, const Factor* f, bool fp64, std::vector<unsigned char>& img, ResLayout& L) {
+  img.clear();
+  auto put = [&](const void* src, size_t bytes) {
+    const size_t off = (img.size() + 15) & ~size_t(15);
+    img.resize(off + bytes);
+    if (bytes) std::memcpy(img.data() + off, src, bytes);
+    return (int)off;
+  };
+  auto putv = [&](const std::vector<double>& v) {
+    if (fp64) return put(v.data(), v.size() * 8);
+    std::vector<float> w(v.begin(), v.end());
+    return put(w.data(), w.size() * 4);
+  };
+  // items (rows) in class order and their wave batches
+  auto lanes = [](int len) {
+    int lg = 0;
+    while (lg < 6 && len > 4 * (1 << lg)) ++lg;
+    return lg;
+  };
+  auto batches = [&](const std::vector<int>& items, const std::vector<int>& len, std::vector<int>& ord,
+                     std::vector<int4>& bat) {
+    std::vector<int> o(items);
+    std::stable_sort(o.begin(), o.end(), [&](int a, int b) { return lanes(len[a]) < lanes(len[b]); });
+    size_t k = 0;
+    while (k < o.size()) {
+      const int lg = lanes(len[o[k]]);
+      size_t e = k;
+      while (e < o.size() && (int)(e - k) < (64 >> lg) && lanes(len[o[e]]) == lg) ++e;
+      bat.push_back(int4{(int)(ord.size() + k), (int)(e - k), lg, 0});
+      k = e;
+    }
+    ord.insert(ord.end(), o.begin(), o.end());
+  };
+  // a CSR matrix with its rows in batch order: row ids, pointers, indices
+  // (and, with `map`, the indices mapped), values
+  auto positions = [&](const std::vector<int>& ord, const std::vector<int>& qp) {
+    std::vector<int4> pos(ord.size());
+    for (size_t q = 0; q < ord.size(); ++q) pos[q] = int4{ord[q], qp[q], qp[q + 1], 0};
+    return put(pos.data(), pos.size() * sizeof(int4));
+  };
+  auto spmv_batches = [&](const std::vector<int>& ptr, const std::vector<int>& ix, const std::vector<double>& vx,
+                          const std::vector<int>* map, int& o_pos, int& o_idx, int& o_val, int* o_map,
+                          int& o_bat, int& nbat) {
+    const int nr = (int)ptr.size() - 1;
+    std::vector<int> items(nr), len(nr), ord;
+    std::vector<int4> bat;
+    for (int r = 0; r < nr; ++r) { items[r] = r; len[r] = ptr[r + 1] - ptr[r]; }
+    batches(items, len, ord, bat);
+    std::vector<int> qp(nr + 1, 0), qi, qm;
+    std::vector<double> qv;
+    for (int q = 0; q < nr; ++q) {
+      const int r = ord[q];
+      qp[q + 1] = qp[q] + len[r];
+      qi.insert(qi.end(), ix.begin() + ptr[r], ix.begin() + ptr[r + 1]);
+      qv.insert(qv.end(), vx.begin() + ptr[r], vx.begin() + ptr[r + 1]);
+      if (map)
+        for (int e = ptr[r]; e < ptr[r + 1]; ++e) qm.push_back((*map)[ix[e]]);
+    }
+    o_pos = positions(ord, qp);
+    qi.resize(qi.size() + 256, 0);  // padding read by the kernel's unconditional loads
+    qv.resize(qv.size() + 256, 0.0);
+    if (map) qm.resize(qm.size() + 256, 0);
+    o_idx = put(qi.data(), qi.size() * 4);
+    o_val = putv(qv);
+    if (map) *o_map = put(qm.data(), qm.size() * 4);
+    o_bat = put(bat.data(), bat.size() * sizeof(int4));
+    nbat = (int)bat.size();
+  };
+  std::memset(&L, 0, sizeof(L));
+  L.D = put(s.D.data(), s.D.size() * 8);
+  L.E = put(s.E.data(), s.E.size() * 8);
+  std::vector<int> pinv;
+  if (f) {
+    pinv.resize(f->N);
+    for (int k = 0; k < f->N; ++k) pinv[f->P[k]] = k;
+  }
+  spmv_batches(s.rp, s.ri, s.rx, f ? &pinv : nullptr, L.rpos, L.ridx, L.rval, &L.ridp, L.rbat, L.nrbat);
+  spmv_batches(s.cp, s.ci, s.cx, nullptr, L.cpos, L.cidx, L.cval, nullptr, L.cbat, L.ncbat);
+  L.soc = put(s.soc_ptr.data(), s.soc_ptr.size() * 4);
+  if (f) {
+    const int N = f->N;
+    L.perm = put(f->P.data(), N * 4);
+    L.pinv = put(pinv.data(), N * 4);
+    // The dense trailing chain T: the forward levels from the last one with
+    // more than one row on each hold a single row (a portfolio's budget and
+    // factor rows); T's rows only depend on earlier rows and on each other,
+    // and T's columns only have entries in T.  At most 64 rows: one wave
+    // solves the chain with its t x t block held densely, forward and then
+    // backward, instead of 2 t barrier-separated levels.
+    const int nfl = f->nfl();
+    int lam0 = nfl;
+    while (lam0 > 0 && f->fl_ptr[lam0] - f->fl_ptr[lam0 - 1] == 1 && nfl - (lam0 - 1) <= 64) --lam0;
+    const int t = nfl - lam0 >= 3 ? nfl - lam0 : 0;
+    if (!t) lam0 = nfl;
+    std::vector<int> tpos(N, -1), trow(t);
+    for (int k = 0; k < t; ++k) {
+      trow[k] = f->fl_rows[f->fl_ptr[lam0 + k]];
+      tpos[trow[k]] = k;
+    }
+    std::vector<double> tl((size_t)t * t, 0.0), tdq(t);
+    for (int k = 0; k < t; ++k) {
+      tdq[k] = f->Ld[trow[k]];
+      for (int p = f->Rp[trow[k]]; p < f->Rp[trow[k] + 1]; ++p)
+        if (tpos[f->Ri[p]] >= 0) tl[(size_t)k * t + tpos[f->Ri[p]]] = f->Rx[p];
+    }
+    L.t = t;
+    L.trow = put(trow.data(), trow.size() * 4);
+    L.tdq = put(tdq.data(), tdq.size() * 8);
+    L.tl = put(tl.data(), tl.size() * 8);
+    // backward levels of the columns outside T, with T's values final
+    std::vector<int> blev(N, -1);
+    int nbl = 0;
+    for (int q = N - 1; q >= 0; --q) {
+      const int j = q;  // pivot order: columns after j are solved first
+      if (tpos[j] >= 0) continue;
+      int lv = 0;
+      for (int p = f->Lp[j]; p < f->Lp[j + 1]; ++p)
+        if (tpos[f->Li[p]] < 0) lv = std::max(lv, blev[f->Li[p]] + 1);
+      blev[j] = lv;
+      nbl = std::max(nbl, lv + 1);
+    }
+    for (int back = 0; back < 2; ++back) {
+      const std::vector<int>& P = back ? f->Lp : f->Rp;
+      const std::vector<int>& I = back ? f->Li : f->Ri;
+      const std::vector<double>& V = back ? f->Lx : f->Rx;
+      // levels: forward, the levels before T, then T's rows with their
+      // entries outside T (one more level); backward, the columns outside T
+      std::vector<std::vector<int>> levels;
+      if (!back) {
+        for (int lv = 0; lv < lam0; ++lv)
+          levels.emplace_back(f->fl_rows.begin() + f->fl_ptr[lv], f->fl_rows.begin() + f->fl_ptr[lv + 1]);
+        if (t) levels.push_back(trow);
+      } else {
+        levels.resize(nbl);
+        for (int j = 0; j < N; ++j)
+          if (blev[j] >= 0) levels[blev[j]].push_back(j);
+      }
+      std::vector<int> len(N, 0), order, lb(1, 0);
+      std::vector<int4> bat;
+      auto kept = [&](int r, int p) { return back || tpos[r] < 0 || tpos[I[p]] < 0; };
+      for (int i = 0; i < N; ++i)
+        for (int p = P[i]; p < P[i + 1]; ++p) len[i] += kept(i, p) ? 1 : 0;
+      for (const auto& items : levels) {
+        batches(items, len, order, bat);
+        lb.push_back((int)bat.size());
+      }
+      const int npos = (int)order.size();
+      std::vector<int> ptr(npos + 1, 0), idx;
+      std::vector<double> val, dq;
+      for (int q = 0; q < npos; ++q) {
+        const int r = order[q];
+        ptr[q + 1] = ptr[q] + len[r];
+        for (int p = P[r]; p < P[r + 1]; ++p)
+          if (kept(r, p)) { idx.push_back(I[p]); val.push_back(V[p]); }
+        if (back) dq.push_back(f->Ld[r]);
+      }
+      const int o_pos = positions(order, ptr);
+      idx.resize(idx.size() + 256, 0);  // padding read by the kernel's unconditional loads
+      val.resize(val.size() + 256, 0.0);
+      const int o_idx = put(idx.data(), idx.size() * 4), o_val = put(val.data(), val.size() * 8);
+      const int o_bat = put(bat.data(), bat.size() * sizeof(int4)), o_lb = put(lb.data(), lb.size() * 4);
+      if (back) {
+        L.bpos = o_pos; L.bidx = o_idx; L.bval = o_val; L.bbat = o_bat; L.blb = o_lb;
+        L.bdq = put(dq.data(), std::max<size_t>(dq.size(), 1) * 8);
+        L.nbl = (int)levels.size();
+      } else {
+        L.fpos = o_pos; L.fidx = o_idx; L.fval = o_val; L.fbat = o_bat; L.flb = o_lb;
+        L.nfl = (int)levels.size();
+      }
+    }
+  }
+  img.resize((img.size() + 15) & ~size_t(15));
+  L.img = (int)img.size();
+  L.b = L.img;
+  L.c = L.b + (int)(((long)s.m * 8 + 15) & ~15L);
+  L.vec = L.c + (int)(((long)s.n * 8 + 15) & ~15L);
+  L.total = L.vec + (int)res_vec_bytes(s.n, s.m, s.n + s.m, f != nullptr, fp64 ? 8 : 4);
+}
+
 bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::string& err) {
   device = device_; fp64 = fp64_; direct = direct_; own_mem = !borrow;
   Ctx& C = ctx();
@@ -247,7 +431,8 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
   Packer P;
   struct Off { size_t soc, rp, ri, rx, cp, ci, cx, rb, cb, b, c, D, E, x0, s0, y0;
                int nrb, ncb;
-               size_t perm, lrp, lri, lrx, lcp, lci, lcx, ld, flp, flr, blp, blr; };
+               size_t perm, lrp, lri, lrx, lcp, lci, lcx, ld, flp, flr, blp, blr;
+               size_t img; int res; ResLayout lay; };
   std::vector<Off> pk(count);
   xo.resize(count); so.resize(count); yo.resize(count);
   if (grp.size() != (size_t)count) {  // no grouping requested: every instance is its own group
@@ -256,10 +441,21 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
     for (int i = 0; i < count; ++i) grp[i] = rep[i] = i;
   }
   lds = 0;
+  const char* rs_env = std::getenv("PDOS_WG_RESIDENT");
+  const bool resident = !(rs_env && !std::strcmp(rs_env, "0"));
+  // members too large for one workgroup's LDS go to the whole-GPU engine
+  is_big.assign(count, 0);
+  big.clear();
+  for (int i = 0; i < count; ++i)
+    if (wg_lds_bytes(S[i].n, S[i].m, S[i].n + S[i].m, direct, tsz) > C.max_lds) {
+      is_big[i] = 1;
+      big.push_back(i);
+    }
   for (int i = 0; i < count; ++i) {
     const Setup& s = S[i];
     Off& k = pk[i];
-    if (rep[grp[i]] != i) {  // shares the representative's matrix, scalings and factor
+    if (is_big[i]) continue;
+    if (rep[grp[i]] != i && !is_big[rep[grp[i]]]) {  // shares the representative's matrix, scalings and factor
       const Off& r = pk[rep[grp[i]]];
       k = r;
     } else {
@@ -277,6 +473,21 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
       k.ncb = (int)cb.size() / 4;
       k.D = put_vec(P, s.D);
       k.E = put_vec(P, s.E);
+      // the resident variant when the group's image and iterates fit in LDS
+      // without fewer workgroups per CU than the streaming variant (or the
+      // batch has at most one workgroup per CU anyway)
+      k.res = 0;
+      if (resident) {
+        std::vector<unsigned char> img;
+        res_image(s, direct ? &F[grp[i]] : nullptr, fp64, img, k.lay);
+        const long cu_lds = 160 * 1024;
+        const int old_b = wg_lds_bytes(s.n, s.m, s.n + s.m, direct, tsz);
+        auto occ = [&](long b) { return std::max<long>(1, std::min<long>(kOccMax, cu_lds / b)); };
+        if (k.lay.total <= C.max_lds && (count <= C.num_cu || occ(k.lay.total) >= occ(old_b))) {
+          k.res = 1;
+          k.img = P.put(img.data(), img.size());
+        }
+      }
     }
     k.b = put_vec(P, s.b);
     k.c = put_vec(P, s.c);
@@ -301,16 +512,11 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
       k.blp = put_vec(P, f.bl_ptr);
       k.blr = put_vec(P, f.bl_rows);
     }
-    lds = std::max(lds, wg_lds_bytes(s.n, s.m, s.n + s.m, direct, tsz));
-  }
-  if (lds > C.max_lds) {
-    err = "instance too large for the workgroup engine (needs " + std::to_string(lds) +
-          " B of LDS, device has " + std::to_string(C.max_lds) + ")";
-    return false;
+    lds = std::max(lds, k.res ? k.lay.total : wg_lds_bytes(s.n, s.m, s.n + s.m, direct, tsz));
   }
   ho.assign(count, 0);
   for (int i = 0; i < count; ++i)  // per row: 5 residuals, then (indirect) 10 x {CG steps, |q|}
-    if (want_hist && par[i].VERBOSE) ho[i] = P.zeros(sizeof(double) * (direct ? 5 : 25) * verbose_hist_rows(par[i].MAX_ITERS));
+    if (want_hist && par[i].VERBOSE && !is_big[i]) ho[i] = P.zeros(sizeof(double) * (direct ? 5 : 25) * verbose_hist_rows(par[i].MAX_ITERS));
   res_off = P.zeros(sizeof(DevResult) * count);
   inst_off = P.zeros(sizeof(WgInst) * count);
   cinst_off = P.zeros(sizeof(WgInst) * count);
@@ -325,6 +531,7 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
   unsigned char* base = static_cast<unsigned char*>(dbuf);
   insts.assign(count, WgInst());
   for (int i = 0; i < count; ++i) {
+    if (is_big[i]) continue;
     const Setup& s = S[i];
     const Off& k = pk[i];
     const Setup& sr = S[rep[grp[i]]];  // holds the matrix arrays
@@ -382,33 +589,44 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
     w.hist = w.hist_rows ? reinterpret_cast<double*>(base + ho[i]) : nullptr;
     w.cg_hist = w.hist_rows && !direct ? w.hist + (size_t)5 * w.hist_rows : nullptr;
     w.lds_bytes = wg_lds_bytes(s.n, s.m, s.n + s.m, direct, tsz);
+    if (k.res) {
+      w.resident = 1;
+      w.img = base + k.img;
+      w.lay = k.lay;
+      w.lds_bytes = k.lay.total;
+    }
   }
   // launch classes by workgroups per CU that the LDS allows (at most
   // kClasses; the engine's registers allow kClasses), biggest first, and
   // within a class the instances with the most nonzeros first
   {
     const long cu_lds = 160 * 1024;
-    auto occ = [&](int i) { return (int)std::max<long>(1, std::min<long>(kClasses, cu_lds / insts[i].lds_bytes)); };
-    order.resize(count);
-    for (int i = 0; i < count; ++i) order[i] = i;
+    auto occ = [&](int i) {
+      return 2 * (int)std::max<long>(1, std::min<long>(kOccMax, cu_lds / insts[i].lds_bytes)) + insts[i].resident;
+    };
+    order.clear();
+    for (int i = 0; i < count; ++i)
+      if (!is_big[i]) order.push_back(i);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
       const int oa = occ(a), ob = occ(b);
       if (oa != ob) return oa < ob;
       return S[rep[grp[a]]].nnz() > S[rep[grp[b]]].nnz();
     });
     ncls = 0;
-    for (int k = 0; k < count; ++k) {
+    const int nwg = (int)order.size();
+    for (int k = 0; k < nwg; ++k) {
       if (k == 0 || occ(order[k]) != occ(order[k - 1])) {
         cls_start[ncls] = k;
         cls_lds[ncls] = 0;
+        cls_res[ncls] = insts[order[k]].resident;
         ++ncls;
       }
       cls_lds[ncls - 1] = std::max(cls_lds[ncls - 1], insts[order[k]].lds_bytes);
     }
-    cls_start[ncls] = count;
+    cls_start[ncls] = nwg;
   }
   std::memcpy(P.buf.data() + inst_off, insts.data(), sizeof(WgInst) * count);
-  for (int k = 0; k < count; ++k)
+  for (size_t k = 0; k < order.size(); ++k)
     std::memcpy(P.buf.data() + cinst_off + sizeof(WgInst) * k, &insts[order[k]], sizeof(WgInst));
   hipError_t e;
   if (borrow) {
@@ -425,6 +643,20 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
     if (e == hipSuccess && !xev[c]) e = hipEventCreateWithFlags(&xev[c], hipEventDisableTiming);
   }
   if (e != hipSuccess) { err = std::string("upload: ") + hipGetErrorString(e); return false; }
+  gsol.clear();
+  for (int i : big) {  // own device memory: the batch may hold the thread's arena
+    Setup full;
+    const Setup* sp = &S[i];
+    if (rep[grp[i]] != i) {  // the group's matrix arrays live with its representative
+      full = S[i];
+      const Setup& r = S[rep[grp[i]]];
+      full.cp = r.cp; full.ci = r.ci; full.cx = r.cx; full.rp = r.rp; full.ri = r.ri; full.rx = r.rx;
+      sp = &full;
+    }
+    std::unique_ptr<GridSolver> g = grid_prepare(*sp, par[i], fp64, direct ? &F[grp[i]] : nullptr, device, false, err);
+    if (!g) { err = "instance " + std::to_string(i) + " (whole-GPU engine): " + err; return false; }
+    gsol.push_back(std::move(g));
+  }
   upload_ms = now_ms() - t0;
   return true;
 }
@@ -433,19 +665,25 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
 bool WgBatch::upload_records(std::string& err) {
   unsigned char* base = static_cast<unsigned char*>(dbuf);
   const size_t count = insts.size();
-  std::vector<WgInst> byc(count);
-  for (size_t k = 0; k < count; ++k) byc[k] = insts[order[k]];
+  std::vector<WgInst> byc(order.size());
+  for (size_t k = 0; k < order.size(); ++k) byc[k] = insts[order[k]];
   hipError_t e = hipMemcpyAsync(base + inst_off, insts.data(), sizeof(WgInst) * count, hipMemcpyHostToDevice,
                                 stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(base + cinst_off, byc.data(), sizeof(WgInst) * count, hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess && !byc.empty())
+    e = hipMemcpyAsync(base + cinst_off, byc.data(), sizeof(WgInst) * byc.size(), hipMemcpyHostToDevice, stream);
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
   if (e != hipSuccess) { err = std::string("records: ") + hipGetErrorString(e); return false; }
   return true;
 }
 
 bool WgBatch::set_warm(const float* dW, long ld, int check, std::string& err) {
+  if (dW && !big.empty()) {
+    err = "set_warm: device warm starts are not supported for members on the whole-GPU engine (instance " +
+          std::to_string(big[0]) + ")";
+    return false;
+  }
   for (size_t i = 0; i < insts.size(); ++i) {
+    if (!is_big.empty() && is_big[i]) continue;
     insts[i].wsrc = dW ? dW + (size_t)i * ld : nullptr;
     insts[i].warm_check = dW ? check : 0;
   }
@@ -467,7 +705,8 @@ bool WgBatch::run(std::string& err) {
     hipStream_t sc = c == 0 ? stream : xs[c - 1];
     if (c > 0) e = hipStreamWaitEvent(sc, ev0, 0);
     if (e == hipSuccess)
-      e = wg_launch(recs + cls_start[c], cls_start[c + 1] - cls_start[c], cls_lds[c], fp64, direct, sc);
+      e = wg_launch(recs + cls_start[c], cls_start[c + 1] - cls_start[c], cls_lds[c], fp64, direct, cls_res[c] != 0,
+                    sc);
     if (e == hipSuccess && c > 0) e = hipEventRecord(xev[c - 1], sc);
     if (e == hipSuccess && c > 0) e = hipStreamWaitEvent(stream, xev[c - 1], 0);
   }
@@ -477,6 +716,10 @@ bool WgBatch::run(std::string& err) {
   if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev0, ev1);
   if (e != hipSuccess) { err = std::string("wg engine: ") + hipGetErrorString(e); return false; }
   kernel_ms = ms;
+  for (size_t b = 0; b < gsol.size(); ++b) {
+    if (!gsol[b]->run(err)) { err = "instance " + std::to_string(big[b]) + ": " + err; return false; }
+    kernel_ms += gsol[b]->kernel_ms;
+  }
   wall_ms = now_ms() - t0;
   return true;
 }
@@ -488,6 +731,7 @@ bool WgBatch::fetch(std::vector<DevResult>& res, std::string& err) {
   hipError_t e = hipMemcpyAsync(res.data(), base + res_off, sizeof(DevResult) * count,
                                 hipMemcpyDeviceToHost, stream);
   for (int i = 0; i < count && e == hipSuccess; ++i) {
+    if (is_big[i]) continue;
     Setup& s = S[i];
     e = hipMemcpyAsync(s.x.data(), base + xo[i], sizeof(double) * s.n, hipMemcpyDeviceToHost, stream);
     if (e == hipSuccess)
@@ -504,6 +748,8 @@ bool WgBatch::fetch(std::vector<DevResult>& res, std::string& err) {
     }
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
   if (e != hipSuccess) { err = std::string("download: ") + hipGetErrorString(e); return false; }
+  for (size_t b = 0; b < gsol.size(); ++b)
+    if (!gsol[b]->fetch(S[big[b]], res[big[b]], nullptr, err)) return false;
   return true;
 }
 
@@ -678,13 +924,10 @@ Sol* pdos_ex(const Data* d, const Cone* k, const PdosOptions* opt, PdosInfo* inf
   int engine = PDOS_ENGINE_WG;
   bool ok;
   if (o.engine == PDOS_ENGINE_GRID ||
-      (o.engine == PDOS_ENGINE_AUTO && !direct && grid_preferred(S[0], o.precision == PDOS_FP64))) {
+      (o.engine == PDOS_ENGINE_AUTO && grid_preferred(S[0], o.precision == PDOS_FP64, direct))) {
     engine = PDOS_ENGINE_GRID;
-    if (direct) {
-      g_err = "the grid engine runs the indirect plugin only";
-      return nullptr;
-    }
-    ok = solve_grid(S[0], *d->p, o.precision == PDOS_FP64, o.device, res, up, sv, err, &hist);
+    ok = solve_grid(S[0], *d->p, o.precision == PDOS_FP64, direct ? &F[0] : nullptr, o.device, res, up, sv, err,
+                    &hist);
   } else {
     ok = solve_wg(S, F, par, o.precision == PDOS_FP64, direct, o.device, res, up, sv, err, &hist);
   }
@@ -776,7 +1019,8 @@ static int batch_fetch(PdosBatchHandle* h, Sol** sols, PdosInfo* infos) {
   for (size_t i = 0; i < res.size(); ++i) {
     sols[i] = make_sol(h->B.S[i], res[i].status);
     if (infos)
-      fill_info(&infos[i], res[i], h->B.S[i], PDOS_ENGINE_WG, h->setup_ms, h->B.wall_ms,
+      fill_info(&infos[i], res[i], h->B.S[i], h->B.is_big[i] ? PDOS_ENGINE_GRID : PDOS_ENGINE_WG, h->setup_ms,
+                h->B.wall_ms,
                 h->B.direct ? (double)h->B.F[h->B.grp[i]].Lp.back() : 0.0);
   }
   return 0;

@@ -14,6 +14,27 @@ struct DevResult {
   double p_res, d_res, p_obj, d_obj, eta;
 };
 
+// Byte offsets, in LDS, of the static data the resident variant of the
+// workgroup engine (k_wg_res) keeps there for the whole solve: a group's
+// image (copied in one coalesced pass) followed by the instance's b and c,
+// its iterates and the reduction slots.
+// Rows are processed in wave batches {first, count, log2 G, 0}: count rows of
+// one class, G lanes each (G = the least power of two, at most 64, with at
+// most 4 entries per lane), so no lane walks a long row alone.
+// A product and A' product keep their rows in batch order: position q is
+// *pos[q] = {row, first entry, end entry, 0} of *idx / *val (direct plugin:
+// A's column indices also as pivot positions, ridp); the sweeps likewise.
+struct ResLayout {
+  int D, E, rpos, ridx, rval, ridp, rbat, cpos, cidx, cval, cbat, soc, perm, pinv;
+  int fpos, fidx, fval, fbat, flb, bpos, bidx, bval, bdq, bbat, blb;
+  int nrbat, ncbat;  // wave batches of A's rows and of A''s
+  int nfl, nbl;      // levels of the two sweeps (T excluded, forward: plus T's outside entries)
+  int t, trow, tdq, tl;  // the dense trailing chain: t rows, their ids, D, the t x t block (row-major)
+  int img;          // image bytes (multiple of 16)
+  int b, c, vec;    // per-instance b, c (binary64) and the iterates
+  int total;        // dynamic LDS bytes
+};
+
 // One problem instance for the workgroup engine.  All pointers are device
 // pointers; matrix values have the engine's bulk type (double or float).
 struct WgInst {
@@ -62,6 +83,11 @@ struct WgInst {
   int hist_rows;           // rows of hist (0: no history)
   double* hist;            // VERBOSE: residuals (p_res, d_res, p_obj, d_obj, eta) of checks 0, 10, 20, ...
   double* cg_hist;         // VERBOSE, indirect: per update {CG steps, final |q|} (hist_rows * 10 rows)
+  // resident variant: matrices, factor and level sets in LDS
+  int resident;            // 1: launched as k_wg_res with `lay`
+  int pad4;
+  const unsigned char* img;  // the group's LDS image in device memory (lay.img bytes)
+  ResLayout lay;
 };
 
 // Rows of the VERBOSE residual history for a MAX_ITERS: one per tenth check
@@ -75,6 +101,17 @@ constexpr int kWgThreads = 256;
 constexpr int kWgChunk = 2048;        // entries per SpMV row block (products staged in LDS)
 constexpr int kWgRowsPerBlock = 1024;  // rows per SpMV row block
 constexpr int kWgLongRow = 64;        // rows with this many entries are summed by a wave
+
+// Iterates of the resident variant, in LDS after b and c: x (n), s, y, s~,
+// A x (m) in the bulk type; indirect: p, q, u (n), t (m); direct: the KKT
+// vector (n + m binary64); then two reduction slots.
+inline long res_vec_bytes(int n, int m, int N, bool direct, int tsize) {
+  auto al = [](long v) { return (long)((v + 15) & ~15L); };
+  long bytes = al((long)(n + 4 * (long)m) * tsize);
+  if (direct) bytes += al((long)N * 8);
+  else bytes += al((long)(3 * (long)n + m) * tsize);
+  return bytes + 2 * 16 * 4 * 8;
+}
 
 // Dynamic LDS bytes for one instance in the workgroup engine.
 inline int wg_lds_bytes(int n, int m, int N, bool direct, int tsize) {

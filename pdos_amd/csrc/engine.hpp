@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -38,8 +39,37 @@ struct Packer {
 double now_ms();
 
 hipError_t device_copy(void* dst, const void* src, size_t bytes, hipStream_t st);
-hipError_t wg_launch(const WgInst* d_insts, int count, int lds_bytes, bool fp64, bool direct,
+
+// Dense shared-A products on split bf16 MFMA (dense_gemm.hip), binary32
+// feature-major right-hand sides (rows x ldb, ldb a multiple of 64):
+//   dense_at_pair  U, V planes (splits x n x ldb, stride `plane`) of A'Y, A'W
+//                  from A' split copies ATh / ATl (n64 x m64)
+//   dense_a        Z = A X from A split copies ARh / ARl (m64 x n64)
+//   dense_at_splits  the K split dense_at_pair uses for a width
+int dense_at_splits(int n, int m64, int ldb);
+hipError_t dense_at_pair(hipStream_t st, int m, int n, int ldb, int m64, const __bf16* ATh, const __bf16* ATl,
+                         const float* Y, const float* W, float* Up, float* Vp, size_t plane, int splits);
+hipError_t dense_a(hipStream_t st, int m, int n, int ldb, int n64, const __bf16* ARh, const __bf16* ARl,
+                   const float* X, float* Z);
+hipError_t wg_launch(const WgInst* d_insts, int count, int lds_bytes, bool fp64, bool direct, bool res,
                      hipStream_t st);
+
+// Whole-GPU engine for one large instance (both plugins), prepared once and
+// re-runnable from the prepared start.
+class GridSolver {
+ public:
+  virtual ~GridSolver() = default;
+  virtual bool run(std::string& err) = 0;
+  // the scaled x, s, y into S, the result and (VERBOSE) the residual history
+  virtual bool fetch(Setup& S, DevResult& r, std::vector<double>* hist, std::string& err) = 0;
+  double upload_ms = 0, solve_ms = 0, kernel_ms = 0;
+};
+
+// F: the direct plugin's factorization (nullptr: indirect plugin).  borrow:
+// the calling thread's device arena and stream (one pdos() call) instead of
+// memory and a stream of its own (a batch member).
+std::unique_ptr<GridSolver> grid_prepare(const Setup& S, const Params& par, bool fp64, const Factor* F, int device,
+                                         bool borrow, std::string& err);
 
 // A prepared batch for the workgroup engine: inputs resident on the device,
 // re-runnable from the same start (the kernel reads x0/s0/y0, writes x/s/y).
@@ -62,16 +92,23 @@ struct WgBatch {
   std::vector<size_t> ho;           // VERBOSE residual history offsets (rows in insts[i].hist_rows)
   std::vector<std::vector<double>> hist;  // fetched histories (VERBOSE instances)
   std::vector<WgInst> insts;  // host copy of the device records
+  // Members whose iterates exceed a workgroup's LDS run on the whole-GPU
+  // engine, one after another, after the workgroup launches.
+  std::vector<int> big;
+  std::vector<std::unique_ptr<GridSolver>> gsol;
+  std::vector<char> is_big;
   int lds = 0;
   // Launch classes: instances grouped by how many of their workgroups fit
   // in a CU's LDS, each class launched with its own LDS size on its own
   // stream; the device holds a second copy of the records in class order.
-  static constexpr int kClasses = 4;
+  // A class is (workgroups per CU, resident variant or not).
+  static constexpr int kOccMax = 4;
+  static constexpr int kClasses = 2 * kOccMax;
   std::vector<int> order;                 // class order -> instance
-  int ncls = 0, cls_start[kClasses + 1] = {0}, cls_lds[kClasses] = {0};
+  int ncls = 0, cls_start[kClasses + 1] = {0}, cls_lds[kClasses] = {0}, cls_res[kClasses] = {0};
   size_t cinst_off = 0;
-  hipStream_t xs[kClasses - 1] = {nullptr, nullptr, nullptr};
-  hipEvent_t xev[kClasses - 1] = {nullptr, nullptr, nullptr};
+  hipStream_t xs[kClasses - 1] = {};
+  hipEvent_t xev[kClasses - 1] = {};
   bool upload_records(std::string& err);
   double upload_ms = 0, kernel_ms = 0, wall_ms = 0;
   ~WgBatch();
@@ -88,11 +125,13 @@ bool solve_wg(std::vector<Setup>& S, std::vector<Factor>& F, const std::vector<c
               bool fp64, bool direct, int device, std::vector<DevResult>& res, double& upload_ms,
               double& solve_ms, std::string& err, std::vector<double>* hist0 = nullptr);
 
-// Whole-GPU engine for one large instance (indirect plugin).
-bool grid_preferred(const Setup& S, bool fp64);
-// hist (optional): with par.VERBOSE, the residuals of checks 0, 10, 20, ...
-// (5 per row, verbose_hist_rows(MAX_ITERS) rows)
-bool solve_grid(Setup& S, const Params& par, bool fp64, int device, std::vector<DevResult>& res,
+// Whether one instance goes to the whole-GPU engine (its iterates exceed a
+// workgroup's LDS, or it is large enough to fill the GPU).
+bool grid_preferred(const Setup& S, bool fp64, bool direct);
+// One solve on the whole-GPU engine; hist (optional): with par.VERBOSE, the
+// residuals of checks 0, 10, 20, ... (5 per row, verbose_hist_rows(MAX_ITERS)
+// rows)
+bool solve_grid(Setup& S, const Params& par, bool fp64, const Factor* F, int device, std::vector<DevResult>& res,
                 double& upload_ms, double& solve_ms, std::string& err, std::vector<double>* hist = nullptr);
 
 }  // namespace pdos_amd

@@ -96,6 +96,10 @@ struct FamArgs {
   const int* lrp; const int* lri; const float* lrx;
   int* done; int* status; int* iters; int* fallback;
   double* red;         // per instance: pres2, by, pres, dres, pobj, dobj, eta, spare
+  // dense A, mixed: U and V arrive as nsplit planes of partial sums over K
+  // (plane stride in elements), added by phase B in plane order
+  int nsplit;
+  long plane;
 };
 
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
@@ -644,6 +648,10 @@ __global__ __launch_bounds__(FT) void k_fam_b(FamArgs<T> a) {
         const size_t o = (size_t)i * a.ldb + inst;
         cv[k] = a.c[o]; xv[k] = a.X[o]; uv[k] = a.U[o];
         if (!a.judge) vv[k] = a.V[o];
+        for (int z = 1; z < a.nsplit; ++z) {
+          uv[k] += a.U[o + (size_t)z * a.plane];
+          if (!a.judge) vv[k] += a.V[o + (size_t)z * a.plane];
+        }
       }
     }
     const T lam = T(a.lam[inst]);
@@ -1410,17 +1418,6 @@ __global__ __launch_bounds__(256) void k_fam_xupdate(int mt, int nt, int ksteps,
   }
 }
 
-// dst = [hi(src); lo(src); hi(src)]: three bf16 planes of `total` values
-// (rows x ld, plane stride total), src = hi + lo to about 2^-16 relative.
-__global__ void k_split3(long total, const float* __restrict__ src, __bf16* __restrict__ dst) {
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
-    const float v = src[e];
-    const __bf16 hi = (__bf16)v;
-    dst[e] = hi;
-    dst[total + e] = (__bf16)(v - float(hi));
-    dst[2 * total + e] = hi;
-  }
-}
 
 template <typename T>
 __global__ void k_gather_out(int rows, int count, int ldb, const T* __restrict__ Z, double* __restrict__ out) {
@@ -1546,13 +1543,13 @@ struct Family {
   void *rp = nullptr, *ri = nullptr, *rx = nullptr, *cp = nullptr, *ci = nullptr, *cx = nullptr;
   void *soc = nullptr, *D = nullptr, *E = nullptr, *bshared = nullptr;
   void* Ad = nullptr;      // dense scaled A (m x n row-major, bulk type) when A is dense
-  // mixed precision, dense A: the A product runs as one bf16 GEMM with fp32
-  // accumulation on split operands (A = A_hi + A_lo, X = X_hi + X_lo, three
-  // products hi.hi + lo.hi + hi.lo; the A' products stay fp32 GEMMs, measured
-  // faster for that shape):
-  //   Ncat  rows [A_hi | A_hi | A_lo] row-major m x 3n
-  //   Ssp   [X_hi; X_lo; X_hi] planes of the multiplied x, 3n x ldb
-  void *Ncat = nullptr, *Ssp = nullptr;
+  // mixed precision, dense A: both products run as the hand-written split
+  // bf16 MFMA kernels (dense_gemm.hip; A = A_hi + A_lo, three products with
+  // fp32 accumulation) on A's split copies in MFMA operand order:
+  //   ATh, ATl  A' rows, n64 x m64;  ARh, ARl  A rows, m64 x n64 (zero padded)
+  //   Upl, Vpl  the A' products' split-K planes
+  void *ATh = nullptr, *ATl = nullptr, *ARh = nullptr, *ARl = nullptr, *Upl = nullptr, *Vpl = nullptr;
+  int m64 = 0, n64 = 0;
   bool split_dense = false;
   bool dense = false;
   void* Gd = nullptr;      // fp64 G (n x n row-major, symmetric) for dgemm
@@ -1957,17 +1954,23 @@ bool Family::create(const Data* d, const Cone* k, const PdosOptions& o, std::str
     if (!Ad) return false;
     split_dense = !fp64;
     if (split_dense) {
-      std::vector<__bf16> nc((size_t)3 * m * n);
+      m64 = (m + 63) & ~63;
+      n64 = (n + 63) & ~63;
+      std::vector<__bf16> th((size_t)n64 * m64, (__bf16)0.f), tl(th), rh((size_t)m64 * n64, (__bf16)0.f), rl(rh);
       for (int r = 0; r < m; ++r)
         for (int c = 0; c < n; ++c) {
           const float v = float(Arow[(size_t)r * n + c]);
           const __bf16 hi = (__bf16)v, lo = (__bf16)(v - float(hi));
-          nc[(size_t)r * 3 * n + c] = hi;
-          nc[(size_t)r * 3 * n + n + c] = hi;
-          nc[(size_t)r * 3 * n + 2 * n + c] = lo;
+          th[(size_t)c * m64 + r] = hi;
+          tl[(size_t)c * m64 + r] = lo;
+          rh[(size_t)r * n64 + c] = hi;
+          rl[(size_t)r * n64 + c] = lo;
         }
-      Ncat = upload(nc, err);
-      if (!Ncat) return false;
+      ATh = upload(th, err);
+      ATl = upload(tl, err);
+      ARh = upload(rh, err);
+      ARl = upload(rl, err);
+      if (!ATh || !ATl || !ARh || !ARl) return false;
     }
   }
   rocblas_int info = 0;
@@ -2064,7 +2067,12 @@ bool Family::load(int cnt, const double* C, const double* B, std::string& err) {
     q.pt0 = dalloc((size_t)std::max(1, S.nsoc()) * ldb * 8, batch_allocs, err);
   }
   Xo = vec(n); Yo = vec(m); So = vec(m);
-  if (split_dense) Ssp = dalloc((size_t)3 * n * ldb * 2, batch_allocs, err);
+  if (split_dense) {  // planes for the widest split of any width the batch compacts to
+    size_t cap = 0;
+    for (int w = 64; w <= ldb; w += 64) cap = std::max(cap, (size_t)dense_at_splits(n, m64, w) * w * n);
+    Upl = dalloc(cap * 4, batch_allocs, err);
+    Vpl = dalloc(cap * 4, batch_allocs, err);
+  }
   Io = dalloc((size_t)3 * ldb * 4, batch_allocs, err);
   Ro = dalloc((size_t)8 * ldb * 8, batch_allocs, err);
   dlist = static_cast<int*>(dalloc((size_t)ldb * 4, batch_allocs, err));
@@ -2119,6 +2127,8 @@ void Family::fill_args(FamArgs<T>& a) {
   a.nx_items = nx_items;
   a.nlin_items = nlin_items;
   a.nprod = nprod;
+  a.nsplit = 1;
+  a.plane = 0;
   if (hybrid()) {
     a.rec8 = rec8d;
     a.AXp = AXp;
@@ -2212,16 +2222,10 @@ bool Family::apply_A(FamArgs<T>& a, const T* Xin, T* Z) {
     return hipGetLastError() == hipSuccess;
   }
   const T one = T(1), zero = T(0);
-  if (split_dense) {
-    // Z' (ldb x m) = [X_hi; X_lo; X_hi]' (ldb x 3n) * Ncat' (Ncat row-major m x 3n = column-major 3n x m)
-    const long tot = (long)n * a.ldb;
-    hipLaunchKernelGGL(k_split3, dim3(2048), dim3(256), 0, st, tot, reinterpret_cast<const float*>(Xin),
-                       static_cast<__bf16*>(Ssp));
-    const float fone = 1.f, fzero = 0.f;
-    return rocblas_gemm_ex(blas, rocblas_operation_none, rocblas_operation_none, a.ldb, m, 3 * n, &fone, Ssp,
-                           rocblas_datatype_bf16_r, a.ldb, Ncat, rocblas_datatype_bf16_r, 3 * n, &fzero, Z,
-                           rocblas_datatype_f32_r, a.ldb, Z, rocblas_datatype_f32_r, a.ldb, rocblas_datatype_f32_r,
-                           rocblas_gemm_algo_standard, 0, 0) == rocblas_status_success;
+  if constexpr (std::is_same<T, float>::value) {
+    if (split_dense)  // Z = A X, split bf16 MFMA (dense_gemm.hip)
+      return dense_a(st, m, n, a.ldb, n64, static_cast<const __bf16*>(ARh), static_cast<const __bf16*>(ARl), Xin, Z) ==
+             hipSuccess;
   }
   // column-major Z' (ldb x m) = X' (ldb x n) * A^T, A^T stored column-major n x m (ld n)
   if constexpr (std::is_same<T, double>::value)
@@ -2277,7 +2281,21 @@ bool Family::at_phase_b(FamArgs<T>& a) {
     hipLaunchKernelGGL(k_fam_at_b<T>, dim3(cols * curAt->nAt), dim3(FT), 0, st, a, in);
     return hipGetLastError() == hipSuccess;
   }
-  if (!apply_At(a, a.Y, a.W, a.U, a.V)) return false;
+  bool done_at = false;
+  if constexpr (std::is_same<T, float>::value) {
+    if (split_dense) {  // U, V = A'Y, A'W as split-K planes (dense_gemm.hip), summed by phase B
+      const int sp = dense_at_splits(n, m64, a.ldb);
+      a.U = static_cast<float*>(Upl);
+      a.V = static_cast<float*>(Vpl);
+      a.nsplit = sp;
+      a.plane = (long)n * a.ldb;
+      if (dense_at_pair(st, m, n, a.ldb, m64, static_cast<const __bf16*>(ATh), static_cast<const __bf16*>(ATl), a.Y,
+                        a.W, a.U, a.V, (size_t)a.plane, sp) != hipSuccess)
+        return false;
+      done_at = true;
+    }
+  }
+  if (!done_at && !apply_At(a, a.Y, a.W, a.U, a.V)) return false;
   const dim3 gn(a.ldb / LANES, (n + RPC - 1) / RPC);
   if (fp64) hipLaunchKernelGGL((k_fam_b<T, false>), gn, dim3(FT), 0, st, a);
   else hipLaunchKernelGGL((k_fam_b<T, true>), gn, dim3(FT), 0, st, a);

@@ -1,6 +1,7 @@
-// grid_engine.hip -- one large cone program on the whole GPU (indirect plugin).
+// grid_engine.hip -- one large cone program on the whole GPU (both plugins).
 //
-// Each phase of the ADMM iteration is one kernel over the whole GPU.
+// Each phase of the ADMM iteration is one kernel over the whole GPU, except
+// the direct plugin's triangular sweeps (below).
 //
 // SpMV (CSR-stream): the host cuts the rows into blocks of nearly equal
 // nonzero count (at most kCapMax; a longer row is a block of its own), sized
@@ -24,6 +25,19 @@
 // iteration with k CG steps: k+1 with A and k+2 with A' (the reference runs
 // k+3 and k+2: the A x row sums of s~ = b - A x are reused by the next
 // residual check and CG start, exactly, since x does not change in between).
+//
+// Direct plugin (direct/private.c:16-56, ldl.c:326-428): the permuted
+// right-hand side, the forward sweep with L, the backward sweep with D and
+// L', and the un-permute are launches of their own.  A sweep follows the
+// host's level sets (rows of a level are independent): its rows are stored
+// in level order with their entries contiguous, so a run of consecutive
+// levels is one contiguous range.  Runs of narrow levels go to one
+// workgroup of 1,024 threads that stages each chunk of levels -- entries,
+// row ids, row pointers, level pointers, D -- in LDS with coalesced loads
+// and keeps z in LDS when it fits, so a level costs LDS accesses and one
+// barrier, not global round trips; levels of many rows (a leaf level of a
+// large problem) are one launch over the whole GPU each.  The sweeps run in
+// binary64 in both precision modes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -52,6 +66,19 @@ template <typename T> using Pair = typename PairOf<T>::type;
 __device__ __forceinline__ double2 make_pair2(double a, double b) { return make_double2(a, b); }
 __device__ __forceinline__ float2 make_pair2(float a, float b) { return make_float2(a, b); }
 
+// One triangular sweep in level order: position q solves z[row[q]] from the
+// entries [ptr[q], ptr[q+1]) (indices into z, binary64 values) and, for the
+// backward sweep, divides by dq[q] (D of L D L' at that row) first.
+struct SweepDev {
+  const int* row;
+  const int* ptr;
+  const int* idx;
+  const double* val;
+  const double* dq;
+  const int* lptr;   // level -> first position
+  const int4* chunk; // staged chunks {lv0, lv1, e0, e1} of runs of narrow levels
+};
+
 struct GridScal {
   double rho[2], beta;  // rho of CG step k in rho[k & 1]
   double pres, dres, pobj, dobj, eta;
@@ -76,6 +103,12 @@ struct GArgs {
   GridScal* sc;
   double* hist;    // VERBOSE: residuals of checks 0, 10, 20, ... (5 each), or null
   int hist_rows;
+  // direct plugin
+  int direct;      // 1: direct plugin (no CG start argument t)
+  int N, nb_N;     // n + m; vector-pass blocks over N
+  const int* perm; // perm[k] = KKT index of pivot k
+  double* z;       // [N] right-hand side and solution, pivot order
+  SweepDev fw, bw; // forward (L) and backward (D, L') sweeps
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -218,7 +251,7 @@ __global__ __launch_bounds__(GT) SPMV_OCC void k_resid(GArgs<T> a) {
       const T r = ((sj - bj) + axj) / T(a.D[j]);
       v[0] += double(r) * double(r);
       v[3] += double(bj) * double(yj);
-      a.t[j] = ((sj + lam * yj) - bj) + axj;
+      if (!a.direct) a.t[j] = ((sj + lam * yj) - bj) + axj;
     }
   }
   block_sum<4>(v, red);
@@ -226,13 +259,11 @@ __global__ __launch_bounds__(GT) SPMV_OCC void k_resid(GArgs<T> a) {
     for (int k = 0; k < 4; ++k) a.p_res[(size_t)bid * 4 + k] = v[k];
 }
 
-// The stop test (every workgroup, same order, same result), then the CG
-// start q = -lam c - A't, p = 0 and |q|^2 partials.
+// The stop test (pdos.c:95-107) from the residual partials: every workgroup
+// sums them in the same order and reaches the same decision; workgroup 0
+// records the residuals, the iteration count and the VERBOSE history.
 template <typename T>
-__global__ __launch_bounds__(GT) SPMV_OCC void k_cginit(GArgs<T> a) {
-  __shared__ T prod[kCapMax];
-  __shared__ double red[GW * 4];
-  if (a.sc->done) return;
+__device__ __forceinline__ bool stop_check(const GArgs<T>& a, double* red) {
   double tot[4];
   sum_partials<4>(a.p_res, a.nb_at + a.nb_m, tot, red);
   const double pres = sqrt(tot[0]), dres = sqrt(tot[1]), pobj = tot[2], dobj = -tot[3];
@@ -253,7 +284,17 @@ __global__ __launch_bounds__(GT) SPMV_OCC void k_cginit(GArgs<T> a) {
       sc->iters += 1;
     }
   }
-  if (stop) return;
+  return stop;
+}
+
+// The stop test, then the CG start q = -lam c - A't, p = 0 and |q|^2
+// partials.
+template <typename T>
+__global__ __launch_bounds__(GT) SPMV_OCC void k_cginit(GArgs<T> a) {
+  __shared__ T prod[kCapMax];
+  __shared__ double red[GW * 4];
+  if (a.sc->done) return;
+  if (stop_check(a, red)) return;
   const T lam = T(a.lambda);
   double v[1] = {0.0};
   stream_block<T>(a.cblk, a.cp, a.ci, a.cx, [&](int c) { return a.t[c]; }, [&](int i, T w) {
@@ -359,7 +400,9 @@ __global__ __launch_bounds__(GT) void k_cg_upd(GArgs<T> a, int k) {
 
 // s~ = b - A x, relax, zero / nonneg cones and their dual update; SOC rows
 // keep s~ and z = s~ - lambda y for k_cone (pdos.c:232-249, 296-303).
-template <typename T>
+// Direct plugin: s~ comes from the KKT solve (k_post) instead of b - A x;
+// A x is still formed for the next residual check.
+template <typename T, bool DIRECT>
 __global__ __launch_bounds__(GT) SPMV_OCC void k_fin(GArgs<T> a) {
   __shared__ T prod[kCapMax];
   __shared__ double red[GW * 4];
@@ -367,7 +410,7 @@ __global__ __launch_bounds__(GT) SPMV_OCC void k_fin(GArgs<T> a) {
   const T lam = T(a.lambda), ra = T(a.relax), rb = T(1.0) - T(a.relax);
   stream_block<T>(a.rblk, a.rp, a.ri, a.rx, [&](int c) { return a.x[c]; }, [&](int j, T w) {
     a.ax[j] = w;
-    T stj = T(a.b[j]) - w;
+    T stj = DIRECT ? a.st[j] : T(a.b[j]) - w;
     stj = ra * stj + rb * a.s[j];
     const T z = stj - lam * a.y[j];
     if (j < a.f) {
@@ -408,18 +451,308 @@ __global__ __launch_bounds__(GT) void k_cone(GArgs<T> a) {
   }
 }
 
+// ---------------------------------------------------------------- direct
+
+// The stop test, then the permuted KKT right-hand side (direct/private.c:
+// 16-31, ldl.c:395-405): z[k] = v[perm[k]], v = [lam c - x ; b - (s + lam y)].
 template <typename T>
-void enqueue_iteration(const GArgs<T>& a, hipStream_t st) {
+__global__ __launch_bounds__(GT) void k_rhs(GArgs<T> a) {
+  __shared__ double red[GW * 4];
+  if (a.sc->done) return;
+  if (stop_check(a, red)) return;
+  const double lam = a.lambda;
+  const int per = (a.N + a.nb_N - 1) / a.nb_N, k0 = blockIdx.x * per, k1 = min(a.N, k0 + per);
+  for (int k = k0 + (int)threadIdx.x; k < k1; k += GT) {
+    const int src = a.perm[k];
+    double v;
+    if (src < a.n) {
+      v = lam * a.c[src] - double(a.x[src]);
+    } else {
+      const int j = src - a.n;
+      v = a.b[j] - (double(a.s[j]) + lam * double(a.y[j]));
+    }
+    a.z[k] = v;
+  }
+}
+
+// Un-permute (ldl.c:412-428) and direct/private.c:48-55: x = z[0:n],
+// s~ = z[n:] + (s + lam y).
+template <typename T>
+__global__ __launch_bounds__(GT) void k_post(GArgs<T> a) {
+  if (a.sc->done) return;
+  const double lam = a.lambda;
+  const int per = (a.N + a.nb_N - 1) / a.nb_N, k0 = blockIdx.x * per, k1 = min(a.N, k0 + per);
+  for (int k = k0 + (int)threadIdx.x; k < k1; k += GT) {
+    const int dst = a.perm[k];
+    const double v = a.z[k];
+    if (dst < a.n) {
+      a.x[dst] = T(v);
+    } else {
+      const int j = dst - a.n;
+      a.st[j] = T(v + (double(a.s[j]) + lam * double(a.y[j])));
+    }
+  }
+}
+
+constexpr int ST = 1024;        // threads of the staged sweep workgroup
+constexpr int SW = ST / 64;
+constexpr int kRunRows = 1024;  // rows (positions) a staged chunk holds
+constexpr int kRunLevels = 1024;  // levels a staged chunk holds
+
+// Solves the positions [q0, q1) of one level.  Rows get a wave each when the
+// level is short or its rows are long (lanes stride the row, then a wave
+// sum), else a thread each (entries in storage order).  Acc supplies row(q),
+// ptr(q), idx(e), val(e), dq(q) (staged or global) and Z the vector.
+template <bool BACK, typename Acc, typename Z>
+__device__ __forceinline__ void sweep_level(int q0, int q1, const Acc& ac, Z& zv) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int nrows = q1 - q0;
+  if (nrows <= 0) return;
+  const bool wave_rows = nrows <= SW || (ac.ptr(q1) - ac.ptr(q0)) >= 32 * nrows;
+  if (wave_rows) {
+    for (int q = q0 + w; q < q1; q += SW) {
+      const int e0 = ac.ptr(q), e1 = ac.ptr(q + 1);
+      double acc = 0.0;
+      for (int e = e0 + lane; e < e1; e += 64) acc += ac.val(e) * zv.get(ac.idx(e));
+      acc = wave_sum(acc);
+      if (lane == 0) {
+        const int i = ac.row(q);
+        zv.set(i, (BACK ? zv.get(i) / ac.dq(q) : zv.get(i)) - acc);
+      }
+    }
+  } else {
+    for (int q = q0 + tid; q < q1; q += ST) {
+      const int i = ac.row(q), e0 = ac.ptr(q), e1 = ac.ptr(q + 1);
+      double v = BACK ? zv.get(i) / ac.dq(q) : zv.get(i);
+      for (int e = e0; e < e1; ++e) v -= ac.val(e) * zv.get(ac.idx(e));
+      zv.set(i, v);
+    }
+  }
+}
+
+struct GlobalAcc {  // a sweep read from global memory
+  const SweepDev* s;
+  __device__ int row(int q) const { return s->row[q]; }
+  __device__ int ptr(int q) const { return s->ptr[q]; }
+  __device__ int idx(int e) const { return s->idx[e]; }
+  __device__ double val(int e) const { return s->val[e]; }
+  __device__ double dq(int q) const { return s->dq[q]; }
+};
+
+struct StagedAcc {  // a chunk staged in LDS (positions and entries rebased)
+  const int* srow; const int* sptr; const int* sidx; const double* sval; const double* sdq;
+  int q0, e0;
+  __device__ int row(int q) const { return srow[q - q0]; }
+  __device__ int ptr(int q) const { return sptr[q - q0] + e0; }
+  __device__ int idx(int e) const { return sidx[e - e0]; }
+  __device__ double val(int e) const { return sval[e - e0]; }
+  __device__ double dq(int q) const { return sdq[q - q0]; }
+};
+
+struct ZGlobal {
+  double* z;
+  __device__ double get(int i) const { return z[i]; }
+  __device__ void set(int i, double v) const { z[i] = v; }
+};
+struct ZShared {
+  double* z;
+  __device__ double get(int i) const { return z[i]; }
+  __device__ void set(int i, double v) const { z[i] = v; }
+};
+
+// Runs the staged chunks [c0, c1) of one sweep in one workgroup.  Dynamic
+// LDS: [z (N doubles, when ZLDS)] [entries: cap doubles, cap ints]
+// [rows: kRunRows ints, kRunRows + 1 ptrs, kRunRows doubles] [kRunLevels + 1
+// level pointers].  A chunk whose level alone exceeds cap entries or
+// kRunRows rows is read from global memory instead.
+template <typename T, bool BACK, bool ZLDS>
+__global__ __launch_bounds__(ST) void k_sweep_run(GArgs<T> a, int c0, int c1, int cap) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  if (a.sc->done) return;
+  const SweepDev& sw = BACK ? a.bw : a.fw;
+  const int tid = threadIdx.x;
+  unsigned char* ptrb = smem;
+  double* zs = nullptr;
+  if (ZLDS) {
+    zs = reinterpret_cast<double*>(ptrb);
+    ptrb += (((size_t)a.N * 8) + 15) & ~(size_t)15;
+  }
+  double* sval = reinterpret_cast<double*>(ptrb);
+  ptrb += (size_t)cap * 8;
+  double* sdq = reinterpret_cast<double*>(ptrb);
+  ptrb += (size_t)kRunRows * 8;
+  int* sidx = reinterpret_cast<int*>(ptrb);
+  ptrb += (((size_t)cap * 4) + 15) & ~(size_t)15;
+  int* srow = reinterpret_cast<int*>(ptrb);
+  ptrb += (size_t)kRunRows * 4;
+  int* sptr = reinterpret_cast<int*>(ptrb);
+  ptrb += (size_t)(kRunRows + 4) * 4;
+  int* slp = reinterpret_cast<int*>(ptrb);
+  if (ZLDS)
+    for (int i = tid; i < a.N; i += ST) zs[i] = a.z[i];
+  auto run_levels = [&](auto& zv) {
+    for (int c = c0; c < c1; ++c) {
+      const int4 ch = sw.chunk[c];
+      const int lv0 = ch.x, lv1 = ch.y, e0 = ch.z, e1 = ch.w;
+      const int p0 = sw.lptr[lv0], p1 = sw.lptr[lv1];
+      const bool staged = e1 - e0 <= cap && p1 - p0 <= kRunRows && lv1 - lv0 <= kRunLevels;
+      __syncthreads();  // the previous chunk's staged data and z writes are done
+      if (staged) {
+        for (int e = e0 + tid; e < e1; e += ST) { sval[e - e0] = sw.val[e]; sidx[e - e0] = sw.idx[e]; }
+        for (int q = p0 + tid; q <= p1; q += ST) {
+          sptr[q - p0] = sw.ptr[q] - e0;
+          if (q < p1) {
+            srow[q - p0] = sw.row[q];
+            if (BACK) sdq[q - p0] = sw.dq[q];
+          }
+        }
+        for (int l = lv0 + tid; l <= lv1; l += ST) slp[l - lv0] = sw.lptr[l];
+        __syncthreads();
+        const StagedAcc ac{srow, sptr, sidx, sval, sdq, p0, e0};
+        for (int lv = lv0; lv < lv1; ++lv) {
+          sweep_level<BACK>(slp[lv - lv0], slp[lv - lv0 + 1], ac, zv);
+          __syncthreads();
+        }
+      } else {
+        const GlobalAcc ac{&sw};
+        for (int lv = lv0; lv < lv1; ++lv) {
+          sweep_level<BACK>(sw.lptr[lv], sw.lptr[lv + 1], ac, zv);
+          __syncthreads();
+        }
+      }
+    }
+  };
+  if (ZLDS) {
+    ZShared zv{zs};
+    run_levels(zv);
+    __syncthreads();
+    for (int i = tid; i < a.N; i += ST) a.z[i] = zs[i];
+  } else {
+    ZGlobal zv{a.z};
+    run_levels(zv);
+  }
+}
+
+// One wide level over the whole GPU: a thread per row.
+template <typename T, bool BACK>
+__global__ __launch_bounds__(GT) void k_sweep_wide(GArgs<T> a, int lv) {
+  if (a.sc->done) return;
+  const SweepDev& sw = BACK ? a.bw : a.fw;
+  const int q = sw.lptr[lv] + blockIdx.x * GT + (int)threadIdx.x;
+  if (q >= sw.lptr[lv + 1]) return;
+  const int i = sw.row[q], e0 = sw.ptr[q], e1 = sw.ptr[q + 1];
+  double v = BACK ? a.z[i] / sw.dq[q] : a.z[i];
+  for (int e = e0; e < e1; ++e) v -= sw.val[e] * a.z[sw.idx[e]];
+  a.z[i] = v;
+}
+
+// Host schedule of one sweep: what each launch covers.
+struct SweepSeg {
+  int wide;      // 1: one wide level `lv` (k_sweep_wide); 0: chunks [c0, c1) (k_sweep_run)
+  int lv, c0, c1;
+  int rows;      // wide: rows of the level
+};
+
+// Host schedule of both sweeps plus what the launches need.
+struct SweepPlan {
+  std::vector<SweepSeg> fw, bw;
+  bool zlds = false;
+  int cap = 0, lds = 0;
+};
+
+template <typename T, bool BACK>
+void enqueue_sweep(const GArgs<T>& a, const std::vector<SweepSeg>& segs, const SweepPlan& pl, hipStream_t st) {
+  for (const SweepSeg& g : segs) {
+    if (g.wide) {
+      hipLaunchKernelGGL((k_sweep_wide<T, BACK>), dim3((g.rows + GT - 1) / GT), dim3(GT), 0, st, a, g.lv);
+    } else if (pl.zlds) {
+      hipLaunchKernelGGL((k_sweep_run<T, BACK, true>), dim3(1), dim3(ST), pl.lds, st, a, g.c0, g.c1, pl.cap);
+    } else {
+      hipLaunchKernelGGL((k_sweep_run<T, BACK, false>), dim3(1), dim3(ST), pl.lds, st, a, g.c0, g.c1, pl.cap);
+    }
+  }
+}
+
+template <typename T>
+void enqueue_iteration(const GArgs<T>& a, const SweepPlan& pl, hipStream_t st) {
   const int cone_blocks = std::max(1, std::min(2048, (a.nsoc + GW - 1) / GW));
   hipLaunchKernelGGL(k_resid<T>, dim3(a.nb_at + a.nb_m), dim3(GT), 0, st, a);
-  hipLaunchKernelGGL(k_cginit<T>, dim3(a.nb_at), dim3(GT), 0, st, a);
-  for (int k = 0; k < a.cg_max; ++k) {
-    hipLaunchKernelGGL(k_cg_a<T>, dim3(a.nb_a), dim3(GT), 0, st, a, k);
-    hipLaunchKernelGGL(k_cg_at<T>, dim3(a.nb_at), dim3(GT), 0, st, a);
-    hipLaunchKernelGGL(k_cg_upd<T>, dim3(a.nb_n), dim3(GT), 0, st, a, k);
+  if (a.direct) {
+    hipLaunchKernelGGL(k_rhs<T>, dim3(a.nb_N), dim3(GT), 0, st, a);
+    enqueue_sweep<T, false>(a, pl.fw, pl, st);
+    enqueue_sweep<T, true>(a, pl.bw, pl, st);
+    hipLaunchKernelGGL(k_post<T>, dim3(a.nb_N), dim3(GT), 0, st, a);
+    hipLaunchKernelGGL((k_fin<T, true>), dim3(a.nb_a), dim3(GT), 0, st, a);
+  } else {
+    hipLaunchKernelGGL(k_cginit<T>, dim3(a.nb_at), dim3(GT), 0, st, a);
+    for (int k = 0; k < a.cg_max; ++k) {
+      hipLaunchKernelGGL(k_cg_a<T>, dim3(a.nb_a), dim3(GT), 0, st, a, k);
+      hipLaunchKernelGGL(k_cg_at<T>, dim3(a.nb_at), dim3(GT), 0, st, a);
+      hipLaunchKernelGGL(k_cg_upd<T>, dim3(a.nb_n), dim3(GT), 0, st, a, k);
+    }
+    hipLaunchKernelGGL((k_fin<T, false>), dim3(a.nb_a), dim3(GT), 0, st, a);
   }
-  hipLaunchKernelGGL(k_fin<T>, dim3(a.nb_a), dim3(GT), 0, st, a);
   hipLaunchKernelGGL(k_cone<T>, dim3(cone_blocks), dim3(GT), 0, st, a);
+}
+
+// One sweep's arrays in level order.  Forward: row i of L (CSR) at each
+// position; backward: column j of L (a row of L') and D[j].
+struct SweepHost {
+  std::vector<int> row, ptr, idx, lptr;
+  std::vector<double> val, dq;
+  std::vector<int4> chunk;
+};
+
+// Levels of at least `wide_rows` rows become wide launches; each run of
+// narrower levels is cut into chunks of whole levels holding at most `cap`
+// entries, kRunRows rows and kRunLevels levels (a level alone beyond that is
+// a chunk of its own, read from global memory by the run kernel).
+void build_sweep(const Factor& F, bool back, int wide_rows, int cap, SweepHost& h, std::vector<SweepSeg>& segs) {
+  const int N = F.N;
+  const std::vector<int>& lp = back ? F.bl_ptr : F.fl_ptr;
+  const std::vector<int>& lr = back ? F.bl_rows : F.fl_rows;
+  const std::vector<int>& P = back ? F.Lp : F.Rp;
+  const std::vector<int>& I = back ? F.Li : F.Ri;
+  const std::vector<double>& V = back ? F.Lx : F.Rx;
+  const int nlev = (int)lp.size() - 1;
+  h.row.assign(lr.begin(), lr.end());
+  h.lptr.assign(lp.begin(), lp.end());
+  h.ptr.assign(N + 1, 0);
+  for (int q = 0; q < N; ++q) h.ptr[q + 1] = h.ptr[q] + (P[h.row[q] + 1] - P[h.row[q]]);
+  h.idx.resize(h.ptr[N]);
+  h.val.resize(h.ptr[N]);
+  for (int q = 0; q < N; ++q) {
+    const int r = h.row[q];
+    std::copy(I.begin() + P[r], I.begin() + P[r + 1], h.idx.begin() + h.ptr[q]);
+    std::copy(V.begin() + P[r], V.begin() + P[r + 1], h.val.begin() + h.ptr[q]);
+  }
+  if (back) {
+    h.dq.resize(N);
+    for (int q = 0; q < N; ++q) h.dq[q] = F.Ld[h.row[q]];
+  }
+  auto rows_of = [&](int lv) { return h.lptr[lv + 1] - h.lptr[lv]; };
+  auto ents = [&](int l0, int l1) { return h.ptr[h.lptr[l1]] - h.ptr[h.lptr[l0]]; };
+  h.chunk.clear();
+  segs.clear();
+  int lv = 0;
+  while (lv < nlev) {
+    if (rows_of(lv) >= wide_rows) {
+      segs.push_back(SweepSeg{1, lv, 0, 0, rows_of(lv)});
+      ++lv;
+      continue;
+    }
+    SweepSeg g{0, 0, (int)h.chunk.size(), 0, 0};
+    while (lv < nlev && rows_of(lv) < wide_rows) {
+      int l1 = lv + 1;
+      while (l1 < nlev && rows_of(l1) < wide_rows && ents(lv, l1 + 1) <= cap &&
+             h.lptr[l1 + 1] - h.lptr[lv] <= kRunRows && l1 + 1 - lv <= kRunLevels)
+        ++l1;
+      h.chunk.push_back(int4{lv, l1, h.ptr[h.lptr[lv]], h.ptr[h.lptr[l1]]});
+      lv = l1;
+    }
+    g.c1 = (int)h.chunk.size();
+    segs.push_back(g);
+  }
 }
 
 // Row blocks {r0, r1, p0, p1} for CSR-stream.  The nonzero cap is chosen so
@@ -456,132 +789,282 @@ size_t put_as(Packer& P, const std::vector<double>& v) {
 }
 
 template <typename T>
-bool run_grid(Setup& S, const Params& par, int device, std::vector<DevResult>& res, double& upload_ms,
-              double& solve_ms, std::string& err, std::vector<double>* hist) {
-  Ctx& C = ctx();
-  if (!C.ensure(device, err)) return false;
-  const double t0 = now_ms();
-  const int n = S.n, m = S.m;
-  int occ = 0;  // resident SpMV workgroups per CU (4 by design)
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_cg_a<T>, GT, 0) != hipSuccess || occ < 1) occ = 4;
-  const int slots = occ * C.num_cu;
-  const std::vector<int4> rb = row_blocks(S.rp, slots), cb = row_blocks(S.cp, slots);
-  const int nb_m = std::max(1, std::min(kVecBlocks, (m + GT - 1) / GT));
-  const int nb_n = std::max(1, std::min(kVecBlocks, (n + GT - 1) / GT));
-  Packer P;
-  const size_t o_rp = P.put(S.rp.data(), S.rp.size() * 4), o_ri = P.put(S.ri.data(), S.ri.size() * 4);
-  const size_t o_rx = put_as<T>(P, S.rx), o_rb = P.put(rb.data(), rb.size() * sizeof(int4));
-  const size_t o_cp = P.put(S.cp.data(), S.cp.size() * 4), o_ci = P.put(S.ci.data(), S.ci.size() * 4);
-  const size_t o_cx = put_as<T>(P, S.cx), o_cb = P.put(cb.data(), cb.size() * sizeof(int4));
-  const size_t o_b = P.put(S.b.data(), m * 8), o_c = P.put(S.c.data(), n * 8);
-  const size_t o_D = P.put(S.D.data(), m * 8), o_E = P.put(S.E.data(), n * 8);
-  const size_t o_soc = P.put(S.soc_ptr.data(), S.soc_ptr.size() * 4);
-  const size_t o_x = put_as<T>(P, S.x), o_s = put_as<T>(P, S.s), o_y = put_as<T>(P, S.y);
-  const size_t o_ax = P.zeros(m * sizeof(T)), o_st = P.zeros(m * sizeof(T)), o_t = P.zeros(m * sizeof(T));
-  const size_t o_pq = P.zeros(n * sizeof(Pair<T>)), o_u = P.zeros(n * sizeof(T));
-  const size_t o_pres = P.zeros((cb.size() + nb_m) * 4 * 8), o_prho = P.zeros(cb.size() * 8);
-  const size_t o_ppu = P.zeros(cb.size() * 8), o_pqn = P.zeros(nb_n * 8);
-  GridScal sc0;
-  std::memset(&sc0, 0, sizeof(sc0));
-  sc0.status = 1;
-  sc0.pres = sc0.dres = sc0.pobj = sc0.dobj = sc0.eta = -1.0;
-  const size_t o_sc = P.put(&sc0, sizeof(sc0));
-  const int hrows = par.VERBOSE ? verbose_hist_rows(par.MAX_ITERS) : 0;
-  const size_t o_hist = P.zeros(sizeof(double) * 5 * (size_t)std::max(hrows, 1));
-  if (!C.reserve(P.buf.size(), std::max(P.buf.size(), (size_t)(n + 2 * m) * 8 + 4096), err)) return false;
-  unsigned char* base = static_cast<unsigned char*>(C.dbuf);
-  auto at = [&](size_t off) { return base + off; };
+class GridImpl : public GridSolver {
+ public:
+  ~GridImpl() override {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (own) {
+      if (dbuf) (void)hipFree(dbuf);
+      if (hbuf) (void)hipHostFree(hbuf);
+      if (stream) (void)hipStreamDestroy(stream);
+    }
+  }
+
+  bool prepare(const Setup& S, const Params& par, const Factor* F, int device, bool borrow, std::string& err) {
+    Ctx& C = ctx();
+    if (!C.ensure(device, err)) return false;
+    own = !borrow;
+    const double t0 = now_ms();
+    n = S.n; m = S.m;
+    const int N = n + m;
+    int occ = 0;  // resident SpMV workgroups per CU (4 by design)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_cg_a<T>, GT, 0) != hipSuccess || occ < 1) occ = 4;
+    const int slots = occ * C.num_cu;
+    const std::vector<int4> rb = row_blocks(S.rp, slots), cb = row_blocks(S.cp, slots);
+    const int nb_m = std::max(1, std::min(kVecBlocks, (m + GT - 1) / GT));
+    const int nb_n = std::max(1, std::min(kVecBlocks, (n + GT - 1) / GT));
+    Packer P;
+    const size_t o_rp = P.put(S.rp.data(), S.rp.size() * 4), o_ri = P.put(S.ri.data(), S.ri.size() * 4);
+    const size_t o_rx = put_as<T>(P, S.rx), o_rb = P.put(rb.data(), rb.size() * sizeof(int4));
+    const size_t o_cp = P.put(S.cp.data(), S.cp.size() * 4), o_ci = P.put(S.ci.data(), S.ci.size() * 4);
+    const size_t o_cx = put_as<T>(P, S.cx), o_cb = P.put(cb.data(), cb.size() * sizeof(int4));
+    const size_t o_b = P.put(S.b.data(), m * 8), o_c = P.put(S.c.data(), n * 8);
+    const size_t o_D = P.put(S.D.data(), m * 8), o_E = P.put(S.E.data(), n * 8);
+    const size_t o_soc = P.put(S.soc_ptr.data(), S.soc_ptr.size() * 4);
+    const size_t o_x0 = put_as<T>(P, S.x), o_s0 = put_as<T>(P, S.s), o_y0 = put_as<T>(P, S.y);
+    const size_t o_x = P.zeros(n * sizeof(T)), o_s = P.zeros(m * sizeof(T)), o_y = P.zeros(m * sizeof(T));
+    const size_t o_ax = P.zeros(m * sizeof(T)), o_st = P.zeros(m * sizeof(T));
+    const bool direct = F != nullptr;
+    const size_t o_t = P.zeros((direct ? 1 : m) * sizeof(T));
+    const size_t o_pq = P.zeros((direct ? 1 : n) * sizeof(Pair<T>)), o_u = P.zeros((direct ? 1 : n) * sizeof(T));
+    const size_t o_pres = P.zeros((cb.size() + nb_m) * 4 * 8), o_prho = P.zeros(cb.size() * 8);
+    const size_t o_ppu = P.zeros(cb.size() * 8), o_pqn = P.zeros(nb_n * 8);
+    std::memset(&sc0, 0, sizeof(sc0));
+    sc0.status = 1;
+    sc0.pres = sc0.dres = sc0.pobj = sc0.dobj = sc0.eta = -1.0;
+    const size_t o_sc = P.put(&sc0, sizeof(sc0));
+    hrows = par.VERBOSE ? verbose_hist_rows(par.MAX_ITERS) : 0;
+    const size_t o_hist = P.zeros(sizeof(double) * 5 * (size_t)std::max(hrows, 1));
+    // direct plugin: permutation, z and both sweeps in level order
+    SweepHost fw, bw;
+    size_t o_perm = 0, o_z = 0, o_f[7] = {0}, o_b7[7] = {0};
+    if (direct) {
+      const char* wr = std::getenv("PDOS_SWEEP_WIDE_ROWS");
+      const int wide_rows = wr && std::atoi(wr) > 0 ? std::atoi(wr) : 8192;
+      const int fixed = kRunRows * 8 + kRunRows * 4 + (kRunRows + 4) * 4 + (kRunLevels + 1) * 4 + 64;
+      const long zbytes = (((long)N * 8) + 15) & ~15L;
+      plan.zlds = zbytes <= 96 * 1024 && zbytes + fixed + 12 * 1024 <= C.max_lds;
+      const long left = (long)C.max_lds - fixed - (plan.zlds ? zbytes : 0);
+      plan.cap = (int)std::max<long>(256, std::min<long>(1 << 20, (left - 16) / 12) & ~3L);
+      plan.lds = (int)((plan.zlds ? zbytes : 0) + (long)plan.cap * 8 + kRunRows * 8 +
+                       ((((long)plan.cap * 4) + 15) & ~15L) + kRunRows * 4 + (kRunRows + 4) * 4 +
+                       (kRunLevels + 1) * 4);
+      if (plan.lds > C.max_lds) { err = "grid engine: sweep workgroup exceeds the LDS"; return false; }
+      build_sweep(*F, false, wide_rows, plan.cap, fw, plan.fw);
+      build_sweep(*F, true, wide_rows, plan.cap, bw, plan.bw);
+      o_perm = P.put(F->P.data(), N * 4);
+      o_z = P.zeros((size_t)N * 8);
+      SweepHost* hs[2] = {&fw, &bw};
+      size_t* os[2] = {o_f, o_b7};
+      for (int d = 0; d < 2; ++d) {
+        SweepHost& h = *hs[d];
+        os[d][0] = P.put(h.row.data(), h.row.size() * 4);
+        os[d][1] = P.put(h.ptr.data(), h.ptr.size() * 4);
+        os[d][2] = P.put(h.idx.data(), h.idx.size() * 4);
+        os[d][3] = P.put(h.val.data(), h.val.size() * 8);
+        os[d][4] = P.put(h.dq.data(), h.dq.size() * 8);
+        os[d][5] = P.put(h.lptr.data(), h.lptr.size() * 4);
+        os[d][6] = P.put(h.chunk.data(), h.chunk.size() * sizeof(int4));
+      }
+    }
+    bytes = P.buf.size();
+    const size_t hbytes = std::max<size_t>(bytes, 4096);
+    hipError_t e = hipSuccess;
+    if (borrow) {
+      if (!C.reserve(bytes, hbytes, err)) return false;
+      dbuf = C.dbuf;
+      hbuf = C.hbuf;
+      stream = C.stream;
+    } else {
+      e = hipMalloc(&dbuf, bytes);
+      if (e == hipSuccess) e = hipHostMalloc(&hbuf, hbytes, hipHostMallocDefault);
+      if (e == hipSuccess) e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
+      if (e != hipSuccess) { err = std::string("grid engine: ") + hipGetErrorString(e); return false; }
+    }
+    unsigned char* base = static_cast<unsigned char*>(dbuf);
+    auto at = [&](size_t off) { return base + off; };
+    a.n = n; a.m = m; a.f = S.f; a.lend = S.f + S.l; a.nsoc = S.nsoc();
+    a.cg_max = direct ? 0 : (int)std::min<long>(std::max<long>(par.CG_MAX_ITS, 0), 100000);
+    a.nb_a = (int)rb.size(); a.nb_at = (int)cb.size(); a.nb_m = nb_m; a.nb_n = nb_n;
+    a.eps = par.EPS_ABS; a.relax = par.ALPHA; a.tol_sq = par.CG_TOL * par.CG_TOL;
+    a.lambda = S.lambda; a.pscale = S.pscale; a.dscale = S.dscale;
+    a.rp = (const int*)at(o_rp); a.ri = (const int*)at(o_ri); a.rx = (const T*)at(o_rx);
+    a.rblk = (const int4*)at(o_rb);
+    a.cp = (const int*)at(o_cp); a.ci = (const int*)at(o_ci); a.cx = (const T*)at(o_cx);
+    a.cblk = (const int4*)at(o_cb);
+    a.b = (const double*)at(o_b); a.c = (const double*)at(o_c);
+    a.D = (const double*)at(o_D); a.E = (const double*)at(o_E);
+    a.soc_ptr = (const int*)at(o_soc);
+    a.x = (T*)at(o_x); a.s = (T*)at(o_s); a.y = (T*)at(o_y);
+    x0 = at(o_x0); s0 = at(o_s0); y0 = at(o_y0);
+    a.ax = (T*)at(o_ax); a.st = (T*)at(o_st); a.t = (T*)at(o_t);
+    a.pq = (Pair<T>*)at(o_pq); a.u = (T*)at(o_u);
+    a.p_res = (double*)at(o_pres); a.p_rho = (double*)at(o_prho);
+    a.p_pu = (double*)at(o_ppu); a.p_q = (double*)at(o_pqn);
+    a.sc = (GridScal*)at(o_sc);
+    a.hist = hrows ? (double*)at(o_hist) : nullptr;
+    a.hist_rows = hrows;
+    a.direct = direct ? 1 : 0;
+    a.N = N;
+    a.nb_N = std::max(1, std::min(kVecBlocks, (N + GT - 1) / GT));
+    a.perm = direct ? (const int*)at(o_perm) : nullptr;
+    a.z = direct ? (double*)at(o_z) : nullptr;
+    std::memset(&a.fw, 0, sizeof(a.fw));
+    std::memset(&a.bw, 0, sizeof(a.bw));
+    if (direct) {
+      SweepDev* sd[2] = {&a.fw, &a.bw};
+      size_t* os[2] = {o_f, o_b7};
+      for (int d = 0; d < 2; ++d) {
+        sd[d]->row = (const int*)at(os[d][0]);
+        sd[d]->ptr = (const int*)at(os[d][1]);
+        sd[d]->idx = (const int*)at(os[d][2]);
+        sd[d]->val = (const double*)at(os[d][3]);
+        sd[d]->dq = d ? (const double*)at(os[d][4]) : nullptr;
+        sd[d]->lptr = (const int*)at(os[d][5]);
+        sd[d]->chunk = (const int4*)at(os[d][6]);
+      }
+      const void* fns[4] = {reinterpret_cast<const void*>(k_sweep_run<T, false, true>),
+                            reinterpret_cast<const void*>(k_sweep_run<T, true, true>),
+                            reinterpret_cast<const void*>(k_sweep_run<T, false, false>),
+                            reinterpret_cast<const void*>(k_sweep_run<T, true, false>)};
+      for (const void* fn : fns)
+        if (e == hipSuccess) e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, plan.lds);
+    }
+    std::memcpy(hbuf, P.buf.data(), bytes);
+    if (e == hipSuccess) e = hipMemcpyAsync(dbuf, hbuf, bytes, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e == hipSuccess) e = hipEventCreate(&ev0);
+    if (e == hipSuccess) e = hipEventCreate(&ev1);
+    if (e != hipSuccess) { err = std::string("upload: ") + hipGetErrorString(e); return false; }
+    max_iters = std::max<long>(par.MAX_ITERS, 0);
+    if (max_iters > 0) {
+      e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+      if (e == hipSuccess) {
+        enqueue_iteration<T>(a, plan, stream);
+        e = hipStreamEndCapture(stream, &graph);
+      }
+      if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      if (e != hipSuccess) { err = std::string("graph: ") + hipGetErrorString(e); return false; }
+    }
+    upload_ms = now_ms() - t0;
+    return true;
+  }
+
+  bool run(std::string& err) override {
+    const double t1 = now_ms();
+    hipError_t e = hipMemcpyAsync(a.x, x0, n * sizeof(T), hipMemcpyDeviceToDevice, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(a.s, s0, m * sizeof(T), hipMemcpyDeviceToDevice, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(a.y, y0, m * sizeof(T), hipMemcpyDeviceToDevice, stream);
+    GridScal* hsc = reinterpret_cast<GridScal*>(hbuf);
+    *hsc = sc0;
+    if (e == hipSuccess) e = hipMemcpyAsync(a.sc, hsc, sizeof(GridScal), hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e == hipSuccess) e = hipEventRecord(ev0, stream);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_ax<T>, dim3(a.nb_a), dim3(GT), 0, stream, a);
+      e = hipGetLastError();
+    }
+    long launched = 0;
+    int chunk = 4;
+    while (e == hipSuccess && launched < max_iters) {
+      const long k = std::min<long>(chunk, max_iters - launched);
+      for (long i = 0; i < k && e == hipSuccess; ++i) e = hipGraphLaunch(exec, stream);
+      launched += k;
+      if (e == hipSuccess) e = hipMemcpyAsync(hsc, a.sc, sizeof(GridScal), hipMemcpyDeviceToHost, stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(stream);
+      if (e != hipSuccess || hsc->done) break;
+      chunk = std::min(chunk * 2, 32);
+    }
+    if (e == hipSuccess) e = hipEventRecord(ev1, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(hsc, a.sc, sizeof(GridScal), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev0, ev1);
+    if (e != hipSuccess) { err = std::string("grid engine: ") + hipGetErrorString(e); return false; }
+    scv = *hsc;
+    kernel_ms = ms;
+    solve_ms = now_ms() - t1;
+    return true;
+  }
+
+  bool fetch(Setup& S, DevResult& r, std::vector<double>* hist, std::string& err) override {
+    std::vector<T> hx(n), hs(m), hy(m);
+    hipError_t e = hipMemcpy(hx.data(), a.x, n * sizeof(T), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(hs.data(), a.s, m * sizeof(T), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(hy.data(), a.y, m * sizeof(T), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && hist) {
+      hist->assign((size_t)5 * hrows, 0.0);
+      if (hrows) e = hipMemcpy(hist->data(), a.hist, sizeof(double) * 5 * hrows, hipMemcpyDeviceToHost);
+    }
+    if (e != hipSuccess) { err = std::string("download: ") + hipGetErrorString(e); return false; }
+    S.x.resize(n);
+    S.s.resize(m);
+    S.y.resize(m);
+    for (int i = 0; i < n; ++i) S.x[i] = double(hx[i]);
+    for (int j = 0; j < m; ++j) { S.s[j] = double(hs[j]); S.y[j] = double(hy[j]); }
+    r = DevResult{};
+    r.iters = scv.iters;
+    r.status = scv.status;
+    r.cg_iters = scv.cg_total;
+    r.p_res = scv.pres; r.d_res = scv.dres; r.p_obj = scv.pobj; r.d_obj = scv.dobj; r.eta = scv.eta;
+    return true;
+  }
+
+ private:
   GArgs<T> a;
-  a.n = n; a.m = m; a.f = S.f; a.lend = S.f + S.l; a.nsoc = S.nsoc();
-  a.cg_max = (int)std::min<long>(std::max<long>(par.CG_MAX_ITS, 0), 100000);
-  a.nb_a = (int)rb.size(); a.nb_at = (int)cb.size(); a.nb_m = nb_m; a.nb_n = nb_n;
-  a.eps = par.EPS_ABS; a.relax = par.ALPHA; a.tol_sq = par.CG_TOL * par.CG_TOL;
-  a.lambda = S.lambda; a.pscale = S.pscale; a.dscale = S.dscale;
-  a.rp = (const int*)at(o_rp); a.ri = (const int*)at(o_ri); a.rx = (const T*)at(o_rx);
-  a.rblk = (const int4*)at(o_rb);
-  a.cp = (const int*)at(o_cp); a.ci = (const int*)at(o_ci); a.cx = (const T*)at(o_cx);
-  a.cblk = (const int4*)at(o_cb);
-  a.b = (const double*)at(o_b); a.c = (const double*)at(o_c);
-  a.D = (const double*)at(o_D); a.E = (const double*)at(o_E);
-  a.soc_ptr = (const int*)at(o_soc);
-  a.x = (T*)at(o_x); a.s = (T*)at(o_s); a.y = (T*)at(o_y);
-  a.ax = (T*)at(o_ax); a.st = (T*)at(o_st); a.t = (T*)at(o_t);
-  a.pq = (Pair<T>*)at(o_pq); a.u = (T*)at(o_u);
-  a.p_res = (double*)at(o_pres); a.p_rho = (double*)at(o_prho);
-  a.p_pu = (double*)at(o_ppu); a.p_q = (double*)at(o_pqn);
-  a.sc = (GridScal*)at(o_sc);
-  a.hist = hrows ? (double*)at(o_hist) : nullptr;
-  a.hist_rows = hrows;
-  std::memcpy(C.hbuf, P.buf.data(), P.buf.size());
-  hipError_t e = hipMemcpyAsync(C.dbuf, C.hbuf, P.buf.size(), hipMemcpyHostToDevice, C.stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(C.stream);
-  if (e != hipSuccess) { err = std::string("upload: ") + hipGetErrorString(e); return false; }
-  const double t1 = now_ms();
-  hipLaunchKernelGGL(k_ax<T>, dim3(a.nb_a), dim3(GT), 0, C.stream, a);
-  const long max_iters = std::max<long>(par.MAX_ITERS, 0);
+  SweepPlan plan;
+  GridScal sc0, scv;
+  int n = 0, m = 0, hrows = 0;
+  long max_iters = 0;
+  size_t bytes = 0;
+  bool own = true;
+  void* dbuf = nullptr;
+  void* hbuf = nullptr;
+  void* x0 = nullptr;
+  void* s0 = nullptr;
+  void* y0 = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  if (max_iters > 0) {
-    e = hipStreamBeginCapture(C.stream, hipStreamCaptureModeThreadLocal);
-    if (e == hipSuccess) {
-      enqueue_iteration<T>(a, C.stream);
-      e = hipStreamEndCapture(C.stream, &graph);
-    }
-    if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    if (e != hipSuccess) { err = std::string("graph: ") + hipGetErrorString(e); return false; }
-  }
-  GridScal* hsc = reinterpret_cast<GridScal*>(C.hbuf);
-  long launched = 0;
-  int chunk = 4;
-  while (launched < max_iters) {
-    const long k = std::min<long>(chunk, max_iters - launched);
-    for (long i = 0; i < k && e == hipSuccess; ++i) e = hipGraphLaunch(exec, C.stream);
-    launched += k;
-    if (e == hipSuccess) e = hipMemcpyAsync(hsc, a.sc, sizeof(GridScal), hipMemcpyDeviceToHost, C.stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(C.stream);
-    if (e != hipSuccess || hsc->done) break;
-    chunk = std::min(chunk * 2, 32);
-  }
-  if (e == hipSuccess) e = hipMemcpyAsync(hsc, a.sc, sizeof(GridScal), hipMemcpyDeviceToHost, C.stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(C.stream);
-  if (exec) (void)hipGraphExecDestroy(exec);
-  if (graph) (void)hipGraphDestroy(graph);
-  if (e != hipSuccess) { err = std::string("grid engine: ") + hipGetErrorString(e); return false; }
-  const double t2 = now_ms();
-  const GridScal scv = *hsc;
-  std::vector<T> hx(n), hs(m), hy(m);
-  e = hipMemcpy(hx.data(), a.x, n * sizeof(T), hipMemcpyDeviceToHost);
-  if (e == hipSuccess) e = hipMemcpy(hs.data(), a.s, m * sizeof(T), hipMemcpyDeviceToHost);
-  if (e == hipSuccess) e = hipMemcpy(hy.data(), a.y, m * sizeof(T), hipMemcpyDeviceToHost);
-  if (e == hipSuccess && hist) {
-    hist->assign((size_t)5 * hrows, 0.0);
-    if (hrows) e = hipMemcpy(hist->data(), a.hist, sizeof(double) * 5 * hrows, hipMemcpyDeviceToHost);
-  }
-  if (e != hipSuccess) { err = std::string("download: ") + hipGetErrorString(e); return false; }
-  for (int i = 0; i < n; ++i) S.x[i] = double(hx[i]);
-  for (int j = 0; j < m; ++j) { S.s[j] = double(hs[j]); S.y[j] = double(hy[j]); }
-  res.assign(1, DevResult{});
-  DevResult& r = res[0];
-  r.iters = scv.iters;
-  r.status = scv.status;
-  r.cg_iters = scv.cg_total;
-  r.p_res = scv.pres; r.d_res = scv.dres; r.p_obj = scv.pobj; r.d_obj = scv.dobj; r.eta = scv.eta;
-  upload_ms = t1 - t0;
-  solve_ms = t2 - t1;
-  return true;
-}
+};
 
 }  // namespace
 
-bool grid_preferred(const Setup& S, bool fp64) {
-  const int lds = wg_lds_bytes(S.n, S.m, S.n + S.m, false, fp64 ? 8 : 4);
+bool grid_preferred(const Setup& S, bool fp64, bool direct) {
+  const int lds = wg_lds_bytes(S.n, S.m, S.n + S.m, direct, fp64 ? 8 : 4);
   return lds > ctx().max_lds || S.nnz() > 200000 || S.m > 16384;
 }
 
-bool solve_grid(Setup& S, const Params& par, bool fp64, int device, std::vector<DevResult>& res,
+std::unique_ptr<GridSolver> grid_prepare(const Setup& S, const Params& par, bool fp64, const Factor* F, int device,
+                                         bool borrow, std::string& err) {
+  std::unique_ptr<GridSolver> g;
+  bool ok;
+  if (fp64) {
+    auto* p = new GridImpl<double>();
+    g.reset(p);
+    ok = p->prepare(S, par, F, device, borrow, err);
+  } else {
+    auto* p = new GridImpl<float>();
+    g.reset(p);
+    ok = p->prepare(S, par, F, device, borrow, err);
+  }
+  if (!ok) g.reset();
+  return g;
+}
+
+bool solve_grid(Setup& S, const Params& par, bool fp64, const Factor* F, int device, std::vector<DevResult>& res,
                 double& upload_ms, double& solve_ms, std::string& err, std::vector<double>* hist) {
-  return fp64 ? run_grid<double>(S, par, device, res, upload_ms, solve_ms, err, hist)
-              : run_grid<float>(S, par, device, res, upload_ms, solve_ms, err, hist);
+  std::unique_ptr<GridSolver> g = grid_prepare(S, par, fp64, F, device, true, err);
+  if (!g) return false;
+  res.assign(1, DevResult{});
+  if (!g->run(err) || !g->fetch(S, res[0], hist, err)) return false;
+  upload_ms = g->upload_ms;
+  solve_ms = g->solve_ms;
+  return true;
 }
 
 }  // namespace pdos_amd

@@ -206,7 +206,8 @@ __device__ __forceinline__ void tri_sweep(int nlev, const int* __restrict__ lptr
 template <typename T, bool DIRECT>
 __global__ __launch_bounds__(NT) void k_wg_solve(const WgInst* __restrict__ insts) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-  const WgInst& I = insts[blockIdx.x];
+  // the record by value: its fields stay in registers across the barriers
+  const WgInst I = insts[blockIdx.x];
   const int n = I.n, m = I.m, tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
 
@@ -449,9 +450,470 @@ __global__ __launch_bounds__(NT) void k_wg_solve(const WgInst* __restrict__ inst
   }
 }
 
+// ---------------------------------------------------------------------------
+// Resident variant: for an instance whose matrices (and, direct plugin, the
+// factor in level order) fit in LDS next to its iterates, the workgroup copies
+// its group's image into LDS once and every SpMV, sweep and reduction of the
+// solve touches LDS only -- no global memory round trip per phase or level.
+//   SpMV and sweeps: rows in wave batches, 1 to 64 lanes per row so that no
+//         lane sums more than 4 entries of a row of up to 256 (a row of one
+//         lane in storage order, the reference's order);
+//   direct: the permuted right-hand side is formed in the residual phase, the
+//         sweeps run in level order (rows of a level stored contiguously),
+//         and the un-permute is folded into the next A product (A's column
+//         indices stored also as pivot positions), so an iteration is one
+//         barrier per level plus three.
+
+// Data-parallel-primitive (DPP) moves of a double between lanes, on its two
+// 32-bit halves; masked-off rows read 0.
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ double dpp_move(double v) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, ROWS, 0xf, false);
+  const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, ROWS, 0xf, false);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// Sum over groups of G = 2^lg consecutive lanes; every lane of a group gets
+// its group's sum.  Up to 16 lanes by DPP (quad permutes, then the half-row
+// and row mirrors: ALU latency, no LDS), beyond that by lane shuffles.
+__device__ __forceinline__ double group_sum(double v, int lg) {
+  if (lg >= 1) v += dpp_move<0xB1>(v);   // quad_perm [1, 0, 3, 2]
+  if (lg >= 2) v += dpp_move<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+  if (lg >= 3) v += dpp_move<0x141>(v);  // row_half_mirror
+  if (lg >= 4) v += dpp_move<0x140>(v);  // row_mirror
+  if (lg >= 5) v += __shfl_xor(v, 16, 64);
+  if (lg >= 6) v += __shfl_xor(v, 32, 64);
+  return v;
+}
+
+// Sum over the wave, by DPP: the 16-lane row sums, then row 1 += row 0 and
+// row 3 += row 2 (row_bcast15), row 3 += row 1 (row_bcast31); lane 63 holds
+// the total, which every lane reads.
+__device__ __forceinline__ double wave_total(double v) {
+  v += dpp_move<0xB1>(v);
+  v += dpp_move<0x4E>(v);
+  v += dpp_move<0x141>(v);
+  v += dpp_move<0x140>(v);
+  v += dpp_move<0x142, 0xA>(v);
+  v += dpp_move<0x143, 0xC>(v);
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, 63);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), 63);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// block_sum with the DPP wave total.
+template <int NV>
+__device__ __forceinline__ void block_total(double (&v)[NV], double* red, int& slot) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = wave_total(v[k]);
+  double* buf = red + slot * (16 * 4);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) buf[w * 4 + k] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double a = 0.0;
+#pragma unroll
+    for (int j = 0; j < NW; ++j) a += buf[j * 4 + k];
+    v[k] = a;
+  }
+  slot ^= 1;
+}
+
+// A lane's share of one row: entries e0, e0 + G, ... below e1.  The first
+// four are loaded unconditionally (the entry arrays carry 256 zero entries of
+// padding, so the reads stay inside them) and masked by selects, so the lane
+// runs straight-line code with all loads of a kind in flight together; the
+// rest (rows of more than 4 G entries) one by one.  Summed in entry order.
+template <typename A, typename V, typename Gather>
+__device__ __forceinline__ A lane_dot(const int* idx, const V* val, int e0, int e1, int G, Gather gx) {
+  int ix[4];
+  V vv[4], gv[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    ix[j] = idx[e0 + j * G];
+    vv[j] = val[e0 + j * G];
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) gv[j] = gx(ix[j]);
+  A acc = A(0);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) acc += e0 + j * G < e1 ? A(vv[j] * gv[j]) : A(0);
+  for (int e = e0 + 4 * G; e < e1; e += G) acc += A(val[e] * gx(idx[e]));
+  return acc;
+}
+
+// y = A x over rows held in LDS, by wave batches {first, count, lg}: lane l
+// of a batch takes position first + (l >> lg), i.e. row pos.x, and every
+// (1 << lg)-th of its entries [pos.y, pos.z); a row of one lane sums them
+// in storage order.  The lane at the start of each group emits.
+// A wave batch read from LDS, made wave-uniform (scalar registers), so the
+// branches on its class are scalar branches, not exec-mask divergence.
+__device__ __forceinline__ int4 uniform_batch(const int4* bat, int b) {
+  const int4 B = bat[b];
+  return int4{__builtin_amdgcn_readfirstlane(B.x), __builtin_amdgcn_readfirstlane(B.y),
+              __builtin_amdgcn_readfirstlane(B.z), 0};
+}
+
+template <typename T, typename Gather, typename Emit>
+__device__ __forceinline__ void spmv_res(const int4* pos, const int* idx, const T* val, const int4* bat, int nbat,
+                                         Gather gx, Emit emit) {
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  for (int b = w; b < nbat; b += NW) {
+    const int4 B = uniform_batch(bat, b);
+    const int k = lane >> B.z, sub = lane & ((1 << B.z) - 1);
+    const bool on = k < B.y;
+    const int4 P = pos[B.x + (on ? k : 0)];
+    if (B.z == 0) {
+      if (on) emit(P.x, lane_dot<T>(idx, val, P.y, P.z, 1, gx));
+    } else {
+      double acc = on ? lane_dot<double>(idx, val, P.y + sub, P.z, 1 << B.z, gx) : 0.0;
+      acc = group_sum(acc, B.z);
+      if (on && sub == 0) emit(P.x, T(acc));
+    }
+  }
+}
+
+// One triangular sweep in LDS, level by level (lb: level -> its batches;
+// position q: pos[q] = {row, entries}; backward: divide by dq[q] first), one
+// barrier per level.
+template <bool BACK>
+__device__ __forceinline__ void res_sweep(int nlev, const int* lb, const int4* bat, const int4* pos, const int* idx,
+                                          const double* val, const double* dq, double* z) {
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  auto zg = [&](int c) { return z[c]; };
+  for (int lv = 0; lv < nlev; ++lv) {
+    const int b1 = __builtin_amdgcn_readfirstlane(lb[lv + 1]);
+    for (int b = __builtin_amdgcn_readfirstlane(lb[lv]) + w; b < b1; b += NW) {
+      const int4 B = uniform_batch(bat, b);
+      const int k = lane >> B.z, sub = lane & ((1 << B.z) - 1);
+      const bool on = k < B.y;
+      const int q = B.x + (on ? k : 0);
+      const int4 P = pos[q];
+      const double zi = z[P.x];
+      const double d = BACK ? dq[q] : 1.0;
+      double acc = on ? lane_dot<double>(idx, val, P.y + sub, P.z, 1 << B.z, zg) : 0.0;
+      acc = group_sum(acc, B.z);
+      if (on && sub == 0) z[P.x] = (BACK ? zi / d : zi) - acc;
+    }
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// The dense trailing chain T of the factor (t <= 64 rows in pivot order, the
+// t x t block tl row-major): wave 0 holds z_T in its lanes (z_T already less
+// the entries outside T), solves the unit lower triangle (lane i subtracts
+// L(i, k) z_k as each z_k becomes final and is read from lane k), then D and
+// the unit upper triangle L' the same way from the last row up.  No barrier
+// inside; the caller's barrier publishes z_T.
+__device__ __forceinline__ void res_chain(int t, const int* trow, const double* tdq, const double* tl, double* z) {
+  if (threadIdx.x >= 64) return;
+  const int lane = threadIdx.x;
+  const bool on = lane < t;
+  const int r = on ? trow[lane] : 0;
+  double v = on ? z[r] : 0.0;
+  for (int k = 0; k < t; ++k) {
+    const double lik = on && lane > k ? tl[lane * t + k] : 0.0;
+    v -= lik * readlane_d(v, k);
+  }
+  v = on ? v / tdq[lane] : 0.0;
+  for (int i = t - 1; i >= 0; --i) {
+    const double lki = on && lane < i ? tl[i * t + lane] : 0.0;
+    v -= lki * readlane_d(v, i);
+  }
+  if (on) z[r] = v;
+}
+
 template <typename T, bool DIRECT>
-hipError_t launch_t(const WgInst* d_insts, int count, int lds_bytes, hipStream_t st) {
-  auto fn = k_wg_solve<T, DIRECT>;
+__global__ __launch_bounds__(NT) void k_wg_res(const WgInst* __restrict__ insts) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  // the record by value: its fields stay in registers across the barriers
+  // (a reference would be re-read from memory after each one)
+  const WgInst I = insts[blockIdx.x];
+  const ResLayout& L = I.lay;
+  const int n = I.n, m = I.m, N = I.N, tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6;
+  {
+    const uint4* src = reinterpret_cast<const uint4*>(I.img);
+    uint4* dst = reinterpret_cast<uint4*>(smem);
+    for (int i = tid; i < L.img / 16; i += NT) dst[i] = src[i];
+  }
+  const double* Dl = reinterpret_cast<const double*>(smem + L.D);
+  const double* El = reinterpret_cast<const double*>(smem + L.E);
+  const int4* rpos = reinterpret_cast<const int4*>(smem + L.rpos);
+  const int* ridx = reinterpret_cast<const int*>(smem + L.ridx);
+  const T* rval = reinterpret_cast<const T*>(smem + L.rval);
+  const int* ridp = reinterpret_cast<const int*>(smem + L.ridp);
+  const int4* cpos = reinterpret_cast<const int4*>(smem + L.cpos);
+  const int* cidx = reinterpret_cast<const int*>(smem + L.cidx);
+  const T* cval = reinterpret_cast<const T*>(smem + L.cval);
+  const int4* rbat = reinterpret_cast<const int4*>(smem + L.rbat);
+  const int4* cbat = reinterpret_cast<const int4*>(smem + L.cbat);
+  const int nrbat = L.nrbat, ncbat = L.ncbat;
+  const int* socp = reinterpret_cast<const int*>(smem + L.soc);
+  double* bl = reinterpret_cast<double*>(smem + L.b);
+  double* cl = reinterpret_cast<double*>(smem + L.c);
+  for (int j = tid; j < m; j += NT) bl[j] = I.b[j];
+  for (int i = tid; i < n; i += NT) cl[i] = I.c[i];
+  auto al = [](size_t v) { return (v + 15) & ~(size_t)15; };
+  unsigned char* vp = smem + L.vec;
+  T* x = reinterpret_cast<T*>(vp);
+  T* s = x + n;
+  T* y = s + m;
+  T* st = y + m;
+  T* ax = st + m;
+  vp += al((size_t)(n + 4 * m) * sizeof(T));
+  T *p = nullptr, *q = nullptr, *u = nullptr, *t = nullptr;
+  double* bp = nullptr;
+  if (DIRECT) {
+    bp = reinterpret_cast<double*>(vp);
+    vp += al((size_t)N * 8);
+  } else {
+    p = reinterpret_cast<T*>(vp);
+    q = p + n;
+    u = q + n;
+    t = u + n;
+    vp += al((size_t)(3 * n + m) * sizeof(T));
+  }
+  double* red = reinterpret_cast<double*>(vp);
+  const int f = I.f, lend = I.f + I.l, nsoc = I.nsoc;
+  int slot = 0;
+  const T lam = T(I.lambda);
+  const T relax_a = T(I.alpha);
+  const T relax_b = T(1.0) - T(I.alpha);
+  __syncthreads();  // image, b and c in LDS
+
+  if (I.wsrc) {  // warm start in original units (common.h:148-161, 188): x/E, y/D, s*D
+    const float* w = I.wsrc;
+    for (int i = tid; i < n; i += NT) x[i] = T(double(w[i]) / El[i]);
+    for (int j = tid; j < m; j += NT) {
+      y[j] = T(double(w[n + j]) / Dl[j]);
+      s[j] = T(double(w[n + m + j]) * Dl[j]);
+    }
+  } else {
+    for (int i = tid; i < n; i += NT) x[i] = T(I.x0[i]);
+    for (int j = tid; j < m; j += NT) { s[j] = T(I.s0[j]); y[j] = T(I.y0[j]); }
+  }
+  __syncthreads();
+  spmv_res<T>(rpos, ridx, rval, rbat, nrbat, [&](int c) { return x[c]; }, [&](int j, T v) { ax[j] = v; });
+  __syncthreads();
+
+  int it = 0, status = 1, cg_total = 0, fallback = 0;
+  bool judge = I.warm_check != 0;
+  double pres = -1.0, dres = -1.0, pobj = -1.0, dobj = -1.0, eta = -1.0;
+
+  for (it = 0; it < I.max_iters; ++it) {
+    // ---- stop check (linAlg.h:158-201); the CG start argument t or the
+    // permuted KKT right-hand side (direct/private.c:16-31) ----
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};  // |pres|^2, |dres|^2, c'x, b'y
+    for (int j = tid; j < m; j += NT) {
+      const T sj = s[j], bj = T(bl[j]), yj = y[j], axj = ax[j];
+      const T r = ((sj - bj) + axj) / T(Dl[j]);
+      acc[0] += double(r) * double(r);
+      acc[3] += double(bj) * double(yj);
+      if (!DIRECT) t[j] = ((sj + lam * yj) - bj) + axj;
+    }
+    spmv_res<T>(cpos, cidx, cval, cbat, ncbat, [&](int c) { return y[c]; }, [&](int i, T v) {
+      const T ci_ = T(cl[i]);
+      const T r = (ci_ + v) / T(El[i]);
+      acc[1] += double(r) * double(r);
+      acc[2] += double(ci_) * double(x[i]);
+    });
+    if (DIRECT) {
+      const int* perm = reinterpret_cast<const int*>(smem + L.perm);
+      for (int kk = tid; kk < N; kk += NT) {
+        const int src = perm[kk];
+        double v;
+        if (src < n) {
+          v = double(lam) * cl[src] - double(x[src]);
+        } else {
+          const int j = src - n;
+          v = bl[j] - (double(s[j]) + double(lam) * double(y[j]));
+        }
+        bp[kk] = v;
+      }
+    }
+    block_total<4>(acc, red, slot);
+    pres = sqrt(acc[0]);
+    dres = sqrt(acc[1]);
+    pobj = acc[2];
+    dobj = -acc[3];
+    eta = fabs(pobj - dobj);
+    if (judge) {
+      judge = false;
+      const double mw = fmax(fmax(pres / I.pscale, dres / I.dscale), eta / (1.0 + fabs(pobj) + fabs(dobj)));
+      const double mc = fmax((I.pscale - 1.0) / I.pscale, (I.dscale - 1.0) / I.dscale);
+      if (!(mw <= mc)) {
+        fallback = 1;
+        for (int i = tid; i < n; i += NT) x[i] = T(0);
+        for (int j = tid; j < m; j += NT) { s[j] = T(0); y[j] = T(0); ax[j] = T(0); }
+        __syncthreads();
+        it = -1;
+        continue;
+      }
+    }
+    if (pres < I.eps * I.pscale && dres < I.eps * I.dscale &&
+        eta < I.eps * (1.0 + fabs(pobj) + fabs(dobj))) {
+      status = 0;
+      break;
+    }
+    if (I.hist && tid == 0 && it % 10 == 0 && it / 10 < I.hist_rows) {
+      double* h = I.hist + (size_t)(it / 10) * 5;
+      h[0] = pres; h[1] = dres; h[2] = pobj; h[3] = dobj; h[4] = eta;
+    }
+
+    if (!DIRECT) {
+      // ---- CG on (I + A'A) x = x - lam c + A'(b - s - lam y) ----
+      double r1[1] = {0.0};
+      spmv_res<T>(cpos, cidx, cval, cbat, ncbat, [&](int c) { return t[c]; }, [&](int i, T v) {
+        const T qi = (-lam) * T(cl[i]) - v;
+        q[i] = qi;
+        p[i] = T(0);
+        r1[0] += double(qi) * double(qi);
+      });
+      block_total<1>(r1, red, slot);
+      double rho = r1[0];
+      T beta = T(0);
+      int k = 0;
+      double qlast = 0.0;
+      if (rho > I.cg_tol_sq) {
+        while (k < I.cg_max_its) {
+          spmv_res<T>(rpos, ridx, rval, rbat, nrbat, [&](int c) { return beta * p[c] + q[c]; },
+                      [&](int j, T v) { t[j] = v; });
+          __syncthreads();
+          double r2[1] = {0.0};
+          spmv_res<T>(cpos, cidx, cval, cbat, ncbat, [&](int c) { return t[c]; }, [&](int i, T v) {
+            const T pi = beta * p[i] + q[i];
+            p[i] = pi;
+            const T ui = pi + v;
+            u[i] = ui;
+            r2[0] += double(pi) * double(ui);
+          });
+          block_total<1>(r2, red, slot);
+          const T alpha = T(rho / r2[0]);
+          double r3[1] = {0.0};
+          for (int i = tid; i < n; i += NT) {
+            x[i] += alpha * p[i];
+            const T qi = q[i] - alpha * u[i];
+            q[i] = qi;
+            r3[0] += double(qi) * double(qi);
+          }
+          block_total<1>(r3, red, slot);
+          ++k;
+          qlast = r3[0];
+          if (r3[0] < I.cg_tol_sq) break;
+          beta = T(r3[0] / rho);
+          rho = r3[0];
+        }
+      }
+      cg_total += k;
+      if (I.cg_hist && tid == 0 && it < I.hist_rows * 10) {
+        I.cg_hist[(size_t)it * 2] = double(k);
+        I.cg_hist[(size_t)it * 2 + 1] = sqrt(qlast);
+      }
+      // ---- s~ = b - A x, relax, project, dual update (pdos.c:232-249,296-303) ----
+      spmv_res<T>(rpos, ridx, rval, rbat, nrbat, [&](int c) { return x[c]; }, [&](int j, T v) {
+        ax[j] = v;
+        T stj = T(bl[j]) - v;
+        stj = relax_a * stj + relax_b * s[j];
+        const T z = stj - lam * y[j];
+        if (j < f) {
+          s[j] = T(0);
+          y[j] += (T(0) - stj) / lam;
+        } else if (j < lend) {
+          const T sj = z < T(0) ? T(0) : z;
+          s[j] = sj;
+          y[j] += (sj - stj) / lam;
+        } else {
+          st[j] = stj;
+          s[j] = z;
+        }
+      });
+    } else {
+      // ---- LDL' sweeps in level order, then x = z[0:n] and s~ = z[n:] +
+      // (s + lam y) read through pivot positions inside the A product ----
+      res_sweep<false>(L.nfl, reinterpret_cast<const int*>(smem + L.flb), reinterpret_cast<const int4*>(smem + L.fbat),
+                       reinterpret_cast<const int4*>(smem + L.fpos), reinterpret_cast<const int*>(smem + L.fidx),
+                       reinterpret_cast<const double*>(smem + L.fval), nullptr, bp);
+      if (L.t) {
+        res_chain(L.t, reinterpret_cast<const int*>(smem + L.trow), reinterpret_cast<const double*>(smem + L.tdq),
+                  reinterpret_cast<const double*>(smem + L.tl), bp);
+        __syncthreads();
+      }
+      res_sweep<true>(L.nbl, reinterpret_cast<const int*>(smem + L.blb), reinterpret_cast<const int4*>(smem + L.bbat),
+                      reinterpret_cast<const int4*>(smem + L.bpos), reinterpret_cast<const int*>(smem + L.bidx),
+                      reinterpret_cast<const double*>(smem + L.bval), reinterpret_cast<const double*>(smem + L.bdq), bp);
+      const int* pinv = reinterpret_cast<const int*>(smem + L.pinv);
+      for (int i = tid; i < n; i += NT) x[i] = T(bp[pinv[i]]);
+      spmv_res<T>(rpos, ridp, rval, rbat, nrbat, [&](int c) { return T(bp[c]); }, [&](int j, T v) {
+        ax[j] = v;
+        T stj = T(bp[pinv[n + j]] + (double(s[j]) + double(lam) * double(y[j])));
+        stj = relax_a * stj + relax_b * s[j];
+        const T z = stj - lam * y[j];
+        if (j < f) {
+          s[j] = T(0);
+          y[j] += (T(0) - stj) / lam;
+        } else if (j < lend) {
+          const T sj = z < T(0) ? T(0) : z;
+          s[j] = sj;
+          y[j] += (sj - stj) / lam;
+        } else {
+          st[j] = stj;
+          s[j] = z;
+        }
+      });
+    }
+    __syncthreads();
+    for (int cn = wv; cn < nsoc; cn += NW) {
+      const int r0 = socp[cn], r1 = socp[cn + 1];
+      double a2 = 0.0;
+      for (int j = r0 + 1 + lane; j < r1; j += 64) a2 += double(s[j]) * double(s[j]);
+      a2 = wave_total(a2);
+      const T ns = T(sqrt(a2));
+      const T t0 = s[r0];
+      const int mode = ns <= t0 ? 0 : (ns <= -t0 ? 1 : 2);
+      const T a = (ns + t0) / T(2);
+      for (int j = r0 + lane; j < r1; j += 64) {
+        const T zj = s[j];
+        const T sj = mode == 0 ? zj : (mode == 1 ? T(0) : (j == r0 ? a : a * zj / ns));
+        s[j] = sj;
+        y[j] += (sj - st[j]) / lam;
+      }
+    }
+    __syncthreads();
+  }
+
+  for (int i = tid; i < n; i += NT) I.x[i] = double(x[i]);
+  for (int j = tid; j < m; j += NT) { I.s[j] = double(s[j]); I.y[j] = double(y[j]); }
+  if (tid == 0) {
+    DevResult* R = I.res;
+    R->iters = it;
+    R->status = status;
+    R->cg_iters = cg_total;
+    R->fallback = fallback;
+    R->p_res = pres;
+    R->d_res = dres;
+    R->p_obj = pobj;
+    R->d_obj = dobj;
+    R->eta = eta;
+  }
+}
+
+template <typename T, bool DIRECT>
+hipError_t launch_t(const WgInst* d_insts, int count, int lds_bytes, bool res, hipStream_t st) {
+  auto fn = res ? k_wg_res<T, DIRECT> : k_wg_solve<T, DIRECT>;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
   if (e != hipSuccess) return e;
@@ -480,13 +942,14 @@ hipError_t device_copy(void* dst, const void* src, size_t bytes, hipStream_t st)
   return hipGetLastError();
 }
 
-// Launches the workgroup engine over `count` instances (one workgroup each).
-hipError_t wg_launch(const WgInst* d_insts, int count, int lds_bytes, bool fp64, bool direct,
+// Launches the workgroup engine over `count` instances (one workgroup each);
+// res: the resident variant (every record carries its LDS layout).
+hipError_t wg_launch(const WgInst* d_insts, int count, int lds_bytes, bool fp64, bool direct, bool res,
                      hipStream_t st) {
-  if (fp64) return direct ? launch_t<double, true>(d_insts, count, lds_bytes, st)
-                          : launch_t<double, false>(d_insts, count, lds_bytes, st);
-  return direct ? launch_t<float, true>(d_insts, count, lds_bytes, st)
-                : launch_t<float, false>(d_insts, count, lds_bytes, st);
+  if (fp64) return direct ? launch_t<double, true>(d_insts, count, lds_bytes, res, st)
+                          : launch_t<double, false>(d_insts, count, lds_bytes, res, st);
+  return direct ? launch_t<float, true>(d_insts, count, lds_bytes, res, st)
+                : launch_t<float, false>(d_insts, count, lds_bytes, res, st);
 }
 
 }  // namespace pdos_amd

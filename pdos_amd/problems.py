@@ -207,8 +207,10 @@ def random_feasible(seed: int = 1, n: int = 100_000, m: int = 300_000, nnz: int 
     ys = ss - z
     b = A @ xs + ss
     c = -(A.T @ ys)
-    return Problem(n, m, A.indptr, A.indices, A.data, b, c, f, l, q, _params(**params),
-                   name=f"random_s{seed}_n{n}_m{m}")
+    p = Problem(n, m, A.indptr, A.indices, A.data, b, c, f, l, q, _params(**params),
+                name=f"random_s{seed}_n{n}_m{m}")
+    p.planted_opt = float(c @ xs)  # the optimal value c'x* = -b'y*
+    return p
 
 
 def jumbled(p: Problem, seed: int, dup_frac: float = 0.3) -> Problem:
