@@ -552,9 +552,14 @@ def run_dense(args):
 def run_wg(args, probs, config, plugin):
     prec = args.precision or "fp64"
     t0 = time.perf_counter()
-    h = pdos_amd.BatchHandle(probs, plugin=plugin, precision=prec)
+    h = pdos_amd.BatchHandle(probs, plugin=plugin, precision=prec, tune=bool(args.tune))
     prep = time.perf_counter() - t0
-    log(f"prepare (host setup + factor + upload): {prep:.2f} s")
+    tuning = h.tuning()
+    log(f"prepare (host setup + factor{' + launch-shape tuning' if args.tune else ''} + upload): {prep:.2f} s")
+    for r in tuning:
+        log(f"  tuned n+m in {r['n_plus_m']}: {r['count']} instances -> {r['variant']}, persistent "
+            f"{r['persistent']}, {r['entries_per_lane']} entries per lane; candidates (variant, persistent, "
+            f"entries per lane, ms): {r['candidates']}")
     for _ in range(args.warmup):
         h.run()
     kms = []
@@ -570,7 +575,9 @@ def run_wg(args, probs, config, plugin):
     acc = {"solved": int(solved), "count": len(outs), "verified_fp64": verify_list(probs, outs),
            "mean_iters": float(iters.mean()), "max_iters": int(iters.max())}
     extra = {"plain_gpu": plain_gpu(probs[:args.plain_sample], plugin, prec) if args.plain_sample else None,
-             "prepare_s": round(prep, 3), "host_threads": int(os.environ.get("PDOS_HOST_THREADS", os.cpu_count()))}
+             "prepare_s": round(prep, 3), "host_threads": int(os.environ.get("PDOS_HOST_THREADS", os.cpu_count())),
+             "tuning": [{k: r[k] for k in ("n_plus_m", "count", "variant", "persistent", "entries_per_lane")}
+                        for r in tuning]}
     return acc["verified_fp64"] / wall, wall, "f64" if prec == "fp64" else "f32", config, roof, acc, extra, \
         probs[:args.cpu_sample], plugin
 
@@ -630,6 +637,8 @@ def main():
                     help="family: error of every result against an FP64 solve of the batch, and FP64 polish")
     ap.add_argument("--errors-out", default="", help="family: write per-instance errors to this .npz")
     ap.add_argument("--products", type=int, default=0, help="family, mixed: bf16 products (1-3)")
+    ap.add_argument("--tune", type=int, default=1,
+                    help="uneven: time candidate launch shapes per size bucket at prepare (PdosOptions.tune)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="end with one 256 MiB device-to-device copy (the reference kernel for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters in profiled runs)")

@@ -50,7 +50,12 @@ typedef struct PdosOptions {
                         GEMM computes the correction x+ - x, whose error shrinks
                         with the step, so one product converges like three. */
   int equil;       /* PDOS_EQUIL_* (the family engine: reference only) */
-  int reserved[1];
+  int tune;        /* batches (pdos_batch, pdos_batch_prepare): 1 = time candidate
+                      launch shapes of the workgroup engine per size bucket of
+                      n + m (kernel variant, persistent work-queue launch,
+                      entries per lane) on samples of the batch and use the
+                      fastest; decisions are cached per process.  0 = the
+                      built-in rules. */
 } PdosOptions;
 
 typedef struct PdosInfo {
@@ -101,6 +106,11 @@ int pdos_batch_fetch(PdosBatchHandle *h, Sol **sols, PdosInfo *infos);
 void pdos_batch_free(PdosBatchHandle *h);
 /* Device memory a prepared batch holds (bytes; 0 for NULL). */
 size_t pdos_batch_device_bytes(const PdosBatchHandle *h);
+/* The tuner's decisions for a batch prepared with opt->tune (JSON: per size
+ * bucket the n + m range, instances, sample size, chosen variant, launch mode
+ * and entries per lane, and every candidate's [variant, persistent,
+ * entries_per_lane, kernel ms]); "" without tuning.  Owned by the handle. */
+const char *pdos_batch_tuning(const PdosBatchHandle *h);
 
 /* Warm start for every instance of a prepared batch from device memory:
  * instance i reads [x (n), y (m), s (m)] in original units (float) at

@@ -13,6 +13,7 @@ library is missing, `lib()` raises.
 from __future__ import annotations
 
 import ctypes as C
+import json
 import os
 
 import numpy as np
@@ -31,7 +32,7 @@ ENGINE = {"auto": 0, "wg": 1, "grid": 2}
 class PdosOptions(C.Structure):
     _fields_ = [("plugin", C.c_int), ("precision", C.c_int), ("engine", C.c_int),
                 ("device", C.c_int), ("block", C.c_int), ("bf16_products", C.c_int),
-                ("equil", C.c_int), ("reserved", C.c_int * 1)]
+                ("equil", C.c_int), ("tune", C.c_int)]
 
 
 class PdosInfo(C.Structure):
@@ -48,7 +49,8 @@ class PdosInfo(C.Structure):
 # Every symbol include/pdos.h and include/pdos_ext.h declare.
 EXPORTS = ["pdos", "freeData", "freeSol", "tic", "toc", "tocq", "pdos_default_options",
            "pdos_ex", "pdos_last_info", "pdos_batch", "pdos_batch_prepare", "pdos_batch_run",
-           "pdos_batch_fetch", "pdos_batch_free", "pdos_batch_device_bytes", "pdos_batch_set_warm_dev", "pdos_device_alloc",
+           "pdos_batch_fetch", "pdos_batch_free", "pdos_batch_device_bytes", "pdos_batch_tuning",
+           "pdos_batch_set_warm_dev", "pdos_device_alloc",
            "pdos_device_free", "pdos_device_upload", "pdos_device_download", "pdos_device_copy",
            "pdos_mlp_create",
            "pdos_mlp_free", "pdos_mlp_num_params", "pdos_mlp_get_params", "pdos_mlp_set_params",
@@ -90,6 +92,8 @@ def lib(plugin: str = "indirect"):
         L.pdos_batch_free.argtypes = [C.c_void_p]
         L.pdos_batch_device_bytes.restype = C.c_size_t
         L.pdos_batch_device_bytes.argtypes = [C.c_void_p]
+        L.pdos_batch_tuning.restype = C.c_char_p
+        L.pdos_batch_tuning.argtypes = [C.c_void_p]
         L.pdos_setup_host.restype = C.c_int
         L.pdos_setup_host.argtypes = [C.POINTER(Data), C.POINTER(Cone), C.c_int, C.POINTER(C.c_double),
                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -167,8 +171,9 @@ def solve_problem(p: Problem, plugin="indirect", precision="fp64", engine="auto"
     return out
 
 
-def solve_batch(problems, plugin="indirect", precision="fp64", equil="reference") -> list:
-    """Solves independent problems in one launch (one workgroup each)."""
+def solve_batch(problems, plugin="indirect", precision="fp64", equil="reference", tune=False) -> list:
+    """Solves independent problems in one launch (one workgroup each); tune:
+    the launch-shape tuner (PdosOptions.tune)."""
     L = lib(plugin)
     n = len(problems)
     keeps = [p.to_c() for p in problems]
@@ -177,6 +182,7 @@ def solve_batch(problems, plugin="indirect", precision="fp64", equil="reference"
     sols = (C.POINTER(Sol) * n)()
     infos = (PdosInfo * n)()
     o = options(plugin, precision, "wg", equil=equil)
+    o.tune = int(tune)
     rc = L.pdos_batch(DA, KA, n, C.byref(o), sols, infos)
     if rc != 0:
         raise RuntimeError(f"pdos_batch failed: {L.pdos_last_error().decode()}")
@@ -229,13 +235,14 @@ def setup_host(p: Problem, equil="reference", plugin="indirect"):
 class BatchHandle:
     """A prepared batch resident on the device (pdos_batch_prepare/run/fetch)."""
 
-    def __init__(self, problems, plugin="indirect", precision="fp64"):
+    def __init__(self, problems, plugin="indirect", precision="fp64", tune=False):
         self.L = lib(plugin)
         self.n = len(problems)
         keeps = [p.to_c() for p in problems]
         DA = (C.POINTER(Data) * self.n)(*[C.pointer(kk[0]) for kk in keeps])
         KA = (C.POINTER(Cone) * self.n)(*[C.pointer(kk[1]) for kk in keeps])
         o = options(plugin, precision, "wg")
+        o.tune = int(tune)
         self.h = self.L.pdos_batch_prepare(DA, KA, self.n, C.byref(o))
         if not self.h:
             raise RuntimeError(f"pdos_batch_prepare failed: {self.L.pdos_last_error().decode()}")
@@ -266,6 +273,11 @@ class BatchHandle:
 
     def device_bytes(self) -> int:
         return int(self.L.pdos_batch_device_bytes(self.h))
+
+    def tuning(self) -> list:
+        """The tuner's decisions per size bucket ([] without tuning)."""
+        t = self.L.pdos_batch_tuning(self.h).decode()
+        return json.loads(t) if t else []
 
     def close(self):
         if self.h:

@@ -13,6 +13,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -240,8 +242,10 @@ bool WgBatch::group_and_factor(std::string& err) {
 // both CSR forms of A (and, direct plugin, A's column indices as pivot
 // positions), the rows' wave batches, cone offsets and, direct plugin, the
 // permutation, its inverse and both sweeps of L in level order.  Fills the
-// layout, including the per-instance part after the image.
-static void res_image(const Setup& s, const Factor* f, bool fp64, std::vector<unsigned char>& img, ResLayout& L) {
+// layout, including the per-instance part after the image.  epl: entries per
+// lane that the rows' wave batches aim for.
+static void res_image(const Setup& s, const Factor* f, bool fp64, int epl, std::vector<unsigned char>& img,
+                      ResLayout& L) {
   img.clear();
   auto put = [&](const void* src, size_t bytes) {
     const size_t off = (img.size() + 15) & ~size_t(15);
@@ -254,10 +258,12 @@ static void res_image(const Setup& s, const Factor* f, bool fp64, std::vector<un
     std::vector<float> w(v.begin(), v.end());
     return put(w.data(), w.size() * 4);
   };
-  // items (rows) in class order and their wave batches
-  auto lanes = [](int len) {
+  // items (rows) in class order and their wave batches: a row takes the
+  // fewest lanes (a power of two, at most 64) that leave each lane at most
+  // `epl` of its entries
+  auto lanes = [epl](int len) {
     int lg = 0;
-    while (lg < 6 && len > 4 * (1 << lg)) ++lg;
+    while (lg < 6 && len > epl * (1 << lg)) ++lg;
     return lg;
   };
   auto batches = [&](const std::vector<int>& items, const std::vector<int>& len, std::vector<int>& ord,
@@ -440,17 +446,32 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
     rep.resize(count);
     for (int i = 0; i < count; ++i) grp[i] = rep[i] = i;
   }
-  lds = 0;
-  const char* rs_env = std::getenv("PDOS_WG_RESIDENT");
-  const bool resident = !(rs_env && !std::strcmp(rs_env, "0"));
+  variant_env = wg_variant_env();
+  persist_env = -1;
+  if (const char* pe = std::getenv("PDOS_WG_PERSISTENT")) persist_env = std::atoi(pe) ? 1 : 0;
+  // the kernels' own LDS (the work-queue slot) comes first
+  const int lds_cap = C.max_lds - 64;
   // members too large for one workgroup's LDS go to the whole-GPU engine
   is_big.assign(count, 0);
   big.clear();
   for (int i = 0; i < count; ++i)
-    if (wg_lds_bytes(S[i].n, S[i].m, S[i].n + S[i].m, direct, tsz) > C.max_lds) {
+    if (wg_lds_bytes(S[i].n, S[i].m, S[i].n + S[i].m, direct, tsz) > lds_cap) {
       is_big[i] = 1;
       big.push_back(i);
     }
+  // LDS images of the groups (on the host threads), unless the streaming
+  // variant is forced
+  std::vector<std::vector<unsigned char>> imgs(count);
+  std::vector<ResLayout> lays(count);
+  if (variant_env != 0) {
+    std::string ierr;
+    parallel_for(count, [&](int i, std::string&) {
+      if (!is_big[i] && rep[grp[i]] == i)
+        res_image(S[i], direct ? &F[grp[i]] : nullptr, fp64, epl.empty() ? 4 : epl[i], imgs[i], lays[i]);
+      return true;
+    }, ierr);
+  }
+  vlds.assign(count, VarLds{0, 0, 0});
   for (int i = 0; i < count; ++i) {
     const Setup& s = S[i];
     Off& k = pk[i];
@@ -473,21 +494,20 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
       k.ncb = (int)cb.size() / 4;
       k.D = put_vec(P, s.D);
       k.E = put_vec(P, s.E);
-      // the resident variant when the group's image and iterates fit in LDS
-      // without fewer workgroups per CU than the streaming variant (or the
-      // batch has at most one workgroup per CU anyway)
+      // the image, kept when at least the variant with the image in device
+      // memory fits (the variant is chosen per record afterwards)
       k.res = 0;
-      if (resident) {
-        std::vector<unsigned char> img;
-        res_image(s, direct ? &F[grp[i]] : nullptr, fp64, img, k.lay);
-        const long cu_lds = 160 * 1024;
-        const int old_b = wg_lds_bytes(s.n, s.m, s.n + s.m, direct, tsz);
-        auto occ = [&](long b) { return std::max<long>(1, std::min<long>(kOccMax, cu_lds / b)); };
-        if (k.lay.total <= C.max_lds && (count <= C.num_cu || occ(k.lay.total) >= occ(old_b))) {
-          k.res = 1;
-          k.img = P.put(img.data(), img.size());
-        }
+      k.lay = lays[i];
+      if (!imgs[i].empty() && k.lay.total - k.lay.img <= lds_cap) {
+        k.res = 1;
+        k.img = P.put(imgs[i].data(), imgs[i].size());
+        std::vector<unsigned char>().swap(imgs[i]);
       }
+    }
+    vlds[i].x = wg_lds_bytes(s.n, s.m, s.n + s.m, direct, tsz);
+    if (k.res) {
+      vlds[i].y = k.lay.total <= lds_cap ? k.lay.total : 0;
+      vlds[i].z = k.lay.total - k.lay.img;
     }
     k.b = put_vec(P, s.b);
     k.c = put_vec(P, s.c);
@@ -512,12 +532,12 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
       k.blp = put_vec(P, f.bl_ptr);
       k.blr = put_vec(P, f.bl_rows);
     }
-    lds = std::max(lds, k.res ? k.lay.total : wg_lds_bytes(s.n, s.m, s.n + s.m, direct, tsz));
   }
   ho.assign(count, 0);
   for (int i = 0; i < count; ++i)  // per row: 5 residuals, then (indirect) 10 x {CG steps, |q|}
     if (want_hist && par[i].VERBOSE && !is_big[i]) ho[i] = P.zeros(sizeof(double) * (direct ? 5 : 25) * verbose_hist_rows(par[i].MAX_ITERS));
   res_off = P.zeros(sizeof(DevResult) * count);
+  queue_off = P.zeros(sizeof(int) * kClasses);
   inst_off = P.zeros(sizeof(WgInst) * count);
   cinst_off = P.zeros(sizeof(WgInst) * count);
   bytes = P.buf.size();
@@ -588,43 +608,12 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
     w.hist_rows = want_hist && par[i].VERBOSE ? verbose_hist_rows(par[i].MAX_ITERS) : 0;
     w.hist = w.hist_rows ? reinterpret_cast<double*>(base + ho[i]) : nullptr;
     w.cg_hist = w.hist_rows && !direct ? w.hist + (size_t)5 * w.hist_rows : nullptr;
-    w.lds_bytes = wg_lds_bytes(s.n, s.m, s.n + s.m, direct, tsz);
     if (k.res) {
-      w.resident = 1;
       w.img = base + k.img;
       w.lay = k.lay;
-      w.lds_bytes = k.lay.total;
     }
   }
-  // launch classes by workgroups per CU that the LDS allows (at most
-  // kClasses; the engine's registers allow kClasses), biggest first, and
-  // within a class the instances with the most nonzeros first
-  {
-    const long cu_lds = 160 * 1024;
-    auto occ = [&](int i) {
-      return 2 * (int)std::max<long>(1, std::min<long>(kOccMax, cu_lds / insts[i].lds_bytes)) + insts[i].resident;
-    };
-    order.clear();
-    for (int i = 0; i < count; ++i)
-      if (!is_big[i]) order.push_back(i);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-      const int oa = occ(a), ob = occ(b);
-      if (oa != ob) return oa < ob;
-      return S[rep[grp[a]]].nnz() > S[rep[grp[b]]].nnz();
-    });
-    ncls = 0;
-    const int nwg = (int)order.size();
-    for (int k = 0; k < nwg; ++k) {
-      if (k == 0 || occ(order[k]) != occ(order[k - 1])) {
-        cls_start[ncls] = k;
-        cls_lds[ncls] = 0;
-        cls_res[ncls] = insts[order[k]].resident;
-        ++ncls;
-      }
-      cls_lds[ncls - 1] = std::max(cls_lds[ncls - 1], insts[order[k]].lds_bytes);
-    }
-    cls_start[ncls] = nwg;
-  }
+  set_shapes();
   std::memcpy(P.buf.data() + inst_off, insts.data(), sizeof(WgInst) * count);
   for (size_t k = 0; k < order.size(); ++k)
     std::memcpy(P.buf.data() + cinst_off + sizeof(WgInst) * k, &insts[order[k]], sizeof(WgInst));
@@ -638,10 +627,6 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
   if (e == hipSuccess && !ev0) e = hipEventCreate(&ev0);
   if (e == hipSuccess && !ev1) e = hipEventCreate(&ev1);
-  for (int c = 0; c + 1 < ncls && e == hipSuccess; ++c) {
-    if (!xs[c]) e = hipStreamCreateWithFlags(&xs[c], hipStreamNonBlocking);
-    if (e == hipSuccess && !xev[c]) e = hipEventCreateWithFlags(&xev[c], hipEventDisableTiming);
-  }
   if (e != hipSuccess) { err = std::string("upload: ") + hipGetErrorString(e); return false; }
   gsol.clear();
   for (int i : big) {  // own device memory: the batch may hold the thread's arena
@@ -658,6 +643,260 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
     gsol.push_back(std::move(g));
   }
   upload_ms = now_ms() - t0;
+  return true;
+}
+
+// -1 automatic; PDOS_WG_VARIANT = stream | resident | image forces 0, 1, 2
+// (where the variant fits), as does PDOS_WG_RESIDENT=0 the streaming one.
+int wg_variant_env() {
+  int v = -1;
+  if (const char* e = std::getenv("PDOS_WG_VARIANT")) {
+    if (!std::strcmp(e, "stream")) v = 0;
+    else if (!std::strcmp(e, "resident")) v = 1;
+    else if (!std::strcmp(e, "image")) v = 2;
+  }
+  if (const char* rs = std::getenv("PDOS_WG_RESIDENT"))
+    if (!std::strcmp(rs, "0")) v = 0;
+  return v;
+}
+
+// Kernel variant and launch mode of every record, then the launch classes.
+// Variant: forced (environment, else the tuner's choice) where it fits;
+// automatic: the resident variant when the image and iterates fit in LDS
+// without fewer workgroups per CU than the streaming variant (or the launch
+// has at most one workgroup per CU anyway), else the image in device memory
+// where its iterates fit, else the streaming variant.  Classes group records
+// by (workgroups per CU that the LDS allows, launch mode:
+// PDOS_WG_PERSISTENT = 1 / 0 forces the work-queue launch on or off), biggest
+// first, and within a class the instances with the most nonzeros first.
+void WgBatch::set_shapes() {
+  const int count = (int)insts.size();
+  const long cu_lds = 160 * 1024;
+  auto wpc = [&](long b) { return (int)std::max<long>(1, std::min<long>(kOccMax, cu_lds / b)); };
+  const int nwg_total = count - (int)big.size();
+  for (int i = 0; i < count; ++i) {
+    if (is_big[i]) continue;
+    WgInst& w = insts[i];
+    const VarLds& L = vlds[i];
+    const int vf = variant_env >= 0 ? variant_env : (var_force.empty() ? -1 : var_force[i]);
+    int v = 0;
+    if (vf == 1) v = L.y ? 1 : 0;
+    else if (vf == 2) v = L.z ? 2 : 0;
+    else if (vf < 0) v = L.y && (nwg_total <= ctx().num_cu || wpc(L.y) >= wpc(L.x)) ? 1 : (L.z ? 2 : 0);
+    w.resident = v;
+    w.lds_bytes = v == 1 ? L.y : (v == 2 ? L.z : L.x);
+  }
+  auto key = [&](int i) {
+    const int p = persist_env >= 0 ? persist_env : (persist.empty() ? 0 : persist[i]);
+    return wpc(insts[i].lds_bytes) * 2 + p;
+  };
+  order.clear();
+  for (int i = 0; i < count; ++i)
+    if (!is_big[i]) order.push_back(i);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    const int ka = key(a), kb = key(b);
+    if (ka != kb) return ka < kb;
+    return S[rep[grp[a]]].nnz() > S[rep[grp[b]]].nnz();
+  });
+  ncls = 0;
+  const int nwg = (int)order.size();
+  for (int k = 0; k < nwg; ++k) {
+    if (k == 0 || key(order[k]) != key(order[k - 1])) {
+      cls_start[ncls] = k;
+      cls_lds[ncls] = 0;
+      cls_persist[ncls] = key(order[k]) & 1;
+      cls_occ[ncls] = key(order[k]) / 2;
+      ++ncls;
+    }
+    cls_lds[ncls - 1] = std::max(cls_lds[ncls - 1], insts[order[k]].lds_bytes);
+  }
+  cls_start[ncls] = nwg;
+  // classes to streams, longest processing time first: a class's work is
+  // estimated as its instances' nonzeros and iterates (A, A' and, direct
+  // plugin, L), divided by the workgroups that run at once (a class that fits
+  // more workgroups per CU finishes the same work sooner)
+  std::vector<double> work(ncls, 0.0);
+  for (int c = 0; c < ncls; ++c) {
+    for (int k = cls_start[c]; k < cls_start[c + 1]; ++k) {
+      const int i = order[k];
+      const Setup& r = S[rep[grp[i]]];
+      double w = 2.0 * (double)r.nnz() + 3.0 * (S[i].n + S[i].m);
+      if (direct) w += 2.0 * (double)F[grp[i]].Lp.back();
+      work[c] += w;
+    }
+    work[c] /= cls_occ[c];
+  }
+  for (int c = 0; c < ncls; ++c) cls_seq[c] = c;
+  std::stable_sort(cls_seq, cls_seq + ncls, [&](int a, int b) { return work[a] > work[b]; });
+  double load[kStreams] = {0};
+  for (int q = 0; q < ncls; ++q) {
+    const int c = cls_seq[q];
+    int best = 0;
+    for (int t = 1; t < kStreams; ++t)
+      if (load[t] < load[best]) best = t;
+    cls_stream[c] = best;
+    load[best] += work[c];
+  }
+}
+
+// Launch-shape tuning.  The candidates per size bucket (half octaves of
+// n + m) are the kernel variant (streaming, resident, image in device
+// memory), the launch mode (a workgroup per instance, or persistent
+// workgroups that take instances from a work queue) and, for the image
+// variants, the entries per lane that the wave batches aim for (4, 8).
+// Each is timed (HIP events) on a sample of the bucket's instances, every
+// one running a fixed kTuneIters iterations (EPS 0, so no early stop): twice
+// as many instances as CUs (repeating instances when the bucket is smaller)
+// when the whole batch holds more instances than there are CUs, else the
+// bucket itself.  Decisions are cached per process by (device, precision,
+// plugin, load, bucket).
+namespace {
+constexpr int kTuneIters = 50;
+constexpr double kPersistMargin = 0.02;
+struct TuneChoice { signed char v, p, e; };
+std::mutex g_tune_mu;
+std::map<std::vector<int>, std::pair<TuneChoice, std::string>> g_tune_cache;
+const char* variant_name(int v) { return v == 1 ? "resident" : (v == 2 ? "image" : "stream"); }
+}  // namespace
+
+bool WgBatch::tune(int dev, bool fp64_, std::string& err) {
+  tuning.clear();
+  if (wg_variant_env() >= 0) return true;  // the environment forces the variant
+  Ctx& C = ctx();
+  if (!C.ensure(dev, err)) return false;
+  const int count = (int)S.size();
+  const int tsz = fp64_ ? 8 : 4;
+  const int lds_cap = C.max_lds - 64;
+  var_force.assign(count, -1);
+  persist.assign(count, 0);
+  epl.assign(count, 4);
+  std::map<int, std::vector<int>> buckets;
+  int nwg = 0;
+  for (int i = 0; i < count; ++i) {
+    const int N = S[i].n + S[i].m;
+    if (wg_lds_bytes(S[i].n, S[i].m, N, direct, tsz) > lds_cap) continue;  // whole-GPU engine
+    ++nwg;
+    buckets[(int)std::floor(2.0 * std::log2((double)std::max(N, 2)))].push_back(i);
+  }
+  const int load = nwg > C.num_cu ? 1 : 0;
+  std::string js = "[";
+  for (const auto& bk : buckets) {
+    const std::vector<int>& mem = bk.second;
+    const std::vector<int> key = {dev, fp64_ ? 1 : 0, direct ? 1 : 0, load, bk.first};
+    TuneChoice best{-1, 0, 4};
+    std::string cands;
+    bool cached = false;
+    {
+      std::lock_guard<std::mutex> g(g_tune_mu);
+      auto it = g_tune_cache.find(key);
+      if (it != g_tune_cache.end()) { best = it->second.first; cands = it->second.second; cached = true; }
+    }
+    int ns = (int)mem.size();
+    if (!cached) {
+      if (load) ns = 2 * C.num_cu;
+      // the sample as a batch of its own, sharing matrix groups like this one
+      auto make = [&](int e, std::unique_ptr<WgBatch>& T) {
+        T.reset(new WgBatch);
+        T->direct = direct;
+        T->S.resize(ns);
+        T->par.resize(ns);
+        T->grp.resize(ns);
+        T->epl.assign(ns, (signed char)e);
+        std::map<int, int> gmap;
+        for (int k = 0; k < ns; ++k) {
+          const int i = mem[(size_t)k * mem.size() / ns];
+          auto g = gmap.find(grp[i]);
+          if (g == gmap.end()) {
+            g = gmap.emplace(grp[i], (int)T->rep.size()).first;
+            T->rep.push_back(k);
+            T->S[k] = S[i];
+            const Setup& r = S[rep[grp[i]]];
+            if (rep[grp[i]] != i) {
+              T->S[k].cp = r.cp; T->S[k].ci = r.ci; T->S[k].cx = r.cx;
+              T->S[k].rp = r.rp; T->S[k].ri = r.ri; T->S[k].rx = r.rx;
+            }
+            if (direct) T->F.push_back(F[grp[i]]);
+          } else {
+            T->S[k] = S[i];  // the matrix arrays live with the group's first sample
+            std::vector<int>().swap(T->S[k].cp); std::vector<int>().swap(T->S[k].ci);
+            std::vector<double>().swap(T->S[k].cx); std::vector<int>().swap(T->S[k].rp);
+            std::vector<int>().swap(T->S[k].ri); std::vector<double>().swap(T->S[k].rx);
+          }
+          T->grp[k] = g->second;
+          Params q = par[i];
+          q.MAX_ITERS = kTuneIters;
+          q.EPS_ABS = 0.0;
+          q.VERBOSE = 0;
+          T->par[k] = q;
+        }
+        return T->prepare(dev, fp64_, direct, false, err);
+      };
+      // one untimed run per sample batch (first launch costs), then one timed
+      // run per candidate
+      auto time = [&](WgBatch& T, int v, int p, double& ms) {
+        T.var_force.assign(ns, (signed char)v);
+        T.persist.assign(ns, (signed char)p);
+        T.set_shapes();
+        if (!T.upload_records(err)) return false;
+        if (!T.warm) {
+          if (!T.run(err)) return false;
+          T.warm = true;
+        }
+        if (!T.run(err)) return false;
+        ms = T.kernel_ms;
+        return true;
+      };
+      auto has = [&](WgBatch& T, int v) {
+        for (const VarLds& L : T.vlds)
+          if (v == 1 ? L.y != 0 : L.z != 0) return true;
+        return false;
+      };
+      char buf[96];
+      double best_ms = 1e30;
+      std::unique_ptr<WgBatch> T;
+      if (!make(4, T)) { err = "tuning: " + err; return false; }
+      for (int v = 0; v < kVariants; ++v) {
+        if (v && !has(*T, v)) continue;
+        for (int p = 0; p < 2; ++p) {
+          double ms;
+          if (!time(*T, v, p, ms)) { err = "tuning: " + err; return false; }
+          std::snprintf(buf, sizeof buf, "%s[\"%s\",%d,4,%.4f]", cands.empty() ? "" : ",", variant_name(v), p, ms);
+          cands += buf;
+          // the work-queue launch must win by kPersistMargin: each launch
+          // mode is a launch class of its own, and classes beyond the
+          // device's hardware queues run one after another
+          const double eff = p ? ms * (1.0 + kPersistMargin) : ms;
+          if (eff < best_ms) { best_ms = eff; best = TuneChoice{(signed char)v, (signed char)p, 4}; }
+        }
+      }
+      if (best.v != 0)
+        for (int e : {8}) {
+          std::unique_ptr<WgBatch> T2;
+          double ms;
+          if (!make(e, T2) || !time(*T2, best.v, best.p, ms)) { err = "tuning: " + err; return false; }
+          std::snprintf(buf, sizeof buf, ",[\"%s\",%d,%d,%.4f]", variant_name(best.v), best.p, e, ms);
+          cands += buf;
+          const double eff = best.p ? ms * (1.0 + kPersistMargin) : ms;
+          if (eff < best_ms) { best_ms = eff; best.e = (signed char)e; }
+        }
+      std::lock_guard<std::mutex> g(g_tune_mu);
+      g_tune_cache[key] = {best, cands};
+    }
+    for (int i : mem) {
+      var_force[i] = best.v;
+      persist[i] = best.p;
+      epl[i] = best.e;
+    }
+    char head[256];
+    std::snprintf(head, sizeof head,
+                  "%s{\"n_plus_m\":[%d,%d],\"count\":%d,\"sample\":%d,\"cached\":%s,\"variant\":\"%s\","
+                  "\"persistent\":%d,\"entries_per_lane\":%d,\"candidates\":[",
+                  js.size() > 1 ? "," : "", (int)std::ceil(std::exp2(bk.first / 2.0)),
+                  (int)std::ceil(std::exp2((bk.first + 1) / 2.0)) - 1, (int)mem.size(), cached ? 0 : ns,
+                  cached ? "true" : "false", variant_name(best.v), best.p, best.e);
+    js += head + cands + "]}";
+  }
+  tuning = js + "]";
   return true;
 }
 
@@ -694,21 +933,35 @@ bool WgBatch::set_warm(const float* dW, long ld, int check, std::string& err) {
   return true;
 }
 
-// One launch per class, class c > 0 on its own stream forked from and joined
-// back into `stream` (ev0 .. ev1 covers all of them).
+// One launch per class, on the class's stream (the extra streams forked from
+// and joined back into `stream`; ev0 .. ev1 covers all of them).
 bool WgBatch::run(std::string& err) {
   unsigned char* base = static_cast<unsigned char*>(dbuf);
   const WgInst* recs = reinterpret_cast<const WgInst*>(base + cinst_off);
   const double t0 = now_ms();
-  hipError_t e = hipEventRecord(ev0, stream);
-  for (int c = 0; c < ncls && e == hipSuccess; ++c) {
-    hipStream_t sc = c == 0 ? stream : xs[c - 1];
-    if (c > 0) e = hipStreamWaitEvent(sc, ev0, 0);
-    if (e == hipSuccess)
-      e = wg_launch(recs + cls_start[c], cls_start[c + 1] - cls_start[c], cls_lds[c], fp64, direct, cls_res[c] != 0,
-                    sc);
-    if (e == hipSuccess && c > 0) e = hipEventRecord(xev[c - 1], sc);
-    if (e == hipSuccess && c > 0) e = hipStreamWaitEvent(stream, xev[c - 1], 0);
+  hipError_t e = hipSuccess;
+  // the extra streams and their join events (created on first use)
+  bool used[kStreams] = {false};
+  for (int c = 0; c < ncls; ++c) used[cls_stream[c]] = true;
+  for (int t = 1; t < kStreams && e == hipSuccess; ++t) {
+    if (!used[t]) continue;
+    if (!xs[t - 1]) e = hipStreamCreateWithFlags(&xs[t - 1], hipStreamNonBlocking);
+    if (e == hipSuccess && !xev[t - 1]) e = hipEventCreateWithFlags(&xev[t - 1], hipEventDisableTiming);
+  }
+  if (e == hipSuccess) e = hipEventRecord(ev0, stream);
+  for (int t = 1; t < kStreams && e == hipSuccess; ++t)
+    if (used[t]) e = hipStreamWaitEvent(xs[t - 1], ev0, 0);
+  for (int q = 0; q < ncls && e == hipSuccess; ++q) {
+    const int c = cls_seq[q];
+    hipStream_t sc = cls_stream[c] == 0 ? stream : xs[cls_stream[c] - 1];
+    e = wg_launch(recs + cls_start[c], cls_start[c + 1] - cls_start[c], cls_lds[c], fp64, direct,
+                  cls_persist[c] ? reinterpret_cast<int*>(base + queue_off) + c : nullptr,
+                  cls_occ[c] * ctx().num_cu, sc);
+  }
+  for (int t = 1; t < kStreams && e == hipSuccess; ++t) {
+    if (!used[t]) continue;
+    e = hipEventRecord(xev[t - 1], xs[t - 1]);
+    if (e == hipSuccess) e = hipStreamWaitEvent(stream, xev[t - 1], 0);
   }
   if (e == hipSuccess) e = hipEventRecord(ev1, stream);
   if (e == hipSuccess) e = hipEventSynchronize(ev1);
@@ -1000,6 +1253,11 @@ static PdosBatchHandle* batch_prepare(const Data* const* ds, const Cone* const* 
     delete h;
     return nullptr;
   }
+  if (o.tune && !h->B.tune(o.device, o.precision == PDOS_FP64, err)) {
+    g_err = err;
+    delete h;
+    return nullptr;
+  }
   if (!h->B.prepare(o.device, o.precision == PDOS_FP64, direct, borrow, err)) {
     g_err = err;
     delete h;
@@ -1052,6 +1310,8 @@ int pdos_batch_fetch(PdosBatchHandle* h, Sol** sols, PdosInfo* infos) {
 void pdos_batch_free(PdosBatchHandle* h) { delete h; }
 
 size_t pdos_batch_device_bytes(const PdosBatchHandle* h) { return h ? h->B.bytes : 0; }
+
+const char* pdos_batch_tuning(const PdosBatchHandle* h) { return h ? h->B.tuning.c_str() : ""; }
 
 int pdos_batch_set_warm_dev(PdosBatchHandle* h, const float* dW, long ld, int check) {
   if (!h) return -1;

@@ -84,7 +84,7 @@ struct WgInst {
   double* hist;            // VERBOSE: residuals (p_res, d_res, p_obj, d_obj, eta) of checks 0, 10, 20, ...
   double* cg_hist;         // VERBOSE, indirect: per update {CG steps, final |q|} (hist_rows * 10 rows)
   // resident variant: matrices, factor and level sets in LDS
-  int resident;            // 1: launched as k_wg_res with `lay`
+  int resident;            // kernel variant: 0 k_wg_solve, 1 k_wg_res (image in LDS), 2 k_wg_res (image in memory)
   int pad4;
   const unsigned char* img;  // the group's LDS image in device memory (lay.img bytes)
   ResLayout lay;

@@ -51,7 +51,7 @@ hipError_t dense_at_pair(hipStream_t st, int m, int n, int ldb, int m64, const _
                          const float* Y, const float* W, float* Up, float* Vp, size_t plane, int splits);
 hipError_t dense_a(hipStream_t st, int m, int n, int ldb, int n64, const __bf16* ARh, const __bf16* ARl,
                    const float* X, float* Z);
-hipError_t wg_launch(const WgInst* d_insts, int count, int lds_bytes, bool fp64, bool direct, bool res,
+hipError_t wg_launch(const WgInst* d_insts, int count, int lds_bytes, bool fp64, bool direct, int* queue, int grid,
                      hipStream_t st);
 
 // Whole-GPU engine for one large instance (both plugins), prepared once and
@@ -97,19 +97,43 @@ struct WgBatch {
   std::vector<int> big;
   std::vector<std::unique_ptr<GridSolver>> gsol;
   std::vector<char> is_big;
-  int lds = 0;
+  // per instance: LDS bytes of the streaming (x), resident (y) and
+  // image-in-device-memory (z) variants, 0 where the variant is unavailable
+  struct VarLds { int x, y, z; };
+  std::vector<VarLds> vlds;
+  int variant_env = -1, persist_env = -1;
   // Launch classes: instances grouped by how many of their workgroups fit
   // in a CU's LDS, each class launched with its own LDS size on its own
   // stream; the device holds a second copy of the records in class order.
-  // A class is (workgroups per CU, resident variant or not).
+  // A class is (workgroups per CU, launch mode: a workgroup per instance or
+  // persistent workgroups draining a work queue); each record names its own
+  // kernel variant (streaming, resident, image in device memory).
   static constexpr int kOccMax = 4;
+  static constexpr int kVariants = 3;
   static constexpr int kClasses = 2 * kOccMax;
   std::vector<int> order;                 // class order -> instance
-  int ncls = 0, cls_start[kClasses + 1] = {0}, cls_lds[kClasses] = {0}, cls_res[kClasses] = {0};
-  size_t cinst_off = 0;
+  int ncls = 0, cls_start[kClasses + 1] = {0}, cls_lds[kClasses] = {0};
+  int cls_persist[kClasses] = {0}, cls_occ[kClasses] = {0};
+  // Classes run concurrently on at most kStreams streams (the device's
+  // hardware queues: more streams would share queues and serialize behind
+  // each other anyway); cls_seq lists the classes in launch order,
+  // cls_stream gives each its stream (0 = `stream`, s > 0 = xs[s - 1]).
+  static constexpr int kStreams = 4;
+  int cls_seq[kClasses] = {0}, cls_stream[kClasses] = {0};
+  size_t cinst_off = 0, queue_off = 0;
+  // tuner choices per instance (empty: automatic): kernel variant (-1
+  // automatic), launch mode (1 persistent), entries per lane of the wave
+  // batches
+  std::vector<signed char> var_force, persist, epl;
+  std::string tuning;  // the tuner's decisions (JSON), empty if it did not run
+  bool warm = false;   // run at least once (the tuner's untimed first run)
   hipStream_t xs[kClasses - 1] = {};
   hipEvent_t xev[kClasses - 1] = {};
   bool upload_records(std::string& err);
+  void set_shapes();
+  // Chooses var_force / persist / epl per size bucket by timing candidate
+  // launches on samples of the batch (before prepare).
+  bool tune(int device, bool fp64, std::string& err);
   double upload_ms = 0, kernel_ms = 0, wall_ms = 0;
   ~WgBatch();
   // Groups instances by their scaled matrix (and scalings, cones) and drops
@@ -120,6 +144,8 @@ struct WgBatch {
   bool fetch(std::vector<DevResult>& res, std::string& err);
   bool set_warm(const float* dW, long ld, int check, std::string& err);
 };
+
+int wg_variant_env();
 
 bool solve_wg(std::vector<Setup>& S, std::vector<Factor>& F, const std::vector<const Params*>& par,
               bool fp64, bool direct, int device, std::vector<DevResult>& res, double& upload_ms,

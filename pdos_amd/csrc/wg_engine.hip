@@ -20,6 +20,8 @@
 //   relax / cones / dual pdos.c:232-249, 296-303; cones.c:5-57
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "device.hpp"
 
 namespace pdos_amd {
@@ -203,11 +205,12 @@ __device__ __forceinline__ void tri_sweep(int nlev, const int* __restrict__ lptr
   }
 }
 
+// The streaming variant, for the instance of record `idx`: the iterates in
+// LDS, the matrices (and factor) read from device memory.
 template <typename T, bool DIRECT>
-__global__ __launch_bounds__(NT) void k_wg_solve(const WgInst* __restrict__ insts) {
-  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+__device__ __forceinline__ void wg_stream_body(const WgInst* __restrict__ insts, int idx, unsigned char* smem) {
   // the record by value: its fields stay in registers across the barriers
-  const WgInst I = insts[blockIdx.x];
+  const WgInst I = insts[idx];
   const int n = I.n, m = I.m, tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
 
@@ -636,39 +639,45 @@ __device__ __forceinline__ void res_chain(int t, const int* trow, const double* 
   if (on) z[r] = v;
 }
 
-template <typename T, bool DIRECT>
-__global__ __launch_bounds__(NT) void k_wg_res(const WgInst* __restrict__ insts) {
-  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+// IMG_LDS = false: the same kernel with the group's image read from device
+// memory (L2 / MALL resident) -- for instances whose matrices do not fit in
+// LDS; only b, c, the iterates and the reduction slots are in LDS.
+template <typename T, bool DIRECT, bool IMG_LDS>
+__device__ __forceinline__ void wg_res_body(const WgInst* __restrict__ insts, int idx, unsigned char* smem) {
   // the record by value: its fields stay in registers across the barriers
   // (a reference would be re-read from memory after each one)
-  const WgInst I = insts[blockIdx.x];
+  const WgInst I = insts[idx];
   const ResLayout& L = I.lay;
   const int n = I.n, m = I.m, N = I.N, tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
-  {
+  if (IMG_LDS) {
     const uint4* src = reinterpret_cast<const uint4*>(I.img);
     uint4* dst = reinterpret_cast<uint4*>(smem);
     for (int i = tid; i < L.img / 16; i += NT) dst[i] = src[i];
   }
-  const double* Dl = reinterpret_cast<const double*>(smem + L.D);
-  const double* El = reinterpret_cast<const double*>(smem + L.E);
-  const int4* rpos = reinterpret_cast<const int4*>(smem + L.rpos);
-  const int* ridx = reinterpret_cast<const int*>(smem + L.ridx);
-  const T* rval = reinterpret_cast<const T*>(smem + L.rval);
-  const int* ridp = reinterpret_cast<const int*>(smem + L.ridp);
-  const int4* cpos = reinterpret_cast<const int4*>(smem + L.cpos);
-  const int* cidx = reinterpret_cast<const int*>(smem + L.cidx);
-  const T* cval = reinterpret_cast<const T*>(smem + L.cval);
-  const int4* rbat = reinterpret_cast<const int4*>(smem + L.rbat);
-  const int4* cbat = reinterpret_cast<const int4*>(smem + L.cbat);
+  // static data: the image in LDS, or in device memory; per-instance data
+  // after the image in LDS, or at the start of LDS
+  const unsigned char* sb = IMG_LDS ? static_cast<const unsigned char*>(smem) : I.img;
+  unsigned char* db = IMG_LDS ? smem : smem - L.img;
+  const double* Dl = reinterpret_cast<const double*>(sb + L.D);
+  const double* El = reinterpret_cast<const double*>(sb + L.E);
+  const int4* rpos = reinterpret_cast<const int4*>(sb + L.rpos);
+  const int* ridx = reinterpret_cast<const int*>(sb + L.ridx);
+  const T* rval = reinterpret_cast<const T*>(sb + L.rval);
+  const int* ridp = reinterpret_cast<const int*>(sb + L.ridp);
+  const int4* cpos = reinterpret_cast<const int4*>(sb + L.cpos);
+  const int* cidx = reinterpret_cast<const int*>(sb + L.cidx);
+  const T* cval = reinterpret_cast<const T*>(sb + L.cval);
+  const int4* rbat = reinterpret_cast<const int4*>(sb + L.rbat);
+  const int4* cbat = reinterpret_cast<const int4*>(sb + L.cbat);
   const int nrbat = L.nrbat, ncbat = L.ncbat;
-  const int* socp = reinterpret_cast<const int*>(smem + L.soc);
-  double* bl = reinterpret_cast<double*>(smem + L.b);
-  double* cl = reinterpret_cast<double*>(smem + L.c);
+  const int* socp = reinterpret_cast<const int*>(sb + L.soc);
+  double* bl = reinterpret_cast<double*>(db + L.b);
+  double* cl = reinterpret_cast<double*>(db + L.c);
   for (int j = tid; j < m; j += NT) bl[j] = I.b[j];
   for (int i = tid; i < n; i += NT) cl[i] = I.c[i];
   auto al = [](size_t v) { return (v + 15) & ~(size_t)15; };
-  unsigned char* vp = smem + L.vec;
+  unsigned char* vp = db + L.vec;
   T* x = reinterpret_cast<T*>(vp);
   T* s = x + n;
   T* y = s + m;
@@ -732,7 +741,7 @@ __global__ __launch_bounds__(NT) void k_wg_res(const WgInst* __restrict__ insts)
       acc[2] += double(ci_) * double(x[i]);
     });
     if (DIRECT) {
-      const int* perm = reinterpret_cast<const int*>(smem + L.perm);
+      const int* perm = reinterpret_cast<const int*>(sb + L.perm);
       for (int kk = tid; kk < N; kk += NT) {
         const int src = perm[kk];
         double v;
@@ -844,18 +853,18 @@ __global__ __launch_bounds__(NT) void k_wg_res(const WgInst* __restrict__ insts)
     } else {
       // ---- LDL' sweeps in level order, then x = z[0:n] and s~ = z[n:] +
       // (s + lam y) read through pivot positions inside the A product ----
-      res_sweep<false>(L.nfl, reinterpret_cast<const int*>(smem + L.flb), reinterpret_cast<const int4*>(smem + L.fbat),
-                       reinterpret_cast<const int4*>(smem + L.fpos), reinterpret_cast<const int*>(smem + L.fidx),
-                       reinterpret_cast<const double*>(smem + L.fval), nullptr, bp);
+      res_sweep<false>(L.nfl, reinterpret_cast<const int*>(sb + L.flb), reinterpret_cast<const int4*>(sb + L.fbat),
+                       reinterpret_cast<const int4*>(sb + L.fpos), reinterpret_cast<const int*>(sb + L.fidx),
+                       reinterpret_cast<const double*>(sb + L.fval), nullptr, bp);
       if (L.t) {
-        res_chain(L.t, reinterpret_cast<const int*>(smem + L.trow), reinterpret_cast<const double*>(smem + L.tdq),
-                  reinterpret_cast<const double*>(smem + L.tl), bp);
+        res_chain(L.t, reinterpret_cast<const int*>(sb + L.trow), reinterpret_cast<const double*>(sb + L.tdq),
+                  reinterpret_cast<const double*>(sb + L.tl), bp);
         __syncthreads();
       }
-      res_sweep<true>(L.nbl, reinterpret_cast<const int*>(smem + L.blb), reinterpret_cast<const int4*>(smem + L.bbat),
-                      reinterpret_cast<const int4*>(smem + L.bpos), reinterpret_cast<const int*>(smem + L.bidx),
-                      reinterpret_cast<const double*>(smem + L.bval), reinterpret_cast<const double*>(smem + L.bdq), bp);
-      const int* pinv = reinterpret_cast<const int*>(smem + L.pinv);
+      res_sweep<true>(L.nbl, reinterpret_cast<const int*>(sb + L.blb), reinterpret_cast<const int4*>(sb + L.bbat),
+                      reinterpret_cast<const int4*>(sb + L.bpos), reinterpret_cast<const int*>(sb + L.bidx),
+                      reinterpret_cast<const double*>(sb + L.bval), reinterpret_cast<const double*>(sb + L.bdq), bp);
+      const int* pinv = reinterpret_cast<const int*>(sb + L.pinv);
       for (int i = tid; i < n; i += NT) x[i] = T(bp[pinv[i]]);
       spmv_res<T>(rpos, ridp, rval, rbat, nrbat, [&](int c) { return T(bp[c]); }, [&](int j, T v) {
         ax[j] = v;
@@ -911,13 +920,55 @@ __global__ __launch_bounds__(NT) void k_wg_res(const WgInst* __restrict__ insts)
   }
 }
 
+// The workgroup engine: one instance per workgroup (queue == nullptr:
+// instance blockIdx.x), or persistent workgroups taking instance after
+// instance from the counter `queue` until it passes count (every workgroup
+// reaches that exit).  Each record names its variant (WgInst::resident: 0
+// streaming, 1 image in LDS, 2 image in device memory), so instances of every
+// variant with the same LDS footprint share one launch.
+// Registers are capped for four waves per SIMD (four workgroups per CU, the
+// most the LDS classes use): the streaming body needs a few more and spills
+// 2-3 values to scratch in binary32 (reloaded once per CG step), about a dozen
+// in binary64.
 template <typename T, bool DIRECT>
-hipError_t launch_t(const WgInst* d_insts, int count, int lds_bytes, bool res, hipStream_t st) {
-  auto fn = res ? k_wg_res<T, DIRECT> : k_wg_solve<T, DIRECT>;
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_wg(const WgInst* __restrict__ insts,
+                                                                                int count, int* queue) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  __shared__ int wg_next;
+  int idx = blockIdx.x;
+  for (;;) {
+    if (queue) {
+      __syncthreads();  // the previous instance is done with LDS and wg_next
+      if (threadIdx.x == 0) wg_next = atomicAdd(queue, 1);
+      __syncthreads();
+      idx = wg_next;
+    }
+    if (idx >= count) break;
+    const int v = __builtin_amdgcn_readfirstlane(insts[idx].resident);
+    if (v == 1) wg_res_body<T, DIRECT, true>(insts, idx, smem);
+    else if (v == 2) wg_res_body<T, DIRECT, false>(insts, idx, smem);
+    else wg_stream_body<T, DIRECT>(insts, idx, smem);
+    if (!queue) break;
+  }
+}
+
+__global__ void k_zero_int(int* p) {
+  if (threadIdx.x == 0) p[0] = 0;
+}
+
+template <typename T, bool DIRECT>
+hipError_t launch_t(const WgInst* d_insts, int count, int lds_bytes, int* queue, int grid, hipStream_t st) {
+  auto fn = k_wg<T, DIRECT>;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(fn, dim3(count), dim3(NT), lds_bytes, st, d_insts);
+  if (queue) {  // a kernel, not a memset node, so that the launch may be captured in a graph
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, queue);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(fn, dim3(queue ? std::max(1, std::min(grid, count)) : count), dim3(NT), lds_bytes, st, d_insts,
+                     count, queue);
   return hipGetLastError();
 }
 
@@ -942,14 +993,16 @@ hipError_t device_copy(void* dst, const void* src, size_t bytes, hipStream_t st)
   return hipGetLastError();
 }
 
-// Launches the workgroup engine over `count` instances (one workgroup each);
-// res: the resident variant (every record carries its LDS layout).
-hipError_t wg_launch(const WgInst* d_insts, int count, int lds_bytes, bool fp64, bool direct, bool res,
+// Launches the workgroup engine over `count` instances (one workgroup each,
+// each in the variant its record names); queue (device int) != nullptr:
+// `grid` persistent workgroups drain the records in order through that
+// counter (zeroed here first).
+hipError_t wg_launch(const WgInst* d_insts, int count, int lds_bytes, bool fp64, bool direct, int* queue, int grid,
                      hipStream_t st) {
-  if (fp64) return direct ? launch_t<double, true>(d_insts, count, lds_bytes, res, st)
-                          : launch_t<double, false>(d_insts, count, lds_bytes, res, st);
-  return direct ? launch_t<float, true>(d_insts, count, lds_bytes, res, st)
-                : launch_t<float, false>(d_insts, count, lds_bytes, res, st);
+  if (fp64) return direct ? launch_t<double, true>(d_insts, count, lds_bytes, queue, grid, st)
+                          : launch_t<double, false>(d_insts, count, lds_bytes, queue, grid, st);
+  return direct ? launch_t<float, true>(d_insts, count, lds_bytes, queue, grid, st)
+                : launch_t<float, false>(d_insts, count, lds_bytes, queue, grid, st);
 }
 
 }  // namespace pdos_amd

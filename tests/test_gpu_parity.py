@@ -289,14 +289,16 @@ def test_batch_verbose_keeps_no_history():
     assert sizes[0] == sizes[1] > 0, sizes
 
 
-@pytest.mark.parametrize("resident", ["0", "1"])
+@pytest.mark.parametrize("variant", ["stream", "resident", "image"])
 @pytest.mark.parametrize("plugin", ["indirect", "direct"])
-def test_wg_streaming_and_resident_variants(monkeypatch, resident, plugin):
-    """The workgroup engine's two variants -- matrices streamed from global
-    memory (k_wg_solve) and held in LDS (k_wg_res, the default where they
-    fit) -- each against the reference on cases covering long rows, jumbled
-    columns, empty rows and warm starts; PDOS_WG_RESIDENT=0 forces the first."""
-    monkeypatch.setenv("PDOS_WG_RESIDENT", resident)
+def test_wg_streaming_and_resident_variants(monkeypatch, variant, plugin):
+    """The workgroup engine's three variants -- matrices streamed from global
+    memory (k_wg_solve), the wave-batched kernel with its image in LDS
+    (k_wg_res, the default where it fits) and the same with the image in
+    device memory -- each against the reference on cases covering long rows,
+    jumbled columns, empty rows and warm starts (PDOS_WG_VARIANT forces one)."""
+    monkeypatch.setenv("PDOS_WG_VARIANT", variant)
+    resident = variant
     for tag in ["fixture01", "portfolio_n101_norm1", "portfolio_n101_warm", "jumbled_n120_m300",
                 "random_n300_m900", "dense_m96_n24", "uneven_1", "b_zero_n60_m150"]:
         print(f"case {tag} {plugin} resident={resident}", flush=True)
