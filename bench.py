@@ -347,6 +347,8 @@ def run_family(args):
     th_te, C_te = fam.sample(seed=1004, count=n_te)
     prec = args.precision or "mixed"
     F = Family(fam.instance(C_te[0]), precision=prec, bf16_products=args.products)
+    if args.engines:
+        F.engines(args.engines)
     info = F.info()
     # 1. training data: the solver's own solutions
     t0 = time.perf_counter()
@@ -637,6 +639,8 @@ def main():
                     help="family: error of every result against an FP64 solve of the batch, and FP64 polish")
     ap.add_argument("--errors-out", default="", help="family: write per-instance errors to this .npz")
     ap.add_argument("--products", type=int, default=0, help="family, mixed: bf16 products (1-3)")
+    ap.add_argument("--engines", type=int, default=0,
+                    help="family: engines the batch is split over (0: the library's default)")
     ap.add_argument("--tune", type=int, default=1,
                     help="uneven: time candidate launch shapes per size bucket at prepare (PdosOptions.tune)")
     ap.add_argument("--calibrate-copy", action="store_true",

@@ -47,6 +47,8 @@ bool Ctx::ensure(int device, std::string& err) {
   if (stream) (void)hipStreamDestroy(stream);
   e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
   if (e != hipSuccess) { err = std::string("hipStreamCreate: ") + hipGetErrorString(e); return false; }
+  if (!ev0 && hipEventCreate(&ev0) != hipSuccess) { err = "hipEventCreate failed"; return false; }
+  if (!ev1 && hipEventCreate(&ev1) != hipSuccess) { err = "hipEventCreate failed"; return false; }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
     max_lds = (int)prop.sharedMemPerBlock;
@@ -143,8 +145,8 @@ static size_t put_vals(Packer& P, const std::vector<double>& v, bool fp64) {
 
 WgBatch::~WgBatch() {
   if (own_mem && dbuf) (void)hipFree(dbuf);
-  if (ev0) (void)hipEventDestroy(ev0);
-  if (ev1) (void)hipEventDestroy(ev1);
+  if (own_mem && ev0) (void)hipEventDestroy(ev0);
+  if (own_mem && ev1) (void)hipEventDestroy(ev1);
   for (int c = 0; c < kClasses - 1; ++c) {
     if (xev[c]) (void)hipEventDestroy(xev[c]);
     if (xs[c]) (void)hipStreamDestroy(xs[c]);
@@ -441,6 +443,7 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
                size_t img; int res; ResLayout lay; };
   std::vector<Off> pk(count);
   xo.resize(count); so.resize(count); yo.resize(count);
+  ho.assign(count, 0);
   if (grp.size() != (size_t)count) {  // no grouping requested: every instance is its own group
     grp.resize(count);
     rep.resize(count);
@@ -517,6 +520,10 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
     xo[i] = P.zeros(sizeof(double) * s.n);
     so[i] = P.zeros(sizeof(double) * s.m);
     yo[i] = P.zeros(sizeof(double) * s.m);
+    if (count == 1) {  // the history and the result right after: fetch takes them in one copy
+      if (want_hist && par[i].VERBOSE) ho[i] = P.zeros(sizeof(double) * (direct ? 5 : 25) * verbose_hist_rows(par[i].MAX_ITERS));
+      res_off = P.zeros(sizeof(DevResult));
+    }
     if (direct && rep[grp[i]] == i) {
       const Factor& f = F[grp[i]];
       k.perm = put_vec(P, f.P);
@@ -533,10 +540,11 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
       k.blr = put_vec(P, f.bl_rows);
     }
   }
-  ho.assign(count, 0);
-  for (int i = 0; i < count; ++i)  // per row: 5 residuals, then (indirect) 10 x {CG steps, |q|}
-    if (want_hist && par[i].VERBOSE && !is_big[i]) ho[i] = P.zeros(sizeof(double) * (direct ? 5 : 25) * verbose_hist_rows(par[i].MAX_ITERS));
-  res_off = P.zeros(sizeof(DevResult) * count);
+  if (count != 1 || is_big[0]) {
+    for (int i = 0; i < count; ++i)  // per row: 5 residuals, then (indirect) 10 x {CG steps, |q|}
+      if (want_hist && par[i].VERBOSE && !is_big[i]) ho[i] = P.zeros(sizeof(double) * (direct ? 5 : 25) * verbose_hist_rows(par[i].MAX_ITERS));
+    res_off = P.zeros(sizeof(DevResult) * count);
+  }
   queue_off = P.zeros(sizeof(int) * kClasses);
   inst_off = P.zeros(sizeof(WgInst) * count);
   cinst_off = P.zeros(sizeof(WgInst) * count);
@@ -619,14 +627,19 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
     std::memcpy(P.buf.data() + cinst_off + sizeof(WgInst) * k, &insts[order[k]], sizeof(WgInst));
   hipError_t e;
   if (borrow) {
+    // from the context's pinned buffer, which no other work of this thread
+    // touches before this call's fetch has synchronized the stream: the
+    // launch follows the copy on the same stream without a host wait
     std::memcpy(C.hbuf, P.buf.data(), bytes);
     e = hipMemcpyAsync(dbuf, C.hbuf, bytes, hipMemcpyHostToDevice, stream);
+    ev0 = C.ev0;
+    ev1 = C.ev1;
   } else {
     e = hipMemcpyAsync(dbuf, P.buf.data(), bytes, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e == hipSuccess && !ev0) e = hipEventCreate(&ev0);
+    if (e == hipSuccess && !ev1) e = hipEventCreate(&ev1);
   }
-  if (e == hipSuccess) e = hipStreamSynchronize(stream);
-  if (e == hipSuccess && !ev0) e = hipEventCreate(&ev0);
-  if (e == hipSuccess && !ev1) e = hipEventCreate(&ev1);
   if (e != hipSuccess) { err = std::string("upload: ") + hipGetErrorString(e); return false; }
   gsol.clear();
   for (int i : big) {  // own device memory: the batch may hold the thread's arena
@@ -981,6 +994,28 @@ bool WgBatch::fetch(std::vector<DevResult>& res, std::string& err) {
   unsigned char* base = static_cast<unsigned char*>(dbuf);
   const int count = (int)S.size();
   res.resize(count);
+  if (!own_mem && count == 1 && !is_big[0]) {
+    // one copy into the context's pinned buffer: x, s, y, the history and
+    // the result lie in that order in [xo, res_off + result)
+    Ctx& C = ctx();
+    const size_t lo = xo[0], hi = res_off + sizeof(DevResult);
+    hipError_t e = hi - lo <= C.hcap ? hipMemcpyAsync(C.hbuf, base + lo, hi - lo, hipMemcpyDeviceToHost, stream)
+                                     : hipErrorInvalidValue;
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) { err = std::string("download: ") + hipGetErrorString(e); return false; }
+    const unsigned char* h = static_cast<const unsigned char*>(C.hbuf) - lo;
+    Setup& s = S[0];
+    std::memcpy(s.x.data(), h + xo[0], sizeof(double) * s.n);
+    std::memcpy(s.s.data(), h + so[0], sizeof(double) * s.m);
+    std::memcpy(s.y.data(), h + yo[0], sizeof(double) * s.m);
+    std::memcpy(res.data(), h + res_off, sizeof(DevResult));
+    hist.assign(1, {});
+    if (insts[0].hist_rows) {
+      hist[0].resize((size_t)(direct ? 5 : 25) * insts[0].hist_rows);
+      std::memcpy(hist[0].data(), h + ho[0], sizeof(double) * hist[0].size());
+    }
+    return true;
+  }
   hipError_t e = hipMemcpyAsync(res.data(), base + res_off, sizeof(DevResult) * count,
                                 hipMemcpyDeviceToHost, stream);
   for (int i = 0; i < count && e == hipSuccess; ++i) {

@@ -24,6 +24,7 @@ struct Ctx {
   size_t hcap = 0;
   int max_lds = 65536;
   int num_cu = 256;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;  // timing events of calls that borrow the context
   bool ensure(int device, std::string& err);
   bool reserve(size_t dbytes, size_t hbytes, std::string& err);
 };

@@ -20,6 +20,8 @@
 //   relax / cones / dual pdos.c:232-249, 296-303; cones.c:5-57
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <algorithm>
 
 #include "device.hpp"
@@ -959,9 +961,17 @@ __global__ void k_zero_int(int* p) {
 template <typename T, bool DIRECT>
 hipError_t launch_t(const WgInst* d_insts, int count, int lds_bytes, int* queue, int grid, hipStream_t st) {
   auto fn = k_wg<T, DIRECT>;
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
-  if (e != hipSuccess) return e;
+  // the kernel's dynamic LDS limit only grows (per process and kernel)
+  static std::atomic<int> lds_set{-1};
+  hipError_t e = hipSuccess;
+  if (lds_bytes > lds_set.load()) {
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            lds_bytes);
+    if (e != hipSuccess) return e;
+    int cur = lds_set.load();
+    while (lds_bytes > cur && !lds_set.compare_exchange_weak(cur, lds_bytes)) {
+    }
+  }
   if (queue) {  // a kernel, not a memset node, so that the launch may be captured in a graph
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, queue);
     e = hipGetLastError();
