@@ -557,7 +557,8 @@ def run_wg(args, probs, config, plugin):
     h = pdos_amd.BatchHandle(probs, plugin=plugin, precision=prec, tune=bool(args.tune))
     prep = time.perf_counter() - t0
     tuning = h.tuning()
-    log(f"prepare (host setup + factor{' + launch-shape tuning' if args.tune else ''} + upload): {prep:.2f} s")
+    parts = h.prepare_times()
+    log(f"prepare: {prep:.2f} s (Python conversion {h.convert_s:.2f} s; library ms {parts})")
     for r in tuning:
         log(f"  tuned n+m in {r['n_plus_m']}: {r['count']} instances -> {r['variant']}, persistent "
             f"{r['persistent']}, {r['entries_per_lane']} entries per lane; candidates (variant, persistent, "
@@ -570,6 +571,7 @@ def run_wg(args, probs, config, plugin):
         kms.append(h.run())
     wall = (time.perf_counter() - t0) / args.steps
     outs = h.fetch()
+    dev_bytes = h.device_bytes()
     h.close()
     solved = sum(o["status"] == "Solved" for o in outs)
     iters = np.array([o["info"]["iters"] for o in outs])
@@ -577,7 +579,9 @@ def run_wg(args, probs, config, plugin):
     acc = {"solved": int(solved), "count": len(outs), "verified_fp64": verify_list(probs, outs),
            "mean_iters": float(iters.mean()), "max_iters": int(iters.max())}
     extra = {"plain_gpu": plain_gpu(probs[:args.plain_sample], plugin, prec) if args.plain_sample else None,
-             "prepare_s": round(prep, 3), "host_threads": int(os.environ.get("PDOS_HOST_THREADS", os.cpu_count())),
+             "prepare_s": round(prep, 3), "prepare_parts_ms": parts, "python_convert_s": round(h.convert_s, 3),
+             "device_bytes": dev_bytes,
+             "host_threads": int(os.environ.get("PDOS_HOST_THREADS", os.cpu_count())),
              "tuning": [{k: r[k] for k in ("n_plus_m", "count", "variant", "persistent", "entries_per_lane")}
                         for r in tuning]}
     return acc["verified_fp64"] / wall, wall, "f64" if prec == "fp64" else "f32", config, roof, acc, extra, \

@@ -111,6 +111,11 @@ size_t pdos_batch_device_bytes(const PdosBatchHandle *h);
  * and entries per lane, and every candidate's [variant, persistent,
  * entries_per_lane, kernel ms]); "" without tuning.  Owned by the handle. */
 const char *pdos_batch_tuning(const PdosBatchHandle *h);
+/* Host wall time (ms) of pdos_batch_prepare's parts: ms[0] setup (scaling and
+ * transpose, host threads), ms[1] grouping by matrix and the direct plugin's
+ * factorizations, ms[2] launch-shape tuning (0 without opt->tune), ms[3]
+ * packing and upload.  Returns 0, or -1 for NULL arguments. */
+int pdos_batch_prepare_times(const PdosBatchHandle *h, double *ms);
 
 /* Warm start for every instance of a prepared batch from device memory:
  * instance i reads [x (n), y (m), s (m)] in original units (float) at

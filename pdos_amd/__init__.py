@@ -15,6 +15,7 @@ from __future__ import annotations
 import ctypes as C
 import json
 import os
+import time
 
 import numpy as np
 
@@ -49,7 +50,7 @@ class PdosInfo(C.Structure):
 # Every symbol include/pdos.h and include/pdos_ext.h declare.
 EXPORTS = ["pdos", "freeData", "freeSol", "tic", "toc", "tocq", "pdos_default_options",
            "pdos_ex", "pdos_last_info", "pdos_batch", "pdos_batch_prepare", "pdos_batch_run",
-           "pdos_batch_fetch", "pdos_batch_free", "pdos_batch_device_bytes", "pdos_batch_tuning",
+           "pdos_batch_fetch", "pdos_batch_free", "pdos_batch_device_bytes", "pdos_batch_tuning", "pdos_batch_prepare_times",
            "pdos_batch_set_warm_dev", "pdos_device_alloc",
            "pdos_device_free", "pdos_device_upload", "pdos_device_download", "pdos_device_copy",
            "pdos_mlp_create",
@@ -94,6 +95,8 @@ def lib(plugin: str = "indirect"):
         L.pdos_batch_device_bytes.argtypes = [C.c_void_p]
         L.pdos_batch_tuning.restype = C.c_char_p
         L.pdos_batch_tuning.argtypes = [C.c_void_p]
+        L.pdos_batch_prepare_times.restype = C.c_int
+        L.pdos_batch_prepare_times.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         L.pdos_setup_host.restype = C.c_int
         L.pdos_setup_host.argtypes = [C.POINTER(Data), C.POINTER(Cone), C.c_int, C.POINTER(C.c_double),
                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -236,6 +239,7 @@ class BatchHandle:
     """A prepared batch resident on the device (pdos_batch_prepare/run/fetch)."""
 
     def __init__(self, problems, plugin="indirect", precision="fp64", tune=False):
+        t0 = time.perf_counter()  # the Python-side conversion is timed apart from the library's prepare
         self.L = lib(plugin)
         self.n = len(problems)
         keeps = [p.to_c() for p in problems]
@@ -243,6 +247,7 @@ class BatchHandle:
         KA = (C.POINTER(Cone) * self.n)(*[C.pointer(kk[1]) for kk in keeps])
         o = options(plugin, precision, "wg")
         o.tune = int(tune)
+        self.convert_s = time.perf_counter() - t0
         self.h = self.L.pdos_batch_prepare(DA, KA, self.n, C.byref(o))
         if not self.h:
             raise RuntimeError(f"pdos_batch_prepare failed: {self.L.pdos_last_error().decode()}")
@@ -273,6 +278,13 @@ class BatchHandle:
 
     def device_bytes(self) -> int:
         return int(self.L.pdos_batch_device_bytes(self.h))
+
+    def prepare_times(self) -> dict:
+        """Host ms of the library's prepare: setup, grouping and factorization,
+        tuning, packing and upload."""
+        v = (C.c_double * 4)()
+        self.L.pdos_batch_prepare_times(self.h, v)
+        return dict(zip(("setup", "group_factor", "tune", "pack_upload"), (round(x, 2) for x in v)))
 
     def tuning(self) -> list:
         """The tuner's decisions per size bucket ([] without tuning)."""

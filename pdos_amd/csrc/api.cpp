@@ -17,6 +17,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/pdos.h"
@@ -153,9 +154,18 @@ WgBatch::~WgBatch() {
   }
 }
 
+// Hash of a byte range, 8 bytes per step (multiply-xorshift mixing); only
+// grouping uses it, and a hash match is confirmed by comparing the data.
 static uint64_t fnv(uint64_t h, const void* p, size_t bytes) {
   const unsigned char* c = static_cast<const unsigned char*>(p);
-  for (size_t i = 0; i < bytes; ++i) h = (h ^ c[i]) * 1099511628211ull;
+  size_t i = 0;
+  for (; i + 8 <= bytes; i += 8) {
+    uint64_t w;
+    std::memcpy(&w, c + i, 8);
+    h = (h ^ w) * 0x9e3779b97f4a7c15ull;
+    h ^= h >> 29;
+  }
+  for (; i < bytes; ++i) h = (h ^ c[i]) * 1099511628211ull;
   return h;
 }
 
@@ -205,23 +215,34 @@ bool WgBatch::group_and_factor(std::string& err) {
   const int count = (int)S.size();
   grp.assign(count, -1);
   rep.clear();
-  std::vector<std::pair<uint64_t, int>> keys;  // (hash, group)
+  std::unordered_multimap<uint64_t, int> keys;  // hash -> group
+  std::vector<uint64_t> hs(count);
+  {
+    std::string herr;
+    parallel_for(count, [&](int i, std::string&) {
+      const Setup& s = S[i];
+      uint64_t h = 1469598103934665603ull;
+      const int dims[4] = {s.n, s.m, s.f, s.l};
+      h = fnv(h, dims, sizeof(dims));
+      h = fnv_vec(h, s.soc_ptr);
+      h = fnv_vec(h, s.cp);
+      h = fnv_vec(h, s.ci);
+      h = fnv_vec(h, s.cx);
+      h = fnv_vec(h, s.D);
+      h = fnv_vec(h, s.E);
+      hs[i] = h;
+      return true;
+    }, herr);
+  }
   for (int i = 0; i < count; ++i) {
     const Setup& s = S[i];
-    uint64_t h = 1469598103934665603ull;
-    const int dims[4] = {s.n, s.m, s.f, s.l};
-    h = fnv(h, dims, sizeof(dims));
-    h = fnv_vec(h, s.soc_ptr);
-    h = fnv_vec(h, s.cp);
-    h = fnv_vec(h, s.ci);
-    h = fnv_vec(h, s.cx);
-    h = fnv_vec(h, s.D);
-    h = fnv_vec(h, s.E);
-    for (const auto& kv : keys)
-      if (kv.first == h && same_matrix(S[rep[kv.second]], s)) { grp[i] = kv.second; break; }
+    const uint64_t h = hs[i];
+    auto range = keys.equal_range(h);
+    for (auto it = range.first; it != range.second; ++it)
+      if (same_matrix(S[rep[it->second]], s)) { grp[i] = it->second; break; }
     if (grp[i] < 0) {
       grp[i] = (int)rep.size();
-      keys.emplace_back(h, grp[i]);
+      keys.emplace(h, grp[i]);
       rep.push_back(i);
     }
   }
@@ -440,7 +461,7 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
   struct Off { size_t soc, rp, ri, rx, cp, ci, cx, rb, cb, b, c, D, E, x0, s0, y0;
                int nrb, ncb;
                size_t perm, lrp, lri, lrx, lcp, lci, lcx, ld, flp, flr, blp, blr;
-               size_t img; int res; ResLayout lay; };
+               size_t img; int res; ResLayout lay; bool stream; };
   std::vector<Off> pk(count);
   xo.resize(count); so.resize(count); yo.resize(count);
   ho.assign(count, 0);
@@ -464,17 +485,26 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
     }
   // LDS images of the groups (on the host threads), unless the streaming
   // variant is forced
+  // per group: a member forced to the streaming variant (it needs the
+  // streaming copies of the matrices); every member forced to it (no image)
+  const int ngrp = (int)rep.size();
+  std::vector<char> grp_stream(ngrp, variant_env == 0 ? 1 : 0), grp_all_stream(ngrp, 1);
+  for (int i = 0; i < count; ++i) {
+    const int vf = variant_env >= 0 ? variant_env : (var_force.empty() ? -1 : var_force[i]);
+    if (vf == 0 || pack_all) grp_stream[grp[i]] = 1;
+    if (vf != 0) grp_all_stream[grp[i]] = 0;
+  }
   std::vector<std::vector<unsigned char>> imgs(count);
   std::vector<ResLayout> lays(count);
-  if (variant_env != 0) {
+  {
     std::string ierr;
     parallel_for(count, [&](int i, std::string&) {
-      if (!is_big[i] && rep[grp[i]] == i)
+      if (!is_big[i] && rep[grp[i]] == i && !grp_all_stream[grp[i]])
         res_image(S[i], direct ? &F[grp[i]] : nullptr, fp64, epl.empty() ? 4 : epl[i], imgs[i], lays[i]);
       return true;
     }, ierr);
   }
-  vlds.assign(count, VarLds{0, 0, 0});
+  vlds.assign(count, VarLds{0, 0, 0, false});
   for (int i = 0; i < count; ++i) {
     const Setup& s = S[i];
     Off& k = pk[i];
@@ -483,20 +513,6 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
       const Off& r = pk[rep[grp[i]]];
       k = r;
     } else {
-      k.soc = put_vec(P, s.soc_ptr);
-      k.rp = put_vec(P, s.rp);
-      k.ri = put_vec(P, s.ri);
-      k.rx = put_vals(P, s.rx, fp64);
-      k.cp = put_vec(P, s.cp);
-      k.ci = put_vec(P, s.ci);
-      k.cx = put_vals(P, s.cx, fp64);
-      const std::vector<int> rb = wg_row_blocks(s.rp), cb = wg_row_blocks(s.cp);
-      k.rb = put_vec(P, rb);
-      k.cb = put_vec(P, cb);
-      k.nrb = (int)rb.size() / 4;
-      k.ncb = (int)cb.size() / 4;
-      k.D = put_vec(P, s.D);
-      k.E = put_vec(P, s.E);
       // the image, kept when at least the variant with the image in device
       // memory fits (the variant is chosen per record afterwards)
       k.res = 0;
@@ -506,8 +522,29 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
         k.img = P.put(imgs[i].data(), imgs[i].size());
         std::vector<unsigned char>().swap(imgs[i]);
       }
+      // the streaming variant's copies of the matrices (and factor): only
+      // where a record may use it (forced, or no image fits; with an image
+      // the automatic choice never streams)
+      k.stream = k.res == 0 || grp_stream[grp[i]];
+      k.soc = put_vec(P, s.soc_ptr);
+      k.D = put_vec(P, s.D);
+      k.E = put_vec(P, s.E);
+      if (k.stream) {
+        k.rp = put_vec(P, s.rp);
+        k.ri = put_vec(P, s.ri);
+        k.rx = put_vals(P, s.rx, fp64);
+        k.cp = put_vec(P, s.cp);
+        k.ci = put_vec(P, s.ci);
+        k.cx = put_vals(P, s.cx, fp64);
+        const std::vector<int> rb = wg_row_blocks(s.rp), cb = wg_row_blocks(s.cp);
+        k.rb = put_vec(P, rb);
+        k.cb = put_vec(P, cb);
+        k.nrb = (int)rb.size() / 4;
+        k.ncb = (int)cb.size() / 4;
+      }
     }
     vlds[i].x = wg_lds_bytes(s.n, s.m, s.n + s.m, direct, tsz);
+    vlds[i].s = k.stream;
     if (k.res) {
       vlds[i].y = k.lay.total <= lds_cap ? k.lay.total : 0;
       vlds[i].z = k.lay.total - k.lay.img;
@@ -524,7 +561,7 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
       if (want_hist && par[i].VERBOSE) ho[i] = P.zeros(sizeof(double) * (direct ? 5 : 25) * verbose_hist_rows(par[i].MAX_ITERS));
       res_off = P.zeros(sizeof(DevResult));
     }
-    if (direct && rep[grp[i]] == i) {
+    if (direct && rep[grp[i]] == i && k.stream) {
       const Factor& f = F[grp[i]];
       k.perm = put_vec(P, f.P);
       k.lrp = put_vec(P, f.Rp);
@@ -577,14 +614,16 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
     w.pscale = s.pscale;
     w.dscale = s.dscale;
     w.soc_ptr = reinterpret_cast<const int*>(base + k.soc);
-    w.rp = reinterpret_cast<const int*>(base + k.rp);
-    w.ri = reinterpret_cast<const int*>(base + k.ri);
-    w.rx = base + k.rx;
-    w.cp = reinterpret_cast<const int*>(base + k.cp);
-    w.ci = reinterpret_cast<const int*>(base + k.ci);
-    w.cx = base + k.cx;
-    w.rblk = reinterpret_cast<const int*>(base + k.rb);
-    w.cblk = reinterpret_cast<const int*>(base + k.cb);
+    if (k.stream) {  // else null: only the image variants run this record
+      w.rp = reinterpret_cast<const int*>(base + k.rp);
+      w.ri = reinterpret_cast<const int*>(base + k.ri);
+      w.rx = base + k.rx;
+      w.cp = reinterpret_cast<const int*>(base + k.cp);
+      w.ci = reinterpret_cast<const int*>(base + k.ci);
+      w.cx = base + k.cx;
+      w.rblk = reinterpret_cast<const int*>(base + k.rb);
+      w.cblk = reinterpret_cast<const int*>(base + k.cb);
+    }
     w.b = reinterpret_cast<const double*>(base + k.b);
     w.c = reinterpret_cast<const double*>(base + k.c);
     w.D = reinterpret_cast<const double*>(base + k.D);
@@ -595,7 +634,7 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
     w.x = reinterpret_cast<double*>(base + xo[i]);
     w.s = reinterpret_cast<double*>(base + so[i]);
     w.y = reinterpret_cast<double*>(base + yo[i]);
-    if (direct) {
+    if (direct && k.stream) {
       const Factor& f = F[grp[i]];
       w.perm = reinterpret_cast<const int*>(base + k.perm);
       w.lrp = reinterpret_cast<const int*>(base + k.lrp);
@@ -674,7 +713,8 @@ int wg_variant_env() {
 }
 
 // Kernel variant and launch mode of every record, then the launch classes.
-// Variant: forced (environment, else the tuner's choice) where it fits;
+// Variant: forced (environment, else the tuner's choice) where it fits (a
+// resident choice that does not fit takes the image in device memory);
 // automatic: the resident variant when the image and iterates fit in LDS
 // without fewer workgroups per CU than the streaming variant (or the launch
 // has at most one workgroup per CU anyway), else the image in device memory
@@ -693,9 +733,11 @@ void WgBatch::set_shapes() {
     const VarLds& L = vlds[i];
     const int vf = variant_env >= 0 ? variant_env : (var_force.empty() ? -1 : var_force[i]);
     int v = 0;
-    if (vf == 1) v = L.y ? 1 : 0;
+    if (vf == 1) v = L.y ? 1 : (L.z ? 2 : 0);
     else if (vf == 2) v = L.z ? 2 : 0;
     else if (vf < 0) v = L.y && (nwg_total <= ctx().num_cu || wpc(L.y) >= wpc(L.x)) ? 1 : (L.z ? 2 : 0);
+    else v = L.s ? 0 : (L.y ? 1 : 2);  // forced streaming where its copies were packed
+    // (v == 0 only with L.s: without an image the copies are always packed)
     w.resident = v;
     w.lds_bytes = v == 1 ? L.y : (v == 2 ? L.z : L.x);
   }
@@ -758,8 +800,8 @@ void WgBatch::set_shapes() {
 // workgroups that take instances from a work queue) and, for the image
 // variants, the entries per lane that the wave batches aim for (4, 8).
 // Each is timed (HIP events) on a sample of the bucket's instances, every
-// one running a fixed kTuneIters iterations (EPS 0, so no early stop): twice
-// as many instances as CUs (repeating instances when the bucket is smaller)
+// one running a fixed kTuneIters iterations (EPS 0, so no early stop): as
+// many instances as CUs (repeating instances when the bucket is smaller)
 // when the whole batch holds more instances than there are CUs, else the
 // bucket itself.  Decisions are cached per process by (device, precision,
 // plugin, load, bucket).
@@ -806,11 +848,12 @@ bool WgBatch::tune(int dev, bool fp64_, std::string& err) {
     }
     int ns = (int)mem.size();
     if (!cached) {
-      if (load) ns = 2 * C.num_cu;
+      if (load) ns = C.num_cu;
       // the sample as a batch of its own, sharing matrix groups like this one
       auto make = [&](int e, std::unique_ptr<WgBatch>& T) {
         T.reset(new WgBatch);
         T->direct = direct;
+        T->pack_all = true;  // every candidate variant runs on the same records
         T->S.resize(ns);
         T->par.resize(ns);
         T->grp.resize(ns);
@@ -1254,6 +1297,9 @@ Sol* pdos(const Data* d, const Cone* k) {
 struct PdosBatchHandle {
   pdos_amd::WgBatch B;
   double setup_ms = 0;
+  // host wall time of prepare's parts: setup (scaling, transpose), grouping
+  // and factorization, launch-shape tuning, packing and upload
+  double part_ms[4] = {0, 0, 0, 0};
 };
 
 static PdosBatchHandle* batch_prepare(const Data* const* ds, const Cone* const* ks, int count,
@@ -1273,11 +1319,18 @@ static PdosBatchHandle* batch_prepare(const Data* const* ds, const Cone* const* 
   h->B.par.resize(count);
   h->B.direct = direct;
   std::string err;
+  double tp = now_ms();
   const int bad = parallel_for(count, [&](int i, std::string& e) {
     if (!ds[i] || !ks[i] || !build_setup(ds[i], ks[i], h->B.S[i], e, o.equil)) return false;
     h->B.par[i] = *ds[i]->p;
     return true;
   }, err);
+  auto lap = [&](int k) {
+    const double t = now_ms();
+    h->part_ms[k] = t - tp;
+    tp = t;
+  };
+  lap(0);
   if (bad >= 0) {
     g_err = "instance " + std::to_string(bad) + ": " + (err.empty() ? "NULL input" : err);
     delete h;
@@ -1288,16 +1341,19 @@ static PdosBatchHandle* batch_prepare(const Data* const* ds, const Cone* const* 
     delete h;
     return nullptr;
   }
+  lap(1);
   if (o.tune && !h->B.tune(o.device, o.precision == PDOS_FP64, err)) {
     g_err = err;
     delete h;
     return nullptr;
   }
+  lap(2);
   if (!h->B.prepare(o.device, o.precision == PDOS_FP64, direct, borrow, err)) {
     g_err = err;
     delete h;
     return nullptr;
   }
+  lap(3);
   h->setup_ms = now_ms() - t0;
   return h;
 }
@@ -1347,6 +1403,12 @@ void pdos_batch_free(PdosBatchHandle* h) { delete h; }
 size_t pdos_batch_device_bytes(const PdosBatchHandle* h) { return h ? h->B.bytes : 0; }
 
 const char* pdos_batch_tuning(const PdosBatchHandle* h) { return h ? h->B.tuning.c_str() : ""; }
+
+int pdos_batch_prepare_times(const PdosBatchHandle* h, double* ms) {
+  if (!h || !ms) return -1;
+  for (int k = 0; k < 4; ++k) ms[k] = h->part_ms[k];
+  return 0;
+}
 
 int pdos_batch_set_warm_dev(PdosBatchHandle* h, const float* dW, long ld, int check) {
   if (!h) return -1;

@@ -99,10 +99,12 @@ struct WgBatch {
   std::vector<std::unique_ptr<GridSolver>> gsol;
   std::vector<char> is_big;
   // per instance: LDS bytes of the streaming (x), resident (y) and
-  // image-in-device-memory (z) variants, 0 where the variant is unavailable
-  struct VarLds { int x, y, z; };
+  // image-in-device-memory (z) variants, 0 where the image variant is
+  // unavailable; s: the streaming variant's matrix copies were packed
+  struct VarLds { int x, y, z; bool s; };
   std::vector<VarLds> vlds;
   int variant_env = -1, persist_env = -1;
+  bool pack_all = false;  // pack the streaming copies even where an image fits (the tuner's samples)
   // Launch classes: instances grouped by how many of their workgroups fit
   // in a CU's LDS, each class launched with its own LDS size on its own
   // stream; the device holds a second copy of the records in class order.
