@@ -261,11 +261,14 @@ def family_roofline(kt, fam_info, nnz, fp64, lin_rows, b_inst, traffic_profile=N
     cone_rows = m - lin_rows
     rhs = n * sv if fp64 else n * 2 * nprod
     if dense and kind in ("At_product", "A_product"):
-        # dense A: GEMMs.  A' products: two fp32 (fp64) GEMMs of 2 m n flops per
-        # column; the A product in mixed mode: one bf16 GEMM over split operands,
-        # three products of 2 m n flops per column
+        # dense A: GEMMs.  A' products: two GEMMs (Y and W) of 2 m n flops per
+        # column; the A product one.  fp64: rocBLAS dgemm against the fp64
+        # peak; mixed: the hand-written kernels run each product as three bf16
+        # products over split operands (hi hi + hi lo + lo hi), the flops
+        # executed on the bf16 MFMA
         if kind == "At_product":
-            flops, peak = 2 * 2.0 * m * n * cols, FP64_PEAK_TFLOPS if fp64 else FP32_PEAK_TFLOPS
+            flops, peak = (2 * 2.0 * m * n * cols, FP64_PEAK_TFLOPS) if fp64 else \
+                (3 * 2 * 2.0 * m * n * cols, BF16_PEAK_TFLOPS)
         else:
             flops, peak = (2.0 * m * n * cols, FP64_PEAK_TFLOPS) if fp64 else (3 * 2.0 * m * n * cols, BF16_PEAK_TFLOPS)
         ach = flops / (ms * 1e-3) / 1e12
@@ -314,7 +317,7 @@ def family_roofline(kt, fam_info, nnz, fp64, lin_rows, b_inst, traffic_profile=N
 
 
 def wg_roofline(probs, outs, plugin, sv, kernel_ms):
-    """SpMV bytes one k_wg_solve launch executed (B_A / B_At per SpMV, s_i = 4)
+    """SpMV bytes one k_wg launch executed (B_A / B_At per SpMV, s_i = 4)
     plus the direct plugin's two triangular sweeps per iteration."""
     si, tot = 4, 0.0
     for p, o in zip(probs, outs):
@@ -330,7 +333,7 @@ def wg_roofline(probs, outs, plugin, sv, kernel_ms):
             tot += (1 + it) * BA + (it + solved) * BAt + it * 2 * (lnz * (8 + si) + (N + 1) * si + 2 * N * 8)
     ach = tot / (kernel_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_wg_solve",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_wg",
             "algorithmic_bytes_per_launch": tot, "avg_launch_us": round(kernel_ms * 1e3, 1)}
 
 

@@ -800,13 +800,14 @@ void WgBatch::set_shapes() {
 // workgroups that take instances from a work queue) and, for the image
 // variants, the entries per lane that the wave batches aim for (4, 8).
 // Each is timed (HIP events) on a sample of the bucket's instances, every
-// one running a fixed kTuneIters iterations (EPS 0, so no early stop): as
-// many instances as CUs (repeating instances when the bucket is smaller)
-// when the whole batch holds more instances than there are CUs, else the
-// bucket itself.  Decisions are cached per process by (device, precision,
+// one running a fixed kTuneIters iterations (EPS 0, so no early stop): when
+// the whole batch holds more instances than there are CUs, enough records to
+// fill the GPU at the highest occupancy (replicas of up to kTuneDistinct
+// instances), else the bucket itself.  Decisions are cached per process by (device, precision,
 // plugin, load, bucket).
 namespace {
 constexpr int kTuneIters = 50;
+constexpr int kTuneDistinct = 64;
 constexpr double kPersistMargin = 0.02;
 struct TuneChoice { signed char v, p, e; };
 std::mutex g_tune_mu;
@@ -848,7 +849,12 @@ bool WgBatch::tune(int dev, bool fp64_, std::string& err) {
     }
     int ns = (int)mem.size();
     if (!cached) {
-      if (load) ns = C.num_cu;
+      // a loaded GPU: as many records as the most workgroups per CU any
+      // variant allows times the CUs, so every candidate runs at its full
+      // occupancy; drawn from at most kTuneDistinct instances of the bucket
+      // (replicas share their instance's packed matrices)
+      const int nd = load ? std::min<int>((int)mem.size(), kTuneDistinct) : (int)mem.size();
+      if (load) ns = kOccMax * C.num_cu;
       // the sample as a batch of its own, sharing matrix groups like this one
       auto make = [&](int e, std::unique_ptr<WgBatch>& T) {
         T.reset(new WgBatch);
@@ -860,7 +866,7 @@ bool WgBatch::tune(int dev, bool fp64_, std::string& err) {
         T->epl.assign(ns, (signed char)e);
         std::map<int, int> gmap;
         for (int k = 0; k < ns; ++k) {
-          const int i = mem[(size_t)k * mem.size() / ns];
+          const int i = mem[(size_t)(k % nd) * mem.size() / nd];
           auto g = gmap.find(grp[i]);
           if (g == gmap.end()) {
             g = gmap.emplace(grp[i], (int)T->rep.size()).first;
@@ -872,11 +878,13 @@ bool WgBatch::tune(int dev, bool fp64_, std::string& err) {
               T->S[k].rp = r.rp; T->S[k].ri = r.ri; T->S[k].rx = r.rx;
             }
             if (direct) T->F.push_back(F[grp[i]]);
-          } else {
-            T->S[k] = S[i];  // the matrix arrays live with the group's first sample
-            std::vector<int>().swap(T->S[k].cp); std::vector<int>().swap(T->S[k].ci);
-            std::vector<double>().swap(T->S[k].cx); std::vector<int>().swap(T->S[k].rp);
-            std::vector<int>().swap(T->S[k].ri); std::vector<double>().swap(T->S[k].rx);
+          } else {  // the matrix arrays live with the group's first sample
+            const Setup& a = S[i];
+            Setup& t = T->S[k];
+            t.n = a.n; t.m = a.m; t.f = a.f; t.l = a.l;
+            t.soc_ptr = a.soc_ptr; t.b = a.b; t.c = a.c; t.D = a.D; t.E = a.E; t.E0 = a.E0;
+            t.lambda = a.lambda; t.pscale = a.pscale; t.dscale = a.dscale;
+            t.x = a.x; t.s = a.s; t.y = a.y;
           }
           T->grp[k] = g->second;
           Params q = par[i];

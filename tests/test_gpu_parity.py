@@ -293,8 +293,8 @@ def test_batch_verbose_keeps_no_history():
 @pytest.mark.parametrize("plugin", ["indirect", "direct"])
 def test_wg_streaming_and_resident_variants(monkeypatch, variant, plugin):
     """The workgroup engine's three variants -- matrices streamed from global
-    memory (k_wg_solve), the wave-batched kernel with its image in LDS
-    (k_wg_res, the default where it fits) and the same with the image in
+    memory (the streaming body), the wave-batched body with its image in LDS
+    (the default where it fits) and the same with the image in
     device memory -- each against the reference on cases covering long rows,
     jumbled columns, empty rows and warm starts (PDOS_WG_VARIANT forces one)."""
     monkeypatch.setenv("PDOS_WG_VARIANT", variant)
