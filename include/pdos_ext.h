@@ -166,6 +166,10 @@ int pdos_family_timing(PdosFamily *h, int enable, double *ms, long *calls, doubl
 /* Engines the next pdos_family_load splits a large mixed-precision batch
  * over (1 or 2; n <= 0 only queries); returns the setting. */
 int pdos_family_engines(PdosFamily *h, int n);
+/* Per loaded instance (the order of pdos_family_load): lambda, and the stop
+ * test's scales 1 + |b| and 1 + |c| (pdos.c:76-77), as the device computed
+ * them at load (common.h:182-205 on the instance's b and c); any may be NULL. */
+int pdos_family_scales(PdosFamily *h, double *lam, double *pscale, double *dscale);
 /* out[9]: n, m, padded n of the GEMM, compactions in the last run, dense A,
  * fp64, bf16 products of the mixed x-update, dense rows of A multiplied as
  * MFMA tiles (0: none), engines the loaded batch is split over (1 or 2). */

@@ -58,7 +58,7 @@ EXPORTS = ["pdos", "freeData", "freeSol", "tic", "toc", "tocq", "pdos_default_op
            "pdos_mlp_set_affine", "pdos_mlp_loss_grad", "pdos_mlp_train", "pdos_mlp_predict",
            "pdos_mlp_predict_dev", "pdos_mlp_predict_dev_noalloc", "pdos_mlp_last_error", "pdos_family_create", "pdos_family_load",
            "pdos_family_run", "pdos_family_fetch", "pdos_family_free", "pdos_family_last_error",
-           "pdos_family_timing", "pdos_family_info", "pdos_family_engines",
+           "pdos_family_timing", "pdos_family_info", "pdos_family_engines", "pdos_family_scales",
            "pdos_kkt_order", "pdos_kkt_solve_host", "pdos_setup_host", "pdos_last_error",
            "pdos_version"]
 
@@ -318,6 +318,14 @@ class DeviceArray:
             raise RuntimeError("device allocation failed")
         if arr is not None:
             self.L.pdos_device_upload(self.ptr, arr.ctypes.data, self.nbytes)
+
+    def upload(self, arr) -> None:
+        """Copies a host array of this array's size into it."""
+        arr = np.ascontiguousarray(arr, dtype=np.float32)
+        if arr.nbytes != self.nbytes:
+            raise ValueError(f"size mismatch: {arr.nbytes} bytes into {self.nbytes}")
+        if self.L.pdos_device_upload(self.ptr, arr.ctypes.data, self.nbytes) != 0:
+            raise RuntimeError("device upload failed")
 
     def numpy(self):
         out = np.empty(self.shape, dtype=np.float32)

@@ -1419,12 +1419,67 @@ __global__ __launch_bounds__(256) void k_fam_xupdate(int mt, int nt, int ksteps,
 }
 
 
+// Unscaled outputs, instance-major binary64 (pdos.c:251-294): out[inst][r] =
+// Z[r][inst] * scale[r] (x = E x, y = D y) or / scale[r] (s = s / D), one
+// rounding each, as the host's unscale.
 template <typename T>
-__global__ void k_gather_out(int rows, int count, int ldb, const T* __restrict__ Z, double* __restrict__ out) {
+__global__ void k_gather_out(int rows, int count, int ldb, const T* __restrict__ Z, const double* __restrict__ scale,
+                             int divide, double* __restrict__ out) {
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < (long)rows * count; e += (long)gridDim.x * blockDim.x) {
     const int inst = (int)(e / rows), r = (int)(e % rows);
-    out[e] = double(Z[(size_t)r * ldb + inst]);
+    const double z = double(Z[(size_t)r * ldb + inst]);
+    out[e] = divide ? z / scale[r] : scale[r] * z;
   }
+}
+
+// Per-instance setup of a loaded batch on the device (common.h:69-215 for
+// the family's per-instance data): thread t scales instance t's c by E0 (the
+// column scaling before the x1e3) and writes it feature-major with the x1e3,
+// the same for b by D when b is per instance, and forms the stop-test scales
+// 1 + |b|, 1 + |c| and lambda = (1e-15 + |b~|) / (1e-15 + |c~|).  Each norm is
+// a sequential sum in index order with separately rounded products and sums
+// (contraction off: no fused multiply-add), the host's arithmetic bit for
+// bit.  With b shared, ps_sh is its 1 + |b| and nbs_sh its |b~|^2.
+// Padding columns get c = b = 0 and scales 1.
+template <typename T>
+__global__ void k_fam_scale_in(int cnt, int n, int m, int ldb, const double* __restrict__ C,
+                               const double* __restrict__ B, const double* __restrict__ E0,
+                               const double* __restrict__ D, double ps_sh, double nbs_sh, T* __restrict__ cf,
+                               T* __restrict__ bf, double* __restrict__ lam, double* __restrict__ ps,
+                               double* __restrict__ ds) {
+#pragma clang fp contract(off)
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ldb) return;
+  if (t >= cnt) {
+    for (int i = 0; i < n; ++i) cf[(size_t)i * ldb + t] = T(0);
+    if (bf)
+      for (int j = 0; j < m; ++j) bf[(size_t)j * ldb + t] = T(0);
+    lam[t] = ps[t] = ds[t] = 1.0;
+    return;
+  }
+  double nc = 0.0, ncs = 0.0, nb = 0.0, nbs = nbs_sh;
+  const double* c = C + (size_t)t * n;
+  for (int i = 0; i < n; ++i) {
+    const double ci = c[i];
+    nc += ci * ci;
+    const double cs = ci * E0[i];
+    ncs += cs * cs;
+    cf[(size_t)i * ldb + t] = T(cs * 1e3);
+  }
+  if (B) {
+    nbs = 0.0;
+    const double* b = B + (size_t)t * m;
+    for (int j = 0; j < m; ++j) {
+      const double bj = b[j];
+      nb += bj * bj;
+      const double bs = bj * D[j];
+      nbs += bs * bs;
+      bf[(size_t)j * ldb + t] = T(bs);
+    }
+  }
+  ps[t] = B ? 1.0 + sqrt(nb) : ps_sh;
+  ds[t] = 1.0 + sqrt(nc);
+  lam[t] = (1e-15 + sqrt(nbs)) / (1e-15 + sqrt(ncs));
 }
 
 // Column moves (compaction, output scatter): a lane copies one column's
@@ -1541,7 +1596,10 @@ struct Family {
   int n = 0, m = 0, np = 0;
   // shared device data
   void *rp = nullptr, *ri = nullptr, *rx = nullptr, *cp = nullptr, *ci = nullptr, *cx = nullptr;
-  void *soc = nullptr, *D = nullptr, *E = nullptr, *bshared = nullptr;
+  void *soc = nullptr, *D = nullptr, *E = nullptr, *E0 = nullptr, *bshared = nullptr;
+  // the loaded batch's raw inputs (instance-major binary64) and the fetch's
+  // instance-major outputs, kept with the batch
+  void *inC = nullptr, *inB = nullptr, *gout = nullptr;
   void* Ad = nullptr;      // dense scaled A (m x n row-major, bulk type) when A is dense
   // mixed precision, dense A: both products run as the hand-written split
   // bf16 MFMA kernels (dense_gemm.hip; A = A_hi + A_lo, three products with
@@ -1685,6 +1743,7 @@ struct Family {
 
   bool create(const Data* d, const Cone* k, const PdosOptions& o, std::string& err);
   bool load(int cnt, const double* C, const double* B, std::string& err);
+  bool alloc_batch(std::string& err);
   bool run(const float* warm, long wld, int judge, std::string& err);
   bool fetch(double* X, double* Y, double* Sx, long* iters, long* status, long* fallback, double* resid,
              std::string& err);
@@ -1784,7 +1843,7 @@ bool Family::create(const Data* d, const Cone* k, const PdosOptions& o, std::str
   }
   rp = upload(S.rp, err); ri = upload(S.ri, err); rx = upload_vals(S.rx, err);
   cp = upload(S.cp, err); ci = upload(S.ci, err); cx = upload_vals(S.cx, err);
-  soc = upload(S.soc_ptr, err); D = upload(S.D, err); E = upload(S.E, err);
+  soc = upload(S.soc_ptr, err); D = upload(S.D, err); E = upload(S.E, err); E0 = upload(S.E0, err);
   bshared = upload_vals(S.b, err);
   if (!err.empty()) return false;
   for (int gi = 0; gi < 2; ++gi) {
@@ -1994,54 +2053,51 @@ bool Family::create(const Data* d, const Cone* k, const PdosOptions& o, std::str
 // scaled exactly as build_setup does (common.h:140-192) and uploaded
 // feature-major together with lambda, pscale, dscale.
 bool Family::load(int cnt, const double* C, const double* B, std::string& err) {
-  free_batch();
+  // a batch of the same padded width (and kind of b) keeps its device
+  // buffers: every run initializes its working sets from the loaded inputs
+  const bool reuse = hflags && count > 0 && pad64(cnt) == ldb && b_inst == (B != nullptr);
+  if (!reuse) free_batch();
   count = cnt;
   ldb = pad64(cnt);
   b_inst = B != nullptr;
   const size_t tsz = fp64 ? 8 : 4;
-  std::vector<double> lam(ldb, 1.0), ps(ldb, 1.0), ds(ldb, 1.0);
-  std::vector<double> cf((size_t)n * ldb, 0.0), bf(b_inst ? (size_t)m * ldb : 0, 0.0);
-  const double Esc = 1e3;
-  for (int t = 0; t < cnt; ++t) {
-    const double* c = C + (size_t)t * n;
-    const double* b = B ? B + (size_t)t * m : nullptr;
-    double nb = 0, nc = 0, nbs = 0, ncs = 0;
-    for (int j = 0; j < m; ++j) {
-      const double bj = b ? b[j] : 0.0;
-      nb += bj * bj;
-      const double bs = b ? bj * S.D[j] : S.b[j];
-      nbs += bs * bs;
-      if (b_inst) bf[(size_t)j * ldb + t] = bs;
-    }
-    if (!b) nb = (S.pscale - 1.0) * (S.pscale - 1.0);
-    for (int i = 0; i < n; ++i) {
-      nc += c[i] * c[i];
-      const double cs = c[i] * S.E0[i];  // E0: the column scaling before the x1e3
-      ncs += cs * cs;
-      cf[(size_t)i * ldb + t] = cs * Esc;
-    }
-    ps[t] = 1.0 + std::sqrt(nb);
-    ds[t] = 1.0 + std::sqrt(nc);
-    lam[t] = (1e-15 + std::sqrt(nbs)) / (1e-15 + std::sqrt(ncs));
-  }
-  auto up = [&](const std::vector<double>& v, bool as_t) -> void* {
-    if (!as_t || fp64) {
-      void* p = dalloc(v.size() * 8, batch_allocs, err);
-      if (p) (void)hipMemcpyAsync(p, v.data(), v.size() * 8, hipMemcpyHostToDevice, st);
-      (void)hipStreamSynchronize(st);
-      return p;
-    }
-    std::vector<float> f(v.begin(), v.end());
-    void* p = dalloc(f.size() * 4, batch_allocs, err);
-    if (p) (void)hipMemcpyAsync(p, f.data(), f.size() * 4, hipMemcpyHostToDevice, st);
-    (void)hipStreamSynchronize(st);
-    return p;
-  };
-  base_c = up(cf, true);
-  base_b = b_inst ? up(bf, true) : nullptr;
-  base_lam = up(lam, false);
-  base_ps = up(ps, false);
-  base_ds = up(ds, false);
+  if (!reuse && !alloc_batch(err)) return false;
+  // the raw inputs to the device; setup per instance there (k_fam_scale_in)
+  (void)hipMemcpyAsync(inC, C, (size_t)cnt * n * 8, hipMemcpyHostToDevice, st);
+  if (B) (void)hipMemcpyAsync(inB, B, (size_t)cnt * m * 8, hipMemcpyHostToDevice, st);
+  // shared b: its stop-test scale 1 + |b| (pdos.c:76) and |b~|^2 of the
+  // scaled b, summed in index order as per instance
+  double nbs_sh = 0.0;
+  for (int j = 0; j < m; ++j) nbs_sh += S.b[j] * S.b[j];
+  const dim3 grid((ldb + 255) / 256);
+  if (fp64)
+    hipLaunchKernelGGL(k_fam_scale_in<double>, grid, dim3(256), 0, st, cnt, n, m, ldb, static_cast<const double*>(inC),
+                       static_cast<const double*>(inB), static_cast<const double*>(E0),
+                       static_cast<const double*>(D), S.pscale, nbs_sh, static_cast<double*>(base_c),
+                       static_cast<double*>(base_b), static_cast<double*>(base_lam), static_cast<double*>(base_ps),
+                       static_cast<double*>(base_ds));
+  else
+    hipLaunchKernelGGL(k_fam_scale_in<float>, grid, dim3(256), 0, st, cnt, n, m, ldb, static_cast<const double*>(inC),
+                       static_cast<const double*>(inB), static_cast<const double*>(E0),
+                       static_cast<const double*>(D), S.pscale, nbs_sh, static_cast<float*>(base_c),
+                       static_cast<float*>(base_b), static_cast<double*>(base_lam), static_cast<double*>(base_ps),
+                       static_cast<double*>(base_ds));
+  if (hipGetLastError() != hipSuccess) { err = "load: scaling kernel failed"; return false; }
+  (void)tsz;
+  return hipStreamSynchronize(st) == hipSuccess;
+}
+
+// Device buffers of a batch of width ldb (count and b_inst set).
+bool Family::alloc_batch(std::string& err) {
+  const size_t tsz = fp64 ? 8 : 4;
+  inC = dalloc((size_t)ldb * n * 8, batch_allocs, err);
+  inB = b_inst ? dalloc((size_t)ldb * m * 8, batch_allocs, err) : nullptr;
+  gout = dalloc((size_t)ldb * (n + 2 * m) * 8, batch_allocs, err);
+  base_c = dalloc((size_t)n * ldb * tsz, batch_allocs, err);
+  base_b = b_inst ? dalloc((size_t)m * ldb * tsz, batch_allocs, err) : nullptr;
+  base_lam = dalloc((size_t)ldb * 8, batch_allocs, err);
+  base_ps = dalloc((size_t)ldb * 8, batch_allocs, err);
+  base_ds = dalloc((size_t)ldb * 8, batch_allocs, err);
   auto vec = [&](int rows) { return dalloc((size_t)rows * ldb * tsz, batch_allocs, err); };
   for (int k = 0; k < 2; ++k) {
     Set& q = set[k];
@@ -2441,40 +2497,35 @@ bool Family::fetch(double* X, double* Y, double* Sx, long* iters, long* status, 
   const int cnt = count;
   std::vector<int> ip((size_t)3 * ldb);
   std::vector<double> red((size_t)8 * ldb);
-  std::vector<double> hx((size_t)n * cnt), hy((size_t)m * cnt), hs((size_t)m * cnt);
-  auto gather = [&](const void* Zp, int rows, double* out) {
-    double* dout = static_cast<double*>(dalloc((size_t)rows * cnt * 8, batch_allocs, err));
-    if (!dout) return;
-    if (fp64) hipLaunchKernelGGL(k_gather_out<double>, dim3(1024), dim3(256), 0, st, rows, cnt, ldb,
-                                 static_cast<const double*>(Zp), dout);
-    else hipLaunchKernelGGL(k_gather_out<float>, dim3(1024), dim3(256), 0, st, rows, cnt, ldb,
-                            static_cast<const float*>(Zp), dout);
-    (void)hipMemcpyAsync(out, dout, (size_t)rows * cnt * 8, hipMemcpyDeviceToHost, st);
+  // unscaled on the device (x = E x, y = D y, s = s / D: pdos.c:251-294) into
+  // instance-major binary64, copied straight into the caller's arrays
+  double* g = static_cast<double*>(gout);
+  auto gather = [&](const void* Zp, int rows, const void* scale, int divide, double* dev, double* out) {
+    if (!out) return;
+    if (fp64)
+      hipLaunchKernelGGL(k_gather_out<double>, dim3(1024), dim3(256), 0, st, rows, cnt, ldb,
+                         static_cast<const double*>(Zp), static_cast<const double*>(scale), divide, dev);
+    else
+      hipLaunchKernelGGL(k_gather_out<float>, dim3(1024), dim3(256), 0, st, rows, cnt, ldb,
+                         static_cast<const float*>(Zp), static_cast<const double*>(scale), divide, dev);
+    (void)hipMemcpyAsync(out, dev, (size_t)rows * cnt * 8, hipMemcpyDeviceToHost, st);
   };
-  gather(Xo, n, hx.data());
-  gather(Yo, m, hy.data());
-  gather(So, m, hs.data());
+  gather(Xo, n, E, 0, g, X);
+  gather(Yo, m, D, 0, g + (size_t)n * ldb, Y);
+  gather(So, m, D, 1, g + (size_t)(n + m) * ldb, Sx);
   (void)hipMemcpyAsync(ip.data(), Io, (size_t)3 * ldb * 4, hipMemcpyDeviceToHost, st);
   (void)hipMemcpyAsync(red.data(), Ro, (size_t)8 * ldb * 8, hipMemcpyDeviceToHost, st);
-  if (hipStreamSynchronize(st) != hipSuccess || !err.empty()) {
-    if (err.empty()) err = "fetch failed";
+  if (hipStreamSynchronize(st) != hipSuccess || hipGetLastError() != hipSuccess) {
+    err = "fetch failed";
     return false;
   }
-  // unscale (pdos.c:251-294): x = E x, y = D y, s = s / D
   for (int t = 0; t < cnt; ++t) {
-    if (X)
-      for (int i = 0; i < n; ++i) X[(size_t)t * n + i] = S.E[i] * hx[(size_t)t * n + i];
-    if (Y)
-      for (int j = 0; j < m; ++j) Y[(size_t)t * m + j] = S.D[j] * hy[(size_t)t * m + j];
-    if (Sx)
-      for (int j = 0; j < m; ++j) Sx[(size_t)t * m + j] = hs[(size_t)t * m + j] / S.D[j];
     if (iters) iters[t] = ip[t];
     if (status) status[t] = ip[ldb + t];
     if (fallback) fallback[t] = ip[2 * ldb + t];
     if (resid)
       for (int q = 0; q < 5; ++q) resid[(size_t)t * 5 + q] = red[(size_t)(2 + q) * ldb + t];
   }
-  // the temporary gather buffers are released with the next load
   return true;
 }
 
@@ -2567,6 +2618,20 @@ int pdos_family_engines(PdosFamily* h, int n) {
   if (!h) return -1;
   if (n > 0) h->max_engines = std::min<int>(n, (int)h->F.size());
   return std::min<int>(h->max_engines, (int)h->F.size());
+}
+
+int pdos_family_scales(PdosFamily* h, double* lam, double* pscale, double* dscale) {
+  if (!h || h->count.empty()) { g_fam_err = "nothing loaded"; return -1; }
+  for (size_t p = 0; p < h->count.size(); ++p) {
+    pdos_amd::Family& F = *h->F[p];
+    const long f = h->first[p], c = h->count[p];
+    bool ok = true;
+    if (lam) ok = ok && hipMemcpy(lam + f, F.base_lam, c * 8, hipMemcpyDeviceToHost) == hipSuccess;
+    if (pscale) ok = ok && hipMemcpy(pscale + f, F.base_ps, c * 8, hipMemcpyDeviceToHost) == hipSuccess;
+    if (dscale) ok = ok && hipMemcpy(dscale + f, F.base_ds, c * 8, hipMemcpyDeviceToHost) == hipSuccess;
+    if (!ok) { g_fam_err = "hipMemcpy failed"; return -1; }
+  }
+  return 0;
 }
 
 int pdos_family_info(PdosFamily* h, long* out) {

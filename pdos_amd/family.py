@@ -31,6 +31,9 @@ def _bind(L):
     L.pdos_family_info.argtypes = [C.c_void_p, _lp]
     L.pdos_family_engines.restype = C.c_int
     L.pdos_family_engines.argtypes = [C.c_void_p, C.c_int]
+    L.pdos_family_scales.restype = C.c_int
+    L.pdos_family_scales.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double)]
     L._fam_bound = True
 
 
@@ -70,6 +73,16 @@ class Family:
         if rc != 0:
             raise RuntimeError(self.L.pdos_family_last_error().decode())
         return ms.value
+
+    def scales(self) -> dict:
+        """Per loaded instance: lambda and the stop test's scales 1 + |b|,
+        1 + |c|, as computed on the device at load."""
+        lam, ps, ds = np.zeros(self.count), np.zeros(self.count), np.zeros(self.count)
+        rc = self.L.pdos_family_scales(self.h, lam.ctypes.data_as(_dp), ps.ctypes.data_as(_dp),
+                                       ds.ctypes.data_as(_dp))
+        if rc != 0:
+            raise RuntimeError(self.L.pdos_family_last_error().decode())
+        return dict(lam=lam, pscale=ps, dscale=ds)
 
     def fetch(self) -> dict:
         c = self.count

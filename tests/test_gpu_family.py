@@ -281,3 +281,55 @@ def test_family_jumbled_csc_with_duplicates():
         x, s, y, c = mix["x"][t], mix["s"][t], mix["y"][t], Cm[t]
         assert np.linalg.norm(A @ x + s - t0.b) < 1.05e-3 * (1 + nb)
         assert np.linalg.norm(c + A.T @ y) < 1.05e-3 * (1 + np.linalg.norm(c))
+
+
+def _seq_norm1p(v):
+    """1 + |v| with the sum of squares in index order, one rounding per
+    operation (the reference's loop, pdos.c:76-77)."""
+    acc = 0.0
+    for x in v:
+        acc = acc + float(x) * float(x)
+    return 1.0 + float(np.sqrt(acc))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_family_load_setup_on_device_bitwise(precision):
+    """pdos_family_load scales each instance's c (and b) on the device: lambda
+    equals the reference's initWork lambda (the oracle's setup of the same
+    instance) bit for bit, and the stop-test scales equal the reference's
+    sequential norms; shared b (portfolio family) and per-instance b (dense
+    family); a second load of the same width reuses the device buffers."""
+    fam = _fam(NORMALIZE=1)
+    _, Cm = fam.sample(seed=9, count=70)
+    print("family load scales, shared b", precision, flush=True)
+    F = Family(fam.instance(Cm[0]), precision=precision)
+    for rep in range(2):  # the second load reuses the batch's buffers
+        F.load(Cm)
+        sc = F.scales()
+        for t in (0, 1, 33, 69):
+            p = fam.instance(Cm[t])
+            _, _, lam = oracle.setup(p)
+            assert sc["lam"][t] == lam, (rep, t, sc["lam"][t], lam)
+            assert sc["pscale"][t] == _seq_norm1p(p.b), t
+            assert sc["dscale"][t] == _seq_norm1p(Cm[t]), t
+    dense = problems.DenseFamily(seed=3, m=96, n=24, l=64, nsoc=4, qdim=8, EPS_ABS=1e-3, NORMALIZE=1)
+    B, C2, _ = dense.sample(seed=5, count=10)
+    print("family load scales, per-instance b", precision, flush=True)
+    F2 = Family(dense.instance(B[0], C2[0]), precision=precision)
+    F2.load(C2, B)
+    sc = F2.scales()
+    for t in range(10):
+        p = dense.instance(B[t], C2[t])
+        _, _, lam = oracle.setup(p)
+        assert sc["lam"][t] == lam, (t, sc["lam"][t], lam)
+        assert sc["pscale"][t] == _seq_norm1p(B[t]), t
+        assert sc["dscale"][t] == _seq_norm1p(C2[t]), t
+    # and the solve after the device-side setup still matches the oracle (fp64)
+    if precision == "fp64":
+        F2.run()
+        out = F2.fetch()
+        for t in range(10):
+            ref = oracle.solve(dense.instance(B[t], C2[t]), "direct")
+            assert out["status"][t] == ref["status"] and out["iters"][t] == ref["iters"], t
+            for k in "xsy":
+                assert _rel(out[k][t], ref[k]) <= 1e-6, (t, k)
