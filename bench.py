@@ -565,8 +565,8 @@ def run_wg(args, probs, config, plugin):
     log(f"prepare: {prep:.2f} s (Python conversion {h.convert_s:.2f} s; library ms {parts})")
     for r in tuning:
         log(f"  tuned n+m in {r['n_plus_m']}: {r['count']} instances -> {r['variant']}, persistent "
-            f"{r['persistent']}, {r['entries_per_lane']} entries per lane; candidates (variant, persistent, "
-            f"entries per lane, ms): {r['candidates']}")
+            f"{r['persistent']}, {r['entries_per_lane']} entries per lane, {r['threads']} threads; candidates "
+            f"(variant, persistent, entries per lane, threads, ms): {r['candidates']}")
     for _ in range(args.warmup):
         h.run()
     kms = []
@@ -586,8 +586,8 @@ def run_wg(args, probs, config, plugin):
              "prepare_s": round(prep, 3), "prepare_parts_ms": parts, "python_convert_s": round(h.convert_s, 3),
              "device_bytes": dev_bytes,
              "host_threads": int(os.environ.get("PDOS_HOST_THREADS", os.cpu_count())),
-             "tuning": [{k: r[k] for k in ("n_plus_m", "count", "variant", "persistent", "entries_per_lane")}
-                        for r in tuning]}
+             "tuning": [{k: r[k] for k in ("n_plus_m", "count", "variant", "persistent", "entries_per_lane",
+                                          "threads")} for r in tuning]}
     return acc["verified_fp64"] / wall, wall, "f64" if prec == "fp64" else "f32", config, roof, acc, extra, \
         probs[:args.cpu_sample], plugin
 

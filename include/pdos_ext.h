@@ -53,7 +53,8 @@ typedef struct PdosOptions {
   int tune;        /* batches (pdos_batch, pdos_batch_prepare): 1 = time candidate
                       launch shapes of the workgroup engine per size bucket of
                       n + m (kernel variant, persistent work-queue launch,
-                      entries per lane) on samples of the batch and use the
+                      entries per lane, threads per workgroup) on samples
+                      of the batch and use the
                       fastest; decisions are cached per process.  0 = the
                       built-in rules. */
 } PdosOptions;
@@ -107,9 +108,10 @@ void pdos_batch_free(PdosBatchHandle *h);
 /* Device memory a prepared batch holds (bytes; 0 for NULL). */
 size_t pdos_batch_device_bytes(const PdosBatchHandle *h);
 /* The tuner's decisions for a batch prepared with opt->tune (JSON: per size
- * bucket the n + m range, instances, sample size, chosen variant, launch mode
- * and entries per lane, and every candidate's [variant, persistent,
- * entries_per_lane, kernel ms]); "" without tuning.  Owned by the handle. */
+ * bucket the n + m range, instances, sample size, chosen variant, launch mode,
+ * entries per lane and threads per workgroup, and every candidate's
+ * [variant, persistent, entries_per_lane, threads, kernel ms]); "" without
+ * tuning.  Owned by the handle. */
 const char *pdos_batch_tuning(const PdosBatchHandle *h);
 /* Host wall time (ms) of pdos_batch_prepare's parts: ms[0] setup (scaling and
  * transpose, host threads), ms[1] grouping by matrix and the direct plugin's

@@ -473,6 +473,11 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
   variant_env = wg_variant_env();
   persist_env = -1;
   if (const char* pe = std::getenv("PDOS_WG_PERSISTENT")) persist_env = std::atoi(pe) ? 1 : 0;
+  threads_env = 0;
+  if (const char* te = std::getenv("PDOS_WG_THREADS")) {
+    const int t = std::atoi(te);
+    threads_env = t == 256 || t == 512 || t == 1024 ? t : 0;
+  }
   // the kernels' own LDS (the work-queue slot) comes first
   const int lds_cap = C.max_lds - 64;
   // members too large for one workgroup's LDS go to the whole-GPU engine
@@ -719,8 +724,9 @@ int wg_variant_env() {
 // without fewer workgroups per CU than the streaming variant (or the launch
 // has at most one workgroup per CU anyway), else the image in device memory
 // where its iterates fit, else the streaming variant.  Classes group records
-// by (workgroups per CU that the LDS allows, launch mode:
-// PDOS_WG_PERSISTENT = 1 / 0 forces the work-queue launch on or off), biggest
+// by (workgroups per CU that the LDS and the waves allow, threads per
+// workgroup, launch mode: PDOS_WG_PERSISTENT = 1 / 0 forces the work-queue
+// launch on or off), biggest
 // first, and within a class the instances with the most nonzeros first.
 void WgBatch::set_shapes() {
   const int count = (int)insts.size();
@@ -741,9 +747,24 @@ void WgBatch::set_shapes() {
     w.resident = v;
     w.lds_bytes = v == 1 ? L.y : (v == 2 ? L.z : L.x);
   }
+  // threads per workgroup: 256 for the streaming variant; the image
+  // variants 256, 512 or 1,024 (PDOS_WG_THREADS, else the tuner's choice,
+  // else 512 when there are no more instances than CUs -- latency decides,
+  // and 8 waves deal a level's rows and batches over more SIMDs: the n=101
+  // portfolio 0.94 -> 0.85 ms per call -- and 256 otherwise)
+  auto threads = [&](int i) {
+    if (insts[i].resident == 0) return 256;
+    if (threads_env) return threads_env;
+    if (!wgsize.empty()) return (int)wgsize[i];
+    return nwg_total <= ctx().num_cu ? 512 : 256;
+  };
+  auto tcode = [&](int i) { const int t = threads(i); return t == 1024 ? 2 : (t == 512 ? 1 : 0); };
+  // workgroups per CU: what the LDS allows, and at most 16 waves per CU
+  // (four per SIMD at the kernels' 128 VGPRs)
+  auto occ = [&](int i) { return std::min(wpc(insts[i].lds_bytes), 16 / (threads(i) / 64)); };
   auto key = [&](int i) {
     const int p = persist_env >= 0 ? persist_env : (persist.empty() ? 0 : persist[i]);
-    return wpc(insts[i].lds_bytes) * 2 + p;
+    return ((occ(i) - 1) * kSizes + tcode(i)) * 2 + p;
   };
   order.clear();
   for (int i = 0; i < count; ++i)
@@ -760,7 +781,8 @@ void WgBatch::set_shapes() {
       cls_start[ncls] = k;
       cls_lds[ncls] = 0;
       cls_persist[ncls] = key(order[k]) & 1;
-      cls_occ[ncls] = key(order[k]) / 2;
+      cls_occ[ncls] = occ(order[k]);
+      cls_threads[ncls] = threads(order[k]);
       ++ncls;
     }
     cls_lds[ncls - 1] = std::max(cls_lds[ncls - 1], insts[order[k]].lds_bytes);
@@ -798,7 +820,8 @@ void WgBatch::set_shapes() {
 // n + m) are the kernel variant (streaming, resident, image in device
 // memory), the launch mode (a workgroup per instance, or persistent
 // workgroups that take instances from a work queue) and, for the image
-// variants, the entries per lane that the wave batches aim for (4, 8).
+// variants, the entries per lane that the wave batches aim for (4, 8) and
+// the threads per workgroup (256, 512, 1,024).
 // Each is timed (HIP events) on a sample of the bucket's instances, every
 // one running a fixed kTuneIters iterations (EPS 0, so no early stop): when
 // the whole batch holds more instances than there are CUs, enough records to
@@ -809,7 +832,7 @@ namespace {
 constexpr int kTuneIters = 50;
 constexpr int kTuneDistinct = 64;
 constexpr double kPersistMargin = 0.02;
-struct TuneChoice { signed char v, p, e; };
+struct TuneChoice { signed char v, p, e; short t; };
 std::mutex g_tune_mu;
 std::map<std::vector<int>, std::pair<TuneChoice, std::string>> g_tune_cache;
 const char* variant_name(int v) { return v == 1 ? "resident" : (v == 2 ? "image" : "stream"); }
@@ -826,6 +849,7 @@ bool WgBatch::tune(int dev, bool fp64_, std::string& err) {
   var_force.assign(count, -1);
   persist.assign(count, 0);
   epl.assign(count, 4);
+  wgsize.assign(count, 256);
   std::map<int, std::vector<int>> buckets;
   int nwg = 0;
   for (int i = 0; i < count; ++i) {
@@ -839,7 +863,7 @@ bool WgBatch::tune(int dev, bool fp64_, std::string& err) {
   for (const auto& bk : buckets) {
     const std::vector<int>& mem = bk.second;
     const std::vector<int> key = {dev, fp64_ ? 1 : 0, direct ? 1 : 0, load, bk.first};
-    TuneChoice best{-1, 0, 4};
+    TuneChoice best{-1, 0, 4, 256};
     std::string cands;
     bool cached = false;
     {
@@ -897,9 +921,10 @@ bool WgBatch::tune(int dev, bool fp64_, std::string& err) {
       };
       // one untimed run per sample batch (first launch costs), then one timed
       // run per candidate
-      auto time = [&](WgBatch& T, int v, int p, double& ms) {
+      auto time = [&](WgBatch& T, int v, int p, double& ms, int th = 256) {
         T.var_force.assign(ns, (signed char)v);
         T.persist.assign(ns, (signed char)p);
+        T.wgsize.assign(ns, (short)th);
         T.set_shapes();
         if (!T.upload_records(err)) return false;
         if (!T.warm) {
@@ -917,32 +942,49 @@ bool WgBatch::tune(int dev, bool fp64_, std::string& err) {
       };
       char buf[96];
       double best_ms = 1e30;
-      std::unique_ptr<WgBatch> T;
+      // the work-queue launch and the wider workgroups must win by
+      // kPersistMargin each: every launch mode and workgroup size is a launch
+      // class of its own, and classes beyond the device's hardware queues run
+      // one after another
+      auto eff = [](double ms, int p, int th) { return ms * (1.0 + kPersistMargin * (p + (th != 256 ? 1 : 0))); };
+      auto note = [&](int v, int p, int e, int th, double ms) {
+        std::snprintf(buf, sizeof buf, "%s[\"%s\",%d,%d,%d,%.4f]", cands.empty() ? "" : ",", variant_name(v), p, e,
+                      th, ms);
+        cands += buf;
+      };
+      std::unique_ptr<WgBatch> T, T2;
       if (!make(4, T)) { err = "tuning: " + err; return false; }
+      // stage 1: every variant and launch mode; stage 2: from the best
+      // image variant, 8 entries per lane and then 512 / 1,024 threads (the
+      // image variants only have those knobs, and the streaming variant
+      // winning stage 1 does not mean an image variant cannot win stage 2)
+      TuneChoice img{-1, 0, 4, 256};
+      double img_ms = 1e30;
+      auto consider = [&](const TuneChoice& c, double ms) {
+        note(c.v, c.p, c.e, c.t, ms);
+        const double x = eff(ms, c.p, c.t);
+        if (x < best_ms) { best_ms = x; best = c; }
+        if (c.v != 0 && x < img_ms) { img_ms = x; img = c; }
+      };
       for (int v = 0; v < kVariants; ++v) {
         if (v && !has(*T, v)) continue;
         for (int p = 0; p < 2; ++p) {
           double ms;
           if (!time(*T, v, p, ms)) { err = "tuning: " + err; return false; }
-          std::snprintf(buf, sizeof buf, "%s[\"%s\",%d,4,%.4f]", cands.empty() ? "" : ",", variant_name(v), p, ms);
-          cands += buf;
-          // the work-queue launch must win by kPersistMargin: each launch
-          // mode is a launch class of its own, and classes beyond the
-          // device's hardware queues run one after another
-          const double eff = p ? ms * (1.0 + kPersistMargin) : ms;
-          if (eff < best_ms) { best_ms = eff; best = TuneChoice{(signed char)v, (signed char)p, 4}; }
+          consider(TuneChoice{(signed char)v, (signed char)p, 4, 256}, ms);
         }
       }
-      if (best.v != 0)
-        for (int e : {8}) {
-          std::unique_ptr<WgBatch> T2;
-          double ms;
-          if (!make(e, T2) || !time(*T2, best.v, best.p, ms)) { err = "tuning: " + err; return false; }
-          std::snprintf(buf, sizeof buf, ",[\"%s\",%d,%d,%.4f]", variant_name(best.v), best.p, e, ms);
-          cands += buf;
-          const double eff = best.p ? ms * (1.0 + kPersistMargin) : ms;
-          if (eff < best_ms) { best_ms = eff; best.e = (signed char)e; }
+      if (img.v > 0) {
+        double ms;
+        if (!make(8, T2) || !time(*T2, img.v, img.p, ms)) { err = "tuning: " + err; return false; }
+        consider(TuneChoice{img.v, img.p, 8, 256}, ms);
+        const TuneChoice base = img;
+        WgBatch& Tb = base.e == 8 ? *T2 : *T;
+        for (int th : {512, 1024}) {
+          if (!time(Tb, base.v, base.p, ms, th)) { err = "tuning: " + err; return false; }
+          consider(TuneChoice{base.v, base.p, base.e, (short)th}, ms);
         }
+      }
       std::lock_guard<std::mutex> g(g_tune_mu);
       g_tune_cache[key] = {best, cands};
     }
@@ -950,14 +992,15 @@ bool WgBatch::tune(int dev, bool fp64_, std::string& err) {
       var_force[i] = best.v;
       persist[i] = best.p;
       epl[i] = best.e;
+      wgsize[i] = best.t;
     }
     char head[256];
     std::snprintf(head, sizeof head,
                   "%s{\"n_plus_m\":[%d,%d],\"count\":%d,\"sample\":%d,\"cached\":%s,\"variant\":\"%s\","
-                  "\"persistent\":%d,\"entries_per_lane\":%d,\"candidates\":[",
+                  "\"persistent\":%d,\"entries_per_lane\":%d,\"threads\":%d,\"candidates\":[",
                   js.size() > 1 ? "," : "", (int)std::ceil(std::exp2(bk.first / 2.0)),
                   (int)std::ceil(std::exp2((bk.first + 1) / 2.0)) - 1, (int)mem.size(), cached ? 0 : ns,
-                  cached ? "true" : "false", variant_name(best.v), best.p, best.e);
+                  cached ? "true" : "false", variant_name(best.v), best.p, best.e, best.t);
     js += head + cands + "]}";
   }
   tuning = js + "]";
@@ -1018,7 +1061,7 @@ bool WgBatch::run(std::string& err) {
   for (int q = 0; q < ncls && e == hipSuccess; ++q) {
     const int c = cls_seq[q];
     hipStream_t sc = cls_stream[c] == 0 ? stream : xs[cls_stream[c] - 1];
-    e = wg_launch(recs + cls_start[c], cls_start[c + 1] - cls_start[c], cls_lds[c], fp64, direct,
+    e = wg_launch(recs + cls_start[c], cls_start[c + 1] - cls_start[c], cls_lds[c], cls_threads[c], fp64, direct,
                   cls_persist[c] ? reinterpret_cast<int*>(base + queue_off) + c : nullptr,
                   cls_occ[c] * ctx().num_cu, sc);
   }

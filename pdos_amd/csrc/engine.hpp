@@ -52,8 +52,8 @@ hipError_t dense_at_pair(hipStream_t st, int m, int n, int ldb, int m64, const _
                          const float* Y, const float* W, float* Up, float* Vp, size_t plane, int splits);
 hipError_t dense_a(hipStream_t st, int m, int n, int ldb, int n64, const __bf16* ARh, const __bf16* ARl,
                    const float* X, float* Z);
-hipError_t wg_launch(const WgInst* d_insts, int count, int lds_bytes, bool fp64, bool direct, int* queue, int grid,
-                     hipStream_t st);
+hipError_t wg_launch(const WgInst* d_insts, int count, int lds_bytes, int threads, bool fp64, bool direct, int* queue,
+                     int grid, hipStream_t st);
 
 // Whole-GPU engine for one large instance (both plugins), prepared once and
 // re-runnable from the prepared start.
@@ -103,7 +103,7 @@ struct WgBatch {
   // unavailable; s: the streaming variant's matrix copies were packed
   struct VarLds { int x, y, z; bool s; };
   std::vector<VarLds> vlds;
-  int variant_env = -1, persist_env = -1;
+  int variant_env = -1, persist_env = -1, threads_env = 0;
   bool pack_all = false;  // pack the streaming copies even where an image fits (the tuner's samples)
   // Launch classes: instances grouped by how many of their workgroups fit
   // in a CU's LDS, each class launched with its own LDS size on its own
@@ -113,10 +113,11 @@ struct WgBatch {
   // kernel variant (streaming, resident, image in device memory).
   static constexpr int kOccMax = 4;
   static constexpr int kVariants = 3;
-  static constexpr int kClasses = 2 * kOccMax;
+  static constexpr int kSizes = 3;  // threads per workgroup: 256, 512, 1,024 (image variants)
+  static constexpr int kClasses = 2 * kSizes * kOccMax;
   std::vector<int> order;                 // class order -> instance
   int ncls = 0, cls_start[kClasses + 1] = {0}, cls_lds[kClasses] = {0};
-  int cls_persist[kClasses] = {0}, cls_occ[kClasses] = {0};
+  int cls_persist[kClasses] = {0}, cls_occ[kClasses] = {0}, cls_threads[kClasses] = {0};
   // Classes run concurrently on at most kStreams streams (the device's
   // hardware queues: more streams would share queues and serialize behind
   // each other anyway); cls_seq lists the classes in launch order,
@@ -128,6 +129,7 @@ struct WgBatch {
   // automatic), launch mode (1 persistent), entries per lane of the wave
   // batches
   std::vector<signed char> var_force, persist, epl;
+  std::vector<short> wgsize;  // threads per workgroup of an image-variant record (empty: 256)
   std::string tuning;  // the tuner's decisions (JSON), empty if it did not run
   bool warm = false;   // run at least once (the tuner's untimed first run)
   hipStream_t xs[kClasses - 1] = {};

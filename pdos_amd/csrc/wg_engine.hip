@@ -520,11 +520,11 @@ __device__ __forceinline__ void block_total(double (&v)[NV], double* red, int& s
     for (int k = 0; k < NV; ++k) buf[w * 4 + k] = v[k];
   }
   __syncthreads();
+  const int nw = blockDim.x >> 6;  // 4, 8 or 16 waves
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     double a = 0.0;
-#pragma unroll
-    for (int j = 0; j < NW; ++j) a += buf[j * 4 + k];
+    for (int j = 0; j < nw; ++j) a += buf[j * 4 + k];
     v[k] = a;
   }
   slot ^= 1;
@@ -569,7 +569,8 @@ template <typename T, typename Gather, typename Emit>
 __device__ __forceinline__ void spmv_res(const int4* pos, const int* idx, const T* val, const int4* bat, int nbat,
                                          Gather gx, Emit emit) {
   const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  for (int b = w; b < nbat; b += NW) {
+  const int nw = blockDim.x >> 6;
+  for (int b = w; b < nbat; b += nw) {
     const int4 B = uniform_batch(bat, b);
     const int k = lane >> B.z, sub = lane & ((1 << B.z) - 1);
     const bool on = k < B.y;
@@ -591,10 +592,11 @@ template <bool BACK>
 __device__ __forceinline__ void res_sweep(int nlev, const int* lb, const int4* bat, const int4* pos, const int* idx,
                                           const double* val, const double* dq, double* z) {
   const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nw = blockDim.x >> 6;
   auto zg = [&](int c) { return z[c]; };
   for (int lv = 0; lv < nlev; ++lv) {
     const int b1 = __builtin_amdgcn_readfirstlane(lb[lv + 1]);
-    for (int b = __builtin_amdgcn_readfirstlane(lb[lv]) + w; b < b1; b += NW) {
+    for (int b = __builtin_amdgcn_readfirstlane(lb[lv]) + w; b < b1; b += nw) {
       const int4 B = uniform_batch(bat, b);
       const int k = lane >> B.z, sub = lane & ((1 << B.z) - 1);
       const bool on = k < B.y;
@@ -652,10 +654,11 @@ __device__ __forceinline__ void wg_res_body(const WgInst* __restrict__ insts, in
   const ResLayout& L = I.lay;
   const int n = I.n, m = I.m, N = I.N, tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
+  const int nt = blockDim.x, nw = nt >> 6;  // 256, 512 or 1,024 threads
   if (IMG_LDS) {
     const uint4* src = reinterpret_cast<const uint4*>(I.img);
     uint4* dst = reinterpret_cast<uint4*>(smem);
-    for (int i = tid; i < L.img / 16; i += NT) dst[i] = src[i];
+    for (int i = tid; i < L.img / 16; i += nt) dst[i] = src[i];
   }
   // static data: the image in LDS, or in device memory; per-instance data
   // after the image in LDS, or at the start of LDS
@@ -676,8 +679,8 @@ __device__ __forceinline__ void wg_res_body(const WgInst* __restrict__ insts, in
   const int* socp = reinterpret_cast<const int*>(sb + L.soc);
   double* bl = reinterpret_cast<double*>(db + L.b);
   double* cl = reinterpret_cast<double*>(db + L.c);
-  for (int j = tid; j < m; j += NT) bl[j] = I.b[j];
-  for (int i = tid; i < n; i += NT) cl[i] = I.c[i];
+  for (int j = tid; j < m; j += nt) bl[j] = I.b[j];
+  for (int i = tid; i < n; i += nt) cl[i] = I.c[i];
   auto al = [](size_t v) { return (v + 15) & ~(size_t)15; };
   unsigned char* vp = db + L.vec;
   T* x = reinterpret_cast<T*>(vp);
@@ -708,14 +711,14 @@ __device__ __forceinline__ void wg_res_body(const WgInst* __restrict__ insts, in
 
   if (I.wsrc) {  // warm start in original units (common.h:148-161, 188): x/E, y/D, s*D
     const float* w = I.wsrc;
-    for (int i = tid; i < n; i += NT) x[i] = T(double(w[i]) / El[i]);
-    for (int j = tid; j < m; j += NT) {
+    for (int i = tid; i < n; i += nt) x[i] = T(double(w[i]) / El[i]);
+    for (int j = tid; j < m; j += nt) {
       y[j] = T(double(w[n + j]) / Dl[j]);
       s[j] = T(double(w[n + m + j]) * Dl[j]);
     }
   } else {
-    for (int i = tid; i < n; i += NT) x[i] = T(I.x0[i]);
-    for (int j = tid; j < m; j += NT) { s[j] = T(I.s0[j]); y[j] = T(I.y0[j]); }
+    for (int i = tid; i < n; i += nt) x[i] = T(I.x0[i]);
+    for (int j = tid; j < m; j += nt) { s[j] = T(I.s0[j]); y[j] = T(I.y0[j]); }
   }
   __syncthreads();
   spmv_res<T>(rpos, ridx, rval, rbat, nrbat, [&](int c) { return x[c]; }, [&](int j, T v) { ax[j] = v; });
@@ -729,7 +732,7 @@ __device__ __forceinline__ void wg_res_body(const WgInst* __restrict__ insts, in
     // ---- stop check (linAlg.h:158-201); the CG start argument t or the
     // permuted KKT right-hand side (direct/private.c:16-31) ----
     double acc[4] = {0.0, 0.0, 0.0, 0.0};  // |pres|^2, |dres|^2, c'x, b'y
-    for (int j = tid; j < m; j += NT) {
+    for (int j = tid; j < m; j += nt) {
       const T sj = s[j], bj = T(bl[j]), yj = y[j], axj = ax[j];
       const T r = ((sj - bj) + axj) / T(Dl[j]);
       acc[0] += double(r) * double(r);
@@ -744,7 +747,7 @@ __device__ __forceinline__ void wg_res_body(const WgInst* __restrict__ insts, in
     });
     if (DIRECT) {
       const int* perm = reinterpret_cast<const int*>(sb + L.perm);
-      for (int kk = tid; kk < N; kk += NT) {
+      for (int kk = tid; kk < N; kk += nt) {
         const int src = perm[kk];
         double v;
         if (src < n) {
@@ -768,8 +771,8 @@ __device__ __forceinline__ void wg_res_body(const WgInst* __restrict__ insts, in
       const double mc = fmax((I.pscale - 1.0) / I.pscale, (I.dscale - 1.0) / I.dscale);
       if (!(mw <= mc)) {
         fallback = 1;
-        for (int i = tid; i < n; i += NT) x[i] = T(0);
-        for (int j = tid; j < m; j += NT) { s[j] = T(0); y[j] = T(0); ax[j] = T(0); }
+        for (int i = tid; i < n; i += nt) x[i] = T(0);
+        for (int j = tid; j < m; j += nt) { s[j] = T(0); y[j] = T(0); ax[j] = T(0); }
         __syncthreads();
         it = -1;
         continue;
@@ -815,7 +818,7 @@ __device__ __forceinline__ void wg_res_body(const WgInst* __restrict__ insts, in
           block_total<1>(r2, red, slot);
           const T alpha = T(rho / r2[0]);
           double r3[1] = {0.0};
-          for (int i = tid; i < n; i += NT) {
+          for (int i = tid; i < n; i += nt) {
             x[i] += alpha * p[i];
             const T qi = q[i] - alpha * u[i];
             q[i] = qi;
@@ -867,7 +870,7 @@ __device__ __forceinline__ void wg_res_body(const WgInst* __restrict__ insts, in
                       reinterpret_cast<const int4*>(sb + L.bpos), reinterpret_cast<const int*>(sb + L.bidx),
                       reinterpret_cast<const double*>(sb + L.bval), reinterpret_cast<const double*>(sb + L.bdq), bp);
       const int* pinv = reinterpret_cast<const int*>(sb + L.pinv);
-      for (int i = tid; i < n; i += NT) x[i] = T(bp[pinv[i]]);
+      for (int i = tid; i < n; i += nt) x[i] = T(bp[pinv[i]]);
       spmv_res<T>(rpos, ridp, rval, rbat, nrbat, [&](int c) { return T(bp[c]); }, [&](int j, T v) {
         ax[j] = v;
         T stj = T(bp[pinv[n + j]] + (double(s[j]) + double(lam) * double(y[j])));
@@ -887,7 +890,7 @@ __device__ __forceinline__ void wg_res_body(const WgInst* __restrict__ insts, in
       });
     }
     __syncthreads();
-    for (int cn = wv; cn < nsoc; cn += NW) {
+    for (int cn = wv; cn < nsoc; cn += nw) {
       const int r0 = socp[cn], r1 = socp[cn + 1];
       double a2 = 0.0;
       for (int j = r0 + 1 + lane; j < r1; j += 64) a2 += double(s[j]) * double(s[j]);
@@ -906,8 +909,8 @@ __device__ __forceinline__ void wg_res_body(const WgInst* __restrict__ insts, in
     __syncthreads();
   }
 
-  for (int i = tid; i < n; i += NT) I.x[i] = double(x[i]);
-  for (int j = tid; j < m; j += NT) { I.s[j] = double(s[j]); I.y[j] = double(y[j]); }
+  for (int i = tid; i < n; i += nt) I.x[i] = double(x[i]);
+  for (int j = tid; j < m; j += nt) { I.s[j] = double(s[j]); I.y[j] = double(y[j]); }
   if (tid == 0) {
     DevResult* R = I.res;
     R->iters = it;
@@ -954,31 +957,62 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
   }
 }
 
+// The image variants with 512 or 1,024 threads per workgroup (8 or 16
+// waves): where the LDS allows few workgroups per CU, more waves per
+// workgroup keep more of the CU's SIMDs busy on the same instance (rows,
+// batches and levels dealt over more waves).  Records of the streaming
+// variant never come here (its row blocks are sized for 256 threads).
+template <typename T, bool DIRECT>
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void k_wg_wide(
+    const WgInst* __restrict__ insts, int count, int* queue) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  __shared__ int wg_next;
+  int idx = blockIdx.x;
+  for (;;) {
+    if (queue) {
+      __syncthreads();  // the previous instance is done with LDS and wg_next
+      if (threadIdx.x == 0) wg_next = atomicAdd(queue, 1);
+      __syncthreads();
+      idx = wg_next;
+    }
+    if (idx >= count) break;
+    const int v = __builtin_amdgcn_readfirstlane(insts[idx].resident);
+    if (v == 1) wg_res_body<T, DIRECT, true>(insts, idx, smem);
+    else wg_res_body<T, DIRECT, false>(insts, idx, smem);
+    if (!queue) break;
+  }
+}
+
 __global__ void k_zero_int(int* p) {
   if (threadIdx.x == 0) p[0] = 0;
 }
 
-template <typename T, bool DIRECT>
-hipError_t launch_t(const WgInst* d_insts, int count, int lds_bytes, int* queue, int grid, hipStream_t st) {
-  auto fn = k_wg<T, DIRECT>;
-  // the kernel's dynamic LDS limit only grows (per process and kernel)
-  static std::atomic<int> lds_set{-1};
-  hipError_t e = hipSuccess;
-  if (lds_bytes > lds_set.load()) {
-    e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
-                            lds_bytes);
-    if (e != hipSuccess) return e;
-    int cur = lds_set.load();
-    while (lds_bytes > cur && !lds_set.compare_exchange_weak(cur, lds_bytes)) {
-    }
+// A kernel's dynamic LDS limit, raised only (per process and kernel).
+hipError_t lds_limit(const void* fn, int lds_bytes, std::atomic<int>& set) {
+  if (lds_bytes <= set.load()) return hipSuccess;
+  const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
+  if (e != hipSuccess) return e;
+  int cur = set.load();
+  while (lds_bytes > cur && !set.compare_exchange_weak(cur, lds_bytes)) {
   }
+  return hipSuccess;
+}
+
+template <typename T, bool DIRECT>
+hipError_t launch_t(const WgInst* d_insts, int count, int lds_bytes, int threads, int* queue, int grid,
+                    hipStream_t st) {
+  static std::atomic<int> lds_set{-1}, lds_set_wide{-1};
+  const bool wide = threads > NT;
+  auto fn = wide ? k_wg_wide<T, DIRECT> : k_wg<T, DIRECT>;
+  hipError_t e = lds_limit(reinterpret_cast<const void*>(fn), lds_bytes, wide ? lds_set_wide : lds_set);
+  if (e != hipSuccess) return e;
   if (queue) {  // a kernel, not a memset node, so that the launch may be captured in a graph
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, queue);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(fn, dim3(queue ? std::max(1, std::min(grid, count)) : count), dim3(NT), lds_bytes, st, d_insts,
-                     count, queue);
+  hipLaunchKernelGGL(fn, dim3(queue ? std::max(1, std::min(grid, count)) : count), dim3(wide ? threads : NT),
+                     lds_bytes, st, d_insts, count, queue);
   return hipGetLastError();
 }
 
@@ -1004,15 +1038,17 @@ hipError_t device_copy(void* dst, const void* src, size_t bytes, hipStream_t st)
 }
 
 // Launches the workgroup engine over `count` instances (one workgroup each,
-// each in the variant its record names); queue (device int) != nullptr:
+// each in the variant its record names; threads 256, or 512 / 1,024 for a
+// class of image-variant records only); queue (device int) != nullptr:
 // `grid` persistent workgroups drain the records in order through that
 // counter (zeroed here first).
-hipError_t wg_launch(const WgInst* d_insts, int count, int lds_bytes, bool fp64, bool direct, int* queue, int grid,
-                     hipStream_t st) {
-  if (fp64) return direct ? launch_t<double, true>(d_insts, count, lds_bytes, queue, grid, st)
-                          : launch_t<double, false>(d_insts, count, lds_bytes, queue, grid, st);
-  return direct ? launch_t<float, true>(d_insts, count, lds_bytes, queue, grid, st)
-                : launch_t<float, false>(d_insts, count, lds_bytes, queue, grid, st);
+hipError_t wg_launch(const WgInst* d_insts, int count, int lds_bytes, int threads, bool fp64, bool direct, int* queue,
+                     int grid, hipStream_t st) {
+  if (threads != 256 && threads != 512 && threads != 1024) return hipErrorInvalidValue;
+  if (fp64) return direct ? launch_t<double, true>(d_insts, count, lds_bytes, threads, queue, grid, st)
+                          : launch_t<double, false>(d_insts, count, lds_bytes, threads, queue, grid, st);
+  return direct ? launch_t<float, true>(d_insts, count, lds_bytes, threads, queue, grid, st)
+                : launch_t<float, false>(d_insts, count, lds_bytes, threads, queue, grid, st);
 }
 
 }  // namespace pdos_amd
