@@ -749,14 +749,18 @@ void WgBatch::set_shapes() {
   }
   // threads per workgroup: 256 for the streaming variant; the image
   // variants 256, 512 or 1,024 (PDOS_WG_THREADS, else the tuner's choice,
-  // else 512 when there are no more instances than CUs -- latency decides,
+  // else: 512 when there are no more instances than CUs -- latency decides,
   // and 8 waves deal a level's rows and batches over more SIMDs: the n=101
-  // portfolio 0.94 -> 0.85 ms per call -- and 256 otherwise)
+  // portfolio 0.94 -> 0.85 ms per call -- and otherwise enough threads that
+  // the workgroups the LDS allows per CU hold 16 waves: 256 at 4 per CU, 512
+  // at 2 or 3, 1,024 at 1)
   auto threads = [&](int i) {
     if (insts[i].resident == 0) return 256;
     if (threads_env) return threads_env;
     if (!wgsize.empty()) return (int)wgsize[i];
-    return nwg_total <= ctx().num_cu ? 512 : 256;
+    if (nwg_total <= ctx().num_cu) return 512;
+    const int w = wpc(insts[i].lds_bytes);
+    return w >= 4 ? 256 : (w >= 2 ? 512 : 1024);
   };
   auto tcode = [&](int i) { const int t = threads(i); return t == 1024 ? 2 : (t == 512 ? 1 : 0); };
   // workgroups per CU: what the LDS allows, and at most 16 waves per CU
