@@ -356,9 +356,12 @@ static void res_image(const Setup& s, const Factor* f, bool fp64, int epl, std::
     // The dense trailing chain T: the forward levels from the last one with
     // more than one row on each hold a single row (a portfolio's budget and
     // factor rows); T's rows only depend on earlier rows and on each other,
-    // and T's columns only have entries in T.  At most 64 rows: one wave
-    // solves the chain with its t x t block held densely, forward and then
-    // backward, instead of 2 t barrier-separated levels.
+    // and T's columns only have entries in T.  At most 64 rows.  Its solve,
+    // z_T <- (L_T D_T L_T')^-1 z_T, is one dense product with that inverse,
+    // formed here once per factorization (forward and back substitution on
+    // the unit vectors, binary64): one wave applies it with every load in
+    // flight at once, instead of 2 t dependent substitution steps (each a
+    // cross-lane read, ~100 cycles) or 2 t barrier-separated levels.
     const int nfl = f->nfl();
     int lam0 = nfl;
     while (lam0 > 0 && f->fl_ptr[lam0] - f->fl_ptr[lam0 - 1] == 1 && nfl - (lam0 - 1) <= 64) --lam0;
@@ -369,16 +372,29 @@ static void res_image(const Setup& s, const Factor* f, bool fp64, int epl, std::
       trow[k] = f->fl_rows[f->fl_ptr[lam0 + k]];
       tpos[trow[k]] = k;
     }
-    std::vector<double> tl((size_t)t * t, 0.0), tdq(t);
+    std::vector<double> tl((size_t)t * t, 0.0), tdq(t), tm((size_t)t * t), col(t);
     for (int k = 0; k < t; ++k) {
       tdq[k] = f->Ld[trow[k]];
       for (int p = f->Rp[trow[k]]; p < f->Rp[trow[k] + 1]; ++p)
         if (tpos[f->Ri[p]] >= 0) tl[(size_t)k * t + tpos[f->Ri[p]]] = f->Rx[p];
     }
+    for (int c = 0; c < t; ++c) {  // column c of the inverse: L D L' col = e_c
+      for (int k = 0; k < t; ++k) {
+        double v = k == c ? 1.0 : 0.0;
+        for (int j = 0; j < k; ++j) v -= tl[(size_t)k * t + j] * col[j];
+        col[k] = v;
+      }
+      for (int k = 0; k < t; ++k) col[k] /= tdq[k];
+      for (int k = t - 1; k >= 0; --k) {
+        double v = col[k];
+        for (int i = k + 1; i < t; ++i) v -= tl[(size_t)i * t + k] * col[i];
+        col[k] = v;
+      }
+      for (int k = 0; k < t; ++k) tm[(size_t)k * t + c] = col[k];
+    }
     L.t = t;
     L.trow = put(trow.data(), trow.size() * 4);
-    L.tdq = put(tdq.data(), tdq.size() * 8);
-    L.tl = put(tl.data(), tl.size() * 8);
+    L.tm = put(tm.data(), tm.size() * 8);
     // backward levels of the columns outside T, with T's values final
     std::vector<int> blev(N, -1);
     int nbl = 0;
@@ -412,8 +428,15 @@ static void res_image(const Setup& s, const Factor* f, bool fp64, int epl, std::
       auto kept = [&](int r, int p) { return back || tpos[r] < 0 || tpos[I[p]] < 0; };
       for (int i = 0; i < N; ++i)
         for (int p = P[i]; p < P[i + 1]; ++p) len[i] += kept(i, p) ? 1 : 0;
+      // forward: a row without entries (all of level 0) leaves z unchanged,
+      // so it is left out, and a level without such rows with it
+      std::vector<int> work;
       for (const auto& items : levels) {
-        batches(items, len, order, bat);
+        work.clear();
+        for (int r : items)
+          if (back || len[r] > 0) work.push_back(r);
+        if (work.empty()) continue;
+        batches(work, len, order, bat);
         lb.push_back((int)bat.size());
       }
       const int npos = (int)order.size();
@@ -434,10 +457,10 @@ static void res_image(const Setup& s, const Factor* f, bool fp64, int epl, std::
       if (back) {
         L.bpos = o_pos; L.bidx = o_idx; L.bval = o_val; L.bbat = o_bat; L.blb = o_lb;
         L.bdq = put(dq.data(), std::max<size_t>(dq.size(), 1) * 8);
-        L.nbl = (int)levels.size();
+        L.nbl = (int)lb.size() - 1;
       } else {
         L.fpos = o_pos; L.fidx = o_idx; L.fval = o_val; L.fbat = o_bat; L.flb = o_lb;
-        L.nfl = (int)levels.size();
+        L.nfl = (int)lb.size() - 1;
       }
     }
   }

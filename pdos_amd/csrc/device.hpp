@@ -29,7 +29,8 @@ struct ResLayout {
   int fpos, fidx, fval, fbat, flb, bpos, bidx, bval, bdq, bbat, blb;
   int nrbat, ncbat;  // wave batches of A's rows and of A''s
   int nfl, nbl;      // levels of the two sweeps (T excluded, forward: plus T's outside entries)
-  int t, trow, tdq, tl;  // the dense trailing chain: t rows, their ids, D, the t x t block (row-major)
+  int t, trow, tm;   // the dense trailing chain: t rows, their ids, the inverse of its t x t L D L' block
+                     // (row-major)
   int img;          // image bytes (multiple of 16)
   int b, c, vec;    // per-instance b, c (binary64) and the iterates
   int total;        // dynamic LDS bytes
@@ -104,11 +105,12 @@ constexpr int kWgLongRow = 64;        // rows with this many entries are summed 
 
 // Iterates of the resident variant, in LDS after b and c: x (n), s, y, s~,
 // A x (m) in the bulk type; indirect: p, q, u (n), t (m); direct: the KKT
-// vector (n + m binary64); then two reduction slots.
+// vector (n + m binary64) and 64 binary64 of staging for the trailing
+// block; then two reduction slots.
 inline long res_vec_bytes(int n, int m, int N, bool direct, int tsize) {
   auto al = [](long v) { return (long)((v + 15) & ~15L); };
   long bytes = al((long)(n + 4 * (long)m) * tsize);
-  if (direct) bytes += al((long)N * 8);
+  if (direct) bytes += al((long)N * 8) + 64 * 8;  // the KKT vector; the trailing block's staging
   else bytes += al((long)(3 * (long)n + m) * tsize);
   return bytes + 2 * 16 * 4 * 8;
 }

@@ -33,6 +33,35 @@ namespace {
 constexpr int NT = kWgThreads;
 constexpr int NW = NT / 64;
 
+// Pointers into a group's image: in LDS (generic pointers derived from the
+// kernel's LDS array, which the compiler resolves to LDS accesses) or in
+// device memory, typed as global so that its loads are global loads, which
+// the compiler may reorder around the LDS stores of the iterates (through
+// generic pointers they would be flat loads, ordered against every LDS
+// access).  The host pass only parses this code and keeps generic pointers.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define PDOS_GLOBAL __attribute__((address_space(1)))
+#else
+#define PDOS_GLOBAL
+#endif
+template <bool IN_LDS>
+struct ImgSpace {
+  using Base = const unsigned char*;
+  template <typename V>
+  using Ptr = const V*;
+};
+template <>
+struct ImgSpace<false> {
+  using Base = const PDOS_GLOBAL unsigned char*;
+  template <typename V>
+  using Ptr = const PDOS_GLOBAL V*;
+};
+template <typename V, bool IN_LDS>
+__device__ __forceinline__ typename ImgSpace<IN_LDS>::template Ptr<V> img_at(typename ImgSpace<IN_LDS>::Base b,
+                                                                              int off) {
+  return reinterpret_cast<typename ImgSpace<IN_LDS>::template Ptr<V>>(b + off);
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -508,7 +537,10 @@ __device__ __forceinline__ double wave_total(double v) {
   return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
-// block_sum with the DPP wave total.
+// block_sum with the DPP wave total; the waves' totals are then summed by
+// the same DPP tree within each row of 16 lanes (lane j of every row reads
+// wave j's total, zero past the last wave), so every lane ends with the
+// block totals without a loop of dependent LDS reads and adds.
 template <int NV>
 __device__ __forceinline__ void block_total(double (&v)[NV], double* red, int& slot) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -520,13 +552,11 @@ __device__ __forceinline__ void block_total(double (&v)[NV], double* red, int& s
     for (int k = 0; k < NV; ++k) buf[w * 4 + k] = v[k];
   }
   __syncthreads();
-  const int nw = blockDim.x >> 6;  // 4, 8 or 16 waves
+  const int nw = blockDim.x >> 6, j = lane & 15;  // 4, 8 or 16 waves
 #pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    double a = 0.0;
-    for (int j = 0; j < nw; ++j) a += buf[j * 4 + k];
-    v[k] = a;
-  }
+  for (int k = 0; k < NV; ++k) v[k] = j < nw ? buf[j * 4 + k] : 0.0;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = group_sum(v[k], 4);
   slot ^= 1;
 }
 
@@ -535,8 +565,8 @@ __device__ __forceinline__ void block_total(double (&v)[NV], double* red, int& s
 // padding, so the reads stay inside them) and masked by selects, so the lane
 // runs straight-line code with all loads of a kind in flight together; the
 // rest (rows of more than 4 G entries) one by one.  Summed in entry order.
-template <typename A, typename V, typename Gather>
-__device__ __forceinline__ A lane_dot(const int* idx, const V* val, int e0, int e1, int G, Gather gx) {
+template <typename A, typename V, typename PI, typename PV, typename Gather>
+__device__ __forceinline__ A lane_dot(PI idx, PV val, int e0, int e1, int G, Gather gx) {
   int ix[4];
   V vv[4], gv[4];
 #pragma unroll
@@ -559,15 +589,15 @@ __device__ __forceinline__ A lane_dot(const int* idx, const V* val, int e0, int 
 // in storage order.  The lane at the start of each group emits.
 // A wave batch read from LDS, made wave-uniform (scalar registers), so the
 // branches on its class are scalar branches, not exec-mask divergence.
-__device__ __forceinline__ int4 uniform_batch(const int4* bat, int b) {
+template <typename PB>
+__device__ __forceinline__ int4 uniform_batch(PB bat, int b) {
   const int4 B = bat[b];
   return int4{__builtin_amdgcn_readfirstlane(B.x), __builtin_amdgcn_readfirstlane(B.y),
               __builtin_amdgcn_readfirstlane(B.z), 0};
 }
 
-template <typename T, typename Gather, typename Emit>
-__device__ __forceinline__ void spmv_res(const int4* pos, const int* idx, const T* val, const int4* bat, int nbat,
-                                         Gather gx, Emit emit) {
+template <typename T, typename P4, typename PI, typename PV, typename Gather, typename Emit>
+__device__ __forceinline__ void spmv_res(P4 pos, PI idx, PV val, P4 bat, int nbat, Gather gx, Emit emit) {
   const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int nw = blockDim.x >> 6;
   for (int b = w; b < nbat; b += nw) {
@@ -576,9 +606,9 @@ __device__ __forceinline__ void spmv_res(const int4* pos, const int* idx, const 
     const bool on = k < B.y;
     const int4 P = pos[B.x + (on ? k : 0)];
     if (B.z == 0) {
-      if (on) emit(P.x, lane_dot<T>(idx, val, P.y, P.z, 1, gx));
+      if (on) emit(P.x, lane_dot<T, T>(idx, val, P.y, P.z, 1, gx));
     } else {
-      double acc = on ? lane_dot<double>(idx, val, P.y + sub, P.z, 1 << B.z, gx) : 0.0;
+      double acc = on ? lane_dot<double, T>(idx, val, P.y + sub, P.z, 1 << B.z, gx) : 0.0;
       acc = group_sum(acc, B.z);
       if (on && sub == 0) emit(P.x, T(acc));
     }
@@ -588,9 +618,8 @@ __device__ __forceinline__ void spmv_res(const int4* pos, const int* idx, const 
 // One triangular sweep in LDS, level by level (lb: level -> its batches;
 // position q: pos[q] = {row, entries}; backward: divide by dq[q] first), one
 // barrier per level.
-template <bool BACK>
-__device__ __forceinline__ void res_sweep(int nlev, const int* lb, const int4* bat, const int4* pos, const int* idx,
-                                          const double* val, const double* dq, double* z) {
+template <bool BACK, typename PI, typename P4, typename PD>
+__device__ __forceinline__ void res_sweep(int nlev, PI lb, P4 bat, P4 pos, PI idx, PD val, PD dq, double* z) {
   const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int nw = blockDim.x >> 6;
   auto zg = [&](int c) { return z[c]; };
@@ -604,7 +633,7 @@ __device__ __forceinline__ void res_sweep(int nlev, const int* lb, const int4* b
       const int4 P = pos[q];
       const double zi = z[P.x];
       const double d = BACK ? dq[q] : 1.0;
-      double acc = on ? lane_dot<double>(idx, val, P.y + sub, P.z, 1 << B.z, zg) : 0.0;
+      double acc = on ? lane_dot<double, double>(idx, val, P.y + sub, P.z, 1 << B.z, zg) : 0.0;
       acc = group_sum(acc, B.z);
       if (on && sub == 0) z[P.x] = (BACK ? zi / d : zi) - acc;
     }
@@ -612,35 +641,32 @@ __device__ __forceinline__ void res_sweep(int nlev, const int* lb, const int4* b
   }
 }
 
-__device__ __forceinline__ double readlane_d(double v, int l) {
-  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
-  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
-  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
-}
-
-// The dense trailing chain T of the factor (t <= 64 rows in pivot order, the
-// t x t block tl row-major): wave 0 holds z_T in its lanes (z_T already less
-// the entries outside T), solves the unit lower triangle (lane i subtracts
-// L(i, k) z_k as each z_k becomes final and is read from lane k), then D and
-// the unit upper triangle L' the same way from the last row up.  No barrier
-// inside; the caller's barrier publishes z_T.
-__device__ __forceinline__ void res_chain(int t, const int* trow, const double* tdq, const double* tl, double* z) {
+// The dense trailing chain T of the factor (t <= 64 rows): z_T <- M z_T with
+// M = (L_T D_T L_T')^-1 formed on the host (tm, t x t row-major).  Wave 0:
+// lane i stages z at row trow[i] in tz, then sums M(i, k) tz[k] over k in
+// four interleaved partial sums (fixed order), every load issued before the
+// first product.  One wave, so its staging writes precede its reads without
+// a barrier; the caller's barrier publishes z_T.
+template <typename PI, typename PD>
+__device__ __forceinline__ void res_tblock(int t, PI trow, PD tm, double* z, double* tz) {
   if (threadIdx.x >= 64) return;
   const int lane = threadIdx.x;
   const bool on = lane < t;
   const int r = on ? trow[lane] : 0;
-  double v = on ? z[r] : 0.0;
-  for (int k = 0; k < t; ++k) {
-    const double lik = on && lane > k ? tl[lane * t + k] : 0.0;
-    v -= lik * readlane_d(v, k);
+  if (on) tz[lane] = z[r];
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < t; k0 += 16) {
+    double mv[16], zv[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int k = k0 + u;
+      mv[u] = on && k < t ? tm[lane * t + k] : 0.0;
+      zv[u] = k < t ? tz[k] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) acc[u & 3] += mv[u] * zv[u];
   }
-  v = on ? v / tdq[lane] : 0.0;
-  for (int i = t - 1; i >= 0; --i) {
-    const double lki = on && lane < i ? tl[i * t + lane] : 0.0;
-    v -= lki * readlane_d(v, i);
-  }
-  if (on) z[r] = v;
+  if (on) z[r] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
 }
 
 // IMG_LDS = false: the same kernel with the group's image read from device
@@ -662,21 +688,23 @@ __device__ __forceinline__ void wg_res_body(const WgInst* __restrict__ insts, in
   }
   // static data: the image in LDS, or in device memory; per-instance data
   // after the image in LDS, or at the start of LDS
-  const unsigned char* sb = IMG_LDS ? static_cast<const unsigned char*>(smem) : I.img;
+  typename ImgSpace<IMG_LDS>::Base sb;
+  if constexpr (IMG_LDS) sb = smem;
+  else sb = (typename ImgSpace<IMG_LDS>::Base)I.img;
   unsigned char* db = IMG_LDS ? smem : smem - L.img;
-  const double* Dl = reinterpret_cast<const double*>(sb + L.D);
-  const double* El = reinterpret_cast<const double*>(sb + L.E);
-  const int4* rpos = reinterpret_cast<const int4*>(sb + L.rpos);
-  const int* ridx = reinterpret_cast<const int*>(sb + L.ridx);
-  const T* rval = reinterpret_cast<const T*>(sb + L.rval);
-  const int* ridp = reinterpret_cast<const int*>(sb + L.ridp);
-  const int4* cpos = reinterpret_cast<const int4*>(sb + L.cpos);
-  const int* cidx = reinterpret_cast<const int*>(sb + L.cidx);
-  const T* cval = reinterpret_cast<const T*>(sb + L.cval);
-  const int4* rbat = reinterpret_cast<const int4*>(sb + L.rbat);
-  const int4* cbat = reinterpret_cast<const int4*>(sb + L.cbat);
+  const auto Dl = img_at<double, IMG_LDS>(sb, L.D);
+  const auto El = img_at<double, IMG_LDS>(sb, L.E);
+  const auto rpos = img_at<int4, IMG_LDS>(sb, L.rpos);
+  const auto ridx = img_at<int, IMG_LDS>(sb, L.ridx);
+  const auto rval = img_at<T, IMG_LDS>(sb, L.rval);
+  const auto ridp = img_at<int, IMG_LDS>(sb, L.ridp);
+  const auto cpos = img_at<int4, IMG_LDS>(sb, L.cpos);
+  const auto cidx = img_at<int, IMG_LDS>(sb, L.cidx);
+  const auto cval = img_at<T, IMG_LDS>(sb, L.cval);
+  const auto rbat = img_at<int4, IMG_LDS>(sb, L.rbat);
+  const auto cbat = img_at<int4, IMG_LDS>(sb, L.cbat);
   const int nrbat = L.nrbat, ncbat = L.ncbat;
-  const int* socp = reinterpret_cast<const int*>(sb + L.soc);
+  const auto socp = img_at<int, IMG_LDS>(sb, L.soc);
   double* bl = reinterpret_cast<double*>(db + L.b);
   double* cl = reinterpret_cast<double*>(db + L.c);
   for (int j = tid; j < m; j += nt) bl[j] = I.b[j];
@@ -690,10 +718,12 @@ __device__ __forceinline__ void wg_res_body(const WgInst* __restrict__ insts, in
   T* ax = st + m;
   vp += al((size_t)(n + 4 * m) * sizeof(T));
   T *p = nullptr, *q = nullptr, *u = nullptr, *t = nullptr;
-  double* bp = nullptr;
+  double *bp = nullptr, *tz = nullptr;
   if (DIRECT) {
     bp = reinterpret_cast<double*>(vp);
     vp += al((size_t)N * 8);
+    tz = reinterpret_cast<double*>(vp);
+    vp += 64 * 8;
   } else {
     p = reinterpret_cast<T*>(vp);
     q = p + n;
@@ -746,7 +776,7 @@ __device__ __forceinline__ void wg_res_body(const WgInst* __restrict__ insts, in
       acc[2] += double(ci_) * double(x[i]);
     });
     if (DIRECT) {
-      const int* perm = reinterpret_cast<const int*>(sb + L.perm);
+      const auto perm = img_at<int, IMG_LDS>(sb, L.perm);
       for (int kk = tid; kk < N; kk += nt) {
         const int src = perm[kk];
         double v;
@@ -858,18 +888,17 @@ __device__ __forceinline__ void wg_res_body(const WgInst* __restrict__ insts, in
     } else {
       // ---- LDL' sweeps in level order, then x = z[0:n] and s~ = z[n:] +
       // (s + lam y) read through pivot positions inside the A product ----
-      res_sweep<false>(L.nfl, reinterpret_cast<const int*>(sb + L.flb), reinterpret_cast<const int4*>(sb + L.fbat),
-                       reinterpret_cast<const int4*>(sb + L.fpos), reinterpret_cast<const int*>(sb + L.fidx),
-                       reinterpret_cast<const double*>(sb + L.fval), nullptr, bp);
+      res_sweep<false>(L.nfl, img_at<int, IMG_LDS>(sb, L.flb), img_at<int4, IMG_LDS>(sb, L.fbat),
+                       img_at<int4, IMG_LDS>(sb, L.fpos), img_at<int, IMG_LDS>(sb, L.fidx),
+                       img_at<double, IMG_LDS>(sb, L.fval), img_at<double, IMG_LDS>(sb, L.fval), bp);
       if (L.t) {
-        res_chain(L.t, reinterpret_cast<const int*>(sb + L.trow), reinterpret_cast<const double*>(sb + L.tdq),
-                  reinterpret_cast<const double*>(sb + L.tl), bp);
+        res_tblock(L.t, img_at<int, IMG_LDS>(sb, L.trow), img_at<double, IMG_LDS>(sb, L.tm), bp, tz);
         __syncthreads();
       }
-      res_sweep<true>(L.nbl, reinterpret_cast<const int*>(sb + L.blb), reinterpret_cast<const int4*>(sb + L.bbat),
-                      reinterpret_cast<const int4*>(sb + L.bpos), reinterpret_cast<const int*>(sb + L.bidx),
-                      reinterpret_cast<const double*>(sb + L.bval), reinterpret_cast<const double*>(sb + L.bdq), bp);
-      const int* pinv = reinterpret_cast<const int*>(sb + L.pinv);
+      res_sweep<true>(L.nbl, img_at<int, IMG_LDS>(sb, L.blb), img_at<int4, IMG_LDS>(sb, L.bbat),
+                      img_at<int4, IMG_LDS>(sb, L.bpos), img_at<int, IMG_LDS>(sb, L.bidx),
+                      img_at<double, IMG_LDS>(sb, L.bval), img_at<double, IMG_LDS>(sb, L.bdq), bp);
+      const auto pinv = img_at<int, IMG_LDS>(sb, L.pinv);
       for (int i = tid; i < n; i += nt) x[i] = T(bp[pinv[i]]);
       spmv_res<T>(rpos, ridp, rval, rbat, nrbat, [&](int c) { return T(bp[c]); }, [&](int j, T v) {
         ax[j] = v;
@@ -892,18 +921,39 @@ __device__ __forceinline__ void wg_res_body(const WgInst* __restrict__ insts, in
     __syncthreads();
     for (int cn = wv; cn < nsoc; cn += nw) {
       const int r0 = socp[cn], r1 = socp[cn + 1];
+      // a lane's entries j = r0 + lane + 64 u in chunks of four, each chunk's
+      // loads issued together (stores to s and y would otherwise order every
+      // load after the previous entry's stores)
       double a2 = 0.0;
-      for (int j = r0 + 1 + lane; j < r1; j += 64) a2 += double(s[j]) * double(s[j]);
+      for (int j0 = r0 + 1 + lane; j0 < r1; j0 += 256) {
+        T sv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) sv[u] = j0 + 64 * u < r1 ? s[j0 + 64 * u] : T(0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) a2 += double(sv[u]) * double(sv[u]);
+      }
       a2 = wave_total(a2);
       const T ns = T(sqrt(a2));
       const T t0 = s[r0];
       const int mode = ns <= t0 ? 0 : (ns <= -t0 ? 1 : 2);
       const T a = (ns + t0) / T(2);
-      for (int j = r0 + lane; j < r1; j += 64) {
-        const T zj = s[j];
-        const T sj = mode == 0 ? zj : (mode == 1 ? T(0) : (j == r0 ? a : a * zj / ns));
-        s[j] = sj;
-        y[j] += (sj - st[j]) / lam;
+      for (int j0 = r0 + lane; j0 < r1; j0 += 256) {
+        T zv[4], tv[4], yv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int j = j0 + 64 * u;
+          if (j < r1) { zv[u] = s[j]; tv[u] = st[j]; yv[u] = y[j]; }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int j = j0 + 64 * u;
+          if (j < r1) {
+            const T zj = zv[u];
+            const T sj = mode == 0 ? zj : (mode == 1 ? T(0) : (j == r0 ? a : a * zj / ns));
+            s[j] = sj;
+            y[j] = yv[u] + (sj - tv[u]) / lam;
+          }
+        }
       }
     }
     __syncthreads();
