@@ -61,4 +61,9 @@ bool kkt_order(const Setup& S, std::vector<int>& perm, std::string& err);
 bool amd_order(int N, const std::vector<int>& Kp, const std::vector<int>& Ki, std::vector<int>& perm,
                std::string& err);
 
+// amd_order through a per-process store of the orderings of recently seen
+// patterns (the ordering depends on the pattern only).
+bool amd_order_cached(int N, const std::vector<int>& Kp, const std::vector<int>& Ki, std::vector<int>& perm,
+                      std::string& err);
+
 }  // namespace pdos_amd

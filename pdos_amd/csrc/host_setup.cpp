@@ -239,7 +239,7 @@ bool kkt_order(const Setup& S, std::vector<int>& perm, std::string& err) {
   std::vector<int> Kp, Ki;
   std::vector<double> Kx;
   form_kkt(S, Kp, Ki, Kx);
-  return amd_order(S.n + S.m, Kp, Ki, perm, err);
+  return amd_order_cached(S.n + S.m, Kp, Ki, perm, err);
 }
 
 bool build_factor(const Setup& S, const std::vector<int>& perm_in, Factor& F, std::string& err) {
@@ -251,7 +251,7 @@ bool build_factor(const Setup& S, const std::vector<int>& perm_in, Factor& F, st
   if (!perm_in.empty()) {
     if ((int)perm_in.size() != N) { err = "permutation has wrong length"; return false; }
     F.P = perm_in;
-  } else if (!amd_order(N, Kp, Ki, F.P, err)) {
+  } else if (!amd_order_cached(N, Kp, Ki, F.P, err)) {
     return false;
   }
   std::vector<int> pinv(N, -1);
@@ -283,8 +283,8 @@ bool build_factor(const Setup& S, const std::vector<int>& perm_in, Factor& F, st
   // Elimination tree (Liu's algorithm with path-compressed ancestors): row r
   // of M links each earlier column c it touches to r via c's current root.
   std::vector<int> parent(N, -1), anc(N, -1);
+  std::vector<int> rp(N + 1, 0), rc(Mp[N]);  // M by rows: columns c <= r of row r
   {
-    std::vector<int> rp(N + 1, 0), rc(Mp[N]);  // M by rows: columns c < r of row r
     for (int p = 0; p < Mp[N]; ++p) rp[Mi[p] + 1]++;
     for (int r = 0; r < N; ++r) rp[r + 1] += rp[r];
     std::vector<int> nxt(rp.begin(), rp.end() - 1);
@@ -302,34 +302,33 @@ bool build_factor(const Setup& S, const std::vector<int>& perm_in, Factor& F, st
         }
       }
   }
-  // Column structures of L: struct(L_j) = rows > j of M(:, j) united with the
-  // structures of j's children in the tree, minus j.
-  std::vector<int> child_head(N, -1), child_next(N, -1);
-  for (int j = N - 1; j >= 0; --j)
-    if (parent[j] >= 0) { child_next[j] = child_head[parent[j]]; child_head[parent[j]] = j; }
-  std::vector<std::vector<int>> cols(N);
-  std::vector<int> seen(N, -1);
+  // Column structures of L from row subtrees: row i of L has a nonzero in
+  // every column on the tree paths from the columns k < i of row i of M up
+  // to i (each path stops at a column already visited for this row).  One
+  // pass counts, the second appends i to those columns; rows are taken in
+  // increasing order, so every column's rows come out sorted.  The work is
+  // nnz(M) + nnz(L).
   F.Lp.assign(N + 1, 0);
-  for (int j = 0; j < N; ++j) {
-    auto& cj = cols[j];
-    seen[j] = j;
-    for (int p = Mp[j]; p < Mp[j + 1]; ++p)
-      if (Mi[p] > j && seen[Mi[p]] != j) { seen[Mi[p]] = j; cj.push_back(Mi[p]); }
-    for (int c = child_head[j]; c != -1; c = child_next[c]) {
-      for (int r : cols[c])
-        if (r > j && seen[r] != j) { seen[r] = j; cj.push_back(r); }
-    }
-    std::sort(cj.begin(), cj.end());
-    F.Lp[j + 1] = F.Lp[j] + (int)cj.size();
+  {
+    std::vector<int> mark(N, -1);
+    auto walk = [&](int i, auto&& visit) {
+      mark[i] = i;
+      for (int q = rp[i]; q < rp[i + 1]; ++q)
+        for (int k = rc[q]; k >= 0 && k < i && mark[k] != i; k = parent[k]) {
+          mark[k] = i;
+          visit(k);
+        }
+    };
+    for (int i = 0; i < N; ++i) walk(i, [&](int k) { F.Lp[k + 1]++; });
+    for (int j = 0; j < N; ++j) F.Lp[j + 1] += F.Lp[j];
+    F.Li.resize(F.Lp[N]);
+    std::vector<int> pos(F.Lp.begin(), F.Lp.end() - 1);
+    std::fill(mark.begin(), mark.end(), -1);
+    for (int i = 0; i < N; ++i) walk(i, [&](int k) { F.Li[pos[k]++] = i; });
   }
   const long lnz_total = F.Lp[N];
-  F.Li.resize(lnz_total);
   F.Lx.assign(lnz_total, 0.0);
   F.Ld.assign(N, 0.0);
-  for (int j = 0; j < N; ++j) {
-    std::copy(cols[j].begin(), cols[j].end(), F.Li.begin() + F.Lp[j]);
-    std::vector<int>().swap(cols[j]);
-  }
   // Left-looking numeric factorization: column j gathers M(:, j) and
   // subtracts L(:, k) d_k L(j, k) for every earlier column k with L(j, k) != 0;
   // the columns waiting for row r are kept in a linked list per row, each at

@@ -27,6 +27,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstring>
+#include <list>
+#include <mutex>
 #include <set>
 #include <string>
 #include <vector>
@@ -257,6 +260,57 @@ bool amd_order(int N, const std::vector<int>& Kp, const std::vector<int>& Ki, st
   for (int i : dense) perm.push_back(i);
   if ((int)perm.size() != N) { err = "ordering: internal error (" + std::to_string(perm.size()) + " of " +
                                      std::to_string(N) + " nodes ordered)"; return false; }
+  return true;
+}
+
+// The ordering is a function of the pattern alone, so a process keeps the
+// orderings of the last kOrderCacheEntries patterns (up to kOrderCacheNnz
+// entries each) and returns a stored one when the same pattern comes again
+// (parametric problems: a pdos() call per instance of a family; a batch
+// member whose matrix group is new but whose pattern is not).  The pattern is
+// compared in full on a hash match.
+namespace {
+constexpr size_t kOrderCacheEntries = 16;
+constexpr size_t kOrderCacheNnz = 1 << 22;
+struct OrderEntry {
+  uint64_t hash;
+  int N;
+  std::vector<int> Kp, Ki, perm;
+};
+std::mutex g_order_mu;
+std::list<OrderEntry> g_order_cache;  // most recently used first
+
+uint64_t pattern_hash(int N, const std::vector<int>& Kp, const std::vector<int>& Ki) {
+  uint64_t h = 1469598103934665603ull ^ (uint64_t)N;
+  auto mix = [&](const std::vector<int>& v) {
+    for (int x : v) h = (h ^ (uint32_t)x) * 1099511628211ull;
+  };
+  mix(Kp);
+  mix(Ki);
+  return h;
+}
+}  // namespace
+
+bool amd_order_cached(int N, const std::vector<int>& Kp, const std::vector<int>& Ki, std::vector<int>& perm,
+                      std::string& err) {
+  const bool cacheable = Ki.size() <= kOrderCacheNnz;
+  uint64_t h = 0;
+  if (cacheable) {
+    h = pattern_hash(N, Kp, Ki);
+    std::lock_guard<std::mutex> lk(g_order_mu);
+    for (auto it = g_order_cache.begin(); it != g_order_cache.end(); ++it)
+      if (it->hash == h && it->N == N && it->Kp == Kp && it->Ki == Ki) {
+        perm = it->perm;
+        g_order_cache.splice(g_order_cache.begin(), g_order_cache, it);
+        return true;
+      }
+  }
+  if (!amd_order(N, Kp, Ki, perm, err)) return false;
+  if (cacheable) {
+    std::lock_guard<std::mutex> lk(g_order_mu);
+    g_order_cache.push_front(OrderEntry{h, N, Kp, Ki, perm});
+    if (g_order_cache.size() > kOrderCacheEntries) g_order_cache.pop_back();
+  }
   return true;
 }
 
