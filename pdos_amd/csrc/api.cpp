@@ -349,6 +349,15 @@ static void res_image(const Setup& s, const Factor* f, bool fp64, int epl, std::
   spmv_batches(s.rp, s.ri, s.rx, f ? &pinv : nullptr, L.rpos, L.ridx, L.rval, &L.ridp, L.rbat, L.nrbat);
   spmv_batches(s.cp, s.ci, s.cx, nullptr, L.cpos, L.cidx, L.cval, nullptr, L.cbat, L.ncbat);
   L.soc = put(s.soc_ptr.data(), s.soc_ptr.size() * 4);
+  {
+    std::vector<int> big;
+    for (int c = 0; c + 1 < (int)s.soc_ptr.size(); ++c)
+      if (s.soc_ptr[c + 1] - s.soc_ptr[c] > kWgBigCone) { big.push_back(s.soc_ptr[c]); big.push_back(s.soc_ptr[c + 1]); }
+    L.nbig = (int)big.size() / 2;
+    if (big.empty()) big.push_back(0);
+    L.big = put(big.data(), big.size() * 4);
+  }
+  L.mbat = std::max(L.nrbat, L.ncbat);
   if (f) {
     const int N = f->N;
     L.perm = put(f->P.data(), N * 4);
@@ -437,6 +446,7 @@ static void res_image(const Setup& s, const Factor* f, bool fp64, int epl, std::
           if (back || len[r] > 0) work.push_back(r);
         if (work.empty()) continue;
         batches(work, len, order, bat);
+        L.mbat = std::max(L.mbat, (int)bat.size() - lb.back());
         lb.push_back((int)bat.size());
       }
       const int npos = (int)order.size();
@@ -774,14 +784,15 @@ void WgBatch::set_shapes() {
   // variants 256, 512 or 1,024 (PDOS_WG_THREADS, else the tuner's choice,
   // else: 512 when there are no more instances than CUs -- latency decides,
   // and 8 waves deal a level's rows and batches over more SIMDs: the n=101
-  // portfolio 0.94 -> 0.85 ms per call -- and otherwise enough threads that
+  // portfolio 0.94 -> 0.85 ms per call -- or 1,024 when some phase has more
+  // than 16 wave batches (two per wave at 512), and otherwise enough threads that
   // the workgroups the LDS allows per CU hold 16 waves: 256 at 4 per CU, 512
   // at 2 or 3, 1,024 at 1)
   auto threads = [&](int i) {
     if (insts[i].resident == 0) return 256;
     if (threads_env) return threads_env;
     if (!wgsize.empty()) return (int)wgsize[i];
-    if (nwg_total <= ctx().num_cu) return 512;
+    if (nwg_total <= ctx().num_cu) return insts[i].lay.mbat > 16 ? 1024 : 512;
     const int w = wpc(insts[i].lds_bytes);
     return w >= 4 ? 256 : (w >= 2 ? 512 : 1024);
   };

@@ -28,6 +28,8 @@ struct ResLayout {
   int D, E, rpos, ridx, rval, ridp, rbat, cpos, cidx, cval, cbat, soc, perm, pinv;
   int fpos, fidx, fval, fbat, flb, bpos, bidx, bval, bdq, bbat, blb;
   int nrbat, ncbat;  // wave batches of A's rows and of A''s
+  int mbat;          // most wave batches any one phase (an SpMV or a sweep level) runs
+  int nbig, big;     // second-order cones of more than kWgBigCone rows: count and {r0, r1} pairs
   int nfl, nbl;      // levels of the two sweeps (T excluded, forward: plus T's outside entries)
   int t, trow, tm;   // the dense trailing chain: t rows, their ids, the inverse of its t x t L D L' block
                      // (row-major)
@@ -99,6 +101,7 @@ inline int verbose_hist_rows(long max_iters) {
 }
 
 constexpr int kWgThreads = 256;
+constexpr int kWgBigCone = 256;       // cones with more rows are projected by the whole workgroup
 constexpr int kWgChunk = 2048;        // entries per SpMV row block (products staged in LDS)
 constexpr int kWgRowsPerBlock = 1024;  // rows per SpMV row block
 constexpr int kWgLongRow = 64;        // rows with this many entries are summed by a wave

@@ -705,6 +705,7 @@ __device__ __forceinline__ void wg_res_body(const WgInst* __restrict__ insts, in
   const auto cbat = img_at<int4, IMG_LDS>(sb, L.cbat);
   const int nrbat = L.nrbat, ncbat = L.ncbat;
   const auto socp = img_at<int, IMG_LDS>(sb, L.soc);
+  const auto bigc = img_at<int, IMG_LDS>(sb, L.big);
   double* bl = reinterpret_cast<double*>(db + L.b);
   double* cl = reinterpret_cast<double*>(db + L.c);
   for (int j = tid; j < m; j += nt) bl[j] = I.b[j];
@@ -921,6 +922,7 @@ __device__ __forceinline__ void wg_res_body(const WgInst* __restrict__ insts, in
     __syncthreads();
     for (int cn = wv; cn < nsoc; cn += nw) {
       const int r0 = socp[cn], r1 = socp[cn + 1];
+      if (r1 - r0 > kWgBigCone) continue;  // below, by the whole workgroup
       // a lane's entries j = r0 + lane + 64 u in chunks of four, each chunk's
       // loads issued together (stores to s and y would otherwise order every
       // load after the previous entry's stores)
@@ -947,6 +949,44 @@ __device__ __forceinline__ void wg_res_body(const WgInst* __restrict__ insts, in
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const int j = j0 + 64 * u;
+          if (j < r1) {
+            const T zj = zv[u];
+            const T sj = mode == 0 ? zj : (mode == 1 ? T(0) : (j == r0 ? a : a * zj / ns));
+            s[j] = sj;
+            y[j] = yv[u] + (sj - tv[u]) / lam;
+          }
+        }
+      }
+    }
+    // cones of more than kWgBigCone rows: each thread takes rows r0 + 1 + tid,
+    // + nt, ... for the norm (one block reduction, which also orders every
+    // read of the cone before the projection's writes), then rows r0 + tid,
+    // + nt, ... for the projection.  The apex is read before the reduction.
+    for (int cb = 0; cb < L.nbig; ++cb) {
+      const int r0 = bigc[2 * cb], r1 = bigc[2 * cb + 1];
+      const T t0 = s[r0];
+      double a2[1] = {0.0};
+      for (int j0 = r0 + 1 + tid; j0 < r1; j0 += 4 * nt) {
+        T sv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) sv[u] = j0 + u * nt < r1 ? s[j0 + u * nt] : T(0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) a2[0] += double(sv[u]) * double(sv[u]);
+      }
+      block_total<1>(a2, red, slot);
+      const T ns = T(sqrt(a2[0]));
+      const int mode = ns <= t0 ? 0 : (ns <= -t0 ? 1 : 2);
+      const T a = (ns + t0) / T(2);
+      for (int j0 = r0 + tid; j0 < r1; j0 += 4 * nt) {
+        T zv[4], tv[4], yv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int j = j0 + u * nt;
+          if (j < r1) { zv[u] = s[j]; tv[u] = st[j]; yv[u] = y[j]; }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int j = j0 + u * nt;
           if (j < r1) {
             const T zj = zv[u];
             const T sj = mode == 0 ? zj : (mode == 1 ? T(0) : (j == r0 ? a : a * zj / ns));
