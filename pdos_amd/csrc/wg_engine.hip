@@ -499,12 +499,20 @@ __device__ __forceinline__ void wg_stream_body(const WgInst* __restrict__ insts,
 //         barrier per level plus three.
 
 // Data-parallel-primitive (DPP) moves of a double between lanes, on its two
-// 32-bit halves; masked-off rows read 0.
+// 32-bit halves; masked-off rows read 0.  A move that writes every row (the
+// quad permutes and mirrors) needs no prior value, so the destination is not
+// zeroed first (two VALU instructions fewer per move).
 template <int CTRL, int ROWS = 0xf>
 __device__ __forceinline__ double dpp_move(double v) {
   const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-  const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, ROWS, 0xf, false);
-  const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, ROWS, 0xf, false);
+  unsigned lo, hi;
+  if constexpr (ROWS == 0xf) {
+    lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)b, CTRL, 0xf, 0xf, false);
+    hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, false);
+  } else {
+    lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, ROWS, 0xf, false);
+    hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, ROWS, 0xf, false);
+  }
   return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
@@ -655,16 +663,16 @@ __device__ __forceinline__ void res_tblock(int t, PI trow, PD tm, double* z, dou
   const int r = on ? trow[lane] : 0;
   if (on) tz[lane] = z[r];
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int k0 = 0; k0 < t; k0 += 16) {
-    double mv[16], zv[16];
+  for (int k0 = 0; k0 < t; k0 += 8) {
+    double mv[8], zv[8];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
+    for (int u = 0; u < 8; ++u) {
       const int k = k0 + u;
       mv[u] = on && k < t ? tm[lane * t + k] : 0.0;
       zv[u] = k < t ? tz[k] : 0.0;
     }
 #pragma unroll
-    for (int u = 0; u < 16; ++u) acc[u & 3] += mv[u] * zv[u];
+    for (int u = 0; u < 8; ++u) acc[u & 3] += mv[u] * zv[u];
   }
   if (on) z[r] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
 }
