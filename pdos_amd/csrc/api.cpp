@@ -88,16 +88,33 @@ bool Ctx::reserve(size_t dbytes, size_t hbytes, std::string& err) {
   return true;
 }
 
+void Packer::grow(size_t need) {
+  size_t cap = std::max<size_t>(cap_ ? cap_ : 4096, 4096);
+  while (cap < need) cap *= 2;
+  std::unique_ptr<unsigned char[]> m(new unsigned char[cap]);
+  if (len_) std::memcpy(m.get(), mem_.get(), len_);
+  mem_ = std::move(m);
+  cap_ = cap;
+}
+
+size_t Packer::slot(size_t bytes) {
+  const size_t off = (len_ + 255) & ~size_t(255);
+  const size_t end = off + std::max<size_t>(bytes, 1);
+  if (end > cap_) grow(end);
+  if (off > len_) std::memset(mem_.get() + len_, 0, off - len_);  // alignment padding
+  len_ = end;
+  return off;
+}
+
 size_t Packer::put(const void* src, size_t bytes) {
-  size_t off = (buf.size() + 255) & ~size_t(255);
-  buf.resize(off + std::max<size_t>(bytes, 1));
-  if (bytes) std::memcpy(buf.data() + off, src, bytes);
+  const size_t off = slot(bytes);
+  if (bytes) std::memcpy(mem_.get() + off, src, bytes);
   return off;
 }
 
 size_t Packer::zeros(size_t bytes) {
-  size_t off = (buf.size() + 255) & ~size_t(255);
-  buf.resize(off + std::max<size_t>(bytes, 1), 0);
+  const size_t off = slot(bytes);
+  std::memset(mem_.get() + off, 0, std::max<size_t>(bytes, 1));
   return off;
 }
 
@@ -543,6 +560,12 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
     }, ierr);
   }
   vlds.assign(count, VarLds{0, 0, 0, false});
+  {  // the images and per-instance vectors, so the staging buffer grows at most once more
+    size_t est = 0;
+    for (int i = 0; i < count; ++i) est += imgs[i].size() + (size_t)(S[i].n + 2 * (size_t)S[i].m) * 64 + 4096;
+    P.reserve(est);
+  }
+  std::vector<int> img_jobs;  // images copied into their slots by the host threads, below
   for (int i = 0; i < count; ++i) {
     const Setup& s = S[i];
     Off& k = pk[i];
@@ -557,8 +580,8 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
       k.lay = lays[i];
       if (!imgs[i].empty() && k.lay.total - k.lay.img <= lds_cap) {
         k.res = 1;
-        k.img = P.put(imgs[i].data(), imgs[i].size());
-        std::vector<unsigned char>().swap(imgs[i]);
+        k.img = P.slot(imgs[i].size());
+        img_jobs.push_back(i);
       }
       // the streaming variant's copies of the matrices (and factor): only
       // where a record may use it (forced, or no image fits; with an image
@@ -623,7 +646,16 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
   queue_off = P.zeros(sizeof(int) * kClasses);
   inst_off = P.zeros(sizeof(WgInst) * count);
   cinst_off = P.zeros(sizeof(WgInst) * count);
-  bytes = P.buf.size();
+  bytes = P.size();
+  {
+    std::string cerr;
+    parallel_for((int)img_jobs.size(), [&](int q, std::string&) {
+      const int i = img_jobs[q];
+      std::memcpy(P.data() + pk[i].img, imgs[i].data(), imgs[i].size());
+      std::vector<unsigned char>().swap(imgs[i]);
+      return true;
+    }, cerr);
+  }
   if (borrow) {
     if (!C.reserve(bytes, bytes, err)) return false;
     dbuf = C.dbuf;
@@ -699,20 +731,20 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
     }
   }
   set_shapes();
-  std::memcpy(P.buf.data() + inst_off, insts.data(), sizeof(WgInst) * count);
+  std::memcpy(P.data() + inst_off, insts.data(), sizeof(WgInst) * count);
   for (size_t k = 0; k < order.size(); ++k)
-    std::memcpy(P.buf.data() + cinst_off + sizeof(WgInst) * k, &insts[order[k]], sizeof(WgInst));
+    std::memcpy(P.data() + cinst_off + sizeof(WgInst) * k, &insts[order[k]], sizeof(WgInst));
   hipError_t e;
   if (borrow) {
     // from the context's pinned buffer, which no other work of this thread
     // touches before this call's fetch has synchronized the stream: the
     // launch follows the copy on the same stream without a host wait
-    std::memcpy(C.hbuf, P.buf.data(), bytes);
+    std::memcpy(C.hbuf, P.data(), bytes);
     e = hipMemcpyAsync(dbuf, C.hbuf, bytes, hipMemcpyHostToDevice, stream);
     ev0 = C.ev0;
     ev1 = C.ev1;
   } else {
-    e = hipMemcpyAsync(dbuf, P.buf.data(), bytes, hipMemcpyHostToDevice, stream);
+    e = hipMemcpyAsync(dbuf, P.data(), bytes, hipMemcpyHostToDevice, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e == hipSuccess && !ev0) e = hipEventCreate(&ev0);
     if (e == hipSuccess && !ev1) e = hipEventCreate(&ev1);

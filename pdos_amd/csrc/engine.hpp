@@ -30,11 +30,23 @@ struct Ctx {
 };
 Ctx& ctx();
 
-// Host staging buffer: arrays at 256-byte aligned offsets.
-struct Packer {
-  std::vector<unsigned char> buf;
+// Host staging buffer: arrays at 256-byte aligned offsets.  The storage
+// grows by doubling and is not zero-filled; slot() reserves bytes that the
+// caller writes later (several threads may fill separate slots at once, once
+// no more space is being added).
+class Packer {
+ public:
   size_t put(const void* src, size_t bytes);
   size_t zeros(size_t bytes);
+  size_t slot(size_t bytes);
+  void reserve(size_t bytes) { if (bytes > cap_) grow(bytes); }
+  unsigned char* data() { return mem_.get(); }
+  size_t size() const { return len_; }
+
+ private:
+  std::unique_ptr<unsigned char[]> mem_;
+  size_t cap_ = 0, len_ = 0;
+  void grow(size_t need);
 };
 
 double now_ms();

@@ -870,7 +870,7 @@ class GridImpl : public GridSolver {
         os[d][6] = P.put(h.chunk.data(), h.chunk.size() * sizeof(int4));
       }
     }
-    bytes = P.buf.size();
+    bytes = P.size();
     const size_t hbytes = std::max<size_t>(bytes, 4096);
     hipError_t e = hipSuccess;
     if (borrow) {
@@ -933,7 +933,7 @@ class GridImpl : public GridSolver {
       for (const void* fn : fns)
         if (e == hipSuccess) e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, plan.lds);
     }
-    std::memcpy(hbuf, P.buf.data(), bytes);
+    std::memcpy(hbuf, P.data(), bytes);
     if (e == hipSuccess) e = hipMemcpyAsync(dbuf, hbuf, bytes, hipMemcpyHostToDevice, stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
     if (e == hipSuccess) e = hipEventCreate(&ev0);
