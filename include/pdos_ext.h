@@ -113,11 +113,18 @@ size_t pdos_batch_device_bytes(const PdosBatchHandle *h);
  * [variant, persistent, entries_per_lane, threads, kernel ms]); "" without
  * tuning.  Owned by the handle. */
 const char *pdos_batch_tuning(const PdosBatchHandle *h);
-/* Host wall time (ms) of pdos_batch_prepare's parts: ms[0] setup (scaling and
- * transpose, host threads), ms[1] grouping by matrix and the direct plugin's
+/* Host wall time (ms) of pdos_batch_prepare's parts: ms[0] setup (host
+ * threads: checks, the row pattern; scaling and transpose where the setup
+ * stays on the host), ms[1] grouping by matrix and the direct plugin's
  * factorizations, ms[2] launch-shape tuning (0 without opt->tune), ms[3]
  * packing and upload.  Returns 0, or -1 for NULL arguments. */
 int pdos_batch_prepare_times(const PdosBatchHandle *h, double *ms);
+/* Scalings of instance i of a prepared batch (common.h:69-192): D (m), E (n,
+ * with the x1e3) and lambda, wherever they were formed (the indirect
+ * plugin's workgroup-engine members are set up on the device; the others on
+ * the host).  Any output may be NULL.  Returns 1 if instance i was set up on
+ * the device, 0 if on the host, -1 for bad arguments. */
+int pdos_batch_scales(const PdosBatchHandle *h, int i, double *D, double *E, double *lambda);
 
 /* Warm start for every instance of a prepared batch from device memory:
  * instance i reads [x (n), y (m), s (m)] in original units (float) at

@@ -59,6 +59,7 @@ EXPORTS = ["pdos", "freeData", "freeSol", "tic", "toc", "tocq", "pdos_default_op
            "pdos_mlp_predict_dev", "pdos_mlp_predict_dev_noalloc", "pdos_mlp_last_error", "pdos_family_create", "pdos_family_load",
            "pdos_family_run", "pdos_family_fetch", "pdos_family_free", "pdos_family_last_error",
            "pdos_family_timing", "pdos_family_info", "pdos_family_engines", "pdos_family_scales",
+           "pdos_batch_scales",
            "pdos_kkt_order", "pdos_kkt_solve_host", "pdos_setup_host", "pdos_last_error",
            "pdos_version"]
 
@@ -95,6 +96,9 @@ def lib(plugin: str = "indirect"):
         L.pdos_batch_device_bytes.argtypes = [C.c_void_p]
         L.pdos_batch_tuning.restype = C.c_char_p
         L.pdos_batch_tuning.argtypes = [C.c_void_p]
+        L.pdos_batch_scales.restype = C.c_int
+        L.pdos_batch_scales.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                        C.POINTER(C.c_double)]
         L.pdos_batch_prepare_times.restype = C.c_int
         L.pdos_batch_prepare_times.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         L.pdos_setup_host.restype = C.c_int
@@ -242,6 +246,7 @@ class BatchHandle:
         t0 = time.perf_counter()  # the Python-side conversion is timed apart from the library's prepare
         self.L = lib(plugin)
         self.n = len(problems)
+        self.dims = [(p.n, p.m) for p in problems]
         keeps = [p.to_c() for p in problems]
         DA = (C.POINTER(Data) * self.n)(*[C.pointer(kk[0]) for kk in keeps])
         KA = (C.POINTER(Cone) * self.n)(*[C.pointer(kk[1]) for kk in keeps])
@@ -285,6 +290,19 @@ class BatchHandle:
         v = (C.c_double * 4)()
         self.L.pdos_batch_prepare_times(self.h, v)
         return dict(zip(("setup", "group_factor", "tune", "pack_upload"), (round(x, 2) for x in v)))
+
+    def scales(self, i: int):
+        """D, E (with the x1e3), lambda of instance i and whether the device
+        formed them (pdos_batch_scales)."""
+        n, m = self.dims[i]
+        D = np.zeros(m)
+        E = np.zeros(n)
+        lam = C.c_double()
+        r = self.L.pdos_batch_scales(self.h, i, D.ctypes.data_as(C.POINTER(C.c_double)),
+                                     E.ctypes.data_as(C.POINTER(C.c_double)), C.byref(lam))
+        if r < 0:
+            raise RuntimeError("pdos_batch_scales: bad arguments")
+        return D, E, lam.value, bool(r)
 
     def tuning(self) -> list:
         """The tuner's decisions per size bucket ([] without tuning)."""

@@ -223,9 +223,13 @@ static int parallel_for(int count, Body body, std::string& err) {
   return -1;
 }
 
+// Same scaled matrix: with host setup the scaled values and scalings match;
+// with device setup (unscaled values) the values and NORMALIZE match, which
+// fixes the same D, E and scaled values.
 static bool same_matrix(const Setup& a, const Setup& b) {
   return a.n == b.n && a.m == b.m && a.f == b.f && a.l == b.l && a.soc_ptr == b.soc_ptr &&
-         a.cp == b.cp && a.ci == b.ci && a.cx == b.cx && a.D == b.D && a.E == b.E;
+         a.dev == b.dev && a.normalize == b.normalize && a.cp == b.cp && a.ci == b.ci && a.cx == b.cx &&
+         a.D == b.D && a.E == b.E;
 }
 
 bool WgBatch::group_and_factor(std::string& err) {
@@ -239,7 +243,7 @@ bool WgBatch::group_and_factor(std::string& err) {
     parallel_for(count, [&](int i, std::string&) {
       const Setup& s = S[i];
       uint64_t h = 1469598103934665603ull;
-      const int dims[4] = {s.n, s.m, s.f, s.l};
+      const int dims[6] = {s.n, s.m, s.f, s.l, s.dev ? 1 : 0, s.normalize};
       h = fnv(h, dims, sizeof(dims));
       h = fnv_vec(h, s.soc_ptr);
       h = fnv_vec(h, s.cp);
@@ -268,6 +272,7 @@ bool WgBatch::group_and_factor(std::string& err) {
     Setup& s = S[i];  // drop the duplicate matrix copies (the device shares the group's)
     std::vector<int>().swap(s.cp); std::vector<int>().swap(s.ci); std::vector<double>().swap(s.cx);
     std::vector<int>().swap(s.rp); std::vector<int>().swap(s.ri); std::vector<double>().swap(s.rx);
+    std::vector<int>().swap(s.csr_src);
   }
   if (direct) {  // one factorization per distinct matrix, on the host threads
     F.assign(rep.size(), Factor());
@@ -284,6 +289,8 @@ bool WgBatch::group_and_factor(std::string& err) {
 // permutation, its inverse and both sweeps of L in level order.  Fills the
 // layout, including the per-instance part after the image.  epl: entries per
 // lane that the rows' wave batches aim for.
+// With s.dev (setup on the device) the value slots and D, E are left zero:
+// the device writes them, walking the rows' positions {row, e0, e1}.
 static void res_image(const Setup& s, const Factor* f, bool fp64, int epl, std::vector<unsigned char>& img,
                       ResLayout& L) {
   img.clear();
@@ -327,9 +334,10 @@ static void res_image(const Setup& s, const Factor* f, bool fp64, int epl, std::
     for (size_t q = 0; q < ord.size(); ++q) pos[q] = int4{ord[q], qp[q], qp[q + 1], 0};
     return put(pos.data(), pos.size() * sizeof(int4));
   };
+  // zero_vals (device setup): the value slots stay zero
   auto spmv_batches = [&](const std::vector<int>& ptr, const std::vector<int>& ix, const std::vector<double>& vx,
                           const std::vector<int>* map, int& o_pos, int& o_idx, int& o_val, int* o_map,
-                          int& o_bat, int& nbat) {
+                          int& o_bat, int& nbat, bool zero_vals) {
     const int nr = (int)ptr.size() - 1;
     std::vector<int> items(nr), len(nr), ord;
     std::vector<int4> bat;
@@ -341,7 +349,8 @@ static void res_image(const Setup& s, const Factor* f, bool fp64, int epl, std::
       const int r = ord[q];
       qp[q + 1] = qp[q] + len[r];
       qi.insert(qi.end(), ix.begin() + ptr[r], ix.begin() + ptr[r + 1]);
-      qv.insert(qv.end(), vx.begin() + ptr[r], vx.begin() + ptr[r + 1]);
+      if (zero_vals) qv.resize(qv.size() + len[r], 0.0);
+      else qv.insert(qv.end(), vx.begin() + ptr[r], vx.begin() + ptr[r + 1]);
       if (map)
         for (int e = ptr[r]; e < ptr[r + 1]; ++e) qm.push_back((*map)[ix[e]]);
     }
@@ -356,15 +365,22 @@ static void res_image(const Setup& s, const Factor* f, bool fp64, int epl, std::
     nbat = (int)bat.size();
   };
   std::memset(&L, 0, sizeof(L));
-  L.D = put(s.D.data(), s.D.size() * 8);
-  L.E = put(s.E.data(), s.E.size() * 8);
+  if (s.dev) {  // formed on the device
+    const std::vector<double> zm(s.m, 0.0), zn(s.n, 0.0);
+    L.D = put(zm.data(), zm.size() * 8);
+    L.E = put(zn.data(), zn.size() * 8);
+  } else {
+    L.D = put(s.D.data(), s.D.size() * 8);
+    L.E = put(s.E.data(), s.E.size() * 8);
+  }
+
   std::vector<int> pinv;
   if (f) {
     pinv.resize(f->N);
     for (int k = 0; k < f->N; ++k) pinv[f->P[k]] = k;
   }
-  spmv_batches(s.rp, s.ri, s.rx, f ? &pinv : nullptr, L.rpos, L.ridx, L.rval, &L.ridp, L.rbat, L.nrbat);
-  spmv_batches(s.cp, s.ci, s.cx, nullptr, L.cpos, L.cidx, L.cval, nullptr, L.cbat, L.ncbat);
+  spmv_batches(s.rp, s.ri, s.rx, f ? &pinv : nullptr, L.rpos, L.ridx, L.rval, &L.ridp, L.rbat, L.nrbat, s.dev);
+  spmv_batches(s.cp, s.ci, s.cx, nullptr, L.cpos, L.cidx, L.cval, nullptr, L.cbat, L.ncbat, s.dev);
   L.soc = put(s.soc_ptr.data(), s.soc_ptr.size() * 4);
   {
     std::vector<int> big;
@@ -511,7 +527,8 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
   struct Off { size_t soc, rp, ri, rx, cp, ci, cx, rb, cb, b, c, D, E, x0, s0, y0;
                int nrb, ncb;
                size_t perm, lrp, lri, lrx, lcp, lci, lcx, ld, flp, flr, blp, blr;
-               size_t img; int res; ResLayout lay; bool stream; };
+               size_t img; int res; ResLayout lay; bool stream;
+               size_t dax, dcp, dci, drp, dsrc, dE0; };
   std::vector<Off> pk(count);
   xo.resize(count); so.resize(count); yo.resize(count);
   ho.assign(count, 0);
@@ -535,6 +552,7 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
   big.clear();
   for (int i = 0; i < count; ++i)
     if (wg_lds_bytes(S[i].n, S[i].m, S[i].n + S[i].m, direct, tsz) > lds_cap) {
+      if (S[i].dev) { err = "internal: an instance past the LDS was set up for the device"; return false; }
       is_big[i] = 1;
       big.push_back(i);
     }
@@ -566,6 +584,17 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
     P.reserve(est);
   }
   std::vector<int> img_jobs;  // images copied into their slots by the host threads, below
+  // the host arrays (alive until the end of prepare) likewise: a slot now,
+  // the copy on the host threads with the images'
+  struct CopyJob { size_t off; const void* src; size_t bytes; };
+  std::vector<CopyJob> jobs;
+  auto defer = [&](const auto& v) {
+    const size_t bytes = v.size() * sizeof(v[0]);
+    const size_t off = P.slot(bytes);
+    if (bytes) jobs.push_back(CopyJob{off, v.data(), bytes});
+    return off;
+  };
+  std::vector<int> dev_groups, dev_insts;  // representatives and instances set up on the device
   for (int i = 0; i < count; ++i) {
     const Setup& s = S[i];
     Off& k = pk[i];
@@ -587,21 +616,35 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
       // where a record may use it (forced, or no image fits; with an image
       // the automatic choice never streams)
       k.stream = k.res == 0 || grp_stream[grp[i]];
-      k.soc = put_vec(P, s.soc_ptr);
-      k.D = put_vec(P, s.D);
-      k.E = put_vec(P, s.E);
+      k.soc = defer(s.soc_ptr);
+      // device setup: D, E and the values are written by k_setup_group
+      // (batch_setup.hip) from the caller's values, uploaded below; D and E
+      // get their slots after this loop, all in one range (one download)
+      if (!s.dev) {
+        k.D = defer(s.D);
+        k.E = defer(s.E);
+      }
       if (k.stream) {
-        k.rp = put_vec(P, s.rp);
-        k.ri = put_vec(P, s.ri);
-        k.rx = put_vals(P, s.rx, fp64);
-        k.cp = put_vec(P, s.cp);
-        k.ci = put_vec(P, s.ci);
-        k.cx = put_vals(P, s.cx, fp64);
+        k.rp = defer(s.rp);
+        k.ri = defer(s.ri);
+        k.rx = s.dev ? P.zeros((size_t)s.nnz() * tsz) : put_vals(P, s.rx, fp64);
+        k.cp = defer(s.cp);
+        k.ci = defer(s.ci);
+        k.cx = s.dev ? P.zeros((size_t)s.nnz() * tsz) : put_vals(P, s.cx, fp64);
         const std::vector<int> rb = wg_row_blocks(s.rp), cb = wg_row_blocks(s.cp);
         k.rb = put_vec(P, rb);
         k.cb = put_vec(P, cb);
         k.nrb = (int)rb.size() / 4;
         k.ncb = (int)cb.size() / 4;
+      }
+      if (s.dev) {
+        k.dax = defer(s.cx);
+        k.dcp = defer(s.cp);
+        k.dci = k.res ? 0 : defer(s.ci);  // with an image its column row indices serve
+        k.drp = defer(s.rp);
+        k.dsrc = defer(s.csr_src);
+        k.dE0 = P.zeros(sizeof(double) * s.n);
+        dev_groups.push_back(i);
       }
     }
     vlds[i].x = wg_lds_bytes(s.n, s.m, s.n + s.m, direct, tsz);
@@ -610,11 +653,12 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
       vlds[i].y = k.lay.total <= lds_cap ? k.lay.total : 0;
       vlds[i].z = k.lay.total - k.lay.img;
     }
-    k.b = put_vec(P, s.b);
-    k.c = put_vec(P, s.c);
-    k.x0 = put_vec(P, s.x);
-    k.s0 = put_vec(P, s.s);
-    k.y0 = put_vec(P, s.y);
+    if (s.dev) dev_insts.push_back(i);
+    k.b = defer(s.b);
+    k.c = defer(s.c);
+    k.x0 = defer(s.x);
+    k.s0 = defer(s.s);
+    k.y0 = defer(s.y);
     xo[i] = P.zeros(sizeof(double) * s.n);
     so[i] = P.zeros(sizeof(double) * s.m);
     yo[i] = P.zeros(sizeof(double) * s.m);
@@ -624,20 +668,32 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
     }
     if (direct && rep[grp[i]] == i && k.stream) {
       const Factor& f = F[grp[i]];
-      k.perm = put_vec(P, f.P);
-      k.lrp = put_vec(P, f.Rp);
-      k.lri = put_vec(P, f.Ri);
-      k.lrx = put_vec(P, f.Rx);
-      k.lcp = put_vec(P, f.Lp);
-      k.lci = put_vec(P, f.Li);
-      k.lcx = put_vec(P, f.Lx);
-      k.ld = put_vec(P, f.Ld);
-      k.flp = put_vec(P, f.fl_ptr);
-      k.flr = put_vec(P, f.fl_rows);
-      k.blp = put_vec(P, f.bl_ptr);
-      k.blr = put_vec(P, f.bl_rows);
+      k.perm = defer(f.P);
+      k.lrp = defer(f.Rp);
+      k.lri = defer(f.Ri);
+      k.lrx = defer(f.Rx);
+      k.lcp = defer(f.Lp);
+      k.lci = defer(f.Li);
+      k.lcx = defer(f.Lx);
+      k.ld = defer(f.Ld);
+      k.flp = defer(f.fl_ptr);
+      k.flr = defer(f.fl_rows);
+      k.blp = defer(f.bl_ptr);
+      k.blr = defer(f.bl_rows);
     }
   }
+  // device setup: every group's D and E in one range, downloaded in one copy
+  const size_t scales_lo = (P.size() + 255) & ~size_t(255);
+  for (int i : dev_groups) {
+    pk[i].D = P.zeros(sizeof(double) * S[i].m);
+    pk[i].E = P.zeros(sizeof(double) * S[i].n);
+  }
+  const size_t scales_hi = P.size();
+  for (int i : dev_insts)
+    if (rep[grp[i]] != i) {
+      pk[i].D = pk[rep[grp[i]]].D;
+      pk[i].E = pk[rep[grp[i]]].E;
+    }
   if (count != 1 || is_big[0]) {
     for (int i = 0; i < count; ++i)  // per row: 5 residuals, then (indirect) 10 x {CG steps, |q|}
       if (want_hist && par[i].VERBOSE && !is_big[i]) ho[i] = P.zeros(sizeof(double) * (direct ? 5 : 25) * verbose_hist_rows(par[i].MAX_ITERS));
@@ -646,15 +702,29 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
   queue_off = P.zeros(sizeof(int) * kClasses);
   inst_off = P.zeros(sizeof(WgInst) * count);
   cinst_off = P.zeros(sizeof(WgInst) * count);
+  const size_t sg_off = P.zeros(sizeof(SetupGroup) * dev_groups.size());
+  const size_t si_off = P.zeros(sizeof(SetupInst) * dev_insts.size());
+  const size_t sout_off = P.zeros(sizeof(double) * 3 * dev_insts.size());
   bytes = P.size();
   {
+    // largest first, dealt round-robin into one contiguous run per thread
+    // (parallel_for gives each thread a contiguous share), so the threads
+    // copy about the same bytes
+    for (int i : img_jobs) jobs.push_back(CopyJob{pk[i].img, imgs[i].data(), imgs[i].size()});
+    std::stable_sort(jobs.begin(), jobs.end(), [](const CopyJob& a, const CopyJob& b) { return a.bytes > b.bytes; });
+    const int nj = (int)jobs.size();
+    const int nt = std::min(host_threads(), std::max(1, nj / 4));
+    std::vector<CopyJob> dealt;
+    dealt.reserve(nj);
+    for (int t = 0; t < nt; ++t)
+      for (int q = t; q < nj; q += nt) dealt.push_back(jobs[q]);
     std::string cerr;
-    parallel_for((int)img_jobs.size(), [&](int q, std::string&) {
-      const int i = img_jobs[q];
-      std::memcpy(P.data() + pk[i].img, imgs[i].data(), imgs[i].size());
-      std::vector<unsigned char>().swap(imgs[i]);
+    parallel_for(nj, [&](int q, std::string&) {
+      const CopyJob& j = dealt[q];
+      std::memcpy(P.data() + j.off, j.src, j.bytes);
       return true;
     }, cerr);
+    for (int i : img_jobs) std::vector<unsigned char>().swap(imgs[i]);
   }
   if (borrow) {
     if (!C.reserve(bytes, bytes, err)) return false;
@@ -734,6 +804,68 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
   std::memcpy(P.data() + inst_off, insts.data(), sizeof(WgInst) * count);
   for (size_t k = 0; k < order.size(); ++k)
     std::memcpy(P.data() + cinst_off + sizeof(WgInst) * k, &insts[order[k]], sizeof(WgInst));
+  // device setup records: per representative its arrays and where D, E and
+  // the values go (the image, and the streaming copies where packed); per
+  // instance its vectors, its group's D and E0 and its two records
+  if (!dev_groups.empty()) {
+    std::vector<SetupGroup> G(dev_groups.size());
+    for (size_t q = 0; q < dev_groups.size(); ++q) {
+      const int i = dev_groups[q];
+      const Setup& s = S[i];
+      const Off& k = pk[i];
+      SetupGroup& g = G[q];
+      std::memset(&g, 0, sizeof(g));
+      g.n = s.n; g.m = s.m; g.nnz = (int)s.nnz(); g.normalize = s.normalize; g.nsoc = s.nsoc();
+      g.cp = reinterpret_cast<const int*>(base + k.dcp);
+      g.ci = k.res ? nullptr : reinterpret_cast<const int*>(base + k.dci);
+      g.ax = reinterpret_cast<double*>(base + k.dax);
+      g.rp = reinterpret_cast<const int*>(base + k.drp);
+      g.csr_src = reinterpret_cast<const int*>(base + k.dsrc);
+      g.soc = reinterpret_cast<const int*>(base + k.soc);
+      g.E0 = reinterpret_cast<double*>(base + k.dE0);
+      if (k.res) {
+        g.D = reinterpret_cast<double*>(base + k.img + k.lay.D);
+        g.E = reinterpret_cast<double*>(base + k.img + k.lay.E);
+        g.D2 = reinterpret_cast<double*>(base + k.D);
+        g.E2 = reinterpret_cast<double*>(base + k.E);
+        g.rval = base + k.img + k.lay.rval;
+        g.cval = base + k.img + k.lay.cval;
+        g.rpos = reinterpret_cast<const int4*>(base + k.img + k.lay.rpos);
+        g.cpos = reinterpret_cast<const int4*>(base + k.img + k.lay.cpos);
+        g.cidx = reinterpret_cast<const int*>(base + k.img + k.lay.cidx);
+      } else {
+        g.D = reinterpret_cast<double*>(base + k.D);
+        g.E = reinterpret_cast<double*>(base + k.E);
+      }
+      if (k.stream) {
+        g.srx = base + k.rx;
+        g.scx = base + k.cx;
+      }
+    }
+    std::memcpy(P.data() + sg_off, G.data(), sizeof(SetupGroup) * G.size());
+  }
+  if (!dev_insts.empty()) {
+    std::vector<int> cls_pos(count, 0);
+    for (size_t q = 0; q < order.size(); ++q) cls_pos[order[q]] = (int)q;
+    std::vector<SetupInst> I(dev_insts.size());
+    for (size_t q = 0; q < dev_insts.size(); ++q) {
+      const int i = dev_insts[q];
+      const Off& k = pk[i];
+      const Off& kr = pk[rep[grp[i]]];
+      SetupInst& t = I[q];
+      std::memset(&t, 0, sizeof(t));
+      t.n = S[i].n; t.m = S[i].m; t.rec = i; t.cls = cls_pos[i];
+      t.b = reinterpret_cast<double*>(base + k.b);
+      t.c = reinterpret_cast<double*>(base + k.c);
+      t.x = reinterpret_cast<double*>(base + k.x0);
+      t.s = reinterpret_cast<double*>(base + k.s0);
+      t.y = reinterpret_cast<double*>(base + k.y0);
+      t.D = reinterpret_cast<const double*>(base + kr.D);
+      t.E0 = reinterpret_cast<const double*>(base + kr.dE0);
+      t.out = reinterpret_cast<double*>(base + sout_off) + 3 * q;
+    }
+    std::memcpy(P.data() + si_off, I.data(), sizeof(SetupInst) * I.size());
+  }
   hipError_t e;
   if (borrow) {
     // from the context's pinned buffer, which no other work of this thread
@@ -750,6 +882,44 @@ bool WgBatch::prepare(int device_, bool fp64_, bool direct_, bool borrow, std::s
     if (e == hipSuccess && !ev1) e = hipEventCreate(&ev1);
   }
   if (e != hipSuccess) { err = std::string("upload: ") + hipGetErrorString(e); return false; }
+  if (!dev_groups.empty() || !dev_insts.empty()) {
+    // the setup kernels, then D, E (with the x1e3) per group and lambda and
+    // the stop-test scales per instance back to the host records (the
+    // unscaling at fetch and later uploads of the records use them)
+    e = batch_setup_launch(reinterpret_cast<const SetupGroup*>(base + sg_off), (int)dev_groups.size(),
+                           reinterpret_cast<const SetupInst*>(base + si_off), (int)dev_insts.size(),
+                           reinterpret_cast<WgInst*>(base + inst_off), reinterpret_cast<WgInst*>(base + cinst_off),
+                           fp64, stream);
+    std::vector<double> outs(3 * dev_insts.size());
+    std::vector<unsigned char> sc(scales_hi - scales_lo);
+    if (e == hipSuccess && !sc.empty())
+      e = hipMemcpyAsync(sc.data(), base + scales_lo, sc.size(), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess && !outs.empty())
+      e = hipMemcpyAsync(outs.data(), base + sout_off, sizeof(double) * outs.size(), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) { err = std::string("device setup: ") + hipGetErrorString(e); return false; }
+    for (int i : dev_groups) {
+      Setup& s = S[i];
+      const double* d = reinterpret_cast<const double*>(sc.data() + (pk[i].D - scales_lo));
+      const double* ee = reinterpret_cast<const double*>(sc.data() + (pk[i].E - scales_lo));
+      s.D.assign(d, d + s.m);
+      s.E.assign(ee, ee + s.n);
+    }
+    for (size_t q = 0; q < dev_insts.size(); ++q) {
+      const int i = dev_insts[q];
+      Setup& s = S[i];
+      s.lambda = outs[3 * q];
+      s.pscale = outs[3 * q + 1];
+      s.dscale = outs[3 * q + 2];
+      insts[i].lambda = s.lambda;
+      insts[i].pscale = s.pscale;
+      insts[i].dscale = s.dscale;
+      if (rep[grp[i]] != i) {
+        s.D = S[rep[grp[i]]].D;
+        s.E = S[rep[grp[i]]].E;
+      }
+    }
+  }
   gsol.clear();
   for (int i : big) {  // own device memory: the batch may hold the thread's arena
     Setup full;
@@ -969,7 +1139,7 @@ bool WgBatch::tune(int dev, bool fp64_, std::string& err) {
             const Setup& r = S[rep[grp[i]]];
             if (rep[grp[i]] != i) {
               T->S[k].cp = r.cp; T->S[k].ci = r.ci; T->S[k].cx = r.cx;
-              T->S[k].rp = r.rp; T->S[k].ri = r.ri; T->S[k].rx = r.rx;
+              T->S[k].rp = r.rp; T->S[k].ri = r.ri; T->S[k].rx = r.rx; T->S[k].csr_src = r.csr_src;
             }
             if (direct) T->F.push_back(F[grp[i]]);
           } else {  // the matrix arrays live with the group's first sample
@@ -979,6 +1149,7 @@ bool WgBatch::tune(int dev, bool fp64_, std::string& err) {
             t.soc_ptr = a.soc_ptr; t.b = a.b; t.c = a.c; t.D = a.D; t.E = a.E; t.E0 = a.E0;
             t.lambda = a.lambda; t.pscale = a.pscale; t.dscale = a.dscale;
             t.x = a.x; t.s = a.s; t.y = a.y;
+            t.dev = a.dev; t.normalize = a.normalize;
           }
           T->grp[k] = g->second;
           Params q = par[i];
@@ -1440,9 +1611,27 @@ static PdosBatchHandle* batch_prepare(const Data* const* ds, const Cone* const* 
   h->B.par.resize(count);
   h->B.direct = direct;
   std::string err;
+  // the setup on the device (batch_setup.hip) for the indirect plugin's
+  // instances that the workgroup engine runs, with the reference scaling;
+  // the direct plugin factors the scaled matrix on the host, and members
+  // past one workgroup's LDS run on the whole-GPU engine from a host setup.
+  // PDOS_HOST_SETUP=1 keeps every setup on the host.
+  bool dev_setup = !direct && o.equil == PDOS_EQUIL_REFERENCE;
+  if (const char* hs = std::getenv("PDOS_HOST_SETUP")) dev_setup = dev_setup && std::atoi(hs) == 0;
+  int lds_cap = 0;
+  if (dev_setup) {
+    Ctx& C = ctx();
+    if (!C.ensure(o.device, err)) { g_err = err; delete h; return nullptr; }
+    lds_cap = C.max_lds - 64;
+  }
+  const int tsz = o.precision == PDOS_FP64 ? 8 : 4;
   double tp = now_ms();
   const int bad = parallel_for(count, [&](int i, std::string& e) {
-    if (!ds[i] || !ks[i] || !build_setup(ds[i], ks[i], h->B.S[i], e, o.equil)) return false;
+    if (!ds[i] || !ks[i]) return false;
+    const bool raw = dev_setup && ds[i]->n > 0 && ds[i]->m > 0 &&
+                     wg_lds_bytes((int)ds[i]->n, (int)ds[i]->m, (int)(ds[i]->n + ds[i]->m), false, tsz) <= lds_cap;
+    if (!(raw ? build_setup_raw(ds[i], ks[i], h->B.S[i], e) : build_setup(ds[i], ks[i], h->B.S[i], e, o.equil)))
+      return false;
     h->B.par[i] = *ds[i]->p;
     return true;
   }, err);
@@ -1529,6 +1718,15 @@ int pdos_batch_prepare_times(const PdosBatchHandle* h, double* ms) {
   if (!h || !ms) return -1;
   for (int k = 0; k < 4; ++k) ms[k] = h->part_ms[k];
   return 0;
+}
+
+int pdos_batch_scales(const PdosBatchHandle* h, int i, double* D, double* E, double* lambda) {
+  if (!h || i < 0 || i >= (int)h->B.S.size()) return -1;
+  const Setup& s = h->B.S[i];
+  if (D) std::copy(s.D.begin(), s.D.end(), D);
+  if (E) std::copy(s.E.begin(), s.E.end(), E);
+  if (lambda) *lambda = s.lambda;
+  return s.dev ? 1 : 0;
 }
 
 int pdos_batch_set_warm_dev(PdosBatchHandle* h, const float* dW, long ld, int check) {

@@ -93,6 +93,43 @@ struct WgInst {
   ResLayout lay;
 };
 
+// Setup on the device (batch_setup.hip): one record per distinct matrix of a
+// batch and one per instance.  All pointers are device pointers into the
+// batch's buffer.
+struct SetupGroup {
+  int n, m, nnz, normalize, nsoc, pad;
+  const int* cp;           // CSC column pointers
+  const int* ci;           // CSC row indices (no image), or null
+  const int* cidx;         // the image's row indices of A's columns (image), or null
+  double* ax;              // the caller's values in CSC order, scaled in place (with the x1e3)
+  const int* rp;           // row pattern of A
+  const int* csr_src;      // CSC entry of each row entry
+  const int* soc;          // nsoc + 1 absolute row offsets
+  double* D;               // the image's D (m) and E (n, with the x1e3)
+  double* E;
+  double* E0;              // E before the x1e3 (n)
+  double* D2;              // the streaming variant's D and E, or null
+  double* E2;
+  void* rval;              // image values of A's rows and columns (bulk type), or null (no image)
+  void* cval;
+  const int4* rpos;        // the image's positions {row, e0, e1, 0} of A's m rows and n columns
+  const int4* cpos;
+  void* srx;               // the streaming variant's values (bulk type) in row and column order, or null
+  void* scx;
+};
+
+struct SetupInst {
+  int n, m, rec, cls;      // record in instance order and in class order
+  double* b;               // the caller's b, c and warm start, scaled in place
+  double* c;
+  double* x;
+  double* s;
+  double* y;
+  const double* D;         // the group's D and E0
+  const double* E0;
+  double* out;             // lambda, pscale, dscale
+};
+
 // Rows of the VERBOSE residual history for a MAX_ITERS: one per tenth check
 // (pdos.c:107 prints checks i % 10 == 0), at most 100,000.
 inline int verbose_hist_rows(long max_iters) {

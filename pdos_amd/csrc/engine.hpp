@@ -64,6 +64,11 @@ hipError_t dense_at_pair(hipStream_t st, int m, int n, int ldb, int m64, const _
                          const float* Y, const float* W, float* Up, float* Vp, size_t plane, int splits);
 hipError_t dense_a(hipStream_t st, int m, int n, int ldb, int n64, const __bf16* ARh, const __bf16* ARl,
                    const float* X, float* Z);
+// Setup of a batch's instances on the device (batch_setup.hip): groups, then
+// instances, on stream st; rec / rec_cls are the instance records in instance
+// and class order (lambda and the stop-test scales are written into both).
+hipError_t batch_setup_launch(const SetupGroup* d_groups, int ngroups, const SetupInst* d_insts, int ninsts,
+                              WgInst* rec, WgInst* rec_cls, bool fp64, hipStream_t st);
 hipError_t wg_launch(const WgInst* d_insts, int count, int lds_bytes, int threads, bool fp64, bool direct, int* queue,
                      int grid, hipStream_t st);
 

@@ -23,6 +23,13 @@ struct Setup {
   std::vector<double> E0;          // E before the x1e3 (1 without NORMALIZE)
   double lambda = 0, pscale = 0, dscale = 0;
   std::vector<double> x, s, y;     // scaled warm start (zeros if absent)
+  // Setup on the device (build_setup_raw): cx, b, c, x, s, y hold the
+  // caller's unscaled values, rx is empty, csr_src maps each entry of the
+  // row form (rp, ri) to its CSC entry, and D, E, E0, lambda, pscale and
+  // dscale are formed by the device (batch_setup.hip) and filled in after.
+  bool dev = false;
+  int normalize = 0;
+  std::vector<int> csr_src;
   int nsoc() const { return (int)soc_ptr.size() - 1; }
   long nnz() const { return (long)cx.size(); }
 };
@@ -32,6 +39,11 @@ struct Setup {
 // indices out of range, sizes beyond int32).  `equil` selects the scaling:
 // 0 common.h:35-218, 1 ruiz_equil_common.h:98-322.
 bool build_setup(const Data* d, const Cone* k, Setup& S, std::string& err, int equil = 0);
+
+// The same checks and the caller's values unscaled, with the row pattern of
+// A and its map to the CSC entries (the stable transpose's order), for the
+// scaling on the device (common.h:69-215 there, bit for bit).
+bool build_setup_raw(const Data* d, const Cone* k, Setup& S, std::string& err);
 
 // LDL' factorization of the permuted quasi-definite KKT [[-I, A'], [A, I]].
 struct Factor {

@@ -98,7 +98,9 @@ void ruiz_scale(const Data* d, const Cone* k, Setup& S) {
 }
 }  // namespace
 
-bool build_setup(const Data* d, const Cone* k, Setup& S, std::string& err, int equil) {
+// Checks the caller's data and copies it into S unscaled (soc offsets, the CSC
+// arrays, b, c and the warm start, zeros where absent).
+static bool load_input(const Data* d, const Cone* k, Setup& S, std::string& err) {
   if (!d || !k) { err = "NULL data or cone"; return false; }
   if (!d->p) { err = "NULL params"; return false; }
   if (d->n < 1 || d->m < 1) { err = "n and m must be positive"; return false; }
@@ -138,6 +140,12 @@ bool build_setup(const Data* d, const Cone* k, Setup& S, std::string& err, int e
   if (d->x) std::memcpy(S.x.data(), d->x, sizeof(double) * n);
   if (d->s) std::memcpy(S.s.data(), d->s, sizeof(double) * m);
   if (d->y) std::memcpy(S.y.data(), d->y, sizeof(double) * m);
+  return true;
+}
+
+bool build_setup(const Data* d, const Cone* k, Setup& S, std::string& err, int equil) {
+  if (!load_input(d, k, S, err)) return false;
+  const long n = d->n, m = d->m, nz = d->Ap[n];
   S.D.assign(m, 0.0);
   S.E.assign(n, 0.0);
 
@@ -204,6 +212,29 @@ bool build_setup(const Data* d, const Cone* k, Setup& S, std::string& err, int e
       const int q = next[S.ci[p]]++;
       S.ri[q] = (int)j;
       S.rx[q] = S.cx[p];
+    }
+  return true;
+}
+
+bool build_setup_raw(const Data* d, const Cone* k, Setup& S, std::string& err) {
+  if (!load_input(d, k, S, err)) return false;
+  const long n = d->n, m = d->m, nz = d->Ap[n];
+  S.dev = true;
+  S.normalize = d->p->NORMALIZE ? 1 : 0;
+  // the row pattern of A by the stable counting sort of build_setup, with the
+  // CSC entry each row entry comes from
+  S.rp.assign(m + 1, 0);
+  S.ri.resize(nz);
+  S.csr_src.resize(nz);
+  std::vector<int> next(m + 1, 0);
+  for (long p = 0; p < nz; ++p) next[S.ci[p] + 1]++;
+  for (long i = 0; i < m; ++i) next[i + 1] += next[i];
+  std::copy(next.begin(), next.end(), S.rp.begin());
+  for (long j = 0; j < n; ++j)
+    for (int p = S.cp[j]; p < S.cp[j + 1]; ++p) {
+      const int q = next[S.ci[p]]++;
+      S.ri[q] = (int)j;
+      S.csr_src[q] = p;
     }
   return true;
 }
