@@ -190,8 +190,12 @@ __global__ void k_loss_grad(int rows, int cols, int ld, const float* __restrict_
   const long total = (long)rows_pad * ld;
   const float scale = 2.0f / (float)((double)rows * cols);
   double acc = 0.0;
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
-    const int r = (int)(e / ld), c = (int)(e % ld);
+  // (r, c) of e, advanced by the stride's row and column steps (no 64-bit
+  // division per element)
+  const long e0 = blockIdx.x * (long)blockDim.x + threadIdx.x, stride = (long)gridDim.x * blockDim.x;
+  int r = (int)(e0 / ld), c = (int)(e0 % ld);
+  const int sr = (int)(stride / ld), sc = (int)(stride % ld);
+  for (long e = e0; e < total; e += stride) {
     float g = 0.f;
     if (r < rows && c < cols) {
       const float d = Y[e] - T[(size_t)r * ldt + c];
@@ -200,6 +204,12 @@ __global__ void k_loss_grad(int rows, int cols, int ld, const float* __restrict_
     }
     dYf[e] = g;
     dYb[e] = (__bf16)g;
+    r += sr;
+    c += sc;
+    if (c >= ld) {
+      c -= ld;
+      ++r;
+    }
   }
   for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
@@ -235,10 +245,18 @@ __global__ __launch_bounds__(512) void k_colsum(int rows, int cols, int ld, cons
   }
 }
 
-// Adam (binary32 master, bias-corrected), then the bf16 mirror.
+// The bf16 mirrors of the weight matrices: master entries [o[k], o[k] + len[k])
+// are mirrored at p[k].
+struct BfMirror {
+  long o[3], len[3];
+  __bf16* p[3];
+};
+
+// Adam (binary32 master, bias-corrected), with the updated weights written to
+// their bf16 mirrors.
 __global__ void k_adam(long n, float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
-                       float* __restrict__ v, __bf16* __restrict__ wb, float lr, float b1, float b2,
-                       float c1, float c2, float eps) {
+                       float* __restrict__ v, BfMirror mir, float lr, float b1, float b2, float c1, float c2,
+                       float eps) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     const float gi = g[i];
     const float mi = b1 * m[i] + (1.f - b1) * gi;
@@ -247,7 +265,9 @@ __global__ void k_adam(long n, float* __restrict__ w, const float* __restrict__ 
     v[i] = vi;
     const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
     w[i] = wi;
-    if (wb) wb[i] = (__bf16)wi;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      if ((unsigned long)(i - mir.o[k]) < (unsigned long)mir.len[k]) mir.p[k][i - mir.o[k]] = (__bf16)wi;
   }
 }
 
@@ -255,9 +275,17 @@ __global__ void k_adam(long n, float* __restrict__ w, const float* __restrict__ 
 __global__ void k_gather_rows(int rows, int cols, const int* __restrict__ idx, const float* __restrict__ src,
                               float* __restrict__ dst) {
   const long total = (long)rows * cols;
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
-    const int r = (int)(e / cols), c = (int)(e % cols);
+  const long e0 = blockIdx.x * (long)blockDim.x + threadIdx.x, stride = (long)gridDim.x * blockDim.x;
+  int r = (int)(e0 / cols), c = (int)(e0 % cols);
+  const int sr = (int)(stride / cols), sc = (int)(stride % cols);
+  for (long e = e0; e < total; e += stride) {
     dst[e] = src[(size_t)idx[r] * cols + c];
+    r += sr;
+    c += sc;
+    if (c >= cols) {
+      c -= cols;
+      ++r;
+    }
   }
 }
 
@@ -278,6 +306,16 @@ hipError_t gemm(hipStream_t st, int M, int N, int K, const __bf16* A, int lda, c
   hipLaunchKernelGGL((k_gemm<TA, TB, EPI, KB>), dim3(N / BN, M / BM), dim3(GT), 0, st, M, N, K, A, lda, B, ldb,
                      Cf, Cb, ldc, bias, aux, omul, oadd);
   return hipGetLastError();
+}
+
+// gemm with 128-wide K steps where K allows (half the dependent load round
+// trips of the 64-wide steps; the MLP products are latency-bound tiles).
+template <bool TA, bool TB, int EPI>
+hipError_t gemm_k(hipStream_t st, int M, int N, int K, const __bf16* A, int lda, const __bf16* B, int ldb,
+                  float* Cf, __bf16* Cb, int ldc, const float* bias = nullptr, const float* aux = nullptr,
+                  const float* omul = nullptr, const float* oadd = nullptr) {
+  if (K % 128 == 0) return gemm<TA, TB, EPI, 128>(st, M, N, K, A, lda, B, ldb, Cf, Cb, ldc, bias, aux, omul, oadd);
+  return gemm<TA, TB, EPI>(st, M, N, K, A, lda, B, ldb, Cf, Cb, ldc, bias, aux, omul, oadd);
 }
 
 }  // namespace
@@ -366,41 +404,55 @@ struct Mlp {
                     __bf16* h2, float* y) {
     const int rp = pad64(rows);
     hipLaunchKernelGGL(k_prep_input, dim3(512), dim3(256), 0, st, rows, nin, nin, X, in_mu, in_sd, pin, rp, xb);
-    note(gemm<false, false, EPI_BIAS_GELU>(st, rp, ph, pin, xb, pin, W1b, ph, z1, h1, ph, b1));
-    note(gemm<false, false, EPI_BIAS_GELU>(st, rp, ph, ph, h1, ph, W2b, ph, z2, h2, ph, b2));
-    note(gemm<false, false, EPI_BIAS_AFFINE>(st, rp, pout, ph, h2, ph, W3b, pout, y, nullptr, pout, b3, nullptr,
+    note(gemm_k<false, false, EPI_BIAS_GELU>(st, rp, ph, pin, xb, pin, W1b, ph, z1, h1, ph, b1));
+    note(gemm_k<false, false, EPI_BIAS_GELU>(st, rp, ph, ph, h1, ph, W2b, ph, z2, h2, ph, b2));
+    note(gemm_k<false, false, EPI_BIAS_AFFINE>(st, rp, pout, ph, h2, ph, W3b, pout, y, nullptr, pout, b3, nullptr,
                                         affine_out ? out_mul : nullptr, affine_out ? out_add : nullptr));
   }
-  // loss and parameter gradients into g for the rows in Xf / T
+  // loss and parameter gradients into g for the rows in Xf / T; the loss
+  // (mean square error) is returned after one synchronization
+  static constexpr int kLossBlocks = 256;
   double backward(int rows) {
+    backward_async(rows, part);
+    std::vector<double> hp(kLossBlocks);
+    (void)hipMemcpyAsync(hp.data(), part, kLossBlocks * sizeof(double), hipMemcpyDeviceToHost, st);
+    (void)hipStreamSynchronize(st);
+    return loss_of(hp.data(), rows);
+  }
+  // the loss from k_loss_grad's per-block partials, summed in block order
+  double loss_of(const double* hp, int rows) const {
+    double s = 0.0;
+    for (int v = 0; v < kLossBlocks; ++v) s += hp[v];
+    return s / ((double)rows * nout);
+  }
+  // the same without waiting: the loss partials go to `partial`
+  // (kLossBlocks doubles), the gradients into g
+  void backward_async(int rows, double* partial) {
     const int rp = pad64(rows);
-    const int nb = 256;
-    hipLaunchKernelGGL(k_loss_grad, dim3(nb), dim3(GT), 0, st, rows, nout, pout, Y, T, nout, rp, dYf, dYb, part);
+    const int nb = kLossBlocks;
+    hipLaunchKernelGGL(k_loss_grad, dim3(nb), dim3(GT), 0, st, rows, nout, pout, Y, T, nout, rp, dYf, dYb, partial);
     float* gW1 = g + off_W1; float* gb1 = g + off_b1;
     float* gW2 = g + off_W2; float* gb2 = g + off_b2;
     float* gW3 = g + off_W3; float* gb3 = g + off_b3;
-    note(gemm<true, false, EPI_F32>(st, ph, pout, rp, H2b, ph, dYb, pout, gW3, nullptr, pout));
+    note(gemm_k<true, false, EPI_F32>(st, ph, pout, rp, H2b, ph, dYb, pout, gW3, nullptr, pout));
     hipLaunchKernelGGL(k_colsum<float>, dim3((pout + 63) / 64), dim3(512), 0, st, rp, pout, pout, dYf, gb3);
-    note(gemm<false, true, EPI_GELU_GRAD>(st, rp, ph, pout, dYb, pout, W3b, pout, nullptr, dZ2b, ph, nullptr, Z2));
-    note(gemm<true, false, EPI_F32>(st, ph, ph, rp, H1b, ph, dZ2b, ph, gW2, nullptr, ph));
+    note(gemm_k<false, true, EPI_GELU_GRAD>(st, rp, ph, pout, dYb, pout, W3b, pout, nullptr, dZ2b, ph, nullptr, Z2));
+    note(gemm_k<true, false, EPI_F32>(st, ph, ph, rp, H1b, ph, dZ2b, ph, gW2, nullptr, ph));
     hipLaunchKernelGGL(k_colsum<__bf16>, dim3((ph + 63) / 64), dim3(512), 0, st, rp, ph, ph, dZ2b, gb2);
-    note(gemm<false, true, EPI_GELU_GRAD>(st, rp, ph, ph, dZ2b, ph, W2b, ph, nullptr, dZ1b, ph, nullptr, Z1));
-    note(gemm<true, false, EPI_F32>(st, pin, ph, rp, Xb, pin, dZ1b, ph, gW1, nullptr, ph));
+    note(gemm_k<false, true, EPI_GELU_GRAD>(st, rp, ph, ph, dZ2b, ph, W2b, ph, nullptr, dZ1b, ph, nullptr, Z1));
+    note(gemm_k<true, false, EPI_F32>(st, pin, ph, rp, Xb, pin, dZ1b, ph, gW1, nullptr, ph));
     hipLaunchKernelGGL(k_colsum<__bf16>, dim3((ph + 63) / 64), dim3(512), 0, st, rp, ph, ph, dZ1b, gb1);
-    std::vector<double> hp(nb);
-    (void)hipMemcpyAsync(hp.data(), part, nb * sizeof(double), hipMemcpyDeviceToHost, st);
-    (void)hipStreamSynchronize(st);
-    double s = 0.0;
-    for (double v : hp) s += v;
-    return s / ((double)rows * nout);
   }
   void adam(float lr) {
     ++step;
     const float b1c = 0.9f, b2c = 0.999f;
     const float c1 = 1.f - std::pow(b1c, (float)step), c2 = 1.f - std::pow(b2c, (float)step);
-    hipLaunchKernelGGL(k_adam, dim3(1024), dim3(256), 0, st, nparam, master, g, mom, vel, (__bf16*)nullptr, lr,
-                       b1c, b2c, c1, c2, 1e-8f);
-    refresh_bf16();
+    BfMirror mir;
+    mir.o[0] = (long)off_W1, mir.len[0] = (long)pin * ph, mir.p[0] = W1b;
+    mir.o[1] = (long)off_W2, mir.len[1] = (long)ph * ph, mir.p[1] = W2b;
+    mir.o[2] = (long)off_W3, mir.len[2] = (long)ph * pout, mir.p[2] = W3b;
+    hipLaunchKernelGGL(k_adam, dim3(1024), dim3(256), 0, st, nparam, master, g, mom, vel, mir, lr, b1c, b2c, c1,
+                       c2, 1e-8f);
   }
 };
 
@@ -607,24 +659,49 @@ int pdos_mlp_train(PdosMlp* P, const float* X, const float* T, int rows, int epo
   for (int i = 0; i < rows; ++i) order[i] = i;
   int* dIdx = nullptr;
   if (hipMalloc(&dIdx, (size_t)rows * 4) != hipSuccess) { g_mlp_err = "hipMalloc failed"; return -1; }
+  // An epoch's steps are enqueued without waiting: each step's loss
+  // partials go to its own slot, and the host waits once per epoch (before
+  // the next epoch's indices are copied from the buffer it shuffles into
+  // while the GPU works); the last epoch's partials give the loss returned,
+  // summed as a per-step wait would have.
+  const int steps = rows / batch;
+  double* dPart = nullptr;
+  if (hipMalloc(&dPart, (size_t)std::max(steps, 1) * Mlp::kLossBlocks * sizeof(double)) != hipSuccess) {
+    g_mlp_err = "hipMalloc failed";
+    return -1;
+  }
+  std::vector<int> order2(order);
+  std::vector<int>* ord[2] = {&order, &order2};
   std::mt19937_64 rng(seed);
+  std::shuffle(order.begin(), order.end(), rng);
   double epoch_loss = 0.0;
   for (int e = 0; e < epochs; ++e) {
-    std::shuffle(order.begin(), order.end(), rng);
-    (void)hipMemcpyAsync(dIdx, order.data(), (size_t)rows * 4, hipMemcpyHostToDevice, M.st);
-    double acc = 0.0;
-    int nb = 0;
-    for (int s = 0; s + batch <= rows; s += batch) {
+    std::vector<int>& cur = *ord[e & 1];
+    (void)hipMemcpyAsync(dIdx, cur.data(), (size_t)rows * 4, hipMemcpyHostToDevice, M.st);
+    for (int q = 0; q < steps; ++q) {
+      const int s = q * batch;
       hipLaunchKernelGGL(pdos_amd::k_gather_rows, dim3(512), dim3(256), 0, M.st, batch, M.nin, dIdx + s, dX, M.Xf);
       hipLaunchKernelGGL(pdos_amd::k_gather_rows, dim3(1024), dim3(256), 0, M.st, batch, M.nout, dIdx + s, dT, M.T);
       M.forward(batch, false);
-      acc += M.backward(batch);
+      M.backward_async(batch, dPart + (size_t)q * Mlp::kLossBlocks);
       M.adam(lr);
-      ++nb;
     }
-    epoch_loss = nb ? acc / nb : 0.0;
+    if (e + 1 < epochs) {  // the next epoch's order, shuffled from this one's as in place
+      std::vector<int>& nxt = *ord[(e + 1) & 1];
+      nxt = cur;
+      std::shuffle(nxt.begin(), nxt.end(), rng);
+    }
+    (void)hipStreamSynchronize(M.st);
+  }
+  if (epochs > 0 && steps > 0) {
+    std::vector<double> hp((size_t)steps * Mlp::kLossBlocks);
+    (void)hipMemcpy(hp.data(), dPart, hp.size() * sizeof(double), hipMemcpyDeviceToHost);
+    double acc = 0.0;
+    for (int q = 0; q < steps; ++q) acc += M.loss_of(hp.data() + (size_t)q * Mlp::kLossBlocks, batch);
+    epoch_loss = acc / steps;
   }
   (void)hipStreamSynchronize(M.st);
+  (void)hipFree(dPart);
   (void)hipFree(dX);
   (void)hipFree(dT);
   (void)hipFree(dIdx);
