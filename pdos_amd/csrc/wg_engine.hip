@@ -10,7 +10,9 @@
 // Per ADMM iteration (pdos.c:94-120) with k CG steps the workgroup runs
 // k + 1 SpMVs with A and k + 2 with A' (the A x row sums of the final
 // s~ = b - A x are reused by the next iteration's residual and CG start,
-// which is exact: x does not change in between), and 4 + 3k barriers.
+// which is exact: x does not change in between), and 4 + 3k barriers (4 + 2k
+// when n <= 128: each wave then keeps all of x, q and p in registers and
+// forms the CG step's dot products and updates itself).
 //
 // Reference map:
 //   residual check      linAlg.h:158-201, pdos.c:95-107
@@ -841,7 +843,66 @@ __device__ __forceinline__ void wg_res_body(const WgInst* __restrict__ insts, in
       T beta = T(0);
       int k = 0;
       double qlast = 0.0;
-      if (rho > I.cg_tol_sq) {
+      if (rho > I.cg_tol_sq && n <= 2 * 64) {
+        // n of at most 128: every wave holds all of x, q and p in registers
+        // (lane l rows l and l + 64) and forms the two dot products and the
+        // vector updates itself, so a step has two barriers (after the A and
+        // the A' product) and no cross-wave sum.  The waves compute the same
+        // values in the same order, so they agree on every decision, and p
+        // (the next A product's input) is written by all of them with equal
+        // values: a wave reads back its own or an equal write, no barrier.
+        T xr[2], qr[2], pr[2];
+#pragma unroll
+        for (int uu = 0; uu < 2; ++uu) {
+          const int i = lane + 64 * uu;
+          xr[uu] = i < n ? x[i] : T(0);
+          qr[uu] = i < n ? q[i] : T(0);
+          pr[uu] = qr[uu];  // beta p + q with p = 0
+          if (i < n) p[i] = pr[uu];
+        }
+        while (k < I.cg_max_its) {
+          spmv_res<T>(rpos, ridx, rval, rbat, nrbat, [&](int c) { return p[c]; }, [&](int j, T v) { t[j] = v; });
+          __syncthreads();
+          spmv_res<T>(cpos, cidx, cval, cbat, ncbat, [&](int c) { return t[c]; }, [&](int i, T v) { u[i] = v; });
+          __syncthreads();
+          T ur[2];
+          double r2 = 0.0;
+#pragma unroll
+          for (int uu = 0; uu < 2; ++uu) {
+            const int i = lane + 64 * uu;
+            ur[uu] = i < n ? pr[uu] + u[i] : T(0);
+            r2 += double(pr[uu]) * double(ur[uu]);
+          }
+          r2 = wave_total(r2);
+          const T alpha = T(rho / r2);
+          double r3 = 0.0;
+#pragma unroll
+          for (int uu = 0; uu < 2; ++uu) {
+            xr[uu] += alpha * pr[uu];
+            qr[uu] = qr[uu] - alpha * ur[uu];
+            r3 += double(qr[uu]) * double(qr[uu]);
+          }
+          r3 = wave_total(r3);
+          ++k;
+          qlast = r3;
+          if (r3 < I.cg_tol_sq) break;
+          beta = T(r3 / rho);
+          rho = r3;
+#pragma unroll
+          for (int uu = 0; uu < 2; ++uu) {
+            const int i = lane + 64 * uu;
+            pr[uu] = beta * pr[uu] + qr[uu];
+            if (i < n) p[i] = pr[uu];
+          }
+        }
+        // x as every wave holds it: each wave writes all of it (equal
+        // values), so its later reads see its own write or an equal one
+#pragma unroll
+        for (int uu = 0; uu < 2; ++uu) {
+          const int i = lane + 64 * uu;
+          if (i < n) x[i] = xr[uu];
+        }
+      } else if (rho > I.cg_tol_sq) {
         while (k < I.cg_max_its) {
           spmv_res<T>(rpos, ridx, rval, rbat, nrbat, [&](int c) { return beta * p[c] + q[c]; },
                       [&](int j, T v) { t[j] = v; });
