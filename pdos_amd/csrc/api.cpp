@@ -1302,9 +1302,13 @@ bool WgBatch::run(std::string& err) {
   for (int q = 0; q < ncls && e == hipSuccess; ++q) {
     const int c = cls_seq[q];
     hipStream_t sc = cls_stream[c] == 0 ? stream : xs[cls_stream[c] - 1];
+    // a CU per workgroup (no more workgroups in the whole batch than CUs),
+    // image variants only
+    bool solo = !cls_persist[c] && (int)order.size() <= ctx().num_cu;
+    for (int k = cls_start[c]; solo && k < cls_start[c + 1]; ++k) solo = insts[order[k]].resident != 0;
     e = wg_launch(recs + cls_start[c], cls_start[c + 1] - cls_start[c], cls_lds[c], cls_threads[c], fp64, direct,
                   cls_persist[c] ? reinterpret_cast<int*>(base + queue_off) + c : nullptr,
-                  cls_occ[c] * ctx().num_cu, sc);
+                  cls_occ[c] * ctx().num_cu, solo, sc);
   }
   for (int t = 1; t < kStreams && e == hipSuccess; ++t) {
     if (!used[t]) continue;

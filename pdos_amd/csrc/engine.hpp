@@ -69,8 +69,10 @@ hipError_t dense_a(hipStream_t st, int m, int n, int ldb, int n64, const __bf16*
 // and class order (lambda and the stop-test scales are written into both).
 hipError_t batch_setup_launch(const SetupGroup* d_groups, int ngroups, const SetupInst* d_insts, int ninsts,
                               WgInst* rec, WgInst* rec_cls, bool fp64, hipStream_t st);
+// solo: every workgroup of the launch has a CU of its own (the kernel may use
+// more registers per lane; 512-thread launches only).
 hipError_t wg_launch(const WgInst* d_insts, int count, int lds_bytes, int threads, bool fp64, bool direct, int* queue,
-                     int grid, hipStream_t st);
+                     int grid, bool solo, hipStream_t st);
 
 // Whole-GPU engine for one large instance (both plugins), prepared once and
 // re-runnable from the prepared start.

@@ -625,6 +625,102 @@ __device__ __forceinline__ void spmv_res(P4 pos, PI idx, PV val, P4 bat, int nba
   }
 }
 
+// A wave's batch of a product (batch w; kSlots > 1 would hold w + nw, ...
+// too: measured slower, at 256 threads as at 512) held in registers across
+// the CG steps: per batch the lane's row, its (up to four)
+// entries' indices and which of them exist (and, in binary32, their values;
+// binary64 values are loaded with the gathers, in flight together, to spare
+// registers), so a step's product is the gathers, the products and the group
+// sums (the descriptor, position and index loads are made once per solve).
+// Same arithmetic and order as spmv_res.
+template <typename T>
+struct RegRows {
+  static constexpr bool kVals = sizeof(T) == 4;
+  static constexpr int kSlots = 1;
+  int lg[kSlots];  // lanes per row (log2), wave-uniform; -1: no batch in the slot
+  int row[kSlots], e0[kSlots], g[kSlots];
+  bool on[kSlots], lead[kSlots];
+  unsigned msk[kSlots];
+  int ix[kSlots][4];
+  T vv[kSlots][kVals ? 4 : 1];
+};
+
+// Loads the wave's batches of (bat, nbat) into R; returns false
+// (wave-uniform) when they cannot be held this way: more than kSlots batches
+// a wave, or a row of more than four entries a lane.
+template <typename T, typename P4, typename PI, typename PV>
+__device__ __forceinline__ bool reg_rows_load(P4 pos, PI idx, PV val, P4 bat, int nbat, RegRows<T>& R) {
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nw = blockDim.x >> 6;
+  bool fits = nbat <= RegRows<T>::kSlots * nw;
+#pragma unroll
+  for (int sl = 0; sl < RegRows<T>::kSlots; ++sl) {
+    // the batches in reverse: their rows' lane groups grow with the batch
+    // index, and the first waves are the first out of the previous phase
+    const int b = nbat - 1 - (w + sl * nw);
+    R.lg[sl] = -1;
+    R.on[sl] = R.lead[sl] = false;
+    R.msk[sl] = 0;
+    R.row[sl] = R.e0[sl] = R.g[sl] = 0;
+    if (!fits || b < 0) continue;
+    const int4 B = uniform_batch(bat, b);
+    const int k = lane >> B.z, sub = lane & ((1 << B.z) - 1), G = 1 << B.z;
+    R.lg[sl] = B.z;
+    R.g[sl] = G;
+    R.on[sl] = k < B.y;
+    R.lead[sl] = R.on[sl] && sub == 0;
+    const int4 P = pos[B.x + (R.on[sl] ? k : 0)];
+    R.row[sl] = P.x;
+    R.e0[sl] = P.y + sub;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int e = R.e0[sl] + j * G;
+      R.ix[sl][j] = idx[e];
+      if constexpr (RegRows<T>::kVals) R.vv[sl][j] = val[e];
+      if (e < P.z) R.msk[sl] |= 1u << j;
+    }
+    fits = fits && __all(!R.on[sl] || R.e0[sl] + 4 * G >= P.z);
+  }
+  return fits;
+}
+
+// The product over a wave's held batches: emit(row, value) on the lanes that
+// spmv_res would emit from.  Both slots' loads are issued before the first
+// product.
+template <typename T, typename PV, typename Gather, typename Emit>
+__device__ __forceinline__ void reg_rows_spmv(const RegRows<T>& R, PV val, Gather gx, Emit emit) {
+  constexpr int S = RegRows<T>::kSlots;
+  T vv[S][4], gv[S][4];
+#pragma unroll
+  for (int sl = 0; sl < S; ++sl) {
+    if (R.lg[sl] < 0) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if constexpr (RegRows<T>::kVals) vv[sl][j] = R.vv[sl][j];
+      else vv[sl][j] = val[R.e0[sl] + j * R.g[sl]];
+      gv[sl][j] = gx(R.ix[sl][j]);
+    }
+  }
+#pragma unroll
+  for (int sl = 0; sl < S; ++sl) {
+    if (R.lg[sl] < 0) continue;
+    if (R.lg[sl] == 0) {
+      T acc = T(0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc += (R.msk[sl] >> j) & 1u ? T(vv[sl][j] * gv[sl][j]) : T(0);
+      if (R.on[sl]) emit(R.row[sl], acc);
+    } else {
+      double acc = 0.0;
+      if (R.on[sl]) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc += (R.msk[sl] >> j) & 1u ? double(vv[sl][j] * gv[sl][j]) : 0.0;
+      }
+      acc = group_sum(acc, R.lg[sl]);
+      if (R.lead[sl]) emit(R.row[sl], T(acc));
+    }
+  }
+}
+
 // One triangular sweep in LDS, level by level (lb: level -> its batches;
 // position q: pos[q] = {row, entries}; backward: divide by dq[q] first), one
 // barrier per level.
@@ -682,7 +778,7 @@ __device__ __forceinline__ void res_tblock(int t, PI trow, PD tm, double* z, dou
 // IMG_LDS = false: the same kernel with the group's image read from device
 // memory (L2 / MALL resident) -- for instances whose matrices do not fit in
 // LDS; only b, c, the iterates and the reduction slots are in LDS.
-template <typename T, bool DIRECT, bool IMG_LDS>
+template <typename T, bool DIRECT, bool IMG_LDS, bool REGS = false>
 __device__ __forceinline__ void wg_res_body(const WgInst* __restrict__ insts, int idx, unsigned char* smem) {
   // the record by value: its fields stay in registers across the barriers
   // (a reference would be re-read from memory after each one)
@@ -843,7 +939,7 @@ __device__ __forceinline__ void wg_res_body(const WgInst* __restrict__ insts, in
       T beta = T(0);
       int k = 0;
       double qlast = 0.0;
-      if (rho > I.cg_tol_sq && n <= 2 * 64) {
+      if (REGS && rho > I.cg_tol_sq && n <= 2 * 64) {
         // n of at most 128: every wave holds all of x, q and p in registers
         // (lane l rows l and l + 64) and forms the two dot products and the
         // vector updates itself, so a step has two barriers (after the A and
@@ -860,10 +956,19 @@ __device__ __forceinline__ void wg_res_body(const WgInst* __restrict__ insts, in
           pr[uu] = qr[uu];  // beta p + q with p = 0
           if (i < n) p[i] = pr[uu];
         }
+        // the wave's batch of each product in registers where it has at
+        // most one and no row of more than four entries a lane
+        RegRows<T> ra, rc;
+        const bool hold_a = reg_rows_load<T>(rpos, ridx, rval, rbat, nrbat, ra);
+        const bool hold_c = reg_rows_load<T>(cpos, cidx, cval, cbat, ncbat, rc);
         while (k < I.cg_max_its) {
-          spmv_res<T>(rpos, ridx, rval, rbat, nrbat, [&](int c) { return p[c]; }, [&](int j, T v) { t[j] = v; });
+          auto emit_t = [&](int j, T v) { t[j] = v; };
+          auto emit_u = [&](int i, T v) { u[i] = v; };
+          if (hold_a) reg_rows_spmv(ra, rval, [&](int c) { return p[c]; }, emit_t);
+          else spmv_res<T>(rpos, ridx, rval, rbat, nrbat, [&](int c) { return p[c]; }, emit_t);
           __syncthreads();
-          spmv_res<T>(cpos, cidx, cval, cbat, ncbat, [&](int c) { return t[c]; }, [&](int i, T v) { u[i] = v; });
+          if (hold_c) reg_rows_spmv(rc, cval, [&](int c) { return t[c]; }, emit_u);
+          else spmv_res<T>(cpos, cidx, cval, cbat, ncbat, [&](int c) { return t[c]; }, emit_u);
           __syncthreads();
           T ur[2];
           double r2 = 0.0;
@@ -1121,25 +1226,44 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 // workgroup keep more of the CU's SIMDs busy on the same instance (rows,
 // batches and levels dealt over more waves).  Records of the streaming
 // variant never come here (its row blocks are sized for 256 threads).
+template <typename T, bool DIRECT, bool REGS>
+__device__ __forceinline__ void wg_image_loop(const WgInst* __restrict__ insts, int count, int* queue,
+                                              unsigned char* smem, int* wg_next) {
+  int idx = blockIdx.x;
+  for (;;) {
+    if (queue) {
+      __syncthreads();  // the previous instance is done with LDS and wg_next
+      if (threadIdx.x == 0) *wg_next = atomicAdd(queue, 1);
+      __syncthreads();
+      idx = *wg_next;
+    }
+    if (idx >= count) break;
+    const int v = __builtin_amdgcn_readfirstlane(insts[idx].resident);
+    if (v == 1) wg_res_body<T, DIRECT, true, REGS>(insts, idx, smem);
+    else wg_res_body<T, DIRECT, false, REGS>(insts, idx, smem);
+    if (!queue) break;
+  }
+}
+
 template <typename T, bool DIRECT>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void k_wg_wide(
     const WgInst* __restrict__ insts, int count, int* queue) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   __shared__ int wg_next;
-  int idx = blockIdx.x;
-  for (;;) {
-    if (queue) {
-      __syncthreads();  // the previous instance is done with LDS and wg_next
-      if (threadIdx.x == 0) wg_next = atomicAdd(queue, 1);
-      __syncthreads();
-      idx = wg_next;
-    }
-    if (idx >= count) break;
-    const int v = __builtin_amdgcn_readfirstlane(insts[idx].resident);
-    if (v == 1) wg_res_body<T, DIRECT, true>(insts, idx, smem);
-    else wg_res_body<T, DIRECT, false>(insts, idx, smem);
-    if (!queue) break;
-  }
+  wg_image_loop<T, DIRECT, false>(insts, count, queue, smem, &wg_next);
+}
+
+// The image variants at up to 512 threads when every workgroup has a CU of
+// its own (a single solve, or no more instances than CUs): at most two waves
+// per SIMD, so the body may hold 256 registers a lane, and the CG steps of
+// an instance with n <= 128 keep x, q, p and the wave's matrix rows in
+// registers.
+template <typename T, bool DIRECT>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void k_wg_solo(
+    const WgInst* __restrict__ insts, int count, int* queue) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  __shared__ int wg_next;
+  wg_image_loop<T, DIRECT, true>(insts, count, queue, smem, &wg_next);
 }
 
 __global__ void k_zero_int(int* p) {
@@ -1158,19 +1282,22 @@ hipError_t lds_limit(const void* fn, int lds_bytes, std::atomic<int>& set) {
 }
 
 template <typename T, bool DIRECT>
-hipError_t launch_t(const WgInst* d_insts, int count, int lds_bytes, int threads, int* queue, int grid,
+hipError_t launch_t(const WgInst* d_insts, int count, int lds_bytes, int threads, int* queue, int grid, bool solo,
                     hipStream_t st) {
-  static std::atomic<int> lds_set{-1}, lds_set_wide{-1};
+  static std::atomic<int> lds_set{-1}, lds_set_wide{-1}, lds_set_solo{-1};
   const bool wide = threads > NT;
-  auto fn = wide ? k_wg_wide<T, DIRECT> : k_wg<T, DIRECT>;
-  hipError_t e = lds_limit(reinterpret_cast<const void*>(fn), lds_bytes, wide ? lds_set_wide : lds_set);
+  solo = solo && threads <= 512;
+  auto fn = solo ? k_wg_solo<T, DIRECT> : (wide ? k_wg_wide<T, DIRECT> : k_wg<T, DIRECT>);
+  hipError_t e = lds_limit(reinterpret_cast<const void*>(fn), lds_bytes,
+                           solo ? lds_set_solo : (wide ? lds_set_wide : lds_set));
   if (e != hipSuccess) return e;
   if (queue) {  // a kernel, not a memset node, so that the launch may be captured in a graph
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, queue);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(fn, dim3(queue ? std::max(1, std::min(grid, count)) : count), dim3(wide ? threads : NT),
+  hipLaunchKernelGGL(fn, dim3(queue ? std::max(1, std::min(grid, count)) : count),
+                     dim3(wide || solo ? threads : NT),
                      lds_bytes, st, d_insts, count, queue);
   return hipGetLastError();
 }
@@ -1202,12 +1329,12 @@ hipError_t device_copy(void* dst, const void* src, size_t bytes, hipStream_t st)
 // `grid` persistent workgroups drain the records in order through that
 // counter (zeroed here first).
 hipError_t wg_launch(const WgInst* d_insts, int count, int lds_bytes, int threads, bool fp64, bool direct, int* queue,
-                     int grid, hipStream_t st) {
+                     int grid, bool solo, hipStream_t st) {
   if (threads != 256 && threads != 512 && threads != 1024) return hipErrorInvalidValue;
-  if (fp64) return direct ? launch_t<double, true>(d_insts, count, lds_bytes, threads, queue, grid, st)
-                          : launch_t<double, false>(d_insts, count, lds_bytes, threads, queue, grid, st);
-  return direct ? launch_t<float, true>(d_insts, count, lds_bytes, threads, queue, grid, st)
-                : launch_t<float, false>(d_insts, count, lds_bytes, threads, queue, grid, st);
+  if (fp64) return direct ? launch_t<double, true>(d_insts, count, lds_bytes, threads, queue, grid, solo, st)
+                          : launch_t<double, false>(d_insts, count, lds_bytes, threads, queue, grid, solo, st);
+  return direct ? launch_t<float, true>(d_insts, count, lds_bytes, threads, queue, grid, solo, st)
+                : launch_t<float, false>(d_insts, count, lds_bytes, threads, queue, grid, solo, st);
 }
 
 }  // namespace pdos_amd
