@@ -627,22 +627,19 @@ __device__ __forceinline__ void spmv_res(P4 pos, PI idx, PV val, P4 bat, int nba
 
 // A wave's batch of a product (batch w; kSlots > 1 would hold w + nw, ...
 // too: measured slower, at 256 threads as at 512) held in registers across
-// the CG steps: per batch the lane's row, its (up to four)
-// entries' indices and which of them exist (and, in binary32, their values;
-// binary64 values are loaded with the gathers, in flight together, to spare
-// registers), so a step's product is the gathers, the products and the group
-// sums (the descriptor, position and index loads are made once per solve).
-// Same arithmetic and order as spmv_res.
+// the CG steps: per batch the lane's row, its (up to four) entries' indices
+// and values and which of them exist, so a step's product is the gathers,
+// the products and the group sum (the descriptor, position and entry loads
+// are made once per solve).  Same arithmetic and order as spmv_res.
 template <typename T>
 struct RegRows {
-  static constexpr bool kVals = sizeof(T) == 4;
   static constexpr int kSlots = 1;
   int lg[kSlots];  // lanes per row (log2), wave-uniform; -1: no batch in the slot
-  int row[kSlots], e0[kSlots], g[kSlots];
+  int row[kSlots];
   bool on[kSlots], lead[kSlots];
   unsigned msk[kSlots];
   int ix[kSlots][4];
-  T vv[kSlots][kVals ? 4 : 1];
+  T vv[kSlots][4];
 };
 
 // Loads the wave's batches of (bat, nbat) into R; returns false
@@ -661,45 +658,40 @@ __device__ __forceinline__ bool reg_rows_load(P4 pos, PI idx, PV val, P4 bat, in
     R.lg[sl] = -1;
     R.on[sl] = R.lead[sl] = false;
     R.msk[sl] = 0;
-    R.row[sl] = R.e0[sl] = R.g[sl] = 0;
+    R.row[sl] = 0;
     if (!fits || b < 0) continue;
     const int4 B = uniform_batch(bat, b);
     const int k = lane >> B.z, sub = lane & ((1 << B.z) - 1), G = 1 << B.z;
     R.lg[sl] = B.z;
-    R.g[sl] = G;
     R.on[sl] = k < B.y;
     R.lead[sl] = R.on[sl] && sub == 0;
     const int4 P = pos[B.x + (R.on[sl] ? k : 0)];
     R.row[sl] = P.x;
-    R.e0[sl] = P.y + sub;
+    const int e0 = P.y + sub;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const int e = R.e0[sl] + j * G;
+      const int e = e0 + j * G;
       R.ix[sl][j] = idx[e];
-      if constexpr (RegRows<T>::kVals) R.vv[sl][j] = val[e];
+      R.vv[sl][j] = val[e];
       if (e < P.z) R.msk[sl] |= 1u << j;
     }
-    fits = fits && __all(!R.on[sl] || R.e0[sl] + 4 * G >= P.z);
+    fits = fits && __all(!R.on[sl] || e0 + 4 * G >= P.z);
   }
   return fits;
 }
 
 // The product over a wave's held batches: emit(row, value) on the lanes that
-// spmv_res would emit from.  Both slots' loads are issued before the first
-// product.
-template <typename T, typename PV, typename Gather, typename Emit>
-__device__ __forceinline__ void reg_rows_spmv(const RegRows<T>& R, PV val, Gather gx, Emit emit) {
+// spmv_res would emit from.  Every slot's gathers are issued before the
+// first product.
+template <typename T, typename Gather, typename Emit>
+__device__ __forceinline__ void reg_rows_spmv(const RegRows<T>& R, Gather gx, Emit emit) {
   constexpr int S = RegRows<T>::kSlots;
-  T vv[S][4], gv[S][4];
+  T gv[S][4];
 #pragma unroll
   for (int sl = 0; sl < S; ++sl) {
     if (R.lg[sl] < 0) continue;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      if constexpr (RegRows<T>::kVals) vv[sl][j] = R.vv[sl][j];
-      else vv[sl][j] = val[R.e0[sl] + j * R.g[sl]];
-      gv[sl][j] = gx(R.ix[sl][j]);
-    }
+    for (int j = 0; j < 4; ++j) gv[sl][j] = gx(R.ix[sl][j]);
   }
 #pragma unroll
   for (int sl = 0; sl < S; ++sl) {
@@ -707,13 +699,13 @@ __device__ __forceinline__ void reg_rows_spmv(const RegRows<T>& R, PV val, Gathe
     if (R.lg[sl] == 0) {
       T acc = T(0);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc += (R.msk[sl] >> j) & 1u ? T(vv[sl][j] * gv[sl][j]) : T(0);
+      for (int j = 0; j < 4; ++j) acc += (R.msk[sl] >> j) & 1u ? T(R.vv[sl][j] * gv[sl][j]) : T(0);
       if (R.on[sl]) emit(R.row[sl], acc);
     } else {
       double acc = 0.0;
       if (R.on[sl]) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc += (R.msk[sl] >> j) & 1u ? double(vv[sl][j] * gv[sl][j]) : 0.0;
+        for (int j = 0; j < 4; ++j) acc += (R.msk[sl] >> j) & 1u ? double(R.vv[sl][j] * gv[sl][j]) : 0.0;
       }
       acc = group_sum(acc, R.lg[sl]);
       if (R.lead[sl]) emit(R.row[sl], T(acc));
@@ -964,10 +956,10 @@ __device__ __forceinline__ void wg_res_body(const WgInst* __restrict__ insts, in
         while (k < I.cg_max_its) {
           auto emit_t = [&](int j, T v) { t[j] = v; };
           auto emit_u = [&](int i, T v) { u[i] = v; };
-          if (hold_a) reg_rows_spmv(ra, rval, [&](int c) { return p[c]; }, emit_t);
+          if (hold_a) reg_rows_spmv(ra, [&](int c) { return p[c]; }, emit_t);
           else spmv_res<T>(rpos, ridx, rval, rbat, nrbat, [&](int c) { return p[c]; }, emit_t);
           __syncthreads();
-          if (hold_c) reg_rows_spmv(rc, cval, [&](int c) { return t[c]; }, emit_u);
+          if (hold_c) reg_rows_spmv(rc, [&](int c) { return t[c]; }, emit_u);
           else spmv_res<T>(cpos, cidx, cval, cbat, ncbat, [&](int c) { return t[c]; }, emit_u);
           __syncthreads();
           T ur[2];
