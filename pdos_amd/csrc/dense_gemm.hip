@@ -22,10 +22,15 @@
 // and writes each as one 16-byte LDS store, so the MFMA B operand is read
 // from LDS as one 16-byte load a lane as well.
 //
-// Tiles: a workgroup of 4 waves (2 x 2) computes a 64 x 64 output tile (per
-// right-hand side); the A' product splits K (= m) over blockIdx.z when the
-// batch is narrow, each split writing its own plane of partial sums, which
-// the consumer (phase B) adds in a fixed order -- deterministic, no atomics.
+// Tiles: a workgroup of 4 waves (4 x 1) computes a 128 x 64 output tile (per
+// right-hand side), each wave 32 rows x 64 columns, so a wave's A fragments
+// serve four column subtiles (a 64 x 64 tile of 2 x 2 waves loaded each A
+// fragment twice and moved twice the bytes per product); the A' product
+// splits K (= m) over blockIdx.z when the batch is narrow, each split
+// writing its own plane of partial sums, which the consumer (phase B) adds
+// in a fixed order -- deterministic, no atomics.  Every output element sums
+// its products in the same order as with 64 x 64 tiles (K step, K slice,
+// then hi hi, hi lo, lo hi).
 #include <hip/hip_runtime.h>
 
 #include "device.hpp"
@@ -37,7 +42,9 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int DT = 64;   // output tile rows and columns
+constexpr int DT = 64;   // output tile columns
+constexpr int DM = 128;  // output tile rows (4 waves of 32)
+constexpr int DJ = DT / 16;  // column subtiles a wave covers
 constexpr int DK = 64;   // K per step: two MFMA K slices of 32
 constexpr int DKP = DK + 8;  // LDS row pitch (bf16) against bank conflicts
 
@@ -92,15 +99,16 @@ struct AFrag {
 };
 
 // acc += a_hi b_hi + a_hi b_lo + a_lo b_hi over one step (two K slices) for
-// the wave's 2 x 2 subtiles of one right-hand side staged in hi / lo.
-__device__ __forceinline__ void mfma_step(f32x4 (&acc)[2][2], const AFrag& A, __bf16 (*hi)[DKP],
-                                          __bf16 (*lo)[DKP], int wn, int fr, int fk) {
+// the wave's 2 x NJ subtiles of one right-hand side staged in hi / lo.
+template <int NJ>
+__device__ __forceinline__ void mfma_step(f32x4 (&acc)[2][NJ], const AFrag& A, __bf16 (*hi)[DKP],
+                                          __bf16 (*lo)[DKP], int fr, int fk) {
 #pragma unroll
   for (int q = 0; q < 2; ++q)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const bf16x8 bh = *reinterpret_cast<const bf16x8*>(&hi[wn + j * 16 + fr][q * 32 + fk]);
-      const bf16x8 bl = *reinterpret_cast<const bf16x8*>(&lo[wn + j * 16 + fr][q * 32 + fk]);
+    for (int j = 0; j < NJ; ++j) {
+      const bf16x8 bh = *reinterpret_cast<const bf16x8*>(&hi[j * 16 + fr][q * 32 + fk]);
+      const bf16x8 bl = *reinterpret_cast<const bf16x8*>(&lo[j * 16 + fr][q * 32 + fk]);
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A.h[i][q], bh, acc[i][j], 0, 0, 0);
@@ -111,23 +119,24 @@ __device__ __forceinline__ void mfma_step(f32x4 (&acc)[2][2], const AFrag& A, __
 }
 
 // accumulator lane l holds rows (l / 16) * 4 + r, column l % 16
-__device__ __forceinline__ void store_tile(const f32x4 (&acc)[2][2], float* __restrict__ P, int ldb, int rows,
-                                           int i0, int j0, int wm, int wn, int lane) {
+template <int NJ>
+__device__ __forceinline__ void store_tile(const f32x4 (&acc)[2][NJ], float* __restrict__ P, int ldb, int rows,
+                                           int i0, int j0, int wm, int lane) {
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < NJ; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = i0 + wm + i * 16 + (lane >> 4) * 4 + r;
-        const int col = j0 + wn + j * 16 + (lane & 15);
+        const int col = j0 + j * 16 + (lane & 15);
         if (row < rows) P[(size_t)row * ldb + col] = acc[i][j][r];
       }
 }
 
-// U (plane z) = A'[i0 : i0 + 64, ks] Y[ks, j0 : j0 + 64], V likewise with W;
+// U (plane z) = A'[i0 : i0 + 128, ks] Y[ks, j0 : j0 + 64], V likewise with W;
 // ks = [z kchunk, (z + 1) kchunk) of the rows of A.  ATh / ATl: A' split,
-// n64 x m64 row-major (zeros beyond n, m).  Planes: n x ldb each, plane z at
+// dense_row_pad(n) x m64 row-major (zeros beyond n, m).  Planes: n x ldb each, plane z at
 // z * plane.  Software-pipelined: the next step's A fragments and
 // right-hand-side rows are loaded while the current step is multiplied.
 __global__ __launch_bounds__(256) void k_dense_at(int m, int n, int ldb, int m64, int kchunk,
@@ -136,18 +145,18 @@ __global__ __launch_bounds__(256) void k_dense_at(int m, int n, int ldb, int m64
                                                   float* __restrict__ Up, float* __restrict__ Vp, size_t plane) {
   __shared__ __attribute__((aligned(16))) __bf16 Bs[4][DT][DKP];  // Y hi, Y lo, W hi, W lo
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const int wm = (wid >> 1) * 32, wn = (wid & 1) * 32;
-  const int j0 = blockIdx.x * DT, i0 = blockIdx.y * DT;
+  const int wm = wid * 32;
+  const int j0 = blockIdx.x * DT, i0 = blockIdx.y * DM;
   const int k0 = blockIdx.z * kchunk, k1 = min(m64, k0 + kchunk);
   const int fr = lane & 15, fk = (lane >> 4) * 8;
   const int sc = tid & 63, sg = tid >> 6;  // staging: column, group of 8 rows per K slice
-  f32x4 acc[2][2][2];
+  f32x4 acc[2][2][DJ];
 #pragma unroll
   for (int h = 0; h < 2; ++h)
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j) acc[h][i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+      for (int j = 0; j < DJ; ++j) acc[h][i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
   AFrag A, An;
   ColPart y, w;
   if (k0 < k1) {
@@ -166,31 +175,32 @@ __global__ __launch_bounds__(256) void k_dense_at(int m, int n, int ldb, int m64
       y.load(Y, ldb, m, k + DK, j0 + sc, sg);
       w.load(W, ldb, m, k + DK, j0 + sc, sg);
     }
-    mfma_step(acc[0], A, Bs[0], Bs[1], wn, fr, fk);
-    mfma_step(acc[1], A, Bs[2], Bs[3], wn, fr, fk);
+    mfma_step<DJ>(acc[0], A, Bs[0], Bs[1], fr, fk);
+    mfma_step<DJ>(acc[1], A, Bs[2], Bs[3], fr, fk);
     if (more) A = An;
   }
-  store_tile(acc[0], Up + (size_t)blockIdx.z * plane, ldb, n, i0, j0, wm, wn, lane);
-  store_tile(acc[1], Vp + (size_t)blockIdx.z * plane, ldb, n, i0, j0, wm, wn, lane);
+  store_tile<DJ>(acc[0], Up + (size_t)blockIdx.z * plane, ldb, n, i0, j0, wm, lane);
+  store_tile<DJ>(acc[1], Vp + (size_t)blockIdx.z * plane, ldb, n, i0, j0, wm, lane);
 }
 
-// Z[i0 : i0 + 64, j0 : j0 + 64] = A[i0 : , :] X[:, j0 : ]: ARh / ARl: A
-// split, m64 x n64 row-major (zeros beyond m, n); X: n x ldb.  Pipelined
-// as k_dense_at.
+// Z[i0 : i0 + 128, j0 : j0 + 64] = A[i0 : , :] X[:, j0 : ]: ARh / ARl: A
+// split, dense_row_pad(m) x n64 row-major (zeros beyond m, n); X: n x ldb.
+// Pipelined as k_dense_at.  (128-column tiles, each A fragment used eight
+// times, measured slower: 264 registers leave one wave per SIMD.)
 __global__ __launch_bounds__(256) void k_dense_a(int m, int n, int ldb, int n64, const __bf16* __restrict__ ARh,
                                                  const __bf16* __restrict__ ARl, const float* __restrict__ X,
                                                  float* __restrict__ Z) {
   __shared__ __attribute__((aligned(16))) __bf16 Bs[2][DT][DKP];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const int wm = (wid >> 1) * 32, wn = (wid & 1) * 32;
-  const int j0 = blockIdx.x * DT, i0 = blockIdx.y * DT;
+  const int wm = wid * 32;
+  const int j0 = blockIdx.x * DT, i0 = blockIdx.y * DM;
   const int fr = lane & 15, fk = (lane >> 4) * 8;
   const int sc = tid & 63, sg = tid >> 6;
-  f32x4 acc[2][2];
+  f32x4 acc[2][DJ];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < DJ; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
   AFrag A, An;
   ColPart x;
   A.load(ARh, ARl, n64, i0 + wm, 0, fr, fk);
@@ -204,16 +214,18 @@ __global__ __launch_bounds__(256) void k_dense_a(int m, int n, int ldb, int n64,
       An.load(ARh, ARl, n64, i0 + wm, k + DK, fr, fk);
       x.load(X, ldb, n, k + DK, j0 + sc, sg);
     }
-    mfma_step(acc, A, Bs[0], Bs[1], wn, fr, fk);
+    mfma_step<DJ>(acc, A, Bs[0], Bs[1], fr, fk);
     if (more) A = An;
   }
-  store_tile(acc, Z, ldb, m, i0, j0, wm, wn, lane);
+  store_tile<DJ>(acc, Z, ldb, m, i0, j0, wm, lane);
 }
 
 }  // namespace
 
+int dense_row_pad(int rows) { return (rows + DM - 1) / DM * DM; }
+
 int dense_at_splits(int n, int m64, int ldb) {
-  const int tiles = ((n + DT - 1) / DT) * (ldb / DT);
+  const int tiles = ((n + DM - 1) / DM) * (ldb / DT);
   int s = 1;
   while (s < 16 && tiles * s * 2 <= 512 && m64 / (s * 2) >= 4 * DK) s *= 2;
   return s;
@@ -223,7 +235,7 @@ hipError_t dense_at_pair(hipStream_t st, int m, int n, int ldb, int m64, const _
                          const float* Y, const float* W, float* Up, float* Vp, size_t plane, int splits) {
   if (ldb % DT || m64 % DK || splits < 1) return hipErrorInvalidValue;
   const int kchunk = ((m64 / splits + DK - 1) / DK) * DK;
-  hipLaunchKernelGGL(k_dense_at, dim3(ldb / DT, (n + DT - 1) / DT, splits), dim3(256), 0, st, m, n, ldb, m64,
+  hipLaunchKernelGGL(k_dense_at, dim3(ldb / DT, (n + DM - 1) / DM, splits), dim3(256), 0, st, m, n, ldb, m64,
                      kchunk, ATh, ATl, Y, W, Up, Vp, plane);
   return hipGetLastError();
 }
@@ -231,7 +243,7 @@ hipError_t dense_at_pair(hipStream_t st, int m, int n, int ldb, int m64, const _
 hipError_t dense_a(hipStream_t st, int m, int n, int ldb, int n64, const __bf16* ARh, const __bf16* ARl,
                    const float* X, float* Z) {
   if (ldb % DT || n64 % DK) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_dense_a, dim3(ldb / DT, (m + DT - 1) / DT), dim3(256), 0, st, m, n, ldb, n64, ARh, ARl, X, Z);
+  hipLaunchKernelGGL(k_dense_a, dim3(ldb / DT, (m + DM - 1) / DM), dim3(256), 0, st, m, n, ldb, n64, ARh, ARl, X, Z);
   return hipGetLastError();
 }
 

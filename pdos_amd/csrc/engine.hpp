@@ -60,6 +60,9 @@ hipError_t device_copy(void* dst, const void* src, size_t bytes, hipStream_t st)
 //   dense_a        Z = A X from A split copies ARh / ARl (m64 x n64)
 //   dense_at_splits  the K split dense_at_pair uses for a width
 int dense_at_splits(int n, int m64, int ldb);
+// rows of the split copies of A' (n) and A (m) padded to the products' tile
+// height (the kernels read whole tiles)
+int dense_row_pad(int rows);
 hipError_t dense_at_pair(hipStream_t st, int m, int n, int ldb, int m64, const __bf16* ATh, const __bf16* ATl,
                          const float* Y, const float* W, float* Up, float* Vp, size_t plane, int splits);
 hipError_t dense_a(hipStream_t st, int m, int n, int ldb, int n64, const __bf16* ARh, const __bf16* ARl,

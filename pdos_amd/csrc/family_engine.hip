@@ -1604,7 +1604,8 @@ struct Family {
   // mixed precision, dense A: both products run as the hand-written split
   // bf16 MFMA kernels (dense_gemm.hip; A = A_hi + A_lo, three products with
   // fp32 accumulation) on A's split copies in MFMA operand order:
-  //   ATh, ATl  A' rows, n64 x m64;  ARh, ARl  A rows, m64 x n64 (zero padded)
+  //   ATh, ATl  A' rows, dense_row_pad(n) x m64;  ARh, ARl  A rows,
+  //   dense_row_pad(m) x n64 (zero padded)
   //   Upl, Vpl  the A' products' split-K planes
   void *ATh = nullptr, *ATl = nullptr, *ARh = nullptr, *ARl = nullptr, *Upl = nullptr, *Vpl = nullptr;
   int m64 = 0, n64 = 0;
@@ -2015,7 +2016,9 @@ bool Family::create(const Data* d, const Cone* k, const PdosOptions& o, std::str
     if (split_dense) {
       m64 = (m + 63) & ~63;
       n64 = (n + 63) & ~63;
-      std::vector<__bf16> th((size_t)n64 * m64, (__bf16)0.f), tl(th), rh((size_t)m64 * n64, (__bf16)0.f), rl(rh);
+      // rows padded to the products' tile height, columns to 64
+      std::vector<__bf16> th((size_t)dense_row_pad(n) * m64, (__bf16)0.f), tl(th),
+          rh((size_t)dense_row_pad(m) * n64, (__bf16)0.f), rl(rh);
       for (int r = 0; r < m; ++r)
         for (int c = 0; c < n; ++c) {
           const float v = float(Arow[(size_t)r * n + c]);
