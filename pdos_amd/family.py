@@ -133,3 +133,20 @@ class Family:
                 self.h = None
         except Exception:
             pass
+
+
+def polish(template: Problem, Cm, mixed: dict, Bm=None, eps_abs: float = 1e-4, max_iters: int = 300,
+           device: int = 0) -> dict:
+    """Binary64 polish of mixed-precision family results: the same instances
+    (template's A, cones and parameters; rows of Cm, and Bm if given) solved
+    by the FP64 family engine from the mixed x, y, s as warm start (original
+    units, passed through binary32 as every family warm start is), without
+    the warm-start judge, to eps_abs within max_iters iterations.  Returns
+    fetch()'s dict of the polished results (iters = polish iterations)."""
+    from . import DeviceArray
+    W = np.concatenate([mixed["x"], mixed["y"], mixed["s"]], axis=1).astype(np.float32)
+    dW = DeviceArray(W)
+    P = Family(template.with_params(EPS_ABS=eps_abs, MAX_ITERS=max_iters), precision="fp64", device=device)
+    P.load(Cm, Bm)
+    P.run(dW, W.shape[1], judge=False)
+    return P.fetch()

@@ -333,3 +333,38 @@ def test_family_load_setup_on_device_bitwise(precision):
             assert out["status"][t] == ref["status"] and out["iters"][t] == ref["iters"], t
             for k in "xsy":
                 assert _rel(out[k][t], ref[k]) <= 1e-6, (t, k)
+
+
+def test_fp64_polish_moves_mixed_results_to_the_fp64_solution():
+    """pdos_amd.family.polish: the FP64 family engine warm-started from the
+    mixed results reaches its tighter tolerance in fewer iterations than a
+    cold FP64 solve, and brings every result closer to the FP64 solution."""
+    from pdos_amd.family import polish
+    fam = _fam(na=60, k=6)
+    _, Cm = fam.sample(seed=17, count=256)
+    print("fp64 polish", flush=True)
+    F = Family(fam.instance(Cm[0]), precision="mixed")
+    F.load(Cm)
+    F.run()
+    mixed = F.fetch()
+    R = Family(fam.instance(Cm[0]).with_params(EPS_ABS=1e-7, MAX_ITERS=20000), precision="fp64")
+    R.load(Cm)
+    R.run()
+    ref = R.fetch()
+    pol = polish(fam.instance(Cm[0]), Cm, mixed, eps_abs=1e-5, max_iters=2000)
+    C5 = Family(fam.instance(Cm[0]).with_params(EPS_ABS=1e-5, MAX_ITERS=2000), precision="fp64")
+    C5.load(Cm)
+    C5.run()
+    cold = C5.fetch()
+    ok = (mixed["status"] == "Solved") & (ref["status"] == "Solved") & (pol["status"] == "Solved")
+    assert ok.sum() >= 0.9 * len(Cm)
+    assert pol["iters"][ok].mean() < cold["iters"][ok].mean()
+
+    def dist(out):
+        num = np.sqrt(sum(np.sum((out[k] - ref[k]) ** 2, axis=1) for k in ("x", "y", "s")))
+        den = np.sqrt(sum(np.sum(ref[k] ** 2, axis=1) for k in ("x", "y", "s")))
+        return (num / den)[ok]
+
+    d0, d1 = dist(mixed), dist(pol)
+    assert np.median(d1) < 0.2 * np.median(d0), (np.median(d0), np.median(d1))
+    assert d1.max() < d0.max()
