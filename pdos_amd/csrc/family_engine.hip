@@ -497,6 +497,10 @@ __global__ __launch_bounds__(FT) void k_fam_at_hyb(FamArgs<T> a, SpmmIn<T, 2> in
   spmm_block(in.cols, in.nitems, bx, ti);
   const int m0 = ti * 64, n0 = bx * LANES;
   const SpmmPre<T> pre = spmm_preload(in, ti);  // in flight with the dense tile's operands
+  // the instance's flag and lambda, also in flight (no load waits behind the barrier)
+  const int inst = n0 + lane;
+  const int dn = a.done[inst];
+  const double lam_d = a.lam[inst];
   {
     const int rg = w >> 1, cg0 = (w & 1) * 2, kq = lane >> 4, col = lane & 15;
     float av[8], yb[2][8], wb[2][8];
@@ -535,9 +539,8 @@ __global__ __launch_bounds__(FT) void k_fam_at_hyb(FamArgs<T> a, SpmmIn<T, 2> in
       }
   }
   __syncthreads();
-  const int inst = n0 + lane;
-  const bool act = inst < a.count && !a.done[inst];
-  const T lam = T(a.lam[inst]);
+  const bool act = inst < a.count && !dn;
+  const T lam = T(lam_d);
   T c_i = T(0), x_i = T(0), uh = T(0), vh = T(0);
   double e_i = 1.0, d2 = 0.0, cx = 0.0;
   const unsigned boff = (unsigned)inst * (unsigned)sizeof(T);
@@ -718,19 +721,21 @@ __device__ __forceinline__ void decide_block(const FamArgs<T>& a, int col, int c
                                              double4 (*part)[LANES]) {
   const int lane = threadIdx.x & 63, w = wave_id();
   const int inst = col * LANES + lane;
-  const bool act = inst < a.count && !a.done[inst];
+  // every operand is loaded for every lane (inst < ldb: in bounds) together
+  // with the done flag, and finished lanes' sums are dropped after
+  const int dn = a.done[inst];
   double4 v = {0.0, 0.0, 0.0, 0.0};  // |pres|^2, b'y, |dres|^2, c'x
   double ps = 1.0, ds = 1.0;
   int it = 0;
-  if (act) {
-    if (w == 0) {
-      ps = a.pscale[inst];
-      ds = a.dscale[inst];
-      it = a.iters[inst];
-    }
-    wave_parts<NW>(a.pa, a.ldb, inst, w, chunks_m, v.x, v.y);
-    wave_parts<NW>(a.pb, a.ldb, inst, w, chunks_n, v.z, v.w);
+  if (w == 0) {
+    ps = a.pscale[inst];
+    ds = a.dscale[inst];
+    it = a.iters[inst];
   }
+  wave_parts<NW>(a.pa, a.ldb, inst, w, chunks_m, v.x, v.y);
+  wave_parts<NW>(a.pb, a.ldb, inst, w, chunks_n, v.z, v.w);
+  const bool act = inst < a.count && !dn;
+  if (!act) v = double4{0.0, 0.0, 0.0, 0.0};
   part[w][lane] = v;
   __syncthreads();
   if (w != 0 || !act) return;
@@ -1322,6 +1327,15 @@ __global__ __launch_bounds__(256) void k_fam_xupdate(int mt, int nt, int ksteps,
     mi = L % mt;
   }
   const int m0 = mi * 64, n0 = ni * 64;
+  // A with dense rows: this tile's values of A_h for the planes below, in
+  // flight with the product's operands
+  float ahv[4][2];
+  if (AXp) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int rg = 0; rg < 2; ++rg) ahv[s][rg] = Ahr[((size_t)((m0 + 16 * q) / 4 + s) * 2 + rg) * 64 + lane];
+  }
   // addend rows 16 q .. 16 q + 15 of the tile, column `lane`, for the epilogue
   float xa[16];
 #pragma unroll
@@ -1388,17 +1402,13 @@ __global__ __launch_bounds__(256) void k_fam_xupdate(int mt, int nt, int ksteps,
 #pragma unroll
     for (int cg = 0; cg < 4; ++cg) ah[rg][cg] = hf32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    const int kk = (m0 + 16 * q) / 4 + s;  // K quad
-    float av[2];
-#pragma unroll
-    for (int rg = 0; rg < 2; ++rg) av[rg] = Ahr[((size_t)kk * 2 + rg) * 64 + lane];
+  for (int s = 0; s < 4; ++s) {  // K quad (m0 + 16 q) / 4 + s
 #pragma unroll
     for (int cg = 0; cg < 4; ++cg) {
       const float xb = red[0][16 * q + 4 * s + fk][cg * 16 + fr];
 #pragma unroll
       for (int rg = 0; rg < 2; ++rg)
-        ah[rg][cg] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rg], xb, ah[rg][cg], 0, 0, 0);
+        ah[rg][cg] = __builtin_amdgcn_mfma_f32_16x16x4f32(ahv[s][rg], xb, ah[rg][cg], 0, 0, 0);
     }
   }
   // partial of wave q: rows (q & 1) * 32 .. of red[1 + q / 2]
