@@ -1635,6 +1635,11 @@ struct Family {
   int* dlist = nullptr;
   int* hflags = nullptr;    // pinned: done flags of the last two chunks (2 x ldb)
   hipEvent_t fev[2] = {nullptr, nullptr};
+  // fetch: the outputs come down through two pinned staging chunks (events
+  // sev) while the host copies the previous chunk out on several threads
+  static constexpr size_t kFetchChunk = (size_t)32 << 20;
+  char* stage = nullptr;
+  hipEvent_t sev[2] = {nullptr, nullptr};
   int compactions = 0;
   int nprod = 1;          // mixed: bf16 products of the split GEMM (1, 2 or 3)
   // work tables (device), built once per family: SpMM wave tasks for A and
@@ -1722,6 +1727,9 @@ struct Family {
     free_batch();
     for (hipEvent_t& e : fev)
       if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t& e : sev)
+      if (e) (void)hipEventDestroy(e);
+    if (stage) (void)hipHostFree(stage);
     for (void* p : allocs) (void)hipFree(p);
     if (blas) rocblas_destroy_handle(blas);
     if (st) (void)hipStreamDestroy(st);
@@ -2511,8 +2519,14 @@ bool Family::fetch(double* X, double* Y, double* Sx, long* iters, long* status, 
   std::vector<int> ip((size_t)3 * ldb);
   std::vector<double> red((size_t)8 * ldb);
   // unscaled on the device (x = E x, y = D y, s = s / D: pdos.c:251-294) into
-  // instance-major binary64, copied straight into the caller's arrays
+  // instance-major binary64, then into the caller's arrays
   double* g = static_cast<double*>(gout);
+  struct Part {
+    const char* dev;
+    char* out;
+    size_t bytes;
+  };
+  std::vector<Part> parts;
   auto gather = [&](const void* Zp, int rows, const void* scale, int divide, double* dev, double* out) {
     if (!out) return;
     if (fp64)
@@ -2521,13 +2535,57 @@ bool Family::fetch(double* X, double* Y, double* Sx, long* iters, long* status, 
     else
       hipLaunchKernelGGL(k_gather_out<float>, dim3(1024), dim3(256), 0, st, rows, cnt, ldb,
                          static_cast<const float*>(Zp), static_cast<const double*>(scale), divide, dev);
-    (void)hipMemcpyAsync(out, dev, (size_t)rows * cnt * 8, hipMemcpyDeviceToHost, st);
+    parts.push_back(Part{reinterpret_cast<const char*>(dev), reinterpret_cast<char*>(out), (size_t)rows * cnt * 8});
   };
   gather(Xo, n, E, 0, g, X);
   gather(Yo, m, D, 0, g + (size_t)n * ldb, Y);
   gather(So, m, D, 1, g + (size_t)(n + m) * ldb, Sx);
   (void)hipMemcpyAsync(ip.data(), Io, (size_t)3 * ldb * 4, hipMemcpyDeviceToHost, st);
   (void)hipMemcpyAsync(red.data(), Ro, (size_t)8 * ldb * 8, hipMemcpyDeviceToHost, st);
+  // the outputs in chunks of at most kFetchChunk bytes, through two pinned
+  // stages: chunk c + 2 goes down while the host copies chunk c + 1 out
+  // (pageable destinations take device copies at a fraction of the link's
+  // rate; the host's copy is spread over up to 16 threads)
+  if (!stage) {
+    if (hipHostMalloc(reinterpret_cast<void**>(&stage), 2 * kFetchChunk, hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&sev[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&sev[1], hipEventDisableTiming) != hipSuccess) {
+      stage = nullptr;
+      err = "fetch: pinned staging failed";
+      return false;
+    }
+  }
+  std::vector<Part> chunks;
+  for (const Part& p : parts)
+    for (size_t o = 0; o < p.bytes; o += kFetchChunk)
+      chunks.push_back(Part{p.dev + o, p.out + o, std::min(kFetchChunk, p.bytes - o)});
+  auto down = [&](size_t c) {
+    char* buf = stage + (c & 1) * kFetchChunk;
+    (void)hipMemcpyAsync(buf, chunks[c].dev, chunks[c].bytes, hipMemcpyDeviceToHost, st);
+    (void)hipEventRecord(sev[c & 1], st);
+  };
+  const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  for (size_t c = 0; c < chunks.size() && c < 2; ++c) down(c);
+  for (size_t c = 0; c < chunks.size(); ++c) {
+    if (hipEventSynchronize(sev[c & 1]) != hipSuccess) {
+      err = "fetch failed";
+      return false;
+    }
+    const char* src = stage + (c & 1) * kFetchChunk;
+    const size_t bytes = chunks[c].bytes, share = (bytes / nth + 63) & ~(size_t)63;
+    std::vector<std::thread> th;
+    for (int t = 1; t < nth; ++t) {
+      const size_t o = (size_t)t * share;
+      if (o >= bytes) break;
+      char* dst = chunks[c].out + o;
+      const char* from = src + o;
+      const size_t len = std::min(share, bytes - o);
+      th.emplace_back([dst, from, len] { std::memcpy(dst, from, len); });
+    }
+    std::memcpy(chunks[c].out, src, std::min(share, bytes));
+    for (std::thread& t : th) t.join();
+    if (c + 2 < chunks.size()) down(c + 2);
+  }
   if (hipStreamSynchronize(st) != hipSuccess || hipGetLastError() != hipSuccess) {
     err = "fetch failed";
     return false;
