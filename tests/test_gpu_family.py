@@ -368,3 +368,32 @@ def test_fp64_polish_moves_mixed_results_to_the_fp64_solution():
     d0, d1 = dist(mixed), dist(pol)
     assert np.median(d1) < 0.2 * np.median(d0), (np.median(d0), np.median(d1))
     assert d1.max() < d0.max()
+
+
+def test_fetch_in_chunks_with_outputs_left_out():
+    """pdos_family_fetch copies x, y, s down through pinned staging chunks
+    (32 MB) into the caller's arrays: a batch whose outputs span several
+    chunks gives the same x and iterations when fetched again with y, s and
+    the residuals left out (NULL), and every y differs from instance to
+    instance (no chunk copied twice or left stale)."""
+    import ctypes as C
+    fam = _fam(na=500, k=20, MAX_ITERS=50)
+    _, Cm = fam.sample(seed=41, count=4096)  # x, y, s: 4096 x (501 + 2 x 1022) doubles = 83 MB
+    print("chunked fetch", flush=True)
+    F = Family(fam.instance(Cm[0]), precision="mixed")
+    F.load(Cm)
+    F.run()
+    full = F.fetch()
+    X = np.zeros((len(Cm), fam.n))
+    it = np.zeros(len(Cm), dtype=np.int64)
+    st = np.zeros(len(Cm), dtype=np.int64)
+    fb = np.zeros(len(Cm), dtype=np.int64)
+    dp, lp = C.POINTER(C.c_double), C.POINTER(C.c_long)
+    rc = F.L.pdos_family_fetch(F.h, X.ctypes.data_as(dp), None, None, it.ctypes.data_as(lp),
+                               st.ctypes.data_as(lp), fb.ctypes.data_as(lp), None)
+    assert rc == 0
+    np.testing.assert_array_equal(X, full["x"])
+    np.testing.assert_array_equal(it, full["iters"])
+    assert np.all(np.isfinite(full["y"])) and np.all(np.isfinite(full["s"]))
+    # y and s differ between instances (their own c), so no chunk was repeated
+    assert len(np.unique(full["y"][:, 0])) > len(Cm) // 2
