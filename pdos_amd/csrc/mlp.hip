@@ -53,8 +53,9 @@ __device__ __forceinline__ float gelu_grad(float z) {
 //   TB = false: B is K x N row-major (ldb);  TB = true: B is stored N x K (ldb)
 // M % 64 == 0, N % 64 == 0, K % KB == 0 (KB = 64 or 128).  Block: 64 x 64
 // tile, 4 waves in a 2 x 2 layout, each wave 32 x 32 = 2 x 2 MFMA tiles of
-// 16 x 16.  The K loop is software-pipelined: the global loads of step k + 1
-// are in flight while step k is multiplied out of LDS.
+// 16 x 16.  The K loop is software-pipelined two steps deep: two register
+// sets alternate, and the global loads of steps k + 1 and k + 2 are in
+// flight while step k is multiplied out of LDS.
 template <bool TA, bool TB, int EPI, int KB = BK>
 __global__ __launch_bounds__(GT) void k_gemm(int M, int N, int K, const __bf16* __restrict__ A, int lda,
                                              const __bf16* __restrict__ B, int ldb, float* __restrict__ Cf,
@@ -77,8 +78,8 @@ __global__ __launch_bounds__(GT) void k_gemm(int M, int N, int K, const __bf16* 
   // Each thread moves H 16-byte pieces of each tile per K step.
   //   row-major source (A: !TA, B: TB):  row t / 4, k (t % 4) * 8 + 32 h
   //   transposed source (A: TA, B: !TB): k t / 8 + 32 h, row (t % 8) * 8 .. + 7
-  bf16x8 ra[H], rb[H];
-  auto gload = [&](int k0) {
+  bf16x8 ra0[H], rb0[H], ra1[H], rb1[H];
+  auto gload = [&](int k0, bf16x8 (&ra)[H], bf16x8 (&rb)[H]) {
 #pragma unroll
     for (int h = 0; h < H; ++h) {
       if (!TA) ra[h] = *reinterpret_cast<const bf16x8*>(&A[(size_t)(m0 + (tid >> 2)) * lda + k0 + (tid & 3) * 8 + h * 32]);
@@ -87,7 +88,7 @@ __global__ __launch_bounds__(GT) void k_gemm(int M, int N, int K, const __bf16* 
       else rb[h] = *reinterpret_cast<const bf16x8*>(&B[(size_t)(k0 + (tid >> 3) + h * 32) * ldb + n0 + (tid & 7) * 8]);
     }
   };
-  auto sstore = [&]() {
+  auto sstore = [&](const bf16x8 (&ra)[H], const bf16x8 (&rb)[H]) {
 #pragma unroll
     for (int h = 0; h < H; ++h) {
       if (!TA) {
@@ -104,7 +105,8 @@ __global__ __launch_bounds__(GT) void k_gemm(int M, int N, int K, const __bf16* 
       }
     }
   };
-  gload(0);
+  gload(0, ra0, rb0);
+  if (KB < K) gload(KB, ra1, rb1);
   // EPI_ADD: the addend tile, loaded now so it arrives during the K loop
   float addv[2][2][4];
   if (EPI == EPI_ADD) {
@@ -119,12 +121,9 @@ __global__ __launch_bounds__(GT) void k_gemm(int M, int N, int K, const __bf16* 
           addv[i][j][r] = aux[(size_t)row * ldc + col];
         }
   }
-  for (int k0 = 0; k0 < K; k0 += KB) {
-    sstore();
-    __syncthreads();
-    if (k0 + KB < K) gload(k0 + KB);
-    // lane l: A rows (l % 16), k = (l / 16) * 8 .. +7; B cols (l % 16), same k
-    const int fr = lane & 15, fk = (lane >> 4) * 8;
+  // lane l: A rows (l % 16), k = (l / 16) * 8 .. +7; B cols (l % 16), same k
+  const int fr = lane & 15, fk = (lane >> 4) * 8;
+  auto multiply = [&]() {
 #pragma unroll
     for (int ks = 0; ks < KB; ks += 32) {
       bf16x8 af[2], bfv[2];
@@ -138,6 +137,20 @@ __global__ __launch_bounds__(GT) void k_gemm(int M, int N, int K, const __bf16* 
         for (int j = 0; j < 2; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfv[j], acc[i][j], 0, 0, 0);
     }
+  };
+  // steps k0 (set 0) and k0 + KB (set 1); a set is refilled with the step
+  // two ahead as soon as it is in LDS
+  for (int k0 = 0; k0 < K; k0 += 2 * KB) {
+    sstore(ra0, rb0);
+    __syncthreads();
+    if (k0 + 2 * KB < K) gload(k0 + 2 * KB, ra0, rb0);
+    multiply();
+    __syncthreads();
+    if (k0 + KB >= K) break;
+    sstore(ra1, rb1);
+    __syncthreads();
+    if (k0 + 3 * KB < K) gload(k0 + 3 * KB, ra1, rb1);
+    multiply();
     __syncthreads();
   }
   // accumulator lane l holds rows (l / 16) * 4 + r, column l % 16
