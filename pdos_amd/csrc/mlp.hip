@@ -238,11 +238,11 @@ __global__ void k_loss_grad(int rows, int cols, int ld, const float* __restrict_
 // threads takes 64 columns (lane = column, coalesced rows); wave w sums rows
 // w, w + 8, ... and the 8 wave sums are added in wave order (deterministic).
 template <typename S>
-__global__ __launch_bounds__(512) void k_colsum(int rows, int cols, int ld, const S* __restrict__ G,
-                                                float* __restrict__ db) {
+__device__ __forceinline__ void colsum_block(int blk, int rows, int cols, int ld, const S* __restrict__ G,
+                                             float* __restrict__ db) {
   __shared__ float part[8][64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int c = blockIdx.x * 64 + lane;
+  const int c = blk * 64 + lane;
   float s = 0.f;
   if (c < cols) {
 #pragma unroll 8
@@ -256,6 +256,20 @@ __global__ __launch_bounds__(512) void k_colsum(int rows, int cols, int ld, cons
     for (int k = 0; k < 8; ++k) t += part[k][lane];
     db[c] = t;
   }
+}
+
+// The three bias gradients of the backward pass in one launch: blocks
+// [0, n3) colsum dY (fp32) into db3, then [n3, n3 + nh) dZ2 (bf16) into db2,
+// then [n3 + nh, ..) dZ1 into db1 -- each column block as colsum_block.
+__global__ __launch_bounds__(512) void k_colsum3(int rows, int pout, int ph, const float* __restrict__ dY,
+                                                 const __bf16* __restrict__ dZ2, const __bf16* __restrict__ dZ1,
+                                                 float* __restrict__ db3, float* __restrict__ db2,
+                                                 float* __restrict__ db1) {
+  const int n3 = (pout + 63) / 64, nh = (ph + 63) / 64;
+  const int b = blockIdx.x;
+  if (b < n3) colsum_block<float>(b, rows, pout, pout, dY, db3);
+  else if (b < n3 + nh) colsum_block<__bf16>(b - n3, rows, ph, ph, dZ2, db2);
+  else colsum_block<__bf16>(b - n3 - nh, rows, ph, ph, dZ1, db1);
 }
 
 // The bf16 mirrors of the weight matrices: master entries [o[k], o[k] + len[k])
@@ -284,19 +298,39 @@ __global__ void k_adam(long n, float* __restrict__ w, const float* __restrict__ 
   }
 }
 
-// dst[i][:] = src[idx[i]][:] for i < rows (minibatch gather).
-__global__ void k_gather_rows(int rows, int cols, const int* __restrict__ idx, const float* __restrict__ src,
-                              float* __restrict__ dst) {
-  const long total = (long)rows * cols;
-  const long e0 = blockIdx.x * (long)blockDim.x + threadIdx.x, stride = (long)gridDim.x * blockDim.x;
-  int r = (int)(e0 / cols), c = (int)(e0 % cols);
-  const int sr = (int)(stride / cols), sc = (int)(stride % cols);
+// A training step's minibatch: Xb[r][c] = bf16((X[idx[r]][c] - mu[c]) / sd[c])
+// over the padded rows_pad x ld_out (zero outside rows x nin; the arithmetic
+// of k_prep_input on the gathered rows), and Td[r][:] = Ts[idx[r]][:].
+__global__ void k_gather_prep(int rows, int nin, int ld_out, int rows_pad, int nout, const int* __restrict__ idx,
+                              const float* __restrict__ X, const float* __restrict__ mu,
+                              const float* __restrict__ sd, __bf16* __restrict__ Xb, const float* __restrict__ Ts,
+                              float* __restrict__ Td) {
+  const long stride = (long)gridDim.x * blockDim.x, e0 = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  {
+    const long total = (long)rows_pad * ld_out;
+    int r = (int)(e0 / ld_out), c = (int)(e0 % ld_out);
+    const int sr = (int)(stride / ld_out), sc = (int)(stride % ld_out);
+    for (long e = e0; e < total; e += stride) {
+      float v = 0.f;
+      if (r < rows && c < nin) v = (X[(size_t)idx[r] * nin + c] - mu[c]) / sd[c];
+      Xb[e] = (__bf16)v;
+      r += sr;
+      c += sc;
+      if (c >= ld_out) {
+        c -= ld_out;
+        ++r;
+      }
+    }
+  }
+  const long total = (long)rows * nout;
+  int r = (int)(e0 / nout), c = (int)(e0 % nout);
+  const int sr = (int)(stride / nout), sc = (int)(stride % nout);
   for (long e = e0; e < total; e += stride) {
-    dst[e] = src[(size_t)idx[r] * cols + c];
+    Td[e] = Ts[(size_t)idx[r] * nout + c];
     r += sr;
     c += sc;
-    if (c >= cols) {
-      c -= cols;
+    if (c >= nout) {
+      c -= nout;
       ++r;
     }
   }
@@ -417,6 +451,12 @@ struct Mlp {
                     __bf16* h2, float* y) {
     const int rp = pad64(rows);
     hipLaunchKernelGGL(k_prep_input, dim3(512), dim3(256), 0, st, rows, nin, nin, X, in_mu, in_sd, pin, rp, xb);
+    forward_layers(rows, affine_out, xb, z1, h1, z2, h2, y);
+  }
+  // the three products from the standardized bf16 input xb
+  void forward_layers(int rows, bool affine_out, const __bf16* xb, float* z1, __bf16* h1, float* z2, __bf16* h2,
+                      float* y) {
+    const int rp = pad64(rows);
     note(gemm_k<false, false, EPI_BIAS_GELU>(st, rp, ph, pin, xb, pin, W1b, ph, z1, h1, ph, b1));
     note(gemm_k<false, false, EPI_BIAS_GELU>(st, rp, ph, ph, h1, ph, W2b, ph, z2, h2, ph, b2));
     note(gemm_k<false, false, EPI_BIAS_AFFINE>(st, rp, pout, ph, h2, ph, W3b, pout, y, nullptr, pout, b3, nullptr,
@@ -448,13 +488,12 @@ struct Mlp {
     float* gW2 = g + off_W2; float* gb2 = g + off_b2;
     float* gW3 = g + off_W3; float* gb3 = g + off_b3;
     note(gemm_k<true, false, EPI_F32>(st, ph, pout, rp, H2b, ph, dYb, pout, gW3, nullptr, pout));
-    hipLaunchKernelGGL(k_colsum<float>, dim3((pout + 63) / 64), dim3(512), 0, st, rp, pout, pout, dYf, gb3);
     note(gemm_k<false, true, EPI_GELU_GRAD>(st, rp, ph, pout, dYb, pout, W3b, pout, nullptr, dZ2b, ph, nullptr, Z2));
     note(gemm_k<true, false, EPI_F32>(st, ph, ph, rp, H1b, ph, dZ2b, ph, gW2, nullptr, ph));
-    hipLaunchKernelGGL(k_colsum<__bf16>, dim3((ph + 63) / 64), dim3(512), 0, st, rp, ph, ph, dZ2b, gb2);
     note(gemm_k<false, true, EPI_GELU_GRAD>(st, rp, ph, ph, dZ2b, ph, W2b, ph, nullptr, dZ1b, ph, nullptr, Z1));
     note(gemm_k<true, false, EPI_F32>(st, pin, ph, rp, Xb, pin, dZ1b, ph, gW1, nullptr, ph));
-    hipLaunchKernelGGL(k_colsum<__bf16>, dim3((ph + 63) / 64), dim3(512), 0, st, rp, ph, ph, dZ1b, gb1);
+    hipLaunchKernelGGL(k_colsum3, dim3((pout + 63) / 64 + 2 * ((ph + 63) / 64)), dim3(512), 0, st, rp, pout, ph, dYf,
+                       dZ2b, dZ1b, gb3, gb2, gb1);
   }
   void adam(float lr) {
     ++step;
@@ -693,9 +732,9 @@ int pdos_mlp_train(PdosMlp* P, const float* X, const float* T, int rows, int epo
     (void)hipMemcpyAsync(dIdx, cur.data(), (size_t)rows * 4, hipMemcpyHostToDevice, M.st);
     for (int q = 0; q < steps; ++q) {
       const int s = q * batch;
-      hipLaunchKernelGGL(pdos_amd::k_gather_rows, dim3(512), dim3(256), 0, M.st, batch, M.nin, dIdx + s, dX, M.Xf);
-      hipLaunchKernelGGL(pdos_amd::k_gather_rows, dim3(1024), dim3(256), 0, M.st, batch, M.nout, dIdx + s, dT, M.T);
-      M.forward(batch, false);
+      hipLaunchKernelGGL(pdos_amd::k_gather_prep, dim3(1024), dim3(256), 0, M.st, batch, M.nin, M.pin,
+                         pdos_amd::pad64(batch), M.nout, dIdx + s, dX, M.in_mu, M.in_sd, M.Xb, dT, M.T);
+      M.forward_layers(batch, false, M.Xb, M.Z1, M.H1b, M.Z2, M.H2b, M.Y);
       M.backward_async(batch, dPart + (size_t)q * Mlp::kLossBlocks);
       M.adam(lr);
     }
