@@ -651,9 +651,28 @@ __global__ __launch_bounds__(FT) void k_fam_b(FamArgs<T> a) {
         const size_t o = (size_t)i * a.ldb + inst;
         cv[k] = a.c[o]; xv[k] = a.X[o]; uv[k] = a.U[o];
         if (!a.judge) vv[k] = a.V[o];
-        for (int z = 1; z < a.nsplit; ++z) {
-          uv[k] += a.U[o + (size_t)z * a.plane];
-          if (!a.judge) vv[k] += a.V[o + (size_t)z * a.plane];
+      }
+    }
+    // The K planes, added in plane order per row; each plane's loads for all
+    // RW rows are issued together (and the unrolled planes ahead of their
+    // adds), not one dependent load per row and plane.
+#pragma unroll 4
+    for (int z = 1; z < a.nsplit; ++z) {
+      T pu[RW], pv[RW];
+#pragma unroll
+      for (int k = 0; k < RW; ++k) {
+        const int i = r0 + w + k * NWV;
+        if (i < r1) {
+          const size_t o = (size_t)i * a.ldb + inst + (size_t)z * a.plane;
+          pu[k] = a.U[o];
+          if (!a.judge) pv[k] = a.V[o];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < RW; ++k) {
+        if (r0 + w + k * NWV < r1) {
+          uv[k] += pu[k];
+          if (!a.judge) vv[k] += pv[k];
         }
       }
     }
