@@ -1488,6 +1488,7 @@ __global__ void k_fam_scale_in(int cnt, int n, int m, int ldb, const double* __r
   }
   double nc = 0.0, ncs = 0.0, nb = 0.0, nbs = nbs_sh;
   const double* c = C + (size_t)t * n;
+#pragma unroll 8
   for (int i = 0; i < n; ++i) {
     const double ci = c[i];
     nc += ci * ci;
@@ -1498,6 +1499,7 @@ __global__ void k_fam_scale_in(int cnt, int n, int m, int ldb, const double* __r
   if (B) {
     nbs = 0.0;
     const double* b = B + (size_t)t * m;
+#pragma unroll 8
     for (int j = 0; j < m; ++j) {
       const double bj = b[j];
       nb += bj * bj;
@@ -2109,15 +2111,17 @@ bool Family::load(int cnt, const double* C, const double* B, std::string& err) {
   // scaled b, summed in index order as per instance
   double nbs_sh = 0.0;
   for (int j = 0; j < m; ++j) nbs_sh += S.b[j] * S.b[j];
-  const dim3 grid((ldb + 255) / 256);
+  // one wave per block: a lane walks its instance's row of C, so blocks of
+  // 64 spread the batch over 4x as many CUs as blocks of 256 would
+  const dim3 grid((ldb + 63) / 64);
   if (fp64)
-    hipLaunchKernelGGL(k_fam_scale_in<double>, grid, dim3(256), 0, st, cnt, n, m, ldb, static_cast<const double*>(inC),
+    hipLaunchKernelGGL(k_fam_scale_in<double>, grid, dim3(64), 0, st, cnt, n, m, ldb, static_cast<const double*>(inC),
                        static_cast<const double*>(inB), static_cast<const double*>(E0),
                        static_cast<const double*>(D), S.pscale, nbs_sh, static_cast<double*>(base_c),
                        static_cast<double*>(base_b), static_cast<double*>(base_lam), static_cast<double*>(base_ps),
                        static_cast<double*>(base_ds));
   else
-    hipLaunchKernelGGL(k_fam_scale_in<float>, grid, dim3(256), 0, st, cnt, n, m, ldb, static_cast<const double*>(inC),
+    hipLaunchKernelGGL(k_fam_scale_in<float>, grid, dim3(64), 0, st, cnt, n, m, ldb, static_cast<const double*>(inC),
                        static_cast<const double*>(inB), static_cast<const double*>(E0),
                        static_cast<const double*>(D), S.pscale, nbs_sh, static_cast<float*>(base_c),
                        static_cast<float*>(base_b), static_cast<double*>(base_lam), static_cast<double*>(base_ps),
